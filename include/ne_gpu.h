@@ -1,0 +1,270 @@
+/*
+ * ne_gpu.h -- C ABI of the MI355X (gfx950) path-tracing backend for
+ * NarvalEngine.  Plain C types only: pointers, sizes, POD structs.
+ *
+ * The library replaces the CPU hot path
+ *   OfflineEngine::renderTile  (src/core/OfflineEngine.cpp:54-76)
+ *     -> Camera::getRayPassingThrough (src/core/Camera.cpp:140-144)
+ *     -> VolumetricPathIntegrator::Li (src/integrators/VolumetricPathIntegrator.cpp:176-301)
+ * behind the reference's Integrator / tile-render interfaces
+ * (src/integrators/Integrator.h:13-25, src/core/OfflineEngine.h:35-59).
+ *
+ * Ownership: the caller owns every host buffer passed in or out; the library
+ * owns device memory behind the opaque handles; descriptors are copied at
+ * upload.  Errors: every entry point returns an ne_status; nothing aborts or
+ * throws across the ABI (the reference LOG(FATAL)s, e.g.
+ * src/io/SceneReader.cpp:24-41, src/materials/GridMedia.cpp:20).
+ * Threading: calls on one context are serialised by an internal mutex and
+ * run on the context's single HIP stream; distinct contexts are independent.
+ */
+#ifndef NE_GPU_H
+#define NE_GPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef int32_t ne_status;
+enum {
+  NE_OK = 0,
+  NE_ERR_INVALID_ARG = 1,
+  NE_ERR_HIP = 2,
+  NE_ERR_OUT_OF_MEMORY = 3,
+  NE_ERR_NO_DEVICE = 4,
+  NE_ERR_UNSUPPORTED = 5,
+  NE_ERR_PARSE = 6,
+  NE_ERR_IO = 7
+};
+
+typedef struct ne_ctx ne_ctx;
+typedef struct ne_scene ne_scene;
+typedef struct ne_film ne_film;
+
+/* Material record: 12 floats
+ *   [0] type (NE_MAT_*), [1..3] albedo, [4] roughness, [5] metallic,
+ *   [6..8] emitted radiance, [9..11] unused.
+ * Glossy = GlossyBSDF (src/core/GlossyBSDF.cpp:4-48, alpha = roughness^2);
+ * emitter = DiffuseLight (src/lights/DiffuseLight.cpp:4-27). */
+enum { NE_MAT_GLOSSY = 0, NE_MAT_EMITTER = 1 };
+/* Environment: none, constant radiance env_c0, or a sky gradient
+ * (zenith env_c0, horizon env_c1, below-horizon env_c2). */
+enum { NE_ENV_NONE = 0, NE_ENV_CONSTANT = 1, NE_ENV_SKY = 2 };
+
+/* Flattened world-space scene.  Mirrors what SceneReader builds
+ * (src/io/SceneReader.cpp:224-648) after InstancedModel transforms:
+ *   triangles      9 floats each (v0, v1, v2)         Triangle (src/primitives/Triangle.cpp)
+ *   spheres        4 floats each (centre, radius)     Sphere (src/primitives/Sphere.cpp)
+ *   rectangles    12 floats each (corner p0, edge eu, edge ev, normal n);
+ *                 the reference's unit quad [-.5,.5]^2 at z=0 with normal
+ *                 (0,0,-1) under its instance transform (SceneReader.cpp:407-419)
+ *   volume         one GridMedia (src/materials/GridMedia.cpp) on the unit
+ *                 cube [-.5,.5]^3 under vol_xf (row-major 4x4, OCS->WCS);
+ *                 grid is x-fastest R32F (to1D, src/utils/Math.h:104-106)
+ *   sun            DirectionalLight (src/lights/DirectionalLight.cpp), sun_dir is
+ *                 the direction light travels, sun_E its radiance scale
+ * The same layout is accepted by the test oracle, so one host-side
+ * descriptor drives both. */
+typedef struct ne_scene_desc {
+  const float* tri_v;
+  const int32_t* tri_mat;
+  uint32_t n_tri;
+  const float* sph;
+  const int32_t* sph_mat;
+  uint32_t n_sph;
+  const float* rect;
+  const int32_t* rect_mat;
+  uint32_t n_rect;
+  const float* mat;
+  uint32_t n_mat;
+  int32_t has_volume;
+  float vol_xf[16];
+  float vol_inv[16];
+  int32_t vol_res[3];
+  const float* grid;
+  float vol_mult;
+  float vol_sigma_s[3];
+  float vol_sigma_a[3];
+  float vol_g;
+  int32_t maj_res; /* majorant cells per axis; 1 = the reference's single global majorant */
+  int32_t has_sun;
+  float sun_dir[3];
+  float sun_E[3];
+  int32_t env_type;
+  float env_c0[3];
+  float env_c1[3];
+  float env_c2[3];
+} ne_scene_desc;
+
+/* Pinhole camera: Camera(lookFrom, lookAt, up, vfov, aspect, ...)
+ * (src/core/Camera.cpp:7-26); ray through (u,v) per getRayPassingThrough
+ * (src/core/Camera.cpp:140-144) with the lens offset at zero. */
+typedef struct ne_camera_desc {
+  float position[3];
+  float look_at[3];
+  float up[3];
+  float vfov;
+  float aspect;
+} ne_camera_desc;
+
+enum { NE_FP32 = 0, NE_FP64 = 1 };
+enum {
+  NE_FLAG_COUNT = 1, /* collect event counters (separate, slower pass) */
+  NE_FLAG_TIME = 2   /* time the path-tracing kernel with HIP events on the context stream */
+};
+
+/* One render call: samples [sample_offset, sample_offset + spp) of every pixel
+ * in [x0,x1) x [y0,y1) of a width x height image (SceneSettings,
+ * src/core/Settings.h:9-14).  Path length: max_bounces scatter vertices. */
+typedef struct ne_render_params {
+  int32_t width, height;
+  int32_t x0, y0, x1, y1;
+  int32_t spp;
+  int32_t max_bounces;
+  uint64_t seed;
+  uint32_t sample_offset;
+  uint32_t job;
+  int32_t precision; /* NE_FP32 (product) or NE_FP64 (converged-oracle mode) */
+  int32_t flags;
+  int32_t samples_per_thread; /* 0 = library default */
+  int32_t block_size;         /* 0 = library default */
+  int32_t reserved[4];
+} ne_render_params;
+
+typedef struct ne_counters {
+  uint64_t paths;
+  uint64_t ext_rays;        /* closest-hit queries */
+  uint64_t shadow_rays;     /* transmittance queries */
+  uint64_t density_lookups; /* trilinear grid lookups (32 B each) */
+  uint64_t majorant_cells;  /* majorant-grid cells visited (4 B each) */
+  uint64_t bvh_nodes;       /* BVH nodes fetched (64 B each) */
+  uint64_t prim_tests;      /* primitive intersection tests */
+  uint64_t reserved;
+} ne_counters;
+
+typedef struct ne_scene_info {
+  uint32_t bvh_nodes;
+  uint32_t bvh_depth;
+  uint32_t bvh_leaf_prims_max;
+  uint32_t maj_res;
+  uint64_t device_bytes;
+  float grid_max_density;
+  float pad;
+} ne_scene_info;
+
+/* ---- context ------------------------------------------------------------ */
+ne_status ne_ctx_create(int32_t device, ne_ctx** out);
+void ne_ctx_destroy(ne_ctx* ctx);
+/* Last error message of this context (static string if ctx is NULL). */
+const char* ne_last_error(const ne_ctx* ctx);
+ne_status ne_synchronize(ne_ctx* ctx);
+/* Size of ne_scene_desc as compiled into the library (ABI check). */
+int32_t ne_sizeof_scene_desc(void);
+
+/* ---- scene ---------------------------------------------------------------
+ * Replaces SceneReader's Scene/BVH/GridMedia construction
+ * (src/io/SceneReader.cpp:10-54, src/primitives/BVH.cpp:6-106,
+ * src/materials/GridMedia.cpp:4-13): builds a binned-SAH BVH and the
+ * majorant grid on the host and uploads everything to HBM. */
+ne_status ne_scene_upload(ne_ctx* ctx, const ne_scene_desc* desc, ne_scene** out);
+void ne_scene_release(ne_scene* scene);
+ne_status ne_scene_get_info(const ne_scene* scene, ne_scene_info* out);
+
+/* ---- film (device FP64 accumulator, x-fastest, 3 doubles per pixel) ------ */
+ne_status ne_film_create(ne_ctx* ctx, int32_t width, int32_t height, ne_film** out);
+void ne_film_release(ne_film* film);
+ne_status ne_film_clear(ne_film* film);
+/* Copies the per-pixel radiance sums (width*height*3 doubles) to the host. */
+ne_status ne_film_read(ne_film* film, double* sum_rgb);
+/* OfflineEngine::postProcessing (src/core/OfflineEngine.cpp:27-52) of
+ * sum/spp into pixels (width*height*3 floats, row-major, x fastest). */
+ne_status ne_film_tonemap(ne_film* film, int32_t spp, float* pixels);
+
+/* ---- render --------------------------------------------------------------
+ * Enqueues the render on the context stream; returns when enqueued unless an
+ * argument error is found.  ne_synchronize waits for completion. */
+ne_status ne_render(ne_ctx* ctx, const ne_scene* scene, const ne_camera_desc* cam, const ne_render_params* params,
+                    ne_film* film);
+/* Tile call with the reference's tiling: 40 x 10 tiles of (W/40) x (H/10)
+ * pixels, tile `index` = (index % 40, index / 40) (OfflineEngine.cpp:18-21,
+ * 54-58).  Renders params->spp samples of that tile, synchronises, and
+ * writes post-processed pixels of the tile into pixels (W*H*3 floats). */
+ne_status ne_render_tile(ne_ctx* ctx, const ne_scene* scene, const ne_camera_desc* cam,
+                         const ne_render_params* params, int32_t index, ne_film* film, float* pixels);
+/* Duration in milliseconds of the path-tracing kernel of the last render
+ * issued with NE_FLAG_TIME (waits for it to finish). */
+ne_status ne_get_kernel_time(ne_ctx* ctx, float* ms);
+/* Event counters of the last render issued with NE_FLAG_COUNT. */
+ne_status ne_get_counters(ne_ctx* ctx, ne_counters* out);
+/* Integrator::Li (src/integrators/Integrator.h:23) for a batch of n rays
+ * (origin xyz, direction xyz): radiance of one path per ray, path stream i
+ * keyed by (seed, pixel = i, sample = sample_index). */
+ne_status ne_li(ne_ctx* ctx, const ne_scene* scene, int32_t n, const float* rays, int32_t max_bounces, uint64_t seed,
+                uint32_t sample_index, float* radiance);
+
+/* ---- per-function parity entry points (device code, host buffers) ------- */
+enum {
+  NE_TEST_AABB = 0,     /* AABB::intersect      src/primitives/AABB.cpp:48-79      prim: min xyz, max xyz   */
+  NE_TEST_TRIANGLE = 1, /* Triangle::intersect  src/primitives/Triangle.cpp:49-81  prim: v0, v1, v2         */
+  NE_TEST_SPHERE = 2,   /* Sphere::intersect    src/primitives/Sphere.cpp:21-44    prim: centre, radius     */
+  NE_TEST_RECTANGLE = 3 /* Rectangle::intersect src/primitives/Rectangle.cpp:50-76 prim: vA, vB, normal     */
+};
+/* Per-ray primitive test in the reference's semantics; t0/t1 = tNear/tFar. */
+ne_status ne_test_primitive(ne_ctx* ctx, int32_t kind, int32_t n, const float* rays, const float* prims, int32_t* hit,
+                            float* t0, float* t1);
+/* Closest surface hit in (tmin, tmax) through the device BVH; type -1 = miss. */
+ne_status ne_test_closest(ne_ctx* ctx, const ne_scene* scene, int32_t n, const float* rays, float tmin, float tmax,
+                          float* t, int32_t* type, int32_t* index);
+/* Shadow transmittance (surface occlusion x ratio tracking,
+ * GridMedia::Tr src/materials/GridMedia.cpp:45-69) along n rays to distance
+ * tmax[i]; stream i keyed by (seed, pixel = i, sample 0). */
+ne_status ne_test_transmittance(ne_ctx* ctx, const ne_scene* scene, int32_t n, const float* rays, const float* tmax,
+                                uint64_t seed, int32_t precision, double* tr);
+/* Delta tracking (GridMedia::sample, src/materials/GridMedia.cpp:71-98) along n
+ * rays through the volume; collision flag and distance. */
+ne_status ne_test_delta_track(ne_ctx* ctx, const ne_scene* scene, int32_t n, const float* rays, uint64_t seed,
+                              int32_t precision, int32_t* scattered, double* t);
+/* Trilinear density (GridMedia::interpolatedDensity, GridMedia.cpp:25-38) at
+ * n grid-space points of the scene's volume. */
+ne_status ne_test_density(ne_ctx* ctx, const ne_scene* scene, int32_t n, const float* gcs, float* out);
+/* Phase / BSDF functions:
+ *   NE_TEST_HG_EVAL:   in a[3n], b[3n], g[n]           -> out[n]   HG::eval (Medium.h:116-128)
+ *   NE_TEST_HG_SAMPLE: in u[2n], g[n]                  -> out[3n]  HG::sample local (Medium.h:96-114)
+ *   NE_TEST_GGX_D:     in h[3n], alpha[n]              -> out[n]   GGXDistribution::D (Microfacet.cpp:5-15)
+ *   NE_TEST_GGX_PDF:   in wo[3n], h[3n], alpha[n]      -> out[n]   GGXDistribution::pdf (Microfacet.cpp:58-65)
+ *   NE_TEST_GLOSSY:    in mat[5n], wo[3n], wi[3n], u[3n] -> out[7n] (f rgb, pdf, sampled wi)
+ *                      corrected GGX+Lambert lobe used by the integrator
+ *   NE_TEST_HG_FWD:    in d[3n], u[2n], g[n]           -> out[4n]  (sampled wi, pdf) forward-convention HG */
+enum { NE_TEST_HG_EVAL = 0, NE_TEST_HG_SAMPLE = 1, NE_TEST_GGX_D = 2, NE_TEST_GGX_PDF = 3, NE_TEST_GLOSSY = 4,
+       NE_TEST_HG_FWD = 5 };
+ne_status ne_test_bsdf(ne_ctx* ctx, int32_t kind, int32_t n, const float* in0, const float* in1, const float* in2,
+                       const float* in3, float* out);
+/* Device Philox-4x32-10 uniform stream: n uniforms of stream (seed, pixel, sample, job). */
+ne_status ne_test_rng(ne_ctx* ctx, uint64_t seed, uint32_t pixel, uint32_t sample, uint32_t job, int32_t n,
+                      float* out);
+
+/* Host-only: builds the BVH that ne_scene_upload would build for desc and
+ * returns it (no device needed).  nodes gets 16 floats per node in the device
+ * layout; refs one (type << 30 | original primitive index) per leaf slot.
+ * Call with NULL nodes/refs to query n_nodes / n_refs first. */
+ne_status ne_bvh_build(const ne_scene_desc* desc, float* nodes, uint32_t* refs, uint32_t* n_nodes, uint32_t* n_refs,
+                       uint32_t* depth);
+
+/* ---- synthetic inputs ----------------------------------------------------
+ * Seeded, platform-independent generators for the benchmark scenes. */
+/* 4-octave Perlin fBm density clamped to [0,1] on res^3 (x fastest). */
+ne_status ne_gen_fbm_grid(int32_t res, uint64_t seed, float base_freq, float* out);
+/* Cumulus-like density: (Worley F1 - fBm) shaped by an ellipsoid, in [0,1]. */
+ne_status ne_gen_cumulus_grid(int32_t res, uint64_t seed, float* out);
+/* Icosphere subdivided `subdiv` times (20*4^subdiv triangles) with vertices
+ * displaced along the normal by amp * fBm(seed); unit radius.  verts gets
+ * n_vertices*3 floats, tris n_triangles*3 indices; pass NULL to query sizes. */
+ne_status ne_gen_icosphere(int32_t subdiv, uint64_t seed, float amp, float* verts, uint32_t* tris,
+                           uint32_t* n_vertices, uint32_t* n_triangles);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* NE_GPU_H */

@@ -1,0 +1,68 @@
+"""Builds libnarval_gpu.so (HIP kernels for gfx950 + C++ host runtime) in-tree.
+
+Every source is compiled by hipcc with:
+  * --offload-arch=gfx950 (CDNA4 only),
+  * -ffp-contract=off: each multiply and add rounds separately, so device
+    arithmetic matches a plain C++ evaluation of the same formula,
+  * -Xclang -target-feature -Xclang -packed-fp32-ops: no packed-FP32 VALU ops.
+"""
+from __future__ import annotations
+
+import concurrent.futures as cf
+import os
+import shutil
+import subprocess
+import sys
+from pathlib import Path
+
+PKG = Path(__file__).resolve().parent
+CSRC = PKG / "csrc"
+BUILD = PKG / "_build"
+LIB = PKG / "libnarval_gpu.so"
+SOURCES = ["render.hip", "api.cpp", "bvh_build.cpp", "generators.cpp"]
+HEADERS = ["ne_math.h", "ne_scene.h", "ne_trace.h", "ne_launch.h", "bvh_build.h"]
+
+
+def hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and Path(cand).exists():
+            return cand
+    raise RuntimeError("hipcc not found (ROCm toolchain required to build the HIP library)")
+
+
+COMMON = [
+    "-O3", "-std=c++17", "-fPIC", "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
+]
+
+
+def _compile(src: str) -> Path:
+    obj = BUILD / (src.replace(".", "_") + ".o")
+    cmd = [hipcc(), "-c", str(CSRC / src), "-o", str(obj), *COMMON]
+    cmd += ["-x", "hip", "--offload-arch=gfx950"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError(f"compilation failed: {src}")
+    return obj
+
+
+def build(verbose: bool = False) -> Path:
+    BUILD.mkdir(exist_ok=True)
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
+        objs = list(ex.map(_compile, SOURCES))
+    tmp = LIB.with_suffix(".so.tmp")
+    cmd = [hipcc(), "-shared", "-o", str(tmp), *map(str, objs), "--offload-arch=gfx950", "-pthread"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("link failed")
+    os.replace(tmp, LIB)
+    if verbose:
+        print(f"built {LIB}")
+    return LIB
+
+
+if __name__ == "__main__":
+    build(verbose=True)

@@ -1,0 +1,763 @@
+// api.cpp -- implementation of the C ABI in include/ne_gpu.h: contexts,
+// scene flattening and upload (SAH BVH + majorant grid), films, render
+// dispatch, counters and the parity entry points.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "../../include/ne_gpu.h"
+#include "bvh_build.h"
+#include "ne_launch.h"
+#include "ne_math.h"
+#include "ne_scene.h"
+
+using namespace ne;
+
+struct ne_ctx {
+  int device = 0;
+  hipStream_t stream = nullptr;
+  std::mutex mu;
+  std::string err;
+  DevCounters* d_counters = nullptr;
+  DevCounters h_counters{};
+  bool counters_valid = false;
+  double* d_partial = nullptr;
+  size_t partial_bytes = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  bool timed = false;
+};
+
+struct ne_scene {
+  ne_ctx* ctx = nullptr;
+  DevScene dev{};
+  std::vector<void*> allocs;
+  uint64_t bytes = 0;
+  ne_scene_info info{};
+};
+
+struct ne_film {
+  ne_ctx* ctx = nullptr;
+  int W = 0, H = 0;
+  double* d_sum = nullptr;
+};
+
+namespace {
+
+thread_local std::string g_err;
+
+ne_status fail(ne_ctx* c, ne_status st, const std::string& msg) {
+  if (c) c->err = msg; else g_err = msg;
+  return st;
+}
+
+#define NE_HIP(ctx, call)                                                                          \
+  do {                                                                                             \
+    hipError_t e_ = (call);                                                                        \
+    if (e_ != hipSuccess)                                                                          \
+      return fail(ctx, e_ == hipErrorOutOfMemory ? NE_ERR_OUT_OF_MEMORY : NE_ERR_HIP,              \
+                  std::string(#call) + ": " + hipGetErrorString(e_));                             \
+  } while (0)
+
+template <class T>
+ne_status dev_upload(ne_ctx* c, ne_scene* s, const T* src, size_t count, const T** dst) {
+  *dst = nullptr;
+  if (count == 0) return NE_OK;
+  void* p = nullptr;
+  size_t bytes = count * sizeof(T);
+  NE_HIP(c, hipMalloc(&p, bytes));
+  s->allocs.push_back(p);
+  s->bytes += bytes;
+  NE_HIP(c, hipMemcpyAsync(p, src, bytes, hipMemcpyHostToDevice, c->stream));
+  *dst = static_cast<const T*>(p);
+  return NE_OK;
+}
+
+// scoped device buffers for the parity entry points
+struct DevBuf {
+  void* p = nullptr;
+  ~DevBuf() { if (p) (void)hipFree(p); }
+};
+
+template <class T>
+ne_status tmp_in(ne_ctx* c, DevBuf& b, const T* src, size_t count) {
+  if (count == 0 || src == nullptr) return NE_OK;
+  NE_HIP(c, hipMalloc(&b.p, count * sizeof(T)));
+  NE_HIP(c, hipMemcpyAsync(b.p, src, count * sizeof(T), hipMemcpyHostToDevice, c->stream));
+  return NE_OK;
+}
+template <class T>
+ne_status tmp_out(ne_ctx* c, DevBuf& b, size_t count) {
+  NE_HIP(c, hipMalloc(&b.p, std::max<size_t>(count, 1) * sizeof(T)));
+  return NE_OK;
+}
+template <class T>
+ne_status tmp_back(ne_ctx* c, T* dst, const DevBuf& b, size_t count) {
+  NE_HIP(c, hipMemcpyAsync(dst, b.p, count * sizeof(T), hipMemcpyDeviceToHost, c->stream));
+  return NE_OK;
+}
+
+inline float len3(const float* v) { return std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }
+
+// Same operation order as the device/oracle: length(cross(eu, ev)).
+inline float rect_area(const float* eu, const float* ev) {
+  float c[3] = {eu[1] * ev[2] - ev[1] * eu[2], eu[2] * ev[0] - ev[2] * eu[0], eu[0] * ev[1] - ev[0] * eu[1]};
+  return std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
+}
+
+ne_status validate_desc(ne_ctx* c, const ne_scene_desc* d) {
+  if (!d) return fail(c, NE_ERR_INVALID_ARG, "scene descriptor is NULL");
+  if (d->n_tri && !d->tri_v) return fail(c, NE_ERR_INVALID_ARG, "tri_v is NULL");
+  if (d->n_sph && (!d->sph || !d->sph_mat)) return fail(c, NE_ERR_INVALID_ARG, "sphere arrays are NULL");
+  if (d->n_rect && (!d->rect || !d->rect_mat)) return fail(c, NE_ERR_INVALID_ARG, "rectangle arrays are NULL");
+  if ((d->n_tri || d->n_sph || d->n_rect) && (d->n_mat == 0 || !d->mat))
+    return fail(c, NE_ERR_INVALID_ARG, "surfaces need at least one material");
+  if (uint64_t(d->n_tri) + d->n_sph + d->n_rect >= (1u << 30))
+    return fail(c, NE_ERR_INVALID_ARG, "too many primitives (limit 2^30)");
+  auto chk_mat = [&](int32_t m) { return m >= 0 && uint32_t(m) < d->n_mat; };
+  for (uint32_t i = 0; i < d->n_tri; ++i)
+    if (!chk_mat(d->tri_mat ? d->tri_mat[i] : 0)) return fail(c, NE_ERR_INVALID_ARG, "triangle material out of range");
+  for (uint32_t i = 0; i < d->n_sph; ++i) {
+    if (!chk_mat(d->sph_mat[i])) return fail(c, NE_ERR_INVALID_ARG, "sphere material out of range");
+    if (!(d->sph[4 * i + 3] > 0)) return fail(c, NE_ERR_INVALID_ARG, "sphere radius must be > 0");
+  }
+  for (uint32_t i = 0; i < d->n_rect; ++i) {
+    if (!chk_mat(d->rect_mat[i])) return fail(c, NE_ERR_INVALID_ARG, "rectangle material out of range");
+    if (!(rect_area(d->rect + 12 * i + 3, d->rect + 12 * i + 6) > 0))
+      return fail(c, NE_ERR_INVALID_ARG, "degenerate rectangle");
+  }
+  for (uint32_t i = 0; i < d->n_mat; ++i) {
+    float t = d->mat[12 * i];
+    if (t != float(NE_MAT_GLOSSY) && t != float(NE_MAT_EMITTER))
+      return fail(c, NE_ERR_INVALID_ARG, "unknown material type");
+  }
+  if (d->has_volume) {
+    if (!d->grid) return fail(c, NE_ERR_INVALID_ARG, "volume grid is NULL");
+    for (int a = 0; a < 3; ++a)
+      if (d->vol_res[a] <= 0 || d->vol_res[a] > 4096) return fail(c, NE_ERR_INVALID_ARG, "volume resolution out of range");
+    if (d->maj_res < 1 || d->maj_res > 256) return fail(c, NE_ERR_INVALID_ARG, "majorant resolution out of range");
+    float st = 0;
+    for (int a = 0; a < 3; ++a) st += d->vol_sigma_s[a] + d->vol_sigma_a[a];
+    if (!(st > 0) || !(d->vol_mult > 0)) return fail(c, NE_ERR_INVALID_ARG, "volume extinction must be > 0");
+    for (int a = 0; a < 3; ++a)
+      if (!(d->vol_sigma_s[a] + d->vol_sigma_a[a] > 0)) return fail(c, NE_ERR_INVALID_ARG, "per-channel extinction must be > 0");
+    if (!(d->vol_g > -1.0f && d->vol_g < 1.0f)) return fail(c, NE_ERR_INVALID_ARG, "HG g must be in (-1, 1)");
+    if (!std::isfinite(d->vol_mult)) return fail(c, NE_ERR_INVALID_ARG, "density multiplier must be finite");
+    // Tracking loops step by -log(1 - xi) / majorant: a NaN or infinite voxel
+    // would stall them, and negative densities break the majorant bound.
+    const size_t nvox = size_t(d->vol_res[0]) * d->vol_res[1] * d->vol_res[2];
+    for (size_t i = 0; i < nvox; ++i)
+      if (!(d->grid[i] >= 0.0f) || !std::isfinite(d->grid[i]))
+        return fail(c, NE_ERR_INVALID_ARG, "grid densities must be finite and >= 0");
+  }
+  if (d->env_type < 0 || d->env_type > 2) return fail(c, NE_ERR_INVALID_ARG, "unknown environment type");
+  if (d->has_sun && !(len3(d->sun_dir) > 0)) return fail(c, NE_ERR_INVALID_ARG, "sun direction is zero");
+  return NE_OK;
+}
+
+ne_status check_params(ne_ctx* c, const ne_render_params* p) {
+  if (!p) return fail(c, NE_ERR_INVALID_ARG, "params is NULL");
+  if (p->width <= 0 || p->height <= 0 || p->width > 32768 || p->height > 32768)
+    return fail(c, NE_ERR_INVALID_ARG, "image size out of range");
+  if (p->x0 < 0 || p->y0 < 0 || p->x1 > p->width || p->y1 > p->height || p->x0 >= p->x1 || p->y0 >= p->y1)
+    return fail(c, NE_ERR_INVALID_ARG, "pixel rectangle out of range or empty");
+  if (p->spp <= 0) return fail(c, NE_ERR_INVALID_ARG, "spp must be > 0");
+  if (p->max_bounces < 0 || p->max_bounces > 4096) return fail(c, NE_ERR_INVALID_ARG, "max_bounces out of range");
+  if (p->precision != NE_FP32 && p->precision != NE_FP64) return fail(c, NE_ERR_INVALID_ARG, "unknown precision");
+  if (p->block_size != 0 && p->block_size != 64 && p->block_size != 128 && p->block_size != 256)
+    return fail(c, NE_ERR_INVALID_ARG, "block_size must be 0, 64, 128 or 256");
+  if (p->samples_per_thread < 0) return fail(c, NE_ERR_INVALID_ARG, "samples_per_thread must be >= 0");
+  return NE_OK;
+}
+
+// World-space bounds of every surface primitive: triangles, then spheres,
+// then rectangles (the BVH builder's input index space).
+std::vector<PrimInput> prim_bounds(const ne_scene_desc* d) {
+  const uint32_t nt = d->n_tri, ns = d->n_sph, nr = d->n_rect;
+  std::vector<PrimInput> prims(size_t(nt) + ns + nr);
+  for (uint32_t i = 0; i < nt; ++i) {
+    const float* v = d->tri_v + 9 * size_t(i);
+    for (int a = 0; a < 3; ++a) {
+      prims[i].bmin[a] = std::min(v[a], std::min(v[3 + a], v[6 + a]));
+      prims[i].bmax[a] = std::max(v[a], std::max(v[3 + a], v[6 + a]));
+    }
+  }
+  for (uint32_t i = 0; i < ns; ++i) {
+    const float* q = d->sph + 4 * size_t(i);
+    for (int a = 0; a < 3; ++a) { prims[nt + i].bmin[a] = q[a] - q[3]; prims[nt + i].bmax[a] = q[a] + q[3]; }
+  }
+  for (uint32_t i = 0; i < nr; ++i) {
+    const float* r = d->rect + 12 * size_t(i);
+    for (int a = 0; a < 3; ++a) {
+      float c0 = r[a], c1 = r[a] + r[3 + a], c2 = r[a] + r[6 + a], c3 = r[a] + r[3 + a] + r[6 + a];
+      prims[nt + ns + i].bmin[a] = std::min(std::min(c0, c1), std::min(c2, c3));
+      prims[nt + ns + i].bmax[a] = std::max(std::max(c0, c1), std::max(c2, c3));
+    }
+  }
+  return prims;
+}
+
+DevCamera make_camera(const ne_camera_desc* cd) {
+  // Camera::Camera (src/core/Camera.cpp:7-26) with focus distance 1 and no lens.
+  auto sub = [](const float* a, const float* b, float* o) { for (int i = 0; i < 3; ++i) o[i] = a[i] - b[i]; };
+  auto nrm = [](float* v) {
+    float d = v[0] * v[0] + v[1] * v[1] + v[2] * v[2];
+    float s = 1.0f / std::sqrt(d);
+    v[0] *= s; v[1] *= s; v[2] *= s;
+  };
+  float front[3], side[3];
+  sub(cd->position, cd->look_at, front);
+  nrm(front);
+  const float* up = cd->up;
+  side[0] = up[1] * front[2] - front[1] * up[2];
+  side[1] = up[2] * front[0] - front[2] * up[0];
+  side[2] = up[0] * front[1] - front[0] * up[1];
+  nrm(side);
+  const float focus = 1.0f;
+  float halfHeight = std::tan(float(cd->vfov * kPI / 180.0) / 2.0f);
+  float halfWidth = cd->aspect * halfHeight;
+  DevCamera c;
+  for (int i = 0; i < 3; ++i) {
+    c.pos[i] = cd->position[i];
+    float a = focus * halfWidth * (-side[i]);
+    float b = focus * halfHeight * up[i];
+    float f = focus * (-front[i]);
+    c.lower_left[i] = ((cd->position[i] - a) - b) - f;
+    c.horizontal[i] = 2.0f * focus * halfWidth * (-side[i]);
+    c.vertical[i] = 2.0f * focus * halfHeight * up[i];
+  }
+  return c;
+}
+
+}  // namespace
+
+extern "C" {
+
+int32_t ne_sizeof_scene_desc(void) { return int32_t(sizeof(ne_scene_desc)); }
+
+const char* ne_last_error(const ne_ctx* ctx) { return ctx ? ctx->err.c_str() : g_err.c_str(); }
+
+ne_status ne_bvh_build(const ne_scene_desc* d, float* nodes, uint32_t* refs, uint32_t* n_nodes, uint32_t* n_refs,
+                       uint32_t* depth) {
+  ne_status st = validate_desc(nullptr, d);
+  if (st != NE_OK) return st;
+  BvhOutput bvh;
+  build_bvh(prim_bounds(d), bvh);
+  if (n_nodes) *n_nodes = uint32_t(bvh.nodes.size() / 16);
+  if (n_refs) *n_refs = uint32_t(bvh.refs.size());
+  if (depth) *depth = bvh.max_depth;
+  if (nodes) std::memcpy(nodes, bvh.nodes.data(), bvh.nodes.size() * sizeof(float));
+  if (refs) {
+    const uint32_t nt = d->n_tri, ns = d->n_sph;
+    for (size_t i = 0; i < bvh.refs.size(); ++i) {
+      uint32_t p = bvh.refs[i];
+      refs[i] = p < nt ? p : (p < nt + ns ? ((uint32_t(PRIM_SPHERE) << 30) | (p - nt))
+                                          : ((uint32_t(PRIM_RECT) << 30) | (p - nt - ns)));
+    }
+  }
+  return NE_OK;
+}
+
+ne_status ne_ctx_create(int32_t device, ne_ctx** out) {
+  if (!out) return fail(nullptr, NE_ERR_INVALID_ARG, "out is NULL");
+  *out = nullptr;
+  int n = 0;
+  if (hipGetDeviceCount(&n) != hipSuccess || n == 0) return fail(nullptr, NE_ERR_NO_DEVICE, "no HIP device");
+  if (device < 0 || device >= n) return fail(nullptr, NE_ERR_INVALID_ARG, "device index out of range");
+  ne_ctx* c = new ne_ctx();
+  c->device = device;
+  if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+      hipMalloc(&c->d_counters, sizeof(DevCounters)) != hipSuccess || hipEventCreate(&c->ev0) != hipSuccess ||
+      hipEventCreate(&c->ev1) != hipSuccess) {
+    delete c;
+    return fail(nullptr, NE_ERR_HIP, "context setup failed");
+  }
+  *out = c;
+  return NE_OK;
+}
+
+void ne_ctx_destroy(ne_ctx* c) {
+  if (!c) return;
+  (void)hipSetDevice(c->device);
+  if (c->stream) (void)hipStreamSynchronize(c->stream);
+  if (c->d_partial) (void)hipFree(c->d_partial);
+  if (c->d_counters) (void)hipFree(c->d_counters);
+  if (c->ev0) (void)hipEventDestroy(c->ev0);
+  if (c->ev1) (void)hipEventDestroy(c->ev1);
+  if (c->stream) (void)hipStreamDestroy(c->stream);
+  delete c;
+}
+
+ne_status ne_synchronize(ne_ctx* c) {
+  if (!c) return fail(nullptr, NE_ERR_INVALID_ARG, "ctx is NULL");
+  std::lock_guard<std::mutex> lk(c->mu);
+  NE_HIP(c, hipSetDevice(c->device));
+  NE_HIP(c, hipStreamSynchronize(c->stream));
+  return NE_OK;
+}
+
+ne_status ne_scene_upload(ne_ctx* c, const ne_scene_desc* d, ne_scene** out) {
+  if (!c || !out) return fail(c, NE_ERR_INVALID_ARG, "ctx/out is NULL");
+  *out = nullptr;
+  std::lock_guard<std::mutex> lk(c->mu);
+  ne_status st = validate_desc(c, d);
+  if (st != NE_OK) return st;
+  NE_HIP(c, hipSetDevice(c->device));
+  ne_scene* s = new ne_scene();
+  s->ctx = c;
+  auto bail = [&](ne_status e) {
+    (void)hipStreamSynchronize(c->stream);
+    for (void* p : s->allocs) (void)hipFree(p);
+    delete s;
+    return e;
+  };
+  DevScene& ds = s->dev;
+  // ---- surfaces: BVH over triangles, spheres, rectangles
+  const uint32_t nt = d->n_tri, ns = d->n_sph, nr = d->n_rect;
+  BvhOutput bvh;
+  build_bvh(prim_bounds(d), bvh);
+  if (bvh.max_depth + 1 > uint32_t(kShortStack + kSpillStack)) return bail(fail(c, NE_ERR_UNSUPPORTED, "BVH too deep"));
+  std::vector<uint32_t> refs(bvh.refs.size());
+  std::vector<float> tris(size_t(nt) * 12);
+  uint32_t tslot = 0;
+  for (size_t i = 0; i < bvh.refs.size(); ++i) {
+    uint32_t p = bvh.refs[i];
+    if (p < nt) {
+      const float* v = d->tri_v + 9 * size_t(p);
+      float* o = &tris[size_t(tslot) * 12];
+      int32_t m = d->tri_mat ? d->tri_mat[p] : 0;
+      o[0] = v[0]; o[1] = v[1]; o[2] = v[2]; std::memcpy(&o[3], &m, 4);
+      o[4] = v[3] - v[0]; o[5] = v[4] - v[1]; o[6] = v[5] - v[2]; std::memcpy(&o[7], &p, 4);
+      o[8] = v[6] - v[0]; o[9] = v[7] - v[1]; o[10] = v[8] - v[2]; o[11] = 0.0f;
+      refs[i] = (uint32_t(PRIM_TRI) << 30) | tslot;
+      ++tslot;
+    } else if (p < nt + ns) {
+      refs[i] = (uint32_t(PRIM_SPHERE) << 30) | (p - nt);
+    } else {
+      refs[i] = (uint32_t(PRIM_RECT) << 30) | (p - nt - ns);
+    }
+  }
+  std::vector<float> sph(size_t(ns) * 4), rects(size_t(nr) * 16);
+  for (uint32_t i = 0; i < ns; ++i) std::memcpy(&sph[4 * size_t(i)], d->sph + 4 * size_t(i), 16);
+  std::vector<int32_t> light_rects;
+  for (uint32_t i = 0; i < nr; ++i) {
+    const float* r = d->rect + 12 * size_t(i);
+    float* o = &rects[16 * size_t(i)];
+    int32_t m = d->rect_mat[i];
+    o[0] = r[0]; o[1] = r[1]; o[2] = r[2]; std::memcpy(&o[3], &m, 4);
+    o[4] = r[3]; o[5] = r[4]; o[6] = r[5]; o[7] = rect_area(r + 3, r + 6);
+    o[8] = r[6]; o[9] = r[7]; o[10] = r[8]; o[11] = 0.0f;
+    o[12] = r[9]; o[13] = r[10]; o[14] = r[11]; o[15] = 0.0f;
+    if (d->mat[12 * size_t(m)] == float(NE_MAT_EMITTER)) light_rects.push_back(int32_t(i));
+  }
+  const float4* f4 = nullptr;
+  if ((st = dev_upload(c, s, reinterpret_cast<const float4*>(bvh.nodes.data()), bvh.nodes.size() / 4, &f4)) != NE_OK)
+    return bail(st);
+  ds.nodes = f4;
+  ds.n_nodes = uint32_t(bvh.nodes.size() / 16);
+  if ((st = dev_upload(c, s, refs.data(), refs.size(), &ds.refs)) != NE_OK) return bail(st);
+  if ((st = dev_upload(c, s, reinterpret_cast<const float4*>(tris.data()), tris.size() / 4, &ds.tris)) != NE_OK)
+    return bail(st);
+  if ((st = dev_upload(c, s, reinterpret_cast<const float4*>(sph.data()), sph.size() / 4, &ds.sph)) != NE_OK)
+    return bail(st);
+  if ((st = dev_upload(c, s, d->sph_mat, ns, &ds.sph_mat)) != NE_OK) return bail(st);
+  if ((st = dev_upload(c, s, reinterpret_cast<const float4*>(rects.data()), rects.size() / 4, &ds.rects)) != NE_OK)
+    return bail(st);
+  if ((st = dev_upload(c, s, d->mat, size_t(d->n_mat) * 12, &ds.mats)) != NE_OK) return bail(st);
+  if ((st = dev_upload(c, s, light_rects.data(), light_rects.size(), &ds.light_rects)) != NE_OK) return bail(st);
+  ds.n_light_rects = int32_t(light_rects.size());
+  // ---- volume
+  ds.has_volume = d->has_volume ? 1 : 0;
+  float gmax = 0.0f;
+  if (d->has_volume) {
+    for (int a = 0; a < 3; ++a) ds.res[a] = d->vol_res[a];
+    for (int i = 0; i < 12; ++i) ds.inv[i] = d->vol_inv[i];
+    float st3[3];
+    for (int a = 0; a < 3; ++a) {
+      st3[a] = d->vol_sigma_a[a] + d->vol_sigma_s[a];
+      ds.sig_s[a] = d->vol_sigma_s[a];
+      ds.sig_t[a] = d->vol_sigma_s[a] + d->vol_sigma_a[a];
+    }
+    // avg(extinction * densityMultiplier), src/materials/GridMedia.cpp:82
+    ds.sigma_bar = (st3[0] * d->vol_mult + st3[1] * d->vol_mult + st3[2] * d->vol_mult) / 3.0f;
+    ds.g = d->vol_g;
+    ds.mres = d->maj_res;
+    size_t nvox = size_t(ds.res[0]) * ds.res[1] * ds.res[2];
+    if ((st = dev_upload(c, s, d->grid, nvox, &ds.grid)) != NE_OK) return bail(st);
+    size_t ncell = size_t(ds.mres) * ds.mres * ds.mres;
+    void* pm = nullptr;
+    NE_HIP(c, hipMalloc(&pm, ncell * sizeof(float)));
+    s->allocs.push_back(pm);
+    s->bytes += ncell * sizeof(float);
+    NE_HIP(c, launch_majorants(ds, static_cast<float*>(pm), c->stream));
+    ds.maj = static_cast<const float*>(pm);
+    std::vector<float> mh(ncell);
+    NE_HIP(c, hipMemcpyAsync(mh.data(), pm, ncell * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    NE_HIP(c, hipStreamSynchronize(c->stream));
+    for (float v : mh) gmax = std::max(gmax, v);
+    gmax = ds.sigma_bar > 0 ? gmax / ds.sigma_bar : 0.0f;
+  }
+  // ---- lights
+  ds.has_sun = d->has_sun ? 1 : 0;
+  for (int a = 0; a < 3; ++a) { ds.sun_dir[a] = d->sun_dir[a]; ds.sun_E[a] = d->sun_E[a]; }
+  ds.env_type = d->env_type;
+  for (int a = 0; a < 3; ++a) { ds.env[a] = d->env_c0[a]; ds.env[3 + a] = d->env_c1[a]; ds.env[6 + a] = d->env_c2[a]; }
+  ds.n_lights = ds.has_sun + (ds.env_type != NE_ENV_NONE ? 1 : 0) + ds.n_light_rects;
+  NE_HIP(c, hipStreamSynchronize(c->stream));
+  s->info.bvh_nodes = ds.n_nodes;
+  s->info.bvh_depth = bvh.max_depth;
+  s->info.bvh_leaf_prims_max = bvh.max_leaf;
+  s->info.maj_res = uint32_t(d->has_volume ? d->maj_res : 0);
+  s->info.device_bytes = s->bytes;
+  s->info.grid_max_density = gmax;
+  *out = s;
+  return NE_OK;
+}
+
+void ne_scene_release(ne_scene* s) {
+  if (!s) return;
+  std::lock_guard<std::mutex> lk(s->ctx->mu);
+  (void)hipSetDevice(s->ctx->device);
+  (void)hipStreamSynchronize(s->ctx->stream);
+  for (void* p : s->allocs) (void)hipFree(p);
+  delete s;
+}
+
+ne_status ne_scene_get_info(const ne_scene* s, ne_scene_info* out) {
+  if (!s || !out) return fail(nullptr, NE_ERR_INVALID_ARG, "scene/out is NULL");
+  *out = s->info;
+  return NE_OK;
+}
+
+ne_status ne_film_create(ne_ctx* c, int32_t w, int32_t h, ne_film** out) {
+  if (!c || !out) return fail(c, NE_ERR_INVALID_ARG, "ctx/out is NULL");
+  *out = nullptr;
+  if (w <= 0 || h <= 0 || w > 32768 || h > 32768) return fail(c, NE_ERR_INVALID_ARG, "film size out of range");
+  std::lock_guard<std::mutex> lk(c->mu);
+  NE_HIP(c, hipSetDevice(c->device));
+  ne_film* f = new ne_film();
+  f->ctx = c; f->W = w; f->H = h;
+  size_t bytes = size_t(w) * h * 3 * sizeof(double);
+  hipError_t e = hipMalloc(&f->d_sum, bytes);
+  if (e != hipSuccess) { delete f; return fail(c, NE_ERR_OUT_OF_MEMORY, "film allocation failed"); }
+  e = hipMemsetAsync(f->d_sum, 0, bytes, c->stream);
+  if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
+  if (e != hipSuccess) { (void)hipFree(f->d_sum); delete f; return fail(c, NE_ERR_HIP, "film clear failed"); }
+  *out = f;
+  return NE_OK;
+}
+
+void ne_film_release(ne_film* f) {
+  if (!f) return;
+  std::lock_guard<std::mutex> lk(f->ctx->mu);
+  (void)hipSetDevice(f->ctx->device);
+  (void)hipStreamSynchronize(f->ctx->stream);
+  (void)hipFree(f->d_sum);
+  delete f;
+}
+
+ne_status ne_film_clear(ne_film* f) {
+  if (!f) return fail(nullptr, NE_ERR_INVALID_ARG, "film is NULL");
+  ne_ctx* c = f->ctx;
+  std::lock_guard<std::mutex> lk(c->mu);
+  NE_HIP(c, hipSetDevice(c->device));
+  NE_HIP(c, hipMemsetAsync(f->d_sum, 0, size_t(f->W) * f->H * 3 * sizeof(double), c->stream));
+  return NE_OK;
+}
+
+ne_status ne_film_read(ne_film* f, double* sum) {
+  if (!f || !sum) return fail(f ? f->ctx : nullptr, NE_ERR_INVALID_ARG, "film/sum is NULL");
+  ne_ctx* c = f->ctx;
+  std::lock_guard<std::mutex> lk(c->mu);
+  NE_HIP(c, hipSetDevice(c->device));
+  NE_HIP(c, hipMemcpyAsync(sum, f->d_sum, size_t(f->W) * f->H * 3 * sizeof(double), hipMemcpyDeviceToHost, c->stream));
+  NE_HIP(c, hipStreamSynchronize(c->stream));
+  return NE_OK;
+}
+
+ne_status ne_film_tonemap(ne_film* f, int32_t spp, float* pixels) {
+  if (!f || !pixels || spp <= 0) return fail(f ? f->ctx : nullptr, NE_ERR_INVALID_ARG, "bad tonemap arguments");
+  ne_ctx* c = f->ctx;
+  std::lock_guard<std::mutex> lk(c->mu);
+  NE_HIP(c, hipSetDevice(c->device));
+  const int npix = f->W * f->H;
+  DevBuf b;
+  ne_status st = tmp_out<float>(c, b, size_t(npix) * 3);
+  if (st != NE_OK) return st;
+  NE_HIP(c, launch_tonemap(f->d_sum, npix, 1.0f / float(spp), static_cast<float*>(b.p), c->stream));
+  st = tmp_back(c, pixels, b, size_t(npix) * 3);
+  if (st != NE_OK) return st;
+  NE_HIP(c, hipStreamSynchronize(c->stream));
+  return NE_OK;
+}
+
+static ne_status render_locked(ne_ctx* c, const ne_scene* s, const ne_camera_desc* cam, const ne_render_params* p,
+                               ne_film* f) {
+  ne_status st = check_params(c, p);
+  if (st != NE_OK) return st;
+  if (!s || !cam || !f) return fail(c, NE_ERR_INVALID_ARG, "scene/camera/film is NULL");
+  if (s->ctx != c || f->ctx != c) return fail(c, NE_ERR_INVALID_ARG, "scene/film belong to another context");
+  if (f->W != p->width || f->H != p->height) return fail(c, NE_ERR_INVALID_ARG, "film size differs from params");
+  NE_HIP(c, hipSetDevice(c->device));
+  RenderArgs a{};
+  a.W = p->width; a.H = p->height; a.x0 = p->x0; a.y0 = p->y0; a.x1 = p->x1; a.y1 = p->y1;
+  a.spp = p->spp; a.max_bounces = p->max_bounces; a.sample_offset = p->sample_offset; a.job = p->job;
+  a.seed = p->seed;
+  const int64_t npix = int64_t(p->x1 - p->x0) * (p->y1 - p->y0);
+  int S = p->samples_per_thread;
+  if (S <= 0) {
+    const int64_t target_threads = int64_t(1) << 22;
+    int64_t want = (npix * p->spp + target_threads - 1) / target_threads;
+    S = int(std::min<int64_t>(std::max<int64_t>(want, 1), p->spp));
+  }
+  S = std::min(S, p->spp);
+  a.S = S;
+  a.nchunks = (p->spp + S - 1) / S;
+  a.ntx = (p->x1 - p->x0 + 7) / 8;
+  a.nty = (p->y1 - p->y0 + 7) / 8;
+  a.counters = c->d_counters;
+  const bool count = (p->flags & NE_FLAG_COUNT) != 0;
+  if (count) {
+    NE_HIP(c, hipMemsetAsync(c->d_counters, 0, sizeof(DevCounters), c->stream));
+    c->counters_valid = true;
+  }
+  if (a.nchunks == 1) {
+    a.out = f->d_sum;
+  } else {
+    size_t need = size_t(npix) * size_t(a.nchunks) * 3 * sizeof(double);
+    if (need > c->partial_bytes) {
+      if (c->d_partial) {
+        NE_HIP(c, hipStreamSynchronize(c->stream));
+        NE_HIP(c, hipFree(c->d_partial));
+        c->d_partial = nullptr;
+        c->partial_bytes = 0;
+      }
+      NE_HIP(c, hipMalloc(&c->d_partial, need));
+      c->partial_bytes = need;
+    }
+    a.out = c->d_partial;
+  }
+  DevCamera dc = make_camera(cam);
+  int block = p->block_size ? p->block_size : 128;
+  const bool timed = (p->flags & NE_FLAG_TIME) != 0;
+  if (timed) NE_HIP(c, hipEventRecord(c->ev0, c->stream));
+  NE_HIP(c, launch_render(s->dev, dc, a, p->precision, count, block, c->stream));
+  if (timed) {
+    NE_HIP(c, hipEventRecord(c->ev1, c->stream));
+    c->timed = true;
+  }
+  if (a.nchunks > 1) NE_HIP(c, launch_reduce_partials(a, f->d_sum, c->stream));
+  return NE_OK;
+}
+
+ne_status ne_render(ne_ctx* c, const ne_scene* s, const ne_camera_desc* cam, const ne_render_params* p, ne_film* f) {
+  if (!c) return fail(nullptr, NE_ERR_INVALID_ARG, "ctx is NULL");
+  std::lock_guard<std::mutex> lk(c->mu);
+  return render_locked(c, s, cam, p, f);
+}
+
+ne_status ne_render_tile(ne_ctx* c, const ne_scene* s, const ne_camera_desc* cam, const ne_render_params* p,
+                         int32_t index, ne_film* f, float* pixels) {
+  if (!c || !p || !pixels || !f) return fail(c, NE_ERR_INVALID_ARG, "NULL argument");
+  // OfflineEngine tiling: tileSize = resolution / (40, 10), tile (index % 40, index / 40)
+  const int tw = p->width / 40, th = p->height / 10;
+  if (index < 0 || index >= 400) return fail(c, NE_ERR_INVALID_ARG, "tile index out of range [0, 400)");
+  if (tw <= 0 || th <= 0) return fail(c, NE_ERR_INVALID_ARG, "image smaller than the 40x10 tile grid");
+  std::lock_guard<std::mutex> lk(c->mu);
+  ne_render_params q = *p;
+  q.x0 = (index % 40) * tw; q.y0 = (index / 40) * th; q.x1 = q.x0 + tw; q.y1 = q.y0 + th;
+  ne_status st = render_locked(c, s, cam, &q, f);
+  if (st != NE_OK) return st;
+  const int npix = f->W * f->H;
+  DevBuf b;
+  if ((st = tmp_out<float>(c, b, size_t(npix) * 3)) != NE_OK) return st;
+  NE_HIP(c, launch_tonemap(f->d_sum, npix, 1.0f / float(p->spp), static_cast<float*>(b.p), c->stream));
+  // copy the tile rows into the caller's full-frame pixel array
+  for (int y = q.y0; y < q.y1; ++y) {
+    size_t off = (size_t(y) * f->W + q.x0) * 3;
+    NE_HIP(c, hipMemcpyAsync(pixels + off, static_cast<float*>(b.p) + off, size_t(tw) * 3 * sizeof(float),
+                             hipMemcpyDeviceToHost, c->stream));
+  }
+  NE_HIP(c, hipStreamSynchronize(c->stream));
+  return NE_OK;
+}
+
+ne_status ne_get_kernel_time(ne_ctx* c, float* ms) {
+  if (!c || !ms) return fail(c, NE_ERR_INVALID_ARG, "ctx/ms is NULL");
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (!c->timed) return fail(c, NE_ERR_INVALID_ARG, "no render with NE_FLAG_TIME has been issued");
+  NE_HIP(c, hipSetDevice(c->device));
+  NE_HIP(c, hipEventSynchronize(c->ev1));
+  NE_HIP(c, hipEventElapsedTime(ms, c->ev0, c->ev1));
+  return NE_OK;
+}
+
+ne_status ne_get_counters(ne_ctx* c, ne_counters* out) {
+  if (!c || !out) return fail(c, NE_ERR_INVALID_ARG, "ctx/out is NULL");
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (!c->counters_valid) return fail(c, NE_ERR_INVALID_ARG, "no render with NE_FLAG_COUNT has been issued");
+  NE_HIP(c, hipSetDevice(c->device));
+  DevCounters h{};
+  NE_HIP(c, hipMemcpyAsync(&h, c->d_counters, sizeof(h), hipMemcpyDeviceToHost, c->stream));
+  NE_HIP(c, hipStreamSynchronize(c->stream));
+  out->paths = h.paths; out->ext_rays = h.ext_rays; out->shadow_rays = h.shadow_rays;
+  out->density_lookups = h.density_lookups; out->majorant_cells = h.majorant_cells;
+  out->bvh_nodes = h.bvh_nodes; out->prim_tests = h.prim_tests; out->reserved = 0;
+  return NE_OK;
+}
+
+ne_status ne_li(ne_ctx* c, const ne_scene* s, int32_t n, const float* rays, int32_t max_bounces, uint64_t seed,
+                uint32_t sample_index, float* radiance) {
+  if (!c || !s || n < 0 || (n > 0 && (!rays || !radiance))) return fail(c, NE_ERR_INVALID_ARG, "bad ne_li arguments");
+  if (n == 0) return NE_OK;
+  std::lock_guard<std::mutex> lk(c->mu);
+  NE_HIP(c, hipSetDevice(c->device));
+  DevBuf br, bo;
+  ne_status st;
+  if ((st = tmp_in(c, br, rays, size_t(n) * 6)) != NE_OK) return st;
+  if ((st = tmp_out<float>(c, bo, size_t(n) * 3)) != NE_OK) return st;
+  NE_HIP(c, launch_li(s->dev, n, static_cast<const float*>(br.p), max_bounces, seed, sample_index,
+                      static_cast<float*>(bo.p), c->stream));
+  if ((st = tmp_back(c, radiance, bo, size_t(n) * 3)) != NE_OK) return st;
+  NE_HIP(c, hipStreamSynchronize(c->stream));
+  return NE_OK;
+}
+
+// ---- parity entry points ---------------------------------------------------
+ne_status ne_test_primitive(ne_ctx* c, int32_t kind, int32_t n, const float* rays, const float* prims, int32_t* hit,
+                            float* t0, float* t1) {
+  if (!c || kind < 0 || kind > 3 || n < 0) return fail(c, NE_ERR_INVALID_ARG, "bad ne_test_primitive arguments");
+  if (n == 0) return NE_OK;
+  if (!rays || !prims || !hit || !t0 || !t1) return fail(c, NE_ERR_INVALID_ARG, "NULL buffer");
+  std::lock_guard<std::mutex> lk(c->mu);
+  NE_HIP(c, hipSetDevice(c->device));
+  const size_t pw = kind == 0 ? 6 : (kind == 2 ? 4 : 9);
+  DevBuf br, bp, bh, b0, b1;
+  ne_status st;
+  if ((st = tmp_in(c, br, rays, size_t(n) * 6)) || (st = tmp_in(c, bp, prims, size_t(n) * pw)) ||
+      (st = tmp_out<int32_t>(c, bh, n)) || (st = tmp_out<float>(c, b0, n)) || (st = tmp_out<float>(c, b1, n)))
+    return st;
+  NE_HIP(c, launch_test_primitive(kind, n, (const float*)br.p, (const float*)bp.p, (int32_t*)bh.p, (float*)b0.p,
+                                  (float*)b1.p, c->stream));
+  if ((st = tmp_back(c, hit, bh, n)) || (st = tmp_back(c, t0, b0, n)) || (st = tmp_back(c, t1, b1, n))) return st;
+  NE_HIP(c, hipStreamSynchronize(c->stream));
+  return NE_OK;
+}
+
+ne_status ne_test_closest(ne_ctx* c, const ne_scene* s, int32_t n, const float* rays, float tmin, float tmax, float* t,
+                          int32_t* type, int32_t* index) {
+  if (!c || !s || n < 0) return fail(c, NE_ERR_INVALID_ARG, "bad ne_test_closest arguments");
+  if (n == 0) return NE_OK;
+  if (!rays || !t || !type || !index) return fail(c, NE_ERR_INVALID_ARG, "NULL buffer");
+  std::lock_guard<std::mutex> lk(c->mu);
+  NE_HIP(c, hipSetDevice(c->device));
+  DevBuf br, bt, by, bi;
+  ne_status st;
+  if ((st = tmp_in(c, br, rays, size_t(n) * 6)) || (st = tmp_out<float>(c, bt, n)) ||
+      (st = tmp_out<int32_t>(c, by, n)) || (st = tmp_out<int32_t>(c, bi, n)))
+    return st;
+  NE_HIP(c, launch_test_closest(s->dev, n, (const float*)br.p, tmin, tmax, (float*)bt.p, (int32_t*)by.p,
+                                (int32_t*)bi.p, c->stream));
+  if ((st = tmp_back(c, t, bt, n)) || (st = tmp_back(c, type, by, n)) || (st = tmp_back(c, index, bi, n))) return st;
+  NE_HIP(c, hipStreamSynchronize(c->stream));
+  return NE_OK;
+}
+
+ne_status ne_test_transmittance(ne_ctx* c, const ne_scene* s, int32_t n, const float* rays, const float* tmax,
+                                uint64_t seed, int32_t precision, double* tr) {
+  if (!c || !s || n < 0 || (precision != 0 && precision != 1)) return fail(c, NE_ERR_INVALID_ARG, "bad arguments");
+  if (n == 0) return NE_OK;
+  if (!rays || !tmax || !tr) return fail(c, NE_ERR_INVALID_ARG, "NULL buffer");
+  std::lock_guard<std::mutex> lk(c->mu);
+  NE_HIP(c, hipSetDevice(c->device));
+  DevBuf br, bm, bo;
+  ne_status st;
+  if ((st = tmp_in(c, br, rays, size_t(n) * 6)) || (st = tmp_in(c, bm, tmax, n)) || (st = tmp_out<double>(c, bo, n)))
+    return st;
+  NE_HIP(c, launch_test_transmittance(s->dev, n, (const float*)br.p, (const float*)bm.p, seed, precision,
+                                      (double*)bo.p, c->stream));
+  if ((st = tmp_back(c, tr, bo, n))) return st;
+  NE_HIP(c, hipStreamSynchronize(c->stream));
+  return NE_OK;
+}
+
+ne_status ne_test_delta_track(ne_ctx* c, const ne_scene* s, int32_t n, const float* rays, uint64_t seed,
+                              int32_t precision, int32_t* scattered, double* t) {
+  if (!c || !s || n < 0 || (precision != 0 && precision != 1)) return fail(c, NE_ERR_INVALID_ARG, "bad arguments");
+  if (n == 0) return NE_OK;
+  if (!rays || !scattered || !t) return fail(c, NE_ERR_INVALID_ARG, "NULL buffer");
+  if (!s->dev.has_volume) return fail(c, NE_ERR_INVALID_ARG, "scene has no volume");
+  std::lock_guard<std::mutex> lk(c->mu);
+  NE_HIP(c, hipSetDevice(c->device));
+  DevBuf br, bs, bt;
+  ne_status st;
+  if ((st = tmp_in(c, br, rays, size_t(n) * 6)) || (st = tmp_out<int32_t>(c, bs, n)) || (st = tmp_out<double>(c, bt, n)))
+    return st;
+  NE_HIP(c, launch_test_delta(s->dev, n, (const float*)br.p, seed, precision, (int32_t*)bs.p, (double*)bt.p,
+                              c->stream));
+  if ((st = tmp_back(c, scattered, bs, n)) || (st = tmp_back(c, t, bt, n))) return st;
+  NE_HIP(c, hipStreamSynchronize(c->stream));
+  return NE_OK;
+}
+
+ne_status ne_test_density(ne_ctx* c, const ne_scene* s, int32_t n, const float* gcs, float* out) {
+  if (!c || !s || n < 0) return fail(c, NE_ERR_INVALID_ARG, "bad arguments");
+  if (n == 0) return NE_OK;
+  if (!gcs || !out) return fail(c, NE_ERR_INVALID_ARG, "NULL buffer");
+  if (!s->dev.has_volume) return fail(c, NE_ERR_INVALID_ARG, "scene has no volume");
+  std::lock_guard<std::mutex> lk(c->mu);
+  NE_HIP(c, hipSetDevice(c->device));
+  DevBuf bg, bo;
+  ne_status st;
+  if ((st = tmp_in(c, bg, gcs, size_t(n) * 3)) || (st = tmp_out<float>(c, bo, n))) return st;
+  NE_HIP(c, launch_test_density(s->dev, n, (const float*)bg.p, (float*)bo.p, c->stream));
+  if ((st = tmp_back(c, out, bo, n))) return st;
+  NE_HIP(c, hipStreamSynchronize(c->stream));
+  return NE_OK;
+}
+
+ne_status ne_test_bsdf(ne_ctx* c, int32_t kind, int32_t n, const float* in0, const float* in1, const float* in2,
+                       const float* in3, float* out) {
+  if (!c || kind < 0 || kind > 5 || n < 0) return fail(c, NE_ERR_INVALID_ARG, "bad ne_test_bsdf arguments");
+  if (n == 0) return NE_OK;
+  // element widths of (in0, in1, in2, in3, out) per kind
+  static const int w[6][5] = {{3, 3, 1, 0, 1}, {2, 1, 0, 0, 3}, {3, 1, 0, 0, 1},
+                              {3, 3, 1, 0, 1}, {5, 3, 3, 3, 7}, {3, 2, 1, 0, 4}};
+  const float* ins[4] = {in0, in1, in2, in3};
+  for (int k = 0; k < 4; ++k)
+    if (w[kind][k] && !ins[k]) return fail(c, NE_ERR_INVALID_ARG, "NULL input buffer");
+  if (!out) return fail(c, NE_ERR_INVALID_ARG, "NULL output buffer");
+  std::lock_guard<std::mutex> lk(c->mu);
+  NE_HIP(c, hipSetDevice(c->device));
+  DevBuf b[4], bo;
+  ne_status st;
+  for (int k = 0; k < 4; ++k)
+    if (w[kind][k] && (st = tmp_in(c, b[k], ins[k], size_t(n) * w[kind][k])) != NE_OK) return st;
+  if ((st = tmp_out<float>(c, bo, size_t(n) * w[kind][4]))) return st;
+  NE_HIP(c, launch_test_bsdf(kind, n, (const float*)b[0].p, (const float*)b[1].p, (const float*)b[2].p,
+                             (const float*)b[3].p, (float*)bo.p, c->stream));
+  if ((st = tmp_back(c, out, bo, size_t(n) * w[kind][4]))) return st;
+  NE_HIP(c, hipStreamSynchronize(c->stream));
+  return NE_OK;
+}
+
+ne_status ne_test_rng(ne_ctx* c, uint64_t seed, uint32_t pixel, uint32_t sample, uint32_t job, int32_t n, float* out) {
+  if (!c || n < 0 || (n > 0 && !out)) return fail(c, NE_ERR_INVALID_ARG, "bad ne_test_rng arguments");
+  if (n == 0) return NE_OK;
+  std::lock_guard<std::mutex> lk(c->mu);
+  NE_HIP(c, hipSetDevice(c->device));
+  DevBuf bo;
+  ne_status st;
+  if ((st = tmp_out<float>(c, bo, n))) return st;
+  NE_HIP(c, launch_test_rng(seed, pixel, sample, job, n, (float*)bo.p, c->stream));
+  if ((st = tmp_back(c, out, bo, n))) return st;
+  NE_HIP(c, hipStreamSynchronize(c->stream));
+  return NE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,23 @@
+// bvh_build.h -- host SAH builder interface.
+#pragma once
+#include <stdint.h>
+
+#include <vector>
+
+namespace ne {
+
+struct PrimInput {
+  float bmin[3], bmax[3];
+};
+
+struct BvhOutput {
+  std::vector<float> nodes;     // 16 floats (4 x float4) per node, ne_scene.h layout
+  std::vector<uint32_t> refs;   // leaf slots -> input primitive index (caller re-encodes)
+  std::vector<uint32_t> order;  // input primitive indices in leaf order
+  uint32_t max_depth = 0;       // inner-node levels (bounds the traversal stack)
+  uint32_t max_leaf = 0;
+};
+
+void build_bvh(const std::vector<PrimInput>& prims, BvhOutput& out);
+
+}  // namespace ne

@@ -1,0 +1,143 @@
+// ne_math.h -- host/device vector math and the counter-based RNG used by
+// every kernel of the path-tracing backend.
+//
+// Conventions follow the reference's glm usage (GLSL semantics):
+//   normalize(v) = v * (1 / sqrt(dot(v, v))), mix(x, y, a) = x*(1-a) + y*a,
+//   cross in glm's operand order.  All code is built with -ffp-contract=off
+//   so each multiply and add rounds on its own, which keeps device results
+//   bit-comparable with a plain C++ build of the same formulas.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define NE_HD __host__ __device__ __forceinline__
+#define NE_DEV __device__ __forceinline__
+
+namespace ne {
+
+constexpr double kPI = 3.14159265358979323846264338327950288;
+constexpr double kTWO_PI = 6.283185307179586476925286766559;
+constexpr double kFOUR_PI = 12.566370614359172953850573533118;
+constexpr float kINV4PI = float(0.07957747154594766788);
+constexpr double kEPS = 0.0000000001;  // EPSILON, src/utils/Math.h:20
+
+// ---- scalar overloads (float / double) ------------------------------------
+NE_HD float xsqrt(float x) { return sqrtf(x); }
+NE_HD double xsqrt(double x) { return sqrt(x); }
+NE_HD float xlog(float x) { return logf(x); }
+NE_HD double xlog(double x) { return log(x); }
+NE_HD float xsin(float x) { return sinf(x); }
+NE_HD double xsin(double x) { return sin(x); }
+NE_HD float xcos(float x) { return cosf(x); }
+NE_HD double xcos(double x) { return cos(x); }
+NE_HD float xfabs(float x) { return fabsf(x); }
+NE_HD double xfabs(double x) { return fabs(x); }
+NE_HD float xfloor(float x) { return floorf(x); }
+NE_HD double xfloor(double x) { return floor(x); }
+template <typename R> NE_HD R xmax(R a, R b) { return (a < b) ? b : a; }  // std::max
+template <typename R> NE_HD R xmin(R a, R b) { return (b < a) ? b : a; }  // std::min
+template <typename R> NE_HD R xinf();
+template <> NE_HD float xinf<float>() { return __builtin_huge_valf(); }
+template <> NE_HD double xinf<double>() { return __builtin_huge_val(); }
+NE_HD bool xisinf(float x) { return __builtin_isinf(x); }
+NE_HD bool xisinf(double x) { return __builtin_isinf(x); }
+
+// ---- vec3 -----------------------------------------------------------------
+template <typename R>
+struct Vec3 {
+  R x, y, z;
+};
+template <typename R> NE_HD Vec3<R> v3(R a, R b, R c) { return Vec3<R>{a, b, c}; }
+template <typename R> NE_HD Vec3<R> v3s(R a) { return Vec3<R>{a, a, a}; }
+template <typename R> NE_HD Vec3<R> v3f(const float* p) { return Vec3<R>{R(p[0]), R(p[1]), R(p[2])}; }
+template <typename R> NE_HD Vec3<R> v3f4(float4 p) { return Vec3<R>{R(p.x), R(p.y), R(p.z)}; }
+template <typename R, typename S> NE_HD Vec3<R> vcast(Vec3<S> a) { return Vec3<R>{R(a.x), R(a.y), R(a.z)}; }
+template <typename R> NE_HD Vec3<R> operator+(Vec3<R> a, Vec3<R> b) { return {a.x + b.x, a.y + b.y, a.z + b.z}; }
+template <typename R> NE_HD Vec3<R> operator-(Vec3<R> a, Vec3<R> b) { return {a.x - b.x, a.y - b.y, a.z - b.z}; }
+template <typename R> NE_HD Vec3<R> operator*(Vec3<R> a, Vec3<R> b) { return {a.x * b.x, a.y * b.y, a.z * b.z}; }
+template <typename R> NE_HD Vec3<R> operator/(Vec3<R> a, Vec3<R> b) { return {a.x / b.x, a.y / b.y, a.z / b.z}; }
+template <typename R> NE_HD Vec3<R> operator*(Vec3<R> a, R s) { return {a.x * s, a.y * s, a.z * s}; }
+template <typename R> NE_HD Vec3<R> operator*(R s, Vec3<R> a) { return {s * a.x, s * a.y, s * a.z}; }
+template <typename R> NE_HD Vec3<R> operator/(Vec3<R> a, R s) { return {a.x / s, a.y / s, a.z / s}; }
+template <typename R> NE_HD Vec3<R> operator-(Vec3<R> a) { return {-a.x, -a.y, -a.z}; }
+template <typename R> NE_HD R dot(Vec3<R> a, Vec3<R> b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+template <typename R> NE_HD Vec3<R> cross(Vec3<R> a, Vec3<R> b) {
+  return {a.y * b.z - b.y * a.z, a.z * b.x - b.z * a.x, a.x * b.y - b.x * a.y};
+}
+template <typename R> NE_HD R length(Vec3<R> a) { return xsqrt(dot(a, a)); }
+template <typename R> NE_HD Vec3<R> normalize(Vec3<R> a) { return a * (R(1) / xsqrt(dot(a, a))); }
+template <typename R> NE_HD R avg3(Vec3<R> v) { return (v.x + v.y + v.z) / R(3); }
+template <typename R> NE_HD bool is_black(Vec3<R> v) { return v.x == 0 && v.y == 0 && v.z == 0; }
+template <typename R> NE_HD R comp(Vec3<R> v, int a) { return a == 0 ? v.x : (a == 1 ? v.y : v.z); }
+NE_HD float mixf(float x, float y, float a) { return x * (1.0f - a) + y * a; }
+
+// generateOrthonormalCS (src/utils/Math.h:591-599): v, u span the plane of n.
+template <typename R>
+NE_HD void onb(Vec3<R> n, Vec3<R>& v, Vec3<R>& u) {
+  if (xfabs(n.x) > xfabs(n.y))
+    v = v3(-n.z, R(0), n.x) / xsqrt(n.x * n.x + n.z * n.z);
+  else
+    v = v3(R(0), n.z, -n.y) / xsqrt(n.y * n.y + n.z * n.z);
+  u = normalize(cross(n, v));
+}
+// toWorld / toLCS (src/utils/Math.h:612-631)
+template <typename R>
+NE_HD Vec3<R> to_world(Vec3<R> v, Vec3<R> ns, Vec3<R> ss, Vec3<R> ts) {
+  return v3(ss.x * v.x + ts.x * v.y + ns.x * v.z, ss.y * v.x + ts.y * v.y + ns.y * v.z,
+            ss.z * v.x + ts.z * v.y + ns.z * v.z);
+}
+template <typename R>
+NE_HD Vec3<R> to_lcs(Vec3<R> v, Vec3<R> ns, Vec3<R> ss, Vec3<R> ts) {
+  return v3(dot(v, ss), dot(v, ts), dot(v, ns));
+}
+// powerHeuristic (src/utils/Math.h:751-753)
+template <typename R> NE_HD R power_heuristic(R a, R b) { return (a * a) / (a * a + b * b); }
+
+// ---- Philox-4x32-10 (Salmon et al., SC'11) ---------------------------------
+NE_HD uint32_t mulhi32(uint32_t a, uint32_t b) {
+#if defined(__HIP_DEVICE_COMPILE__)
+  return __umulhi(a, b);
+#else
+  return uint32_t((uint64_t(a) * uint64_t(b)) >> 32);
+#endif
+}
+
+NE_HD void philox10(uint32_t c[4], uint32_t k0, uint32_t k1) {
+#pragma unroll
+  for (int r = 0; r < 10; ++r) {
+    if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+    uint32_t hi0 = mulhi32(0xD2511F53u, c[0]), lo0 = 0xD2511F53u * c[0];
+    uint32_t hi1 = mulhi32(0xCD9E8D57u, c[2]), lo1 = 0xCD9E8D57u * c[2];
+    uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
+    c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
+  }
+}
+
+// Per-path uniform stream.  Counter words: (pixel, sample, block, job);
+// key: the 64-bit seed.  One Philox call yields four 32-bit draws; a uniform
+// keeps the top 24 bits: u = (x >> 8) * 2^-24 in [0, 1).
+struct PathRng {
+  uint32_t pixel, sample, block, job, k0, k1;
+  uint32_t buf[4];
+  int idx;
+  NE_HD void init(uint64_t seed, uint32_t px, uint32_t smp, uint32_t jb) {
+    pixel = px; sample = smp; block = 0; job = jb;
+    k0 = uint32_t(seed); k1 = uint32_t(seed >> 32);
+    idx = 4;
+  }
+  NE_HD uint32_t next_u32() {
+    if (idx == 4) {
+      buf[0] = pixel; buf[1] = sample; buf[2] = block; buf[3] = job;
+      philox10(buf, k0, k1);
+      block += 1;
+      idx = 0;
+    }
+    // branch-free select keeps buf in registers
+    uint32_t r = idx == 0 ? buf[0] : (idx == 1 ? buf[1] : (idx == 2 ? buf[2] : buf[3]));
+    idx += 1;
+    return r;
+  }
+  NE_HD float next() { return float(next_u32() >> 8) * (1.0f / 16777216.0f); }
+};
+
+}  // namespace ne

@@ -1,0 +1,71 @@
+// ne_scene.h -- device-resident scene layout (POD, shared by host upload
+// code and kernels).
+//
+// HBM layout (all arrays 16-byte aligned):
+//   nodes   BVH2, 64 B per node = 4 x float4:
+//             f0 = (lmin.xyz, lmax.x)   f1 = (lmax.yz, rmin.xy)
+//             f2 = (rmin.z, rmax.xyz)   f3 = int(left, right, lcount, rcount)
+//           a child with count > 0 is a leaf over refs[index, index+count);
+//           count == 0 is an inner node index; index < 0 marks an empty slot.
+//   refs    one uint32 per leaf slot: (type << 30) | storage index
+//           (type 0 triangle, 1 sphere, 2 rectangle).
+//   tris    48 B per triangle in leaf order: (v0, material), (v1-v0, original
+//           index), (v2-v0, 0).  Original index = position in the input.
+//   sph     (centre, radius) + material array; rect 4 x float4:
+//           (p0, material), (eu, area), (ev, 0), (n, 0).
+//   mats    12 floats per material (see ne_gpu.h).
+//   grid    R32F x-fastest density; maj: mres^3 floats, sigma_t majorant per
+//           cell (grid value max over the cell's trilinear support x sigma_bar).
+#pragma once
+#include <stdint.h>
+
+namespace ne {
+
+enum { PRIM_TRI = 0, PRIM_SPHERE = 1, PRIM_RECT = 2 };
+
+// BVH traversal stack: kShortStack entries per lane in LDS ([depth][lane]
+// layout, no bank conflicts), deeper entries spill to a private array.  The
+// builder bounds tree depth so the total never overflows.
+constexpr int kShortStack = 16;
+constexpr int kSpillStack = 48;
+
+struct DevScene {
+  const float4* nodes;
+  const uint32_t* refs;
+  const float4* tris;
+  const float4* sph;
+  const int32_t* sph_mat;
+  const float4* rects;
+  const float* mats;
+  const int32_t* light_rects;
+  uint32_t n_nodes;
+  int32_t n_light_rects;
+  // volume
+  int32_t has_volume;
+  int32_t res[3];
+  int32_t mres;
+  float inv[12];          // WCS -> OCS, rows of a 3x4 matrix
+  const float* grid;
+  const float* maj;
+  float sigma_bar;        // avg(sigma_t * densityMultiplier)
+  float sig_s[3];
+  float sig_t[3];
+  float g;
+  // lights
+  int32_t has_sun;
+  float sun_dir[3];
+  float sun_E[3];
+  int32_t env_type;
+  float env[9];
+  int32_t n_lights;
+};
+
+struct DevCamera {
+  float pos[3], lower_left[3], horizontal[3], vertical[3];
+};
+
+struct DevCounters {
+  unsigned long long paths, ext_rays, shadow_rays, density_lookups, majorant_cells, bvh_nodes, prim_tests, pad;
+};
+
+}  // namespace ne

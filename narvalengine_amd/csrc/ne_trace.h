@@ -1,0 +1,730 @@
+// ne_trace.h -- device functions of the path tracer: primitive tests, BVH
+// traversal with an LDS short stack, majorant-grid delta/ratio tracking,
+// phase/BSDF sampling, and the per-path integrator.
+//
+// Everything is templated on R (float for the product path, double for the
+// converged-oracle mode) and on COUNT (event counters compiled in or out).
+#pragma once
+#include "ne_math.h"
+#include "ne_scene.h"
+
+namespace ne {
+
+// Event counters held in registers by one thread.
+template <bool COUNT>
+struct Count {
+  NE_DEV void ext() {}
+  NE_DEV void shadow() {}
+  NE_DEV void lookup() {}
+  NE_DEV void cell() {}
+  NE_DEV void node() {}
+  NE_DEV void prim() {}
+};
+template <>
+struct Count<true> {
+  uint32_t n_ext = 0, n_shadow = 0, n_lookup = 0, n_cell = 0, n_node = 0, n_prim = 0;
+  NE_DEV void ext() { ++n_ext; }
+  NE_DEV void shadow() { ++n_shadow; }
+  NE_DEV void lookup() { ++n_lookup; }
+  NE_DEV void cell() { ++n_cell; }
+  NE_DEV void node() { ++n_node; }
+  NE_DEV void prim() { ++n_prim; }
+};
+
+// ---------------------------------------------------------------------------
+// Reference-semantics primitive tests (FP32), used by the parity entry points.
+// ---------------------------------------------------------------------------
+// AABB::intersect, src/primitives/AABB.cpp:48-79
+NE_DEV bool ref_aabb(Vec3<float> bmin, Vec3<float> bmax, Vec3<float> o, Vec3<float> d, float& tn, float& tf) {
+  Vec3<float> s = v3((d.x < 0.0) ? 1.0f : -1.0f, (d.y < 0.0) ? 1.0f : -1.0f, (d.z < 0.0) ? 1.0f : -1.0f);
+  Vec3<float> invD = v3s(1.0f) / d;
+  Vec3<float> size = bmax - bmin;
+  Vec3<float> half = size / 2.0f;
+  Vec3<float> center = bmax - size / 2.0f;
+  Vec3<float> t1 = (center - o + s * half) * invD;
+  Vec3<float> t2 = (center - o - s * half) * invD;
+  Vec3<float> mn = v3(xmin(t1.x, t2.x), xmin(t1.y, t2.y), xmin(t1.z, t2.z));
+  Vec3<float> mx = v3(xmax(t1.x, t2.x), xmax(t1.y, t2.y), xmax(t1.z, t2.z));
+  float a = xmax(xmax(mn.x, mn.y), mn.z);
+  float b = xmin(xmin(mx.x, mx.y), mx.z);
+  if (a > b) { tn = __builtin_huge_valf(); tf = -__builtin_huge_valf(); return false; }
+  tn = a; tf = b;
+  return true;
+}
+// Triangle::intersect, src/primitives/Triangle.cpp:49-81
+NE_DEV bool ref_triangle(Vec3<float> v0, Vec3<float> v1, Vec3<float> v2, Vec3<float> o, Vec3<float> d, float& t) {
+  Vec3<float> v1v0 = v1 - v0, v2v0 = v2 - v0, rov0 = o - v0;
+  Vec3<float> n = cross(v1v0, v2v0);
+  Vec3<float> q = cross(rov0, d);
+  float denom = float(1.0 / double(dot(d, n)));
+  float u = dot(-q, v2v0) * denom;
+  float v = dot(q, v1v0) * denom;
+  float tt = dot(-n, rov0) * denom;
+  t = __builtin_huge_valf();
+  if (__builtin_isnan(tt) || tt < 0) return false;
+  if (u < 0.0f || u > 1.0f || v < 0.0f || (u + v) > 1.0f) return false;
+  t = tt;
+  return true;
+}
+// Sphere::intersect, src/primitives/Sphere.cpp:21-44
+NE_DEV bool ref_sphere(Vec3<float> c, float r, Vec3<float> o, Vec3<float> d, float& tn, float& tf) {
+  Vec3<float> oc = o - c;
+  float a = dot(d, d), b = dot(oc, d), cc = dot(oc, oc) - r * r;
+  float disc = b * b - a * cc;
+  tn = __builtin_huge_valf(); tf = -__builtin_huge_valf();
+  if (disc >= 0) {
+    float t1 = (-b - sqrtf(disc)) / a, t2 = (-b + sqrtf(disc)) / a;
+    tn = xmin(t1, t2); tf = xmax(t1, t2);
+    return true;
+  }
+  return false;
+}
+// Rectangle::intersect, src/primitives/Rectangle.cpp:16-76
+NE_DEV bool ref_rect(Vec3<float> vA, Vec3<float> vB, Vec3<float> nrm, Vec3<float> o, Vec3<float> d, float& t) {
+  t = __builtin_huge_valf();
+  float denom = dot(nrm, d);
+  if (fabsf(denom) < float(kEPS)) return false;
+  float num = dot(nrm, vB - o);
+  float tt = num / denom;
+  if (tt < 0) return false;
+  Vec3<float> p = o + tt * d;
+  Vec3<float> a = vA, b = vB;
+  if (a.x == b.x) { a.x = 0; b.x = 0; }
+  if (a.y == b.y) { a.y = 0; b.y = 0; }
+  if (a.z == b.z) { a.z = 0; b.z = 0; }
+  Vec3<float> size = v3(fabsf(a.x) + fabsf(b.x), fabsf(a.y) + fabsf(b.y), fabsf(a.z) + fabsf(b.z));
+  Vec3<float> mn = v3(xmin(vA.x, vB.x), xmin(vA.y, vB.y), xmin(vA.z, vB.z));
+  Vec3<float> mx = v3(xmax(vA.x, vB.x), xmax(vA.y, vB.y), xmax(vA.z, vB.z));
+  if (size.x > 0 && (p.x < mn.x || p.x > mx.x)) return false;
+  if (size.y > 0 && (p.y < mn.y || p.y > mx.y)) return false;
+  if (size.z > 0 && (p.z < mn.z || p.z > mx.z)) return false;
+  t = tt;
+  return true;
+}
+
+// ---------------------------------------------------------------------------
+// Integrator-side primitive tests: hit in the open interval (tmin, tmax).
+// ---------------------------------------------------------------------------
+template <typename R>
+NE_DEV bool isect_tri(const float4* tri, Vec3<R> o, Vec3<R> d, R tmin, R tmax, R& tout) {
+  float4 a = tri[0], b = tri[1], c = tri[2];
+  Vec3<R> v0 = v3f4<R>(a), e1 = v3f4<R>(b), e2 = v3f4<R>(c);
+  Vec3<R> rov0 = o - v0;
+  Vec3<R> n = cross(e1, e2), q = cross(rov0, d);
+  R inv = R(1) / dot(d, n);
+  R u = dot(-q, e2) * inv, v = dot(q, e1) * inv, t = dot(-n, rov0) * inv;
+  if (!(t > tmin && t < tmax)) return false;
+  if (u < R(0) || u > R(1) || v < R(0) || (u + v) > R(1)) return false;
+  tout = t;
+  return true;
+}
+template <typename R>
+NE_DEV bool isect_sphere(float4 s, Vec3<R> o, Vec3<R> d, R tmin, R tmax, R& tout) {
+  Vec3<R> c = v3f4<R>(s);
+  R r = R(s.w);
+  Vec3<R> oc = o - c;
+  R a = dot(d, d), b = dot(oc, d), cc = dot(oc, oc) - r * r;
+  R disc = b * b - a * cc;
+  if (disc < R(0)) return false;
+  R sq = xsqrt(disc);
+  R t = (-b - sq) / a;
+  if (!(t > tmin)) t = (-b + sq) / a;
+  if (!(t > tmin && t < tmax)) return false;
+  tout = t;
+  return true;
+}
+template <typename R>
+NE_DEV bool isect_rect(const float4* rc, Vec3<R> o, Vec3<R> d, R tmin, R tmax, R& tout) {
+  float4 f0 = rc[0], f1 = rc[1], f2 = rc[2], f3 = rc[3];
+  Vec3<R> p0 = v3f4<R>(f0), eu = v3f4<R>(f1), ev = v3f4<R>(f2), n = v3f4<R>(f3);
+  R den = dot(n, d);
+  if (xfabs(den) < R(float(kEPS))) return false;
+  R t = dot(n, p0 - o) / den;
+  if (!(t > tmin && t < tmax)) return false;
+  Vec3<R> h = (o + t * d) - p0;
+  R a = dot(h, eu) / dot(eu, eu), b = dot(h, ev) / dot(ev, ev);
+  if (a < R(0) || a > R(1) || b < R(0) || b > R(1)) return false;
+  tout = t;
+  return true;
+}
+
+// Conservative slab test against a float box (Ize 2013: far distances scaled
+// by 1 + 2 gamma(3)); NaN from 0 * inf leaves the running interval alone.
+template <typename R>
+NE_DEV bool slab(float mnx, float mny, float mnz, float mxx, float mxy, float mxz, Vec3<R> o, Vec3<R> invd, R tmax,
+                 R& tent) {
+  R t0 = R(0), t1 = tmax;
+  R tn = (R(mnx) - o.x) * invd.x, tf = (R(mxx) - o.x) * invd.x;
+  if (tn > tf) { R s = tn; tn = tf; tf = s; }
+  tf = tf * R(1.0000003576);
+  t0 = tn > t0 ? tn : t0; t1 = tf < t1 ? tf : t1;
+  tn = (R(mny) - o.y) * invd.y; tf = (R(mxy) - o.y) * invd.y;
+  if (tn > tf) { R s = tn; tn = tf; tf = s; }
+  tf = tf * R(1.0000003576);
+  t0 = tn > t0 ? tn : t0; t1 = tf < t1 ? tf : t1;
+  tn = (R(mnz) - o.z) * invd.z; tf = (R(mxz) - o.z) * invd.z;
+  if (tn > tf) { R s = tn; tn = tf; tf = s; }
+  tf = tf * R(1.0000003576);
+  t0 = tn > t0 ? tn : t0; t1 = tf < t1 ? tf : t1;
+  tent = t0;
+  return t0 <= t1;
+}
+
+template <typename R>
+struct SurfHit {
+  R t;
+  int type;
+  uint32_t key;   // original primitive index (ties break on (type, key))
+  uint32_t slot;  // storage index
+};
+
+
+template <typename R, bool ANY, bool C>
+NE_DEV bool traverse(const DevScene& s, Vec3<R> o, Vec3<R> d, R tmin, R tmax, uint32_t* lds, int lane_stride,
+                     SurfHit<R>& hit, Count<C>& cnt) {
+  if (s.n_nodes == 0) return false;
+  Vec3<R> invd = v3(R(1) / d.x, R(1) / d.y, R(1) / d.z);
+  uint32_t spill[kSpillStack];
+  int sp = 0;
+  uint32_t node = 0;
+  R best = tmax;
+  bool found = false;
+  int btype = 0;
+  uint32_t bkey = 0, bslot = 0;
+  while (true) {
+    const float4* nd = s.nodes + 4 * size_t(node);
+    float4 f0 = nd[0], f1 = nd[1], f2 = nd[2];
+    int4 f3 = reinterpret_cast<const int4*>(nd)[3];
+    cnt.node();
+    R tl, tr;
+    bool hl = f3.x >= 0 && slab<R>(f0.x, f0.y, f0.z, f0.w, f1.x, f1.y, o, invd, best, tl);
+    bool hr = f3.y >= 0 && slab<R>(f1.z, f1.w, f2.x, f2.y, f2.z, f2.w, o, invd, best, tr);
+#pragma unroll
+    for (int side = 0; side < 2; ++side) {
+      bool h = side == 0 ? hl : hr;
+      int cntp = side == 0 ? f3.z : f3.w;
+      if (!(h && cntp > 0)) continue;
+      int first = side == 0 ? f3.x : f3.y;
+      for (int i = first; i < first + cntp; ++i) {
+        uint32_t ref = s.refs[i];
+        uint32_t ty = ref >> 30, idx = ref & 0x3fffffffu;
+        R t;
+        bool ok;
+        uint32_t key;
+        cnt.prim();
+        if (ty == PRIM_TRI) {
+          const float4* tp = s.tris + 3 * size_t(idx);
+          ok = isect_tri<R>(tp, o, d, tmin, tmax, t);
+          key = ok ? __float_as_uint(tp[1].w) : 0u;
+        } else if (ty == PRIM_SPHERE) {
+          ok = isect_sphere<R>(s.sph[idx], o, d, tmin, tmax, t);
+          key = idx;
+        } else {
+          ok = isect_rect<R>(s.rects + 4 * size_t(idx), o, d, tmin, tmax, t);
+          key = idx;
+        }
+        if (ok) {
+          if (ANY) return true;
+          if (t < best || (found && t == best && (int(ty) < btype || (int(ty) == btype && key < bkey)))) {
+            best = t; found = true; btype = int(ty); bkey = key; bslot = idx;
+          }
+        }
+      }
+      if (side == 0) hl = false; else hr = false;
+    }
+    if (hl && tl > best) hl = false;
+    if (hr && tr > best) hr = false;
+    if (hl && hr) {
+      uint32_t near = tl <= tr ? uint32_t(f3.x) : uint32_t(f3.y);
+      uint32_t far = tl <= tr ? uint32_t(f3.y) : uint32_t(f3.x);
+      if (sp < kShortStack) lds[sp * lane_stride] = far;
+      else spill[sp - kShortStack] = far;
+      ++sp;
+      node = near;
+    } else if (hl) {
+      node = uint32_t(f3.x);
+    } else if (hr) {
+      node = uint32_t(f3.y);
+    } else {
+      if (sp == 0) break;
+      --sp;
+      node = sp < kShortStack ? lds[sp * lane_stride] : spill[sp - kShortStack];
+    }
+  }
+  if (!found) return false;
+  hit.t = best; hit.type = btype; hit.key = bkey; hit.slot = bslot;
+  return true;
+}
+
+// Geometric normal and material of a surface hit.
+template <typename R>
+NE_DEV void hit_normal(const DevScene& s, const SurfHit<R>& h, Vec3<R> o, Vec3<R> d, Vec3<R>& n, int& mat) {
+  if (h.type == PRIM_TRI) {
+    const float4* tp = s.tris + 3 * size_t(h.slot);
+    float4 a = tp[0];
+    n = normalize(cross(v3f4<R>(tp[1]), v3f4<R>(tp[2])));
+    mat = __float_as_int(a.w);
+  } else if (h.type == PRIM_SPHERE) {
+    float4 sp = s.sph[h.slot];
+    n = ((o + h.t * d) - v3f4<R>(sp)) / R(sp.w);
+    mat = s.sph_mat[h.slot];
+  } else {
+    const float4* rc = s.rects + 4 * size_t(h.slot);
+    n = v3f4<R>(rc[3]);
+    mat = __float_as_int(rc[0].w);
+  }
+}
+
+// ---------------------------------------------------------------------------
+// Volume: unit-cube interval, trilinear density, majorant DDA, tracking.
+// ---------------------------------------------------------------------------
+template <typename R>
+NE_DEV Vec3<R> xf_point(const float* m, Vec3<R> p) {
+  return v3(R(m[0]) * p.x + R(m[1]) * p.y + R(m[2]) * p.z + R(m[3]),
+            R(m[4]) * p.x + R(m[5]) * p.y + R(m[6]) * p.z + R(m[7]),
+            R(m[8]) * p.x + R(m[9]) * p.y + R(m[10]) * p.z + R(m[11]));
+}
+template <typename R>
+NE_DEV Vec3<R> xf_vec(const float* m, Vec3<R> p) {
+  return v3(R(m[0]) * p.x + R(m[1]) * p.y + R(m[2]) * p.z, R(m[4]) * p.x + R(m[5]) * p.y + R(m[6]) * p.z,
+            R(m[8]) * p.x + R(m[9]) * p.y + R(m[10]) * p.z);
+}
+
+template <typename R>
+NE_DEV bool box_interval(const DevScene& s, Vec3<R> oo, Vec3<R> dd, R& ta, R& tb) {
+  R t0 = -xinf<R>(), t1 = xinf<R>();
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    R oa = comp(oo, a), da = comp(dd, a);
+    R id = R(1) / da;
+    R tn = (R(-0.5) - oa) * id, tf = (R(0.5) - oa) * id;
+    if (tn > tf) { R x = tn; tn = tf; tf = x; }
+    t0 = tn > t0 ? tn : t0;
+    t1 = tf < t1 ? tf : t1;
+  }
+  ta = t0; tb = t1;
+  return t0 <= t1;
+}
+
+NE_DEV float grid_at(const DevScene& s, int x, int y, int z) {
+  if (x >= s.res[0] || y >= s.res[1] || z >= s.res[2]) return 0.0f;
+  return s.grid[size_t(z) * size_t(s.res[0]) * size_t(s.res[1]) + size_t(y) * size_t(s.res[0]) + size_t(x)];
+}
+// GridMedia::sampleAt -> interpolatedDensity (src/materials/GridMedia.cpp:25-43):
+// OCS [-.5,.5]^3 -> grid space (p + .5) * res, clamp, cell-corner trilinear,
+// corners at index >= res read as 0.
+NE_DEV float density_gcs(const DevScene& s, Vec3<float> gp) {
+  gp = v3(xmax(0.0f, gp.x), xmax(0.0f, gp.y), xmax(0.0f, gp.z));
+  gp = v3(xmin(gp.x, float(s.res[0])), xmin(gp.y, float(s.res[1])), xmin(gp.z, float(s.res[2])));
+  int ix = int(floorf(gp.x)), iy = int(floorf(gp.y)), iz = int(floorf(gp.z));
+  float dx = gp.x - float(ix), dy = gp.y - float(iy), dz = gp.z - float(iz);
+  float c000 = grid_at(s, ix, iy, iz), c100 = grid_at(s, ix + 1, iy, iz);
+  float c010 = grid_at(s, ix, iy + 1, iz), c110 = grid_at(s, ix + 1, iy + 1, iz);
+  float c001 = grid_at(s, ix, iy, iz + 1), c101 = grid_at(s, ix + 1, iy, iz + 1);
+  float c011 = grid_at(s, ix, iy + 1, iz + 1), c111 = grid_at(s, ix + 1, iy + 1, iz + 1);
+  float d00 = mixf(c000, c100, dx), d10 = mixf(c010, c110, dx);
+  float d01 = mixf(c001, c101, dx), d11 = mixf(c011, c111, dx);
+  float d0 = mixf(d00, d10, dy), d1 = mixf(d01, d11, dy);
+  return mixf(d0, d1, dz);
+}
+NE_DEV float density_ocs(const DevScene& s, Vec3<float> p) {
+  return density_gcs(s, (p + v3s(0.5f)) * v3(float(s.res[0]), float(s.res[1]), float(s.res[2])));
+}
+
+// 3-D DDA over the majorant grid along the OCS ray (oo, dd) for t in [ta, tb).
+// Calls f(c0, c1, mu) per cell; f returns false to stop.
+template <typename R, bool C, typename F>
+NE_DEV void majorant_dda(const DevScene& s, Vec3<R> oo, Vec3<R> dd, R ta, R tb, Count<C>& cnt, F&& f) {
+  const int M = s.mres;
+  const R Mr = R(M);
+  int cell[3], step[3];
+  R tnext[3];
+  Vec3<R> pe = oo + ta * dd;
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    R g = (comp(pe, a) + R(0.5)) * Mr;
+    int c = int(xfloor(g));
+    c = c < 0 ? 0 : (c > M - 1 ? M - 1 : c);
+    cell[a] = c;
+    R da = comp(dd, a), oa = comp(oo, a);
+    if (da > R(0)) { step[a] = 1; tnext[a] = ((R(c + 1) / Mr - R(0.5)) - oa) / da; }
+    else if (da < R(0)) { step[a] = -1; tnext[a] = ((R(c) / Mr - R(0.5)) - oa) / da; }
+    else { step[a] = 0; tnext[a] = xinf<R>(); }
+  }
+  R t = ta;
+  while (t < tb) {
+    int ax = 0;
+    if (tnext[1] < tnext[ax]) ax = 1;
+    if (tnext[2] < tnext[ax]) ax = 2;
+    R tna = ax == 0 ? tnext[0] : (ax == 1 ? tnext[1] : tnext[2]);
+    R tc = xmin(tna, tb);
+    cnt.cell();
+    R mu = R(s.maj[(size_t(cell[2]) * M + cell[1]) * M + cell[0]]);
+    if (tc > t && !f(t, tc, mu)) return;
+    t = tc;
+    if (tna >= tb) break;
+    int c = (ax == 0 ? cell[0] : (ax == 1 ? cell[1] : cell[2])) + (ax == 0 ? step[0] : (ax == 1 ? step[1] : step[2]));
+    if (c < 0 || c >= M) break;
+    R bnd = (ax == 0 ? step[0] : (ax == 1 ? step[1] : step[2])) > 0 ? R(c + 1) / Mr : R(c) / Mr;
+    R nt = ((bnd - R(0.5)) - comp(oo, ax)) / comp(dd, ax);
+    if (ax == 0) { cell[0] = c; tnext[0] = nt; }
+    else if (ax == 1) { cell[1] = c; tnext[1] = nt; }
+    else { cell[2] = c; tnext[2] = nt; }
+  }
+}
+
+template <typename R, bool C>
+NE_DEV R sigma_at(const DevScene& s, Vec3<R> oo, Vec3<R> dd, R t, Count<C>& cnt) {
+  Vec3<R> p = oo + t * dd;
+  cnt.lookup();
+  return R(density_ocs(s, v3(float(p.x), float(p.y), float(p.z)))) * R(s.sigma_bar);
+}
+
+// Delta tracking (GridMedia::sample, src/materials/GridMedia.cpp:71-98) with
+// per-cell majorants: step t -= log(1 - xi) / mu, accept if sigma/mu > xi'.
+template <typename R, bool C>
+NE_DEV bool delta_track(const DevScene& s, Vec3<R> o, Vec3<R> d, R ta, R tb, PathRng& rng, Count<C>& cnt, R& th) {
+  Vec3<R> oo = xf_point<R>(s.inv, o), dd = xf_vec<R>(s.inv, d);
+  bool hit = false;
+  majorant_dda<R, C>(s, oo, dd, ta, tb, cnt, [&](R c0, R c1, R mu) {
+    if (!(mu > R(0)) || xisinf(mu)) return true;  // empty cell; grids are validated finite at upload
+    R t = c0;
+    while (true) {
+      R r = R(rng.next());
+      t -= xlog(R(1) - r) / mu;
+      if (!(t < c1)) return true;
+      R sig = sigma_at<R, C>(s, oo, dd, t, cnt);
+      R ra = R(rng.next());
+      if (sig / mu > ra) { hit = true; th = t; return false; }
+    }
+  });
+  return hit;
+}
+
+// Ratio tracking with the reference's Russian roulette (GridMedia.cpp:54-66).
+template <typename R, bool C>
+NE_DEV R ratio_track(const DevScene& s, Vec3<R> o, Vec3<R> d, R ta, R tb, PathRng& rng, Count<C>& cnt) {
+  Vec3<R> oo = xf_point<R>(s.inv, o), dd = xf_vec<R>(s.inv, d);
+  R Tr = R(1);
+  majorant_dda<R, C>(s, oo, dd, ta, tb, cnt, [&](R c0, R c1, R mu) {
+    if (!(mu > R(0)) || xisinf(mu)) return true;  // empty cell; grids are validated finite at upload
+    R t = c0;
+    while (true) {
+      t -= xlog(R(1) - R(rng.next())) / mu;
+      if (!(t < c1)) return true;
+      R sig = sigma_at<R, C>(s, oo, dd, t, cnt);
+      Tr *= R(1) - xmax(R(0), sig / mu);
+      if (Tr < R(0.1)) {
+        R q = xmax(R(0.05), R(1) - Tr);
+        if (R(rng.next()) < q) { Tr = R(0); return false; }
+        Tr /= R(1) - q;
+      }
+    }
+  });
+  return Tr;
+}
+
+// ---------------------------------------------------------------------------
+// Phase function and surface BSDF
+// ---------------------------------------------------------------------------
+// HG::eval, src/materials/Medium.h:116-128 (reference argument convention)
+NE_DEV float ref_hg_eval(float g, Vec3<float> a, Vec3<float> b) {
+  float c = dot(normalize(a), normalize(b));
+  float den = 1.0f + g * g - 2.0f * g * c;
+  return kINV4PI * (1.0f - g * g) / (den * sqrtf(den));
+}
+// HG::sample local frame, src/materials/Medium.h:96-114
+NE_DEV Vec3<float> ref_hg_sample(float g, float u0, float u1) {
+  float ct;
+  if (fabsf(g) < 1e-3f) ct = 1.0f - 2.0f * u0;
+  else {
+    float sqr = (1.0f - g * g) / (1.0f + g - 2.0f * g * u0);
+    sqr = sqr * sqr;
+    ct = -1.0f / (2.0f * g) * (1.0f + g * g - sqr);
+  }
+  float st = sqrtf(1.0f - ct * ct);
+  float ph = float(2.0f * kPI * u1);
+  return v3(cosf(ph) * st, sinf(ph) * st, ct);
+}
+// HG in the propagation convention: p(cos) with cos = dot(d, wi).
+template <typename R>
+NE_DEV R hg_p(R g, R c) {
+  R den = R(1) + g * g - R(2) * g * c;
+  return R(kINV4PI) * (R(1) - g * g) / (den * xsqrt(den));
+}
+// Inverse-CDF sample of the same density, framed on d (sign flipped
+// relative to the reference sampler so g > 0 scatters forward).
+template <typename R>
+NE_DEV Vec3<R> hg_sample(R g, Vec3<R> d, R u0, R u1) {
+  R ct;
+  if (xfabs(g) < R(1e-3)) ct = R(1) - R(2) * u0;
+  else {
+    R sq = (R(1) - g * g) / (R(1) + g - R(2) * g * u0);
+    ct = (R(1) + g * g - sq * sq) / (R(2) * g);
+  }
+  ct = xmin(R(1), xmax(R(-1), ct));
+  R st = xsqrt(xmax(R(0), R(1) - ct * ct));
+  R ph = R(kTWO_PI) * u1;
+  Vec3<R> ss, ts;
+  onb(d, ss, ts);
+  return to_world(v3(st * xcos(ph), st * xsin(ph), ct), d, ss, ts);
+}
+
+// GGXDistribution::D and ::pdf, src/core/Microfacet.cpp:5-15,58-65
+NE_DEV float ref_ggx_D(float alpha, Vec3<float> h) {
+  float a2 = alpha * alpha, c = h.z, c2 = c * c;
+  float den = c2 * (a2 - 1.0f) + 1.0f;
+  den = xmax(float(kEPS), float(kPI * den * den));
+  return a2 / den;
+}
+NE_DEV float ref_ggx_pdf(float alpha, Vec3<float> sc, Vec3<float> h) {
+  return ref_ggx_D(alpha, h) * h.z / (4 * dot(sc, h));
+}
+
+// Glossy lobe of GlossyBSDF (src/core/GlossyBSDF.cpp:26-48: GGX D, Schlick-GGX
+// G with k = alpha/2, Schlick Fresnel f0 = 0.04, Lambert term weighted by
+// (1-F)(1-metallic)), with the cosine applied by the caller, G on the shading
+// normal, and sampled as a mixture of GGX half-vector sampling
+// (reflection about h) and cosine-weighted hemisphere sampling.
+template <typename R>
+struct Glossy {
+  Vec3<R> albedo;
+  R alpha, metallic;
+  NE_DEV static R D(R alpha, R ch) {
+    R a2 = alpha * alpha, c2 = ch * ch;
+    R den = c2 * (a2 - R(1)) + R(1);
+    den = xmax(R(kEPS), R(kPI) * den * den);
+    return a2 / den;
+  }
+  NE_DEV static R G1(R alpha, R c) {
+    R k = alpha / R(2);
+    return c / xmax(R(kEPS), c * (R(1) - k) + k);
+  }
+  NE_DEV static R F(R c) {
+    R m = R(1) - c, m2 = m * m;
+    return R(0.04) + R(0.96) * (m2 * m2 * m);
+  }
+  NE_DEV R p_spec(R co) const {
+    R fs = F(co);
+    R kd = avg3(albedo) * (R(1) - fs) * (R(1) - metallic);
+    R p = fs / (fs + kd);
+    return xmin(R(0.9), xmax(R(0.1), p));
+  }
+  NE_DEV Vec3<R> eval(Vec3<R> wo, Vec3<R> wi) const {
+    if (wo.z <= R(0) || wi.z <= R(0)) return v3s(R(0));
+    Vec3<R> h = normalize(wo + wi);
+    R Fh = F(dot(wo, h));
+    R spec = D(alpha, h.z) * G1(alpha, wo.z) * G1(alpha, wi.z) * Fh / (R(4) * wo.z * wi.z);
+    R kd = (R(1) - Fh) * (R(1) - metallic);
+    return albedo * (kd / R(kPI)) + v3s(spec);
+  }
+  NE_DEV R pdf(Vec3<R> wo, Vec3<R> wi) const {
+    if (wo.z <= R(0) || wi.z <= R(0)) return R(0);
+    Vec3<R> h = normalize(wo + wi);
+    R ps = p_spec(wo.z);
+    R pspec = D(alpha, h.z) * h.z / (R(4) * xfabs(dot(wo, h)));
+    R pdiff = wi.z / R(kPI);
+    return ps * pspec + (R(1) - ps) * pdiff;
+  }
+  NE_DEV Vec3<R> sample(Vec3<R> wo, R u0, R u1, R u2) const {
+    R ps = p_spec(wo.z);
+    if (u0 < ps) {
+      R a2 = alpha * alpha;
+      R ct = xmin(R(1), xsqrt((R(1) - u1) / xmax(R(kEPS), u1 * (a2 - R(1)) + R(1))));
+      R st = xsqrt(xmax(R(0), R(1) - ct * ct));
+      R ph = R(kTWO_PI) * u2;
+      Vec3<R> h = v3(st * xcos(ph), st * xsin(ph), ct);
+      return R(2) * dot(wo, h) * h - wo;
+    }
+    R r = xsqrt(u1), ph = R(kTWO_PI) * u2;
+    return v3(r * xcos(ph), r * xsin(ph), xsqrt(xmax(R(0), R(1) - u1)));
+  }
+};
+
+// ---------------------------------------------------------------------------
+// Lights
+// ---------------------------------------------------------------------------
+template <typename R>
+NE_DEV Vec3<R> env_le(const DevScene& s, Vec3<R> d) {
+  if (s.env_type == 1) return v3(R(s.env[0]), R(s.env[1]), R(s.env[2]));
+  if (s.env_type == 2) {
+    if (d.y >= R(0))
+      return v3(R(s.env[3]), R(s.env[4]), R(s.env[5])) * (R(1) - d.y) + v3(R(s.env[0]), R(s.env[1]), R(s.env[2])) * d.y;
+    return v3(R(s.env[6]), R(s.env[7]), R(s.env[8]));
+  }
+  return v3s(R(0));
+}
+
+// Shadow transmittance: surface occlusion, then ratio tracking through the
+// volume over [0, tmax].
+template <typename R, bool C>
+NE_DEV R transmittance(const DevScene& s, Vec3<R> o, Vec3<R> d, R tmax, PathRng& rng, uint32_t* lds, int stride,
+                       Count<C>& cnt) {
+  cnt.shadow();
+  SurfHit<R> h;
+  if (traverse<R, true, C>(s, o, d, R(1e-4), tmax, lds, stride, h, cnt)) return R(0);
+  if (!s.has_volume) return R(1);
+  Vec3<R> oo = xf_point<R>(s.inv, o), dd = xf_vec<R>(s.inv, d);
+  R ta, tb;
+  if (!box_interval<R>(s, oo, dd, ta, tb)) return R(1);
+  ta = xmax(ta, R(0));
+  tb = xmin(tb, tmax);
+  if (ta >= tb) return R(1);
+  return ratio_track<R, C>(s, o, d, ta, tb, rng, cnt);
+}
+
+// ---------------------------------------------------------------------------
+// One path (corrected semantics).  The event and random-draw order is the
+// oracle's Li (camera jitter is drawn by the caller).
+// ---------------------------------------------------------------------------
+template <typename R, bool C>
+NE_DEV Vec3<R> trace_path(const DevScene& s, Vec3<R> o, Vec3<R> d, int max_bounces, PathRng& rng, uint32_t* lds,
+                          int stride, Count<C>& cnt) {
+  Vec3<R> L = v3s(R(0)), beta = v3s(R(1));
+  R prev_pdf = R(0);
+  bool prev_delta = true;
+  int depth = 0;
+  const R tmin = R(1e-4);
+  const R g = R(s.g);
+  const R nl = R(s.n_lights);
+  while (true) {
+    cnt.ext();
+    SurfHit<R> hit;
+    bool hs = traverse<R, false, C>(s, o, d, tmin, xinf<R>(), lds, stride, hit, cnt);
+    R tS = hs ? hit.t : xinf<R>();
+    bool medium = false;
+    Vec3<R> x, n;
+    int mat = 0;
+    if (s.has_volume) {
+      Vec3<R> oo = xf_point<R>(s.inv, o), dd = xf_vec<R>(s.inv, d);
+      R ta, tb;
+      if (box_interval<R>(s, oo, dd, ta, tb)) {
+        ta = xmax(ta, R(0));
+        tb = xmin(tb, tS);
+        R th;
+        if (ta < tb && delta_track<R, C>(s, o, d, ta, tb, rng, cnt, th)) { medium = true; x = o + th * d; }
+      }
+    }
+    if (!medium) {
+      if (!hs) {
+        if (s.env_type != 0) {
+          R w = prev_delta ? R(1) : power_heuristic(prev_pdf, R(1.0 / kFOUR_PI));
+          L = L + beta * env_le<R>(s, d) * w;
+        }
+        break;
+      }
+      hit_normal<R>(s, hit, o, d, n, mat);
+      const float* m = s.mats + 12 * size_t(mat);
+      if (m[0] == 1.0f) {  // emitter
+        Vec3<R> Le = v3f<R>(m + 6);
+        R w = R(1);
+        if (!prev_delta) {
+          const float4* rc = s.rects + 4 * size_t(hit.slot);
+          R area = R(rc[1].w);
+          R cosl = xfabs(dot(n, d));
+          R pl = (hit.t * hit.t) / (cosl * area);
+          w = power_heuristic(prev_pdf, pl);
+        }
+        L = L + beta * Le * w;
+        break;
+      }
+      x = o + hit.t * d;
+    }
+    if (depth >= max_bounces) break;
+    // ---- vertex: next-event estimation
+    Vec3<R> wo = -d;
+    Glossy<R> gm;
+    Vec3<R> ns, ss, ts;
+    if (medium) {
+      beta = beta * v3(R(s.sig_s[0]) / R(s.sig_t[0]), R(s.sig_s[1]) / R(s.sig_t[1]), R(s.sig_s[2]) / R(s.sig_t[2]));
+    } else {
+      const float* m = s.mats + 12 * size_t(mat);
+      gm.albedo = v3f<R>(m + 1);
+      R rough = R(m[4]);
+      gm.alpha = rough * rough;
+      gm.metallic = R(m[5]);
+      ns = dot(n, wo) < R(0) ? -n : n;
+      onb(ns, ss, ts);
+    }
+    R ul = R(rng.next()), ua = R(rng.next()), ub = R(rng.next());
+    if (s.n_lights > 0) {
+      int li = int(ul * nl);
+      li = li < s.n_lights - 1 ? li : s.n_lights - 1;
+      Vec3<R> wi, Le = v3s(R(0));
+      R pl = R(0), dist = xinf<R>();
+      bool delta = false;
+      const int env_slot = s.has_sun ? 1 : 0;
+      if (s.has_sun && li == 0) {
+        wi = -v3f<R>(s.sun_dir);
+        Le = v3f<R>(s.sun_E);
+        pl = R(1);
+        delta = true;
+      } else if (s.env_type != 0 && li == env_slot) {
+        R z = R(1) - R(2) * ua, r = xsqrt(xmax(R(0), R(1) - z * z)), ph = R(kTWO_PI) * ub;
+        wi = v3(r * xcos(ph), r * xsin(ph), z);
+        Le = env_le<R>(s, wi);
+        pl = R(1.0 / kFOUR_PI);
+      } else {
+        int ri = s.light_rects[li - env_slot - (s.env_type != 0 ? 1 : 0)];
+        const float4* rc = s.rects + 4 * size_t(ri);
+        float4 f0 = rc[0], f1 = rc[1], f2 = rc[2], f3 = rc[3];
+        Vec3<R> p0 = v3f4<R>(f0), eu = v3f4<R>(f1), ev = v3f4<R>(f2), ln = v3f4<R>(f3);
+        Vec3<R> y = p0 + ua * eu + ub * ev;
+        Vec3<R> dv = y - x;
+        R d2 = dot(dv, dv);
+        dist = xsqrt(d2);
+        wi = dv / dist;
+        R cosl = xfabs(dot(ln, wi));
+        R area = length(cross(eu, ev));
+        const float* lm = s.mats + 12 * size_t(__float_as_int(f0.w));
+        Le = v3f<R>(lm + 6);
+        pl = cosl > R(0) ? d2 / (cosl * area) : R(0);
+      }
+      if (pl > R(0)) {
+        Vec3<R> f;
+        R pf;
+        if (medium) {
+          R p = hg_p(g, dot(d, wi));
+          f = v3s(p);
+          pf = p;
+        } else {
+          Vec3<R> wol = to_lcs(wo, ns, ss, ts), wil = to_lcs(wi, ns, ss, ts);
+          f = gm.eval(wol, wil) * xfabs(wil.z);
+          pf = gm.pdf(wol, wil);
+        }
+        if (!is_black(f)) {
+          R tmax = xisinf(dist) ? xinf<R>() : dist * R(1 - 1e-4);
+          R tr = transmittance<R, C>(s, x, wi, tmax, rng, lds, stride, cnt);
+          if (tr > R(0)) {
+            R w = delta ? R(1) : power_heuristic(pl, pf);
+            L = L + beta * f * Le * (tr * w * nl / pl);
+          }
+        }
+      }
+    }
+    // ---- continue the path
+    if (medium) {
+      R u0 = R(rng.next()), u1 = R(rng.next());
+      Vec3<R> wi = hg_sample(g, d, u0, u1);
+      prev_pdf = hg_p(g, dot(d, wi));
+      d = wi;
+    } else {
+      R u0 = R(rng.next()), u1 = R(rng.next()), u2 = R(rng.next());
+      Vec3<R> wol = to_lcs(wo, ns, ss, ts);
+      Vec3<R> wil = gm.sample(wol, u0, u1, u2);
+      R pf = gm.pdf(wol, wil);
+      if (!(pf > R(0))) break;
+      Vec3<R> f = gm.eval(wol, wil) * xfabs(wil.z);
+      if (is_black(f)) break;
+      beta = beta * f / pf;
+      prev_pdf = pf;
+      d = to_world(wil, ns, ss, ts);
+    }
+    prev_delta = false;
+    o = x;
+    depth++;
+  }
+  return L;
+}
+
+}  // namespace ne

@@ -1,0 +1,203 @@
+"""Context / scene / film handles over the C ABI (include/ne_gpu.h)."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from .abi import (Counters, NarvalError, RenderParams, SceneInfo, check, lib, NE_FP32, NE_FLAG_COUNT, NE_FLAG_TIME)
+
+
+def _fp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_float))
+
+
+def _dp(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_double))
+
+
+def _ip(a: np.ndarray):
+    return a.ctypes.data_as(C.POINTER(C.c_int32))
+
+
+class Context:
+    """One HIP device + stream (ne_ctx)."""
+
+    def __init__(self, device: int = 0):
+        self._lib = lib()
+        h = C.c_void_p()
+        check(self._lib.ne_ctx_create(device, C.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            self._lib.ne_ctx_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def check(self, st: int):
+        check(st, self.h)
+
+    def synchronize(self):
+        self.check(self._lib.ne_synchronize(self.h))
+
+    def upload(self, scene) -> "GpuScene":
+        return GpuScene(self, scene)
+
+    def film(self, width: int, height: int) -> "Film":
+        return Film(self, width, height)
+
+    def render(self, gscene: "GpuScene", camera, film: "Film", *, spp: int, max_bounces: int, seed: int = 0,
+               sample_offset: int = 0, job: int = 0, rect=None, precision: int = NE_FP32, count: bool = False,
+               samples_per_thread: int = 0, block_size: int = 0, sync: bool = True, timed: bool = False):
+        p = RenderParams()
+        p.width, p.height = film.width, film.height
+        x0, y0, x1, y1 = rect if rect is not None else (0, 0, film.width, film.height)
+        p.x0, p.y0, p.x1, p.y1 = x0, y0, x1, y1
+        p.spp, p.max_bounces, p.seed = spp, max_bounces, seed
+        p.sample_offset, p.job, p.precision = sample_offset, job, precision
+        p.flags = (NE_FLAG_COUNT if count else 0) | (NE_FLAG_TIME if timed else 0)
+        p.samples_per_thread, p.block_size = samples_per_thread, block_size
+        cam = camera.desc()
+        self.check(self._lib.ne_render(self.h, gscene.h, C.byref(cam), C.byref(p), film.h))
+        if sync:
+            self.synchronize()
+
+    def kernel_ms(self) -> float:
+        ms = C.c_float()
+        self.check(self._lib.ne_get_kernel_time(self.h, C.byref(ms)))
+        return float(ms.value)
+
+    def counters(self) -> dict:
+        c = Counters()
+        self.check(self._lib.ne_get_counters(self.h, C.byref(c)))
+        return c.as_dict()
+
+    def li(self, gscene: "GpuScene", rays: np.ndarray, max_bounces: int, seed: int = 0, sample: int = 0) -> np.ndarray:
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 6)
+        out = np.zeros((len(rays), 3), np.float32)
+        self.check(self._lib.ne_li(self.h, gscene.h, len(rays), _fp(rays), max_bounces, seed, sample, _fp(out)))
+        return out
+
+    # ---- parity entry points
+    def test_primitive(self, kind: int, rays: np.ndarray, prims: np.ndarray):
+        rays = np.ascontiguousarray(rays, np.float32)
+        prims = np.ascontiguousarray(prims, np.float32)
+        n = len(rays)
+        hit = np.zeros(n, np.int32)
+        t0 = np.zeros(n, np.float32)
+        t1 = np.zeros(n, np.float32)
+        self.check(self._lib.ne_test_primitive(self.h, kind, n, _fp(rays), _fp(prims), _ip(hit), _fp(t0), _fp(t1)))
+        return hit, t0, t1
+
+    def test_closest(self, gscene, rays, tmin=1e-4, tmax=np.inf):
+        rays = np.ascontiguousarray(rays, np.float32)
+        n = len(rays)
+        t = np.zeros(n, np.float32)
+        ty = np.zeros(n, np.int32)
+        ix = np.zeros(n, np.int32)
+        self.check(self._lib.ne_test_closest(self.h, gscene.h, n, _fp(rays), tmin, tmax, _fp(t), _ip(ty), _ip(ix)))
+        return t, ty, ix
+
+    def test_transmittance(self, gscene, rays, tmax, seed=0, precision=NE_FP32):
+        rays = np.ascontiguousarray(rays, np.float32)
+        tmax = np.ascontiguousarray(tmax, np.float32)
+        out = np.zeros(len(rays), np.float64)
+        self.check(self._lib.ne_test_transmittance(self.h, gscene.h, len(rays), _fp(rays), _fp(tmax), seed, precision,
+                                                   _dp(out)))
+        return out
+
+    def test_delta_track(self, gscene, rays, seed=0, precision=NE_FP32):
+        rays = np.ascontiguousarray(rays, np.float32)
+        sc = np.zeros(len(rays), np.int32)
+        t = np.zeros(len(rays), np.float64)
+        self.check(self._lib.ne_test_delta_track(self.h, gscene.h, len(rays), _fp(rays), seed, precision, _ip(sc),
+                                                 _dp(t)))
+        return sc, t
+
+    def test_density(self, gscene, gcs):
+        gcs = np.ascontiguousarray(gcs, np.float32)
+        out = np.zeros(len(gcs), np.float32)
+        self.check(self._lib.ne_test_density(self.h, gscene.h, len(gcs), _fp(gcs), _fp(out)))
+        return out
+
+    def test_bsdf(self, kind: int, n: int, ins, out_width: int):
+        arrs = [None if a is None else np.ascontiguousarray(a, np.float32) for a in ins] + [None] * (4 - len(ins))
+        out = np.zeros((n, out_width), np.float32)
+        ptrs = [C.cast(None, C.POINTER(C.c_float)) if a is None else _fp(a) for a in arrs]
+        self.check(self._lib.ne_test_bsdf(self.h, kind, n, *ptrs, _fp(out)))
+        return out
+
+    def test_rng(self, seed, pixel, sample, job, n):
+        out = np.zeros(n, np.float32)
+        self.check(self._lib.ne_test_rng(self.h, seed, pixel, sample, job, n, _fp(out)))
+        return out
+
+
+class GpuScene:
+    """A scene resident in HBM (ne_scene)."""
+
+    def __init__(self, ctx: Context, scene):
+        self.ctx = ctx
+        desc, keep = scene.desc()
+        h = C.c_void_p()
+        ctx.check(ctx._lib.ne_scene_upload(ctx.h, C.byref(desc), C.byref(h)))
+        self.h = h
+
+    def info(self) -> dict:
+        i = SceneInfo()
+        self.ctx.check(self.ctx._lib.ne_scene_get_info(self.h, C.byref(i)))
+        return {n: getattr(i, n) for n, _ in i._fields_ if n != "pad"}
+
+    def close(self):
+        if self.h:
+            self.ctx._lib.ne_scene_release(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Film:
+    """FP64 per-pixel radiance sums in HBM (ne_film)."""
+
+    def __init__(self, ctx: Context, width: int, height: int):
+        self.ctx, self.width, self.height = ctx, width, height
+        h = C.c_void_p()
+        ctx.check(ctx._lib.ne_film_create(ctx.h, width, height, C.byref(h)))
+        self.h = h
+
+    def clear(self):
+        self.ctx.check(self.ctx._lib.ne_film_clear(self.h))
+
+    def read(self) -> np.ndarray:
+        out = np.zeros((self.height, self.width, 3), np.float64)
+        self.ctx.check(self.ctx._lib.ne_film_read(self.h, _dp(out)))
+        return out
+
+    def tonemap(self, spp: int) -> np.ndarray:
+        out = np.zeros((self.height, self.width, 3), np.float32)
+        self.ctx.check(self.ctx._lib.ne_film_tonemap(self.h, spp, _fp(out)))
+        return out
+
+    def close(self):
+        if self.h:
+            self.ctx._lib.ne_film_release(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+__all__ = ["Context", "GpuScene", "Film", "NarvalError"]

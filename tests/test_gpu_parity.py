@@ -1,0 +1,355 @@
+"""GPU parity: device code (through the C ABI) against the oracle on the same
+seeded inputs.  Bit-exact for hit/miss, primitive ids and RNG words; t within
+1e-5 relative; pdf/eval within 1e-5 relative; images statistically equal."""
+import numpy as np
+import pytest
+
+import oracle as O
+import scenes
+from conftest import print_case
+from narvalengine_amd import abi
+from narvalengine_amd.abi import NE_FP32, NE_FP64
+
+pytestmark = pytest.mark.gpu
+RTOL = 1e-5
+
+
+def _rel(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return np.abs(a - b) / np.maximum(np.abs(b), 1e-30)
+
+
+# ---- RNG ---------------------------------------------------------------------
+@pytest.mark.parametrize("seed,pixel,sample,job", [(0, 0, 0, 0), (0x0123456789abcdef, 1234567, 99, 7),
+                                                  (2 ** 64 - 1, 2 ** 32 - 1, 2 ** 32 - 1, 5)])
+def test_rng_stream_bit_exact(ctx, seed, pixel, sample, job):
+    print_case(f"rng {seed} {pixel} {sample} {job}")
+    g = ctx.test_rng(seed, pixel, sample, job, 37)
+    o = O.rng_uniforms(seed, pixel, sample, job, 37)
+    np.testing.assert_array_equal(g, o)
+
+
+# ---- primitives (reference semantics) --------------------------------------
+def _prim_cases(kind, n, seed):
+    rng = np.random.default_rng(seed)
+    rays = scenes.random_rays(n, seed=seed, origin_box=1.5)
+    if kind == abi.NE_TEST_AABB:
+        lo = rng.uniform(-1, 0.5, (n, 3))
+        prims = np.concatenate([lo, lo + rng.uniform(0.01, 1.0, (n, 3))], axis=1)
+        # aim half of the rays at the box centres so both outcomes are frequent
+        c = (prims[:, :3] + prims[:, 3:]) / 2
+    elif kind == abi.NE_TEST_TRIANGLE:
+        c = rng.uniform(-0.5, 0.5, (n, 3))
+        prims = (c[:, None, :] + rng.normal(0, 0.3, (n, 3, 3))).reshape(n, 9)
+        c = prims.reshape(n, 3, 3).mean(axis=1)
+    elif kind == abi.NE_TEST_SPHERE:
+        c = rng.uniform(-0.5, 0.5, (n, 3))
+        prims = np.concatenate([c, rng.uniform(0.05, 0.6, (n, 1))], axis=1)
+    else:
+        # reference rectangle: vertexData[0], vertexData[1] corners of an
+        # axis-aligned quad, normal along the flat axis
+        c = rng.uniform(-0.5, 0.5, (n, 3))
+        axis = rng.integers(0, 3, n)
+        a = c - rng.uniform(0.05, 0.5, (n, 3))
+        b = c + rng.uniform(0.05, 0.5, (n, 3))
+        nrm = np.zeros((n, 3))
+        for i in range(n):
+            b[i, axis[i]] = a[i, axis[i]] = c[i, axis[i]]
+            nrm[i, axis[i]] = -1.0
+        prims = np.concatenate([a, b, nrm], axis=1)
+    aim = rng.random(n) < 0.5
+    d = c - rays[:, :3]
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays[aim, 3:] = d[aim]
+    return rays.astype(np.float32), prims.astype(np.float32)
+
+
+@pytest.mark.parametrize("kind", [abi.NE_TEST_AABB, abi.NE_TEST_TRIANGLE, abi.NE_TEST_SPHERE, abi.NE_TEST_RECTANGLE])
+def test_primitive_parity(ctx, kind):
+    print_case(f"primitive kind {kind}")
+    n = 20000
+    rays, prims = _prim_cases(kind, n, seed=100 + kind)
+    hit, t0, t1 = ctx.test_primitive(kind, rays, prims)
+    if kind == abi.NE_TEST_AABB:
+        oh, ot0, ot1, _ = O.aabb(rays, prims)
+    elif kind == abi.NE_TEST_TRIANGLE:
+        oh, ot0, _ = O.triangle(rays, prims)
+        ot1 = ot0
+    elif kind == abi.NE_TEST_SPHERE:
+        oh, ot0, ot1 = O.sphere(rays, prims)
+    else:
+        oh, ot0 = O.rectangle(rays, prims)
+        ot1 = ot0
+    np.testing.assert_array_equal(hit, oh)
+    assert 0.1 < hit.mean() < 0.9
+    m = oh == 1
+    assert (_rel(t0[m], ot0[m]) <= RTOL).all()
+    assert (_rel(t1[m], ot1[m]) <= RTOL).all()
+
+
+def test_reference_goldens_on_device(ctx):
+    print_case("goldens on device")
+    tri = np.array([0, 0, 0, 0, 1, 0, 1, 0, 0], np.float32)
+    rays = np.array([[0, 0, 0, 0, 0, 1], [0, 0, -1, 0, 0, 1], [0, -1, 0, 0, 1, 0], [0.25, 0.25, -1, 0, 0, 1],
+                     [0, 0, 1, 0, 0, 1]], np.float32)
+    hit, t, _ = ctx.test_primitive(abi.NE_TEST_TRIANGLE, rays, np.tile(tri, (5, 1)))
+    assert list(hit) == [1, 1, 0, 1, 0]
+    assert t[0] == 0.0 and t[1] == 1.0 and t[3] == 1.0
+    box = np.array([-0.5, -0.5, -0.5, 0.5, 0.5, 0.5], np.float32)
+    rays = np.array([[0, 0, -1, 0, 0, 1], [0, 0, -0.5, 0, 0, 1], [0, -0.5, -0.5, 0, 0, 1], [0, 0, 0, 0, 0, 1],
+                     [0, 0.55, 0, 0, 0, 1]], np.float32)
+    hit, tn, tf = ctx.test_primitive(abi.NE_TEST_AABB, rays, np.tile(box, (5, 1)))
+    assert list(hit) == [1, 1, 1, 1, 0]
+    np.testing.assert_allclose(tn[:4], [0.5, 0, 0, -0.5], atol=1e-3)
+    np.testing.assert_allclose(tf[:4], [1.5, 1, 1, 0.5], atol=1e-3)
+    assert tn[4] == np.inf and tf[4] == -np.inf
+
+
+# ---- phase function / BSDF ---------------------------------------------------
+def test_hg_and_ggx_parity(ctx):
+    print_case("hg/ggx")
+    rng = np.random.default_rng(7)
+    n = 20000
+    a = rng.normal(size=(n, 3)).astype(np.float32)
+    b = rng.normal(size=(n, 3)).astype(np.float32)
+    g = rng.uniform(-0.95, 0.95, n).astype(np.float32)
+    got = ctx.test_bsdf(abi.NE_TEST_HG_EVAL, n, [a, b, g], 1)[:, 0]
+    exp = np.zeros(n, np.float32)
+    # oracle HG eval takes a scalar g: evaluate per element
+    for i in range(n):
+        out = np.zeros(1, np.float32)
+        O.lib().neo_hg_eval(1, float(g[i]), O.fp(O.f(a[i:i + 1])), O.fp(O.f(b[i:i + 1])), O.fp(out))
+        exp[i] = out[0]
+    assert (_rel(got, exp) <= RTOL).all()
+    # HG sampler (reference local frame)
+    u = rng.random((n, 2)).astype(np.float32)
+    got = ctx.test_bsdf(abi.NE_TEST_HG_SAMPLE, n, [u, g], 3)
+    exp = np.zeros((n, 3), np.float32)
+    for i in range(n):
+        out = np.zeros(3, np.float32)
+        O.lib().neo_hg_sample_local(1, float(g[i]), O.fp(O.f(u[i:i + 1])), O.fp(out))
+        exp[i] = out
+    np.testing.assert_allclose(got, exp, atol=2e-6, rtol=RTOL)
+    # GGX D and pdf
+    h = rng.normal(size=(n, 3))
+    h[:, 2] = np.abs(h[:, 2])
+    h = (h / np.linalg.norm(h, axis=1, keepdims=True)).astype(np.float32)
+    alpha = rng.uniform(0.0025, 0.64, n).astype(np.float32)
+    got = ctx.test_bsdf(abi.NE_TEST_GGX_D, n, [h, alpha], 1)[:, 0]
+    exp = O.vec_op("neo_ggx_D", n, alpha, h, out_shape=(n,))
+    assert (_rel(got, exp) <= RTOL).all()
+    wo = rng.normal(size=(n, 3))
+    wo[:, 2] = np.abs(wo[:, 2])
+    wo = (wo / np.linalg.norm(wo, axis=1, keepdims=True)).astype(np.float32)
+    got = ctx.test_bsdf(abi.NE_TEST_GGX_PDF, n, [wo, h, alpha], 1)[:, 0]
+    exp = O.vec_op("neo_ggx_pdf", n, alpha, wo, h, out_shape=(n,))
+    assert (_rel(got, exp) <= RTOL).all()
+
+
+def test_glossy_and_forward_hg_parity(ctx):
+    print_case("glossy/hg forward")
+    rng = np.random.default_rng(8)
+    n = 20000
+    mat = np.concatenate([rng.uniform(0.1, 0.9, (n, 3)), rng.uniform(0.05, 0.8, (n, 1)), rng.uniform(0, 1, (n, 1))],
+                         axis=1).astype(np.float32)
+    wo = rng.normal(size=(n, 3))
+    wo[:, 2] = np.abs(wo[:, 2]) + 0.01
+    wo = (wo / np.linalg.norm(wo, axis=1, keepdims=True)).astype(np.float32)
+    wi = rng.normal(size=(n, 3))
+    wi = (wi / np.linalg.norm(wi, axis=1, keepdims=True)).astype(np.float32)
+    u = rng.random((n, 3)).astype(np.float32)
+    got = ctx.test_bsdf(abi.NE_TEST_GLOSSY, n, [mat, wo, wi, u], 7)
+    f = np.zeros((n, 3), np.float32)
+    p = np.zeros(n, np.float32)
+    sm = np.zeros((n, 3), np.float32)
+    O.lib().neo_glossy_c(n, O.fp(mat), O.fp(wo), O.fp(wi), O.fp(u), O.fp(f), O.fp(p), O.fp(sm))
+    np.testing.assert_allclose(got[:, :3], f, rtol=RTOL, atol=1e-30)
+    np.testing.assert_allclose(got[:, 3], p, rtol=RTOL, atol=1e-30)
+    np.testing.assert_allclose(got[:, 4:7], sm, rtol=RTOL, atol=2e-6)
+    d = wi
+    g = rng.uniform(-0.95, 0.95, n).astype(np.float32)
+    u2 = rng.random((n, 2)).astype(np.float32)
+    got = ctx.test_bsdf(abi.NE_TEST_HG_FWD, n, [d, u2, g], 4)
+    ws = O.vec_op("neo_hg_sample", n, g, d, u2, out_shape=(n, 3))
+    np.testing.assert_allclose(got[:, :3], ws, rtol=RTOL, atol=2e-6)
+    pp = O.vec_op("neo_hg_p", n, g, np.sum(d * got[:, :3], axis=1), out_shape=(n,))
+    assert (_rel(got[:, 3], pp) <= 1e-4).all()
+
+
+# ---- BVH closest hit against the oracle's brute-force scan ---------------------
+@pytest.mark.parametrize("which", ["soup", "box"])
+def test_closest_hit_matches_brute_force(ctx, which):
+    print_case(f"closest {which}")
+    s = scenes.triangle_soup() if which == "soup" else scenes.small_surface(subdiv=4)
+    g = ctx.upload(s)
+    w = O.World(s)
+    n = 20000
+    rays = scenes.random_rays(n, seed=9, origin_box=1.2) if which == "soup" else \
+        scenes.rays_towards(n, target_box=0.9, origin_radius=2.5, seed=9)
+    if which == "box":
+        rays[:, 1] += 1.0
+    t, ty, ix = ctx.test_closest(g, rays)
+    bt, bty, bix = w.closest(rays, brute=True)
+    np.testing.assert_array_equal(ty, bty)
+    np.testing.assert_array_equal(ix, bix)
+    m = bty >= 0
+    assert m.mean() > 0.3
+    assert (_rel(t[m], bt[m]) <= RTOL).all()
+
+
+# ---- volume -------------------------------------------------------------------
+def test_density_bit_exact(ctx):
+    print_case("density")
+    s = scenes.small_volume(res=32)
+    g = ctx.upload(s)
+    rng = np.random.default_rng(10)
+    pts = rng.uniform(-1, 33, (20000, 3)).astype(np.float32)
+    pts[:100] = np.round(pts[:100])  # exact voxel corners, incl. the upper faces
+    pts[100:200, 0] = 32.0
+    pts = np.clip(pts, 0, None)  # the reference aborts on negative grid coordinates
+    got = ctx.test_density(g, pts)
+    grid = s.volume["grid"]
+    exp = np.zeros(len(pts), np.float32)
+    O.lib().neo_interp_density(32, 32, 32, O.fp(grid), len(pts), O.fp(O.f(pts)), O.fp(exp))
+    np.testing.assert_array_equal(got, exp)
+
+
+@pytest.mark.parametrize("maj_res", [1, 4])
+def test_delta_tracking_trajectories(ctx, maj_res):
+    print_case(f"delta tracking maj {maj_res}")
+    s = scenes.small_volume(res=32, maj_res=maj_res)
+    g = ctx.upload(s)
+    w = O.World(s)
+    n = 20000
+    rays = scenes.rays_towards(n, target_box=0.5, origin_radius=1.5, seed=11)
+    sc, t = ctx.test_delta_track(g, rays, seed=3)
+    osc, ot = w.delta_track(rays, seed=3)
+    same = (sc == osc) & (np.abs(t - ot) <= 1e-5 * np.maximum(np.abs(ot), 1e-6))
+    print(f"delta tracking identical trajectories: {same.mean():.5f}")
+    assert same.mean() > 0.99
+    # statistical agreement of the collision probability
+    p, q = sc.mean(), osc.mean()
+    assert abs(p - q) < 4 * np.sqrt(p * (1 - p) / n) + 1e-3
+
+
+def test_transmittance_matches_oracle(ctx):
+    print_case("transmittance")
+    s = scenes.small_volume(res=32, maj_res=4)
+    g = ctx.upload(s)
+    w = O.World(s)
+    n = 20000
+    rays = scenes.rays_towards(n, target_box=0.5, origin_radius=1.5, seed=12)
+    tmax = np.full(n, np.inf, np.float32)
+    tr = ctx.test_transmittance(g, rays, tmax, seed=4)
+    otr = w.transmittance(rays, tmax, seed=4)
+    same = np.abs(tr - otr) <= 1e-4 * np.maximum(otr, 1e-6)
+    print(f"transmittance identical: {same.mean():.5f}")
+    assert same.mean() > 0.99
+    assert abs(tr.mean() - otr.mean()) < 4 * np.sqrt((tr.var() + otr.var()) / n) + 1e-4
+    # FP64 path agrees with FP32 in expectation
+    tr64 = ctx.test_transmittance(g, rays, tmax, seed=4, precision=NE_FP64)
+    assert abs(tr64.mean() - tr.mean()) < 4 * np.sqrt(2 * tr.var() / n) + 1e-4
+
+
+# ---- full paths -----------------------------------------------------------------
+def _render_pair(ctx, s, cam, W, H, spp, bounces, seed=21, precision=NE_FP32):
+    g = ctx.upload(s)
+    film = ctx.film(W, H)
+    ctx.render(g, cam, film, spp=spp, max_bounces=bounces, seed=seed, precision=precision)
+    gpu = film.read()
+    w = O.World(s, fp64=precision == NE_FP64)
+    cpu = w.render(cam, W, H, spp, bounces, seed=seed, threads=8)
+    return gpu, cpu
+
+
+@pytest.mark.parametrize("which", ["volume", "surface", "cloud"])
+def test_render_matches_oracle(ctx, which):
+    print_case(f"render {which}")
+    if which == "volume":
+        s, cam, bounces = scenes.small_volume(res=32, maj_res=4), scenes.VOLUME_CAMERA, 16
+    elif which == "cloud":
+        s, cam, bounces = scenes.small_cloud(), scenes.VOLUME_CAMERA, 64
+    else:
+        s, cam, bounces = scenes.small_surface(subdiv=3), scenes.BOX_CAMERA, 8
+    W = H = 32
+    spp = 16
+    gpu, cpu = _render_pair(ctx, s, cam, W, H, spp, bounces)
+    same = np.all(np.abs(gpu - cpu) <= 1e-4 * np.maximum(np.abs(cpu), 1e-3), axis=2)
+    print(f"{which}: identical pixels {same.mean():.4f}; mean gpu {gpu.mean() / spp:.5f} cpu {cpu.mean() / spp:.5f}")
+    assert same.mean() > 0.9
+    # image means agree within Monte Carlo error
+    assert abs(gpu.mean() - cpu.mean()) <= 0.02 * abs(cpu.mean()) + 1e-6
+
+
+def test_render_fp64_mode_matches_oracle_fp64(ctx):
+    print_case("render fp64")
+    s = scenes.small_volume(res=32, maj_res=4)
+    gpu, cpu = _render_pair(ctx, s, scenes.VOLUME_CAMERA, 24, 24, 8, 16, precision=NE_FP64)
+    same = np.all(np.abs(gpu - cpu) <= 1e-9 * np.maximum(np.abs(cpu), 1e-3), axis=2)
+    print(f"fp64 identical pixels {same.mean():.4f}")
+    assert same.mean() > 0.9
+
+
+def test_render_is_deterministic_across_launch_shapes(ctx):
+    print_case("determinism")
+    s = scenes.small_volume(res=32, maj_res=4)
+    g = ctx.upload(s)
+    cam = scenes.VOLUME_CAMERA
+    outs = []
+    for spt, blk in [(0, 0), (1, 64), (3, 256), (16, 128)]:
+        film = ctx.film(40, 24)
+        ctx.render(g, cam, film, spp=16, max_bounces=16, seed=5, samples_per_thread=spt, block_size=blk)
+        outs.append(film.read())
+    for o in outs[1:]:
+        np.testing.assert_allclose(o, outs[0], rtol=1e-12, atol=0)
+
+
+def test_split_sample_ranges_add_up(ctx):
+    print_case("sample offsets")
+    s = scenes.small_surface(subdiv=2)
+    g = ctx.upload(s)
+    cam = scenes.BOX_CAMERA
+    a = ctx.film(16, 16)
+    ctx.render(g, cam, a, spp=8, max_bounces=8, seed=9)
+    b = ctx.film(16, 16)
+    ctx.render(g, cam, b, spp=3, max_bounces=8, seed=9)
+    ctx.render(g, cam, b, spp=5, max_bounces=8, seed=9, sample_offset=3)
+    np.testing.assert_allclose(a.read(), b.read(), rtol=1e-12)
+
+
+def test_tile_api_matches_full_frame(ctx):
+    print_case("tiles")
+    from narvalengine_amd import OfflineEngine, SceneSettings
+    s = scenes.small_volume(res=32, maj_res=4)
+    settings = SceneSettings(resolution=(80, 40), spp=4, bounces=8)
+    cam = scenes.VOLUME_CAMERA
+    cam = type(cam)(position=cam.position, look_at=cam.look_at, up=cam.up, vfov=cam.vfov, aspect=2.0)
+    eng = OfflineEngine(cam, settings, s, ctx=ctx, seed=3)
+    for i in range(400):
+        eng.renderTile(cam, i)
+    full = ctx.film(80, 40)
+    ctx.render(eng.gscene, cam, full, spp=4, max_bounces=8, seed=3)
+    ref = full.tonemap(4)
+    np.testing.assert_allclose(eng.pixels, ref, rtol=1e-6, atol=1e-7)
+    # post-processing matches the oracle's OfflineEngine::postProcessing
+    lin = full.read() / 4
+    pp = O.vec_op("neo_post_processing", lin.size // 3, lin.reshape(-1, 3), out_shape=(lin.size // 3, 3))
+    np.testing.assert_allclose(ref.reshape(-1, 3), pp, rtol=1e-5, atol=1e-6)
+
+
+def test_invalid_arguments_are_reported(ctx):
+    print_case("errors")
+    from narvalengine_amd import NarvalError
+    from narvalengine_amd.scene import Scene
+    s = Scene()
+    m = s.add_glossy((0.5, 0.5, 0.5), 0.5)
+    s.add_sphere((0, 0, 0), -1.0, m)
+    with pytest.raises(NarvalError, match="radius"):
+        ctx.upload(s)
+    g = ctx.upload(scenes.small_volume(res=8, maj_res=2))
+    film = ctx.film(8, 8)
+    with pytest.raises(NarvalError, match="spp"):
+        ctx.render(g, scenes.VOLUME_CAMERA, film, spp=0, max_bounces=4)
+    with pytest.raises(NarvalError, match="rectangle"):
+        ctx.render(g, scenes.VOLUME_CAMERA, film, spp=1, max_bounces=4, rect=(0, 0, 9, 8))
