@@ -156,6 +156,8 @@ typedef struct ne_scene_info {
 
 /* ---- context ------------------------------------------------------------ */
 ne_status ne_ctx_create(int32_t device, ne_ctx** out);
+/* Releases the caller's context handle.  Scenes and films created on the
+ * context keep it alive until they are released, in any order. */
 void ne_ctx_destroy(ne_ctx* ctx);
 /* Last error message of this context (static string if ctx is NULL). */
 const char* ne_last_error(const ne_ctx* ctx);

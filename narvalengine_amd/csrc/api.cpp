@@ -7,6 +7,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <atomic>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -31,6 +32,9 @@ struct ne_ctx {
   size_t partial_bytes = 0;
   hipEvent_t ev0 = nullptr, ev1 = nullptr;
   bool timed = false;
+  // One reference for the caller's handle plus one per live scene/film: the
+  // context outlives every object created on it, whatever the release order.
+  std::atomic<int> refs{1};
 };
 
 struct ne_scene {
@@ -281,8 +285,7 @@ ne_status ne_ctx_create(int32_t device, ne_ctx** out) {
   return NE_OK;
 }
 
-void ne_ctx_destroy(ne_ctx* c) {
-  if (!c) return;
+static void ctx_free(ne_ctx* c) {
   (void)hipSetDevice(c->device);
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->d_partial) (void)hipFree(c->d_partial);
@@ -291,6 +294,15 @@ void ne_ctx_destroy(ne_ctx* c) {
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
   delete c;
+}
+
+static void ctx_unref(ne_ctx* c) {
+  if (c->refs.fetch_sub(1) == 1) ctx_free(c);
+}
+
+void ne_ctx_destroy(ne_ctx* c) {
+  if (!c) return;
+  ctx_unref(c);
 }
 
 ne_status ne_synchronize(ne_ctx* c) {
@@ -415,17 +427,22 @@ ne_status ne_scene_upload(ne_ctx* c, const ne_scene_desc* d, ne_scene** out) {
   s->info.maj_res = uint32_t(d->has_volume ? d->maj_res : 0);
   s->info.device_bytes = s->bytes;
   s->info.grid_max_density = gmax;
+  c->refs.fetch_add(1);
   *out = s;
   return NE_OK;
 }
 
 void ne_scene_release(ne_scene* s) {
   if (!s) return;
-  std::lock_guard<std::mutex> lk(s->ctx->mu);
-  (void)hipSetDevice(s->ctx->device);
-  (void)hipStreamSynchronize(s->ctx->stream);
-  for (void* p : s->allocs) (void)hipFree(p);
-  delete s;
+  ne_ctx* c = s->ctx;
+  {
+    std::lock_guard<std::mutex> lk(c->mu);
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    for (void* p : s->allocs) (void)hipFree(p);
+    delete s;
+  }
+  ctx_unref(c);
 }
 
 ne_status ne_scene_get_info(const ne_scene* s, ne_scene_info* out) {
@@ -448,17 +465,22 @@ ne_status ne_film_create(ne_ctx* c, int32_t w, int32_t h, ne_film** out) {
   e = hipMemsetAsync(f->d_sum, 0, bytes, c->stream);
   if (e == hipSuccess) e = hipStreamSynchronize(c->stream);
   if (e != hipSuccess) { (void)hipFree(f->d_sum); delete f; return fail(c, NE_ERR_HIP, "film clear failed"); }
+  c->refs.fetch_add(1);
   *out = f;
   return NE_OK;
 }
 
 void ne_film_release(ne_film* f) {
   if (!f) return;
-  std::lock_guard<std::mutex> lk(f->ctx->mu);
-  (void)hipSetDevice(f->ctx->device);
-  (void)hipStreamSynchronize(f->ctx->stream);
-  (void)hipFree(f->d_sum);
-  delete f;
+  ne_ctx* c = f->ctx;
+  {
+    std::lock_guard<std::mutex> lk(c->mu);
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(f->d_sum);
+    delete f;
+  }
+  ctx_unref(c);
 }
 
 ne_status ne_film_clear(ne_film* f) {

@@ -353,3 +353,17 @@ def test_invalid_arguments_are_reported(ctx):
         ctx.render(g, scenes.VOLUME_CAMERA, film, spp=0, max_bounces=4)
     with pytest.raises(NarvalError, match="rectangle"):
         ctx.render(g, scenes.VOLUME_CAMERA, film, spp=1, max_bounces=4, rect=(0, 0, 9, 8))
+
+
+def test_handles_release_in_any_order():
+    print_case("release order")
+    from narvalengine_amd import Context
+    c = Context(0)
+    g = c.upload(scenes.small_volume(res=8, maj_res=2))
+    f = c.film(8, 8)
+    c.render(g, scenes.VOLUME_CAMERA, f, spp=2, max_bounces=4)
+    c.close()          # context handle first: scene and film keep it alive
+    before = f.read()  # film still usable for reads
+    assert np.isfinite(before).all()
+    g.close()
+    f.close()
