@@ -7,6 +7,7 @@ from pathlib import Path
 NE_FP32, NE_FP64 = 0, 1
 NE_FLAG_COUNT = 1
 NE_FLAG_TIME = 2
+NE_MODE_AUTO, NE_MODE_MEGAKERNEL, NE_MODE_WAVEFRONT = 0, 1, 2
 NE_MAT_GLOSSY, NE_MAT_EMITTER = 0, 1
 NE_ENV_NONE, NE_ENV_CONSTANT, NE_ENV_SKY = 0, 1, 2
 NE_TEST_AABB, NE_TEST_TRIANGLE, NE_TEST_SPHERE, NE_TEST_RECTANGLE = 0, 1, 2, 3
@@ -48,7 +49,7 @@ class RenderParams(C.Structure):
         ("seed", C.c_uint64), ("sample_offset", C.c_uint32), ("job", C.c_uint32),
         ("precision", C.c_int32), ("flags", C.c_int32),
         ("samples_per_thread", C.c_int32), ("block_size", C.c_int32),
-        ("reserved", C.c_int32 * 4),
+        ("occupancy", C.c_int32), ("mode", C.c_int32), ("pool_size", C.c_int32),
     ]
 
 

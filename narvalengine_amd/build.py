@@ -19,7 +19,7 @@ PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 BUILD = PKG / "_build"
 LIB = PKG / "libnarval_gpu.so"
-SOURCES = ["render.hip", "api.cpp", "bvh_build.cpp", "generators.cpp"]
+SOURCES = ["render.hip", "wavefront.hip", "api.cpp", "bvh_build.cpp", "generators.cpp"]
 HEADERS = ["ne_math.h", "ne_scene.h", "ne_trace.h", "ne_launch.h", "bvh_build.h"]
 
 

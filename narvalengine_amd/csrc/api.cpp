@@ -35,6 +35,9 @@ struct ne_ctx {
   // One reference for the caller's handle plus one per live scene/film: the
   // context outlives every object created on it, whatever the release order.
   std::atomic<int> refs{1};
+  void* wf_buf = nullptr;  // wavefront path pool (grown on demand)
+  uint32_t wf_P = 0;
+  uint32_t* h_status = nullptr;  // pinned status word read by the wavefront loop
 };
 
 struct ne_scene {
@@ -49,6 +52,7 @@ struct ne_film {
   ne_ctx* ctx = nullptr;
   int W = 0, H = 0;
   double* d_sum = nullptr;
+  long long* d_fixed = nullptr;  // wavefront fixed-point accumulator, flushed into d_sum per render
 };
 
 namespace {
@@ -171,11 +175,19 @@ ne_status check_params(ne_ctx* c, const ne_render_params* p) {
   if (p->x0 < 0 || p->y0 < 0 || p->x1 > p->width || p->y1 > p->height || p->x0 >= p->x1 || p->y0 >= p->y1)
     return fail(c, NE_ERR_INVALID_ARG, "pixel rectangle out of range or empty");
   if (p->spp <= 0) return fail(c, NE_ERR_INVALID_ARG, "spp must be > 0");
-  if (p->max_bounces < 0 || p->max_bounces > 4096) return fail(c, NE_ERR_INVALID_ARG, "max_bounces out of range");
+  if (p->max_bounces < 0 || p->max_bounces > kMaxVertices)
+    return fail(c, NE_ERR_INVALID_ARG, "max_bounces out of range [0, 2047]");
   if (p->precision != NE_FP32 && p->precision != NE_FP64) return fail(c, NE_ERR_INVALID_ARG, "unknown precision");
   if (p->block_size != 0 && p->block_size != 64 && p->block_size != 128 && p->block_size != 256)
     return fail(c, NE_ERR_INVALID_ARG, "block_size must be 0, 64, 128 or 256");
   if (p->samples_per_thread < 0) return fail(c, NE_ERR_INVALID_ARG, "samples_per_thread must be >= 0");
+  if (p->occupancy != 0 && p->occupancy != 4 && p->occupancy != 6)
+    return fail(c, NE_ERR_INVALID_ARG, "occupancy must be 0, 4 or 6");
+  if (p->mode < NE_MODE_AUTO || p->mode > NE_MODE_WAVEFRONT) return fail(c, NE_ERR_INVALID_ARG, "unknown mode");
+  if (p->mode == NE_MODE_WAVEFRONT && p->precision != NE_FP32)
+    return fail(c, NE_ERR_INVALID_ARG, "the wavefront pipeline is FP32 only");
+  if (p->pool_size < 0 || p->pool_size > (1 << 28))
+    return fail(c, NE_ERR_INVALID_ARG, "pool_size must be in [0, 2^28]");
   return NE_OK;
 }
 
@@ -190,10 +202,12 @@ std::vector<PrimInput> prim_bounds(const ne_scene_desc* d) {
       prims[i].bmin[a] = std::min(v[a], std::min(v[3 + a], v[6 + a]));
       prims[i].bmax[a] = std::max(v[a], std::max(v[3 + a], v[6 + a]));
     }
+    prims[i].type = PRIM_TRI;
   }
   for (uint32_t i = 0; i < ns; ++i) {
     const float* q = d->sph + 4 * size_t(i);
     for (int a = 0; a < 3; ++a) { prims[nt + i].bmin[a] = q[a] - q[3]; prims[nt + i].bmax[a] = q[a] + q[3]; }
+    prims[nt + i].type = PRIM_SPHERE;
   }
   for (uint32_t i = 0; i < nr; ++i) {
     const float* r = d->rect + 12 * size_t(i);
@@ -202,6 +216,7 @@ std::vector<PrimInput> prim_bounds(const ne_scene_desc* d) {
       prims[nt + ns + i].bmin[a] = std::min(std::min(c0, c1), std::min(c2, c3));
       prims[nt + ns + i].bmax[a] = std::max(std::max(c0, c1), std::max(c2, c3));
     }
+    prims[nt + ns + i].type = PRIM_RECT;
   }
   return prims;
 }
@@ -276,7 +291,9 @@ ne_status ne_ctx_create(int32_t device, ne_ctx** out) {
   ne_ctx* c = new ne_ctx();
   c->device = device;
   if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-      hipMalloc(&c->d_counters, sizeof(DevCounters)) != hipSuccess || hipEventCreate(&c->ev0) != hipSuccess ||
+      hipMalloc(&c->d_counters, sizeof(DevCounters)) != hipSuccess ||
+      hipHostMalloc((void**)&c->h_status, 64 * sizeof(uint32_t), hipHostMallocDefault) != hipSuccess ||
+      hipEventCreate(&c->ev0) != hipSuccess ||
       hipEventCreate(&c->ev1) != hipSuccess) {
     delete c;
     return fail(nullptr, NE_ERR_HIP, "context setup failed");
@@ -290,6 +307,8 @@ static void ctx_free(ne_ctx* c) {
   if (c->stream) (void)hipStreamSynchronize(c->stream);
   if (c->d_partial) (void)hipFree(c->d_partial);
   if (c->d_counters) (void)hipFree(c->d_counters);
+  if (c->wf_buf) (void)hipFree(c->wf_buf);
+  if (c->h_status) (void)hipHostFree(c->h_status);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -329,16 +348,19 @@ ne_status ne_scene_upload(ne_ctx* c, const ne_scene_desc* d, ne_scene** out) {
     return e;
   };
   DevScene& ds = s->dev;
-  // ---- surfaces: BVH over triangles, spheres, rectangles
+  // ---- surfaces: BVH over triangles, spheres, rectangles.  Each type is
+  // stored in leaf order, and leaf children point at their type's storage.
   const uint32_t nt = d->n_tri, ns = d->n_sph, nr = d->n_rect;
   BvhOutput bvh;
   build_bvh(prim_bounds(d), bvh);
   if (bvh.max_depth + 1 > uint32_t(kShortStack + kSpillStack)) return bail(fail(c, NE_ERR_UNSUPPORTED, "BVH too deep"));
-  std::vector<uint32_t> refs(bvh.refs.size());
-  std::vector<float> tris(size_t(nt) * 12);
-  uint32_t tslot = 0;
+  std::vector<float> tris(size_t(nt) * 12), sph(size_t(ns) * 4), rects(size_t(nr) * 16);
+  std::vector<int32_t> sph_mat(ns), sph_orig(ns);
+  std::vector<uint32_t> storage(bvh.refs.size());  // refs position -> slot within its type's array
+  std::vector<int32_t> rect_slot(nr, -1);
+  uint32_t tslot = 0, sslot = 0, rslot = 0;
   for (size_t i = 0; i < bvh.refs.size(); ++i) {
-    uint32_t p = bvh.refs[i];
+    const uint32_t p = bvh.refs[i];
     if (p < nt) {
       const float* v = d->tri_v + 9 * size_t(p);
       float* o = &tris[size_t(tslot) * 12];
@@ -346,38 +368,53 @@ ne_status ne_scene_upload(ne_ctx* c, const ne_scene_desc* d, ne_scene** out) {
       o[0] = v[0]; o[1] = v[1]; o[2] = v[2]; std::memcpy(&o[3], &m, 4);
       o[4] = v[3] - v[0]; o[5] = v[4] - v[1]; o[6] = v[5] - v[2]; std::memcpy(&o[7], &p, 4);
       o[8] = v[6] - v[0]; o[9] = v[7] - v[1]; o[10] = v[8] - v[2]; o[11] = 0.0f;
-      refs[i] = (uint32_t(PRIM_TRI) << 30) | tslot;
-      ++tslot;
+      storage[i] = tslot++;
     } else if (p < nt + ns) {
-      refs[i] = (uint32_t(PRIM_SPHERE) << 30) | (p - nt);
+      const uint32_t q = p - nt;
+      std::memcpy(&sph[4 * size_t(sslot)], d->sph + 4 * size_t(q), 16);
+      sph_mat[sslot] = d->sph_mat[q];
+      sph_orig[sslot] = int32_t(q);
+      storage[i] = sslot++;
     } else {
-      refs[i] = (uint32_t(PRIM_RECT) << 30) | (p - nt - ns);
+      const uint32_t q = p - nt - ns;
+      const float* r = d->rect + 12 * size_t(q);
+      float* o = &rects[16 * size_t(rslot)];
+      int32_t m = d->rect_mat[q];
+      o[0] = r[0]; o[1] = r[1]; o[2] = r[2]; std::memcpy(&o[3], &m, 4);
+      o[4] = r[3]; o[5] = r[4]; o[6] = r[5]; o[7] = rect_area(r + 3, r + 6);
+      o[8] = r[6]; o[9] = r[7]; o[10] = r[8]; std::memcpy(&o[11], &q, 4);
+      o[12] = r[9]; o[13] = r[10]; o[14] = r[11]; o[15] = 0.0f;
+      rect_slot[q] = int32_t(rslot);
+      storage[i] = rslot++;
     }
   }
-  std::vector<float> sph(size_t(ns) * 4), rects(size_t(nr) * 16);
-  for (uint32_t i = 0; i < ns; ++i) std::memcpy(&sph[4 * size_t(i)], d->sph + 4 * size_t(i), 16);
-  std::vector<int32_t> light_rects;
-  for (uint32_t i = 0; i < nr; ++i) {
-    const float* r = d->rect + 12 * size_t(i);
-    float* o = &rects[16 * size_t(i)];
-    int32_t m = d->rect_mat[i];
-    o[0] = r[0]; o[1] = r[1]; o[2] = r[2]; std::memcpy(&o[3], &m, 4);
-    o[4] = r[3]; o[5] = r[4]; o[6] = r[5]; o[7] = rect_area(r + 3, r + 6);
-    o[8] = r[6]; o[9] = r[7]; o[10] = r[8]; o[11] = 0.0f;
-    o[12] = r[9]; o[13] = r[10]; o[14] = r[11]; o[15] = 0.0f;
-    if (d->mat[12 * size_t(m)] == float(NE_MAT_EMITTER)) light_rects.push_back(int32_t(i));
+  // leaf children: refs position -> storage slot of the leaf's type
+  for (size_t k = 0; k < bvh.nodes.size() / 16; ++k) {
+    float* f = &bvh.nodes[k * 16];
+    for (int side = 0; side < 2; ++side) {
+      int32_t first, ct;
+      std::memcpy(&first, &f[12 + side], 4);
+      std::memcpy(&ct, &f[14 + side], 4);
+      if (first < 0 || (uint32_t(ct) & kLeafCountMask) == 0) continue;
+      int32_t slot = int32_t(storage[size_t(first)]);
+      std::memcpy(&f[12 + side], &slot, 4);
+    }
   }
+  // emitter rectangles in input order (the light-selection order), as storage slots
+  std::vector<int32_t> light_rects;
+  for (uint32_t q = 0; q < nr; ++q)
+    if (d->mat[12 * size_t(d->rect_mat[q])] == float(NE_MAT_EMITTER)) light_rects.push_back(rect_slot[q]);
   const float4* f4 = nullptr;
   if ((st = dev_upload(c, s, reinterpret_cast<const float4*>(bvh.nodes.data()), bvh.nodes.size() / 4, &f4)) != NE_OK)
     return bail(st);
   ds.nodes = f4;
   ds.n_nodes = uint32_t(bvh.nodes.size() / 16);
-  if ((st = dev_upload(c, s, refs.data(), refs.size(), &ds.refs)) != NE_OK) return bail(st);
   if ((st = dev_upload(c, s, reinterpret_cast<const float4*>(tris.data()), tris.size() / 4, &ds.tris)) != NE_OK)
     return bail(st);
   if ((st = dev_upload(c, s, reinterpret_cast<const float4*>(sph.data()), sph.size() / 4, &ds.sph)) != NE_OK)
     return bail(st);
-  if ((st = dev_upload(c, s, d->sph_mat, ns, &ds.sph_mat)) != NE_OK) return bail(st);
+  if ((st = dev_upload(c, s, sph_mat.data(), ns, &ds.sph_mat)) != NE_OK) return bail(st);
+  if ((st = dev_upload(c, s, sph_orig.data(), ns, &ds.sph_orig)) != NE_OK) return bail(st);
   if ((st = dev_upload(c, s, reinterpret_cast<const float4*>(rects.data()), rects.size() / 4, &ds.rects)) != NE_OK)
     return bail(st);
   if ((st = dev_upload(c, s, d->mat, size_t(d->n_mat) * 12, &ds.mats)) != NE_OK) return bail(st);
@@ -478,6 +515,7 @@ void ne_film_release(ne_film* f) {
     (void)hipSetDevice(c->device);
     (void)hipStreamSynchronize(c->stream);
     (void)hipFree(f->d_sum);
+    if (f->d_fixed) (void)hipFree(f->d_fixed);
     delete f;
   }
   ctx_unref(c);
@@ -518,6 +556,47 @@ ne_status ne_film_tonemap(ne_film* f, int32_t spp, float* pixels) {
   return NE_OK;
 }
 
+static ne_status render_wavefront(ne_ctx* c, const ne_scene* s, const DevCamera& dc, const ne_render_params* p,
+                                  ne_film* f, bool count, bool timed) {
+  const uint64_t total = uint64_t(p->x1 - p->x0) * uint64_t(p->y1 - p->y0) * uint64_t(p->spp);
+  // pool slots: whole chunks, no more than the job needs
+  uint64_t P = p->pool_size > 0 ? uint64_t(p->pool_size) : (uint64_t(1) << 21);
+  P = std::min<uint64_t>(P, total);
+  P = (P + kChunk - 1) / kChunk * kChunk;
+  if (P > c->wf_P) {
+    if (c->wf_buf) {
+      NE_HIP(c, hipStreamSynchronize(c->stream));
+      NE_HIP(c, hipFree(c->wf_buf));
+      c->wf_buf = nullptr;
+      c->wf_P = 0;
+    }
+    NE_HIP(c, hipMalloc(&c->wf_buf, wf_bytes(uint32_t(P))));
+    c->wf_P = uint32_t(P);
+  }
+  if (!f->d_fixed) {
+    const size_t bytes = size_t(f->W) * f->H * 3 * sizeof(long long);
+    NE_HIP(c, hipMalloc(&f->d_fixed, bytes));
+    NE_HIP(c, hipMemsetAsync(f->d_fixed, 0, bytes, c->stream));
+  }
+  WfState w{};
+  wf_carve(c->wf_buf, uint32_t(P), w);
+  WfArgs a{};
+  a.W = p->width; a.H = p->height; a.x0 = p->x0; a.y0 = p->y0; a.x1 = p->x1; a.y1 = p->y1;
+  a.max_bounces = p->max_bounces; a.sample_offset = p->sample_offset; a.job = p->job; a.nchunks = uint32_t(P / kChunk);
+  a.seed = p->seed;
+  a.total_paths = total;
+  a.counters = c->d_counters;
+  a.film_fixed = f->d_fixed;
+  a.film = f->d_sum;
+  if (timed) NE_HIP(c, hipEventRecord(c->ev0, c->stream));
+  NE_HIP(c, launch_wavefront(s->dev, dc, w, a, count, c->stream, c->h_status));
+  if (timed) {
+    NE_HIP(c, hipEventRecord(c->ev1, c->stream));
+    c->timed = true;
+  }
+  return NE_OK;
+}
+
 static ne_status render_locked(ne_ctx* c, const ne_scene* s, const ne_camera_desc* cam, const ne_render_params* p,
                                ne_film* f) {
   ne_status st = check_params(c, p);
@@ -526,6 +605,15 @@ static ne_status render_locked(ne_ctx* c, const ne_scene* s, const ne_camera_des
   if (s->ctx != c || f->ctx != c) return fail(c, NE_ERR_INVALID_ARG, "scene/film belong to another context");
   if (f->W != p->width || f->H != p->height) return fail(c, NE_ERR_INVALID_ARG, "film size differs from params");
   NE_HIP(c, hipSetDevice(c->device));
+  const bool count = (p->flags & NE_FLAG_COUNT) != 0;
+  const bool timed = (p->flags & NE_FLAG_TIME) != 0;
+  if (count) {
+    NE_HIP(c, hipMemsetAsync(c->d_counters, 0, sizeof(DevCounters), c->stream));
+    c->counters_valid = true;
+  }
+  DevCamera dc = make_camera(cam);
+  const bool wavefront = p->mode == NE_MODE_WAVEFRONT || (p->mode == NE_MODE_AUTO && p->precision == NE_FP32);
+  if (wavefront) return render_wavefront(c, s, dc, p, f, count, timed);
   RenderArgs a{};
   a.W = p->width; a.H = p->height; a.x0 = p->x0; a.y0 = p->y0; a.x1 = p->x1; a.y1 = p->y1;
   a.spp = p->spp; a.max_bounces = p->max_bounces; a.sample_offset = p->sample_offset; a.job = p->job;
@@ -543,11 +631,6 @@ static ne_status render_locked(ne_ctx* c, const ne_scene* s, const ne_camera_des
   a.ntx = (p->x1 - p->x0 + 7) / 8;
   a.nty = (p->y1 - p->y0 + 7) / 8;
   a.counters = c->d_counters;
-  const bool count = (p->flags & NE_FLAG_COUNT) != 0;
-  if (count) {
-    NE_HIP(c, hipMemsetAsync(c->d_counters, 0, sizeof(DevCounters), c->stream));
-    c->counters_valid = true;
-  }
   if (a.nchunks == 1) {
     a.out = f->d_sum;
   } else {
@@ -564,11 +647,9 @@ static ne_status render_locked(ne_ctx* c, const ne_scene* s, const ne_camera_des
     }
     a.out = c->d_partial;
   }
-  DevCamera dc = make_camera(cam);
   int block = p->block_size ? p->block_size : 128;
-  const bool timed = (p->flags & NE_FLAG_TIME) != 0;
   if (timed) NE_HIP(c, hipEventRecord(c->ev0, c->stream));
-  NE_HIP(c, launch_render(s->dev, dc, a, p->precision, count, block, c->stream));
+  NE_HIP(c, launch_render(s->dev, dc, a, p->precision, count, block, p->occupancy, c->stream));
   if (timed) {
     NE_HIP(c, hipEventRecord(c->ev1, c->stream));
     c->timed = true;

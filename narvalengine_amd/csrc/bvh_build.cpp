@@ -5,7 +5,8 @@
 //
 // Replaces the reference's midpoint-split builder (src/primitives/BVH.cpp:6-106);
 // traversal results (closest primitive, t) are the contract, not the tree
-// shape.
+// shape.  Leaves hold one primitive type, so a leaf child points straight at
+// that type's storage (no per-primitive indirection in traversal).
 #include "bvh_build.h"
 
 #include <algorithm>
@@ -94,7 +95,25 @@ void build_bvh(const std::vector<PrimInput>& prims, BvhOutput& out) {
       m.count = cnt;
       m.left = m.right = -1;
     };
-    if (cnt <= 1 || tk.depth >= kMaxDepth) { make_leaf(); continue; }
+    // a leaf over a range that mixes primitive types is split by type instead
+    auto leaf_or_type_split = [&]() {
+      const uint32_t t0 = prims[idx[tk.start]].type;
+      bool mixed = false;
+      for (uint32_t i = tk.start + 1; i < tk.end && !mixed; ++i) mixed = prims[idx[i]].type != t0;
+      if (!mixed) { make_leaf(); return; }
+      auto it = std::stable_partition(idx.begin() + tk.start, idx.begin() + tk.end,
+                                      [&](uint32_t p) { return prims[p].type == t0; });
+      const uint32_t mid = uint32_t(it - idx.begin());
+      int32_t l = int32_t(bn.size());
+      bn.emplace_back();
+      int32_t r = int32_t(bn.size());
+      bn.emplace_back();
+      bn[size_t(tk.node)].left = l;
+      bn[size_t(tk.node)].right = r;
+      tasks.push_back({r, mid, tk.end, tk.depth + 1});
+      tasks.push_back({l, tk.start, mid, tk.depth + 1});
+    };
+    if (cnt <= 1 || tk.depth >= kMaxDepth) { leaf_or_type_split(); continue; }
     // binned SAH over the three axes
     float best_cost = std::numeric_limits<float>::infinity();
     int best_axis = -1, best_split = -1;
@@ -136,7 +155,7 @@ void build_bvh(const std::vector<PrimInput>& prims, BvhOutput& out) {
       }
     }
     const float leaf_cost = kCostIsect * float(cnt);
-    if (cnt <= uint32_t(kMaxLeaf) && (best_axis < 0 || leaf_cost <= best_cost)) { make_leaf(); continue; }
+    if (cnt <= uint32_t(kMaxLeaf) && (best_axis < 0 || leaf_cost <= best_cost)) { leaf_or_type_split(); continue; }
     uint32_t mid;
     if (best_axis < 0) {
       // degenerate centroids: split the range in half
@@ -200,7 +219,7 @@ void build_bvh(const std::vector<PrimInput>& prims, BvhOutput& out) {
     }
     put_box(f, side, leaf.box);
     put_int(f, side, first);
-    put_int(f, 2 + side, int32_t(leaf.count));
+    put_int(f, 2 + side, int32_t(leaf.count | (prims[idx[leaf.start]].type << kLeafTypeShift)));
     out.max_leaf = std::max(out.max_leaf, leaf.count);
   };
   if (root_leaf) {

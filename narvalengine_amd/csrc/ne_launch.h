@@ -21,7 +21,64 @@ struct RenderArgs {
 };
 
 hipError_t launch_render(const DevScene& s, const DevCamera& cam, const RenderArgs& a, int precision, bool count,
-                         int block, hipStream_t st);
+                         int block, int occupancy, hipStream_t st);
+
+// ---- wavefront pipeline ---------------------------------------------------
+// The path pool is split into chunks of kChunk slots; stage kernels run one
+// block of kChunk threads per chunk and keep each chunk's live paths packed.
+constexpr uint32_t kChunk = 256;
+enum { WF_KIND_MEDIUM = 0, WF_KIND_SURFACE = 1, WF_KIND_MISS = 2 };
+constexpr uint32_t kNoHit = 0xFFFFFFFFu;
+constexpr uint32_t kPrevDelta = 1u << 16;  // depth word: set while the last vertex was the camera
+
+// One path buffer, structure of arrays over P slots.
+struct WfPaths {
+  float *ox, *oy, *oz, *dx, *dy, *dz;  // current ray
+  float *bx, *by, *bz;                 // path throughput
+  float *Lx, *Ly, *Lz;                 // radiance gathered so far
+  float* prev_pdf;                     // pdf of the sampled direction (emission MIS)
+  float* hit_t;                        // surface hit / medium collision distance
+  float* pad;
+  uint32_t *pix, *smp, *rngn, *depth;  // pixel, sample, draws consumed, depth | kPrevDelta
+  uint32_t* hit_ref;                   // (prim type << 30) | storage slot
+  uint32_t* kind;                      // WF_KIND_*
+};
+// Shadow-ray queue (per chunk, packed): origin, direction, extent, (beta f) Le,
+// MIS weight, light pdf, target (slot within the chunk of the next buffer, or
+// film pixel | 2^31), RNG key.
+struct WfShadow {
+  float *ox, *oy, *oz, *dx, *dy, *dz, *tmax, *cx, *cy, *cz, *wt, *pl;
+  uint32_t *target, *pix, *smp, *vert;
+};
+struct WfState {
+  WfPaths paths[2];
+  WfShadow shadow;
+  uint32_t* cnt_paths[2];          // live paths per chunk, per buffer
+  uint32_t* cnt_shadow;            // shadow rays per chunk
+  uint32_t* regen_m;               // new paths per chunk this iteration
+  unsigned long long* regen_base;  // first new path id per chunk
+  uint32_t* status;                // [0]: paths in flight this iteration
+  unsigned long long* next_path;   // work counter
+};
+size_t wf_bytes(uint32_t P);
+void wf_carve(void* base, uint32_t P, WfState& w);
+
+struct WfArgs {
+  int32_t W, H, x0, y0, x1, y1;
+  int32_t max_bounces;
+  uint32_t sample_offset, job, nchunks;
+  uint64_t seed;
+  unsigned long long total_paths;
+  int32_t cur;
+  DevCounters* counters;
+  long long* film_fixed;
+  double* film;
+};
+
+// Renders a.total_paths paths through a pool of a.nchunks * kChunk slots;
+// returns after the film flush is enqueued.  h_status: pinned host word.
+hipError_t launch_wavefront(const DevScene& s, const DevCamera& cam, WfState& w, WfArgs a, bool count,
+                            hipStream_t st, uint32_t* h_status);
 hipError_t launch_reduce_partials(const RenderArgs& a, double* film, hipStream_t st);
 hipError_t launch_tonemap(const double* film, int npix, float inv_spp, float* pixels, hipStream_t st);
 

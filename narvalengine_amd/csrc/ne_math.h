@@ -116,22 +116,46 @@ NE_HD void philox10(uint32_t c[4], uint32_t k0, uint32_t k1) {
 // Per-path uniform stream.  Counter words: (pixel, sample, block, job);
 // key: the 64-bit seed.  One Philox call yields four 32-bit draws; a uniform
 // keeps the top 24 bits: u = (x >> 8) * 2^-24 in [0, 1).
+// Block numbers [0, 2^31) carry the path's main stream; the shadow ray of
+// scatter vertex v draws from its own sub-stream starting at block
+// 2^31 + v * 2^20, so shadow rays can be traced in any order or stage.
+constexpr uint32_t kShadowBlockBase = 0x80000000u;
+constexpr int kMaxVertices = 2047;
+
 struct PathRng {
   uint32_t pixel, sample, block, job, k0, k1;
   uint32_t buf[4];
   int idx;
-  NE_HD void init(uint64_t seed, uint32_t px, uint32_t smp, uint32_t jb) {
-    pixel = px; sample = smp; block = 0; job = jb;
+  NE_HD void init(uint64_t seed, uint32_t px, uint32_t smp, uint32_t jb, uint32_t block0 = 0) {
+    pixel = px; sample = smp; block = block0; job = jb;
     k0 = uint32_t(seed); k1 = uint32_t(seed >> 32);
     idx = 4;
   }
-  NE_HD uint32_t next_u32() {
-    if (idx == 4) {
-      buf[0] = pixel; buf[1] = sample; buf[2] = block; buf[3] = job;
-      philox10(buf, k0, k1);
-      block += 1;
-      idx = 0;
+  // Resume after n draws of the stream that starts at block0.
+  NE_HD void seek(uint32_t n, uint32_t block0 = 0) {
+    block = block0 + (n >> 2);
+    idx = 4;
+    if (n & 3u) {
+      refill();
+      idx = int(n & 3u);
     }
+  }
+  // Draws consumed so far (main stream).
+  NE_HD uint32_t consumed() const { return (block << 2) - (idx == 4 ? 0u : uint32_t(4 - idx)); }
+  NE_HD PathRng shadow_stream(int vertex) const {
+    PathRng r = *this;
+    r.block = kShadowBlockBase + (uint32_t(vertex) << 20);
+    r.idx = 4;
+    return r;
+  }
+  NE_HD void refill() {
+    buf[0] = pixel; buf[1] = sample; buf[2] = block; buf[3] = job;
+    philox10(buf, k0, k1);
+    block += 1;
+    idx = 0;
+  }
+  NE_HD uint32_t next_u32() {
+    if (idx == 4) refill();
     // branch-free select keeps buf in registers
     uint32_t r = idx == 0 ? buf[0] : (idx == 1 ? buf[1] : (idx == 2 ? buf[2] : buf[3]));
     idx += 1;

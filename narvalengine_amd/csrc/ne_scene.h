@@ -5,14 +5,16 @@
 //   nodes   BVH2, 64 B per node = 4 x float4:
 //             f0 = (lmin.xyz, lmax.x)   f1 = (lmax.yz, rmin.xy)
 //             f2 = (rmin.z, rmax.xyz)   f3 = int(left, right, lcount, rcount)
-//           a child with count > 0 is a leaf over refs[index, index+count);
-//           count == 0 is an inner node index; index < 0 marks an empty slot.
-//   refs    one uint32 per leaf slot: (type << 30) | storage index
-//           (type 0 triangle, 1 sphere, 2 rectangle).
+//           a child whose count word is 0 is an inner node index; otherwise
+//           it is a leaf: count = word & 0x0FFFFFFF primitives of type
+//           word >> 28 (0 triangle, 1 sphere, 2 rectangle) at slots
+//           [index, index + count) of that type's array.  index < 0 marks an
+//           empty slot.
 //   tris    48 B per triangle in leaf order: (v0, material), (v1-v0, original
 //           index), (v2-v0, 0).  Original index = position in the input.
-//   sph     (centre, radius) + material array; rect 4 x float4:
-//           (p0, material), (eu, area), (ev, 0), (n, 0).
+//   sph     (centre, radius) in leaf order + material and original-index
+//           arrays; rect 4 x float4 in leaf order:
+//           (p0, material), (eu, area), (ev, original index), (n, 0).
 //   mats    12 floats per material (see ne_gpu.h).
 //   grid    R32F x-fastest density; maj: mres^3 floats, sigma_t majorant per
 //           cell (grid value max over the cell's trilinear support x sigma_bar).
@@ -31,10 +33,10 @@ constexpr int kSpillStack = 48;
 
 struct DevScene {
   const float4* nodes;
-  const uint32_t* refs;
   const float4* tris;
   const float4* sph;
   const int32_t* sph_mat;
+  const int32_t* sph_orig;
   const float4* rects;
   const float* mats;
   const int32_t* light_rects;

@@ -179,11 +179,15 @@ struct SurfHit {
 };
 
 
+// Closest (or, with ANY, first) surface hit in (tmin, tmax).  Nodes are
+// visited near child first; the far child goes on the lane's stack.  Leaves
+// are tested as soon as their box is hit.  Ties on t resolve to the lower
+// (type, original index) key, as the oracle's linear scan does.
 template <typename R, bool ANY, bool C>
 NE_DEV bool traverse(const DevScene& s, Vec3<R> o, Vec3<R> d, R tmin, R tmax, uint32_t* lds, int lane_stride,
                      SurfHit<R>& hit, Count<C>& cnt) {
   if (s.n_nodes == 0) return false;
-  Vec3<R> invd = v3(R(1) / d.x, R(1) / d.y, R(1) / d.z);
+  const Vec3<R> invd = v3(R(1) / d.x, R(1) / d.y, R(1) / d.z);
   uint32_t spill[kSpillStack];
   int sp = 0;
   uint32_t node = 0;
@@ -193,40 +197,40 @@ NE_DEV bool traverse(const DevScene& s, Vec3<R> o, Vec3<R> d, R tmin, R tmax, ui
   uint32_t bkey = 0, bslot = 0;
   while (true) {
     const float4* nd = s.nodes + 4 * size_t(node);
-    float4 f0 = nd[0], f1 = nd[1], f2 = nd[2];
-    int4 f3 = reinterpret_cast<const int4*>(nd)[3];
+    const float4 f0 = nd[0], f1 = nd[1], f2 = nd[2];
+    const int4 f3 = reinterpret_cast<const int4*>(nd)[3];
     cnt.node();
     R tl, tr;
     bool hl = f3.x >= 0 && slab<R>(f0.x, f0.y, f0.z, f0.w, f1.x, f1.y, o, invd, best, tl);
     bool hr = f3.y >= 0 && slab<R>(f1.z, f1.w, f2.x, f2.y, f2.z, f2.w, o, invd, best, tr);
 #pragma unroll
     for (int side = 0; side < 2; ++side) {
-      bool h = side == 0 ? hl : hr;
-      int cntp = side == 0 ? f3.z : f3.w;
-      if (!(h && cntp > 0)) continue;
-      int first = side == 0 ? f3.x : f3.y;
-      for (int i = first; i < first + cntp; ++i) {
-        uint32_t ref = s.refs[i];
-        uint32_t ty = ref >> 30, idx = ref & 0x3fffffffu;
+      const bool h = side == 0 ? hl : hr;
+      const uint32_t ct = uint32_t(side == 0 ? f3.z : f3.w);
+      if (!(h && ct != 0u)) continue;
+      const uint32_t ty = ct >> 28, n = ct & 0x0FFFFFFFu;
+      const uint32_t first = uint32_t(side == 0 ? f3.x : f3.y);
+      for (uint32_t k = first; k < first + n; ++k) {
         R t;
         bool ok;
-        uint32_t key;
+        uint32_t key = 0;
         cnt.prim();
         if (ty == PRIM_TRI) {
-          const float4* tp = s.tris + 3 * size_t(idx);
+          const float4* tp = s.tris + 3 * size_t(k);
           ok = isect_tri<R>(tp, o, d, tmin, tmax, t);
-          key = ok ? __float_as_uint(tp[1].w) : 0u;
+          if (ok) key = __float_as_uint(tp[1].w);
         } else if (ty == PRIM_SPHERE) {
-          ok = isect_sphere<R>(s.sph[idx], o, d, tmin, tmax, t);
-          key = idx;
+          ok = isect_sphere<R>(s.sph[k], o, d, tmin, tmax, t);
+          if (ok) key = uint32_t(s.sph_orig[k]);
         } else {
-          ok = isect_rect<R>(s.rects + 4 * size_t(idx), o, d, tmin, tmax, t);
-          key = idx;
+          const float4* rc = s.rects + 4 * size_t(k);
+          ok = isect_rect<R>(rc, o, d, tmin, tmax, t);
+          if (ok) key = __float_as_uint(rc[2].w);
         }
         if (ok) {
           if (ANY) return true;
           if (t < best || (found && t == best && (int(ty) < btype || (int(ty) == btype && key < bkey)))) {
-            best = t; found = true; btype = int(ty); bkey = key; bslot = idx;
+            best = t; found = true; btype = int(ty); bkey = key; bslot = k;
           }
         }
       }
@@ -235,8 +239,8 @@ NE_DEV bool traverse(const DevScene& s, Vec3<R> o, Vec3<R> d, R tmin, R tmax, ui
     if (hl && tl > best) hl = false;
     if (hr && tr > best) hr = false;
     if (hl && hr) {
-      uint32_t near = tl <= tr ? uint32_t(f3.x) : uint32_t(f3.y);
-      uint32_t far = tl <= tr ? uint32_t(f3.y) : uint32_t(f3.x);
+      const uint32_t near = tl <= tr ? uint32_t(f3.x) : uint32_t(f3.y);
+      const uint32_t far = tl <= tr ? uint32_t(f3.y) : uint32_t(f3.x);
       if (sp < kShortStack) lds[sp * lane_stride] = far;
       else spill[sp - kShortStack] = far;
       ++sp;
@@ -694,7 +698,8 @@ NE_DEV Vec3<R> trace_path(const DevScene& s, Vec3<R> o, Vec3<R> d, int max_bounc
         }
         if (!is_black(f)) {
           R tmax = xisinf(dist) ? xinf<R>() : dist * R(1 - 1e-4);
-          R tr = transmittance<R, C>(s, x, wi, tmax, rng, lds, stride, cnt);
+          PathRng srng = rng.shadow_stream(depth);
+          R tr = transmittance<R, C>(s, x, wi, tmax, srng, lds, stride, cnt);
           if (tr > R(0)) {
             R w = delta ? R(1) : power_heuristic(pl, pf);
             L = L + beta * f * Le * (tr * w * nl / pl);

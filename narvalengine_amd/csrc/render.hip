@@ -25,8 +25,9 @@ __device__ __forceinline__ uint32_t xcd_logical_block(uint32_t nreal) {
   return lb;
 }
 
-template <typename R, bool C, int BLOCK>
-__global__ __launch_bounds__(BLOCK) void k_render(DevScene s, DevCamera cam, RenderArgs a, uint32_t nreal_blocks) {
+template <typename R, bool C, int BLOCK, int WAVES = 0>
+__global__ __launch_bounds__(BLOCK, WAVES) void k_render(DevScene s, DevCamera cam, RenderArgs a,
+                                                          uint32_t nreal_blocks) {
   __shared__ uint32_t lds_stack[kShortStack * BLOCK];
   uint32_t lb = xcd_logical_block(nreal_blocks);
   if (lb >= nreal_blocks) return;
@@ -102,16 +103,18 @@ __global__ void k_tonemap(const double* film, int n, float inv_spp, float* pixel
 
 static inline uint32_t pad8(uint32_t n) { return (n + kNumXcd - 1) / kNumXcd * kNumXcd; }
 
-template <typename R, bool C, int BLOCK>
+template <typename R, bool C, int BLOCK, int WAVES = 0>
 static hipError_t launch_render_t(const DevScene& s, const DevCamera& cam, const RenderArgs& a, hipStream_t st) {
   uint32_t waves = uint32_t(a.ntx) * uint32_t(a.nty) * uint32_t(a.nchunks);
   uint32_t nreal = (waves + (BLOCK / 64) - 1) / (BLOCK / 64);
-  hipLaunchKernelGGL((k_render<R, C, BLOCK>), dim3(pad8(nreal)), dim3(BLOCK), 0, st, s, cam, a, nreal);
+  hipLaunchKernelGGL((k_render<R, C, BLOCK, WAVES>), dim3(pad8(nreal)), dim3(BLOCK), 0, st, s, cam, a, nreal);
   return hipGetLastError();
 }
 
 hipError_t launch_render(const DevScene& s, const DevCamera& cam, const RenderArgs& a, int precision, bool count,
-                         int block, hipStream_t st) {
+                         int block, int occupancy, hipStream_t st) {
+  if (precision == 0 && !count && occupancy == 4) return launch_render_t<float, false, 128, 4>(s, cam, a, st);
+  if (precision == 0 && !count && occupancy == 6) return launch_render_t<float, false, 128, 6>(s, cam, a, st);
 #define NE_DISPATCH(BLK)                                                                       \
   if (precision == 1) {                                                                        \
     return count ? launch_render_t<double, true, BLK>(s, cam, a, st)                           \

@@ -1,0 +1,630 @@
+// wavefront.hip -- wavefront path tracer for gfx950 (FP32 product path).
+//
+// Path state lives in HBM as two structure-of-arrays buffers A and B of P
+// slots, split into chunks of 256 slots.  Every stage kernel runs one
+// 256-thread block per chunk, thread t owning slot t of its chunk, and the
+// live paths of a chunk are kept packed at its front, so all state accesses
+// are coalesced.  One iteration:
+//
+//   plan      one block scans the chunks' free slots and hands each chunk a
+//             contiguous range of new path ids (no atomics: the assignment is
+//             a deterministic prefix sum)
+//   regen     fills the free tail of each chunk with camera rays for its ids
+//             (ids -> pixels in 8x8 tiles: neighbouring slots trace
+//             neighbouring pixels)
+//   trace     BVH closest hit, LDS short stack            (scenes with surfaces)
+//   medium    delta tracking through the majorant grid    (scenes with a volume)
+//   shade     emission / escape / depth limit, then the phase-function or
+//             glossy vertex: NEE light sample -> shadow queue of the chunk,
+//             continuation -> chunk of B (block-local compaction in LDS),
+//             ended paths -> film
+//   shadow    any-hit occlusion and/or ratio tracking; the NEE contribution
+//             goes to the continuing path in B or to the film
+//   swap A/B
+//
+// Same-address global atomics serialise across the XCDs (about 11 ns each on
+// MI355X), so nothing here appends to a shared queue: compaction is a
+// per-block prefix sum and the only global atomics are film additions, spread
+// over pixels.  The film is accumulated in 64-bit fixed point (2^-30 units):
+// the sum does not depend on completion order.  Random numbers follow the
+// megakernel's per-path Philox stream (each path carries the number of draws
+// consumed) and shadow rays use their vertex's sub-stream, so both modes trace
+// the same paths with the same arithmetic.
+#include <cstring>
+
+#include "ne_launch.h"
+#include "ne_trace.h"
+
+namespace ne {
+
+constexpr float kFixedScale = 1073741824.0f;  // 2^30 film units per radiance unit
+constexpr uint32_t kToFilm = 0x80000000u;      // shadow target flag: add to film pixel
+constexpr int kPlanThreads = 1024;
+
+__device__ __forceinline__ void film_add(long long* film, uint32_t pix, float r, float g, float b) {
+  long long* f = film + size_t(pix) * 3;
+  if (r != 0.0f) atomicAdd((unsigned long long*)&f[0], (unsigned long long)__float2ll_rn(r * kFixedScale));
+  if (g != 0.0f) atomicAdd((unsigned long long*)&f[1], (unsigned long long)__float2ll_rn(g * kFixedScale));
+  if (b != 0.0f) atomicAdd((unsigned long long*)&f[2], (unsigned long long)__float2ll_rn(b * kFixedScale));
+}
+
+// Position of this thread's item among the block's selected items (in thread
+// order) and the block total.  Every thread of the block must call it.
+__device__ __forceinline__ uint32_t block_compact(bool pred, uint32_t* wsum, uint32_t& total) {
+  const unsigned long long mask = __ballot(pred);
+  const int lane = __lane_id(), wave = int(threadIdx.x >> 6);
+  if (lane == 0) wsum[wave] = uint32_t(__popcll(mask));
+  __syncthreads();
+  uint32_t before = 0, all = 0;
+  for (int k = 0; k < int(kChunk / 64); ++k) {
+    before += k < wave ? wsum[k] : 0u;
+    all += wsum[k];
+  }
+  __syncthreads();  // wsum is reused by the next call
+  total = all;
+  return before + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
+}
+
+// Sums a per-thread counter over the block and adds it once to *dst.
+template <bool C>
+__device__ __forceinline__ void block_count(unsigned long long* dst, uint32_t v, uint32_t* red) {
+  if constexpr (C) {
+    unsigned long long s = v;
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_down(s, o);
+    if (__lane_id() == 0) red[threadIdx.x >> 6] = uint32_t(s);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      unsigned long long t = 0;
+      for (int k = 0; k < int(kChunk / 64); ++k) t += red[k];
+      if (t) atomicAdd(dst, t);
+    }
+    __syncthreads();
+  }
+}
+
+__global__ void k_wf_init(WfState w, uint32_t nchunks) {
+  const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < nchunks) w.cnt_paths[0][c] = 0;
+  if (c == 0) *w.next_path = 0;
+}
+
+// path id -> (pixel, sample): one sample index at a time over the rect; full
+// 8x8 tiles first (row-major tile order), then the right strip, then the
+// bottom strip (row-major): a bijection onto the rect.
+__device__ __forceinline__ void path_pixel(const WfArgs& a, unsigned long long id, uint32_t& pix, uint32_t& smp) {
+  const uint32_t rw = uint32_t(a.x1 - a.x0), rh = uint32_t(a.y1 - a.y0);
+  const unsigned long long npix = (unsigned long long)rw * rh;
+  smp = uint32_t(id / npix);
+  const uint32_t p = uint32_t(id % npix);
+  const uint32_t full_rows = rh / 8, full_cols = rw / 8;
+  const uint32_t tiles_full = full_rows * full_cols;
+  uint32_t px, py;
+  if (p < tiles_full * 64) {
+    const uint32_t t = p / 64, l = p % 64;
+    px = (t % full_cols) * 8 + (l & 7);
+    py = (t / full_cols) * 8 + (l >> 3);
+  } else {
+    uint32_t r = p - tiles_full * 64;
+    const uint32_t strip_w = rw - full_cols * 8;
+    const uint32_t right = strip_w * full_rows * 8;
+    if (r < right) {
+      px = full_cols * 8 + r % strip_w;
+      py = r / strip_w;
+    } else {
+      r -= right;
+      px = r % rw;
+      py = full_rows * 8 + r / rw;
+    }
+  }
+  pix = uint32_t(a.y0 + int(py)) * uint32_t(a.W) + uint32_t(a.x0 + int(px));
+}
+
+// One block: exclusive scan of the chunks' free slots -> each chunk's first new
+// path id and count; advances the work counter; writes the number of paths in
+// flight this iteration to status[0].
+__global__ __launch_bounds__(kPlanThreads) void k_wf_plan(WfState w, WfArgs a) {
+  __shared__ unsigned long long part[kPlanThreads];
+  const uint32_t NC = a.nchunks;
+  const uint32_t* cntA = w.cnt_paths[a.cur];
+  const uint32_t per = (NC + kPlanThreads - 1) / kPlanThreads;
+  const uint32_t c0 = threadIdx.x * per, c1 = c0 + per < NC ? c0 + per : NC;
+  unsigned long long fsum = 0, live = 0;
+  for (uint32_t c = c0; c < c1; ++c) {
+    fsum += kChunk - cntA[c];
+    live += cntA[c];
+  }
+  part[threadIdx.x] = fsum;
+  __syncthreads();
+  // inclusive Hillis-Steele scan of the per-thread free counts
+  for (int o = 1; o < kPlanThreads; o <<= 1) {
+    unsigned long long v = threadIdx.x >= uint32_t(o) ? part[threadIdx.x - o] : 0ull;
+    __syncthreads();
+    part[threadIdx.x] += v;
+    __syncthreads();
+  }
+  const unsigned long long total_free = part[kPlanThreads - 1];
+  const unsigned long long base = *w.next_path;
+  unsigned long long off = base + part[threadIdx.x] - fsum;
+  for (uint32_t c = c0; c < c1; ++c) {
+    const uint32_t free_c = kChunk - cntA[c];
+    const unsigned long long left = off < a.total_paths ? a.total_paths - off : 0ull;
+    w.regen_base[c] = off;
+    w.regen_m[c] = uint32_t(left < free_c ? left : free_c);
+    off += free_c;
+  }
+  // live-path total (block reduction reuses part[] after the scan)
+  __syncthreads();
+  part[threadIdx.x] = live;
+  __syncthreads();
+  for (int o = kPlanThreads / 2; o > 0; o >>= 1) {
+    if (threadIdx.x < uint32_t(o)) part[threadIdx.x] += part[threadIdx.x + o];
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    const unsigned long long left = base < a.total_paths ? a.total_paths - base : 0ull;
+    const unsigned long long m = left < total_free ? left : total_free;
+    *w.next_path = base + m;
+    const unsigned long long inflight = part[0] + m;
+    w.status[0] = uint32_t(inflight < 0xFFFFFFFFull ? inflight : 0xFFFFFFFFull);
+  }
+}
+
+// New paths fill the free tail of each chunk.
+__global__ __launch_bounds__(kChunk) void k_wf_regen(WfState w, WfArgs a, DevCamera cam) {
+  const WfPaths A = w.paths[a.cur];
+  const uint32_t c = blockIdx.x, t = threadIdx.x;
+  const uint32_t n0 = w.cnt_paths[a.cur][c], m = w.regen_m[c];
+  if (t < m) {
+    const uint32_t slot = c * kChunk + n0 + t;
+    uint32_t pix, smp;
+    path_pixel(a, w.regen_base[c] + t, pix, smp);
+    smp += a.sample_offset;
+    PathRng rng;
+    rng.init(a.seed, pix, smp, a.job);
+    const float ju = rng.next(), jv = rng.next();
+    const uint32_t px = pix % uint32_t(a.W), py = pix / uint32_t(a.W);
+    const float u = (float(px) + ju) / float(a.W), v = (float(py) + jv) / float(a.H);
+    const Vec3<float> pos = v3f<float>(cam.pos), ll = v3f<float>(cam.lower_left);
+    const Vec3<float> hz = v3f<float>(cam.horizontal), vt = v3f<float>(cam.vertical);
+    const Vec3<float> dir = -normalize(ll + u * hz + v * vt - pos);
+    A.ox[slot] = pos.x; A.oy[slot] = pos.y; A.oz[slot] = pos.z;
+    A.dx[slot] = dir.x; A.dy[slot] = dir.y; A.dz[slot] = dir.z;
+    A.bx[slot] = 1.0f; A.by[slot] = 1.0f; A.bz[slot] = 1.0f;
+    A.Lx[slot] = 0.0f; A.Ly[slot] = 0.0f; A.Lz[slot] = 0.0f;
+    A.pix[slot] = pix; A.smp[slot] = smp;
+    A.rngn[slot] = rng.consumed();
+    A.depth[slot] = kPrevDelta;  // depth 0, camera ray (delta)
+    A.prev_pdf[slot] = 0.0f;
+  }
+  __syncthreads();  // every thread has read n0
+  if (t == 0) w.cnt_paths[a.cur][c] = n0 + m;
+}
+
+// Closest surface hit of every live ray: pure BVH traversal (no random
+// numbers, few live registers).
+template <bool C>
+__global__ __launch_bounds__(kChunk) void k_wf_trace(WfState w, WfArgs a, DevScene s) {
+  __shared__ uint32_t lds_stack[kShortStack * kChunk];
+  __shared__ uint32_t red[kChunk / 64];
+  const WfPaths A = w.paths[a.cur];
+  const uint32_t c = blockIdx.x, t = threadIdx.x, i = c * kChunk + t;
+  Count<C> cnt;
+  if (t < w.cnt_paths[a.cur][c]) {
+    const Vec3<float> o = v3(A.ox[i], A.oy[i], A.oz[i]), d = v3(A.dx[i], A.dy[i], A.dz[i]);
+    cnt.ext();
+    SurfHit<float> hit;
+    const bool hs = traverse<float, false, C>(s, o, d, 1e-4f, xinf<float>(), lds_stack + t, kChunk, hit, cnt);
+    A.hit_t[i] = hs ? hit.t : xinf<float>();
+    A.hit_ref[i] = hs ? ((uint32_t(hit.type) << 30) | hit.slot) : kNoHit;
+    A.kind[i] = hs ? uint32_t(WF_KIND_SURFACE) : uint32_t(WF_KIND_MISS);
+  }
+  if constexpr (C) {
+    block_count<C>(&a.counters->ext_rays, cnt.n_ext, red);
+    block_count<C>(&a.counters->bvh_nodes, cnt.n_node, red);
+    block_count<C>(&a.counters->prim_tests, cnt.n_prim, red);
+  }
+}
+
+// Delta tracking through the volume up to the surface hit (or infinity).
+// HAS_SURF: the trace stage ran and set kind / hit_t.
+template <bool C, bool HAS_SURF>
+__global__ __launch_bounds__(kChunk) void k_wf_medium(WfState w, WfArgs a, DevScene s) {
+  __shared__ uint32_t red[kChunk / 64];
+  const WfPaths A = w.paths[a.cur];
+  const uint32_t c = blockIdx.x, t = threadIdx.x, i = c * kChunk + t;
+  Count<C> cnt;
+  if (t < w.cnt_paths[a.cur][c]) {
+    const Vec3<float> o = v3(A.ox[i], A.oy[i], A.oz[i]), d = v3(A.dx[i], A.dy[i], A.dz[i]);
+    const float tS = HAS_SURF ? A.hit_t[i] : xinf<float>();
+    if (!HAS_SURF) cnt.ext();
+    bool medium = false;
+    float th = 0.0f;
+    const Vec3<float> oo = xf_point<float>(s.inv, o), dd = xf_vec<float>(s.inv, d);
+    float ta, tb;
+    if (box_interval<float>(s, oo, dd, ta, tb)) {
+      ta = xmax(ta, 0.0f);
+      tb = xmin(tb, tS);
+      if (ta < tb) {
+        PathRng rng;
+        rng.init(a.seed, A.pix[i], A.smp[i], a.job);
+        rng.seek(A.rngn[i]);
+        medium = delta_track<float, C>(s, o, d, ta, tb, rng, cnt, th);
+        A.rngn[i] = rng.consumed();
+      }
+    }
+    if (medium) {
+      A.hit_t[i] = th;
+      A.kind[i] = WF_KIND_MEDIUM;
+    } else if (!HAS_SURF) {
+      A.kind[i] = WF_KIND_MISS;
+    }
+  }
+  if constexpr (C) {
+    if (!HAS_SURF) block_count<C>(&a.counters->ext_rays, cnt.n_ext, red);
+    block_count<C>(&a.counters->density_lookups, cnt.n_lookup, red);
+    block_count<C>(&a.counters->majorant_cells, cnt.n_cell, red);
+  }
+}
+
+// Scenes with neither surfaces nor a volume: every ray escapes.
+__global__ __launch_bounds__(kChunk) void k_wf_all_miss(WfState w, WfArgs a) {
+  const uint32_t c = blockIdx.x, t = threadIdx.x;
+  if (t < w.cnt_paths[a.cur][c]) w.paths[a.cur].kind[c * kChunk + t] = WF_KIND_MISS;
+}
+
+// NEE light sample at x (trace_path's light-selection block).  Returns false
+// when no shadow ray is needed.
+template <bool MEDIUM>
+__device__ __forceinline__ bool nee_sample(const DevScene& s, Vec3<float> x, Vec3<float> d, const Glossy<float>& gm,
+                                           Vec3<float> wo, Vec3<float> ns, Vec3<float> ss, Vec3<float> ts, float ul,
+                                           float ua, float ub, Vec3<float>& wi, float& tmax, Vec3<float>& f,
+                                           Vec3<float>& Le, float& wt, float& pl) {
+  if (s.n_lights <= 0) return false;
+  const float nl = float(s.n_lights);
+  int li = int(ul * nl);
+  li = li < s.n_lights - 1 ? li : s.n_lights - 1;
+  Le = v3s(0.0f);
+  pl = 0.0f;
+  float dist = xinf<float>();
+  bool delta = false;
+  const int env_slot = s.has_sun ? 1 : 0;
+  if (s.has_sun && li == 0) {
+    wi = -v3f<float>(s.sun_dir);
+    Le = v3f<float>(s.sun_E);
+    pl = 1.0f;
+    delta = true;
+  } else if (s.env_type != 0 && li == env_slot) {
+    const float z = 1.0f - 2.0f * ua, r = sqrtf(xmax(0.0f, 1.0f - z * z)), ph = float(kTWO_PI) * ub;
+    wi = v3(r * cosf(ph), r * sinf(ph), z);
+    Le = env_le<float>(s, wi);
+    pl = float(1.0 / kFOUR_PI);
+  } else {
+    const int ri = s.light_rects[li - env_slot - (s.env_type != 0 ? 1 : 0)];
+    const float4* rc = s.rects + 4 * size_t(ri);
+    const float4 f0 = rc[0], f1 = rc[1], f2 = rc[2], f3 = rc[3];
+    const Vec3<float> p0 = v3f4<float>(f0), eu = v3f4<float>(f1), ev = v3f4<float>(f2), ln = v3f4<float>(f3);
+    const Vec3<float> y = p0 + ua * eu + ub * ev;
+    const Vec3<float> dv = y - x;
+    const float d2 = dot(dv, dv);
+    dist = sqrtf(d2);
+    wi = dv / dist;
+    const float cosl = fabsf(dot(ln, wi));
+    const float area = length(cross(eu, ev));
+    const float* lm = s.mats + 12 * size_t(__float_as_int(f0.w));
+    Le = v3f<float>(lm + 6);
+    pl = cosl > 0.0f ? d2 / (cosl * area) : 0.0f;
+  }
+  if (!(pl > 0.0f)) return false;
+  float pf;
+  if (MEDIUM) {
+    const float p = hg_p(s.g, dot(d, wi));
+    f = v3s(p);
+    pf = p;
+  } else {
+    const Vec3<float> wol = to_lcs(wo, ns, ss, ts), wil = to_lcs(wi, ns, ss, ts);
+    f = gm.eval(wol, wil) * fabsf(wil.z);
+    pf = gm.pdf(wol, wil);
+  }
+  if (is_black(f)) return false;
+  tmax = xisinf(dist) ? xinf<float>() : dist * float(1 - 1e-4);
+  wt = delta ? 1.0f : power_heuristic(pl, pf);
+  return true;
+}
+
+__global__ __launch_bounds__(kChunk) void k_wf_shade(WfState w, WfArgs a, DevScene s) {
+  __shared__ uint32_t wsum[kChunk / 64];
+  const WfPaths A = w.paths[a.cur], B = w.paths[a.cur ^ 1];
+  const WfShadow S = w.shadow;
+  const uint32_t c = blockIdx.x, t = threadIdx.x, i = c * kChunk + t;
+  const bool active = t < w.cnt_paths[a.cur][c];
+  bool shadow = false, cont = false, ends = false;
+  Vec3<float> x = v3s(0.0f), nd = v3s(0.0f), beta = v3s(0.0f), wi = v3s(0.0f), f = v3s(0.0f), Le = v3s(0.0f);
+  float tmax = 0.0f, wt = 0.0f, pl = 0.0f, ppdf = 0.0f;
+  uint32_t pix = 0, smp = 0, rngn = 0, depth = 0;
+  float lx = 0.0f, ly = 0.0f, lz = 0.0f;
+  if (active) {
+    const uint32_t kind = A.kind[i];
+    const Vec3<float> o = v3(A.ox[i], A.oy[i], A.oz[i]), d = v3(A.dx[i], A.dy[i], A.dz[i]);
+    const uint32_t dep = A.depth[i];
+    depth = dep & 0xFFFFu;
+    const bool prev_delta = (dep & kPrevDelta) != 0;
+    pix = A.pix[i];
+    smp = A.smp[i];
+    lx = A.Lx[i]; ly = A.Ly[i]; lz = A.Lz[i];
+    beta = v3(A.bx[i], A.by[i], A.bz[i]);
+    const float th = A.hit_t[i];
+    bool vertex = false;
+    Glossy<float> gm;
+    Vec3<float> ns, ss, ts;
+    if (kind == WF_KIND_MISS) {
+      if (s.env_type != 0) {
+        const float wm = prev_delta ? 1.0f : power_heuristic(A.prev_pdf[i], float(1.0 / kFOUR_PI));
+        const Vec3<float> E = env_le<float>(s, d);
+        lx += beta.x * E.x * wm; ly += beta.y * E.y * wm; lz += beta.z * E.z * wm;
+      }
+      ends = true;
+    } else if (kind == WF_KIND_SURFACE) {
+      const uint32_t ref = A.hit_ref[i];
+      SurfHit<float> hit;
+      hit.t = th; hit.type = int(ref >> 30); hit.slot = ref & 0x3FFFFFFFu; hit.key = 0;
+      Vec3<float> nrm;
+      int mat = 0;
+      hit_normal<float>(s, hit, o, d, nrm, mat);
+      const float* m = s.mats + 12 * size_t(mat);
+      if (m[0] == 1.0f) {  // emitter: MIS-weighted emission, the path ends
+        const Vec3<float> E = v3f<float>(m + 6);
+        float wm = 1.0f;
+        if (!prev_delta) {
+          const float4* rc = s.rects + 4 * size_t(hit.slot);
+          const float area = rc[1].w;
+          const float cosl = fabsf(dot(nrm, d));
+          const float plh = (th * th) / (cosl * area);
+          wm = power_heuristic(A.prev_pdf[i], plh);
+        }
+        lx += beta.x * E.x * wm; ly += beta.y * E.y * wm; lz += beta.z * E.z * wm;
+        ends = true;
+      } else if (int(depth) >= a.max_bounces) {
+        ends = true;
+      } else {
+        vertex = true;
+        const Vec3<float> wo = -d;
+        gm.albedo = v3f<float>(m + 1);
+        const float rough = m[4];
+        gm.alpha = rough * rough;
+        gm.metallic = m[5];
+        ns = dot(nrm, wo) < 0.0f ? -nrm : nrm;
+        onb(ns, ss, ts);
+      }
+    } else {  // medium collision
+      if (int(depth) >= a.max_bounces) {
+        ends = true;
+      } else {
+        vertex = true;
+        beta = beta * v3(s.sig_s[0] / s.sig_t[0], s.sig_s[1] / s.sig_t[1], s.sig_s[2] / s.sig_t[2]);
+      }
+    }
+    if (vertex) {
+      const bool medium = kind == WF_KIND_MEDIUM;
+      x = o + th * d;
+      const Vec3<float> wo = -d;
+      PathRng rng;
+      rng.init(a.seed, pix, smp, a.job);
+      rng.seek(A.rngn[i]);
+      const float ul = rng.next(), ua = rng.next(), ub = rng.next();
+      shadow = medium ? nee_sample<true>(s, x, d, gm, wo, ns, ss, ts, ul, ua, ub, wi, tmax, f, Le, wt, pl)
+                      : nee_sample<false>(s, x, d, gm, wo, ns, ss, ts, ul, ua, ub, wi, tmax, f, Le, wt, pl);
+      if (shadow) f = beta * f * Le;  // NEE weight with the throughput before this vertex's BSDF weight
+      if (medium) {
+        const float u0 = rng.next(), u1 = rng.next();
+        nd = hg_sample(s.g, d, u0, u1);
+        ppdf = hg_p(s.g, dot(d, nd));
+        cont = true;
+      } else {
+        const float u0 = rng.next(), u1 = rng.next(), u2 = rng.next();
+        const Vec3<float> wol = to_lcs(wo, ns, ss, ts);
+        const Vec3<float> wil = gm.sample(wol, u0, u1, u2);
+        const float pf = gm.pdf(wol, wil);
+        if (pf > 0.0f) {
+          const Vec3<float> fs = gm.eval(wol, wil) * fabsf(wil.z);
+          if (!is_black(fs)) {
+            ppdf = pf;
+            nd = to_world(wil, ns, ss, ts);
+            cont = true;
+            beta = beta * fs / pf;
+          }
+        }
+      }
+      ends = !cont;
+      rngn = rng.consumed();
+    }
+  }
+  uint32_t ncont, nshadow;
+  const uint32_t j = block_compact(cont, wsum, ncont);
+  const uint32_t k = block_compact(shadow, wsum, nshadow);
+  if (cont) {
+    const uint32_t b = c * kChunk + j;
+    B.ox[b] = x.x; B.oy[b] = x.y; B.oz[b] = x.z;
+    B.dx[b] = nd.x; B.dy[b] = nd.y; B.dz[b] = nd.z;
+    B.bx[b] = beta.x; B.by[b] = beta.y; B.bz[b] = beta.z;
+    B.Lx[b] = lx; B.Ly[b] = ly; B.Lz[b] = lz;
+    B.pix[b] = pix; B.smp[b] = smp; B.rngn[b] = rngn;
+    B.depth[b] = depth + 1u;  // prev_delta cleared
+    B.prev_pdf[b] = ppdf;
+  } else if (ends) {
+    film_add(a.film_fixed, pix, lx, ly, lz);
+  }
+  if (shadow) {
+    const uint32_t q = c * kChunk + k;
+    S.ox[q] = x.x; S.oy[q] = x.y; S.oz[q] = x.z;
+    S.dx[q] = wi.x; S.dy[q] = wi.y; S.dz[q] = wi.z;
+    S.tmax[q] = tmax;
+    S.cx[q] = f.x; S.cy[q] = f.y; S.cz[q] = f.z;
+    S.wt[q] = wt; S.pl[q] = pl;
+    S.target[q] = cont ? j : (kToFilm | pix);
+    S.pix[q] = pix; S.smp[q] = smp; S.vert[q] = depth;
+  }
+  if (t == 0) {
+    w.cnt_paths[a.cur ^ 1][c] = ncont;
+    w.cnt_shadow[c] = nshadow;
+  }
+  if (a.counters) {
+    uint32_t nend;
+    (void)block_compact(ends && !cont, wsum, nend);
+    if (t == 0 && nend) atomicAdd(&a.counters->paths, (unsigned long long)nend);
+  }
+}
+
+// Shadow rays: any-hit occlusion (HAS_SURF) then ratio tracking (HAS_VOL),
+// the megakernel's transmittance() split by scene content.
+template <bool C, bool HAS_SURF, bool HAS_VOL>
+__global__ __launch_bounds__(kChunk) void k_wf_shadow(WfState w, WfArgs a, DevScene s) {
+  __shared__ uint32_t lds_stack[HAS_SURF ? kShortStack * kChunk : 1];
+  __shared__ uint32_t red[kChunk / 64];
+  const WfShadow S = w.shadow;
+  const WfPaths B = w.paths[a.cur ^ 1];
+  const uint32_t c = blockIdx.x, t = threadIdx.x, k = c * kChunk + t;
+  Count<C> cnt;
+  if (t < w.cnt_shadow[c]) {
+    PathRng rng;
+    rng.init(a.seed, S.pix[k], S.smp[k], a.job);
+    PathRng srng = rng.shadow_stream(int(S.vert[k]));
+    const Vec3<float> x = v3(S.ox[k], S.oy[k], S.oz[k]), wi = v3(S.dx[k], S.dy[k], S.dz[k]);
+    const float tmax = S.tmax[k];
+    cnt.shadow();
+    float tr = 1.0f;
+    if (HAS_SURF) {
+      SurfHit<float> h;
+      if (traverse<float, true, C>(s, x, wi, 1e-4f, tmax, lds_stack + t, kChunk, h, cnt)) tr = 0.0f;
+    }
+    if (HAS_VOL && tr > 0.0f) {
+      const Vec3<float> oo = xf_point<float>(s.inv, x), dd = xf_vec<float>(s.inv, wi);
+      float ta, tb;
+      if (box_interval<float>(s, oo, dd, ta, tb)) {
+        ta = xmax(ta, 0.0f);
+        tb = xmin(tb, tmax);
+        if (ta < tb) tr = ratio_track<float, C>(s, x, wi, ta, tb, srng, cnt);
+      }
+    }
+    if (tr > 0.0f) {
+      const float q = tr * S.wt[k] * float(s.n_lights) / S.pl[k];
+      const float cx = S.cx[k] * q, cy = S.cy[k] * q, cz = S.cz[k] * q;
+      const uint32_t tg = S.target[k];
+      if (tg & kToFilm) {
+        film_add(a.film_fixed, tg & ~kToFilm, cx, cy, cz);
+      } else {
+        const uint32_t b = c * kChunk + tg;
+        B.Lx[b] += cx; B.Ly[b] += cy; B.Lz[b] += cz;
+      }
+    }
+  }
+  if constexpr (C) {
+    block_count<C>(&a.counters->shadow_rays, cnt.n_shadow, red);
+    block_count<C>(&a.counters->density_lookups, cnt.n_lookup, red);
+    block_count<C>(&a.counters->majorant_cells, cnt.n_cell, red);
+    block_count<C>(&a.counters->bvh_nodes, cnt.n_node, red);
+    block_count<C>(&a.counters->prim_tests, cnt.n_prim, red);
+  }
+}
+
+// film += fixed / 2^30 and clear the fixed-point buffer
+__global__ void k_wf_flush_film(long long* fixed, double* film, size_t n) {
+  size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  long long v = fixed[i];
+  if (v != 0) {
+    film[i] += double(v) * (1.0 / 1073741824.0);
+    fixed[i] = 0;
+  }
+}
+
+// ---- pool layout ------------------------------------------------------------
+// A and B path buffers (15 float + 6 uint32 arrays each), the shadow queue
+// (12 float + 4 uint32 arrays), per-chunk counters (A, B, shadow, regen count
+// and 64-bit regen base) and the status / work-counter words, each array
+// 256-byte aligned, in one allocation.
+static constexpr int kPathF = 15, kPathU = 6, kShadowF = 12, kShadowU = 4;
+static inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
+
+size_t wf_bytes(uint32_t P) {
+  const size_t NC = P / kChunk;
+  return size_t(2 * (kPathF + kPathU) + kShadowF + kShadowU) * align256(size_t(P) * 4) + 4 * align256(NC * 4) +
+         align256(NC * 8) + 2 * 256;
+}
+
+void wf_carve(void* base, uint32_t P, WfState& w) {
+  char* p = static_cast<char*>(base);
+  const size_t stride = align256(size_t(P) * 4);
+  const size_t NC = P / kChunk;
+  for (int b = 0; b < 2; ++b) {
+    WfPaths& q = w.paths[b];
+    float** fl[kPathF] = {&q.ox, &q.oy, &q.oz, &q.dx, &q.dy, &q.dz, &q.bx, &q.by,
+                          &q.bz, &q.Lx, &q.Ly, &q.Lz, &q.prev_pdf, &q.hit_t, &q.pad};
+    for (int k = 0; k < kPathF; ++k, p += stride) *fl[k] = reinterpret_cast<float*>(p);
+    uint32_t** us[kPathU] = {&q.pix, &q.smp, &q.rngn, &q.depth, &q.hit_ref, &q.kind};
+    for (int k = 0; k < kPathU; ++k, p += stride) *us[k] = reinterpret_cast<uint32_t*>(p);
+  }
+  WfShadow& S = w.shadow;
+  float** sf[kShadowF] = {&S.ox, &S.oy, &S.oz, &S.dx, &S.dy, &S.dz, &S.tmax, &S.cx, &S.cy, &S.cz, &S.wt, &S.pl};
+  for (int k = 0; k < kShadowF; ++k, p += stride) *sf[k] = reinterpret_cast<float*>(p);
+  uint32_t** su[kShadowU] = {&S.target, &S.pix, &S.smp, &S.vert};
+  for (int k = 0; k < kShadowU; ++k, p += stride) *su[k] = reinterpret_cast<uint32_t*>(p);
+  uint32_t** cs[4] = {&w.cnt_paths[0], &w.cnt_paths[1], &w.cnt_shadow, &w.regen_m};
+  for (int k = 0; k < 4; ++k, p += align256(NC * 4)) *cs[k] = reinterpret_cast<uint32_t*>(p);
+  w.regen_base = reinterpret_cast<unsigned long long*>(p);
+  p += align256(NC * 8);
+  w.status = reinterpret_cast<uint32_t*>(p);
+  p += 256;
+  w.next_path = reinterpret_cast<unsigned long long*>(p);
+}
+
+hipError_t launch_wavefront(const DevScene& s, const DevCamera& cam, WfState& w, WfArgs a, bool count,
+                            hipStream_t st, uint32_t* h_status) {
+  const uint32_t NC = a.nchunks;
+  hipLaunchKernelGGL(k_wf_init, dim3((NC + 255) / 256), dim3(256), 0, st, w, NC);
+  const dim3 grid = dim3(NC);
+  const dim3 blk = dim3(unsigned(kChunk));
+  if (!count) a.counters = nullptr;
+  hipError_t e = hipSuccess;
+  const bool hs = s.n_nodes > 0, hv = s.has_volume != 0;
+  for (uint32_t it = 0;; ++it) {
+    a.cur = int(it & 1u);
+    hipLaunchKernelGGL(k_wf_plan, dim3(1), dim3(unsigned(kPlanThreads)), 0, st, w, a);
+    // every 8 iterations: stop once nothing is in flight (status written by plan)
+    if ((it & 7u) == 7u) {
+      if ((e = hipMemcpyAsync(h_status, w.status, sizeof(uint32_t), hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
+      if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+      if (h_status[0] == 0) break;
+    }
+    hipLaunchKernelGGL(k_wf_regen, grid, blk, 0, st, w, a, cam);
+    if (hs) {
+      if (count) hipLaunchKernelGGL(k_wf_trace<true>, grid, blk, 0, st, w, a, s);
+      else hipLaunchKernelGGL(k_wf_trace<false>, grid, blk, 0, st, w, a, s);
+    }
+    if (hv) {
+      if (count) {
+        if (hs) hipLaunchKernelGGL((k_wf_medium<true, true>), grid, blk, 0, st, w, a, s);
+        else hipLaunchKernelGGL((k_wf_medium<true, false>), grid, blk, 0, st, w, a, s);
+      } else {
+        if (hs) hipLaunchKernelGGL((k_wf_medium<false, true>), grid, blk, 0, st, w, a, s);
+        else hipLaunchKernelGGL((k_wf_medium<false, false>), grid, blk, 0, st, w, a, s);
+      }
+    }
+    if (!hs && !hv) hipLaunchKernelGGL(k_wf_all_miss, grid, blk, 0, st, w, a);
+    hipLaunchKernelGGL(k_wf_shade, grid, blk, 0, st, w, a, s);
+#define NE_SHADOW(CNT, S_, V_) hipLaunchKernelGGL((k_wf_shadow<CNT, S_, V_>), grid, blk, 0, st, w, a, s)
+    if (count) {
+      if (hs && hv) NE_SHADOW(true, true, true); else if (hs) NE_SHADOW(true, true, false);
+      else if (hv) NE_SHADOW(true, false, true); else NE_SHADOW(true, false, false);
+    } else {
+      if (hs && hv) NE_SHADOW(false, true, true); else if (hs) NE_SHADOW(false, true, false);
+      else if (hv) NE_SHADOW(false, false, true); else NE_SHADOW(false, false, false);
+    }
+#undef NE_SHADOW
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+  }
+  const size_t nf = size_t(a.W) * size_t(a.H) * 3;
+  hipLaunchKernelGGL(k_wf_flush_film, dim3(uint32_t((nf + 255) / 256)), dim3(256), 0, st, a.film_fixed, a.film, nf);
+  return hipGetLastError();
+}
+
+}  // namespace ne

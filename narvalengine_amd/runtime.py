@@ -54,7 +54,8 @@ class Context:
 
     def render(self, gscene: "GpuScene", camera, film: "Film", *, spp: int, max_bounces: int, seed: int = 0,
                sample_offset: int = 0, job: int = 0, rect=None, precision: int = NE_FP32, count: bool = False,
-               samples_per_thread: int = 0, block_size: int = 0, sync: bool = True, timed: bool = False):
+               samples_per_thread: int = 0, block_size: int = 0, sync: bool = True, timed: bool = False,
+               occupancy: int = 0, mode: int = 0, pool_size: int = 0):
         p = RenderParams()
         p.width, p.height = film.width, film.height
         x0, y0, x1, y1 = rect if rect is not None else (0, 0, film.width, film.height)
@@ -62,7 +63,8 @@ class Context:
         p.spp, p.max_bounces, p.seed = spp, max_bounces, seed
         p.sample_offset, p.job, p.precision = sample_offset, job, precision
         p.flags = (NE_FLAG_COUNT if count else 0) | (NE_FLAG_TIME if timed else 0)
-        p.samples_per_thread, p.block_size = samples_per_thread, block_size
+        p.samples_per_thread, p.block_size, p.occupancy = samples_per_thread, block_size, occupancy
+        p.mode, p.pool_size = mode, pool_size
         cam = camera.desc()
         self.check(self._lib.ne_render(self.h, gscene.h, C.byref(cam), C.byref(p), film.h))
         if sync:

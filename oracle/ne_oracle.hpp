@@ -121,6 +121,14 @@ struct Rng {
     return buf[idx++];
   }
   float next() { return float(next_u32() >> 8) * (1.0f / 16777216.0f); }
+  // Shadow ray of scatter vertex v: same (pixel, sample, job, key), blocks
+  // from 2^31 + v * 2^20 on, disjoint from the main stream's blocks.
+  Rng shadow_stream(int vertex) const {
+    Rng r = *this;
+    r.ctr[2] = 0x80000000u + (uint32_t(vertex) << 20);
+    r.idx = 4;
+    return r;
+  }
 };
 
 // ---------------------------------------------------------------------------
@@ -1082,7 +1090,8 @@ V3<T> Li(const World<T>& w, V3<T> o, V3<T> d, int max_bounces, Rng& rng, PathSta
         if (!is_black(f)) {
           if (st) st->shadow_rays++;
           T tmax = std::isinf(dist) ? std::numeric_limits<T>::infinity() : dist * T(1 - 1e-4);
-          T tr = w.transmittance(x, wi, tmax, rng);
+          Rng srng = rng.shadow_stream(depth);
+          T tr = w.transmittance(x, wi, tmax, srng);
           if (tr > T(0)) {
             T wgt = delta ? T(1) : power_heuristic(pl, pf);
             L = L + beta * f * Le * (tr * wgt * nl / pl);

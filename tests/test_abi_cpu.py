@@ -120,7 +120,9 @@ def test_bvh_is_well_formed(which):
         assert not visited[k]
         visited[k] = True
         for side in (0, 1):
-            idx, cnt = ints[k, side], ints[k, 2 + side]
+            # leaf count word: count | (primitive type << 28); leaves hold one type
+            idx, word = ints[k, side], int(ints[k, 2 + side]) & 0xFFFFFFFF
+            cnt, leaf_type = word & 0x0FFFFFFF, word >> 28
             lo, hi = boxes[side][0][k], boxes[side][1][k]
             if idx < 0:
                 continue
@@ -128,6 +130,7 @@ def test_bvh_is_well_formed(which):
                 assert 0 <= idx and idx + cnt <= len(refs)
                 slots += cnt
                 for r in refs[idx:idx + cnt]:
+                    assert int(r >> 30) == leaf_type
                     pl, ph = prim_box(r)
                     assert (pl >= lo - 1e-6).all() and (ph <= hi + 1e-6).all()
             else:

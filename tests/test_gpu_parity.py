@@ -367,3 +367,25 @@ def test_handles_release_in_any_order():
     assert np.isfinite(before).all()
     g.close()
     f.close()
+
+
+@pytest.mark.parametrize("which", ["volume", "surface", "cloud"])
+def test_wavefront_matches_megakernel(ctx, which):
+    print_case(f"wavefront vs megakernel {which}")
+    from narvalengine_amd.abi import NE_MODE_MEGAKERNEL, NE_MODE_WAVEFRONT
+    if which == "volume":
+        s, cam, bounces = scenes.small_volume(res=32, maj_res=4), scenes.VOLUME_CAMERA, 16
+    elif which == "cloud":
+        s, cam, bounces = scenes.small_cloud(), scenes.VOLUME_CAMERA, 64
+    else:
+        s, cam, bounces = scenes.small_surface(subdiv=3), scenes.BOX_CAMERA, 8
+    g = ctx.upload(s)
+    films = []
+    for mode, pool in [(NE_MODE_MEGAKERNEL, 0), (NE_MODE_WAVEFRONT, 0), (NE_MODE_WAVEFRONT, 1000)]:
+        f = ctx.film(40, 24)
+        ctx.render(g, cam, f, spp=16, max_bounces=bounces, seed=13, mode=mode, pool_size=pool)
+        films.append(f.read())
+    # same paths, same per-path arithmetic; only the film accumulation differs
+    # (FP64 sums vs 2^-30 fixed point)
+    np.testing.assert_allclose(films[1], films[0], rtol=1e-6, atol=16 * 2.0 ** -30)
+    np.testing.assert_array_equal(films[1], films[2])  # the pool size does not change the result
