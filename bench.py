@@ -7,9 +7,11 @@ resident in HBM.  The default workload is `surface1m` (1024x1024, 256 spp,
 --workload volume128 / cloud512 for the volume scenes.
 
 Prints one JSON line: samples/s (value), rays/s, the dominant kernel's
-HBM roofline fraction (algorithmic bytes from a separate counting run divided
-by its HIP-event-timed duration) and the CPU baseline (the oracle timed on this
-host's cores on a bounded crop of the same workload).
+HBM roofline fraction and the CPU baseline (the oracle timed on this host's
+cores on a bounded crop of the same workload).  Inside the timed steps every
+kernel launch is bracketed by HIP events on the library's stream, which gives
+each stage's average launch duration; the stage's algorithmic bytes come from
+a separate counting run of the same frame (same launches, same paths).
 """
 from __future__ import annotations
 
@@ -24,8 +26,52 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0
-# algorithmic bytes per counted event (units defined in ne_counters)
-BYTES = {"density_lookups": 32, "majorant_cells": 4, "bvh_nodes": 64, "prim_tests": 48}
+PROFILES = ROOT / "profiles"
+
+# Algorithmic bytes per counted event: a trilinear density lookup reads 8 FP32
+# corners, a majorant cell 4 B, a BVH node record 64 B (both child boxes), a
+# primitive record 48 B (triangle: 3 x float4).  Ray-state traffic of the
+# wavefront stages per ray (structure-of-arrays records):
+#   trace  reads origin + direction (24 B), writes hit t / ref / kind (12 B)
+#   medium reads origin, direction, hit t, pixel, sample, RNG count (40 B),
+#          writes hit t, kind, RNG count (12 B)
+#   shadow reads the 64 B queue record, read-modify-writes the 12 B radiance
+#   shade  reads the 84 B path record, writes 64 B per continuation and per
+#          shadow ray
+# The film costs 24 B read + 24 B write per pixel (FP64 RGB), once per frame.
+EV = {"density_lookups": 32, "majorant_cells": 4, "bvh_nodes": 64, "prim_tests": 48}
+RAY_BYTES = {"trace": 36, "medium": 52, "shadow": 88}
+SHADE_IN, SHADE_OUT = 84, 64
+
+
+def stage_bytes(stage: str, c: dict, film_bytes: int) -> int:
+    """Algorithmic bytes a stage moves over one frame, from the event counters."""
+    ext = {k: c[k] - c["shadow_" + k] for k in EV}
+    sh = {k: c["shadow_" + k] for k in EV}
+    if stage == "megakernel":
+        return sum(c[k] * b for k, b in EV.items()) + film_bytes
+    if stage == "trace":
+        return ext["bvh_nodes"] * 64 + ext["prim_tests"] * 48 + c["ext_rays"] * RAY_BYTES["trace"]
+    if stage == "medium":
+        return ext["density_lookups"] * 32 + ext["majorant_cells"] * 4 + c["ext_rays"] * RAY_BYTES["medium"]
+    if stage == "shadow":
+        return sum(sh[k] * b for k, b in EV.items()) + c["shadow_rays"] * RAY_BYTES["shadow"]
+    if stage == "shade":
+        cont = max(c["ext_rays"] - c["paths"], 0)
+        return c["ext_rays"] * SHADE_IN + (cont + c["shadow_rays"]) * SHADE_OUT
+    return 0
+
+
+def measured_traffic(kernel: str, workload: str):
+    """HBM bytes per launch of `kernel` from the committed rocprofv3 PMC summary
+    (FETCH_SIZE + WRITE_SIZE, scaled by the copy-kernel calibration), or None."""
+    f = PROFILES / f"traffic_{kernel}_{workload}.txt"
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        if line.startswith("TRAFFIC_BYTES_PER_LAUNCH="):
+            return int(float(line.split("=", 1)[1]))
+    return None
 
 
 def log(msg):
@@ -61,24 +107,46 @@ def cpu_baseline(wl, budget_s: float):
                       f"not timed)"}
 
 
+def copy_calibration(nbytes: int, reps: int = 5):
+    """Plain device-to-device copies of a known size, for scaling the rocprofv3
+    FETCH_SIZE / WRITE_SIZE counters (each copy reads and writes nbytes)."""
+    import torch
+    a = torch.ones(nbytes // 4, dtype=torch.float32, device="cuda:0")
+    b = torch.empty_like(a)
+    torch.cuda.synchronize()
+    for _ in range(reps):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_calibration_bytes": nbytes, "reps": reps}))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="surface1m", choices=["surface1m", "volume128", "cloud512"])
+    ap.add_argument("--mode", default="auto", choices=["auto", "megakernel", "wavefront"])
+    ap.add_argument("--pool", type=int, default=0, help="wavefront path slots (0 = library default)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-count", action="store_true")
     ap.add_argument("--samples-per-thread", type=int, default=0)
     ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
+                    help="only run plain copies of BYTES (counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the path shards by "
               "pixel tiles / sample ranges with a final FP64 reduce, which is not part of this build", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration(args.copy_calibration)
+        return
 
     from narvalengine_amd import Context, configs
+    from narvalengine_amd.abi import NE_MODE_AUTO, NE_MODE_MEGAKERNEL, NE_MODE_WAVEFRONT
+    mode = {"auto": NE_MODE_AUTO, "megakernel": NE_MODE_MEGAKERNEL, "wavefront": NE_MODE_WAVEFRONT}[args.mode]
     t0 = time.perf_counter()
     wl = configs.CONFIGS[args.workload]()
     log(f"workload {wl.name}: {wl.width}x{wl.height} spp {wl.spp} bounces {wl.max_bounces} "
@@ -89,51 +157,72 @@ def main():
     log(f"scene on device: {info}")
     film = ctx.film(wl.width, wl.height)
     kw = dict(spp=wl.spp, max_bounces=wl.max_bounces, seed=1, samples_per_thread=args.samples_per_thread,
-              block_size=args.block)
+              block_size=args.block, mode=mode, pool_size=args.pool)
     for i in range(args.warmup):
         ctx.render(g, wl.camera, film, sample_offset=(1000 + i) * wl.spp, **kw)
         log(f"warmup {i + 1}/{args.warmup} done")
     ctx.synchronize()
-    kernel_ms = []
+    # timed region: K full frames; every kernel launch is bracketed by HIP
+    # events on the context stream (per-stage durations)
+    stage_ms, stage_n = {}, {}
     t_start = time.perf_counter()
     for k in range(args.steps):
-        ctx.render(g, wl.camera, film, sample_offset=k * wl.spp, sync=False, timed=True, **kw)
-        kernel_ms.append(ctx.kernel_ms())
-        log(f"step {k + 1}/{args.steps}: kernel {kernel_ms[-1]:.1f} ms")
+        ctx.render(g, wl.camera, film, sample_offset=k * wl.spp, sync=False, stage_timed=True, **kw)
+        st = ctx.stage_times()
+        for name, (ms, n) in st.items():
+            stage_ms[name] = stage_ms.get(name, 0.0) + ms
+            stage_n[name] = stage_n.get(name, 0) + n
+        log(f"step {k + 1}/{args.steps}: " + ", ".join(f"{n} {ms:.1f} ms" for n, (ms, c) in st.items() if c))
     ctx.synchronize()
     wall = time.perf_counter() - t_start
     paths_per_step = wl.width * wl.height * wl.spp
     value = args.steps * paths_per_step / wall
-    avg_kernel_s = sum(kernel_ms) / len(kernel_ms) / 1e3
+    step_s = wall / args.steps
 
-    roofline = None
-    rays = None
-    counters = None
+    roofline, rays, counters, stages = None, None, None, None
+    dom = max(stage_ms, key=lambda n: stage_ms[n])
     if not args.no_count:
         cfilm = ctx.film(wl.width, wl.height)
         ctx.render(g, wl.camera, cfilm, count=True, **kw)
         counters = ctx.counters()
         log(f"counters: {counters}")
         rays = counters["ext_rays"] + counters["shadow_rays"]
-        film_bytes = 48 * wl.width * wl.height  # FP64 RGB read + write per pixel
-        algo = sum(counters[k] * b for k, b in BYTES.items()) + film_bytes
-        achieved = algo / avg_kernel_s / 1e9
-        roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                    "kernel": "k_render", "algorithmic_bytes_per_launch": int(algo),
-                    "avg_launch_ms": round(avg_kernel_s * 1e3, 3)}
+        film_bytes = 48 * wl.width * wl.height
+        stages = {}
+        for name in stage_ms:
+            if not stage_n[name]:
+                continue
+            per_launch_ms = stage_ms[name] / stage_n[name]
+            launches_per_frame = stage_n[name] / args.steps
+            b = stage_bytes(name, counters, film_bytes) / launches_per_frame
+            stages[name] = {"ms_per_frame": round(stage_ms[name] / args.steps, 3),
+                            "launches_per_frame": launches_per_frame,
+                            "avg_launch_ms": round(per_launch_ms, 4),
+                            "algorithmic_bytes_per_launch": int(b),
+                            "achieved_GBs": round(b / (per_launch_ms * 1e-3) / 1e9, 2) if b else None}
+        d = stages[dom]
+        kernel = {"megakernel": "k_render", "trace": "k_wf_trace", "medium": "k_wf_medium", "shade": "k_wf_shade",
+                  "shadow": "k_wf_shadow", "plan": "k_wf_plan", "regen": "k_wf_regen"}.get(dom, dom)
+        achieved = d["achieved_GBs"] or 0.0
+        roofline = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 5),
+                    "traffic": measured_traffic(kernel, wl.name),
+                    "kernel": kernel, "algorithmic_bytes_per_launch": d["algorithmic_bytes_per_launch"],
+                    "avg_launch_ms": d["avg_launch_ms"]}
     cpu = None if args.no_cpu else cpu_baseline(wl, args.cpu_seconds)
     out = {
         "metric": "samples/sec", "value": round(value, 1), "unit": "samples/s", "n_gpus": 1,
-        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(step_s * 1e3, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded procedural scene)",
         "config": {"workload": wl.name, "width": wl.width, "height": wl.height, "spp": wl.spp,
-                   "max_bounces": wl.max_bounces, "paths_per_step": paths_per_step, "accumulation": "f64"},
-        "rays_per_sec": round(rays / avg_kernel_s, 1) if rays else None,
+                   "max_bounces": wl.max_bounces, "paths_per_step": paths_per_step, "accumulation": "f64",
+                   "mode": args.mode, "pool": args.pool},
+        "rays_per_sec": round(rays / step_s, 1) if rays else None,
         "roofline": roofline,
         "cpu_baseline": cpu,
         "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
+        "stages": stages,
         "counters_per_step": counters,
     }
     print(json.dumps(out))

@@ -111,8 +111,22 @@ typedef struct ne_camera_desc {
 
 enum { NE_FP32 = 0, NE_FP64 = 1 };
 enum {
-  NE_FLAG_COUNT = 1, /* collect event counters (separate, slower pass) */
-  NE_FLAG_TIME = 2   /* time the path-tracing kernel with HIP events on the context stream */
+  NE_FLAG_COUNT = 1,      /* collect event counters (separate, slower pass) */
+  NE_FLAG_TIME = 2,       /* time the path-tracing kernels with HIP events on the context stream */
+  NE_FLAG_STAGE_TIME = 4  /* also time every kernel launch and sum the times per stage (NE_STAGE_*) */
+};
+
+/* Kernel stages reported by ne_get_stage_times. */
+enum {
+  NE_STAGE_MEGAKERNEL = 0, /* one thread per (pixel, sample chunk), whole paths */
+  NE_STAGE_PLAN = 1,       /* wavefront: hand out new path ids per chunk */
+  NE_STAGE_REGEN = 2,      /* wavefront: camera rays into free slots */
+  NE_STAGE_TRACE = 3,      /* wavefront: BVH closest hit */
+  NE_STAGE_MEDIUM = 4,     /* wavefront: delta tracking */
+  NE_STAGE_SHADE = 5,      /* wavefront: emission, NEE sample, scattering, compaction */
+  NE_STAGE_SHADOW = 6,     /* wavefront: occlusion + ratio tracking */
+  NE_STAGE_OTHER = 7,      /* initialisation, film reduction / flush */
+  NE_STAGE_COUNT = 8
 };
 
 /* One render call: samples [sample_offset, sample_offset + spp) of every pixel
@@ -132,7 +146,7 @@ typedef struct ne_render_params {
   int32_t block_size;         /* 0 = library default */
   int32_t occupancy;          /* megakernel FP32 variant built for >= this many waves per SIMD: 0, 4, 6 */
   int32_t mode;               /* NE_MODE_*: 0 = auto (wavefront for FP32, megakernel for FP64) */
-  int32_t pool_size;          /* wavefront path slots (rounded up to 256); 0 = library default (2^21) */
+  int32_t pool_size;          /* wavefront path slots (rounded up to 256); 0 = library default (2^22) */
 } ne_render_params;
 
 enum { NE_MODE_AUTO = 0, NE_MODE_MEGAKERNEL = 1, NE_MODE_WAVEFRONT = 2 };
@@ -145,7 +159,12 @@ typedef struct ne_counters {
   uint64_t majorant_cells;  /* majorant-grid cells visited (4 B each) */
   uint64_t bvh_nodes;       /* BVH nodes fetched (64 B each) */
   uint64_t prim_tests;      /* primitive intersection tests */
-  uint64_t reserved;
+  /* the part of the four counts above spent on shadow rays (wavefront
+   * pipeline; zero for the megakernel, which does not split them) */
+  uint64_t shadow_density_lookups;
+  uint64_t shadow_majorant_cells;
+  uint64_t shadow_bvh_nodes;
+  uint64_t shadow_prim_tests;
 } ne_counters;
 
 typedef struct ne_scene_info {
@@ -202,6 +221,11 @@ ne_status ne_render_tile(ne_ctx* ctx, const ne_scene* scene, const ne_camera_des
 /* Duration in milliseconds of the path-tracing kernel of the last render
  * issued with NE_FLAG_TIME (waits for it to finish). */
 ne_status ne_get_kernel_time(ne_ctx* ctx, float* ms);
+/* Per-stage kernel time of the last render issued with NE_FLAG_STAGE_TIME
+ * (waits for it to finish): ms[NE_STAGE_COUNT] summed durations (each launch
+ * timed from the event before it to the event after it on the context
+ * stream) and launches[NE_STAGE_COUNT] launch counts. */
+ne_status ne_get_stage_times(ne_ctx* ctx, float* ms, uint32_t* launches);
 /* Event counters of the last render issued with NE_FLAG_COUNT. */
 ne_status ne_get_counters(ne_ctx* ctx, ne_counters* out);
 /* Integrator::Li (src/integrators/Integrator.h:23) for a batch of n rays

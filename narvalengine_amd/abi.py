@@ -7,6 +7,8 @@ from pathlib import Path
 NE_FP32, NE_FP64 = 0, 1
 NE_FLAG_COUNT = 1
 NE_FLAG_TIME = 2
+NE_FLAG_STAGE_TIME = 4
+STAGES = ("megakernel", "plan", "regen", "trace", "medium", "shade", "shadow", "other")
 NE_MODE_AUTO, NE_MODE_MEGAKERNEL, NE_MODE_WAVEFRONT = 0, 1, 2
 NE_MAT_GLOSSY, NE_MAT_EMITTER = 0, 1
 NE_ENV_NONE, NE_ENV_CONSTANT, NE_ENV_SKY = 0, 1, 2
@@ -56,10 +58,11 @@ class RenderParams(C.Structure):
 class Counters(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in
                 ("paths", "ext_rays", "shadow_rays", "density_lookups", "majorant_cells", "bvh_nodes",
-                 "prim_tests", "reserved")]
+                 "prim_tests", "shadow_density_lookups", "shadow_majorant_cells", "shadow_bvh_nodes",
+                 "shadow_prim_tests")]
 
     def as_dict(self) -> dict:
-        return {n: int(getattr(self, n)) for n, _ in self._fields_ if n != "reserved"}
+        return {n: int(getattr(self, n)) for n, _ in self._fields_}
 
 
 class SceneInfo(C.Structure):
@@ -92,6 +95,7 @@ SIGNATURES = [
                                    C.c_int32, C.c_void_p, _F]),
     ("ne_get_kernel_time", C.c_int32, [C.c_void_p, _F]),
     ("ne_get_counters", C.c_int32, [C.c_void_p, C.POINTER(Counters)]),
+    ("ne_get_stage_times", C.c_int32, [C.c_void_p, _F, C.POINTER(C.c_uint32)]),
     ("ne_li", C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32, _F, C.c_int32, C.c_uint64, C.c_uint32, _F]),
     ("ne_test_primitive", C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, _F, _F, _I, _F, _F]),
     ("ne_test_closest", C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32, _F, C.c_float, C.c_float, _F, _I, _I]),

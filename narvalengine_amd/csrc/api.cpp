@@ -38,6 +38,9 @@ struct ne_ctx {
   void* wf_buf = nullptr;  // wavefront path pool (grown on demand)
   uint32_t wf_P = 0;
   uint32_t* h_status = nullptr;  // pinned status word read by the wavefront loop
+  std::vector<hipEvent_t> ev_pool;  // per-launch timing events (NE_FLAG_STAGE_TIME)
+  std::vector<int8_t> ev_stages;    // stage of the launch after each recorded event
+  bool stage_timed = false;
 };
 
 struct ne_scene {
@@ -309,6 +312,7 @@ static void ctx_free(ne_ctx* c) {
   if (c->d_counters) (void)hipFree(c->d_counters);
   if (c->wf_buf) (void)hipFree(c->wf_buf);
   if (c->h_status) (void)hipHostFree(c->h_status);
+  for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -557,10 +561,10 @@ ne_status ne_film_tonemap(ne_film* f, int32_t spp, float* pixels) {
 }
 
 static ne_status render_wavefront(ne_ctx* c, const ne_scene* s, const DevCamera& dc, const ne_render_params* p,
-                                  ne_film* f, bool count, bool timed) {
+                                  ne_film* f, bool count, bool timed, StageTimer* tm) {
   const uint64_t total = uint64_t(p->x1 - p->x0) * uint64_t(p->y1 - p->y0) * uint64_t(p->spp);
   // pool slots: whole chunks, no more than the job needs
-  uint64_t P = p->pool_size > 0 ? uint64_t(p->pool_size) : (uint64_t(1) << 21);
+  uint64_t P = p->pool_size > 0 ? uint64_t(p->pool_size) : (uint64_t(1) << 22);
   P = std::min<uint64_t>(P, total);
   P = (P + kChunk - 1) / kChunk * kChunk;
   if (P > c->wf_P) {
@@ -589,7 +593,7 @@ static ne_status render_wavefront(ne_ctx* c, const ne_scene* s, const DevCamera&
   a.film_fixed = f->d_fixed;
   a.film = f->d_sum;
   if (timed) NE_HIP(c, hipEventRecord(c->ev0, c->stream));
-  NE_HIP(c, launch_wavefront(s->dev, dc, w, a, count, c->stream, c->h_status));
+  NE_HIP(c, launch_wavefront(s->dev, dc, w, a, count, c->stream, c->h_status, tm));
   if (timed) {
     NE_HIP(c, hipEventRecord(c->ev1, c->stream));
     c->timed = true;
@@ -612,8 +616,19 @@ static ne_status render_locked(ne_ctx* c, const ne_scene* s, const ne_camera_des
     c->counters_valid = true;
   }
   DevCamera dc = make_camera(cam);
+  StageTimer timer{&c->ev_pool, &c->ev_stages, c->stream};
+  StageTimer* tm = nullptr;
+  c->stage_timed = false;
+  if (p->flags & NE_FLAG_STAGE_TIME) {
+    c->ev_stages.clear();
+    tm = &timer;
+  }
   const bool wavefront = p->mode == NE_MODE_WAVEFRONT || (p->mode == NE_MODE_AUTO && p->precision == NE_FP32);
-  if (wavefront) return render_wavefront(c, s, dc, p, f, count, timed);
+  if (wavefront) {
+    st = render_wavefront(c, s, dc, p, f, count, timed, tm);
+    c->stage_timed = st == NE_OK && tm;
+    return st;
+  }
   RenderArgs a{};
   a.W = p->width; a.H = p->height; a.x0 = p->x0; a.y0 = p->y0; a.x1 = p->x1; a.y1 = p->y1;
   a.spp = p->spp; a.max_bounces = p->max_bounces; a.sample_offset = p->sample_offset; a.job = p->job;
@@ -649,12 +664,20 @@ static ne_status render_locked(ne_ctx* c, const ne_scene* s, const ne_camera_des
   }
   int block = p->block_size ? p->block_size : 128;
   if (timed) NE_HIP(c, hipEventRecord(c->ev0, c->stream));
+  if (tm) NE_HIP(c, tm->mark(kStageMegakernel));
   NE_HIP(c, launch_render(s->dev, dc, a, p->precision, count, block, p->occupancy, c->stream));
   if (timed) {
     NE_HIP(c, hipEventRecord(c->ev1, c->stream));
     c->timed = true;
   }
-  if (a.nchunks > 1) NE_HIP(c, launch_reduce_partials(a, f->d_sum, c->stream));
+  if (a.nchunks > 1) {
+    if (tm) NE_HIP(c, tm->mark(kStageOther));
+    NE_HIP(c, launch_reduce_partials(a, f->d_sum, c->stream));
+  }
+  if (tm) {
+    NE_HIP(c, tm->mark(kStageEnd));
+    c->stage_timed = true;
+  }
   return NE_OK;
 }
 
@@ -700,6 +723,29 @@ ne_status ne_get_kernel_time(ne_ctx* c, float* ms) {
   return NE_OK;
 }
 
+ne_status ne_get_stage_times(ne_ctx* c, float* ms, uint32_t* launches) {
+  if (!c || !ms || !launches) return fail(c, NE_ERR_INVALID_ARG, "ctx/ms/launches is NULL");
+  std::lock_guard<std::mutex> lk(c->mu);
+  if (!c->stage_timed || c->ev_stages.empty())
+    return fail(c, NE_ERR_INVALID_ARG, "no render with NE_FLAG_STAGE_TIME has been issued");
+  NE_HIP(c, hipSetDevice(c->device));
+  for (int k = 0; k < NE_STAGE_COUNT; ++k) {
+    ms[k] = 0.0f;
+    launches[k] = 0;
+  }
+  const size_t n = c->ev_stages.size();
+  NE_HIP(c, hipEventSynchronize(c->ev_pool[n - 1]));
+  for (size_t k = 0; k + 1 < n; ++k) {
+    const int stage = c->ev_stages[k];
+    if (stage < 0 || stage >= NE_STAGE_COUNT) continue;
+    float t = 0.0f;
+    NE_HIP(c, hipEventElapsedTime(&t, c->ev_pool[k], c->ev_pool[k + 1]));
+    ms[stage] += t;
+    launches[stage] += 1;
+  }
+  return NE_OK;
+}
+
 ne_status ne_get_counters(ne_ctx* c, ne_counters* out) {
   if (!c || !out) return fail(c, NE_ERR_INVALID_ARG, "ctx/out is NULL");
   std::lock_guard<std::mutex> lk(c->mu);
@@ -709,8 +755,11 @@ ne_status ne_get_counters(ne_ctx* c, ne_counters* out) {
   NE_HIP(c, hipMemcpyAsync(&h, c->d_counters, sizeof(h), hipMemcpyDeviceToHost, c->stream));
   NE_HIP(c, hipStreamSynchronize(c->stream));
   out->paths = h.paths; out->ext_rays = h.ext_rays; out->shadow_rays = h.shadow_rays;
-  out->density_lookups = h.density_lookups; out->majorant_cells = h.majorant_cells;
-  out->bvh_nodes = h.bvh_nodes; out->prim_tests = h.prim_tests; out->reserved = 0;
+  // device fields hold the extension-ray share; totals add the shadow share
+  out->density_lookups = h.density_lookups + h.sh_lookups; out->majorant_cells = h.majorant_cells + h.sh_cells;
+  out->bvh_nodes = h.bvh_nodes + h.sh_nodes; out->prim_tests = h.prim_tests + h.sh_prims;
+  out->shadow_density_lookups = h.sh_lookups; out->shadow_majorant_cells = h.sh_cells;
+  out->shadow_bvh_nodes = h.sh_nodes; out->shadow_prim_tests = h.sh_prims;
   return NE_OK;
 }
 

@@ -4,9 +4,31 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <vector>
+
 #include "ne_scene.h"
 
 namespace ne {
+
+// Per-launch timing: an event is recorded before every kernel launch and once
+// at the end; the interval from one event to the next is charged to the stage
+// of the launch between them.  Events come from a pool that grows on demand.
+struct StageTimer {
+  std::vector<hipEvent_t>* pool;
+  std::vector<int8_t>* stages;
+  hipStream_t st;
+  hipError_t mark(int stage) {
+    const size_t k = stages->size();
+    if (k == pool->size()) {
+      hipEvent_t e;
+      const hipError_t r = hipEventCreate(&e);
+      if (r != hipSuccess) return r;
+      pool->push_back(e);
+    }
+    stages->push_back(int8_t(stage));
+    return hipEventRecord((*pool)[k], st);
+  }
+};
 
 struct RenderArgs {
   int32_t W, H, x0, y0, x1, y1;
@@ -27,6 +49,9 @@ hipError_t launch_render(const DevScene& s, const DevCamera& cam, const RenderAr
 // The path pool is split into chunks of kChunk slots; stage kernels run one
 // block of kChunk threads per chunk and keep each chunk's live paths packed.
 constexpr uint32_t kChunk = 256;
+// stage ids (the public NE_STAGE_* values)
+enum { kStageMegakernel = 0, kStagePlan, kStageRegen, kStageTrace, kStageMedium, kStageShade, kStageShadow,
+       kStageOther, kStageCount, kStageEnd = -1 };
 enum { WF_KIND_MEDIUM = 0, WF_KIND_SURFACE = 1, WF_KIND_MISS = 2 };
 constexpr uint32_t kNoHit = 0xFFFFFFFFu;
 constexpr uint32_t kPrevDelta = 1u << 16;  // depth word: set while the last vertex was the camera
@@ -78,7 +103,7 @@ struct WfArgs {
 // Renders a.total_paths paths through a pool of a.nchunks * kChunk slots;
 // returns after the film flush is enqueued.  h_status: pinned host word.
 hipError_t launch_wavefront(const DevScene& s, const DevCamera& cam, WfState& w, WfArgs a, bool count,
-                            hipStream_t st, uint32_t* h_status);
+                            hipStream_t st, uint32_t* h_status, StageTimer* tm);
 hipError_t launch_reduce_partials(const RenderArgs& a, double* film, hipStream_t st);
 hipError_t launch_tonemap(const double* film, int npix, float inv_spp, float* pixels, hipStream_t st);
 

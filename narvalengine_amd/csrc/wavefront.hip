@@ -519,10 +519,10 @@ __global__ __launch_bounds__(kChunk) void k_wf_shadow(WfState w, WfArgs a, DevSc
   }
   if constexpr (C) {
     block_count<C>(&a.counters->shadow_rays, cnt.n_shadow, red);
-    block_count<C>(&a.counters->density_lookups, cnt.n_lookup, red);
-    block_count<C>(&a.counters->majorant_cells, cnt.n_cell, red);
-    block_count<C>(&a.counters->bvh_nodes, cnt.n_node, red);
-    block_count<C>(&a.counters->prim_tests, cnt.n_prim, red);
+    block_count<C>(&a.counters->sh_lookups, cnt.n_lookup, red);
+    block_count<C>(&a.counters->sh_cells, cnt.n_cell, red);
+    block_count<C>(&a.counters->sh_nodes, cnt.n_node, red);
+    block_count<C>(&a.counters->sh_prims, cnt.n_prim, red);
   }
 }
 
@@ -578,16 +578,22 @@ void wf_carve(void* base, uint32_t P, WfState& w) {
 }
 
 hipError_t launch_wavefront(const DevScene& s, const DevCamera& cam, WfState& w, WfArgs a, bool count,
-                            hipStream_t st, uint32_t* h_status) {
+                            hipStream_t st, uint32_t* h_status, StageTimer* tm) {
+  hipError_t e = hipSuccess;
+#define NE_MARK(stage)                                          \
+  do {                                                          \
+    if (tm && (e = tm->mark(stage)) != hipSuccess) return e;    \
+  } while (0)
   const uint32_t NC = a.nchunks;
+  NE_MARK(kStageOther);
   hipLaunchKernelGGL(k_wf_init, dim3((NC + 255) / 256), dim3(256), 0, st, w, NC);
   const dim3 grid = dim3(NC);
   const dim3 blk = dim3(unsigned(kChunk));
   if (!count) a.counters = nullptr;
-  hipError_t e = hipSuccess;
   const bool hs = s.n_nodes > 0, hv = s.has_volume != 0;
   for (uint32_t it = 0;; ++it) {
     a.cur = int(it & 1u);
+    NE_MARK(kStagePlan);
     hipLaunchKernelGGL(k_wf_plan, dim3(1), dim3(unsigned(kPlanThreads)), 0, st, w, a);
     // every 8 iterations: stop once nothing is in flight (status written by plan)
     if ((it & 7u) == 7u) {
@@ -595,12 +601,15 @@ hipError_t launch_wavefront(const DevScene& s, const DevCamera& cam, WfState& w,
       if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
       if (h_status[0] == 0) break;
     }
+    NE_MARK(kStageRegen);
     hipLaunchKernelGGL(k_wf_regen, grid, blk, 0, st, w, a, cam);
     if (hs) {
+      NE_MARK(kStageTrace);
       if (count) hipLaunchKernelGGL(k_wf_trace<true>, grid, blk, 0, st, w, a, s);
       else hipLaunchKernelGGL(k_wf_trace<false>, grid, blk, 0, st, w, a, s);
     }
     if (hv) {
+      NE_MARK(kStageMedium);
       if (count) {
         if (hs) hipLaunchKernelGGL((k_wf_medium<true, true>), grid, blk, 0, st, w, a, s);
         else hipLaunchKernelGGL((k_wf_medium<true, false>), grid, blk, 0, st, w, a, s);
@@ -609,8 +618,13 @@ hipError_t launch_wavefront(const DevScene& s, const DevCamera& cam, WfState& w,
         else hipLaunchKernelGGL((k_wf_medium<false, false>), grid, blk, 0, st, w, a, s);
       }
     }
-    if (!hs && !hv) hipLaunchKernelGGL(k_wf_all_miss, grid, blk, 0, st, w, a);
+    if (!hs && !hv) {
+      NE_MARK(kStageOther);
+      hipLaunchKernelGGL(k_wf_all_miss, grid, blk, 0, st, w, a);
+    }
+    NE_MARK(kStageShade);
     hipLaunchKernelGGL(k_wf_shade, grid, blk, 0, st, w, a, s);
+    NE_MARK(kStageShadow);
 #define NE_SHADOW(CNT, S_, V_) hipLaunchKernelGGL((k_wf_shadow<CNT, S_, V_>), grid, blk, 0, st, w, a, s)
     if (count) {
       if (hs && hv) NE_SHADOW(true, true, true); else if (hs) NE_SHADOW(true, true, false);
@@ -623,7 +637,10 @@ hipError_t launch_wavefront(const DevScene& s, const DevCamera& cam, WfState& w,
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
   const size_t nf = size_t(a.W) * size_t(a.H) * 3;
+  NE_MARK(kStageOther);
   hipLaunchKernelGGL(k_wf_flush_film, dim3(uint32_t((nf + 255) / 256)), dim3(256), 0, st, a.film_fixed, a.film, nf);
+  NE_MARK(kStageEnd);
+#undef NE_MARK
   return hipGetLastError();
 }
 

@@ -5,7 +5,8 @@ import ctypes as C
 
 import numpy as np
 
-from .abi import (Counters, NarvalError, RenderParams, SceneInfo, check, lib, NE_FP32, NE_FLAG_COUNT, NE_FLAG_TIME)
+from .abi import (Counters, NarvalError, RenderParams, SceneInfo, check, lib, NE_FP32, NE_FLAG_COUNT, NE_FLAG_TIME,
+                  NE_FLAG_STAGE_TIME, STAGES)
 
 
 def _fp(a: np.ndarray):
@@ -55,6 +56,7 @@ class Context:
     def render(self, gscene: "GpuScene", camera, film: "Film", *, spp: int, max_bounces: int, seed: int = 0,
                sample_offset: int = 0, job: int = 0, rect=None, precision: int = NE_FP32, count: bool = False,
                samples_per_thread: int = 0, block_size: int = 0, sync: bool = True, timed: bool = False,
+               stage_timed: bool = False,
                occupancy: int = 0, mode: int = 0, pool_size: int = 0):
         p = RenderParams()
         p.width, p.height = film.width, film.height
@@ -62,7 +64,8 @@ class Context:
         p.x0, p.y0, p.x1, p.y1 = x0, y0, x1, y1
         p.spp, p.max_bounces, p.seed = spp, max_bounces, seed
         p.sample_offset, p.job, p.precision = sample_offset, job, precision
-        p.flags = (NE_FLAG_COUNT if count else 0) | (NE_FLAG_TIME if timed else 0)
+        p.flags = (NE_FLAG_COUNT if count else 0) | (NE_FLAG_TIME if timed else 0) | \
+            (NE_FLAG_STAGE_TIME if stage_timed else 0)
         p.samples_per_thread, p.block_size, p.occupancy = samples_per_thread, block_size, occupancy
         p.mode, p.pool_size = mode, pool_size
         cam = camera.desc()
@@ -74,6 +77,13 @@ class Context:
         ms = C.c_float()
         self.check(self._lib.ne_get_kernel_time(self.h, C.byref(ms)))
         return float(ms.value)
+
+    def stage_times(self) -> dict:
+        """{stage: (total ms, launches)} of the last render issued with stage_timed=True."""
+        ms = (C.c_float * len(STAGES))()
+        n = (C.c_uint32 * len(STAGES))()
+        self.check(self._lib.ne_get_stage_times(self.h, ms, n))
+        return {name: (float(ms[k]), int(n[k])) for k, name in enumerate(STAGES)}
 
     def counters(self) -> dict:
         c = Counters()

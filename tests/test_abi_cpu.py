@@ -26,7 +26,7 @@ def declared_symbols():
 def test_library_exports_every_declared_symbol():
     L = abi.lib()
     names = declared_symbols()
-    assert len(names) >= 29
+    assert len(names) >= 30
     for n in names:
         assert hasattr(L, n), n
     bound = {s[0] for s in abi.SIGNATURES}

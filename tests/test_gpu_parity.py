@@ -389,3 +389,42 @@ def test_wavefront_matches_megakernel(ctx, which):
     # (FP64 sums vs 2^-30 fixed point)
     np.testing.assert_allclose(films[1], films[0], rtol=1e-6, atol=16 * 2.0 ** -30)
     np.testing.assert_array_equal(films[1], films[2])  # the pool size does not change the result
+
+
+@pytest.mark.parametrize("which", ["volume", "surface"])
+def test_wavefront_counters_and_stage_times(ctx, which):
+    """Both pipelines trace the same paths: equal event counts; the per-launch
+    stage times add up to the whole-render time."""
+    print_case(f"wavefront counters/stage times {which}")
+    from narvalengine_amd.abi import NE_MODE_MEGAKERNEL, NE_MODE_WAVEFRONT
+    if which == "volume":
+        s, cam, bounces = scenes.small_volume(res=32, maj_res=4), scenes.VOLUME_CAMERA, 16
+    else:
+        s, cam, bounces = scenes.small_surface(subdiv=3), scenes.BOX_CAMERA, 8
+    g = ctx.upload(s)
+    counts = {}
+    for mode in (NE_MODE_MEGAKERNEL, NE_MODE_WAVEFRONT):
+        f = ctx.film(40, 24)
+        ctx.render(g, cam, f, spp=16, max_bounces=bounces, seed=5, mode=mode, count=True)
+        counts[mode] = ctx.counters()
+    mk, wf = counts[NE_MODE_MEGAKERNEL], counts[NE_MODE_WAVEFRONT]
+    for k in ("paths", "ext_rays", "shadow_rays", "density_lookups", "majorant_cells", "bvh_nodes", "prim_tests"):
+        assert mk[k] == wf[k], (k, mk[k], wf[k])
+    assert wf["paths"] == 40 * 24 * 16
+    assert all(mk["shadow_" + k] == 0 for k in ("density_lookups", "majorant_cells", "bvh_nodes", "prim_tests"))
+    if which == "volume":
+        assert 0 < wf["shadow_density_lookups"] < wf["density_lookups"]
+    else:
+        assert 0 < wf["shadow_bvh_nodes"] < wf["bvh_nodes"]
+    f = ctx.film(40, 24)
+    ctx.render(g, cam, f, spp=16, max_bounces=bounces, seed=5, mode=NE_MODE_WAVEFRONT, timed=True,
+               stage_timed=True)
+    total = ctx.kernel_ms()
+    st = ctx.stage_times()
+    assert st["megakernel"] == (0.0, 0)
+    n = {k: v[1] for k, v in st.items()}
+    assert n["plan"] >= n["regen"] == n["shade"] == n["shadow"] >= 1
+    assert (n["trace"] == n["shade"]) == (which == "surface")
+    assert (n["medium"] == n["shade"]) == (which == "volume")
+    staged = sum(v[0] for v in st.values())
+    assert 0 < staged <= total * 1.02 + 0.05 and staged >= 0.5 * total
