@@ -1,0 +1,378 @@
+"""Pins of the oracle that do not come from the reference's own vectors:
+closed forms, normalisation integrals, sampler/pdf consistency, exactness on
+affine fields, special cases and Monte-Carlo expectations with known values.
+Each check is chosen so that a plausible slip (a dropped Jacobian, a wrong sign
+or axis, a missing MIS term) fails it.  CPU only; statistical checks use fixed
+seeds and 5-sigma bounds."""
+import ctypes as C
+
+import numpy as np
+import pytest
+
+import oracle as O
+import scenes
+from narvalengine_amd.abi import NE_ENV_CONSTANT
+from narvalengine_amd.scene import Camera, Scene
+
+L = O.lib
+
+
+def _unit(v):
+    v = np.asarray(v, np.float64)
+    return v / np.linalg.norm(v, axis=-1, keepdims=True)
+
+
+def _hemisphere_grid(n_mu=2000, n_phi=256, sharp=False):
+    """Midpoint quadrature nodes on the upper hemisphere, weights in steradians.
+    sharp: nodes packed near the pole (t = 1 - mu on a geometric grid)."""
+    if sharp:
+        edges = 1.0 - np.concatenate([[0.0], np.geomspace(1e-9, 1.0, n_mu)])[::-1]
+    else:
+        edges = np.linspace(0.0, 1.0, n_mu + 1)
+    mu = 0.5 * (edges[1:] + edges[:-1])
+    dmu = np.diff(edges)
+    phi = (np.arange(n_phi) + 0.5) * (2 * np.pi / n_phi)
+    M, P = np.meshgrid(mu, phi, indexing="ij")
+    s = np.sqrt(np.maximum(0.0, 1 - M * M))
+    dirs = np.stack([s * np.cos(P), s * np.sin(P), M], -1).reshape(-1, 3)
+    w = (dmu[:, None] * np.full(n_phi, 2 * np.pi / n_phi)[None, :]).reshape(-1)
+    return dirs, w
+
+
+# ---- Henyey-Greenstein, propagation convention (corrected mode) -------------
+@pytest.mark.parametrize("g", [-0.6, 0.0, 0.3, 0.85, 0.877])
+def test_hg_forward_is_normalised_with_mean_cosine_g(g):
+    mu = np.linspace(-1, 1, 400001).astype(np.float32)
+    p = np.zeros_like(mu)
+    L().neo_hg_p(len(mu), O.fp(np.full(len(mu), g, np.float32)), O.fp(mu), O.fp(p))
+    p = p.astype(np.float64)
+    # closed form of the HG density
+    ref = (1 - g * g) / (4 * np.pi * (1 + g * g - 2 * g * mu.astype(np.float64)) ** 1.5)
+    np.testing.assert_allclose(p, ref, rtol=2e-5)
+    assert abs(2 * np.pi * np.trapezoid(p, mu) - 1) < 1e-4
+    assert abs(2 * np.pi * np.trapezoid(p * mu, mu) - g) < 1e-4
+
+
+@pytest.mark.parametrize("g", [0.0, 0.5, 0.877, -0.4])
+def test_hg_sampler_matches_its_pdf_around_the_propagation_direction(g):
+    n = 200000
+    rng = np.random.default_rng(11)
+    d = _unit([0.3, -0.5, 0.8]).astype(np.float32)
+    u = rng.random((n, 2), dtype=np.float32)
+    out = np.zeros((n, 3), np.float32)
+    L().neo_hg_sample(n, O.fp(np.full(n, g, np.float32)), O.fp(np.tile(d, (n, 1))), O.fp(u), O.fp(out))
+    np.testing.assert_allclose(np.linalg.norm(out, axis=1), 1, atol=2e-6)
+    c = out.astype(np.float64) @ d.astype(np.float64)
+    # mean cosine with the propagation direction is g (forward scattering for g > 0)
+    assert abs(c.mean() - g) < 5 * c.std() / np.sqrt(n) + 1e-4
+    # CDF of cos theta against the closed-form HG CDF (Kolmogorov-Smirnov)
+    cs = np.sort(c)
+    if abs(g) < 1e-3:
+        cdf = (cs + 1) / 2
+    else:
+        cdf = (1 - g * g) / (2 * g) * (1 / np.sqrt(1 + g * g - 2 * g * cs) - 1 / (1 + g))
+    ks = np.max(np.abs(cdf - (np.arange(1, n + 1) - 0.5) / n))
+    assert ks < 1.63 / np.sqrt(n) * 1.5
+    # azimuth uniform about d: the transverse component averages to zero
+    t = out - c[:, None].astype(np.float32) * d
+    assert np.abs(t.mean(0)).max() < 5 / np.sqrt(n)
+
+
+# ---- GGX (reference function-level semantics) --------------------------------
+@pytest.mark.parametrize("alpha", [0.05, 0.2, 0.5, 0.9])
+def test_ggx_distribution_is_normalised(alpha):
+    # projected-area normalisation: integral of D(h) cos(theta_h) over the hemisphere = 1
+    dirs, w = _hemisphere_grid(4000, 4, sharp=True)
+    n = len(dirs)
+    D = np.zeros(n, np.float32)
+    L().neo_ggx_D(n, O.fp(np.full(n, alpha, np.float32)), O.fp(dirs.astype(np.float32)), O.fp(D))
+    assert abs(np.sum(D * dirs[:, 2] * w) - 1) < 2e-3
+
+
+@pytest.mark.parametrize("alpha", [0.1, 0.4, 0.8])
+def test_ggx_half_vector_sampler_matches_D_cos(alpha):
+    n = 100000
+    u = np.random.default_rng(5).random((n, 2), dtype=np.float32)
+    h = np.zeros((n, 3), np.float32)
+    L().neo_ggx_sample_h(n, O.fp(np.full(n, alpha, np.float32)), O.fp(u), O.fp(h))
+    np.testing.assert_allclose(np.linalg.norm(h, axis=1), 1, atol=3e-6)
+    assert (h[:, 2] >= 0).all()
+    # distribution of cos(theta_h) against the CDF integrated from D(h) cos(theta_h)
+    mu = 1.0 - np.concatenate([[0.0], np.geomspace(1e-9, 1.0, 20000)])[::-1]
+    hh = np.stack([np.sqrt(1 - mu * mu), np.zeros_like(mu), mu], -1).astype(np.float32)
+    D = np.zeros(len(mu), np.float32)
+    L().neo_ggx_D(len(mu), O.fp(np.full(len(mu), alpha, np.float32)), O.fp(hh), O.fp(D))
+    dens = 2 * np.pi * D.astype(np.float64) * mu
+    cdf = np.concatenate([[0.0], np.cumsum(0.5 * (dens[1:] + dens[:-1]) * np.diff(mu))])
+    assert abs(cdf[-1] - 1) < 2e-3
+    F = np.interp(np.sort(h[:, 2].astype(np.float64)), mu, cdf / cdf[-1])
+    ks = np.max(np.abs(F - (np.arange(1, n + 1) - 0.5) / n))
+    assert ks < 1.63 / np.sqrt(n) * 1.5
+
+
+def test_ggx_geometry_term_special_values():
+    n = 5
+    alpha = np.array([0.1, 0.5, 0.9, 0.5, 0.5], np.float32)
+    wi = _unit([[0, 0, 1], [0, 0, 1], [0, 0, 1], [0.6, 0, 0.8], [1, 0, 1e-7]]).astype(np.float32)
+    wo = _unit([[0, 0, 1], [0, 0, 1], [0, 0, 1], [0, 0.6, 0.8], [-1, 0, 1e-7]]).astype(np.float32)
+    G = np.zeros(n, np.float32)
+    L().neo_ggx_G(n, O.fp(alpha), O.fp(wi), O.fp(wo), O.fp(G))
+    np.testing.assert_allclose(G[:3], 1.0, rtol=1e-6)  # both directions along h: G1 = 1
+    G2 = np.zeros(n, np.float32)
+    L().neo_ggx_G(n, O.fp(alpha), O.fp(wo), O.fp(wi), O.fp(G2))
+    np.testing.assert_array_equal(G, G2)  # symmetric
+    # wi = (0.6,0,0.8), wo = (0,0.6,0.8): h along (0.6,0.6,1.6)/|.|, G1 = c / (c (1 - k) + k)
+    h = _unit([0.6, 0.6, 1.6])
+    c = float(h @ wi[3])
+    k = 0.25
+    assert abs(G[3] - (c / (c * (1 - k) + k)) ** 2) < 1e-6
+    assert G[4] < 1e-5  # opposite grazing directions: h is the normal, both cosines ~0
+
+
+# ---- corrected glossy BSDF: sampler, pdf and eval agree ----------------------
+@pytest.mark.parametrize("albedo,rough,metal,wo", [((0.8, 0.5, 0.2), 0.3, 0.0, (0.3, 0.1, 0.9)),
+                                                   ((0.6, 0.6, 0.6), 0.7, 0.5, (-0.7, 0.2, 0.4)),
+                                                   ((0.9, 0.9, 0.9), 0.2, 1.0, (0.1, -0.2, 0.95))])
+def test_glossy_sampler_pdf_and_eval_are_consistent(albedo, rough, metal, wo):
+    wo = _unit(wo).astype(np.float32)
+    mat = np.array([*albedo, rough, metal], np.float32)
+    # quadrature over the upper hemisphere: integral of the pdf and of eval * cos
+    dirs, w = _hemisphere_grid(2000, 1024)
+    n = len(dirs)
+    f, pdf, smp = np.zeros((n, 3), np.float32), np.zeros(n, np.float32), np.zeros((n, 3), np.float32)
+    u = np.zeros((n, 3), np.float32)
+    L().neo_glossy_c(n, O.fp(np.tile(mat, (n, 1))), O.fp(np.tile(wo, (n, 1))), O.fp(dirs.astype(np.float32)),
+                     O.fp(u), O.fp(f), O.fp(pdf), O.fp(smp))
+    pdf_mass = np.sum(pdf * w)
+    assert 0.5 < pdf_mass <= 1.0 + 2e-3  # sampled lobe mass below the horizon is lost, never gained
+    integral = (f * dirs[:, 2:3] * w[:, None]).sum(0)
+    # Monte Carlo with the sampler: E[eval cos / pdf] over upper-hemisphere samples
+    m = 400000
+    rng = np.random.default_rng(9)
+    u = rng.random((m, 3), dtype=np.float32)
+    probe = np.tile(_unit([0, 0, 1]).astype(np.float32), (m, 1))
+    f1, p1, s1 = np.zeros((m, 3), np.float32), np.zeros(m, np.float32), np.zeros((m, 3), np.float32)
+    L().neo_glossy_c(m, O.fp(np.tile(mat, (m, 1))), O.fp(np.tile(wo, (m, 1))), O.fp(probe), O.fp(u), O.fp(f1),
+                     O.fp(p1), O.fp(s1))
+    f2, p2, s2 = np.zeros((m, 3), np.float32), np.zeros(m, np.float32), np.zeros((m, 3), np.float32)
+    L().neo_glossy_c(m, O.fp(np.tile(mat, (m, 1))), O.fp(np.tile(wo, (m, 1))), O.fp(s1), O.fp(u), O.fp(f2),
+                     O.fp(p2), O.fp(s2))
+    ok = (s1[:, 2] > 0) & (p2 > 0)
+    est = np.where(ok[:, None], f2 * np.abs(s1[:, 2:3]) / np.where(ok, p2, 1)[:, None], 0.0)
+    # fraction of samples above the horizon equals the pdf mass
+    assert abs(ok.mean() - pdf_mass) < 5 * np.sqrt(pdf_mass * (1 - pdf_mass) / m) + 3e-3
+    mean, se = est.mean(0), est.std(0) / np.sqrt(m)
+    assert (np.abs(mean - integral) < 5 * se + 3e-3 * integral).all(), (mean, integral, se)
+
+
+# ---- grid media ----------------------------------------------------------------
+def test_trilinear_is_exact_on_affine_fields_and_reads_zero_above():
+    W, H, D = 7, 5, 6
+    z, y, x = np.meshgrid(np.arange(D), np.arange(H), np.arange(W), indexing="ij")
+    grid = (0.25 + 0.05 * x + 0.03 * y + 0.02 * z).astype(np.float32)  # (D, H, W), x fastest
+    rng = np.random.default_rng(1)
+    p = rng.uniform(0, 1, (5000, 3)) * np.array([W - 1, H - 1, D - 1])
+    out = np.zeros(len(p), np.float32)
+    L().neo_interp_density(W, H, D, O.fp(grid), len(p), O.fp(p.astype(np.float32)), O.fp(out))
+    np.testing.assert_allclose(out, 0.25 + 0.05 * p[:, 0] + 0.03 * p[:, 1] + 0.02 * p[:, 2], rtol=2e-6, atol=1e-6)
+    # cell-corner convention: no half-voxel shift, integer points read the voxel itself
+    q = np.array([[2, 3, 1], [0, 0, 0], [W - 1, H - 1, D - 1]], np.float32)
+    out = np.zeros(3, np.float32)
+    L().neo_interp_density(W, H, D, O.fp(grid), 3, O.fp(q), O.fp(out))
+    np.testing.assert_allclose(out, [grid[1, 3, 2], grid[0, 0, 0], grid[D - 1, H - 1, W - 1]], rtol=1e-6)
+    # beyond the last voxel the upper neighbour reads 0; points clamp to [0, res]
+    q = np.array([[W - 0.5, 2, 2], [W, 2, 2], [-3, 2, 2], [W + 9, 2, 2]], np.float32)
+    out = np.zeros(4, np.float32)
+    L().neo_interp_density(W, H, D, O.fp(grid), 4, O.fp(q), O.fp(out))
+    np.testing.assert_allclose(out, [0.5 * grid[2, 2, W - 1], 0, grid[2, 2, 0], 0], rtol=1e-6, atol=1e-7)
+    assert L().neo_max_density(W, H, D, O.fp(grid)) == grid.max()
+
+
+def _const_volume(res, rho, mult, albedo=1.0, maj_res=4, linear_x=False):
+    z, y, x = np.meshgrid(np.arange(res), np.arange(res), np.arange(res), indexing="ij")
+    grid = np.full((res, res, res), rho, np.float32)
+    if linear_x:
+        grid = (rho * x / (res - 1)).astype(np.float32)
+    s = Scene()
+    s.set_volume(grid, mult, (albedo,) * 3, (1 - albedo,) * 3, 0.0, maj_res=maj_res)
+    return s, grid
+
+
+def _rays_along_x(n, res):
+    # OCS x from -0.5 (GCS 0) to 0.5 - 1/res (GCS res - 1): inside the region where the
+    # trilinear field is exactly the voxel field; y, z interior
+    o = np.tile([-0.5, 0.1, -0.2, 1.0, 0.0, 0.0], (n, 1)).astype(np.float32)
+    return o, 1.0 - 1.0 / res
+
+
+@pytest.mark.parametrize("maj_res", [1, 4])
+def test_delta_tracking_free_flight_law_on_a_constant_grid(maj_res):
+    res, rho, mult = 16, 0.5, 3.0
+    s, _ = _const_volume(res, rho, mult, maj_res=maj_res)
+    w = O.World(s)
+    n = 100000
+    rays, Lx = _rays_along_x(n, res)
+    sc, t = w.delta_track(rays)
+    sigma = rho * mult
+    # collision before Lx with probability 1 - exp(-sigma Lx); collision distance exponential
+    hit_lx = sc.astype(bool) & (t < Lx)
+    p = 1 - np.exp(-sigma * Lx)
+    assert abs(hit_lx.mean() - p) < 5 * np.sqrt(p * (1 - p) / n)
+    for s_ in (0.1, 0.3, 0.6):
+        q = 1 - np.exp(-sigma * s_)
+        assert abs((sc.astype(bool) & (t < s_)).mean() - q) < 5 * np.sqrt(q * (1 - q) / n)
+
+
+@pytest.mark.parametrize("linear", [False, True])
+def test_ratio_tracking_expectation_is_exp_minus_optical_depth(linear):
+    res, rho, mult = 16, 0.8, 2.5
+    s, grid = _const_volume(res, rho, mult, linear_x=linear, maj_res=4)
+    w = O.World(s)
+    n = 100000
+    rays, Lx = _rays_along_x(n, res)
+    tr = w.transmittance(rays, np.full(n, Lx, np.float32), seed=3)
+    # optical depth along x: integral of mult * density over [0, Lx] in OCS units
+    tau = mult * rho * Lx * (0.5 if linear else 1.0)
+    assert abs(tr.mean() - np.exp(-tau)) < 5 * tr.std() / np.sqrt(n) + 1e-4
+    assert (tr >= 0).all()
+
+
+# ---- post-processing, camera, tiles -------------------------------------------
+def test_post_processing_special_values():
+    c = np.array([[0, 0, 0], [1e4, 1e4, 1e4], [2 * np.log(2)] * 3, [0.25, 0.5, 3]], np.float32)
+    out = np.zeros_like(c)
+    L().neo_post_processing(4, O.fp(c), O.fp(out))
+    np.testing.assert_allclose(out[0], 0, atol=0)
+    np.testing.assert_allclose(out[1], 1, atol=0)
+    np.testing.assert_allclose(out[2], 0.5 ** (1 / 2.2), rtol=1e-6)
+    assert 0 < out[3, 0] < out[3, 1] < out[3, 2] < 1  # monotone, inside (0, 1)
+
+
+def test_camera_pixel_directions_follow_the_pinhole_geometry():
+    cam = Camera(position=(1.0, 2.0, -3.0), look_at=(0.2, 0.5, 0.4), up=(0, 1, 0), vfov=50.0, aspect=1.5)
+    uv = np.array([[0.5, 0.5], [0.5, 1.0], [0.5, 0.0], [1.0, 0.5], [0.0, 0.5]], np.float32)
+    out = np.zeros((5, 3), np.float32)
+    cd = cam.desc()
+    L().neo_camera_dirs(C.byref(cd), 5, O.fp(uv), O.fp(out))
+    fwd = _unit(np.array(cam.look_at) - np.array(cam.position))
+    np.testing.assert_allclose(out[0], fwd, atol=2e-6)
+    # the focus plane at unit distance along fwd is spanned by the horizontal axis
+    # normalize(cross(up, -fwd)) and the given up vector (not re-orthogonalised):
+    # a pixel (u, v) looks along fwd + (2u - 1) tan(hfov) side + (1 - 2v) tan(vfov/2) up,
+    # so row 0 (v = 0) is the top of the image
+    up = np.array([0.0, 1.0, 0.0])
+    side = _unit(np.cross(up, -fwd))
+    hh = np.tan(np.radians(25.0))
+    hw = 1.5 * hh
+    expect = [fwd - hh * up, fwd + hh * up, fwd + hw * side, fwd - hw * side]
+    for k, e in enumerate(expect):
+        np.testing.assert_allclose(out[k + 1], _unit(e), atol=3e-6)
+    assert np.cross(fwd, up) @ side > 0  # u grows to the right of the view direction
+
+
+@pytest.mark.parametrize("W,H", [(512, 512), (1024, 1024), (1920, 1080), (256, 256), (1280, 720)])
+def test_tiles_partition_the_rendered_region(W, H):
+    cover = np.zeros((H, W), np.int32)
+    r = np.zeros(4, np.int32)
+    for i in range(400):
+        L().neo_tile_rect(W, H, i, O.ip(r))
+        cover[r[1]:r[3], r[0]:r[2]] += 1
+    tw, th = W // 40, H // 10
+    assert (cover[:10 * th, :40 * tw] == 1).all()  # each rendered pixel exactly once
+    assert cover[10 * th:, :].sum() == 0 and cover[:, 40 * tw:].sum() == 0  # remainder dropped
+    if (W, H) == (512, 512):
+        assert (tw, th) == (12, 51)
+
+
+def test_oracle_bvh_matches_brute_force():
+    for s in (scenes.triangle_soup(), scenes.small_surface(subdiv=2)):
+        w = O.World(s)
+        rays = scenes.random_rays(3000, seed=4, origin_box=1.5)
+        t, ty, ix = w.closest(rays)
+        tb, tyb, ixb = w.closest(rays, brute=True)
+        np.testing.assert_array_equal(ty, tyb)
+        np.testing.assert_array_equal(ix, ixb)
+        np.testing.assert_array_equal(t, tb)
+
+
+# ---- integrator level --------------------------------------------------------
+def _pixel_stats(world, cam, W, H, spp, bounces, batches=8, seed=1):
+    vals = []
+    for b in range(batches):
+        img = world.render(cam, W, H, spp, bounces, seed=seed, sample_offset=b * spp, threads=8)
+        vals.append(img / spp)
+    v = np.stack(vals)  # (batches, H, W, 3)
+    return v.mean(0), v.std(0, ddof=1) / np.sqrt(batches)
+
+
+def test_white_furnace_scattering_medium_under_uniform_light():
+    """Albedo-1 medium lit by a constant environment: every pixel equals the
+    environment radiance (NEE + phase sampling combined by MIS must add to
+    exactly the incident radiance, at every vertex)."""
+    s = Scene()
+    s.set_volume(scenes.configs.fbm_grid(16, 3), 6.0, (1.0,) * 3, (0.0,) * 3, 0.7, maj_res=4)
+    Le = np.array([0.3, 0.5, 0.7])
+    s.set_env(NE_ENV_CONSTANT, c0=tuple(Le))
+    w = O.World(s, fp64=True)
+    mean, se = _pixel_stats(w, scenes.VOLUME_CAMERA, 6, 6, 64, 256)
+    full = mean.mean((0, 1))
+    full_se = np.sqrt((se ** 2).sum((0, 1))) / 36
+    assert (np.abs(full - Le) < 5 * full_se + 1e-3 * Le).all(), (full, Le, full_se)
+    assert (np.abs(mean - Le) < 6 * se + 1e-3).all()
+
+
+def test_purely_absorbing_slab_transmits_exp_minus_tau():
+    """Camera ray along z through a constant absorbing grid: radiance =
+    L_env exp(-tau), tau from the closed-form integral of the trilinear field
+    (the top voxel layer ramps to zero: tau = mult rho (1 - 1/(2 res)))."""
+    res, rho, mult = 8, 0.6, 2.0
+    s = Scene()
+    s.set_volume(np.full((res, res, res), rho, np.float32), mult, (0.0,) * 3, (1.0,) * 3, 0.0, maj_res=2)
+    s.set_env(NE_ENV_CONSTANT, c0=(1.0, 1.0, 1.0))
+    cam = Camera(position=(-0.1, 0.05, -5.0), look_at=(-0.1, 0.05, 0.0), up=(0, 1, 0), vfov=0.01, aspect=1.0)
+    w = O.World(s)
+    mean, se = _pixel_stats(w, cam, 1, 1, 20000, 8)
+    tau = mult * rho * (1 - 0.5 / res)
+    assert (np.abs(mean[0, 0] - np.exp(-tau)) < 5 * se[0, 0] + 1e-4).all(), (mean, np.exp(-tau), se)
+
+
+def test_direct_light_on_a_glossy_floor_matches_quadrature():
+    """One scatter vertex on a floor under a rectangular emitter: the MIS
+    combination of light and BSDF sampling must equal the area integral
+    of f * Le * cos_x * cos_y / d^2 over the emitter."""
+    s = Scene()
+    albedo, rough, metal = (0.7, 0.5, 0.3), 0.45, 0.2
+    floor = s.add_glossy(albedo, rough, metal)
+    s.add_rectangle(floor, position=(0, 0, 0), rotation=(90, 0, 0), scale=(4, 4, 1))
+    Le = np.array([6.0, 5.0, 4.0])
+    light = s.add_emitter(tuple(Le))
+    s.add_rectangle(light, position=(0.3, 0.8, 0.2), rotation=(-90, 0, 0), scale=(0.6, 0.4, 1))
+    cam = Camera(position=(0.0, 1.0, -1.0), look_at=(0.0, 0.0, 0.0), up=(0, 1, 0), vfov=0.005, aspect=1.0)
+    w = O.World(s, fp64=True)
+    mean, se = _pixel_stats(w, cam, 1, 1, 40000, 1)
+    # quadrature over the emitter (midpoint rule), BSDF from the oracle's glossy eval
+    rect = s.arrays()["rect"][1].astype(np.float64)
+    p0, eu, ev, ln = rect[:3], rect[3:6], rect[6:9], rect[9:12]
+    k = 200
+    a = (np.arange(k) + 0.5) / k
+    A, B = np.meshgrid(a, a, indexing="ij")
+    y = p0 + A.reshape(-1, 1) * eu + B.reshape(-1, 1) * ev
+    area = np.linalg.norm(np.cross(eu, ev))
+    x = np.zeros(3)
+    wo = _unit(np.array(cam.position) - x)
+    dv = y - x
+    d2 = (dv * dv).sum(1)
+    wi = dv / np.sqrt(d2)[:, None]
+    # local frame with z = floor normal (+y); the BSDF is isotropic so any tangent works
+    to_local = lambda v: np.stack([v[..., 0], v[..., 2], v[..., 1]], -1)
+    n = len(wi)
+    f, pdf, smp = np.zeros((n, 3), np.float32), np.zeros(n, np.float32), np.zeros((n, 3), np.float32)
+    mat = np.array([*albedo, rough, metal], np.float32)
+    L().neo_glossy_c(n, O.fp(np.tile(mat, (n, 1))), O.fp(np.tile(to_local(wo), (n, 1)).astype(np.float32)),
+                     O.fp(to_local(wi).astype(np.float32)), O.fp(np.zeros((n, 3), np.float32)), O.fp(f), O.fp(pdf),
+                     O.fp(smp))
+    cos_x = wi[:, 1]
+    cos_y = np.abs(wi @ ln)
+    Ld = (f * (cos_x * cos_y / d2)[:, None]).sum(0) * area / n * Le
+    got = mean[0, 0]
+    assert (np.abs(got - Ld) < 5 * se[0, 0] + 2e-3 * Ld).all(), (got, Ld, se)
