@@ -214,8 +214,9 @@ ne_status ne_render(ne_ctx* ctx, const ne_scene* scene, const ne_camera_desc* ca
                     ne_film* film);
 /* Tile call with the reference's tiling: 40 x 10 tiles of (W/40) x (H/10)
  * pixels, tile `index` = (index % 40, index / 40) (OfflineEngine.cpp:18-21,
- * 54-58).  Renders params->spp samples of that tile, synchronises, and
- * writes post-processed pixels of the tile into pixels (W*H*3 floats). */
+ * 54-58).  Clears the tile's film sums, renders params->spp samples of that
+ * tile, synchronises, and writes post-processed pixels of the tile into
+ * pixels (W*H*3 floats); other pixels of the array are left untouched. */
 ne_status ne_render_tile(ne_ctx* ctx, const ne_scene* scene, const ne_camera_desc* cam,
                          const ne_render_params* params, int32_t index, ne_film* film, float* pixels);
 /* Duration in milliseconds of the path-tracing kernel of the last render

@@ -19,6 +19,9 @@ PKG = Path(__file__).resolve().parent
 CSRC = PKG / "csrc"
 BUILD = PKG / "_build"
 LIB = PKG / "libnarval_gpu.so"
+ROOT = PKG.parent
+FACADE_SRC = ROOT / "tests" / "cpp" / "facade_check.cpp"
+FACADE_BIN = ROOT / "tests" / "cpp" / "facade_check"
 SOURCES = ["render.hip", "wavefront.hip", "api.cpp", "bvh_build.cpp", "generators.cpp"]
 HEADERS = ["ne_math.h", "ne_scene.h", "ne_trace.h", "ne_launch.h", "bvh_build.h"]
 
@@ -59,9 +62,22 @@ def build(verbose: bool = False) -> Path:
         sys.stderr.write(res.stdout + res.stderr)
         raise RuntimeError("link failed")
     os.replace(tmp, LIB)
+    build_facade_check()
     if verbose:
         print(f"built {LIB}")
     return LIB
+
+
+def build_facade_check() -> Path:
+    """g++ build of the C++ facade check program (include/ne_gpu.hpp over the
+    C ABI), linked against the in-tree library."""
+    cmd = ["g++", "-O2", "-std=c++17", "-Wall", f"-I{ROOT / 'include'}", str(FACADE_SRC), "-o", str(FACADE_BIN),
+           f"-L{PKG}", "-lnarval_gpu", "-Wl,-rpath,$ORIGIN/../../narvalengine_amd"]
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError("facade check build failed")
+    return FACADE_BIN
 
 
 if __name__ == "__main__":

@@ -697,6 +697,10 @@ ne_status ne_render_tile(ne_ctx* c, const ne_scene* s, const ne_camera_desc* cam
   std::lock_guard<std::mutex> lk(c->mu);
   ne_render_params q = *p;
   q.x0 = (index % 40) * tw; q.y0 = (index / 40) * th; q.x1 = q.x0 + tw; q.y1 = q.y0 + th;
+  if (f->W != p->width || f->H != p->height) return fail(c, NE_ERR_INVALID_ARG, "film size differs from params");
+  NE_HIP(c, hipSetDevice(c->device));
+  // renderTile recomputes the tile: its film sums restart from zero
+  NE_HIP(c, launch_clear_rect(f->d_sum, f->W, q.x0, q.y0, q.x1, q.y1, c->stream));
   ne_status st = render_locked(c, s, cam, &q, f);
   if (st != NE_OK) return st;
   const int npix = f->W * f->H;

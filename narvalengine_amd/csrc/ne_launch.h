@@ -105,6 +105,7 @@ struct WfArgs {
 hipError_t launch_wavefront(const DevScene& s, const DevCamera& cam, WfState& w, WfArgs a, bool count,
                             hipStream_t st, uint32_t* h_status, StageTimer* tm);
 hipError_t launch_reduce_partials(const RenderArgs& a, double* film, hipStream_t st);
+hipError_t launch_clear_rect(double* film, int W, int x0, int y0, int x1, int y1, hipStream_t st);
 hipError_t launch_tonemap(const double* film, int npix, float inv_spp, float* pixels, hipStream_t st);
 
 hipError_t launch_test_primitive(int kind, int n, const float* rays, const float* prims, int32_t* hit, float* t0,

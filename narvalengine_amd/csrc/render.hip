@@ -101,6 +101,24 @@ __global__ void k_tonemap(const double* film, int n, float inv_spp, float* pixel
   pixels[i] = xmin(xmax(m, 0.0f), 1.0f);
 }
 
+// Zeroes the film sums of the pixel rectangle [x0,x1) x [y0,y1).
+__global__ void k_clear_rect(double* film, int W, int x0, int y0, int x1, int y1) {
+  const int rw = x1 - x0;
+  const long long n = (long long)rw * (y1 - y0) * 3;
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const long long p = i / 3;
+  const int x = x0 + int(p % rw), y = y0 + int(p / rw);
+  film[(size_t(y) * size_t(W) + size_t(x)) * 3 + size_t(i % 3)] = 0.0;
+}
+
+hipError_t launch_clear_rect(double* film, int W, int x0, int y0, int x1, int y1, hipStream_t st) {
+  const long long n = (long long)(x1 - x0) * (y1 - y0) * 3;
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_clear_rect, dim3(uint32_t((n + 255) / 256)), dim3(256), 0, st, film, W, x0, y0, x1, y1);
+  return hipGetLastError();
+}
+
 static inline uint32_t pad8(uint32_t n) { return (n + kNumXcd - 1) / kNumXcd * kNumXcd; }
 
 template <typename R, bool C, int BLOCK, int WAVES = 0>
