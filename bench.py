@@ -80,31 +80,29 @@ def log(msg):
 
 
 def cpu_baseline(wl, budget_s: float):
-    """Oracle (CPU restatement, FP32, same counter RNG and estimator) on a
-    centred crop sized for about budget_s seconds, samples/s."""
+    """Oracle (CPU restatement, FP32, same counter RNG and estimator) over the
+    whole frame at the first k samples per pixel, k sized for about budget_s
+    seconds: the same distribution of paths as the GPU frame (every pixel), so
+    samples/s compare like for like.  A 1-sample pilot frame sets k."""
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle as O
     threads = max(1, min(16, os.cpu_count() or 1))
     t0 = time.perf_counter()
     w = O.World(wl.scene)
     build_s = time.perf_counter() - t0
-    cx, cy = wl.width // 2, wl.height // 2
 
-    def run(side):
-        rect = (cx - side // 2, cy - side // 2, cx - side // 2 + side, cy - side // 2 + side)
+    def run(k, offset):
         t = time.perf_counter()
-        w.render(wl.camera, wl.width, wl.height, wl.spp, wl.max_bounces, rect=rect, seed=1, threads=threads)
-        return time.perf_counter() - t, side * side * wl.spp
+        w.render(wl.camera, wl.width, wl.height, k, wl.max_bounces, seed=1, sample_offset=offset, threads=threads)
+        return time.perf_counter() - t, wl.width * wl.height * k
 
-    dt, n = run(8)
-    rate = n / max(dt, 1e-9)
-    side = int(max(8, min(256, (budget_s * rate / wl.spp) ** 0.5)))
-    side = max(8, side - side % 8)
-    dt, n = run(side)
+    dt, n = run(1, 0)
+    k = int(max(1, min(wl.spp - 1, budget_s * (n / max(dt, 1e-9)) / (wl.width * wl.height))))
+    dt, n = run(k, 1)
     return {"value": n / dt, "unit": "samples/s", "cores": threads, "kind": "port",
-            "sample": f"{side}x{side} centre crop of the {wl.width}x{wl.height} frame at {wl.spp} spp, "
-                      f"{wl.max_bounces} bounces ({n} paths, {dt:.1f} s; oracle scene build {build_s:.1f} s "
-                      f"not timed)"}
+            "sample": f"whole {wl.width}x{wl.height} frame at samples 1..{k} of {wl.spp} per pixel, "
+                      f"{wl.max_bounces} bounces ({n} paths, {dt:.1f} s; 1-sample pilot frame and oracle scene "
+                      f"build {build_s:.1f} s not timed)"}
 
 
 def copy_calibration(nbytes: int, reps: int = 5):

@@ -146,7 +146,8 @@ typedef struct ne_render_params {
   int32_t block_size;         /* 0 = library default */
   int32_t occupancy;          /* megakernel FP32 variant built for >= this many waves per SIMD: 0, 4, 6 */
   int32_t mode;               /* NE_MODE_*: 0 = auto (wavefront for FP32, megakernel for FP64) */
-  int32_t pool_size;          /* wavefront path slots (rounded up to 256); 0 = library default (2^22) */
+  int32_t pool_size;          /* wavefront path slots (rounded up to 256); 0 = library default
+                                 (2^23 for surface-only scenes, else 2^22) */
 } ne_render_params;
 
 enum { NE_MODE_AUTO = 0, NE_MODE_MEGAKERNEL = 1, NE_MODE_WAVEFRONT = 2 };

@@ -41,6 +41,7 @@ struct ne_ctx {
   std::vector<hipEvent_t> ev_pool;  // per-launch timing events (NE_FLAG_STAGE_TIME)
   std::vector<int8_t> ev_stages;    // stage of the launch after each recorded event
   bool stage_timed = false;
+  uint32_t refill_blocks = 0;  // resident lane-refill blocks on the device (CUs x blocks per CU)
 };
 
 struct ne_scene {
@@ -128,8 +129,8 @@ ne_status validate_desc(ne_ctx* c, const ne_scene_desc* d) {
   if (d->n_rect && (!d->rect || !d->rect_mat)) return fail(c, NE_ERR_INVALID_ARG, "rectangle arrays are NULL");
   if ((d->n_tri || d->n_sph || d->n_rect) && (d->n_mat == 0 || !d->mat))
     return fail(c, NE_ERR_INVALID_ARG, "surfaces need at least one material");
-  if (uint64_t(d->n_tri) + d->n_sph + d->n_rect >= (1u << 30))
-    return fail(c, NE_ERR_INVALID_ARG, "too many primitives (limit 2^30)");
+  if (d->n_tri > kItemFirstMask || d->n_sph > kItemFirstMask || d->n_rect > kItemFirstMask)
+    return fail(c, NE_ERR_INVALID_ARG, "too many primitives of one type (limit 2^26)");
   auto chk_mat = [&](int32_t m) { return m >= 0 && uint32_t(m) < d->n_mat; };
   for (uint32_t i = 0; i < d->n_tri; ++i)
     if (!chk_mat(d->tri_mat ? d->tri_mat[i] : 0)) return fail(c, NE_ERR_INVALID_ARG, "triangle material out of range");
@@ -358,6 +359,8 @@ ne_status ne_scene_upload(ne_ctx* c, const ne_scene_desc* d, ne_scene** out) {
   BvhOutput bvh;
   build_bvh(prim_bounds(d), bvh);
   if (bvh.max_depth + 1 > uint32_t(kShortStack + kSpillStack)) return bail(fail(c, NE_ERR_UNSUPPORTED, "BVH too deep"));
+  if (bvh.max_leaf > kItemMaxLeaf)
+    return bail(fail(c, NE_ERR_UNSUPPORTED, "degenerate geometry: a BVH leaf holds more than 8 primitives"));
   std::vector<float> tris(size_t(nt) * 12), sph(size_t(ns) * 4), rects(size_t(nr) * 16);
   std::vector<int32_t> sph_mat(ns), sph_orig(ns);
   std::vector<uint32_t> storage(bvh.refs.size());  // refs position -> slot within its type's array
@@ -563,8 +566,12 @@ ne_status ne_film_tonemap(ne_film* f, int32_t spp, float* pixels) {
 static ne_status render_wavefront(ne_ctx* c, const ne_scene* s, const DevCamera& dc, const ne_render_params* p,
                                   ne_film* f, bool count, bool timed, StageTimer* tm) {
   const uint64_t total = uint64_t(p->x1 - p->x0) * uint64_t(p->y1 - p->y0) * uint64_t(p->spp);
-  // pool slots: whole chunks, no more than the job needs
-  uint64_t P = p->pool_size > 0 ? uint64_t(p->pool_size) : (uint64_t(1) << 22);
+  // pool slots: whole chunks, no more than the job needs.  Default (measured
+  // on MI355X): 2^23 for surface-only scenes, where fewer, larger BVH launches
+  // amortise each launch's tail; 2^22 with a volume, where a larger pool only
+  // adds cache pressure.
+  const uint64_t def_pool = (s->dev.n_nodes > 0 && !s->dev.has_volume) ? (uint64_t(1) << 23) : (uint64_t(1) << 22);
+  uint64_t P = p->pool_size > 0 ? uint64_t(p->pool_size) : def_pool;
   P = std::min<uint64_t>(P, total);
   P = (P + kChunk - 1) / kChunk * kChunk;
   if (P > c->wf_P) {
@@ -588,6 +595,14 @@ static ne_status render_wavefront(ne_ctx* c, const ne_scene* s, const DevCamera&
   a.W = p->width; a.H = p->height; a.x0 = p->x0; a.y0 = p->y0; a.x1 = p->x1; a.y1 = p->y1;
   a.max_bounces = p->max_bounces; a.sample_offset = p->sample_offset; a.job = p->job; a.nchunks = uint32_t(P / kChunk);
   a.seed = p->seed;
+  if (c->refill_blocks == 0) {
+    int cus = 0, per_cu = 0;
+    NE_HIP(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+    NE_HIP(c, wf_refill_blocks_per_cu(&per_cu));
+    c->refill_blocks = uint32_t(std::max(1, cus * std::max(1, per_cu)));
+  }
+  // chunks per refill block: one round of resident blocks covers the pool
+  a.refill = std::min<uint32_t>(kMaxRefill, std::max<uint32_t>(1, (a.nchunks + c->refill_blocks - 1) / c->refill_blocks));
   a.total_paths = total;
   a.counters = c->d_counters;
   a.film_fixed = f->d_fixed;

@@ -49,6 +49,7 @@ hipError_t launch_render(const DevScene& s, const DevCamera& cam, const RenderAr
 // The path pool is split into chunks of kChunk slots; stage kernels run one
 // block of kChunk threads per chunk and keep each chunk's live paths packed.
 constexpr uint32_t kChunk = 256;
+constexpr uint32_t kMaxRefill = 64;  // chunks per block in the lane-refill kernels
 // stage ids (the public NE_STAGE_* values)
 enum { kStageMegakernel = 0, kStagePlan, kStageRegen, kStageTrace, kStageMedium, kStageShade, kStageShadow,
        kStageOther, kStageCount, kStageEnd = -1 };
@@ -86,12 +87,15 @@ struct WfState {
   unsigned long long* next_path;   // work counter
 };
 size_t wf_bytes(uint32_t P);
+// resident 256-thread blocks per CU of the lane-refill trace kernel
+hipError_t wf_refill_blocks_per_cu(int* blocks);
 void wf_carve(void* base, uint32_t P, WfState& w);
 
 struct WfArgs {
   int32_t W, H, x0, y0, x1, y1;
   int32_t max_bounces;
   uint32_t sample_offset, job, nchunks;
+  uint32_t refill;  // chunks per block of the lane-refill kernels, 1..kMaxRefill
   uint64_t seed;
   unsigned long long total_paths;
   int32_t cur;

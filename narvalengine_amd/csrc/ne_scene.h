@@ -31,6 +31,13 @@ enum { PRIM_TRI = 0, PRIM_SPHERE = 1, PRIM_RECT = 2 };
 constexpr int kShortStack = 16;
 constexpr int kSpillStack = 48;
 
+// Traversal work item: an inner node index, or a leaf packed as
+// kItemLeaf | type << 29 | (count - 1) << 26 | first slot.
+constexpr uint32_t kItemLeaf = 0x80000000u;
+constexpr uint32_t kItemFirstMask = (1u << 26) - 1;  // slots per primitive type < 2^26
+constexpr uint32_t kItemCountShift = 26;
+constexpr uint32_t kItemMaxLeaf = 8;  // count - 1 in 3 bits
+
 struct DevScene {
   const float4* nodes;
   const float4* tris;

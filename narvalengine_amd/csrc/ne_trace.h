@@ -183,80 +183,131 @@ struct SurfHit {
 // visited near child first; the far child goes on the lane's stack.  Leaves
 // are tested as soon as their box is hit.  Ties on t resolve to the lower
 // (type, original index) key, as the oracle's linear scan does.
+// Resumable BVH traversal: the per-lane state of one ray's closest-hit (or
+// any-hit) query.  The current work item is either an inner node (its index)
+// or a leaf (kItemLeaf | type << 29 | (count - 1) << 26 | first slot); one
+// trav_step visits one node or tests one primitive, so kernels can interleave
+// rays of different lengths in one wave (lane refill) and a wave never pays
+// for a whole leaf's tests on every node step.  traverse() runs it to
+// completion.  Closest-hit results do not depend on the visiting order: ties
+// at equal t break on (type, original index).
+template <typename R>
+struct TravState {
+  Vec3<R> o, d, invd;
+  R tmin, tmax, best;
+  uint32_t cur, bkey, bslot;
+  int sp, btype;
+  bool found;
+};
+
+template <typename R>
+NE_DEV void trav_init(TravState<R>& st, Vec3<R> o, Vec3<R> d, R tmin, R tmax) {
+  st.o = o;
+  st.d = d;
+  st.invd = v3(R(1) / d.x, R(1) / d.y, R(1) / d.z);
+  st.tmin = tmin;
+  st.tmax = tmax;
+  st.best = tmax;
+  st.cur = 0;
+  st.sp = 0;
+  st.found = false;
+  st.btype = 0;
+  st.bkey = 0;
+  st.bslot = 0;
+}
+
+// leaf child (first slot, count word = count | type << 28) -> work item
+NE_DEV uint32_t leaf_item(uint32_t first, uint32_t cw) {
+  return kItemLeaf | ((cw >> 28) << 29) | (((cw & 0x0FFFFFFFu) - 1u) << kItemCountShift) | first;
+}
+
+// Next item from the stack; true when the stack is empty (query complete).
+template <typename R>
+NE_DEV bool trav_pop(TravState<R>& st, const uint32_t* lds, int lane_stride, const uint32_t* spill) {
+  if (st.sp == 0) return true;
+  --st.sp;
+  st.cur = st.sp < kShortStack ? lds[st.sp * lane_stride] : spill[st.sp - kShortStack];
+  return false;
+}
+
+// One node visit or one primitive test.  Returns true when the query is
+// complete: the stack is empty, or (ANY) a hit was found.  lds: this lane's
+// short stack (entries lane_stride apart); spill: its private overflow array.
+template <typename R, bool ANY, bool C>
+NE_DEV bool trav_step(const DevScene& s, TravState<R>& st, uint32_t* lds, int lane_stride, uint32_t* spill,
+                      Count<C>& cnt) {
+  const uint32_t it = st.cur;
+  if (it & kItemLeaf) {
+    const uint32_t ty = (it >> 29) & 3u, k = it & kItemFirstMask;
+    R t;
+    bool ok;
+    uint32_t key = 0;
+    cnt.prim();
+    if (ty == PRIM_TRI) {
+      const float4* tp = s.tris + 3 * size_t(k);
+      ok = isect_tri<R>(tp, st.o, st.d, st.tmin, st.tmax, t);
+      if (ok) key = __float_as_uint(tp[1].w);
+    } else if (ty == PRIM_SPHERE) {
+      ok = isect_sphere<R>(s.sph[k], st.o, st.d, st.tmin, st.tmax, t);
+      if (ok) key = uint32_t(s.sph_orig[k]);
+    } else {
+      const float4* rc = s.rects + 4 * size_t(k);
+      ok = isect_rect<R>(rc, st.o, st.d, st.tmin, st.tmax, t);
+      if (ok) key = __float_as_uint(rc[2].w);
+    }
+    if (ok) {
+      if (ANY) {
+        st.found = true;
+        return true;
+      }
+      if (t < st.best ||
+          (st.found && t == st.best && (int(ty) < st.btype || (int(ty) == st.btype && key < st.bkey)))) {
+        st.best = t; st.found = true; st.btype = int(ty); st.bkey = key; st.bslot = k;
+      }
+    }
+    if (it & (7u << kItemCountShift)) {  // more primitives in this leaf
+      st.cur = it + 1u - (1u << kItemCountShift);
+      return false;
+    }
+    return trav_pop(st, lds, lane_stride, spill);
+  }
+  const float4* nd = s.nodes + 4 * size_t(it);
+  const float4 f0 = nd[0], f1 = nd[1], f2 = nd[2];
+  const int4 f3 = reinterpret_cast<const int4*>(nd)[3];
+  cnt.node();
+  R tl, tr;
+  const bool hl = f3.x >= 0 && slab<R>(f0.x, f0.y, f0.z, f0.w, f1.x, f1.y, st.o, st.invd, st.best, tl);
+  const bool hr = f3.y >= 0 && slab<R>(f1.z, f1.w, f2.x, f2.y, f2.z, f2.w, st.o, st.invd, st.best, tr);
+  const uint32_t il = f3.z ? leaf_item(uint32_t(f3.x), uint32_t(f3.z)) : uint32_t(f3.x);
+  const uint32_t ir = f3.w ? leaf_item(uint32_t(f3.y), uint32_t(f3.w)) : uint32_t(f3.y);
+  if (hl && hr) {
+    const bool lnear = tl <= tr;
+    const uint32_t far = lnear ? ir : il;
+    if (st.sp < kShortStack) lds[st.sp * lane_stride] = far;
+    else spill[st.sp - kShortStack] = far;
+    ++st.sp;
+    st.cur = lnear ? il : ir;
+  } else if (hl) {
+    st.cur = il;
+  } else if (hr) {
+    st.cur = ir;
+  } else {
+    return trav_pop(st, lds, lane_stride, spill);
+  }
+  return false;
+}
+
 template <typename R, bool ANY, bool C>
 NE_DEV bool traverse(const DevScene& s, Vec3<R> o, Vec3<R> d, R tmin, R tmax, uint32_t* lds, int lane_stride,
                      SurfHit<R>& hit, Count<C>& cnt) {
   if (s.n_nodes == 0) return false;
-  const Vec3<R> invd = v3(R(1) / d.x, R(1) / d.y, R(1) / d.z);
   uint32_t spill[kSpillStack];
-  int sp = 0;
-  uint32_t node = 0;
-  R best = tmax;
-  bool found = false;
-  int btype = 0;
-  uint32_t bkey = 0, bslot = 0;
-  while (true) {
-    const float4* nd = s.nodes + 4 * size_t(node);
-    const float4 f0 = nd[0], f1 = nd[1], f2 = nd[2];
-    const int4 f3 = reinterpret_cast<const int4*>(nd)[3];
-    cnt.node();
-    R tl, tr;
-    bool hl = f3.x >= 0 && slab<R>(f0.x, f0.y, f0.z, f0.w, f1.x, f1.y, o, invd, best, tl);
-    bool hr = f3.y >= 0 && slab<R>(f1.z, f1.w, f2.x, f2.y, f2.z, f2.w, o, invd, best, tr);
-#pragma unroll
-    for (int side = 0; side < 2; ++side) {
-      const bool h = side == 0 ? hl : hr;
-      const uint32_t ct = uint32_t(side == 0 ? f3.z : f3.w);
-      if (!(h && ct != 0u)) continue;
-      const uint32_t ty = ct >> 28, n = ct & 0x0FFFFFFFu;
-      const uint32_t first = uint32_t(side == 0 ? f3.x : f3.y);
-      for (uint32_t k = first; k < first + n; ++k) {
-        R t;
-        bool ok;
-        uint32_t key = 0;
-        cnt.prim();
-        if (ty == PRIM_TRI) {
-          const float4* tp = s.tris + 3 * size_t(k);
-          ok = isect_tri<R>(tp, o, d, tmin, tmax, t);
-          if (ok) key = __float_as_uint(tp[1].w);
-        } else if (ty == PRIM_SPHERE) {
-          ok = isect_sphere<R>(s.sph[k], o, d, tmin, tmax, t);
-          if (ok) key = uint32_t(s.sph_orig[k]);
-        } else {
-          const float4* rc = s.rects + 4 * size_t(k);
-          ok = isect_rect<R>(rc, o, d, tmin, tmax, t);
-          if (ok) key = __float_as_uint(rc[2].w);
-        }
-        if (ok) {
-          if (ANY) return true;
-          if (t < best || (found && t == best && (int(ty) < btype || (int(ty) == btype && key < bkey)))) {
-            best = t; found = true; btype = int(ty); bkey = key; bslot = k;
-          }
-        }
-      }
-      if (side == 0) hl = false; else hr = false;
-    }
-    if (hl && tl > best) hl = false;
-    if (hr && tr > best) hr = false;
-    if (hl && hr) {
-      const uint32_t near = tl <= tr ? uint32_t(f3.x) : uint32_t(f3.y);
-      const uint32_t far = tl <= tr ? uint32_t(f3.y) : uint32_t(f3.x);
-      if (sp < kShortStack) lds[sp * lane_stride] = far;
-      else spill[sp - kShortStack] = far;
-      ++sp;
-      node = near;
-    } else if (hl) {
-      node = uint32_t(f3.x);
-    } else if (hr) {
-      node = uint32_t(f3.y);
-    } else {
-      if (sp == 0) break;
-      --sp;
-      node = sp < kShortStack ? lds[sp * lane_stride] : spill[sp - kShortStack];
-    }
+  TravState<R> st;
+  trav_init(st, o, d, tmin, tmax);
+  while (!trav_step<R, ANY, C>(s, st, lds, lane_stride, spill, cnt)) {
   }
-  if (!found) return false;
-  hit.t = best; hit.type = btype; hit.key = bkey; hit.slot = bslot;
+  if (!st.found) return false;
+  hit.t = st.best; hit.type = st.btype; hit.key = st.bkey; hit.slot = st.bslot;
   return true;
 }
 

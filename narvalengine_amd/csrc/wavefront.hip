@@ -200,23 +200,96 @@ __global__ __launch_bounds__(kChunk) void k_wf_regen(WfState w, WfArgs a, DevCam
   if (t == 0) w.cnt_paths[a.cur][c] = n0 + m;
 }
 
+// ---- lane refill -------------------------------------------------------------
+// Traversal lengths are heavy-tailed (rays grazing dense geometry visit many
+// nodes), so a wave that holds one ray per lane waits for its slowest lane.
+// The BVH kernels below give each 256-thread block a pool of a.refill chunks
+// (sized on the host so the grid is one round of resident blocks) and run one
+// flattened loop: every iteration, lanes without a ray take the next one from
+// the pool (one LDS atomic per wave) and every lane with a ray does one node
+// visit.  A lane leaves the loop only when the pool is empty.
+
+// Per-block pool over the chunks blockIdx.x + j * gridDim.x (j < n): chunks
+// map to image regions, so interleaving spreads expensive regions over all
+// blocks.  Items are numbered chunk by chunk, pre[j] = first item of chunk j,
+// pre[n] = total.
+struct RefillPool {
+  uint32_t pre[kMaxRefill + 1];
+  uint32_t n, next;
+};
+
+__device__ __forceinline__ uint32_t pool_chunk(uint32_t j) { return blockIdx.x + j * gridDim.x; }
+
+__device__ __forceinline__ void pool_init(RefillPool& p, const uint32_t* cnt, uint32_t nchunks) {
+  if (threadIdx.x == 0) {
+    uint32_t n = 0, acc = 0;
+    while (n < kMaxRefill && pool_chunk(n) < nchunks) {
+      p.pre[n] = acc;
+      acc += cnt[pool_chunk(n)];
+      ++n;
+    }
+    p.pre[n] = acc;
+    p.n = n;
+    p.next = 0;
+  }
+  __syncthreads();
+}
+
+// Lanes with need == true take consecutive items (wave-aggregated LDS atomic).
+// Returns the slot (chunk * kChunk + position) or kNoHit when the pool is empty.
+// Every lane still in the caller's loop must call it.
+__device__ __forceinline__ uint32_t pool_fetch(RefillPool& p, bool need) {
+  const unsigned long long m = __ballot(need);
+  if (m == 0) return kNoHit;
+  const int lane = __lane_id();
+  const int leader = __ffsll((long long)m) - 1;
+  uint32_t base = 0;
+  if (lane == leader) base = atomicAdd(&p.next, uint32_t(__popcll(m)));
+  base = __shfl(base, leader);
+  if (!need) return kNoHit;
+  const uint32_t k = base + uint32_t(__popcll(m & ((1ull << lane) - 1ull)));
+  const uint32_t n = p.n;
+  if (k >= p.pre[n]) return kNoHit;
+  // largest j < n with pre[j] <= k (pre is non-decreasing, pre[0] = 0)
+  uint32_t lo = 0, hi = n - 1;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) >> 1;
+    if (p.pre[mid] <= k) lo = mid; else hi = mid - 1;
+  }
+  return pool_chunk(lo) * kChunk + (k - p.pre[lo]);
+}
+
 // Closest surface hit of every live ray: pure BVH traversal (no random
-// numbers, few live registers).
+// numbers), lane refill over the block's pool.
 template <bool C>
 __global__ __launch_bounds__(kChunk) void k_wf_trace(WfState w, WfArgs a, DevScene s) {
   __shared__ uint32_t lds_stack[kShortStack * kChunk];
   __shared__ uint32_t red[kChunk / 64];
+  __shared__ RefillPool pool;
   const WfPaths A = w.paths[a.cur];
-  const uint32_t c = blockIdx.x, t = threadIdx.x, i = c * kChunk + t;
+  pool_init(pool, w.cnt_paths[a.cur], a.nchunks);
+  uint32_t* stk = lds_stack + threadIdx.x;
+  uint32_t spill[kSpillStack];
   Count<C> cnt;
-  if (t < w.cnt_paths[a.cur][c]) {
-    const Vec3<float> o = v3(A.ox[i], A.oy[i], A.oz[i]), d = v3(A.dx[i], A.dy[i], A.dz[i]);
-    cnt.ext();
-    SurfHit<float> hit;
-    const bool hs = traverse<float, false, C>(s, o, d, 1e-4f, xinf<float>(), lds_stack + t, kChunk, hit, cnt);
-    A.hit_t[i] = hs ? hit.t : xinf<float>();
-    A.hit_ref[i] = hs ? ((uint32_t(hit.type) << 30) | hit.slot) : kNoHit;
-    A.kind[i] = hs ? uint32_t(WF_KIND_SURFACE) : uint32_t(WF_KIND_MISS);
+  TravState<float> st;
+  uint32_t slot = kNoHit;
+  bool have = false;
+  while (true) {
+    const uint32_t got = pool_fetch(pool, !have);  // every lane still looping calls it
+    if (!have) {
+      if (got == kNoHit) break;
+      slot = got;
+      have = true;
+      cnt.ext();
+      trav_init(st, v3(A.ox[slot], A.oy[slot], A.oz[slot]), v3(A.dx[slot], A.dy[slot], A.dz[slot]), 1e-4f,
+                xinf<float>());
+    }
+    if (trav_step<float, false, C>(s, st, stk, kChunk, spill, cnt)) {
+      A.hit_t[slot] = st.found ? st.best : xinf<float>();
+      A.hit_ref[slot] = st.found ? ((uint32_t(st.btype) << 30) | st.bslot) : kNoHit;
+      A.kind[slot] = st.found ? uint32_t(WF_KIND_SURFACE) : uint32_t(WF_KIND_MISS);
+      have = false;
+    }
   }
   if constexpr (C) {
     block_count<C>(&a.counters->ext_rays, cnt.n_ext, red);
@@ -474,6 +547,58 @@ __global__ __launch_bounds__(kChunk) void k_wf_shade(WfState w, WfArgs a, DevSce
   }
 }
 
+// NEE contribution of an unoccluded shadow ray k (transmittance tr): to the
+// continuing path in the next buffer (same chunk) or straight to the film.
+__device__ __forceinline__ void shadow_deliver(const WfShadow& S, const WfPaths& B, const WfArgs& a,
+                                               const DevScene& s, uint32_t k, float tr) {
+  const float q = tr * S.wt[k] * float(s.n_lights) / S.pl[k];
+  const float cx = S.cx[k] * q, cy = S.cy[k] * q, cz = S.cz[k] * q;
+  const uint32_t tg = S.target[k];
+  if (tg & kToFilm) {
+    film_add(a.film_fixed, tg & ~kToFilm, cx, cy, cz);
+  } else {
+    const uint32_t b = (k / kChunk) * kChunk + tg;
+    B.Lx[b] += cx; B.Ly[b] += cy; B.Lz[b] += cz;
+  }
+}
+
+// Shadow rays of scenes without a volume: any-hit BVH occlusion with lane
+// refill over the block's pool of shadow-queue chunks.
+template <bool C>
+__global__ __launch_bounds__(kChunk) void k_wf_shadow_surf(WfState w, WfArgs a, DevScene s) {
+  __shared__ uint32_t lds_stack[kShortStack * kChunk];
+  __shared__ uint32_t red[kChunk / 64];
+  __shared__ RefillPool pool;
+  const WfShadow S = w.shadow;
+  const WfPaths B = w.paths[a.cur ^ 1];
+  pool_init(pool, w.cnt_shadow, a.nchunks);
+  uint32_t* stk = lds_stack + threadIdx.x;
+  uint32_t spill[kSpillStack];
+  Count<C> cnt;
+  TravState<float> st;
+  uint32_t k = kNoHit;
+  bool have = false;
+  while (true) {
+    const uint32_t got = pool_fetch(pool, !have);  // every lane still looping calls it
+    if (!have) {
+      if (got == kNoHit) break;
+      k = got;
+      have = true;
+      cnt.shadow();
+      trav_init(st, v3(S.ox[k], S.oy[k], S.oz[k]), v3(S.dx[k], S.dy[k], S.dz[k]), 1e-4f, S.tmax[k]);
+    }
+    if (trav_step<float, true, C>(s, st, stk, kChunk, spill, cnt)) {
+      if (!st.found) shadow_deliver(S, B, a, s, k, 1.0f);
+      have = false;
+    }
+  }
+  if constexpr (C) {
+    block_count<C>(&a.counters->shadow_rays, cnt.n_shadow, red);
+    block_count<C>(&a.counters->sh_nodes, cnt.n_node, red);
+    block_count<C>(&a.counters->sh_prims, cnt.n_prim, red);
+  }
+}
+
 // Shadow rays: any-hit occlusion (HAS_SURF) then ratio tracking (HAS_VOL),
 // the megakernel's transmittance() split by scene content.
 template <bool C, bool HAS_SURF, bool HAS_VOL>
@@ -505,17 +630,7 @@ __global__ __launch_bounds__(kChunk) void k_wf_shadow(WfState w, WfArgs a, DevSc
         if (ta < tb) tr = ratio_track<float, C>(s, x, wi, ta, tb, srng, cnt);
       }
     }
-    if (tr > 0.0f) {
-      const float q = tr * S.wt[k] * float(s.n_lights) / S.pl[k];
-      const float cx = S.cx[k] * q, cy = S.cy[k] * q, cz = S.cz[k] * q;
-      const uint32_t tg = S.target[k];
-      if (tg & kToFilm) {
-        film_add(a.film_fixed, tg & ~kToFilm, cx, cy, cz);
-      } else {
-        const uint32_t b = c * kChunk + tg;
-        B.Lx[b] += cx; B.Ly[b] += cy; B.Lz[b] += cz;
-      }
-    }
+    if (tr > 0.0f) shadow_deliver(S, B, a, s, k, tr);
   }
   if constexpr (C) {
     block_count<C>(&a.counters->shadow_rays, cnt.n_shadow, red);
@@ -577,6 +692,11 @@ void wf_carve(void* base, uint32_t P, WfState& w) {
   w.next_path = reinterpret_cast<unsigned long long*>(p);
 }
 
+hipError_t wf_refill_blocks_per_cu(int* blocks) {
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, reinterpret_cast<const void*>(&k_wf_trace<false>),
+                                                      int(kChunk), 0);
+}
+
 hipError_t launch_wavefront(const DevScene& s, const DevCamera& cam, WfState& w, WfArgs a, bool count,
                             hipStream_t st, uint32_t* h_status, StageTimer* tm) {
   hipError_t e = hipSuccess;
@@ -588,6 +708,7 @@ hipError_t launch_wavefront(const DevScene& s, const DevCamera& cam, WfState& w,
   NE_MARK(kStageOther);
   hipLaunchKernelGGL(k_wf_init, dim3((NC + 255) / 256), dim3(256), 0, st, w, NC);
   const dim3 grid = dim3(NC);
+  const dim3 pgrid = dim3((NC + a.refill - 1) / a.refill);  // lane-refill kernels
   const dim3 blk = dim3(unsigned(kChunk));
   if (!count) a.counters = nullptr;
   const bool hs = s.n_nodes > 0, hv = s.has_volume != 0;
@@ -605,8 +726,8 @@ hipError_t launch_wavefront(const DevScene& s, const DevCamera& cam, WfState& w,
     hipLaunchKernelGGL(k_wf_regen, grid, blk, 0, st, w, a, cam);
     if (hs) {
       NE_MARK(kStageTrace);
-      if (count) hipLaunchKernelGGL(k_wf_trace<true>, grid, blk, 0, st, w, a, s);
-      else hipLaunchKernelGGL(k_wf_trace<false>, grid, blk, 0, st, w, a, s);
+      if (count) hipLaunchKernelGGL(k_wf_trace<true>, pgrid, blk, 0, st, w, a, s);
+      else hipLaunchKernelGGL(k_wf_trace<false>, pgrid, blk, 0, st, w, a, s);
     }
     if (hv) {
       NE_MARK(kStageMedium);
@@ -626,11 +747,14 @@ hipError_t launch_wavefront(const DevScene& s, const DevCamera& cam, WfState& w,
     hipLaunchKernelGGL(k_wf_shade, grid, blk, 0, st, w, a, s);
     NE_MARK(kStageShadow);
 #define NE_SHADOW(CNT, S_, V_) hipLaunchKernelGGL((k_wf_shadow<CNT, S_, V_>), grid, blk, 0, st, w, a, s)
-    if (count) {
-      if (hs && hv) NE_SHADOW(true, true, true); else if (hs) NE_SHADOW(true, true, false);
+    if (hs && !hv) {
+      if (count) hipLaunchKernelGGL(k_wf_shadow_surf<true>, pgrid, blk, 0, st, w, a, s);
+      else hipLaunchKernelGGL(k_wf_shadow_surf<false>, pgrid, blk, 0, st, w, a, s);
+    } else if (count) {
+      if (hs) NE_SHADOW(true, true, true);
       else if (hv) NE_SHADOW(true, false, true); else NE_SHADOW(true, false, false);
     } else {
-      if (hs && hv) NE_SHADOW(false, true, true); else if (hs) NE_SHADOW(false, true, false);
+      if (hs) NE_SHADOW(false, true, true);
       else if (hv) NE_SHADOW(false, false, true); else NE_SHADOW(false, false, false);
     }
 #undef NE_SHADOW
