@@ -124,8 +124,8 @@ constexpr int kMaxVertices = 2047;
 
 struct PathRng {
   uint32_t pixel, sample, block, job, k0, k1;
-  uint32_t buf[4];
-  int idx;
+  uint32_t b0, b1, b2, b3;  // undrawn words of the current block, next first
+  int idx;                  // words of the current block drawn (4: none buffered)
   NE_HD void init(uint64_t seed, uint32_t px, uint32_t smp, uint32_t jb, uint32_t block0 = 0) {
     pixel = px; sample = smp; block = block0; job = jb;
     k0 = uint32_t(seed); k1 = uint32_t(seed >> 32);
@@ -137,7 +137,7 @@ struct PathRng {
     idx = 4;
     if (n & 3u) {
       refill();
-      idx = int(n & 3u);
+      for (uint32_t k = 0; k < (n & 3u); ++k) shift();
     }
   }
   // Draws consumed so far (main stream).
@@ -149,16 +149,22 @@ struct PathRng {
     return r;
   }
   NE_HD void refill() {
-    buf[0] = pixel; buf[1] = sample; buf[2] = block; buf[3] = job;
-    philox10(buf, k0, k1);
+    uint32_t c[4] = {pixel, sample, block, job};
+    philox10(c, k0, k1);
+    b0 = c[0]; b1 = c[1]; b2 = c[2]; b3 = c[3];
     block += 1;
     idx = 0;
   }
+  // Drops the next buffered word.  The buffer is a queue of scalars (never an
+  // indexed array) so it stays in registers.
+  NE_HD void shift() {
+    b0 = b1; b1 = b2; b2 = b3;
+    idx += 1;
+  }
   NE_HD uint32_t next_u32() {
     if (idx == 4) refill();
-    // branch-free select keeps buf in registers
-    uint32_t r = idx == 0 ? buf[0] : (idx == 1 ? buf[1] : (idx == 2 ? buf[2] : buf[3]));
-    idx += 1;
+    const uint32_t r = b0;
+    shift();
     return r;
   }
   NE_HD float next() { return float(next_u32() >> 8) * (1.0f / 16777216.0f); }
