@@ -236,6 +236,43 @@ ne_status ne_get_counters(ne_ctx* ctx, ne_counters* out);
 ne_status ne_li(ne_ctx* ctx, const ne_scene* scene, int32_t n, const float* rays, int32_t max_bounces, uint64_t seed,
                 uint32_t sample_index, float* radiance);
 
+/* ---- batched rendering ----------------------------------------------------
+ * Many render jobs of uneven size (own scene, camera, image size, spp) in one
+ * persistent kernel launch, captured in a hipGraph.  The reference renders one
+ * scene per OfflineEngine (src/core/OfflineEngine.cpp:8-25, 78-145); this
+ * entry point has no counterpart there.  Work items (a tile of one job x a
+ * range of samples) go through one global queue; per-job sums are 64-bit
+ * fixed point (2^-30 units), so a job's result is bitwise independent of the
+ * batch it runs in and of the item shape.  Scenes must outlive the batch. */
+typedef struct ne_job {
+  const ne_scene* scene;
+  ne_camera_desc camera;
+  int32_t width, height;
+  int32_t spp;
+  int32_t max_bounces;
+  uint64_t seed;
+  uint32_t job; /* RNG stream word: paths are keyed (seed; pixel, sample, job) */
+  uint32_t reserved;
+} ne_job;
+/* Work-item shape: tile_w x tile_h pixels x samples_per_item samples. */
+typedef struct ne_batch_config {
+  int32_t tile_w, tile_h, samples_per_item, reserved;
+} ne_batch_config;
+typedef struct ne_batch ne_batch;
+ne_status ne_batch_create(ne_ctx* ctx, const ne_job* jobs, int32_t n, ne_batch** out);
+void ne_batch_release(ne_batch* batch);
+/* Times each candidate item shape on a calibration run (every job at
+ * ceil(spp / spp_divisor) samples), keeps the fastest for later runs and
+ * returns it with its time. */
+ne_status ne_batch_autotune(ne_batch* batch, int32_t spp_divisor, ne_batch_config* best, float* best_ms);
+/* Renders every job at full spp in one graph launch (cfg NULL: the tuned
+ * shape, else 8 x 8 x 4) and waits; ms = launch duration (HIP events). */
+ne_status ne_batch_run(ne_batch* batch, const ne_batch_config* cfg, float* ms);
+/* Completion time of each job in the last run, ms from the launch start. */
+ne_status ne_batch_job_times(ne_batch* batch, float* ms);
+/* Linear radiance sums of job `job` (width * height * 3 doubles). */
+ne_status ne_batch_read(ne_batch* batch, int32_t job, double* sum_rgb);
+
 /* ---- per-function parity entry points (device code, host buffers) ------- */
 enum {
   NE_TEST_AABB = 0,     /* AABB::intersect      src/primitives/AABB.cpp:48-79      prim: min xyz, max xyz   */

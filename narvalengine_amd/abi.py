@@ -76,6 +76,19 @@ class NarvalError(RuntimeError):
 
 
 # (name, restype, argtypes) for every entry point declared in include/ne_gpu.h
+class Job(C.Structure):
+    """ne_job: one render job of a batch."""
+    _fields_ = [("scene", C.c_void_p), ("camera", CameraDesc), ("width", C.c_int32), ("height", C.c_int32),
+                ("spp", C.c_int32), ("max_bounces", C.c_int32), ("seed", C.c_uint64), ("job", C.c_uint32),
+                ("reserved", C.c_uint32)]
+
+
+class BatchConfig(C.Structure):
+    """ne_batch_config: work-item shape."""
+    _fields_ = [("tile_w", C.c_int32), ("tile_h", C.c_int32), ("samples_per_item", C.c_int32),
+                ("reserved", C.c_int32)]
+
+
 SIGNATURES = [
     ("ne_ctx_create", C.c_int32, [C.c_int32, C.POINTER(C.c_void_p)]),
     ("ne_ctx_destroy", None, [C.c_void_p]),
@@ -108,6 +121,12 @@ SIGNATURES = [
     ("ne_gen_fbm_grid", C.c_int32, [C.c_int32, C.c_uint64, C.c_float, _F]),
     ("ne_gen_cumulus_grid", C.c_int32, [C.c_int32, C.c_uint64, _F]),
     ("ne_gen_icosphere", C.c_int32, [C.c_int32, C.c_uint64, C.c_float, _F, _U, _U, _U]),
+    ("ne_batch_create", C.c_int32, [C.c_void_p, C.POINTER(Job), C.c_int32, C.POINTER(C.c_void_p)]),
+    ("ne_batch_release", None, [C.c_void_p]),
+    ("ne_batch_autotune", C.c_int32, [C.c_void_p, C.c_int32, C.POINTER(BatchConfig), _F]),
+    ("ne_batch_run", C.c_int32, [C.c_void_p, C.POINTER(BatchConfig), _F]),
+    ("ne_batch_job_times", C.c_int32, [C.c_void_p, _F]),
+    ("ne_batch_read", C.c_int32, [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]),
 ]
 
 _LIB = None

@@ -932,3 +932,250 @@ ne_status ne_test_rng(ne_ctx* c, uint64_t seed, uint32_t pixel, uint32_t sample,
 }
 
 }  // extern "C"
+
+// ---- batched rendering -------------------------------------------------------
+struct ne_batch {
+  ne_ctx* ctx = nullptr;
+  std::vector<ne_job> jobs;
+  std::vector<size_t> film_off;   // per job, in long long words
+  size_t film_words = 0;
+  long long* d_films = nullptr;
+  BatchJob* d_jobs = nullptr;
+  unsigned long long* d_misc = nullptr;  // [0] queue counter, [1] start time
+  float* d_job_ms = nullptr;
+  unsigned long long* d_item_time = nullptr;
+  size_t item_cap = 0;
+  ne_batch_config tuned{8, 8, 4, 0};
+  hipGraphExec_t exec = nullptr;
+  ne_batch_config exec_cfg{};
+  int exec_div = 0;
+  hipEvent_t ev0 = nullptr, ev1 = nullptr;
+  std::vector<float> job_ms;
+  uint32_t grid_blocks = 0;
+  double ticks_per_ms = 0;
+};
+
+namespace {
+
+void batch_free(ne_batch* b) {
+  if (b->exec) (void)hipGraphExecDestroy(b->exec);
+  if (b->d_films) (void)hipFree(b->d_films);
+  if (b->d_jobs) (void)hipFree(b->d_jobs);
+  if (b->d_misc) (void)hipFree(b->d_misc);
+  if (b->d_job_ms) (void)hipFree(b->d_job_ms);
+  if (b->d_item_time) (void)hipFree(b->d_item_time);
+  if (b->ev0) (void)hipEventDestroy(b->ev0);
+  if (b->ev1) (void)hipEventDestroy(b->ev1);
+}
+
+bool cfg_valid(const ne_batch_config& c) {
+  return c.tile_w >= 1 && c.tile_h >= 1 && c.tile_w * c.tile_h <= 4096 && c.samples_per_item >= 1 &&
+         c.samples_per_item <= 65536;
+}
+
+// Builds the job table for an item shape and sample divisor and (re)captures
+// the launch sequence into a graph when either changed.
+ne_status batch_prepare(ne_batch* b, const ne_batch_config& cfg, int div) {
+  ne_ctx* c = b->ctx;
+  std::vector<BatchJob> table(b->jobs.size());
+  unsigned long long items = 0;
+  for (size_t j = 0; j < b->jobs.size(); ++j) {
+    const ne_job& J = b->jobs[j];
+    BatchJob& T = table[j];
+    T.scene = J.scene->dev;
+    T.cam = make_camera(&J.camera);
+    T.W = J.width; T.H = J.height;
+    T.spp = (J.spp + div - 1) / div;
+    T.max_bounces = J.max_bounces;
+    T.seed = J.seed;
+    T.job = J.job;
+    T.tiles_x = uint32_t((J.width + cfg.tile_w - 1) / cfg.tile_w);
+    T.tiles_y = uint32_t((J.height + cfg.tile_h - 1) / cfg.tile_h);
+    T.chunks = uint32_t((T.spp + cfg.samples_per_item - 1) / cfg.samples_per_item);
+    T.first_item = items;
+    T.film = b->d_films + b->film_off[j];
+    items += (unsigned long long)T.tiles_x * T.tiles_y * T.chunks;
+  }
+  if (items > b->item_cap) {
+    NE_HIP(c, hipStreamSynchronize(c->stream));
+    if (b->d_item_time) NE_HIP(c, hipFree(b->d_item_time));
+    b->d_item_time = nullptr;
+    b->item_cap = 0;
+    if (b->exec) {
+      NE_HIP(c, hipGraphExecDestroy(b->exec));
+      b->exec = nullptr;
+    }
+    NE_HIP(c, hipMalloc(&b->d_item_time, items * sizeof(unsigned long long)));
+    b->item_cap = items;
+  }
+  NE_HIP(c, hipMemcpyAsync(b->d_jobs, table.data(), table.size() * sizeof(BatchJob), hipMemcpyHostToDevice,
+                           c->stream));
+  const bool same = b->exec && b->exec_div == div && b->exec_cfg.tile_w == cfg.tile_w &&
+                    b->exec_cfg.tile_h == cfg.tile_h && b->exec_cfg.samples_per_item == cfg.samples_per_item;
+  if (same) return NE_OK;
+  if (b->exec) {
+    NE_HIP(c, hipStreamSynchronize(c->stream));
+    NE_HIP(c, hipGraphExecDestroy(b->exec));
+    b->exec = nullptr;
+  }
+  BatchArgs a{};
+  a.jobs = b->d_jobs;
+  a.njobs = int32_t(b->jobs.size());
+  a.tile_w = uint32_t(cfg.tile_w); a.tile_h = uint32_t(cfg.tile_h); a.spi = uint32_t(cfg.samples_per_item);
+  a.total_items = items;
+  a.next_item = b->d_misc;
+  a.start_time = b->d_misc + 1;
+  a.item_time = b->d_item_time;
+  a.job_ms = b->d_job_ms;
+  a.films = b->d_films;
+  // the job table copy above is ordinary stream work; capture starts after it
+  NE_HIP(c, hipStreamSynchronize(c->stream));
+  hipGraph_t graph = nullptr;
+  NE_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+  hipError_t e = launch_batch(a, b->film_words, b->grid_blocks, b->ticks_per_ms, c->stream);
+  hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
+  if (e != hipSuccess || e2 != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    return fail(c, NE_ERR_HIP, std::string("batch graph capture failed: ") + hipGetErrorString(e != hipSuccess ? e : e2));
+  }
+  e = hipGraphInstantiate(&b->exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) {
+    b->exec = nullptr;
+    return fail(c, NE_ERR_HIP, std::string("batch graph instantiation failed: ") + hipGetErrorString(e));
+  }
+  b->exec_cfg = cfg;
+  b->exec_div = div;
+  return NE_OK;
+}
+
+ne_status batch_launch(ne_batch* b, const ne_batch_config& cfg, int div, float* ms) {
+  ne_ctx* c = b->ctx;
+  ne_status st = batch_prepare(b, cfg, div);
+  if (st != NE_OK) return st;
+  NE_HIP(c, hipEventRecord(b->ev0, c->stream));
+  NE_HIP(c, hipGraphLaunch(b->exec, c->stream));
+  NE_HIP(c, hipEventRecord(b->ev1, c->stream));
+  NE_HIP(c, hipEventSynchronize(b->ev1));
+  float t = 0;
+  NE_HIP(c, hipEventElapsedTime(&t, b->ev0, b->ev1));
+  if (ms) *ms = t;
+  b->job_ms.assign(b->jobs.size(), 0.0f);
+  NE_HIP(c, hipMemcpy(b->job_ms.data(), b->d_job_ms, b->jobs.size() * sizeof(float), hipMemcpyDeviceToHost));
+  return NE_OK;
+}
+
+}  // namespace
+
+ne_status ne_batch_create(ne_ctx* c, const ne_job* jobs, int32_t n, ne_batch** out) {
+  if (!c || !out) return fail(c, NE_ERR_INVALID_ARG, "ctx/out is NULL");
+  *out = nullptr;
+  if (n < 1 || n > kMaxBatchJobs || !jobs) return fail(c, NE_ERR_INVALID_ARG, "job count must be in [1, 1024]");
+  for (int32_t j = 0; j < n; ++j) {
+    const ne_job& J = jobs[j];
+    if (!J.scene || J.scene->ctx != c) return fail(c, NE_ERR_INVALID_ARG, "job scene is NULL or from another context");
+    if (J.width <= 0 || J.height <= 0 || J.width > 32768 || J.height > 32768)
+      return fail(c, NE_ERR_INVALID_ARG, "job image size out of range");
+    if (J.spp <= 0) return fail(c, NE_ERR_INVALID_ARG, "job spp must be positive");
+    if (J.max_bounces < 0 || J.max_bounces > kMaxVertices) return fail(c, NE_ERR_INVALID_ARG, "job max_bounces out of range");
+  }
+  std::lock_guard<std::mutex> lk(c->mu);
+  NE_HIP(c, hipSetDevice(c->device));
+  ne_batch* b = new ne_batch();
+  b->ctx = c;
+  b->jobs.assign(jobs, jobs + n);
+  for (int32_t j = 0; j < n; ++j) {
+    b->film_off.push_back(b->film_words);
+    b->film_words += size_t(jobs[j].width) * size_t(jobs[j].height) * 3;
+  }
+  auto bail = [&](ne_status st) { batch_free(b); delete b; return st; };
+  int cus = 0, per_cu = 0, khz = 0;
+  if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess ||
+      batch_blocks_per_cu(&per_cu) != hipSuccess ||
+      hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device) != hipSuccess || khz <= 0)
+    return bail(fail(c, NE_ERR_HIP, "device attribute query failed"));
+  b->grid_blocks = uint32_t(std::max(1, cus * std::max(1, per_cu)));
+  b->ticks_per_ms = double(khz);
+  if (hipMalloc(&b->d_films, b->film_words * sizeof(long long)) != hipSuccess ||
+      hipMalloc(&b->d_jobs, size_t(n) * sizeof(BatchJob)) != hipSuccess ||
+      hipMalloc(&b->d_misc, 2 * sizeof(unsigned long long)) != hipSuccess ||
+      hipMalloc(&b->d_job_ms, size_t(n) * sizeof(float)) != hipSuccess)
+    return bail(fail(c, NE_ERR_OUT_OF_MEMORY, "batch allocation failed"));
+  if (hipEventCreate(&b->ev0) != hipSuccess || hipEventCreate(&b->ev1) != hipSuccess)
+    return bail(fail(c, NE_ERR_HIP, "event creation failed"));
+  c->refs.fetch_add(1);
+  *out = b;
+  return NE_OK;
+}
+
+void ne_batch_release(ne_batch* b) {
+  if (!b) return;
+  ne_ctx* c = b->ctx;
+  {
+    std::lock_guard<std::mutex> lk(c->mu);
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    batch_free(b);
+  }
+  delete b;
+  ctx_unref(c);
+}
+
+ne_status ne_batch_autotune(ne_batch* b, int32_t div, ne_batch_config* best, float* best_ms) {
+  if (!b) return fail(nullptr, NE_ERR_INVALID_ARG, "batch is NULL");
+  ne_ctx* c = b->ctx;
+  if (div < 1) return fail(c, NE_ERR_INVALID_ARG, "spp_divisor must be >= 1");
+  std::lock_guard<std::mutex> lk(c->mu);
+  NE_HIP(c, hipSetDevice(c->device));
+  // candidate item shapes: square and wide tiles, few to many samples per item
+  static const ne_batch_config cands[] = {{8, 8, 1, 0},  {8, 8, 4, 0},  {8, 8, 16, 0},  {16, 4, 4, 0},
+                                          {16, 4, 16, 0}, {16, 16, 1, 0}, {16, 16, 4, 0}, {32, 8, 4, 0}};
+  float bms = 0;
+  ne_batch_config bc = cands[0];
+  bool first = true;
+  for (const ne_batch_config& cfg : cands) {
+    float ms = 0;
+    ne_status st = batch_launch(b, cfg, div, nullptr);  // graph build + warm-up
+    if (st == NE_OK) st = batch_launch(b, cfg, div, &ms);
+    if (st != NE_OK) return st;
+    if (first || ms < bms) { bms = ms; bc = cfg; first = false; }
+  }
+  b->tuned = bc;
+  if (best) *best = bc;
+  if (best_ms) *best_ms = bms;
+  return NE_OK;
+}
+
+ne_status ne_batch_run(ne_batch* b, const ne_batch_config* cfg, float* ms) {
+  if (!b) return fail(nullptr, NE_ERR_INVALID_ARG, "batch is NULL");
+  ne_ctx* c = b->ctx;
+  const ne_batch_config use = cfg ? *cfg : b->tuned;
+  if (!cfg_valid(use)) return fail(c, NE_ERR_INVALID_ARG, "batch item shape out of range");
+  std::lock_guard<std::mutex> lk(c->mu);
+  NE_HIP(c, hipSetDevice(c->device));
+  return batch_launch(b, use, 1, ms);
+}
+
+ne_status ne_batch_job_times(ne_batch* b, float* ms) {
+  if (!b || !ms) return fail(b ? b->ctx : nullptr, NE_ERR_INVALID_ARG, "batch/ms is NULL");
+  std::lock_guard<std::mutex> lk(b->ctx->mu);
+  if (b->job_ms.empty()) return fail(b->ctx, NE_ERR_INVALID_ARG, "the batch has not run");
+  std::copy(b->job_ms.begin(), b->job_ms.end(), ms);
+  return NE_OK;
+}
+
+ne_status ne_batch_read(ne_batch* b, int32_t job, double* sum) {
+  if (!b || !sum) return fail(b ? b->ctx : nullptr, NE_ERR_INVALID_ARG, "batch/sum is NULL");
+  ne_ctx* c = b->ctx;
+  if (job < 0 || size_t(job) >= b->jobs.size()) return fail(c, NE_ERR_INVALID_ARG, "job index out of range");
+  std::lock_guard<std::mutex> lk(c->mu);
+  NE_HIP(c, hipSetDevice(c->device));
+  const size_t n = size_t(b->jobs[job].width) * size_t(b->jobs[job].height) * 3;
+  DevBuf t;
+  ne_status st = tmp_out<double>(c, t, n);
+  if (st != NE_OK) return st;
+  NE_HIP(c, launch_fixed_to_double(b->d_films + b->film_off[job], static_cast<double*>(t.p), n, c->stream));
+  if ((st = tmp_back(c, sum, t, n)) != NE_OK) return st;
+  NE_HIP(c, hipStreamSynchronize(c->stream));
+  return NE_OK;
+}

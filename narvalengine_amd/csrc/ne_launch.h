@@ -108,6 +108,35 @@ struct WfArgs {
 // returns after the film flush is enqueued.  h_status: pinned host word.
 hipError_t launch_wavefront(const DevScene& s, const DevCamera& cam, WfState& w, WfArgs a, bool count,
                             hipStream_t st, uint32_t* h_status, StageTimer* tm);
+// ---- persistent batched renderer -------------------------------------------
+constexpr int kMaxBatchJobs = 1024;
+struct BatchJob {
+  DevScene scene;
+  DevCamera cam;
+  int32_t W, H, spp, max_bounces;
+  uint64_t seed;
+  uint32_t job;
+  uint32_t tiles_x, tiles_y, chunks;  // work items of the job: tiles_x * tiles_y * chunks
+  unsigned long long first_item;      // global index of its first item
+  long long* film;                    // W * H * 3 fixed-point sums
+};
+struct BatchArgs {
+  const BatchJob* jobs;
+  int32_t njobs;
+  uint32_t tile_w, tile_h, spi;       // item shape: tile and samples per item
+  unsigned long long total_items;
+  unsigned long long* next_item;      // queue counter
+  unsigned long long* item_time;      // per-item completion timestamp (wall clock ticks)
+  unsigned long long* start_time;
+  float* job_ms;                      // per-job completion time
+  long long* films;                   // all films, contiguous
+};
+hipError_t batch_blocks_per_cu(int* blocks);
+// init (clear films, queue, timestamps) + persistent kernel + per-job times
+hipError_t launch_batch(const BatchArgs& b, size_t film_words, uint32_t grid_blocks, double ticks_per_ms,
+                        hipStream_t st);
+hipError_t launch_fixed_to_double(const long long* in, double* out, size_t n, hipStream_t st);
+
 hipError_t launch_reduce_partials(const RenderArgs& a, double* film, hipStream_t st);
 hipError_t launch_clear_rect(double* film, int W, int x0, int y0, int x1, int y1, hipStream_t st);
 hipError_t launch_tonemap(const double* film, int npix, float inv_spp, float* pixels, hipStream_t st);

@@ -632,17 +632,42 @@ NE_DEV R transmittance(const DevScene& s, Vec3<R> o, Vec3<R> d, R tmax, PathRng&
 // One path (corrected semantics).  The event and random-draw order is the
 // oracle's Li (camera jitter is drawn by the caller).
 // ---------------------------------------------------------------------------
+template <typename R>
+struct PathState {
+  Vec3<R> o, d;     // current ray
+  Vec3<R> L, beta;  // radiance gathered, throughput
+  R prev_pdf;       // pdf of the direction that produced the ray (emission MIS)
+  bool prev_delta;  // the ray left the camera (no MIS on what it hits)
+  int depth;        // scatter vertices so far
+};
+
+template <typename R>
+NE_DEV void path_init(PathState<R>& p, Vec3<R> o, Vec3<R> d) {
+  p.o = o;
+  p.d = d;
+  p.L = v3s(R(0));
+  p.beta = v3s(R(1));
+  p.prev_pdf = R(0);
+  p.prev_delta = true;
+  p.depth = 0;
+}
+
+// One bounce of the path: the segment from p.o along p.d (surface hit and/or
+// medium collision), emission / escape, and at a scatter vertex next-event
+// estimation plus the sampled continuation.  Returns true when the path has
+// ended (p.L is final).  trace_path loops over it; the persistent batch
+// kernel interleaves bounces of different paths in one wave.
 template <typename R, bool C>
-NE_DEV Vec3<R> trace_path(const DevScene& s, Vec3<R> o, Vec3<R> d, int max_bounces, PathRng& rng, uint32_t* lds,
-                          int stride, Count<C>& cnt) {
-  Vec3<R> L = v3s(R(0)), beta = v3s(R(1));
-  R prev_pdf = R(0);
-  bool prev_delta = true;
-  int depth = 0;
+NE_DEV bool path_bounce(const DevScene& s, PathState<R>& p, int max_bounces, PathRng& rng, uint32_t* lds,
+                        int stride, Count<C>& cnt) {
+  Vec3<R> o = p.o, d = p.d, L = p.L, beta = p.beta;
+  const R prev_pdf = p.prev_pdf;
+  const bool prev_delta = p.prev_delta;
+  const int depth = p.depth;
   const R tmin = R(1e-4);
   const R g = R(s.g);
   const R nl = R(s.n_lights);
-  while (true) {
+  {
     cnt.ext();
     SurfHit<R> hit;
     bool hs = traverse<R, false, C>(s, o, d, tmin, xinf<R>(), lds, stride, hit, cnt);
@@ -666,7 +691,8 @@ NE_DEV Vec3<R> trace_path(const DevScene& s, Vec3<R> o, Vec3<R> d, int max_bounc
           R w = prev_delta ? R(1) : power_heuristic(prev_pdf, R(1.0 / kFOUR_PI));
           L = L + beta * env_le<R>(s, d) * w;
         }
-        break;
+        p.L = L;
+        return true;
       }
       hit_normal<R>(s, hit, o, d, n, mat);
       const float* m = s.mats + 12 * size_t(mat);
@@ -681,11 +707,15 @@ NE_DEV Vec3<R> trace_path(const DevScene& s, Vec3<R> o, Vec3<R> d, int max_bounc
           w = power_heuristic(prev_pdf, pl);
         }
         L = L + beta * Le * w;
-        break;
+        p.L = L;
+        return true;
       }
       x = o + hit.t * d;
     }
-    if (depth >= max_bounces) break;
+    if (depth >= max_bounces) {
+      p.L = L;
+      return true;
+    }
     // ---- vertex: next-event estimation
     Vec3<R> wo = -d;
     Glossy<R> gm;
@@ -762,25 +792,56 @@ NE_DEV Vec3<R> trace_path(const DevScene& s, Vec3<R> o, Vec3<R> d, int max_bounc
     if (medium) {
       R u0 = R(rng.next()), u1 = R(rng.next());
       Vec3<R> wi = hg_sample(g, d, u0, u1);
-      prev_pdf = hg_p(g, dot(d, wi));
-      d = wi;
+      p.prev_pdf = hg_p(g, dot(d, wi));
+      p.d = wi;
+      p.beta = beta;  // scattering albedo applied at the vertex
     } else {
       R u0 = R(rng.next()), u1 = R(rng.next()), u2 = R(rng.next());
       Vec3<R> wol = to_lcs(wo, ns, ss, ts);
       Vec3<R> wil = gm.sample(wol, u0, u1, u2);
       R pf = gm.pdf(wol, wil);
-      if (!(pf > R(0))) break;
+      if (!(pf > R(0))) {
+        p.L = L;
+        return true;
+      }
       Vec3<R> f = gm.eval(wol, wil) * xfabs(wil.z);
-      if (is_black(f)) break;
-      beta = beta * f / pf;
-      prev_pdf = pf;
-      d = to_world(wil, ns, ss, ts);
+      if (is_black(f)) {
+        p.L = L;
+        return true;
+      }
+      p.beta = beta * f / pf;
+      p.prev_pdf = pf;
+      p.d = to_world(wil, ns, ss, ts);
     }
-    prev_delta = false;
-    o = x;
-    depth++;
+    p.L = L;
+    p.prev_delta = false;
+    p.o = x;
+    p.depth = depth + 1;
   }
-  return L;
+  return false;
+}
+
+// One whole path (the oracle's Li event and draw order; camera jitter is drawn
+// by the caller).
+template <typename R, bool C>
+NE_DEV Vec3<R> trace_path(const DevScene& s, Vec3<R> o, Vec3<R> d, int max_bounces, PathRng& rng, uint32_t* lds,
+                          int stride, Count<C>& cnt) {
+  PathState<R> p;
+  path_init(p, o, d);
+  while (!path_bounce<R, C>(s, p, max_bounces, rng, lds, stride, cnt)) {
+  }
+  return p.L;
+}
+
+// Order-independent film accumulation: 64-bit fixed point, 2^-30 radiance
+// units per count; integer atomics make the sum independent of the order in
+// which paths finish.
+constexpr float kFixedScale = 1073741824.0f;
+NE_DEV void film_add(long long* film, uint32_t pix, float r, float g, float b) {
+  long long* f = film + size_t(pix) * 3;
+  if (r != 0.0f) atomicAdd((unsigned long long*)&f[0], (unsigned long long)__float2ll_rn(r * kFixedScale));
+  if (g != 0.0f) atomicAdd((unsigned long long*)&f[1], (unsigned long long)__float2ll_rn(g * kFixedScale));
+  if (b != 0.0f) atomicAdd((unsigned long long*)&f[2], (unsigned long long)__float2ll_rn(b * kFixedScale));
 }
 
 }  // namespace ne

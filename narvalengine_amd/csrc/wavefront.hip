@@ -37,16 +37,8 @@
 
 namespace ne {
 
-constexpr float kFixedScale = 1073741824.0f;  // 2^30 film units per radiance unit
 constexpr uint32_t kToFilm = 0x80000000u;      // shadow target flag: add to film pixel
 constexpr int kPlanThreads = 1024;
-
-__device__ __forceinline__ void film_add(long long* film, uint32_t pix, float r, float g, float b) {
-  long long* f = film + size_t(pix) * 3;
-  if (r != 0.0f) atomicAdd((unsigned long long*)&f[0], (unsigned long long)__float2ll_rn(r * kFixedScale));
-  if (g != 0.0f) atomicAdd((unsigned long long*)&f[1], (unsigned long long)__float2ll_rn(g * kFixedScale));
-  if (b != 0.0f) atomicAdd((unsigned long long*)&f[2], (unsigned long long)__float2ll_rn(b * kFixedScale));
-}
 
 // Position of this thread's item among the block's selected items (in thread
 // order) and the block total.  Every thread of the block must call it.

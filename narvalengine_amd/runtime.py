@@ -5,8 +5,8 @@ import ctypes as C
 
 import numpy as np
 
-from .abi import (Counters, NarvalError, RenderParams, SceneInfo, check, lib, NE_FP32, NE_FLAG_COUNT, NE_FLAG_TIME,
-                  NE_FLAG_STAGE_TIME, STAGES)
+from .abi import (BatchConfig, Counters, Job, NarvalError, RenderParams, SceneInfo, check, lib, NE_FP32,
+                  NE_FLAG_COUNT, NE_FLAG_TIME, NE_FLAG_STAGE_TIME, STAGES)
 
 
 def _fp(a: np.ndarray):
@@ -213,3 +213,60 @@ class Film:
 
 
 __all__ = ["Context", "GpuScene", "Film", "NarvalError"]
+
+
+class Batch:
+    """Many render jobs in one persistent launch (ne_batch).  jobs: list of
+    dicts with scene (GpuScene), camera, width, height, spp, max_bounces and
+    optional seed / job."""
+
+    def __init__(self, ctx: Context, jobs: list):
+        self.ctx = ctx
+        self.jobs = jobs
+        arr = (Job * len(jobs))()
+        for k, j in enumerate(jobs):
+            a = arr[k]
+            a.scene = j["scene"].h.value if isinstance(j["scene"].h, C.c_void_p) else j["scene"].h
+            a.camera = j["camera"].desc()
+            a.width, a.height, a.spp, a.max_bounces = j["width"], j["height"], j["spp"], j["max_bounces"]
+            a.seed, a.job = j.get("seed", 0), j.get("job", 0)
+        self._keep = [j["scene"] for j in jobs]  # scenes outlive the batch
+        h = C.c_void_p()
+        ctx.check(ctx._lib.ne_batch_create(ctx.h, arr, len(jobs), C.byref(h)))
+        self.h = h
+
+    def autotune(self, spp_divisor: int = 8):
+        cfg, ms = BatchConfig(), C.c_float()
+        self.ctx.check(self.ctx._lib.ne_batch_autotune(self.h, spp_divisor, C.byref(cfg), C.byref(ms)))
+        return (cfg.tile_w, cfg.tile_h, cfg.samples_per_item), float(ms.value)
+
+    def run(self, cfg=None) -> float:
+        ms = C.c_float()
+        p = None
+        if cfg is not None:
+            c = BatchConfig(cfg[0], cfg[1], cfg[2], 0)
+            p = C.byref(c)
+        self.ctx.check(self.ctx._lib.ne_batch_run(self.h, p, C.byref(ms)))
+        return float(ms.value)
+
+    def job_times(self) -> np.ndarray:
+        out = np.zeros(len(self.jobs), np.float32)
+        self.ctx.check(self.ctx._lib.ne_batch_job_times(self.h, _fp(out)))
+        return out
+
+    def read(self, k: int) -> np.ndarray:
+        j = self.jobs[k]
+        out = np.zeros((j["height"], j["width"], 3), np.float64)
+        self.ctx.check(self.ctx._lib.ne_batch_read(self.h, k, _dp(out)))
+        return out
+
+    def close(self):
+        if self.h:
+            self.ctx._lib.ne_batch_release(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass

@@ -1,0 +1,103 @@
+"""Batched persistent renderer: per-job sums bitwise independent of the batch
+and of the work-item shape, equal to the single-job render up to the film's
+fixed-point quantisation, and statistically equal to the oracle."""
+import numpy as np
+import pytest
+
+import oracle as O
+import scenes
+from conftest import print_case
+from narvalengine_amd import Batch, configs
+from narvalengine_amd.abi import NE_FP32, NE_MODE_MEGAKERNEL
+from narvalengine_amd.scene import Camera
+
+pytestmark = pytest.mark.gpu
+
+
+def _jobs(ctx):
+    specs = [(16, 1, 40, 24, 8), (24, 2, 33, 17, 5), (16, 3, 20, 40, 12), (32, 4, 8, 8, 3)]
+    out = []
+    for res, seed, w, h, spp in specs:
+        s = scenes.small_volume(res=res, maj_res=4)
+        s.volume["grid"] = configs.fbm_grid(res, seed)
+        cam = Camera(position=(0.0, 0.25, -1.8), look_at=(0, 0, 0), up=(0, 1, 0), vfov=45.0, aspect=w / h)
+        out.append(dict(scene=ctx.upload(s), scene_host=s, camera=cam, width=w, height=h, spp=spp, max_bounces=12,
+                        seed=7 + seed, job=seed))
+    return out
+
+
+def test_batch_is_bitwise_independent_of_batching_and_item_shape(ctx):
+    print_case("batch vs single-job batches, two item shapes")
+    jobs = _jobs(ctx)
+    b = Batch(ctx, jobs)
+    ms = b.run((8, 8, 4))
+    assert ms > 0
+    full = [b.read(k) for k in range(len(jobs))]
+    b.run((16, 4, 1))
+    for k in range(len(jobs)):
+        np.testing.assert_array_equal(b.read(k), full[k])
+    for k, j in enumerate(jobs):
+        single = Batch(ctx, [j])
+        single.run((32, 8, 16))
+        np.testing.assert_array_equal(single.read(0), full[k])
+        single.close()
+    t = b.job_times()
+    assert (t > 0).all() and (t <= b.run((16, 4, 1)) * 1.05 + 0.5).all()
+    b.close()
+
+
+def test_batch_matches_the_single_job_render(ctx):
+    """Same paths as ne_render (same counter RNG streams, same per-path
+    arithmetic); only the accumulation differs (FP64 sums vs 2^-30 fixed point)."""
+    print_case("batch vs megakernel render")
+    jobs = _jobs(ctx)
+    b = Batch(ctx, jobs)
+    b.run()
+    for k, j in enumerate(jobs):
+        f = ctx.film(j["width"], j["height"])
+        ctx.render(j["scene"], j["camera"], f, spp=j["spp"], max_bounces=j["max_bounces"], seed=j["seed"],
+                   job=j["job"], mode=NE_MODE_MEGAKERNEL)
+        ref = f.read()
+        np.testing.assert_allclose(b.read(k), ref, rtol=1e-6, atol=j["spp"] * 2.0 ** -30)
+    b.close()
+
+
+def test_batch_matches_the_oracle(ctx):
+    print_case("batch vs oracle")
+    jobs = _jobs(ctx)[:2]
+    b = Batch(ctx, jobs)
+    b.run()
+    for k, j in enumerate(jobs):
+        w = O.World(j["scene_host"])
+        ref = w.render(j["camera"], j["width"], j["height"], j["spp"], j["max_bounces"], seed=j["seed"], job=j["job"])
+        got = b.read(k)
+        same = np.isclose(got, ref, rtol=1e-5, atol=1e-6).mean()
+        assert same > 0.9, same
+        assert abs(got.mean() - ref.mean()) < 0.02 * ref.mean()
+    b.close()
+
+
+def test_batch_autotune_picks_a_candidate_and_reports_times(ctx):
+    print_case("batch autotune")
+    jobs = _jobs(ctx)
+    b = Batch(ctx, jobs)
+    cfg, ms = b.autotune(spp_divisor=2)
+    assert ms > 0 and cfg[0] * cfg[1] >= 32 and cfg[2] >= 1
+    full = b.run()  # tuned shape
+    assert full > 0 and len(b.job_times()) == len(jobs)
+    b.close()
+
+
+def test_batch_rejects_bad_jobs(ctx):
+    print_case("batch invalid arguments")
+    from narvalengine_amd.abi import NarvalError
+    jobs = _jobs(ctx)[:1]
+    bad = dict(jobs[0], spp=0)
+    with pytest.raises(NarvalError):
+        Batch(ctx, [bad])
+    b = Batch(ctx, jobs)
+    with pytest.raises(NarvalError):
+        b.run((0, 8, 4))
+    with pytest.raises(NarvalError):
+        b.job_times()
+    b.close()
