@@ -4,7 +4,9 @@ One step = one full-frame render of the workload at its full spp (every
 pixel, every sample, up to max_bounces scatter vertices), inputs already
 resident in HBM.  The default workload is `surface1m` (1024x1024, 256 spp,
 8 bounces, 1.31M-triangle mesh + spheres + rectangles, SAH BVH); pass
---workload volume128 / cloud512 for the volume scenes.
+--workload volume128 / cloud512 for the volume scenes, or batch48 for the
+48-job batch in one persistent graph launch (one step = one launch over every
+job at full spp).
 
 Prints one JSON line: samples/s (value), rays/s, the dominant kernel's
 HBM roofline fraction and the CPU baseline (the oracle timed on this host's
@@ -21,6 +23,8 @@ import os
 import sys
 import time
 from pathlib import Path
+
+import numpy as np
 
 ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
@@ -118,12 +122,89 @@ def copy_calibration(nbytes: int, reps: int = 5):
     print(json.dumps({"copy_calibration_bytes": nbytes, "reps": reps}))
 
 
+def bench_batch(args):
+    """48 uneven volume jobs in one persistent graph launch.  The item shape is
+    autotuned first (untimed, on a 1/16-spp calibration run); a step is one
+    launch that renders every job at full spp."""
+    from narvalengine_amd import Batch, Context, configs
+    from narvalengine_amd.abi import NE_MODE_MEGAKERNEL
+    t0 = time.perf_counter()
+    wls = configs.batch48()
+    log(f"batch48: {len(wls)} jobs built in {time.perf_counter() - t0:.1f} s")
+    ctx = Context(0)
+    jobs = [dict(scene=ctx.upload(w.scene), camera=w.camera, width=w.width, height=w.height, spp=w.spp,
+                 max_bounces=w.max_bounces, seed=1, job=k) for k, w in enumerate(wls)]
+    b = Batch(ctx, jobs)
+    cfg, tune_ms = b.autotune(spp_divisor=16)
+    log(f"autotuned item shape {cfg} ({tune_ms:.1f} ms calibration run)")
+    for i in range(args.warmup):
+        b.run()
+    paths = sum(j["width"] * j["height"] * j["spp"] for j in jobs)
+    ms, job_ms = [], None
+    ctx.synchronize()
+    t_start = time.perf_counter()
+    for k in range(args.steps):
+        ms.append(b.run())
+        job_ms = b.job_times()
+        log(f"step {k + 1}/{args.steps}: launch {ms[-1]:.1f} ms")
+    wall = time.perf_counter() - t_start
+    value = args.steps * paths / wall
+    launch_s = sum(ms) / len(ms) / 1e3
+    roofline, counters = None, None
+    if not args.no_count:
+        tot = {}
+        for j in jobs:
+            f = ctx.film(j["width"], j["height"])
+            ctx.render(j["scene"], j["camera"], f, spp=j["spp"], max_bounces=j["max_bounces"], seed=1, job=j["job"],
+                       mode=NE_MODE_MEGAKERNEL, count=True)
+            for key, v in ctx.counters().items():
+                tot[key] = tot.get(key, 0) + v
+            f.close()
+        counters = tot
+        algo = sum(tot[k] * b_ for k, b_ in EV.items()) + 2 * 8 * 3 * sum(j["width"] * j["height"] for j in jobs)
+        achieved = algo / launch_s / 1e9
+        roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic("k_batch", "batch48"),
+                    "kernel": "k_batch", "algorithmic_bytes_per_launch": int(algo),
+                    "avg_launch_ms": round(launch_s * 1e3, 3)}
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, str(ROOT / "tests"))
+        import oracle as O
+        threads = max(1, min(16, os.cpu_count() or 1))
+        n, dt = 0, 0.0
+        for w in wls:  # every job, whole frame, first sample of each pixel
+            world = O.World(w.scene)
+            t = time.perf_counter()
+            world.render(w.camera, w.width, w.height, 1, w.max_bounces, seed=1, threads=threads)
+            dt += time.perf_counter() - t
+            n += w.width * w.height
+        cpu = {"value": n / dt, "unit": "samples/s", "cores": threads, "kind": "port",
+               "sample": f"every job, whole frame at 1 sample per pixel ({n} paths, {dt:.1f} s)"}
+    jm = np.asarray(job_ms, np.float64)
+    out = {
+        "metric": "samples/sec", "value": round(value, 1), "unit": "samples/s", "n_gpus": 1,
+        "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 3),
+        "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
+        "data": "synthetic (seeded procedural grids)",
+        "config": {"workload": "batch48", "jobs": len(jobs), "paths_per_step": paths, "item_shape": list(cfg),
+                   "accumulation": "fixed-point i64"},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
+        "job_completion_ms": {"min": round(float(jm.min()), 2), "median": round(float(np.median(jm)), 2),
+                              "max": round(float(jm.max()), 2), "all": [round(float(x), 2) for x in jm]},
+        "counters_per_step": counters,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="surface1m", choices=["surface1m", "volume128", "cloud512"])
+    ap.add_argument("--workload", default="surface1m", choices=["surface1m", "volume128", "cloud512", "batch48"])
     ap.add_argument("--mode", default="auto", choices=["auto", "megakernel", "wavefront"])
     ap.add_argument("--pool", type=int, default=0, help="wavefront path slots (0 = library default)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
@@ -140,6 +221,10 @@ def main():
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration(args.copy_calibration)
+        return
+
+    if args.workload == "batch48":
+        bench_batch(args)
         return
 
     from narvalengine_amd import Context, configs

@@ -133,4 +133,16 @@ def batch48_table(seed: int = 48):
     return jobs
 
 
+def batch48(seed: int = 48, bounces: int = 16) -> list:
+    """The 48 uneven jobs as workloads: each its own fBm grid (built like
+    volume128 from its seed), image size and spp; medium, lights and camera
+    placement as volume128."""
+    out = []
+    for res, gseed, w, h, spp in batch48_table(seed):
+        s = volume_scene(fbm_grid(res, gseed), maj_res=16)
+        cam = Camera(position=(0.0, 0.25, -1.8), look_at=(0, 0, 0), up=(0, 1, 0), vfov=45.0, aspect=w / h)
+        out.append(Workload(f"batch48/job{gseed}", s, cam, w, h, spp, bounces))
+    return out
+
+
 CONFIGS = {"volume128": volume128, "surface1m": surface1m, "cloud512": cloud512}
