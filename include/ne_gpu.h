@@ -273,6 +273,53 @@ ne_status ne_batch_job_times(ne_batch* batch, float* ms);
 /* Linear radiance sums of job `job` (width * height * 3 doubles). */
 ne_status ne_batch_read(ne_batch* batch, int32_t job, double* sum_rgb);
 
+/* ---- learned directional guide --------------------------------------------
+ * A per-position, per-direction proposal for medium scattering vertices: the
+ * unit-cube position of the volume through a multiresolution hash grid
+ * (16 levels x 2 features, 2^log2_table entries per level) plus the incoming
+ * direction feed an MLP 35 -> 64 x 4 (ReLU) -> 40 on BF16 MFMA tiles (FP32
+ * accumulation) that outputs an 8-lobe von Mises-Fisher mixture.  It is trained
+ * online with Adam on the tracer's own radiance samples and combined with HG
+ * sampling by one-sample MIS (0.5 / 0.5), so the estimator stays unbiased.
+ * New component: the reference has no learned sampling. */
+typedef struct ne_guide_config {
+  int32_t log2_table; /* hash entries per level = 2^log2_table (default 19, range 10..22) */
+  int32_t base_res;   /* coarsest level resolution (default 16) */
+  int32_t max_res;    /* finest level resolution (default 512) */
+  int32_t batch;      /* training samples per step (default 65536) */
+  float lr_table, lr_mlp, beta1, beta2, eps; /* Adam (defaults 1e-2, 1e-3, 0.9, 0.99, 1e-8) */
+  uint64_t seed;      /* parameter initialisation */
+} ne_guide_config;
+typedef struct ne_guide_stats {
+  uint64_t steps;     /* Adam steps taken */
+  uint64_t samples;   /* training samples consumed */
+  float last_loss;    /* weighted loss of the last step */
+  float pad;
+} ne_guide_stats;
+typedef struct ne_guide ne_guide;
+/* cfg NULL: defaults.  Parameters are initialised from cfg->seed. */
+ne_status ne_guide_create(ne_ctx* ctx, const ne_guide_config* cfg, ne_guide** out);
+void ne_guide_release(ne_guide* guide);
+/* Re-initialises the parameters and the optimiser state. */
+ne_status ne_guide_reset(ne_guide* guide);
+ne_status ne_guide_get_stats(ne_guide* guide, ne_guide_stats* out);
+/* n queries (unit-cube position xyz, direction xyz) -> 40 floats each: per
+ * lobe (weight, unit mean xyz, kappa); raw != 0: the network's 40 raw outputs. */
+ne_status ne_guide_eval(ne_guide* guide, int32_t n, const float* queries, int32_t raw, float* out);
+/* One training step on n <= batch samples of 10 floats (position xyz,
+ * incoming direction xyz, sampled direction xyz, weight r >= 0): forward,
+ * gradient of -sum r log q / sum r and, if update != 0, an Adam step.  loss:
+ * the weighted loss before the step. */
+ne_status ne_guide_train_step(ne_guide* guide, int32_t n, const float* samples, int32_t update, float* loss);
+/* Sizes: hash-table floats, MLP floats (weights [in][out] per layer, padded to
+ * 48/64, then biases) and the 16 level resolutions. */
+ne_status ne_guide_sizes(ne_guide* guide, uint64_t* table_floats, int32_t* mlp_floats, int32_t* level_res);
+ne_status ne_guide_get_params(ne_guide* guide, float* table, float* mlp);
+ne_status ne_guide_set_params(ne_guide* guide, const float* table, const float* mlp);
+/* Gradients left by the last train step taken with update = 0 (an update
+ * consumes and clears them). */
+ne_status ne_guide_get_grads(ne_guide* guide, float* table_grad, float* mlp_grad);
+
 /* ---- per-function parity entry points (device code, host buffers) ------- */
 enum {
   NE_TEST_AABB = 0,     /* AABB::intersect      src/primitives/AABB.cpp:48-79      prim: min xyz, max xyz   */

@@ -11,6 +11,6 @@ from .abi import (  # noqa: F401
     CameraDesc, Counters, NarvalError, RenderParams, SceneDesc, SceneInfo, lib, lib_path,
     NE_FP32, NE_FP64, NE_FLAG_COUNT, NE_MAT_GLOSSY, NE_MAT_EMITTER, NE_ENV_NONE, NE_ENV_CONSTANT, NE_ENV_SKY,
 )
-from .runtime import Batch, Context, Film, GpuScene  # noqa: F401
+from .runtime import Batch, Context, Film, GpuScene, Guide  # noqa: F401
 from .scene import Scene, Camera  # noqa: F401
 from .engine import OfflineEngine, GpuPathIntegrator, SceneSettings  # noqa: F401

@@ -89,6 +89,17 @@ class BatchConfig(C.Structure):
                 ("reserved", C.c_int32)]
 
 
+class GuideConfig(C.Structure):
+    """ne_guide_config."""
+    _fields_ = [("log2_table", C.c_int32), ("base_res", C.c_int32), ("max_res", C.c_int32), ("batch", C.c_int32),
+                ("lr_table", C.c_float), ("lr_mlp", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float),
+                ("eps", C.c_float), ("seed", C.c_uint64)]
+
+
+class GuideStats(C.Structure):
+    _fields_ = [("steps", C.c_uint64), ("samples", C.c_uint64), ("last_loss", C.c_float), ("pad", C.c_float)]
+
+
 SIGNATURES = [
     ("ne_ctx_create", C.c_int32, [C.c_int32, C.POINTER(C.c_void_p)]),
     ("ne_ctx_destroy", None, [C.c_void_p]),
@@ -127,6 +138,16 @@ SIGNATURES = [
     ("ne_batch_run", C.c_int32, [C.c_void_p, C.POINTER(BatchConfig), _F]),
     ("ne_batch_job_times", C.c_int32, [C.c_void_p, _F]),
     ("ne_batch_read", C.c_int32, [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]),
+    ("ne_guide_create", C.c_int32, [C.c_void_p, C.POINTER(GuideConfig), C.POINTER(C.c_void_p)]),
+    ("ne_guide_release", None, [C.c_void_p]),
+    ("ne_guide_reset", C.c_int32, [C.c_void_p]),
+    ("ne_guide_get_stats", C.c_int32, [C.c_void_p, C.POINTER(GuideStats)]),
+    ("ne_guide_eval", C.c_int32, [C.c_void_p, C.c_int32, _F, C.c_int32, _F]),
+    ("ne_guide_train_step", C.c_int32, [C.c_void_p, C.c_int32, _F, C.c_int32, _F]),
+    ("ne_guide_sizes", C.c_int32, [C.c_void_p, C.POINTER(C.c_uint64), C.POINTER(C.c_int32), C.POINTER(C.c_int32)]),
+    ("ne_guide_get_params", C.c_int32, [C.c_void_p, _F, _F]),
+    ("ne_guide_set_params", C.c_int32, [C.c_void_p, _F, _F]),
+    ("ne_guide_get_grads", C.c_int32, [C.c_void_p, _F, _F]),
 ]
 
 _LIB = None

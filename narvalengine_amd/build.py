@@ -22,7 +22,7 @@ LIB = PKG / "libnarval_gpu.so"
 ROOT = PKG.parent
 FACADE_SRC = ROOT / "tests" / "cpp" / "facade_check.cpp"
 FACADE_BIN = ROOT / "tests" / "cpp" / "facade_check"
-SOURCES = ["render.hip", "wavefront.hip", "batch.hip", "api.cpp", "bvh_build.cpp", "generators.cpp"]
+SOURCES = ["render.hip", "wavefront.hip", "batch.hip", "guide.hip", "api.cpp", "bvh_build.cpp", "generators.cpp"]
 HEADERS = ["ne_math.h", "ne_scene.h", "ne_trace.h", "ne_launch.h", "bvh_build.h"]
 
 

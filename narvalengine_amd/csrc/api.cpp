@@ -1179,3 +1179,210 @@ ne_status ne_batch_read(ne_batch* b, int32_t job, double* sum) {
   NE_HIP(c, hipStreamSynchronize(c->stream));
   return NE_OK;
 }
+
+// ---- learned guide -------------------------------------------------------------
+struct ne_guide {
+  ne_ctx* ctx = nullptr;
+  ne_guide_config cfg{};
+  GuideNet net{};
+  GuideBatch batch{};
+  GuideTrain train{};
+  std::vector<void*> allocs;
+  size_t table_n = 0;
+  uint64_t steps = 0, samples = 0;
+  float last_loss = 0.0f;
+};
+
+namespace {
+
+void guide_free(ne_guide* g) {
+  for (void* p : g->allocs) (void)hipFree(p);
+  g->allocs.clear();
+}
+
+template <class T>
+bool guide_alloc(ne_guide* g, T*& p, size_t count) {
+  void* q = nullptr;
+  if (hipMalloc(&q, std::max<size_t>(count, 1) * sizeof(T)) != hipSuccess) return false;
+  g->allocs.push_back(q);
+  p = static_cast<T*>(q);
+  return true;
+}
+
+}  // namespace
+
+ne_status ne_guide_create(ne_ctx* c, const ne_guide_config* cfg_in, ne_guide** out) {
+  if (!c || !out) return fail(c, NE_ERR_INVALID_ARG, "ctx/out is NULL");
+  *out = nullptr;
+  ne_guide_config cfg{19, 16, 512, 65536, 1e-2f, 1e-3f, 0.9f, 0.99f, 1e-8f, 0};
+  if (cfg_in) cfg = *cfg_in;
+  if (cfg.log2_table < 10 || cfg.log2_table > 22) return fail(c, NE_ERR_INVALID_ARG, "log2_table must be in [10, 22]");
+  if (cfg.base_res < 2 || cfg.max_res < cfg.base_res || cfg.max_res > 4096)
+    return fail(c, NE_ERR_INVALID_ARG, "level resolutions out of range");
+  if (cfg.batch < 16 || cfg.batch > (1 << 22)) return fail(c, NE_ERR_INVALID_ARG, "batch must be in [16, 2^22]");
+  if (!(cfg.lr_table >= 0) || !(cfg.lr_mlp >= 0) || !(cfg.beta1 >= 0 && cfg.beta1 < 1) ||
+      !(cfg.beta2 >= 0 && cfg.beta2 < 1) || !(cfg.eps > 0))
+    return fail(c, NE_ERR_INVALID_ARG, "Adam parameters out of range");
+  std::lock_guard<std::mutex> lk(c->mu);
+  NE_HIP(c, hipSetDevice(c->device));
+  ne_guide* g = new ne_guide();
+  g->ctx = c;
+  g->cfg = cfg;
+  cfg.batch = (cfg.batch + 15) / 16 * 16;
+  g->cfg.batch = cfg.batch;
+  GuideNet& n = g->net;
+  n.log2T = uint32_t(cfg.log2_table);
+  const double growth = std::exp((std::log(double(cfg.max_res)) - std::log(double(cfg.base_res))) / (kGuideLevels - 1));
+  for (int l = 0; l < kGuideLevels; ++l) n.res[l] = int32_t(std::floor(double(cfg.base_res) * std::pow(growth, l) + 1e-6));
+  g->table_n = size_t(kGuideLevels) * (size_t(1) << n.log2T) * kGuideFeat;
+  const size_t cap = size_t(cfg.batch);
+  GuideTrain& t = g->train;
+  t.waves = 512;
+  auto bail = [&](ne_status st) { guide_free(g); delete g; return st; };
+  bool ok = guide_alloc(g, n.table, g->table_n) && guide_alloc(g, n.table_grad, g->table_n) &&
+            guide_alloc(g, n.table_m, g->table_n) && guide_alloc(g, n.table_v, g->table_n) &&
+            guide_alloc(g, n.mlp, kGuideParams) && guide_alloc(g, n.mlp_grad, kGuideParams) &&
+            guide_alloc(g, n.mlp_m, kGuideParams) && guide_alloc(g, n.mlp_v, kGuideParams) &&
+            guide_alloc(g, n.wf, kGuideWeights) && guide_alloc(g, n.wb, kGuideWeights) &&
+            guide_alloc(g, g->batch.s, 10 * cap) && guide_alloc(g, g->batch.count, 1) &&
+            guide_alloc(g, t.act, size_t(kGuideLayers) * cap * kGuideWidth) &&
+            guide_alloc(g, t.dy, cap * kGuideWidth) && guide_alloc(g, t.dy2, cap * kGuideWidth) &&
+            guide_alloc(g, t.denc, cap * kGuideEnc) && guide_alloc(g, t.partial, size_t(t.waves) * kGuideParams) &&
+            guide_alloc(g, t.loss, 2);
+  if (!ok) return bail(fail(c, NE_ERR_OUT_OF_MEMORY, "guide allocation failed"));
+  g->batch.cap = uint32_t(cap);
+  if (hipMemsetAsync(g->batch.count, 0, sizeof(uint32_t), c->stream) != hipSuccess ||
+      launch_guide_init(n, cfg.seed, c->stream) != hipSuccess || hipStreamSynchronize(c->stream) != hipSuccess)
+    return bail(fail(c, NE_ERR_HIP, "guide initialisation failed"));
+  c->refs.fetch_add(1);
+  *out = g;
+  return NE_OK;
+}
+
+void ne_guide_release(ne_guide* g) {
+  if (!g) return;
+  ne_ctx* c = g->ctx;
+  {
+    std::lock_guard<std::mutex> lk(c->mu);
+    (void)hipSetDevice(c->device);
+    (void)hipStreamSynchronize(c->stream);
+    guide_free(g);
+  }
+  delete g;
+  ctx_unref(c);
+}
+
+ne_status ne_guide_reset(ne_guide* g) {
+  if (!g) return fail(nullptr, NE_ERR_INVALID_ARG, "guide is NULL");
+  ne_ctx* c = g->ctx;
+  std::lock_guard<std::mutex> lk(c->mu);
+  NE_HIP(c, hipSetDevice(c->device));
+  NE_HIP(c, launch_guide_init(g->net, g->cfg.seed, c->stream));
+  NE_HIP(c, hipStreamSynchronize(c->stream));
+  g->steps = 0;
+  g->samples = 0;
+  g->last_loss = 0.0f;
+  return NE_OK;
+}
+
+ne_status ne_guide_get_stats(ne_guide* g, ne_guide_stats* out) {
+  if (!g || !out) return fail(g ? g->ctx : nullptr, NE_ERR_INVALID_ARG, "guide/out is NULL");
+  std::lock_guard<std::mutex> lk(g->ctx->mu);
+  out->steps = g->steps;
+  out->samples = g->samples;
+  out->last_loss = g->last_loss;
+  out->pad = 0.0f;
+  return NE_OK;
+}
+
+ne_status ne_guide_eval(ne_guide* g, int32_t n, const float* q, int32_t raw, float* out) {
+  if (!g || n < 0 || (n > 0 && (!q || !out))) return fail(g ? g->ctx : nullptr, NE_ERR_INVALID_ARG, "bad guide_eval arguments");
+  if (n == 0) return NE_OK;
+  ne_ctx* c = g->ctx;
+  std::lock_guard<std::mutex> lk(c->mu);
+  NE_HIP(c, hipSetDevice(c->device));
+  DevBuf bi, bo;
+  ne_status st = tmp_in(c, bi, q, size_t(n) * 6);
+  if (st == NE_OK) st = tmp_out<float>(c, bo, size_t(n) * kGuideOutputs);
+  if (st != NE_OK) return st;
+  if (raw) NE_HIP(c, launch_guide_raw(g->net, n, static_cast<float*>(bi.p), static_cast<float*>(bo.p), c->stream));
+  else NE_HIP(c, launch_guide_eval(g->net, n, static_cast<float*>(bi.p), static_cast<float*>(bo.p), c->stream));
+  if ((st = tmp_back(c, out, bo, size_t(n) * kGuideOutputs)) != NE_OK) return st;
+  NE_HIP(c, hipStreamSynchronize(c->stream));
+  return NE_OK;
+}
+
+ne_status ne_guide_train_step(ne_guide* g, int32_t n, const float* smp, int32_t update, float* loss) {
+  if (!g) return fail(nullptr, NE_ERR_INVALID_ARG, "guide is NULL");
+  ne_ctx* c = g->ctx;
+  if (n < 1 || uint32_t(n) > g->batch.cap || !smp) return fail(c, NE_ERR_INVALID_ARG, "sample count must be in [1, batch]");
+  for (int32_t i = 0; i < n; ++i)
+    if (!(smp[10 * i + 9] >= 0.0f) || !std::isfinite(smp[10 * i + 9]))
+      return fail(c, NE_ERR_INVALID_ARG, "sample weights must be finite and >= 0");
+  std::lock_guard<std::mutex> lk(c->mu);
+  NE_HIP(c, hipSetDevice(c->device));
+  // samples: AoS from the caller -> SoA batch buffer
+  const size_t cap = g->batch.cap;
+  std::vector<float> soa(10 * cap, 0.0f);
+  for (int32_t i = 0; i < n; ++i)
+    for (int k = 0; k < 10; ++k) soa[size_t(k) * cap + size_t(i)] = smp[10 * i + k];
+  const uint32_t cnt = uint32_t(n);
+  NE_HIP(c, hipMemcpyAsync(g->batch.s, soa.data(), soa.size() * sizeof(float), hipMemcpyHostToDevice, c->stream));
+  NE_HIP(c, hipMemcpyAsync(g->batch.count, &cnt, sizeof(cnt), hipMemcpyHostToDevice, c->stream));
+  NE_HIP(c, hipMemsetAsync(g->net.table_grad, 0, g->table_n * sizeof(float), c->stream));
+  NE_HIP(c, launch_guide_grad(g->net, g->batch, g->train, c->stream));
+  float lw[2] = {0, 0};
+  NE_HIP(c, hipMemcpyAsync(lw, g->train.loss, sizeof(lw), hipMemcpyDeviceToHost, c->stream));
+  NE_HIP(c, hipStreamSynchronize(c->stream));
+  g->last_loss = lw[1] > 0 ? lw[0] / lw[1] : 0.0f;
+  if (loss) *loss = g->last_loss;
+  g->samples += uint64_t(n);
+  if (update) {
+    g->steps += 1;
+    NE_HIP(c, launch_guide_adam(g->net, g->cfg.lr_table, g->cfg.lr_mlp, g->cfg.beta1, g->cfg.beta2, g->cfg.eps,
+                                int(g->steps), c->stream));
+    NE_HIP(c, hipStreamSynchronize(c->stream));
+  }
+  return NE_OK;
+}
+
+ne_status ne_guide_sizes(ne_guide* g, uint64_t* table_floats, int32_t* mlp_floats, int32_t* level_res) {
+  if (!g) return fail(nullptr, NE_ERR_INVALID_ARG, "guide is NULL");
+  if (table_floats) *table_floats = g->table_n;
+  if (mlp_floats) *mlp_floats = kGuideParams;
+  if (level_res)
+    for (int l = 0; l < kGuideLevels; ++l) level_res[l] = g->net.res[l];
+  return NE_OK;
+}
+
+ne_status ne_guide_get_params(ne_guide* g, float* table, float* mlp) {
+  if (!g) return fail(nullptr, NE_ERR_INVALID_ARG, "guide is NULL");
+  ne_ctx* c = g->ctx;
+  std::lock_guard<std::mutex> lk(c->mu);
+  NE_HIP(c, hipSetDevice(c->device));
+  if (table) NE_HIP(c, hipMemcpy(table, g->net.table, g->table_n * sizeof(float), hipMemcpyDeviceToHost));
+  if (mlp) NE_HIP(c, hipMemcpy(mlp, g->net.mlp, kGuideParams * sizeof(float), hipMemcpyDeviceToHost));
+  return NE_OK;
+}
+
+ne_status ne_guide_set_params(ne_guide* g, const float* table, const float* mlp) {
+  if (!g) return fail(nullptr, NE_ERR_INVALID_ARG, "guide is NULL");
+  ne_ctx* c = g->ctx;
+  std::lock_guard<std::mutex> lk(c->mu);
+  NE_HIP(c, hipSetDevice(c->device));
+  if (table) NE_HIP(c, hipMemcpy(g->net.table, table, g->table_n * sizeof(float), hipMemcpyHostToDevice));
+  if (mlp) NE_HIP(c, hipMemcpy(g->net.mlp, mlp, kGuideParams * sizeof(float), hipMemcpyHostToDevice));
+  NE_HIP(c, launch_guide_pack(g->net, c->stream));
+  NE_HIP(c, hipStreamSynchronize(c->stream));
+  return NE_OK;
+}
+
+ne_status ne_guide_get_grads(ne_guide* g, float* table_grad, float* mlp_grad) {
+  if (!g) return fail(nullptr, NE_ERR_INVALID_ARG, "guide is NULL");
+  ne_ctx* c = g->ctx;
+  std::lock_guard<std::mutex> lk(c->mu);
+  NE_HIP(c, hipSetDevice(c->device));
+  if (table_grad) NE_HIP(c, hipMemcpy(table_grad, g->net.table_grad, g->table_n * sizeof(float), hipMemcpyDeviceToHost));
+  if (mlp_grad) NE_HIP(c, hipMemcpy(mlp_grad, g->net.mlp_grad, kGuideParams * sizeof(float), hipMemcpyDeviceToHost));
+  return NE_OK;
+}

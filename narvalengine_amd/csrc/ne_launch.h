@@ -137,6 +137,69 @@ hipError_t launch_batch(const BatchArgs& b, size_t film_words, uint32_t grid_blo
                         hipStream_t st);
 hipError_t launch_fixed_to_double(const long long* in, double* out, size_t n, hipStream_t st);
 
+// ---- learned directional guide ------------------------------------------------
+// Hash-grid encoding (kGuideLevels x 2 features) + MLP 35 -> 64 x 4 (ReLU) -> 40
+// (8 vMF lobes x {logit, mean xyz, log kappa}), widths padded to multiples of
+// 16 for the 16x16x16 BF16 MFMA tiles: layer l maps kGuideIn[l] -> kGuideOut[l].
+constexpr int kGuideLevels = 16, kGuideFeat = 2, kGuideLobes = 8, kGuideLayers = 5;
+constexpr int kGuideEnc = kGuideLevels * kGuideFeat;  // 32
+constexpr int kGuideInputs = kGuideEnc + 3;           // 35: encoding + direction
+constexpr int kGuideOutputs = kGuideLobes * 5;        // 40
+constexpr int kGuideWidth = 64;                       // activation row width in memory
+__host__ __device__ constexpr int guide_in(int l) { return l == 0 ? 48 : 64; }
+__host__ __device__ constexpr int guide_out(int l) { return l == kGuideLayers - 1 ? 48 : 64; }
+__host__ __device__ constexpr int guide_w_off(int l) {
+  return l == 0 ? 0 : guide_w_off(l - 1) + guide_in(l - 1) * guide_out(l - 1);
+}
+constexpr int kGuideWeights = guide_w_off(kGuideLayers);                                  // 18432
+__host__ __device__ constexpr int guide_b_off(int l) {
+  return kGuideWeights + (l == 0 ? 0 : guide_b_off(l - 1) - kGuideWeights + guide_out(l - 1));
+}
+constexpr int kGuideParams = guide_b_off(kGuideLayers);                                   // + 304 biases
+
+struct GuideNet {
+  float* table;        // kGuideLevels * T * 2 hash features (FP32, read by the encoder)
+  float* table_grad;
+  float* table_m;
+  float* table_v;
+  uint32_t log2T;
+  int32_t res[kGuideLevels];  // grid resolution per level
+  float* mlp;          // kGuideParams FP32 master: W_l [in][out] row-major, then biases
+  float* mlp_grad;
+  float* mlp_m;
+  float* mlp_v;
+  uint16_t* wf;        // BF16 W_l^T [out][in] per layer (forward operand), offsets guide_w_off
+  uint16_t* wb;        // BF16 W_l [in][out] per layer (backward operand)
+};
+
+// Training batch (SoA floats): unit-cube position xyz, incoming direction xyz,
+// sampled direction xyz, weight r = Li * f / pdf; n = *count (<= cap).
+struct GuideBatch {
+  float* s;            // 10 x cap floats
+  uint32_t* count;
+  uint32_t cap;
+};
+struct GuideTrain {
+  uint16_t* act;       // 5 layers x cap x 64 BF16 layer inputs
+  uint16_t* dy;        // cap x 64 BF16 output gradient of the current layer
+  uint16_t* dy2;       // cap x 64 BF16 (ping-pong)
+  float* denc;         // cap x 32 encoding gradient
+  float* partial;      // waves x kGuideParams weight/bias gradient partials
+  float* loss;         // [0] summed loss, [1] summed weight
+  int32_t waves;
+};
+
+hipError_t launch_guide_init(const GuideNet& g, uint64_t seed, hipStream_t st);
+hipError_t launch_guide_pack(const GuideNet& g, hipStream_t st);  // FP32 master -> BF16 operands
+// outputs: n x 40 mixture parameters (weights, unit means, kappa)
+hipError_t launch_guide_eval(const GuideNet& g, int n, const float* in6, float* out40, hipStream_t st);
+// raw network outputs (n x 40) before the head, for parity tests
+hipError_t launch_guide_raw(const GuideNet& g, int n, const float* in6, float* out40, hipStream_t st);
+// one step on the batch: forward, loss gradient, backward into mlp_grad / table_grad (no update)
+hipError_t launch_guide_grad(const GuideNet& g, const GuideBatch& b, const GuideTrain& t, hipStream_t st);
+hipError_t launch_guide_adam(const GuideNet& g, float lr_table, float lr_mlp, float beta1, float beta2, float eps,
+                             int step, hipStream_t st);
+
 hipError_t launch_reduce_partials(const RenderArgs& a, double* film, hipStream_t st);
 hipError_t launch_clear_rect(double* film, int W, int x0, int y0, int x1, int y1, hipStream_t st);
 hipError_t launch_tonemap(const double* film, int npix, float inv_spp, float* pixels, hipStream_t st);

@@ -5,7 +5,7 @@ import ctypes as C
 
 import numpy as np
 
-from .abi import (BatchConfig, Counters, Job, NarvalError, RenderParams, SceneInfo, check, lib, NE_FP32,
+from .abi import (BatchConfig, Counters, GuideConfig, GuideStats, Job, NarvalError, RenderParams, SceneInfo, check, lib, NE_FP32,
                   NE_FLAG_COUNT, NE_FLAG_TIME, NE_FLAG_STAGE_TIME, STAGES)
 
 
@@ -263,6 +263,73 @@ class Batch:
     def close(self):
         if self.h:
             self.ctx._lib.ne_batch_release(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Guide:
+    """The learned directional guide (ne_guide)."""
+
+    def __init__(self, ctx: Context, log2_table: int = 19, base_res: int = 16, max_res: int = 512,
+                 batch: int = 65536, lr_table: float = 1e-2, lr_mlp: float = 1e-3, beta1: float = 0.9,
+                 beta2: float = 0.99, eps: float = 1e-8, seed: int = 0):
+        self.ctx = ctx
+        cfg = GuideConfig(log2_table, base_res, max_res, batch, lr_table, lr_mlp, beta1, beta2, eps, seed)
+        h = C.c_void_p()
+        ctx.check(ctx._lib.ne_guide_create(ctx.h, C.byref(cfg), C.byref(h)))
+        self.h = h
+        tn, mn = C.c_uint64(), C.c_int32()
+        res = (C.c_int32 * 16)()
+        ctx.check(ctx._lib.ne_guide_sizes(self.h, C.byref(tn), C.byref(mn), res))
+        self.table_floats, self.mlp_floats, self.level_res = int(tn.value), int(mn.value), list(res)
+        self.log2_table = log2_table
+
+    def eval(self, queries: np.ndarray, raw: bool = False) -> np.ndarray:
+        q = np.ascontiguousarray(queries, np.float32).reshape(-1, 6)
+        out = np.zeros((len(q), 40), np.float32)
+        self.ctx.check(self.ctx._lib.ne_guide_eval(self.h, len(q), _fp(q), 1 if raw else 0, _fp(out)))
+        return out
+
+    def train_step(self, samples: np.ndarray, update: bool = True) -> float:
+        s = np.ascontiguousarray(samples, np.float32).reshape(-1, 10)
+        loss = C.c_float()
+        self.ctx.check(self.ctx._lib.ne_guide_train_step(self.h, len(s), _fp(s), 1 if update else 0, C.byref(loss)))
+        return float(loss.value)
+
+    def params(self):
+        t = np.zeros(self.table_floats, np.float32)
+        m = np.zeros(self.mlp_floats, np.float32)
+        self.ctx.check(self.ctx._lib.ne_guide_get_params(self.h, _fp(t), _fp(m)))
+        return t, m
+
+    def set_params(self, table=None, mlp=None):
+        t = None if table is None else np.ascontiguousarray(table, np.float32)
+        m = None if mlp is None else np.ascontiguousarray(mlp, np.float32)
+        self.ctx.check(self.ctx._lib.ne_guide_set_params(self.h, _fp(t) if t is not None else None,
+                                                         _fp(m) if m is not None else None))
+
+    def grads(self):
+        t = np.zeros(self.table_floats, np.float32)
+        m = np.zeros(self.mlp_floats, np.float32)
+        self.ctx.check(self.ctx._lib.ne_guide_get_grads(self.h, _fp(t), _fp(m)))
+        return t, m
+
+    def stats(self) -> dict:
+        s = GuideStats()
+        self.ctx.check(self.ctx._lib.ne_guide_get_stats(self.h, C.byref(s)))
+        return {"steps": int(s.steps), "samples": int(s.samples), "last_loss": float(s.last_loss)}
+
+    def reset(self):
+        self.ctx.check(self.ctx._lib.ne_guide_reset(self.h))
+
+    def close(self):
+        if self.h:
+            self.ctx._lib.ne_guide_release(self.h)
             self.h = None
 
     def __del__(self):
