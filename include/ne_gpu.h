@@ -288,6 +288,10 @@ typedef struct ne_guide_config {
   int32_t max_res;    /* finest level resolution (default 512) */
   int32_t batch;      /* training samples per step (default 65536) */
   float lr_table, lr_mlp, beta1, beta2, eps; /* Adam (defaults 1e-2, 1e-3, 0.9, 0.99, 1e-8) */
+  int32_t guided_vertices; /* the first this many scattering vertices of a path use the guide
+                              (default 3): each mixture-sampled vertex scales the throughput by
+                              phase / mixture pdf <= 2, so the product stays bounded */
+  int32_t reserved;
   uint64_t seed;      /* parameter initialisation */
 } ne_guide_config;
 typedef struct ne_guide_stats {
@@ -311,6 +315,14 @@ ne_status ne_guide_eval(ne_guide* guide, int32_t n, const float* queries, int32_
  * gradient of -sum r log q / sum r and, if update != 0, an Adam step.  loss:
  * the weighted loss before the step. */
 ne_status ne_guide_train_step(ne_guide* guide, int32_t n, const float* samples, int32_t update, float* loss);
+/* ne_render with the guide: the first guided_vertices medium scattering
+ * vertices of every path sample the one-sample MIS mixture 0.5 HG + 0.5 guide,
+ * later ones HG (FP32 wavefront pipeline).  With
+ * train != 0 the first guided vertex of every path is recorded; when the path
+ * ends it becomes a training sample (weight Li * phase / pdf), and an Adam step
+ * runs whenever half a batch is queued. */
+ne_status ne_render_guided(ne_ctx* ctx, const ne_scene* scene, const ne_camera_desc* cam,
+                           const ne_render_params* params, ne_film* film, ne_guide* guide, int32_t train);
 /* Sizes: hash-table floats, MLP floats (weights [in][out] per layer, padded to
  * 48/64, then biases) and the 16 level resolutions. */
 ne_status ne_guide_sizes(ne_guide* guide, uint64_t* table_floats, int32_t* mlp_floats, int32_t* level_res);

@@ -93,7 +93,7 @@ class GuideConfig(C.Structure):
     """ne_guide_config."""
     _fields_ = [("log2_table", C.c_int32), ("base_res", C.c_int32), ("max_res", C.c_int32), ("batch", C.c_int32),
                 ("lr_table", C.c_float), ("lr_mlp", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float),
-                ("eps", C.c_float), ("seed", C.c_uint64)]
+                ("eps", C.c_float), ("guided_vertices", C.c_int32), ("reserved", C.c_int32), ("seed", C.c_uint64)]
 
 
 class GuideStats(C.Structure):
@@ -139,6 +139,8 @@ SIGNATURES = [
     ("ne_batch_job_times", C.c_int32, [C.c_void_p, _F]),
     ("ne_batch_read", C.c_int32, [C.c_void_p, C.c_int32, C.POINTER(C.c_double)]),
     ("ne_guide_create", C.c_int32, [C.c_void_p, C.POINTER(GuideConfig), C.POINTER(C.c_void_p)]),
+    ("ne_render_guided", C.c_int32, [C.c_void_p, C.c_void_p, C.POINTER(CameraDesc), C.POINTER(RenderParams),
+                                     C.c_void_p, C.c_void_p, C.c_int32]),
     ("ne_guide_release", None, [C.c_void_p]),
     ("ne_guide_reset", C.c_int32, [C.c_void_p]),
     ("ne_guide_get_stats", C.c_int32, [C.c_void_p, C.POINTER(GuideStats)]),

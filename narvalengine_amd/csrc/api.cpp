@@ -42,6 +42,8 @@ struct ne_ctx {
   std::vector<int8_t> ev_stages;    // stage of the launch after each recorded event
   bool stage_timed = false;
   uint32_t refill_blocks = 0;  // resident lane-refill blocks on the device (CUs x blocks per CU)
+  void* wfg_buf = nullptr;      // guided-mode training records (grown on demand)
+  uint32_t wfg_P = 0;
 };
 
 struct ne_scene {
@@ -314,6 +316,7 @@ static void ctx_free(ne_ctx* c) {
   if (c->wf_buf) (void)hipFree(c->wf_buf);
   if (c->h_status) (void)hipHostFree(c->h_status);
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
+  if (c->wfg_buf) (void)hipFree(c->wfg_buf);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -564,7 +567,7 @@ ne_status ne_film_tonemap(ne_film* f, int32_t spp, float* pixels) {
 }
 
 static ne_status render_wavefront(ne_ctx* c, const ne_scene* s, const DevCamera& dc, const ne_render_params* p,
-                                  ne_film* f, bool count, bool timed, StageTimer* tm) {
+                                  ne_film* f, bool count, bool timed, StageTimer* tm, WfGuideHook* gh = nullptr) {
   const uint64_t total = uint64_t(p->x1 - p->x0) * uint64_t(p->y1 - p->y0) * uint64_t(p->spp);
   // pool slots: whole chunks, no more than the job needs.  Default (measured
   // on MI355X): 2^23 for surface-only scenes, where fewer, larger BVH launches
@@ -607,8 +610,24 @@ static ne_status render_wavefront(ne_ctx* c, const ne_scene* s, const DevCamera&
   a.counters = c->d_counters;
   a.film_fixed = f->d_fixed;
   a.film = f->d_sum;
+  a.gstate[0] = a.gstate[1] = nullptr;
+  if (gh) {
+    if (P > c->wfg_P) {
+      if (c->wfg_buf) {
+        NE_HIP(c, hipStreamSynchronize(c->stream));
+        NE_HIP(c, hipFree(c->wfg_buf));
+        c->wfg_buf = nullptr;
+        c->wfg_P = 0;
+      }
+      NE_HIP(c, hipMalloc(&c->wfg_buf, wf_guide_bytes(uint32_t(P))));
+      c->wfg_P = uint32_t(P);
+    }
+    wf_guide_carve(c->wfg_buf, uint32_t(P), gh->g);
+    a.gstate[0] = gh->g.rec[0].state;
+    a.gstate[1] = gh->g.rec[1].state;
+  }
   if (timed) NE_HIP(c, hipEventRecord(c->ev0, c->stream));
-  NE_HIP(c, launch_wavefront(s->dev, dc, w, a, count, c->stream, c->h_status, tm));
+  NE_HIP(c, launch_wavefront(s->dev, dc, w, a, count, c->stream, c->h_status, tm, gh));
   if (timed) {
     NE_HIP(c, hipEventRecord(c->ev1, c->stream));
     c->timed = true;
@@ -1214,12 +1233,14 @@ bool guide_alloc(ne_guide* g, T*& p, size_t count) {
 ne_status ne_guide_create(ne_ctx* c, const ne_guide_config* cfg_in, ne_guide** out) {
   if (!c || !out) return fail(c, NE_ERR_INVALID_ARG, "ctx/out is NULL");
   *out = nullptr;
-  ne_guide_config cfg{19, 16, 512, 65536, 1e-2f, 1e-3f, 0.9f, 0.99f, 1e-8f, 0};
+  ne_guide_config cfg{19, 16, 512, 65536, 1e-2f, 1e-3f, 0.9f, 0.99f, 1e-8f, 3, 0, 0};
   if (cfg_in) cfg = *cfg_in;
   if (cfg.log2_table < 10 || cfg.log2_table > 22) return fail(c, NE_ERR_INVALID_ARG, "log2_table must be in [10, 22]");
   if (cfg.base_res < 2 || cfg.max_res < cfg.base_res || cfg.max_res > 4096)
     return fail(c, NE_ERR_INVALID_ARG, "level resolutions out of range");
   if (cfg.batch < 16 || cfg.batch > (1 << 22)) return fail(c, NE_ERR_INVALID_ARG, "batch must be in [16, 2^22]");
+  if (cfg.guided_vertices < 0 || cfg.guided_vertices > kMaxVertices)
+    return fail(c, NE_ERR_INVALID_ARG, "guided_vertices out of range");
   if (!(cfg.lr_table >= 0) || !(cfg.lr_mlp >= 0) || !(cfg.beta1 >= 0 && cfg.beta1 < 1) ||
       !(cfg.beta2 >= 0 && cfg.beta2 < 1) || !(cfg.eps > 0))
     return fail(c, NE_ERR_INVALID_ARG, "Adam parameters out of range");
@@ -1385,4 +1406,50 @@ ne_status ne_guide_get_grads(ne_guide* g, float* table_grad, float* mlp_grad) {
   if (table_grad) NE_HIP(c, hipMemcpy(table_grad, g->net.table_grad, g->table_n * sizeof(float), hipMemcpyDeviceToHost));
   if (mlp_grad) NE_HIP(c, hipMemcpy(mlp_grad, g->net.mlp_grad, kGuideParams * sizeof(float), hipMemcpyDeviceToHost));
   return NE_OK;
+}
+
+ne_status ne_render_guided(ne_ctx* c, const ne_scene* s, const ne_camera_desc* cam, const ne_render_params* p,
+                           ne_film* f, ne_guide* g, int32_t train) {
+  if (!c) return fail(nullptr, NE_ERR_INVALID_ARG, "ctx is NULL");
+  if (!g || g->ctx != c) return fail(c, NE_ERR_INVALID_ARG, "guide is NULL or from another context");
+  if (!p || p->precision != NE_FP32 || p->mode == NE_MODE_MEGAKERNEL)
+    return fail(c, NE_ERR_INVALID_ARG, "guided rendering runs the FP32 wavefront pipeline");
+  std::lock_guard<std::mutex> lk(c->mu);
+  ne_status st = check_params(c, p);
+  if (st != NE_OK) return st;
+  if (!s || !cam || !f) return fail(c, NE_ERR_INVALID_ARG, "scene/camera/film is NULL");
+  if (s->ctx != c || f->ctx != c) return fail(c, NE_ERR_INVALID_ARG, "scene/film belong to another context");
+  if (f->W != p->width || f->H != p->height) return fail(c, NE_ERR_INVALID_ARG, "film size differs from params");
+  NE_HIP(c, hipSetDevice(c->device));
+  const bool count = (p->flags & NE_FLAG_COUNT) != 0;
+  const bool timed = (p->flags & NE_FLAG_TIME) != 0;
+  if (count) {
+    NE_HIP(c, hipMemsetAsync(c->d_counters, 0, sizeof(DevCounters), c->stream));
+    c->counters_valid = true;
+  }
+  StageTimer timer{&c->ev_pool, &c->ev_stages, c->stream};
+  StageTimer* tm = nullptr;
+  c->stage_timed = false;
+  if (p->flags & NE_FLAG_STAGE_TIME) {
+    c->ev_stages.clear();
+    tm = &timer;
+  }
+  WfGuideHook hook{};
+  hook.g.net = g->net;
+  hook.g.batch = g->batch;
+  hook.g.train = train ? 1 : 0;
+  hook.g.max_depth = g->cfg.guided_vertices;
+  hook.train = g->train;
+  hook.lr_table = g->cfg.lr_table; hook.lr_mlp = g->cfg.lr_mlp;
+  hook.beta1 = g->cfg.beta1; hook.beta2 = g->cfg.beta2; hook.eps = g->cfg.eps;
+  hook.min_batch = g->batch.cap / 2;
+  hook.steps = g->steps;
+  hook.samples = 0;
+  if (train) NE_HIP(c, hipMemsetAsync(g->batch.count, 0, sizeof(uint32_t), c->stream));
+  const DevCamera dc = make_camera(cam);
+  st = render_wavefront(c, s, dc, p, f, count, timed, tm, &hook);
+  c->stage_timed = st == NE_OK && tm;
+  g->steps = hook.steps;
+  g->samples += hook.samples;
+  return st;
 }

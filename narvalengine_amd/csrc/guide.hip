@@ -20,167 +20,9 @@
 // per-wave partial sums in a fixed order (deterministic), hash-feature
 // gradients are scattered with atomics.  Adam updates FP32 masters; BF16
 // operand copies are re-packed after each step.
-#include "ne_launch.h"
-#include "ne_math.h"
+#include "ne_guide.h"
 
 namespace ne {
-
-typedef short bf16x4 __attribute__((ext_vector_type(4)));
-typedef float f32x4 __attribute__((ext_vector_type(4)));
-
-constexpr int kLdA = 68;  // LDS row stride (BF16) of an activation tile: conflict-free column reads
-constexpr int kLdO = 49;  // LDS row stride (FP32) of an output tile
-constexpr int kGuideBlock = 256;
-constexpr int kWavesPerBlock = kGuideBlock / 64;
-
-// Orders one wave's LDS writes before other lanes of the same wave read them.
-__device__ __forceinline__ void wave_sync() {
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// ---- BF16 ---------------------------------------------------------------------
-__host__ __device__ inline uint16_t f2bf(float f) {  // round to nearest even
-  uint32_t u;
-  __builtin_memcpy(&u, &f, 4);
-  u += 0x7FFFu + ((u >> 16) & 1u);
-  return uint16_t(u >> 16);
-}
-__host__ __device__ inline float bf2f(uint16_t h) {
-  uint32_t u = uint32_t(h) << 16;
-  float f;
-  __builtin_memcpy(&f, &u, 4);
-  return f;
-}
-
-// ---- hash grid ----------------------------------------------------------------
-__device__ __forceinline__ uint32_t guide_cell_index(const GuideNet& g, int l, uint32_t x, uint32_t y, uint32_t z) {
-  const uint32_t T = 1u << g.log2T;
-  const uint32_t r1 = uint32_t(g.res[l]) + 1u;
-  if ((unsigned long long)r1 * r1 * r1 <= T) return x + r1 * (y + r1 * z);  // dense level
-  return (x ^ (y * 2654435761u) ^ (z * 805459861u)) & (T - 1u);
-}
-
-// Trilinear corner k (bit 0 x, bit 1 y, bit 2 z) of level l at unit position p:
-// table word offset of feature 0 and the weight.
-__device__ __forceinline__ void guide_corner(const GuideNet& g, int l, Vec3<float> p, int k, uint32_t& off,
-                                             float& w) {
-  const float r = float(g.res[l]);
-  const float sx = xmin(xmax(p.x, 0.0f), 1.0f) * r, sy = xmin(xmax(p.y, 0.0f), 1.0f) * r,
-              sz = xmin(xmax(p.z, 0.0f), 1.0f) * r;
-  // lower corner in [0, res - 1] so the upper corner stays inside the level's grid
-  const float fx = xmin(floorf(sx), r - 1.0f), fy = xmin(floorf(sy), r - 1.0f), fz = xmin(floorf(sz), r - 1.0f);
-  const float tx = sx - fx, ty = sy - fy, tz = sz - fz;
-  const uint32_t x = uint32_t(fx) + (k & 1), y = uint32_t(fy) + ((k >> 1) & 1), z = uint32_t(fz) + ((k >> 2) & 1);
-  w = ((k & 1) ? tx : 1.0f - tx) * (((k >> 1) & 1) ? ty : 1.0f - ty) * (((k >> 2) & 1) ? tz : 1.0f - tz);
-  const uint32_t T = 1u << g.log2T;
-  off = (uint32_t(l) * T + guide_cell_index(g, l, x, y, z)) * kGuideFeat;
-}
-
-// ---- MLP tile (one wave, 16 rows) -------------------------------------------------
-// xs: LDS [16][kLdA] BF16 inputs of layer l; ys: LDS outputs.  Hidden layers
-// write ReLU(h) as BF16 into ys; the last layer writes FP32 into out [16][kLdO].
-template <int L>
-__device__ __forceinline__ void guide_layer(const GuideNet& g, const uint16_t* xs, uint16_t* ys, float* out) {
-  constexpr int KIN = guide_in(L), NOUT = guide_out(L);
-  const int lane = __lane_id();
-  const int r = lane & 15, kq = 4 * (lane >> 4);
-  f32x4 acc[NOUT / 16];
-#pragma unroll
-  for (int nt = 0; nt < NOUT / 16; ++nt) acc[nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
-  const uint16_t* wf = g.wf + guide_w_off(L);
-#pragma unroll
-  for (int ks = 0; ks < KIN / 16; ++ks) {
-    const bf16x4 a = *reinterpret_cast<const bf16x4*>(xs + r * kLdA + 16 * ks + kq);
-#pragma unroll
-    for (int nt = 0; nt < NOUT / 16; ++nt) {
-      const bf16x4 b = *reinterpret_cast<const bf16x4*>(wf + (16 * nt + r) * KIN + 16 * ks + kq);
-      acc[nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[nt], 0, 0, 0);
-    }
-  }
-  const float* bias = g.mlp + guide_b_off(L);
-#pragma unroll
-  for (int nt = 0; nt < NOUT / 16; ++nt) {
-    const int col = 16 * nt + r;
-    const float bc = bias[col];
-#pragma unroll
-    for (int j = 0; j < 4; ++j) {
-      const int row = kq + j;
-      const float h = acc[nt][j] + bc;
-      if (L < kGuideLayers - 1) ys[row * kLdA + col] = f2bf(h > 0.0f ? h : 0.0f);
-      else out[row * kLdO + col] = h;
-    }
-  }
-}
-
-// Encoding + direction of 16 queries into xs (BF16 [16][kLdA], columns 35..47 zero).
-// pos/dir of row i are read through the accessor f(i, pos, dir, valid).
-template <typename F>
-__device__ __forceinline__ void guide_encode_tile(const GuideNet& g, uint16_t* xs, F&& f) {
-  const int lane = __lane_id();
-  const int row = lane & 15;
-  Vec3<float> p, d;
-  bool valid;
-  f(row, p, d, valid);
-#pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    const int l = (lane >> 4) + 4 * i;
-    float e0 = 0.0f, e1 = 0.0f;
-    if (valid) {
-#pragma unroll
-      for (int k = 0; k < 8; ++k) {
-        uint32_t off;
-        float w;
-        guide_corner(g, l, p, k, off, w);
-        e0 += w * g.table[off];
-        e1 += w * g.table[off + 1];
-      }
-    }
-    xs[row * kLdA + 2 * l] = f2bf(e0);
-    xs[row * kLdA + 2 * l + 1] = f2bf(e1);
-  }
-  if (lane < 16) {
-    xs[row * kLdA + 32] = f2bf(valid ? d.x : 0.0f);
-    xs[row * kLdA + 33] = f2bf(valid ? d.y : 0.0f);
-    xs[row * kLdA + 34] = f2bf(valid ? d.z : 0.0f);
-  } else {
-    for (int c = 35 + (lane >> 4) - 1; c < 64; c += 3) xs[row * kLdA + c] = 0;
-  }
-}
-
-// Full forward of one tile: out [16][kLdO] raw outputs (40 used).
-__device__ __forceinline__ void guide_mlp_tile(const GuideNet& g, uint16_t* xa, uint16_t* xb, float* out) {
-  wave_sync();
-  guide_layer<0>(g, xa, xb, out);
-  wave_sync();
-  guide_layer<1>(g, xb, xa, out);
-  wave_sync();
-  guide_layer<2>(g, xa, xb, out);
-  wave_sync();
-  guide_layer<3>(g, xb, xa, out);
-  wave_sync();
-  guide_layer<4>(g, xa, xb, out);
-  wave_sync();
-}
-
-// Mixture head of one output row: pi (softmax of logits), unit means, kappa.
-__device__ __forceinline__ void guide_head(const float* y, float* mix) {
-  float mx = y[0];
-  for (int k = 1; k < kGuideLobes; ++k) mx = xmax(mx, y[5 * k]);
-  float z = 0.0f;
-  for (int k = 0; k < kGuideLobes; ++k) z += expf(y[5 * k] - mx);
-  for (int k = 0; k < kGuideLobes; ++k) {
-    const float* o = y + 5 * k;
-    const float n = sqrtf(o[1] * o[1] + o[2] * o[2] + o[3] * o[3]);
-    const float inv = n > 1e-12f ? 1.0f / n : 0.0f;
-    mix[5 * k] = expf(o[0] - mx) / z;
-    mix[5 * k + 1] = n > 1e-12f ? o[1] * inv : 0.0f;
-    mix[5 * k + 2] = n > 1e-12f ? o[2] * inv : 0.0f;
-    mix[5 * k + 3] = n > 1e-12f ? o[3] * inv : 1.0f;
-    mix[5 * k + 4] = expf(xmin(o[4], 8.0f));
-  }
-}
 
 __global__ __launch_bounds__(kGuideBlock) void k_guide_eval(GuideNet g, int n, const float* in6, float* out40,
                                                             int raw) {
@@ -211,13 +53,6 @@ __global__ __launch_bounds__(kGuideBlock) void k_guide_eval(GuideNet g, int n, c
       }
     }
   }
-}
-
-// ---- vMF mixture ------------------------------------------------------------------
-// vMF density kappa / (2 pi (1 - exp(-2 kappa))) exp(kappa (mu.w - 1)), stable for
-// large kappa; the normaliser equals kappa / (4 pi sinh kappa) exp(kappa).
-__device__ __forceinline__ float vmf_pdf(float kappa, float cosv) {
-  return kappa / (float(kTWO_PI) * (1.0f - expf(-2.0f * kappa))) * expf(kappa * (cosv - 1.0f));
 }
 
 // ---- training: forward + loss gradient --------------------------------------------
@@ -285,7 +120,7 @@ __global__ __launch_bounds__(kGuideBlock) void k_guide_train_fwd(GuideNet g, Gui
           pi[k] = expf(o[0] - mx) / z;
           const float nm = xmax(sqrtf(o[1] * o[1] + o[2] * o[2] + o[3] * o[3]), 1e-12f);
           const float c = (o[1] * w.x + o[2] * w.y + o[3] * w.z) / nm;
-          const float kap = expf(xmin(o[4], 8.0f));
+          const float kap = guide_kappa(o[4]);
           v[k] = vmf_pdf(kap, c);
           qsum += pi[k] * v[k];
         }
@@ -299,13 +134,11 @@ __global__ __launch_bounds__(kGuideBlock) void k_guide_train_fwd(GuideNet g, Gui
           const float nm = xmax(sqrtf(o[1] * o[1] + o[2] * o[2] + o[3] * o[3]), 1e-12f);
           const float mu[3] = {o[1] / nm, o[2] / nm, o[3] / nm};
           const float c = mu[0] * w.x + mu[1] * w.y + mu[2] * w.z;
-          const float kap = expf(xmin(o[4], 8.0f));
+          const float kap = guide_kappa(o[4]);
           const float pv = s * pi[k] * v[k];
           const float wk[3] = {w.x, w.y, w.z};
           for (int a = 0; a < 3; ++a) dy[5 * k + 1 + a] = pv * kap * (wk[a] - c * mu[a]) / nm;
-          const float e2 = expf(-2.0f * kap);
-          const float dlog = 1.0f / kap - 2.0f * e2 / (1.0f - e2) + (c - 1.0f);
-          dy[5 * k + 4] = o[4] < 8.0f ? pv * kap * dlog : 0.0f;
+          dy[5 * k + 4] = (o[4] < 8.0f && o[4] > -10.0f) ? pv * kap * vmf_dlog_dkappa(kap, c) : 0.0f;
         }
         for (int c = 0; c < 48; ++c) t.dy[size_t(q) * kGuideWidth + c] = f2bf(c < kGuideOutputs ? dy[c] : 0.0f);
         for (int c = 48; c < 64; ++c) t.dy[size_t(q) * kGuideWidth + c] = 0;

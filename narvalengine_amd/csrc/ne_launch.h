@@ -102,12 +102,14 @@ struct WfArgs {
   DevCounters* counters;
   long long* film_fixed;
   double* film;
+  uint32_t* gstate[2];  // guided mode: training-record state per path buffer (else null)
 };
 
+struct WfGuideHook;  // guided mode: proposal, training records, online training
 // Renders a.total_paths paths through a pool of a.nchunks * kChunk slots;
-// returns after the film flush is enqueued.  h_status: pinned host word.
+// returns after the film flush is enqueued.  h_status: pinned host words.
 hipError_t launch_wavefront(const DevScene& s, const DevCamera& cam, WfState& w, WfArgs a, bool count,
-                            hipStream_t st, uint32_t* h_status, StageTimer* tm);
+                            hipStream_t st, uint32_t* h_status, StageTimer* tm, WfGuideHook* gh);
 // ---- persistent batched renderer -------------------------------------------
 constexpr int kMaxBatchJobs = 1024;
 struct BatchJob {
@@ -188,6 +190,38 @@ struct GuideTrain {
   float* loss;         // [0] summed loss, [1] summed weight
   int32_t waves;
 };
+
+// Guided wavefront: per-path training record (one per slot and path buffer).
+// state 0: none; 1: a guided vertex was sampled, the radiance baseline is
+// taken when the next segment is shaded; 2: recording until the path ends.
+struct WfGuideRec {
+  uint32_t* state;
+  float *px, *py, *pz;  // unit-cube position of the recorded vertex
+  float *dx, *dy, *dz;  // incoming direction
+  float *wx, *wy, *wz;  // sampled direction
+  float* ratio;         // phase / mixture pdf of the sampled direction
+  float *bx, *by, *bz;  // throughput after the vertex
+  float *lx, *ly, *lz;  // radiance gathered when the next segment starts
+};
+constexpr int kGuideRecF = 16, kGuideRecU = 1;
+struct WfGuide {
+  GuideNet net;
+  GuideBatch batch;
+  WfGuideRec rec[2];
+  int32_t train;      // collect training samples
+  int32_t max_depth;  // scattering vertices with depth < max_depth use the guide
+};
+
+struct WfGuideHook {
+  WfGuide g;
+  GuideTrain train;
+  float lr_table, lr_mlp, beta1, beta2, eps;
+  uint32_t min_batch;   // a training step runs once this many samples are queued
+  uint64_t steps;       // Adam steps (updated by the launcher)
+  uint64_t samples;     // samples consumed
+};
+size_t wf_guide_bytes(uint32_t P);
+void wf_guide_carve(void* base, uint32_t P, WfGuide& g);
 
 hipError_t launch_guide_init(const GuideNet& g, uint64_t seed, hipStream_t st);
 hipError_t launch_guide_pack(const GuideNet& g, hipStream_t st);  // FP32 master -> BF16 operands

@@ -30,8 +30,10 @@
 // megakernel's per-path Philox stream (each path carries the number of draws
 // consumed) and shadow rays use their vertex's sub-stream, so both modes trace
 // the same paths with the same arithmetic.
+#include <algorithm>
 #include <cstring>
 
+#include "ne_guide.h"
 #include "ne_launch.h"
 #include "ne_trace.h"
 
@@ -187,6 +189,7 @@ __global__ __launch_bounds__(kChunk) void k_wf_regen(WfState w, WfArgs a, DevCam
     A.rngn[slot] = rng.consumed();
     A.depth[slot] = kPrevDelta;  // depth 0, camera ray (delta)
     A.prev_pdf[slot] = 0.0f;
+    if (a.gstate[a.cur]) a.gstate[a.cur][slot] = 0u;
   }
   __syncthreads();  // every thread has read n0
   if (t == 0) w.cnt_paths[a.cur][c] = n0 + m;
@@ -343,7 +346,7 @@ template <bool MEDIUM>
 __device__ __forceinline__ bool nee_sample(const DevScene& s, Vec3<float> x, Vec3<float> d, const Glossy<float>& gm,
                                            Vec3<float> wo, Vec3<float> ns, Vec3<float> ss, Vec3<float> ts, float ul,
                                            float ua, float ub, Vec3<float>& wi, float& tmax, Vec3<float>& f,
-                                           Vec3<float>& Le, float& wt, float& pl) {
+                                           Vec3<float>& Le, float& wt, float& pl, bool& delta_out) {
   if (s.n_lights <= 0) return false;
   const float nl = float(s.n_lights);
   int li = int(ul * nl);
@@ -393,11 +396,24 @@ __device__ __forceinline__ bool nee_sample(const DevScene& s, Vec3<float> x, Vec
   if (is_black(f)) return false;
   tmax = xisinf(dist) ? xinf<float>() : dist * float(1 - 1e-4);
   wt = delta ? 1.0f : power_heuristic(pl, pf);
+  delta_out = delta;
   return true;
 }
 
-__global__ __launch_bounds__(kChunk) void k_wf_shade(WfState w, WfArgs a, DevScene s) {
+// Shading.  GUIDED: medium vertices sample a one-sample MIS mixture of HG and
+// the learned vMF proposal (0.5 / 0.5); the block evaluates the proposal for
+// its medium vertices in MFMA tiles (16 queries per wave) into LDS.  With
+// training on, the first guided vertex of a path is recorded and becomes a
+// training sample when the path ends: weight r = Li * phase / pdf, Li the
+// radiance gathered after the vertex divided by the throughput after it.
+template <bool GUIDED>
+__global__ __launch_bounds__(kChunk) void k_wf_shade(WfState w, WfArgs a, DevScene s, WfGuide gd) {
   __shared__ uint32_t wsum[kChunk / 64];
+  __shared__ float gq[GUIDED ? kChunk * 6 : 1];     // queries: unit position, direction
+  __shared__ float gmix[GUIDED ? kChunk * 40 : 1];  // mixture of each query
+  __shared__ uint16_t gxa[GUIDED ? 4 * 16 * kLdA : 1];
+  __shared__ uint16_t gxb[GUIDED ? 4 * 16 * kLdA : 1];
+  __shared__ float gout[GUIDED ? 4 * 16 * kLdO : 1];
   const WfPaths A = w.paths[a.cur], B = w.paths[a.cur ^ 1];
   const WfShadow S = w.shadow;
   const uint32_t c = blockIdx.x, t = threadIdx.x, i = c * kChunk + t;
@@ -407,9 +423,19 @@ __global__ __launch_bounds__(kChunk) void k_wf_shade(WfState w, WfArgs a, DevSce
   float tmax = 0.0f, wt = 0.0f, pl = 0.0f, ppdf = 0.0f;
   uint32_t pix = 0, smp = 0, rngn = 0, depth = 0;
   float lx = 0.0f, ly = 0.0f, lz = 0.0f;
+  // phase 1: what the segment hit
+  uint32_t kind = WF_KIND_MISS;
+  Vec3<float> o = v3s(0.0f), d = v3s(0.0f);
+  bool vertex = false, medium = false;
+  Glossy<float> gm;
+  Vec3<float> ns = v3s(0.0f), ss = v3s(0.0f), ts = v3s(0.0f);
+  // guided training record carried by the path
+  uint32_t rst = 0;
+  float rec[kGuideRecF];
   if (active) {
-    const uint32_t kind = A.kind[i];
-    const Vec3<float> o = v3(A.ox[i], A.oy[i], A.oz[i]), d = v3(A.dx[i], A.dy[i], A.dz[i]);
+    kind = A.kind[i];
+    o = v3(A.ox[i], A.oy[i], A.oz[i]);
+    d = v3(A.dx[i], A.dy[i], A.dz[i]);
     const uint32_t dep = A.depth[i];
     depth = dep & 0xFFFFu;
     const bool prev_delta = (dep & kPrevDelta) != 0;
@@ -417,10 +443,20 @@ __global__ __launch_bounds__(kChunk) void k_wf_shade(WfState w, WfArgs a, DevSce
     smp = A.smp[i];
     lx = A.Lx[i]; ly = A.Ly[i]; lz = A.Lz[i];
     beta = v3(A.bx[i], A.by[i], A.bz[i]);
+    if constexpr (GUIDED) {
+      const WfGuideRec& R = gd.rec[a.cur];
+      rst = R.state[i];
+      if (rst != 0) {
+        float* const src[kGuideRecF] = {R.px, R.py, R.pz, R.dx, R.dy, R.dz, R.wx, R.wy,
+                                        R.wz, R.ratio, R.bx, R.by, R.bz, R.lx, R.ly, R.lz};
+        for (int k = 0; k < kGuideRecF; ++k) rec[k] = src[k][i];
+        if (rst == 1) {  // radiance baseline: everything gathered before this segment
+          rec[13] = lx; rec[14] = ly; rec[15] = lz;
+          rst = 2;
+        }
+      }
+    }
     const float th = A.hit_t[i];
-    bool vertex = false;
-    Glossy<float> gm;
-    Vec3<float> ns, ss, ts;
     if (kind == WF_KIND_MISS) {
       if (s.env_type != 0) {
         const float wm = prev_delta ? 1.0f : power_heuristic(A.prev_pdf[i], float(1.0 / kFOUR_PI));
@@ -465,43 +501,98 @@ __global__ __launch_bounds__(kChunk) void k_wf_shade(WfState w, WfArgs a, DevSce
         ends = true;
       } else {
         vertex = true;
+        medium = true;
         beta = beta * v3(s.sig_s[0] / s.sig_t[0], s.sig_s[1] / s.sig_t[1], s.sig_s[2] / s.sig_t[2]);
       }
     }
-    if (vertex) {
-      const bool medium = kind == WF_KIND_MEDIUM;
-      x = o + th * d;
-      const Vec3<float> wo = -d;
-      PathRng rng;
-      rng.init(a.seed, pix, smp, a.job);
-      rng.seek(A.rngn[i]);
-      const float ul = rng.next(), ua = rng.next(), ub = rng.next();
-      shadow = medium ? nee_sample<true>(s, x, d, gm, wo, ns, ss, ts, ul, ua, ub, wi, tmax, f, Le, wt, pl)
-                      : nee_sample<false>(s, x, d, gm, wo, ns, ss, ts, ul, ua, ub, wi, tmax, f, Le, wt, pl);
-      if (shadow) f = beta * f * Le;  // NEE weight with the throughput before this vertex's BSDF weight
-      if (medium) {
-        const float u0 = rng.next(), u1 = rng.next();
-        nd = hg_sample(s.g, d, u0, u1);
-        ppdf = hg_p(s.g, dot(d, nd));
+    if (vertex) x = o + th * d;
+  }
+  // guided: proposal of every medium vertex of the block (all threads take part)
+  uint32_t qi = 0;
+  if constexpr (GUIDED) {
+    uint32_t nq;
+    qi = block_compact(medium && int(depth) < gd.max_depth, wsum, nq);
+    if (medium && int(depth) < gd.max_depth) {
+      const Vec3<float> p = xf_point<float>(s.inv, x);
+      float* q = gq + 6 * qi;
+      q[0] = p.x + 0.5f; q[1] = p.y + 0.5f; q[2] = p.z + 0.5f;
+      q[3] = d.x; q[4] = d.y; q[5] = d.z;
+    }
+    __syncthreads();
+    const int wv = int(t >> 6), lane = __lane_id();
+    for (uint32_t tile = uint32_t(wv); tile * 16 < nq; tile += 4) {
+      uint16_t* xa = gxa + wv * 16 * kLdA;
+      uint16_t* xb = gxb + wv * 16 * kLdA;
+      float* out = gout + wv * 16 * kLdO;
+      wave_sync();
+      guide_encode_tile(gd.net, xa, [&](int r, Vec3<float>& p, Vec3<float>& dd, bool& valid) {
+        const uint32_t k = tile * 16 + uint32_t(r);
+        valid = k < nq;
+        const float* q = gq + 6 * (valid ? k : 0u);
+        p = v3(q[0], q[1], q[2]);
+        dd = v3(q[3], q[4], q[5]);
+      });
+      guide_mlp_tile(gd.net, xa, xb, out);
+      if (lane < 16 && tile * 16 + uint32_t(lane) < nq) guide_head(out + lane * kLdO, gmix + 40 * (tile * 16 + lane));
+    }
+    __syncthreads();
+  }
+  // phase 2: next-event estimation and the continuation
+  if (active && vertex) {
+    const Vec3<float> wo = -d;
+    PathRng rng;
+    rng.init(a.seed, pix, smp, a.job);
+    rng.seek(A.rngn[i]);
+    const float ul = rng.next(), ua = rng.next(), ub = rng.next();
+    bool delta = false;
+    shadow = medium ? nee_sample<true>(s, x, d, gm, wo, ns, ss, ts, ul, ua, ub, wi, tmax, f, Le, wt, pl, delta)
+                    : nee_sample<false>(s, x, d, gm, wo, ns, ss, ts, ul, ua, ub, wi, tmax, f, Le, wt, pl, delta);
+    const bool guided = GUIDED && medium && int(depth) < gd.max_depth;
+    const float* mix = guided ? gmix + 40 * qi : nullptr;
+    if (guided && shadow && !delta)  // the phase-sampling strategy is the mixture
+      wt = power_heuristic(pl, 0.5f * hg_p(s.g, dot(d, wi)) + 0.5f * vmf_mixture_pdf(mix, wi));
+    if (shadow) f = beta * f * Le;  // NEE weight with the throughput before this vertex's BSDF weight
+    if (medium && !guided) {
+      const float u0 = rng.next(), u1 = rng.next();
+      nd = hg_sample(s.g, d, u0, u1);
+      ppdf = hg_p(s.g, dot(d, nd));
+      cont = true;
+    } else if (medium) {
+      const float us = rng.next(), u0 = rng.next(), u1 = rng.next(), u2 = rng.next();
+      nd = us < 0.5f ? vmf_mixture_sample(mix, u2, u0, u1) : hg_sample(s.g, d, u0, u1);
+      const float ph = hg_p(s.g, dot(d, nd));
+      const float pm = 0.5f * ph + 0.5f * vmf_mixture_pdf(mix, nd);
+      if (pm > 0.0f) {
+        ppdf = pm;
+        beta = beta * (ph / pm);
         cont = true;
-      } else {
-        const float u0 = rng.next(), u1 = rng.next(), u2 = rng.next();
-        const Vec3<float> wol = to_lcs(wo, ns, ss, ts);
-        const Vec3<float> wil = gm.sample(wol, u0, u1, u2);
-        const float pf = gm.pdf(wol, wil);
-        if (pf > 0.0f) {
-          const Vec3<float> fs = gm.eval(wol, wil) * fabsf(wil.z);
-          if (!is_black(fs)) {
-            ppdf = pf;
-            nd = to_world(wil, ns, ss, ts);
-            cont = true;
-            beta = beta * fs / pf;
-          }
+        if (gd.train && rst == 0) {  // record the first guided vertex of the path
+          const Vec3<float> p = xf_point<float>(s.inv, x);
+          rec[0] = p.x + 0.5f; rec[1] = p.y + 0.5f; rec[2] = p.z + 0.5f;
+          rec[3] = d.x; rec[4] = d.y; rec[5] = d.z;
+          rec[6] = nd.x; rec[7] = nd.y; rec[8] = nd.z;
+          rec[9] = ph / pm;
+          rec[10] = beta.x; rec[11] = beta.y; rec[12] = beta.z;
+          rst = 1;
         }
       }
-      ends = !cont;
-      rngn = rng.consumed();
+    } else {
+      const float u0 = rng.next(), u1 = rng.next(), u2 = rng.next();
+      const Vec3<float> wol = to_lcs(wo, ns, ss, ts);
+      const Vec3<float> wil = gm.sample(wol, u0, u1, u2);
+      const float pf = gm.pdf(wol, wil);
+      if (pf > 0.0f) {
+        const Vec3<float> fs = gm.eval(wol, wil) * fabsf(wil.z);
+        if (!is_black(fs)) {
+          ppdf = pf;
+          nd = to_world(wil, ns, ss, ts);
+          cont = true;
+          beta = beta * fs / pf;
+        }
+      }
     }
+    ends = !cont;
+    rngn = rng.consumed();
   }
   uint32_t ncont, nshadow;
   const uint32_t j = block_compact(cont, wsum, ncont);
@@ -515,8 +606,44 @@ __global__ __launch_bounds__(kChunk) void k_wf_shade(WfState w, WfArgs a, DevSce
     B.pix[b] = pix; B.smp[b] = smp; B.rngn[b] = rngn;
     B.depth[b] = depth + 1u;  // prev_delta cleared
     B.prev_pdf[b] = ppdf;
+    if constexpr (GUIDED) {
+      const WfGuideRec& R = gd.rec[a.cur ^ 1];
+      R.state[b] = rst;
+      if (rst != 0) {
+        float* const dst[kGuideRecF] = {R.px, R.py, R.pz, R.dx, R.dy, R.dz, R.wx, R.wy,
+                                        R.wz, R.ratio, R.bx, R.by, R.bz, R.lx, R.ly, R.lz};
+        for (int q = 0; q < kGuideRecF; ++q) dst[q][b] = rec[q];
+      }
+    }
   } else if (ends) {
     film_add(a.film_fixed, pix, lx, ly, lz);
+  }
+  if constexpr (GUIDED) {
+    // a recording path that ends now becomes a training sample
+    float r = 0.0f;
+    if (ends && !cont && rst == 2) {
+      const float dl[3] = {lx - rec[13], ly - rec[14], lz - rec[15]};
+      float li = 0.0f;
+      int nc = 0;
+      for (int q = 0; q < 3; ++q)
+        if (rec[10 + q] > 0.0f) {
+          li += dl[q] / rec[10 + q];
+          ++nc;
+        }
+      r = nc ? li / float(nc) * rec[9] : 0.0f;
+      if (!(r > 0.0f) || xisinf(r)) r = 0.0f;
+    }
+    uint32_t nsmp;
+    const uint32_t si = block_compact(r > 0.0f, wsum, nsmp);
+    __shared__ uint32_t sbase;
+    if (t == 0) sbase = nsmp ? atomicAdd(gd.batch.count, nsmp) : 0u;
+    __syncthreads();
+    const uint32_t q = sbase + si;
+    if (r > 0.0f && q < gd.batch.cap) {
+      const uint32_t cap = gd.batch.cap;
+      for (int e = 0; e < 9; ++e) gd.batch.s[size_t(e) * cap + q] = rec[e];
+      gd.batch.s[size_t(9) * cap + q] = r;
+    }
   }
   if (shadow) {
     const uint32_t q = c * kChunk + k;
@@ -684,13 +811,31 @@ void wf_carve(void* base, uint32_t P, WfState& w) {
   w.next_path = reinterpret_cast<unsigned long long*>(p);
 }
 
+// guided-mode record arrays: 2 buffers x (kGuideRecF floats + 1 state word) per slot
+size_t wf_guide_bytes(uint32_t P) { return size_t(2 * (kGuideRecF + kGuideRecU)) * align256(size_t(P) * 4); }
+
+void wf_guide_carve(void* base, uint32_t P, WfGuide& g) {
+  char* p = static_cast<char*>(base);
+  const size_t stride = align256(size_t(P) * 4);
+  for (int b = 0; b < 2; ++b) {
+    WfGuideRec& R = g.rec[b];
+    float** fl[kGuideRecF] = {&R.px, &R.py, &R.pz, &R.dx, &R.dy, &R.dz, &R.wx, &R.wy,
+                              &R.wz, &R.ratio, &R.bx, &R.by, &R.bz, &R.lx, &R.ly, &R.lz};
+    for (int k = 0; k < kGuideRecF; ++k, p += stride) *fl[k] = reinterpret_cast<float*>(p);
+    R.state = reinterpret_cast<uint32_t*>(p);
+    p += stride;
+  }
+}
+
+__global__ void k_guide_reset_count(uint32_t* count) { *count = 0u; }
+
 hipError_t wf_refill_blocks_per_cu(int* blocks) {
   return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, reinterpret_cast<const void*>(&k_wf_trace<false>),
                                                       int(kChunk), 0);
 }
 
 hipError_t launch_wavefront(const DevScene& s, const DevCamera& cam, WfState& w, WfArgs a, bool count,
-                            hipStream_t st, uint32_t* h_status, StageTimer* tm) {
+                            hipStream_t st, uint32_t* h_status, StageTimer* tm, WfGuideHook* gh) {
   hipError_t e = hipSuccess;
 #define NE_MARK(stage)                                          \
   do {                                                          \
@@ -708,11 +853,26 @@ hipError_t launch_wavefront(const DevScene& s, const DevCamera& cam, WfState& w,
     a.cur = int(it & 1u);
     NE_MARK(kStagePlan);
     hipLaunchKernelGGL(k_wf_plan, dim3(1), dim3(unsigned(kPlanThreads)), 0, st, w, a);
-    // every 8 iterations: stop once nothing is in flight (status written by plan)
+    // every 8 iterations: stop once nothing is in flight (status written by
+    // plan); guided training: a step on the queued samples when enough are in
     if ((it & 7u) == 7u) {
       if ((e = hipMemcpyAsync(h_status, w.status, sizeof(uint32_t), hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
+      if (gh && gh->g.train &&
+          (e = hipMemcpyAsync(h_status + 1, gh->g.batch.count, sizeof(uint32_t), hipMemcpyDeviceToHost, st)) !=
+              hipSuccess)
+        return e;
       if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
       if (h_status[0] == 0) break;
+      if (gh && gh->g.train && h_status[1] >= gh->min_batch) {
+        NE_MARK(kStageOther);
+        if ((e = launch_guide_grad(gh->g.net, gh->g.batch, gh->train, st)) != hipSuccess) return e;
+        gh->steps += 1;
+        gh->samples += std::min(h_status[1], gh->g.batch.cap);
+        if ((e = launch_guide_adam(gh->g.net, gh->lr_table, gh->lr_mlp, gh->beta1, gh->beta2, gh->eps,
+                                   int(gh->steps), st)) != hipSuccess)
+          return e;
+        hipLaunchKernelGGL(k_guide_reset_count, dim3(1), dim3(1), 0, st, gh->g.batch.count);
+      }
     }
     NE_MARK(kStageRegen);
     hipLaunchKernelGGL(k_wf_regen, grid, blk, 0, st, w, a, cam);
@@ -736,7 +896,8 @@ hipError_t launch_wavefront(const DevScene& s, const DevCamera& cam, WfState& w,
       hipLaunchKernelGGL(k_wf_all_miss, grid, blk, 0, st, w, a);
     }
     NE_MARK(kStageShade);
-    hipLaunchKernelGGL(k_wf_shade, grid, blk, 0, st, w, a, s);
+    if (gh) hipLaunchKernelGGL(k_wf_shade<true>, grid, blk, 0, st, w, a, s, gh->g);
+    else hipLaunchKernelGGL(k_wf_shade<false>, grid, blk, 0, st, w, a, s, WfGuide{});
     NE_MARK(kStageShadow);
 #define NE_SHADOW(CNT, S_, V_) hipLaunchKernelGGL((k_wf_shadow<CNT, S_, V_>), grid, blk, 0, st, w, a, s)
     if (hs && !hv) {

@@ -56,7 +56,7 @@ class Context:
     def render(self, gscene: "GpuScene", camera, film: "Film", *, spp: int, max_bounces: int, seed: int = 0,
                sample_offset: int = 0, job: int = 0, rect=None, precision: int = NE_FP32, count: bool = False,
                samples_per_thread: int = 0, block_size: int = 0, sync: bool = True, timed: bool = False,
-               stage_timed: bool = False,
+               stage_timed: bool = False, guide=None, train: bool = False,
                occupancy: int = 0, mode: int = 0, pool_size: int = 0):
         p = RenderParams()
         p.width, p.height = film.width, film.height
@@ -69,7 +69,11 @@ class Context:
         p.samples_per_thread, p.block_size, p.occupancy = samples_per_thread, block_size, occupancy
         p.mode, p.pool_size = mode, pool_size
         cam = camera.desc()
-        self.check(self._lib.ne_render(self.h, gscene.h, C.byref(cam), C.byref(p), film.h))
+        if guide is not None:
+            self.check(self._lib.ne_render_guided(self.h, gscene.h, C.byref(cam), C.byref(p), film.h, guide.h,
+                                                  1 if train else 0))
+        else:
+            self.check(self._lib.ne_render(self.h, gscene.h, C.byref(cam), C.byref(p), film.h))
         if sync:
             self.synchronize()
 
@@ -277,9 +281,10 @@ class Guide:
 
     def __init__(self, ctx: Context, log2_table: int = 19, base_res: int = 16, max_res: int = 512,
                  batch: int = 65536, lr_table: float = 1e-2, lr_mlp: float = 1e-3, beta1: float = 0.9,
-                 beta2: float = 0.99, eps: float = 1e-8, seed: int = 0):
+                 beta2: float = 0.99, eps: float = 1e-8, seed: int = 0, guided_vertices: int = 3):
         self.ctx = ctx
-        cfg = GuideConfig(log2_table, base_res, max_res, batch, lr_table, lr_mlp, beta1, beta2, eps, seed)
+        cfg = GuideConfig(log2_table, base_res, max_res, batch, lr_table, lr_mlp, beta1, beta2, eps, guided_vertices,
+                          0, seed)
         h = C.c_void_p()
         ctx.check(ctx._lib.ne_guide_create(ctx.h, C.byref(cfg), C.byref(h)))
         self.h = h

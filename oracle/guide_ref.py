@@ -14,7 +14,7 @@ file to follow; its definition is:
     x_{l+1} = ReLU(h_l) for l < 4; outputs = h_4[:40].  Device arithmetic:
     weights and layer inputs rounded to BF16, FP32 accumulation.
   head: 8 lobes of (logit, mean xyz, log kappa): pi = softmax(logits),
-    mu = mean / |mean|, kappa = exp(min(log kappa, 8)).
+    mu = mean / |mean|, kappa = exp(clamp(log kappa, -10, 8)).
   vMF: p(w) = kappa / (2 pi (1 - exp(-2 kappa))) exp(kappa (mu.w - 1)).
   loss = -sum_i r_i log q(w_i) / sum_i r_i,  q = sum_k pi_k vMF_k.
 """
@@ -110,13 +110,13 @@ def head(y):
     m = y[:, :, 1:4]
     n = np.linalg.norm(m, axis=2, keepdims=True)
     mu = m / np.maximum(n, 1e-12)
-    kap = np.exp(np.minimum(y[:, :, 4], 8.0))
+    kap = np.exp(np.clip(y[:, :, 4], -10.0, 8.0))
     return pi, mu, kap
 
 
 def vmf_pdf(kappa, cosv):
     kappa = np.asarray(kappa, np.float64)
-    return kappa / (2 * np.pi * (1 - np.exp(-2 * kappa))) * np.exp(kappa * (np.asarray(cosv) - 1))
+    return kappa / (2 * np.pi * -np.expm1(-2 * kappa)) * np.exp(kappa * (np.asarray(cosv) - 1))
 
 
 def mixture_pdf(y, w):
@@ -144,9 +144,8 @@ def loss_and_grad(samples, table, mlp, res_list, log2T, quantize=False):
     dy[:, :, 0] = sc * pi * (v - q[:, None])
     pv = sc * pi * v
     dy[:, :, 1:4] = (pv * kap / nm)[:, :, None] * (w[:, None, :] - c[:, :, None] * mu)
-    e2 = np.exp(-2 * kap)
-    dlog = 1 / kap - 2 * e2 / (1 - e2) + (c - 1)
-    dy[:, :, 4] = np.where(yy[:, :, 4] < 8.0, pv * kap * dlog, 0.0)
+    dlog = 1 / kap - 2 / np.expm1(2 * kap) + (c - 1)  # d log vMF / d kappa
+    dy[:, :, 4] = np.where((yy[:, :, 4] < 8.0) & (yy[:, :, 4] > -10.0), pv * kap * dlog, 0.0)
     g = np.zeros((len(s), 48))
     g[:, :40] = dy.reshape(len(s), 40)
     W, _ = unpack(mlp)

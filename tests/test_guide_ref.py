@@ -110,3 +110,14 @@ def test_analytic_gradient_matches_finite_differences():
         fd = (L(tp, mlp) - L(tm, mlp)) / (2 * h)
         assert abs(fd - gt[k]) < 1e-7 + 1e-5 * abs(fd), (k, fd, gt[k])
     assert np.isfinite(loss)
+
+
+def test_vmf_sampler_inverse_cdf_matches_the_density():
+    """cos theta = 1 + log1p((1 - u) expm1(-2 kappa)) / kappa is the inverse CDF
+    of the vMF density in cos theta, from tiny to large kappa."""
+    u = (np.arange(20000) + 0.5) / 20000
+    for kappa in (4.5e-5, 1e-2, 1.0, 30.0, 2980.0):
+        c = 1 + np.log1p((1 - u) * np.expm1(-2 * kappa)) / kappa
+        # CDF of cos theta: 2 pi int_{-1}^{c} p = (exp(kappa (c - 1)) - exp(-2 kappa)) / (1 - exp(-2 kappa))
+        cdf = (np.exp(kappa * (c - 1)) - np.exp(-2 * kappa)) / -np.expm1(-2 * kappa)
+        np.testing.assert_allclose(cdf, u, atol=2e-9 if kappa > 1e-3 else 2e-6)
