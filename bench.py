@@ -199,12 +199,92 @@ def bench_batch(args):
     print(json.dumps(out))
 
 
+MFMA_BF16_PEAK_TFLOPS = 2500.0
+GUIDE_FLOP_PER_QUERY = 2 * (35 * 64 + 3 * 64 * 64 + 64 * 40)  # 34,176
+
+
+def rel_mse(x, ref, eps=1e-2):
+    """mean over pixels and channels of (x - y)^2 / (y^2 + eps), linear radiance"""
+    return float(np.mean((x - ref) ** 2 / (ref ** 2 + eps)))
+
+
+def bench_guided(args):
+    """Learned proposal vs plain HG sampling at equal wall time on the 512^3
+    cloud (the cloud512 scene, camera and bounce limit at 480x270): relMSE of
+    each against a 16384-spp reference.  The guided run trains online during the
+    render; its spp are chosen from a pilot so that its wall time, training
+    included, matches the plain run's."""
+    from narvalengine_amd import Context, Guide, configs
+    W, H = 480, 270
+    wl = configs.cloud512(width=W, height=H)
+    ctx = Context(0)
+    g_s = ctx.upload(wl.scene)
+    kw = dict(max_bounces=wl.max_bounces)
+    ref_spp, plain_spp = 16384, args.plain_spp
+    gkw = dict(guided_vertices=args.guided_vertices, guide_prob=args.guide_prob)
+
+    def timed_render(spp, seed, **extra):
+        f = ctx.film(W, H)
+        ctx.synchronize()
+        t = time.perf_counter()
+        ctx.render(g_s, wl.camera, f, spp=spp, seed=seed, **kw, **extra)
+        dt = time.perf_counter() - t
+        img = f.read() / spp
+        f.close()
+        return img, dt
+
+    timed_render(64, 7)  # warm-up
+    ref, t_ref = timed_render(ref_spp, 1000)
+    log(f"reference: {ref_spp} spp in {t_ref:.2f} s")
+    plain, t_plain = timed_render(plain_spp, 1)
+    log(f"plain: {plain_spp} spp in {t_plain * 1e3:.1f} ms")
+    pilot = Guide(ctx, seed=11, **gkw)
+    timed_render(256, 2, guide=pilot, train=True)  # first guided render allocates the record pool
+    _, t_pilot = timed_render(256, 5, guide=pilot, train=True)
+    pilot.close()
+    guided_spp = max(16, int(256 * t_plain / t_pilot))  # same wall time as the plain run
+    guide = Guide(ctx, seed=12, **gkw)
+    guided, t_guided = timed_render(guided_spp, 3, guide=guide, train=True)
+    st = guide.stats()
+    log(f"guided: {guided_spp} spp in {t_guided * 1e3:.1f} ms, guide {st}")
+    # stage times and proposal queries of a guided frame (trained guide, no training)
+    f = ctx.film(W, H)
+    ctx.render(g_s, wl.camera, f, spp=256, seed=4, guide=guide, stage_timed=True, **kw)
+    stages = ctx.stage_times()
+    ctx.render(g_s, wl.camera, f, spp=256, seed=4, guide=guide, count=True, **kw)
+    cnt = ctx.counters()
+    g_ms, g_n = stages["guide"]
+    flops = cnt["guide_queries"] * GUIDE_FLOP_PER_QUERY
+    achieved = flops / (g_ms * 1e-3) / 1e12 if g_ms > 0 else 0.0
+    roofline = {"bound": "mfma", "achieved": round(achieved, 3), "peak": MFMA_BF16_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / MFMA_BF16_PEAK_TFLOPS, 6), "traffic": None, "kernel": "k_wf_guide",
+                "flop_per_launch": int(flops / max(g_n, 1)), "avg_launch_ms": round(g_ms / max(g_n, 1), 4)}
+    e_plain, e_guided = rel_mse(plain, ref), rel_mse(guided, ref)
+    # per-sample efficiency: relMSE scales as 1 / spp for an unbiased estimator
+    per_sample = (e_guided * guided_spp) / (e_plain * plain_spp)
+    out = {
+        "metric": "relMSE at equal time (guided / plain)", "value": round(e_guided / e_plain, 4), "unit": "ratio",
+        "n_gpus": 1, "higher_is_better": False, "dtype": "f32 (MLP bf16 / fp32 acc)",
+        "data": "synthetic (seeded procedural grid)",
+        "config": {"workload": "cloud512_guided", "width": W, "height": H, "max_bounces": wl.max_bounces,
+                   "reference_spp": ref_spp, "plain_spp": plain_spp, "guided_spp": guided_spp,
+                   "guided_vertices": args.guided_vertices, "guide_prob": args.guide_prob},
+        "relmse_plain": e_plain, "relmse_guided": e_guided, "relmse_ratio_at_equal_spp": round(per_sample, 4),
+        "ms_plain": round(t_plain * 1e3, 2), "ms_guided_incl_training": round(t_guided * 1e3, 2),
+        "guide": st, "guided_stage_ms": {k: round(v[0], 2) for k, v in stages.items() if v[1]},
+        "guide_queries_per_frame": cnt["guide_queries"],
+        "roofline": roofline,
+    }
+    print(json.dumps(out))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="surface1m", choices=["surface1m", "volume128", "cloud512", "batch48"])
+    ap.add_argument("--workload", default="surface1m",
+                    choices=["surface1m", "volume128", "cloud512", "batch48", "guided"])
     ap.add_argument("--mode", default="auto", choices=["auto", "megakernel", "wavefront"])
     ap.add_argument("--pool", type=int, default=0, help="wavefront path slots (0 = library default)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
@@ -212,6 +292,9 @@ def main():
     ap.add_argument("--no-count", action="store_true")
     ap.add_argument("--samples-per-thread", type=int, default=0)
     ap.add_argument("--block", type=int, default=0)
+    ap.add_argument("--plain-spp", type=int, default=1024, help="guided workload: plain run's spp (sets the time)")
+    ap.add_argument("--guided-vertices", type=int, default=1)
+    ap.add_argument("--guide-prob", type=float, default=0.25)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run plain copies of BYTES (counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
@@ -225,6 +308,9 @@ def main():
 
     if args.workload == "batch48":
         bench_batch(args)
+        return
+    if args.workload == "guided":
+        bench_guided(args)
         return
 
     from narvalengine_amd import Context, configs

@@ -126,7 +126,9 @@ enum {
   NE_STAGE_SHADE = 5,      /* wavefront: emission, NEE sample, scattering, compaction */
   NE_STAGE_SHADOW = 6,     /* wavefront: occlusion + ratio tracking */
   NE_STAGE_OTHER = 7,      /* initialisation, film reduction / flush */
-  NE_STAGE_COUNT = 8
+  NE_STAGE_GUIDE = 8,      /* guided wavefront: learned proposal evaluation (MFMA) */
+  NE_STAGE_TRAIN = 9,      /* guided wavefront: online training steps */
+  NE_STAGE_COUNT = 10
 };
 
 /* One render call: samples [sample_offset, sample_offset + spp) of every pixel
@@ -166,6 +168,7 @@ typedef struct ne_counters {
   uint64_t shadow_majorant_cells;
   uint64_t shadow_bvh_nodes;
   uint64_t shadow_prim_tests;
+  uint64_t guide_queries;   /* learned-proposal evaluations (guided rendering; 34,176 MLP FLOP each) */
 } ne_counters;
 
 typedef struct ne_scene_info {
@@ -290,8 +293,10 @@ typedef struct ne_guide_config {
   float lr_table, lr_mlp, beta1, beta2, eps; /* Adam (defaults 1e-2, 1e-3, 0.9, 0.99, 1e-8) */
   int32_t guided_vertices; /* the first this many scattering vertices of a path use the guide
                               (default 3): each mixture-sampled vertex scales the throughput by
-                              phase / mixture pdf <= 2, so the product stays bounded */
-  int32_t reserved;
+                              phase / mixture pdf <= 1 / (1 - guide_prob), so the product stays
+                              bounded */
+  float guide_prob;        /* probability of sampling the guide at a guided vertex (default 0.5,
+                              range [0, 0.9]); the rest samples HG (one-sample MIS) */
   uint64_t seed;      /* parameter initialisation */
 } ne_guide_config;
 typedef struct ne_guide_stats {
@@ -316,8 +321,9 @@ ne_status ne_guide_eval(ne_guide* guide, int32_t n, const float* queries, int32_
  * the weighted loss before the step. */
 ne_status ne_guide_train_step(ne_guide* guide, int32_t n, const float* samples, int32_t update, float* loss);
 /* ne_render with the guide: the first guided_vertices medium scattering
- * vertices of every path sample the one-sample MIS mixture 0.5 HG + 0.5 guide,
- * later ones HG (FP32 wavefront pipeline).  With
+ * vertices of every path sample the one-sample MIS mixture
+ * (1 - guide_prob) HG + guide_prob guide, later ones HG (FP32 wavefront
+ * pipeline).  With
  * train != 0 the first guided vertex of every path is recorded; when the path
  * ends it becomes a training sample (weight Li * phase / pdf), and an Adam step
  * runs whenever half a batch is queued. */

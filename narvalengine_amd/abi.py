@@ -8,7 +8,7 @@ NE_FP32, NE_FP64 = 0, 1
 NE_FLAG_COUNT = 1
 NE_FLAG_TIME = 2
 NE_FLAG_STAGE_TIME = 4
-STAGES = ("megakernel", "plan", "regen", "trace", "medium", "shade", "shadow", "other")
+STAGES = ("megakernel", "plan", "regen", "trace", "medium", "shade", "shadow", "other", "guide", "train")
 NE_MODE_AUTO, NE_MODE_MEGAKERNEL, NE_MODE_WAVEFRONT = 0, 1, 2
 NE_MAT_GLOSSY, NE_MAT_EMITTER = 0, 1
 NE_ENV_NONE, NE_ENV_CONSTANT, NE_ENV_SKY = 0, 1, 2
@@ -59,7 +59,7 @@ class Counters(C.Structure):
     _fields_ = [(n, C.c_uint64) for n in
                 ("paths", "ext_rays", "shadow_rays", "density_lookups", "majorant_cells", "bvh_nodes",
                  "prim_tests", "shadow_density_lookups", "shadow_majorant_cells", "shadow_bvh_nodes",
-                 "shadow_prim_tests")]
+                 "shadow_prim_tests", "guide_queries")]
 
     def as_dict(self) -> dict:
         return {n: int(getattr(self, n)) for n, _ in self._fields_}
@@ -93,7 +93,7 @@ class GuideConfig(C.Structure):
     """ne_guide_config."""
     _fields_ = [("log2_table", C.c_int32), ("base_res", C.c_int32), ("max_res", C.c_int32), ("batch", C.c_int32),
                 ("lr_table", C.c_float), ("lr_mlp", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float),
-                ("eps", C.c_float), ("guided_vertices", C.c_int32), ("reserved", C.c_int32), ("seed", C.c_uint64)]
+                ("eps", C.c_float), ("guided_vertices", C.c_int32), ("guide_prob", C.c_float), ("seed", C.c_uint64)]
 
 
 class GuideStats(C.Structure):

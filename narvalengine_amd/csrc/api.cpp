@@ -798,6 +798,7 @@ ne_status ne_get_counters(ne_ctx* c, ne_counters* out) {
   out->bvh_nodes = h.bvh_nodes + h.sh_nodes; out->prim_tests = h.prim_tests + h.sh_prims;
   out->shadow_density_lookups = h.sh_lookups; out->shadow_majorant_cells = h.sh_cells;
   out->shadow_bvh_nodes = h.sh_nodes; out->shadow_prim_tests = h.sh_prims;
+  out->guide_queries = h.guide_queries;
   return NE_OK;
 }
 
@@ -1233,7 +1234,7 @@ bool guide_alloc(ne_guide* g, T*& p, size_t count) {
 ne_status ne_guide_create(ne_ctx* c, const ne_guide_config* cfg_in, ne_guide** out) {
   if (!c || !out) return fail(c, NE_ERR_INVALID_ARG, "ctx/out is NULL");
   *out = nullptr;
-  ne_guide_config cfg{19, 16, 512, 65536, 1e-2f, 1e-3f, 0.9f, 0.99f, 1e-8f, 3, 0, 0};
+  ne_guide_config cfg{19, 16, 512, 65536, 1e-2f, 1e-3f, 0.9f, 0.99f, 1e-8f, 3, 0.5f, 0};
   if (cfg_in) cfg = *cfg_in;
   if (cfg.log2_table < 10 || cfg.log2_table > 22) return fail(c, NE_ERR_INVALID_ARG, "log2_table must be in [10, 22]");
   if (cfg.base_res < 2 || cfg.max_res < cfg.base_res || cfg.max_res > 4096)
@@ -1241,6 +1242,7 @@ ne_status ne_guide_create(ne_ctx* c, const ne_guide_config* cfg_in, ne_guide** o
   if (cfg.batch < 16 || cfg.batch > (1 << 22)) return fail(c, NE_ERR_INVALID_ARG, "batch must be in [16, 2^22]");
   if (cfg.guided_vertices < 0 || cfg.guided_vertices > kMaxVertices)
     return fail(c, NE_ERR_INVALID_ARG, "guided_vertices out of range");
+  if (!(cfg.guide_prob >= 0.0f && cfg.guide_prob <= 0.9f)) return fail(c, NE_ERR_INVALID_ARG, "guide_prob must be in [0, 0.9]");
   if (!(cfg.lr_table >= 0) || !(cfg.lr_mlp >= 0) || !(cfg.beta1 >= 0 && cfg.beta1 < 1) ||
       !(cfg.beta2 >= 0 && cfg.beta2 < 1) || !(cfg.eps > 0))
     return fail(c, NE_ERR_INVALID_ARG, "Adam parameters out of range");
@@ -1439,6 +1441,7 @@ ne_status ne_render_guided(ne_ctx* c, const ne_scene* s, const ne_camera_desc* c
   hook.g.batch = g->batch;
   hook.g.train = train ? 1 : 0;
   hook.g.max_depth = g->cfg.guided_vertices;
+  hook.g.prob = g->cfg.guide_prob;
   hook.train = g->train;
   hook.lr_table = g->cfg.lr_table; hook.lr_mlp = g->cfg.lr_mlp;
   hook.beta1 = g->cfg.beta1; hook.beta2 = g->cfg.beta2; hook.eps = g->cfg.eps;
@@ -1447,9 +1450,16 @@ ne_status ne_render_guided(ne_ctx* c, const ne_scene* s, const ne_camera_desc* c
   hook.samples = 0;
   if (train) NE_HIP(c, hipMemsetAsync(g->batch.count, 0, sizeof(uint32_t), c->stream));
   const DevCamera dc = make_camera(cam);
+  const uint64_t steps0 = g->steps;
   st = render_wavefront(c, s, dc, p, f, count, timed, tm, &hook);
   c->stage_timed = st == NE_OK && tm;
   g->steps = hook.steps;
   g->samples += hook.samples;
+  if (st == NE_OK && g->steps > steps0) {  // loss of the last step taken during the render
+    float lw[2] = {0, 0};
+    NE_HIP(c, hipMemcpyAsync(lw, g->train.loss, sizeof(lw), hipMemcpyDeviceToHost, c->stream));
+    NE_HIP(c, hipStreamSynchronize(c->stream));
+    g->last_loss = lw[1] > 0 ? lw[0] / lw[1] : 0.0f;
+  }
   return st;
 }

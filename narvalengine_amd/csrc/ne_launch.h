@@ -52,7 +52,7 @@ constexpr uint32_t kChunk = 256;
 constexpr uint32_t kMaxRefill = 64;  // chunks per block in the lane-refill kernels
 // stage ids (the public NE_STAGE_* values)
 enum { kStageMegakernel = 0, kStagePlan, kStageRegen, kStageTrace, kStageMedium, kStageShade, kStageShadow,
-       kStageOther, kStageCount, kStageEnd = -1 };
+       kStageOther, kStageGuide, kStageTrain, kStageCount, kStageEnd = -1 };
 enum { WF_KIND_MEDIUM = 0, WF_KIND_SURFACE = 1, WF_KIND_MISS = 2 };
 constexpr uint32_t kNoHit = 0xFFFFFFFFu;
 constexpr uint32_t kPrevDelta = 1u << 16;  // depth word: set while the last vertex was the camera
@@ -210,6 +210,8 @@ struct WfGuide {
   WfGuideRec rec[2];
   int32_t train;      // collect training samples
   int32_t max_depth;  // scattering vertices with depth < max_depth use the guide
+  float prob;         // probability of sampling the proposal at a guided vertex
+  float* mix;         // 40 mixture floats per path slot (this iteration's guided vertices)
 };
 
 struct WfGuideHook {

@@ -76,6 +76,7 @@ struct DevCamera {
 struct DevCounters {
   unsigned long long paths, ext_rays, shadow_rays, density_lookups, majorant_cells, bvh_nodes, prim_tests;
   unsigned long long sh_lookups, sh_cells, sh_nodes, sh_prims;  // shadow-ray share (wavefront)
+  unsigned long long guide_queries;                             // learned-proposal evaluations
 };
 
 }  // namespace ne

@@ -400,20 +400,63 @@ __device__ __forceinline__ bool nee_sample(const DevScene& s, Vec3<float> x, Vec
   return true;
 }
 
-// Shading.  GUIDED: medium vertices sample a one-sample MIS mixture of HG and
-// the learned vMF proposal (0.5 / 0.5); the block evaluates the proposal for
-// its medium vertices in MFMA tiles (16 queries per wave) into LDS.  With
+// Guided mode: the learned proposal of every medium vertex that will be
+// guided (depth < max_depth, below the bounce limit), 16 queries per MFMA tile,
+// into gd.mix (40 floats per path slot).
+__global__ __launch_bounds__(kChunk) void k_wf_guide(WfState w, WfArgs a, DevScene s, WfGuide gd) {
+  __shared__ uint32_t wsum[kChunk / 64];
+  __shared__ float gq[kChunk * 6];
+  __shared__ uint32_t gslot[kChunk];
+  __shared__ uint16_t gxa[4 * 16 * kLdA];
+  __shared__ uint16_t gxb[4 * 16 * kLdA];
+  __shared__ float gout[4 * 16 * kLdO];
+  const WfPaths A = w.paths[a.cur];
+  const uint32_t c = blockIdx.x, t = threadIdx.x, i = c * kChunk + t;
+  bool q = false;
+  if (t < w.cnt_paths[a.cur][c] && A.kind[i] == WF_KIND_MEDIUM) {
+    const int depth = int(A.depth[i] & 0xFFFFu);
+    q = depth < gd.max_depth && depth < a.max_bounces;
+  }
+  uint32_t nq;
+  const uint32_t qi = block_compact(q, wsum, nq);
+  if (a.counters && t == 0 && nq) atomicAdd(&a.counters->guide_queries, (unsigned long long)nq);
+  if (q) {
+    const Vec3<float> o = v3(A.ox[i], A.oy[i], A.oz[i]), d = v3(A.dx[i], A.dy[i], A.dz[i]);
+    const Vec3<float> p = xf_point<float>(s.inv, o + A.hit_t[i] * d);
+    float* e = gq + 6 * qi;
+    e[0] = p.x + 0.5f; e[1] = p.y + 0.5f; e[2] = p.z + 0.5f;
+    e[3] = d.x; e[4] = d.y; e[5] = d.z;
+    gslot[qi] = i;
+  }
+  __syncthreads();
+  const int wv = int(t >> 6), lane = __lane_id();
+  for (uint32_t tile = uint32_t(wv); tile * 16 < nq; tile += 4) {
+    uint16_t* xa = gxa + wv * 16 * kLdA;
+    uint16_t* xb = gxb + wv * 16 * kLdA;
+    float* out = gout + wv * 16 * kLdO;
+    wave_sync();
+    guide_encode_tile(gd.net, xa, [&](int r, Vec3<float>& p, Vec3<float>& dd, bool& valid) {
+      const uint32_t k = tile * 16 + uint32_t(r);
+      valid = k < nq;
+      const float* e = gq + 6 * (valid ? k : 0u);
+      p = v3(e[0], e[1], e[2]);
+      dd = v3(e[3], e[4], e[5]);
+    });
+    guide_mlp_tile(gd.net, xa, xb, out);
+    if (lane < 16 && tile * 16 + uint32_t(lane) < nq)
+      guide_head(out + lane * kLdO, gd.mix + 40 * size_t(gslot[tile * 16 + lane]));
+  }
+}
+
+// Shading.  GUIDED: medium vertices at depth < max_depth sample a one-sample
+// MIS mixture of HG and the learned vMF proposal (probabilities 1 - prob,
+// prob) that k_wf_guide evaluated for them.  With
 // training on, the first guided vertex of a path is recorded and becomes a
 // training sample when the path ends: weight r = Li * phase / pdf, Li the
 // radiance gathered after the vertex divided by the throughput after it.
 template <bool GUIDED>
 __global__ __launch_bounds__(kChunk) void k_wf_shade(WfState w, WfArgs a, DevScene s, WfGuide gd) {
   __shared__ uint32_t wsum[kChunk / 64];
-  __shared__ float gq[GUIDED ? kChunk * 6 : 1];     // queries: unit position, direction
-  __shared__ float gmix[GUIDED ? kChunk * 40 : 1];  // mixture of each query
-  __shared__ uint16_t gxa[GUIDED ? 4 * 16 * kLdA : 1];
-  __shared__ uint16_t gxb[GUIDED ? 4 * 16 * kLdA : 1];
-  __shared__ float gout[GUIDED ? 4 * 16 * kLdO : 1];
   const WfPaths A = w.paths[a.cur], B = w.paths[a.cur ^ 1];
   const WfShadow S = w.shadow;
   const uint32_t c = blockIdx.x, t = threadIdx.x, i = c * kChunk + t;
@@ -507,36 +550,6 @@ __global__ __launch_bounds__(kChunk) void k_wf_shade(WfState w, WfArgs a, DevSce
     }
     if (vertex) x = o + th * d;
   }
-  // guided: proposal of every medium vertex of the block (all threads take part)
-  uint32_t qi = 0;
-  if constexpr (GUIDED) {
-    uint32_t nq;
-    qi = block_compact(medium && int(depth) < gd.max_depth, wsum, nq);
-    if (medium && int(depth) < gd.max_depth) {
-      const Vec3<float> p = xf_point<float>(s.inv, x);
-      float* q = gq + 6 * qi;
-      q[0] = p.x + 0.5f; q[1] = p.y + 0.5f; q[2] = p.z + 0.5f;
-      q[3] = d.x; q[4] = d.y; q[5] = d.z;
-    }
-    __syncthreads();
-    const int wv = int(t >> 6), lane = __lane_id();
-    for (uint32_t tile = uint32_t(wv); tile * 16 < nq; tile += 4) {
-      uint16_t* xa = gxa + wv * 16 * kLdA;
-      uint16_t* xb = gxb + wv * 16 * kLdA;
-      float* out = gout + wv * 16 * kLdO;
-      wave_sync();
-      guide_encode_tile(gd.net, xa, [&](int r, Vec3<float>& p, Vec3<float>& dd, bool& valid) {
-        const uint32_t k = tile * 16 + uint32_t(r);
-        valid = k < nq;
-        const float* q = gq + 6 * (valid ? k : 0u);
-        p = v3(q[0], q[1], q[2]);
-        dd = v3(q[3], q[4], q[5]);
-      });
-      guide_mlp_tile(gd.net, xa, xb, out);
-      if (lane < 16 && tile * 16 + uint32_t(lane) < nq) guide_head(out + lane * kLdO, gmix + 40 * (tile * 16 + lane));
-    }
-    __syncthreads();
-  }
   // phase 2: next-event estimation and the continuation
   if (active && vertex) {
     const Vec3<float> wo = -d;
@@ -548,9 +561,9 @@ __global__ __launch_bounds__(kChunk) void k_wf_shade(WfState w, WfArgs a, DevSce
     shadow = medium ? nee_sample<true>(s, x, d, gm, wo, ns, ss, ts, ul, ua, ub, wi, tmax, f, Le, wt, pl, delta)
                     : nee_sample<false>(s, x, d, gm, wo, ns, ss, ts, ul, ua, ub, wi, tmax, f, Le, wt, pl, delta);
     const bool guided = GUIDED && medium && int(depth) < gd.max_depth;
-    const float* mix = guided ? gmix + 40 * qi : nullptr;
+    const float* mix = guided ? gd.mix + 40 * size_t(i) : nullptr;  // written by k_wf_guide
     if (guided && shadow && !delta)  // the phase-sampling strategy is the mixture
-      wt = power_heuristic(pl, 0.5f * hg_p(s.g, dot(d, wi)) + 0.5f * vmf_mixture_pdf(mix, wi));
+      wt = power_heuristic(pl, (1.0f - gd.prob) * hg_p(s.g, dot(d, wi)) + gd.prob * vmf_mixture_pdf(mix, wi));
     if (shadow) f = beta * f * Le;  // NEE weight with the throughput before this vertex's BSDF weight
     if (medium && !guided) {
       const float u0 = rng.next(), u1 = rng.next();
@@ -559,9 +572,9 @@ __global__ __launch_bounds__(kChunk) void k_wf_shade(WfState w, WfArgs a, DevSce
       cont = true;
     } else if (medium) {
       const float us = rng.next(), u0 = rng.next(), u1 = rng.next(), u2 = rng.next();
-      nd = us < 0.5f ? vmf_mixture_sample(mix, u2, u0, u1) : hg_sample(s.g, d, u0, u1);
+      nd = us < gd.prob ? vmf_mixture_sample(mix, u2, u0, u1) : hg_sample(s.g, d, u0, u1);
       const float ph = hg_p(s.g, dot(d, nd));
-      const float pm = 0.5f * ph + 0.5f * vmf_mixture_pdf(mix, nd);
+      const float pm = (1.0f - gd.prob) * ph + gd.prob * vmf_mixture_pdf(mix, nd);
       if (pm > 0.0f) {
         ppdf = pm;
         beta = beta * (ph / pm);
@@ -811,8 +824,11 @@ void wf_carve(void* base, uint32_t P, WfState& w) {
   w.next_path = reinterpret_cast<unsigned long long*>(p);
 }
 
-// guided-mode record arrays: 2 buffers x (kGuideRecF floats + 1 state word) per slot
-size_t wf_guide_bytes(uint32_t P) { return size_t(2 * (kGuideRecF + kGuideRecU)) * align256(size_t(P) * 4); }
+// guided-mode arrays: 2 buffers x (kGuideRecF floats + 1 state word) per slot,
+// then 40 mixture floats per slot
+size_t wf_guide_bytes(uint32_t P) {
+  return size_t(2 * (kGuideRecF + kGuideRecU)) * align256(size_t(P) * 4) + align256(size_t(P) * 40 * 4);
+}
 
 void wf_guide_carve(void* base, uint32_t P, WfGuide& g) {
   char* p = static_cast<char*>(base);
@@ -825,6 +841,7 @@ void wf_guide_carve(void* base, uint32_t P, WfGuide& g) {
     R.state = reinterpret_cast<uint32_t*>(p);
     p += stride;
   }
+  g.mix = reinterpret_cast<float*>(p);
 }
 
 __global__ void k_guide_reset_count(uint32_t* count) { *count = 0u; }
@@ -864,7 +881,7 @@ hipError_t launch_wavefront(const DevScene& s, const DevCamera& cam, WfState& w,
       if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
       if (h_status[0] == 0) break;
       if (gh && gh->g.train && h_status[1] >= gh->min_batch) {
-        NE_MARK(kStageOther);
+        NE_MARK(kStageTrain);
         if ((e = launch_guide_grad(gh->g.net, gh->g.batch, gh->train, st)) != hipSuccess) return e;
         gh->steps += 1;
         gh->samples += std::min(h_status[1], gh->g.batch.cap);
@@ -894,6 +911,10 @@ hipError_t launch_wavefront(const DevScene& s, const DevCamera& cam, WfState& w,
     if (!hs && !hv) {
       NE_MARK(kStageOther);
       hipLaunchKernelGGL(k_wf_all_miss, grid, blk, 0, st, w, a);
+    }
+    if (gh) {
+      NE_MARK(kStageGuide);
+      hipLaunchKernelGGL(k_wf_guide, grid, blk, 0, st, w, a, s, gh->g);
     }
     NE_MARK(kStageShade);
     if (gh) hipLaunchKernelGGL(k_wf_shade<true>, grid, blk, 0, st, w, a, s, gh->g);
