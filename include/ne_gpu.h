@@ -292,10 +292,10 @@ typedef struct ne_guide_config {
   int32_t batch;      /* training samples per step (default 65536) */
   float lr_table, lr_mlp, beta1, beta2, eps; /* Adam (defaults 1e-2, 1e-3, 0.9, 0.99, 1e-8) */
   int32_t guided_vertices; /* the first this many scattering vertices of a path use the guide
-                              (default 3): each mixture-sampled vertex scales the throughput by
+                              (default 1): each mixture-sampled vertex scales the throughput by
                               phase / mixture pdf <= 1 / (1 - guide_prob), so the product stays
                               bounded */
-  float guide_prob;        /* probability of sampling the guide at a guided vertex (default 0.5,
+  float guide_prob;        /* probability of sampling the guide at a guided vertex (default 0.25,
                               range [0, 0.9]); the rest samples HG (one-sample MIS) */
   uint64_t seed;      /* parameter initialisation */
 } ne_guide_config;

@@ -1234,7 +1234,7 @@ bool guide_alloc(ne_guide* g, T*& p, size_t count) {
 ne_status ne_guide_create(ne_ctx* c, const ne_guide_config* cfg_in, ne_guide** out) {
   if (!c || !out) return fail(c, NE_ERR_INVALID_ARG, "ctx/out is NULL");
   *out = nullptr;
-  ne_guide_config cfg{19, 16, 512, 65536, 1e-2f, 1e-3f, 0.9f, 0.99f, 1e-8f, 3, 0.5f, 0};
+  ne_guide_config cfg{19, 16, 512, 65536, 1e-2f, 1e-3f, 0.9f, 0.99f, 1e-8f, 1, 0.25f, 0};
   if (cfg_in) cfg = *cfg_in;
   if (cfg.log2_table < 10 || cfg.log2_table > 22) return fail(c, NE_ERR_INVALID_ARG, "log2_table must be in [10, 22]");
   if (cfg.base_res < 2 || cfg.max_res < cfg.base_res || cfg.max_res > 4096)

@@ -281,8 +281,8 @@ class Guide:
 
     def __init__(self, ctx: Context, log2_table: int = 19, base_res: int = 16, max_res: int = 512,
                  batch: int = 65536, lr_table: float = 1e-2, lr_mlp: float = 1e-3, beta1: float = 0.9,
-                 beta2: float = 0.99, eps: float = 1e-8, seed: int = 0, guided_vertices: int = 3,
-                 guide_prob: float = 0.5):
+                 beta2: float = 0.99, eps: float = 1e-8, seed: int = 0, guided_vertices: int = 1,
+                 guide_prob: float = 0.25):
         self.ctx = ctx
         cfg = GuideConfig(log2_table, base_res, max_res, batch, lr_table, lr_mlp, beta1, beta2, eps, guided_vertices,
                           guide_prob, seed)

@@ -166,7 +166,7 @@ def test_guided_render_is_unbiased(ctx, trained):
     print_case(f"guided render unbiased (trained={trained})")
     s, cam = _cloud()
     gs = ctx.upload(s)
-    g = Guide(ctx, log2_table=16, batch=16384, seed=4)
+    g = Guide(ctx, log2_table=16, batch=16384, seed=4, guided_vertices=3, guide_prob=0.5)
     W = H = 32
     if trained:
         f = ctx.film(W, H)
