@@ -62,7 +62,10 @@ enum { NE_ENV_NONE = 0, NE_ENV_CONSTANT = 1, NE_ENV_SKY = 2 };
  *                 (0,0,-1) under its instance transform (SceneReader.cpp:407-419)
  *   volume         one GridMedia (src/materials/GridMedia.cpp) on the unit
  *                 cube [-.5,.5]^3 under vol_xf (row-major 4x4, OCS->WCS);
- *                 grid is x-fastest R32F (to1D, src/utils/Math.h:104-106)
+ *                 grid is x-fastest R32F (to1D, src/utils/Math.h:104-106);
+ *                 with vol_homogeneous set, a HomogeneousMedia
+ *                 (src/materials/HomogeneousMedia.cpp:5-51) of density
+ *                 vol_mult on the same cube instead (grid and vol_res unused)
  *   sun            DirectionalLight (src/lights/DirectionalLight.cpp), sun_dir is
  *                 the direction light travels, sun_E its radiance scale
  * The same layout is accepted by the test oracle, so one host-side
@@ -96,6 +99,7 @@ typedef struct ne_scene_desc {
   float env_c0[3];
   float env_c1[3];
   float env_c2[3];
+  int32_t vol_homogeneous; /* 1: constant density vol_mult inside the cube; analytic transmittance */
 } ne_scene_desc;
 
 /* Pinhole camera: Camera(lookFrom, lookAt, up, vfov, aspect, ...)
@@ -397,6 +401,57 @@ ne_status ne_gen_cumulus_grid(int32_t res, uint64_t seed, float* out);
  * n_vertices*3 floats, tris n_triangles*3 indices; pass NULL to query sizes. */
 ne_status ne_gen_icosphere(int32_t subdiv, uint64_t seed, float amp, float* verts, uint32_t* tris,
                            uint32_t* n_vertices, uint32_t* n_triangles);
+
+/* ---- scene files -----------------------------------------------------------
+ * SceneReader::loadScene (src/io/SceneReader.cpp:10-54) for JSON scene files
+ * (version 0.0.1a): materials (microfacet, emitter, directionalLight,
+ * infiniteAreaLight, volume), primitives (rectangle, sphere, obj, point,
+ * volume) with their instance transforms, camera and renderer settings, read
+ * into the flattened ne_scene_desc, an ne_camera_desc and ne_scene_settings.
+ * Host-only: no context or device is needed.  Errors: NE_ERR_IO for files that
+ * cannot be read, NE_ERR_PARSE for malformed JSON, missing members and types
+ * the reference rejects with LOG(FATAL), NE_ERR_UNSUPPORTED for valid input
+ * this backend cannot represent (image textures, .vdb grids, glTF models,
+ * point lights, a second volume); the message is in ne_last_error(NULL). */
+typedef struct ne_scene_file ne_scene_file;
+/* SceneSettings (src/core/Settings.h:9-14) plus the renderer mode. */
+typedef struct ne_scene_settings {
+  int32_t width, height; /* renderer.resolution */
+  int32_t spp;           /* renderer.spp */
+  int32_t bounces;       /* renderer.bounces */
+  int32_t hdr;           /* renderer.HDR */
+  int32_t realtime;      /* renderer.mode == "realtime" */
+} ne_scene_settings;
+enum {
+  /* Substitute what the backend cannot represent instead of failing, and list
+   * each substitution in ne_scene_file_warnings: image albedo -> grey 0.5,
+   * OBJ without materialName -> glossy grey 0.5 roughness 0.65, non-colliding
+   * emitter sphere -> colliding, constant normal maps ignored. */
+  NE_LOAD_LENIENT = 1
+};
+typedef struct ne_load_options {
+  int32_t flags;             /* NE_LOAD_* */
+  int32_t maj_res;           /* majorant cells per axis; 0: 16, capped at the largest grid dimension */
+  const char* resources_dir; /* base of relative asset paths (the reference's RESOURCES_DIR);
+                                NULL: the scene file's directory (the working directory for parse) */
+} ne_load_options;
+/* Reads and flattens the scene file at `path` (opt may be NULL). */
+ne_status ne_scene_file_read(const char* path, const ne_load_options* opt, ne_scene_file** out);
+/* Same from JSON text in memory. */
+ne_status ne_scene_file_parse(const char* json, size_t length, const ne_load_options* opt, ne_scene_file** out);
+/* The flattened scene, the main camera (up fixed to (0,1,0) and aspect
+ * width/height, as SceneReader.cpp:612) and the settings.  Pointers in *desc
+ * stay valid until the file is released.  Any output may be NULL. */
+ne_status ne_scene_file_get(const ne_scene_file* file, ne_scene_desc* desc, ne_camera_desc* camera,
+                            ne_scene_settings* settings);
+/* Substitutions made under NE_LOAD_LENIENT, one per line ("" if none). */
+const char* ne_scene_file_warnings(const ne_scene_file* file);
+void ne_scene_file_release(ne_scene_file* file);
+/* Text grid reader, ResourceManager::loadVolasTexture
+ * (src/core/ResourceManager.cpp:222-286): line 1 "W H D", line 2 ignored,
+ * then W*H*D whitespace-separated floats, x fastest.  grid == NULL queries the
+ * resolution only; otherwise capacity (floats) must hold W*H*D. */
+ne_status ne_vol_read(const char* path, int32_t res[3], float* grid, uint64_t capacity);
 
 #ifdef __cplusplus
 }

@@ -35,6 +35,7 @@ class SceneDesc(C.Structure):
         ("maj_res", C.c_int32),
         ("has_sun", C.c_int32), ("sun_dir", C.c_float * 3), ("sun_E", C.c_float * 3),
         ("env_type", C.c_int32), ("env_c0", C.c_float * 3), ("env_c1", C.c_float * 3), ("env_c2", C.c_float * 3),
+        ("vol_homogeneous", C.c_int32),
     ]
 
 
@@ -100,6 +101,17 @@ class GuideStats(C.Structure):
     _fields_ = [("steps", C.c_uint64), ("samples", C.c_uint64), ("last_loss", C.c_float), ("pad", C.c_float)]
 
 
+class SceneSettings(C.Structure):
+    _fields_ = [("width", C.c_int32), ("height", C.c_int32), ("spp", C.c_int32), ("bounces", C.c_int32),
+                ("hdr", C.c_int32), ("realtime", C.c_int32)]
+
+
+class LoadOptions(C.Structure):
+    _fields_ = [("flags", C.c_int32), ("maj_res", C.c_int32), ("resources_dir", C.c_char_p)]
+
+
+NE_LOAD_LENIENT = 1
+
 SIGNATURES = [
     ("ne_ctx_create", C.c_int32, [C.c_int32, C.POINTER(C.c_void_p)]),
     ("ne_ctx_destroy", None, [C.c_void_p]),
@@ -150,6 +162,13 @@ SIGNATURES = [
     ("ne_guide_get_params", C.c_int32, [C.c_void_p, _F, _F]),
     ("ne_guide_set_params", C.c_int32, [C.c_void_p, _F, _F]),
     ("ne_guide_get_grads", C.c_int32, [C.c_void_p, _F, _F]),
+    ("ne_scene_file_read", C.c_int32, [C.c_char_p, C.POINTER(LoadOptions), C.POINTER(C.c_void_p)]),
+    ("ne_scene_file_parse", C.c_int32, [C.c_char_p, C.c_size_t, C.POINTER(LoadOptions), C.POINTER(C.c_void_p)]),
+    ("ne_scene_file_get", C.c_int32, [C.c_void_p, C.POINTER(SceneDesc), C.POINTER(CameraDesc),
+                                      C.POINTER(SceneSettings)]),
+    ("ne_scene_file_warnings", C.c_char_p, [C.c_void_p]),
+    ("ne_scene_file_release", None, [C.c_void_p]),
+    ("ne_vol_read", C.c_int32, [C.c_char_p, C.POINTER(C.c_int32), _F, C.c_uint64]),
 ]
 
 _LIB = None

@@ -151,10 +151,12 @@ ne_status validate_desc(ne_ctx* c, const ne_scene_desc* d) {
       return fail(c, NE_ERR_INVALID_ARG, "unknown material type");
   }
   if (d->has_volume) {
-    if (!d->grid) return fail(c, NE_ERR_INVALID_ARG, "volume grid is NULL");
-    for (int a = 0; a < 3; ++a)
+    const bool homog = d->vol_homogeneous != 0;
+    if (!homog && !d->grid) return fail(c, NE_ERR_INVALID_ARG, "volume grid is NULL");
+    for (int a = 0; a < 3 && !homog; ++a)
       if (d->vol_res[a] <= 0 || d->vol_res[a] > 4096) return fail(c, NE_ERR_INVALID_ARG, "volume resolution out of range");
-    if (d->maj_res < 1 || d->maj_res > 256) return fail(c, NE_ERR_INVALID_ARG, "majorant resolution out of range");
+    if (!homog && (d->maj_res < 1 || d->maj_res > 256))
+      return fail(c, NE_ERR_INVALID_ARG, "majorant resolution out of range");
     float st = 0;
     for (int a = 0; a < 3; ++a) st += d->vol_sigma_s[a] + d->vol_sigma_a[a];
     if (!(st > 0) || !(d->vol_mult > 0)) return fail(c, NE_ERR_INVALID_ARG, "volume extinction must be > 0");
@@ -164,7 +166,7 @@ ne_status validate_desc(ne_ctx* c, const ne_scene_desc* d) {
     if (!std::isfinite(d->vol_mult)) return fail(c, NE_ERR_INVALID_ARG, "density multiplier must be finite");
     // Tracking loops step by -log(1 - xi) / majorant: a NaN or infinite voxel
     // would stall them, and negative densities break the majorant bound.
-    const size_t nvox = size_t(d->vol_res[0]) * d->vol_res[1] * d->vol_res[2];
+    const size_t nvox = homog ? 0 : size_t(d->vol_res[0]) * d->vol_res[1] * d->vol_res[2];
     for (size_t i = 0; i < nvox; ++i)
       if (!(d->grid[i] >= 0.0f) || !std::isfinite(d->grid[i]))
         return fail(c, NE_ERR_INVALID_ARG, "grid densities must be finite and >= 0");
@@ -260,6 +262,11 @@ DevCamera make_camera(const ne_camera_desc* cd) {
 }
 
 }  // namespace
+
+namespace ne {
+// Error of a context-free entry point (scene files), read by ne_last_error(NULL).
+ne_status host_fail(ne_status st, const std::string& msg) { return fail(nullptr, st, msg); }
+}
 
 extern "C" {
 
@@ -367,7 +374,7 @@ ne_status ne_scene_upload(ne_ctx* c, const ne_scene_desc* d, ne_scene** out) {
   std::vector<float> tris(size_t(nt) * 12), sph(size_t(ns) * 4), rects(size_t(nr) * 16);
   std::vector<int32_t> sph_mat(ns), sph_orig(ns);
   std::vector<uint32_t> storage(bvh.refs.size());  // refs position -> slot within its type's array
-  std::vector<int32_t> rect_slot(nr, -1);
+  std::vector<int32_t> rect_slot(nr, -1), sph_slot(ns, -1), tri_slot(nt, -1);
   uint32_t tslot = 0, sslot = 0, rslot = 0;
   for (size_t i = 0; i < bvh.refs.size(); ++i) {
     const uint32_t p = bvh.refs[i];
@@ -378,12 +385,14 @@ ne_status ne_scene_upload(ne_ctx* c, const ne_scene_desc* d, ne_scene** out) {
       o[0] = v[0]; o[1] = v[1]; o[2] = v[2]; std::memcpy(&o[3], &m, 4);
       o[4] = v[3] - v[0]; o[5] = v[4] - v[1]; o[6] = v[5] - v[2]; std::memcpy(&o[7], &p, 4);
       o[8] = v[6] - v[0]; o[9] = v[7] - v[1]; o[10] = v[8] - v[2]; o[11] = 0.0f;
+      tri_slot[p] = int32_t(tslot);
       storage[i] = tslot++;
     } else if (p < nt + ns) {
       const uint32_t q = p - nt;
       std::memcpy(&sph[4 * size_t(sslot)], d->sph + 4 * size_t(q), 16);
       sph_mat[sslot] = d->sph_mat[q];
       sph_orig[sslot] = int32_t(q);
+      sph_slot[q] = int32_t(sslot);
       storage[i] = sslot++;
     } else {
       const uint32_t q = p - nt - ns;
@@ -410,10 +419,18 @@ ne_status ne_scene_upload(ne_ctx* c, const ne_scene_desc* d, ne_scene** out) {
       std::memcpy(&f[12 + side], &slot, 4);
     }
   }
-  // emitter rectangles in input order (the light-selection order), as storage slots
-  std::vector<int32_t> light_rects;
+  // emissive primitives in the light-selection order: rectangles, spheres,
+  // triangles, each in input order; codes are type << 30 | storage slot
+  std::vector<uint32_t> emitters;
+  auto emits = [&](int32_t m) { return d->mat[12 * size_t(m)] == float(NE_MAT_EMITTER); };
   for (uint32_t q = 0; q < nr; ++q)
-    if (d->mat[12 * size_t(d->rect_mat[q])] == float(NE_MAT_EMITTER)) light_rects.push_back(rect_slot[q]);
+    if (emits(d->rect_mat[q])) emitters.push_back(uint32_t(PRIM_RECT) << 30 | uint32_t(rect_slot[q]));
+  for (uint32_t q = 0; q < ns; ++q)
+    if (emits(d->sph_mat[q])) emitters.push_back(uint32_t(PRIM_SPHERE) << 30 | uint32_t(sph_slot[q]));
+  for (uint32_t q = 0; q < nt; ++q)
+    if (emits(d->tri_mat ? d->tri_mat[q] : 0)) emitters.push_back(uint32_t(PRIM_TRI) << 30 | uint32_t(tri_slot[q]));
+  for (uint32_t e : emitters)
+    if ((e & 0x3FFFFFFFu) == 0x3FFFFFFFu) return bail(fail(c, NE_ERR_INVALID_ARG, "emitter missing from the BVH"));
   const float4* f4 = nullptr;
   if ((st = dev_upload(c, s, reinterpret_cast<const float4*>(bvh.nodes.data()), bvh.nodes.size() / 4, &f4)) != NE_OK)
     return bail(st);
@@ -428,13 +445,18 @@ ne_status ne_scene_upload(ne_ctx* c, const ne_scene_desc* d, ne_scene** out) {
   if ((st = dev_upload(c, s, reinterpret_cast<const float4*>(rects.data()), rects.size() / 4, &ds.rects)) != NE_OK)
     return bail(st);
   if ((st = dev_upload(c, s, d->mat, size_t(d->n_mat) * 12, &ds.mats)) != NE_OK) return bail(st);
-  if ((st = dev_upload(c, s, light_rects.data(), light_rects.size(), &ds.light_rects)) != NE_OK) return bail(st);
-  ds.n_light_rects = int32_t(light_rects.size());
+  if ((st = dev_upload(c, s, emitters.data(), emitters.size(), &ds.emitters)) != NE_OK) return bail(st);
+  ds.n_emitters = int32_t(emitters.size());
   // ---- volume
   ds.has_volume = d->has_volume ? 1 : 0;
   float gmax = 0.0f;
   if (d->has_volume) {
-    for (int a = 0; a < 3; ++a) ds.res[a] = d->vol_res[a];
+    // A homogeneous medium is a 1^3 grid of density 1 read as constant, with
+    // one majorant cell: tracking samples the exponential free flight exactly.
+    static const float kOne = 1.0f;
+    const bool homog = d->vol_homogeneous != 0;
+    ds.homog = homog ? 1 : 0;
+    for (int a = 0; a < 3; ++a) ds.res[a] = homog ? 1 : d->vol_res[a];
     for (int i = 0; i < 12; ++i) ds.inv[i] = d->vol_inv[i];
     float st3[3];
     for (int a = 0; a < 3; ++a) {
@@ -445,9 +467,9 @@ ne_status ne_scene_upload(ne_ctx* c, const ne_scene_desc* d, ne_scene** out) {
     // avg(extinction * densityMultiplier), src/materials/GridMedia.cpp:82
     ds.sigma_bar = (st3[0] * d->vol_mult + st3[1] * d->vol_mult + st3[2] * d->vol_mult) / 3.0f;
     ds.g = d->vol_g;
-    ds.mres = d->maj_res;
+    ds.mres = homog ? 1 : d->maj_res;
     size_t nvox = size_t(ds.res[0]) * ds.res[1] * ds.res[2];
-    if ((st = dev_upload(c, s, d->grid, nvox, &ds.grid)) != NE_OK) return bail(st);
+    if ((st = dev_upload(c, s, homog ? &kOne : d->grid, nvox, &ds.grid)) != NE_OK) return bail(st);
     size_t ncell = size_t(ds.mres) * ds.mres * ds.mres;
     void* pm = nullptr;
     NE_HIP(c, hipMalloc(&pm, ncell * sizeof(float)));
@@ -466,12 +488,12 @@ ne_status ne_scene_upload(ne_ctx* c, const ne_scene_desc* d, ne_scene** out) {
   for (int a = 0; a < 3; ++a) { ds.sun_dir[a] = d->sun_dir[a]; ds.sun_E[a] = d->sun_E[a]; }
   ds.env_type = d->env_type;
   for (int a = 0; a < 3; ++a) { ds.env[a] = d->env_c0[a]; ds.env[3 + a] = d->env_c1[a]; ds.env[6 + a] = d->env_c2[a]; }
-  ds.n_lights = ds.has_sun + (ds.env_type != NE_ENV_NONE ? 1 : 0) + ds.n_light_rects;
+  ds.n_lights = ds.has_sun + (ds.env_type != NE_ENV_NONE ? 1 : 0) + ds.n_emitters;
   NE_HIP(c, hipStreamSynchronize(c->stream));
   s->info.bvh_nodes = ds.n_nodes;
   s->info.bvh_depth = bvh.max_depth;
   s->info.bvh_leaf_prims_max = bvh.max_leaf;
-  s->info.maj_res = uint32_t(d->has_volume ? d->maj_res : 0);
+  s->info.maj_res = uint32_t(d->has_volume ? ds.mres : 0);
   s->info.device_bytes = s->bytes;
   s->info.grid_max_density = gmax;
   c->refs.fetch_add(1);

@@ -26,6 +26,8 @@ NE_HD float xsqrt(float x) { return sqrtf(x); }
 NE_HD double xsqrt(double x) { return sqrt(x); }
 NE_HD float xlog(float x) { return logf(x); }
 NE_HD double xlog(double x) { return log(x); }
+NE_HD float xexp(float x) { return expf(x); }
+NE_HD double xexp(double x) { return exp(x); }
 NE_HD float xsin(float x) { return sinf(x); }
 NE_HD double xsin(double x) { return sin(x); }
 NE_HD float xcos(float x) { return cosf(x); }

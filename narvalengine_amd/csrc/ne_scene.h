@@ -46,11 +46,12 @@ struct DevScene {
   const int32_t* sph_orig;
   const float4* rects;
   const float* mats;
-  const int32_t* light_rects;
+  const uint32_t* emitters;  // emissive primitives: type << 30 | storage slot
   uint32_t n_nodes;
-  int32_t n_light_rects;
+  int32_t n_emitters;
   // volume
   int32_t has_volume;
+  int32_t homog;          // homogeneous medium: density 1 everywhere in the cube
   int32_t res[3];
   int32_t mres;
   float inv[12];          // WCS -> OCS, rows of a 3x4 matrix

@@ -368,7 +368,10 @@ NE_DEV float grid_at(const DevScene& s, int x, int y, int z) {
 // GridMedia::sampleAt -> interpolatedDensity (src/materials/GridMedia.cpp:25-43):
 // OCS [-.5,.5]^3 -> grid space (p + .5) * res, clamp, cell-corner trilinear,
 // corners at index >= res read as 0.
+// A homogeneous medium (HomogeneousMedia, src/materials/HomogeneousMedia.cpp)
+// reads density 1 everywhere; the multiplier carries its density.
 NE_DEV float density_gcs(const DevScene& s, Vec3<float> gp) {
+  if (s.homog) return 1.0f;
   gp = v3(xmax(0.0f, gp.x), xmax(0.0f, gp.y), xmax(0.0f, gp.z));
   gp = v3(xmin(gp.x, float(s.res[0])), xmin(gp.y, float(s.res[1])), xmin(gp.z, float(s.res[2])));
   int ix = int(floorf(gp.x)), iy = int(floorf(gp.y)), iz = int(floorf(gp.z));
@@ -458,7 +461,11 @@ NE_DEV bool delta_track(const DevScene& s, Vec3<R> o, Vec3<R> d, R ta, R tb, Pat
 
 // Ratio tracking with the reference's Russian roulette (GridMedia.cpp:54-66).
 template <typename R, bool C>
+// A homogeneous medium takes the expectation in closed form,
+// HomogeneousMedia::Tr (HomogeneousMedia.cpp:15-18) with the grey extinction
+// of the tracking step law, and draws nothing.
 NE_DEV R ratio_track(const DevScene& s, Vec3<R> o, Vec3<R> d, R ta, R tb, PathRng& rng, Count<C>& cnt) {
+  if (s.homog) return xexp(-R(s.sigma_bar) * (tb - ta));
   Vec3<R> oo = xf_point<R>(s.inv, o), dd = xf_vec<R>(s.inv, d);
   R Tr = R(1);
   majorant_dda<R, C>(s, oo, dd, ta, tb, cnt, [&](R c0, R c1, R mu) {
@@ -610,6 +617,111 @@ NE_DEV Vec3<R> env_le(const DevScene& s, Vec3<R> d) {
   return v3s(R(0));
 }
 
+// Emissive primitives: rectangles, spheres and triangles with an emitter
+// material (DiffuseLight on a primitive, src/lights/DiffuseLight.cpp:8-20).
+// Light codes are type << 30 | storage slot.  Emission is two-sided.
+//   rectangle  uniform point on the quad, area pdf -> solid angle
+//              (Rectangle::samplePointOnSurface/pdf, src/primitives/Rectangle.cpp:78-88,159-171)
+//   triangle   uniform point (b1 = v sqrt(u), b2 = sqrt(u)(1 - v)), area pdf
+//              (Triangle::samplePointOnSurface, src/primitives/Triangle.cpp:114-119,159-161)
+//   sphere     from outside: a uniform direction in the cone the sphere
+//              subtends, pdf 1 / (2 pi (1 - cos theta_max)); from inside: a
+//              uniform point on the surface.  The reference returns the
+//              nearest surface point with a mis-scaled area pdf
+//              (src/primitives/Sphere.cpp:46-67); the cone is the unbiased form.
+template <typename R>
+NE_DEV R sphere_cone_width(R dc2, R r2) {  // 1 - cos theta_max, cancellation-free
+  const R sin2 = r2 / dc2;
+  return sin2 / (R(1) + xsqrt(xmax(R(0), R(1) - sin2)));
+}
+template <typename R>
+NE_DEV void sample_emitter(const DevScene& s, uint32_t code, Vec3<R> x, R ua, R ub, Vec3<R>& wi, R& dist, R& pl,
+                           Vec3<R>& Le) {
+  const int type = int(code >> 30);
+  const uint32_t slot = code & 0x3FFFFFFFu;
+  int mat;
+  if (type == PRIM_RECT) {
+    const float4* rc = s.rects + 4 * size_t(slot);
+    const float4 f0 = rc[0], f1 = rc[1], f2 = rc[2], f3 = rc[3];
+    const Vec3<R> p0 = v3f4<R>(f0), eu = v3f4<R>(f1), ev = v3f4<R>(f2), ln = v3f4<R>(f3);
+    const Vec3<R> y = p0 + ua * eu + ub * ev;
+    const Vec3<R> dv = y - x;
+    const R d2 = dot(dv, dv);
+    dist = xsqrt(d2);
+    wi = dv / dist;
+    const R cosl = xfabs(dot(ln, wi));
+    const R area = length(cross(eu, ev));
+    mat = __float_as_int(f0.w);
+    pl = cosl > R(0) ? d2 / (cosl * area) : R(0);
+  } else if (type == PRIM_SPHERE) {
+    const float4 sp = s.sph[slot];
+    const Vec3<R> c = v3f4<R>(sp);
+    const R r = R(sp.w), r2 = r * r;
+    const Vec3<R> oc = c - x;
+    const R dc2 = dot(oc, oc);
+    mat = s.sph_mat[slot];
+    if (dc2 > r2) {
+      const R dc = xsqrt(dc2);
+      const Vec3<R> w = oc / dc;
+      const R wid = sphere_cone_width(dc2, r2);
+      const R q = ua * wid;  // 1 - cos theta
+      const R ct = R(1) - q, st = xsqrt(xmax(R(0), q * (R(2) - q)));
+      const R ph = R(kTWO_PI) * ub;
+      Vec3<R> u, v;
+      onb(w, u, v);
+      wi = u * (st * xcos(ph)) + v * (st * xsin(ph)) + w * ct;
+      dist = dc * ct - xsqrt(xmax(R(0), r2 - dc2 * st * st));
+      pl = R(1) / (R(kTWO_PI) * wid);
+    } else {
+      const R z = R(1) - R(2) * ua, rr = xsqrt(xmax(R(0), R(1) - z * z)), ph = R(kTWO_PI) * ub;
+      const Vec3<R> nl = v3(rr * xcos(ph), rr * xsin(ph), z);
+      const Vec3<R> dv = (c + r * nl) - x;
+      const R d2 = dot(dv, dv);
+      dist = xsqrt(d2);
+      wi = dv / dist;
+      const R cosl = xfabs(dot(nl, wi));
+      pl = cosl > R(0) ? d2 / (cosl * R(2.0 * kTWO_PI) * r2) : R(0);
+    }
+  } else {
+    const float4* tp = s.tris + 3 * size_t(slot);
+    const float4 a = tp[0];
+    const Vec3<R> v0 = v3f4<R>(a), e1 = v3f4<R>(tp[1]), e2 = v3f4<R>(tp[2]);
+    const R su = xsqrt(ua);
+    const Vec3<R> y = v0 + (ub * su) * e1 + (su * (R(1) - ub)) * e2;
+    const Vec3<R> dv = y - x;
+    const R d2 = dot(dv, dv);
+    dist = xsqrt(d2);
+    wi = dv / dist;
+    const Vec3<R> cr = cross(e1, e2);
+    const R len = length(cr);
+    const R cosl = xfabs(dot(cr, wi)) / len;
+    mat = __float_as_int(a.w);
+    pl = cosl > R(0) ? d2 / (cosl * (R(0.5) * len)) : R(0);
+  }
+  Le = v3f<R>(s.mats + 12 * size_t(mat) + 6);
+}
+// Solid-angle pdf with which sample_emitter picks the point that the ray
+// (o, d) hits at distance t on emitter (type, slot) with normal n.
+template <typename R>
+NE_DEV R emitter_hit_pdf(const DevScene& s, int type, uint32_t slot, Vec3<R> o, Vec3<R> d, R t, Vec3<R> n) {
+  const R cosl = xfabs(dot(n, d));
+  if (type == PRIM_RECT) {
+    const R area = R(s.rects[4 * size_t(slot) + 1].w);
+    return (t * t) / (cosl * area);
+  }
+  if (type == PRIM_SPHERE) {
+    const float4 sp = s.sph[slot];
+    const R r2 = R(sp.w) * R(sp.w);
+    const Vec3<R> oc = v3f4<R>(sp) - o;
+    const R dc2 = dot(oc, oc);
+    if (dc2 > r2) return R(1) / (R(kTWO_PI) * sphere_cone_width(dc2, r2));
+    return (t * t) / (cosl * R(2.0 * kTWO_PI) * r2);
+  }
+  const float4* tp = s.tris + 3 * size_t(slot);
+  const R len = length(cross(v3f4<R>(tp[1]), v3f4<R>(tp[2])));
+  return (t * t) / (cosl * (R(0.5) * len));
+}
+
 // Shadow transmittance: surface occlusion, then ratio tracking through the
 // volume over [0, tmax].
 template <typename R, bool C>
@@ -699,13 +811,7 @@ NE_DEV bool path_bounce(const DevScene& s, PathState<R>& p, int max_bounces, Pat
       if (m[0] == 1.0f) {  // emitter
         Vec3<R> Le = v3f<R>(m + 6);
         R w = R(1);
-        if (!prev_delta) {
-          const float4* rc = s.rects + 4 * size_t(hit.slot);
-          R area = R(rc[1].w);
-          R cosl = xfabs(dot(n, d));
-          R pl = (hit.t * hit.t) / (cosl * area);
-          w = power_heuristic(prev_pdf, pl);
-        }
+        if (!prev_delta) w = power_heuristic(prev_pdf, emitter_hit_pdf<R>(s, hit.type, hit.slot, o, d, hit.t, n));
         L = L + beta * Le * w;
         p.L = L;
         return true;
@@ -750,20 +856,8 @@ NE_DEV bool path_bounce(const DevScene& s, PathState<R>& p, int max_bounces, Pat
         Le = env_le<R>(s, wi);
         pl = R(1.0 / kFOUR_PI);
       } else {
-        int ri = s.light_rects[li - env_slot - (s.env_type != 0 ? 1 : 0)];
-        const float4* rc = s.rects + 4 * size_t(ri);
-        float4 f0 = rc[0], f1 = rc[1], f2 = rc[2], f3 = rc[3];
-        Vec3<R> p0 = v3f4<R>(f0), eu = v3f4<R>(f1), ev = v3f4<R>(f2), ln = v3f4<R>(f3);
-        Vec3<R> y = p0 + ua * eu + ub * ev;
-        Vec3<R> dv = y - x;
-        R d2 = dot(dv, dv);
-        dist = xsqrt(d2);
-        wi = dv / dist;
-        R cosl = xfabs(dot(ln, wi));
-        R area = length(cross(eu, ev));
-        const float* lm = s.mats + 12 * size_t(__float_as_int(f0.w));
-        Le = v3f<R>(lm + 6);
-        pl = cosl > R(0) ? d2 / (cosl * area) : R(0);
+        const uint32_t code = s.emitters[li - env_slot - (s.env_type != 0 ? 1 : 0)];
+        sample_emitter<R>(s, code, x, ua, ub, wi, dist, pl, Le);
       }
       if (pl > R(0)) {
         Vec3<R> f;

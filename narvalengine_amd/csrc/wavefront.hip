@@ -367,20 +367,8 @@ __device__ __forceinline__ bool nee_sample(const DevScene& s, Vec3<float> x, Vec
     Le = env_le<float>(s, wi);
     pl = float(1.0 / kFOUR_PI);
   } else {
-    const int ri = s.light_rects[li - env_slot - (s.env_type != 0 ? 1 : 0)];
-    const float4* rc = s.rects + 4 * size_t(ri);
-    const float4 f0 = rc[0], f1 = rc[1], f2 = rc[2], f3 = rc[3];
-    const Vec3<float> p0 = v3f4<float>(f0), eu = v3f4<float>(f1), ev = v3f4<float>(f2), ln = v3f4<float>(f3);
-    const Vec3<float> y = p0 + ua * eu + ub * ev;
-    const Vec3<float> dv = y - x;
-    const float d2 = dot(dv, dv);
-    dist = sqrtf(d2);
-    wi = dv / dist;
-    const float cosl = fabsf(dot(ln, wi));
-    const float area = length(cross(eu, ev));
-    const float* lm = s.mats + 12 * size_t(__float_as_int(f0.w));
-    Le = v3f<float>(lm + 6);
-    pl = cosl > 0.0f ? d2 / (cosl * area) : 0.0f;
+    const uint32_t code = s.emitters[li - env_slot - (s.env_type != 0 ? 1 : 0)];
+    sample_emitter<float>(s, code, x, ua, ub, wi, dist, pl, Le);
   }
   if (!(pl > 0.0f)) return false;
   float pf;
@@ -518,13 +506,7 @@ __global__ __launch_bounds__(kChunk) void k_wf_shade(WfState w, WfArgs a, DevSce
       if (m[0] == 1.0f) {  // emitter: MIS-weighted emission, the path ends
         const Vec3<float> E = v3f<float>(m + 6);
         float wm = 1.0f;
-        if (!prev_delta) {
-          const float4* rc = s.rects + 4 * size_t(hit.slot);
-          const float area = rc[1].w;
-          const float cosl = fabsf(dot(nrm, d));
-          const float plh = (th * th) / (cosl * area);
-          wm = power_heuristic(A.prev_pdf[i], plh);
-        }
+        if (!prev_delta) wm = power_heuristic(A.prev_pdf[i], emitter_hit_pdf<float>(s, hit.type, hit.slot, o, d, th, nrm));
         lx += beta.x * E.x * wm; ly += beta.y * E.y * wm; lz += beta.z * E.z * wm;
         ends = true;
       } else if (int(depth) >= a.max_bounces) {

@@ -8,7 +8,8 @@ Follows what the reference's SceneReader builds (src/io/SceneReader.cpp:224-648)
     normalised (src/primitives/InstancedModel.cpp:31-34);
   * sphere: centre at the instance position, radius as given (translation only,
     SceneReader.cpp:385);
-  * volume: GridMedia on the unit cube [-.5,.5]^3 under the instance transform.
+  * volume: GridMedia on the unit cube [-.5,.5]^3 under the instance transform,
+    or HomogeneousMedia (constant density) on the same cube.
 The flattened, world-space arrays are the ABI input shared by the GPU library
 and the test oracle.
 """
@@ -125,6 +126,15 @@ class Scene:
         self.volume = dict(grid=grid, mult=float(density_mult), sigma_s=tuple(sigma_s), sigma_a=tuple(sigma_a),
                            g=float(g), xf=M, inv=np.linalg.inv(M), maj_res=int(maj_res))
 
+    def set_homogeneous_volume(self, density: float, sigma_s, sigma_a, g: float,
+                               position=(0, 0, 0), rotation=(0, 0, 0), scale=(1, 1, 1)):
+        """HomogeneousMedia (src/materials/HomogeneousMedia.cpp:5-51) on the unit
+        cube under the instance transform; the material's "density" is the
+        constant density."""
+        M = transform(position, rotation, scale)
+        self.volume = dict(grid=None, mult=float(density), sigma_s=tuple(sigma_s), sigma_a=tuple(sigma_a),
+                           g=float(g), xf=M, inv=np.linalg.inv(M), maj_res=1)
+
     def set_sun(self, direction, radiance=(1.0, 1.0, 1.0)):
         d = np.asarray(direction, np.float64)
         self.sun = dict(dir=d / np.linalg.norm(d), E=tuple(radiance))
@@ -143,7 +153,7 @@ class Scene:
             rect=cat(self.rect, 12, np.float32), rect_mat=np.asarray(self.rect_mat, np.int32),
             mat=cat(self.mats, 12, np.float32),
         )
-        if self.volume is not None:
+        if self.volume is not None and self.volume["grid"] is not None:
             a["grid"] = self.volume["grid"]
         return a
 
@@ -160,9 +170,12 @@ class Scene:
             d.has_volume = 1
             d.vol_xf[:] = [float(x) for x in v["xf"].reshape(-1)]
             d.vol_inv[:] = [float(x) for x in v["inv"].reshape(-1)]
-            D, H, W = v["grid"].shape
-            d.vol_res[:] = [W, H, D]
-            d.grid = _ptr(a["grid"], C.c_float)
+            if v["grid"] is None:
+                d.vol_homogeneous = 1
+            else:
+                D, H, W = v["grid"].shape
+                d.vol_res[:] = [W, H, D]
+                d.grid = _ptr(a["grid"], C.c_float)
             d.vol_mult = v["mult"]
             d.vol_sigma_s[:] = v["sigma_s"]
             d.vol_sigma_a[:] = v["sigma_a"]

@@ -409,6 +409,34 @@ inline float ref_ratio_track(const Grid& g, float inv_max, float sigma_bar, V3f 
   return Tr;
 }
 
+// HomogeneousMedia::Tr, src/materials/HomogeneousMedia.cpp:15-22:
+// exp(-extinction * distance * density) per channel.
+inline V3f ref_homogeneous_tr(V3f extinction, float density, float distance) {
+  V3f v = extinction * distance * density;
+  return V3f(std::exp(-v.x), std::exp(-v.y), std::exp(-v.z));
+}
+// HomogeneousMedia::sample, src/materials/HomogeneousMedia.cpp:24-50, as the
+// reference writes it: the free-flight distance ignores the density and is
+// divided by the chord length (tFar - tNear), and the weight is
+// Tr * scattering / avg(extinction * Tr) on a collision, Tr / avg(Tr) else.
+// Returns the collision flag; *t_out is the sampled t (or the chord length).
+template <class R>
+inline bool ref_homogeneous_sample(V3f scattering, V3f absorption, float density, float tNear, float tFar, R& rng,
+                                   float* t_out, V3f* weight) {
+  V3f ext = absorption + scattering;
+  float t = -std::log(1 - rng.next()) / avg(ext);
+  float dist = tFar - tNear;
+  t = t / dist;
+  bool sampled = t < dist;
+  V3f tr = ref_homogeneous_tr(ext, density, t);
+  V3f dens = sampled ? ext * tr : tr;
+  float pdf = avg(dens);
+  if (pdf == 0) pdf = 1;
+  *weight = sampled ? tr * scattering / pdf : tr / pdf;
+  *t_out = sampled ? t : dist;
+  return sampled;
+}
+
 // OfflineEngine::postProcessing, src/core/OfflineEngine.cpp:27-52.
 inline V3f ref_post_processing(V3f c) {
   const float exposure = 0.5f;
@@ -467,6 +495,7 @@ struct SceneDesc {
   int maj_res;                                                      // majorant cells per axis (1 = global)
   int has_sun; float sun_dir[3]; float sun_E[3];                    // sun_dir: direction light travels
   int env_type; float env_c0[3], env_c1[3], env_c2[3];              // const: c0; sky: zenith c0, horizon c1, ground c2
+  int vol_homogeneous;                                              // constant density vol_mult; grid unused
 };
 
 struct CameraDesc {
@@ -507,27 +536,36 @@ struct World {
   int mres = 1;
   std::vector<float> maj;      // majorant of sigma_t per cell (mres^3)
   float sigma_bar = 0;         // avg(sigma_t * mult)
-  int n_lights = 0;            // sun? + env? + emitter rects
-  std::vector<int> light_rects;
+  int n_lights = 0;            // sun? + env? + emissive primitives
+  struct Emitter { int type; uint32_t index; };  // type 0 tri, 1 sphere, 2 rect; input index
+  std::vector<Emitter> emitters;
 
   static V ldv(const float* q) { return V(T(q[0]), T(q[1]), T(q[2])); }
 
   explicit World(const SceneDesc* sd) : s(sd) {
     build_bvh();
     if (s->has_volume) {
-      grid.W = s->vol_res[0]; grid.H = s->vol_res[1]; grid.D = s->vol_res[2]; grid.data = s->grid;
+      static const float kOne = 1.0f;
+      if (s->vol_homogeneous) { grid.W = grid.H = grid.D = 1; grid.data = &kOne; }
+      else { grid.W = s->vol_res[0]; grid.H = s->vol_res[1]; grid.D = s->vol_res[2]; grid.data = s->grid; }
       for (int i = 0; i < 16; ++i) { xf.m[i] = T(s->vol_xf[i]); inv.m[i] = T(s->vol_inv[i]); }
       V3f st(s->vol_sigma_a[0] + s->vol_sigma_s[0], s->vol_sigma_a[1] + s->vol_sigma_s[1],
              s->vol_sigma_a[2] + s->vol_sigma_s[2]);
       sigma_bar = avg(st * s->vol_mult);
-      mres = std::max(1, s->maj_res);
+      mres = s->vol_homogeneous ? 1 : std::max(1, s->maj_res);
       build_majorants();
     }
     if (s->has_sun) n_lights++;
     if (s->env_type != ENV_NONE) n_lights++;
+    // light-selection order: emissive rectangles, spheres, triangles, each in input order
+    auto emits = [&](int m) { return s->mat[12 * size_t(m)] == float(MAT_EMITTER); };
     for (uint32_t i = 0; i < s->n_rect; ++i)
-      if (s->mat[12 * s->rect_mat[i]] == float(MAT_EMITTER)) light_rects.push_back(int(i));
-    n_lights += int(light_rects.size());
+      if (emits(s->rect_mat[i])) emitters.push_back({2, i});
+    for (uint32_t i = 0; i < s->n_sph; ++i)
+      if (emits(s->sph_mat[i])) emitters.push_back({1, i});
+    for (uint32_t i = 0; i < s->n_tri; ++i)
+      if (emits(s->tri_mat ? s->tri_mat[i] : 0)) emitters.push_back({0, i});
+    n_lights += int(emitters.size());
   }
 
   // Majorant per cell: max voxel over the cell's trilinear support (the
@@ -784,7 +822,9 @@ struct World {
     return t0 <= t1;
   }
 
+  // A homogeneous medium has density 1 (times vol_mult) everywhere in the cube.
   T density_ocs(V p) const {
+    if (s->vol_homogeneous) return T(1);
     V3f pf(float(p.x), float(p.y), float(p.z));
     return T(ref_density_ocs(grid, pf));
   }
@@ -845,7 +885,11 @@ struct World {
   }
 
   // Ratio tracking with the reference's Russian roulette (GridMedia.cpp:54-66).
+  // In a homogeneous medium the ratio-tracking expectation is taken in closed
+  // form: exp(-sigma_bar (tb - ta)), HomogeneousMedia::Tr with the grey
+  // extinction of the tracking step law.
   T ratio_track(V o, V d, T ta, T tb, Rng& rng) const {
+    if (s->vol_homogeneous) return std::exp(-T(sigma_bar) * (tb - ta));
     V oo = inv.point(o), dd = inv.vec(d);
     T Tr = T(1);
     dda(oo, dd, ta, tb, [&](T c0, T c1, T mu) {
@@ -877,6 +921,99 @@ struct World {
       return gnd;
     }
     return V(T(0));
+  }
+
+  // Emissive primitives (corrected forms; the reference's sphere sampler
+  // returns the nearest surface point with a mis-scaled area pdf,
+  // src/primitives/Sphere.cpp:46-67):
+  //   rectangle: uniform point, area pdf d^2 / (cos area)
+  //   triangle:  uniform point (b1 = v sqrt(u), b2 = sqrt(u)(1 - v)), area pdf
+  //   sphere:    uniform direction in the subtended cone, pdf 1 / (2 pi (1 - cos theta_max));
+  //              uniform surface point when x is inside.
+  static T cone_width(T dc2, T r2) {
+    T sin2 = r2 / dc2;
+    return sin2 / (T(1) + std::sqrt(std::max(T(0), T(1) - sin2)));
+  }
+  void sample_emitter(const Emitter& e, V x, T ua, T ub, V& wi, T& dist, T& pl, V& Le) const {
+    int m;
+    if (e.type == 2) {
+      const float* q = s->rect + 12 * size_t(e.index);
+      V p0 = ldv(q), eu = ldv(q + 3), ev = ldv(q + 6), ln = ldv(q + 9);
+      V y = p0 + ua * eu + ub * ev;
+      V dv = y - x;
+      T d2 = dot(dv, dv);
+      dist = std::sqrt(d2);
+      wi = dv / dist;
+      T cosl = std::fabs(dot(ln, wi));
+      T area = length(cross(eu, ev));
+      m = s->rect_mat[e.index];
+      pl = cosl > T(0) ? d2 / (cosl * area) : T(0);
+    } else if (e.type == 1) {
+      const float* q = s->sph + 4 * size_t(e.index);
+      V c = ldv(q);
+      T r = T(q[3]), r2 = r * r;
+      V oc = c - x;
+      T dc2 = dot(oc, oc);
+      m = s->sph_mat[e.index];
+      if (dc2 > r2) {
+        T dc = std::sqrt(dc2);
+        V w = oc / dc;
+        T wid = cone_width(dc2, r2);
+        T qq = ua * wid;
+        T ct = T(1) - qq, st = std::sqrt(std::max(T(0), qq * (T(2) - qq)));
+        T ph = T(kTWO_PI) * ub;
+        V u, v;
+        ref_onb(w, u, v);
+        wi = u * (st * std::cos(ph)) + v * (st * std::sin(ph)) + w * ct;
+        dist = dc * ct - std::sqrt(std::max(T(0), r2 - dc2 * st * st));
+        pl = T(1) / (T(kTWO_PI) * wid);
+      } else {
+        T z = T(1) - T(2) * ua, rr = std::sqrt(std::max(T(0), T(1) - z * z)), ph = T(kTWO_PI) * ub;
+        V nl(rr * std::cos(ph), rr * std::sin(ph), z);
+        V dv = (c + r * nl) - x;
+        T d2 = dot(dv, dv);
+        dist = std::sqrt(d2);
+        wi = dv / dist;
+        T cosl = std::fabs(dot(nl, wi));
+        pl = cosl > T(0) ? d2 / (cosl * T(2.0 * kTWO_PI) * r2) : T(0);
+      }
+    } else {
+      const float* q = s->tri_v + 9 * size_t(e.index);
+      V v0 = ldv(q), e1 = ldv(q + 3) - v0, e2 = ldv(q + 6) - v0;
+      T su = std::sqrt(ua);
+      V y = v0 + (ub * su) * e1 + (su * (T(1) - ub)) * e2;
+      V dv = y - x;
+      T d2 = dot(dv, dv);
+      dist = std::sqrt(d2);
+      wi = dv / dist;
+      V cr = cross(e1, e2);
+      T len = length(cr);
+      T cosl = std::fabs(dot(cr, wi)) / len;
+      m = s->tri_mat ? s->tri_mat[e.index] : 0;
+      pl = cosl > T(0) ? d2 / (cosl * (T(0.5) * len)) : T(0);
+    }
+    Le = ldv(s->mat + 12 * size_t(m) + 6);
+  }
+  // Solid-angle pdf of sample_emitter for the point the ray (o, d) hits.
+  T emitter_hit_pdf(const SHit& h, V o, V d) const {
+    T cosl = std::fabs(dot(h.n, d));
+    if (h.type == 2) {
+      const float* q = s->rect + 12 * size_t(h.index);
+      T area = length(cross(ldv(q + 3), ldv(q + 6)));
+      return (h.t * h.t) / (cosl * area);
+    }
+    if (h.type == 1) {
+      const float* q = s->sph + 4 * size_t(h.index);
+      T r2 = T(q[3]) * T(q[3]);
+      V oc = ldv(q) - o;
+      T dc2 = dot(oc, oc);
+      if (dc2 > r2) return T(1) / (T(kTWO_PI) * cone_width(dc2, r2));
+      return (h.t * h.t) / (cosl * T(2.0 * kTWO_PI) * r2);
+    }
+    const float* q = s->tri_v + 9 * size_t(h.index);
+    V v0 = ldv(q);
+    T len = length(cross(ldv(q + 3) - v0, ldv(q + 6) - v0));
+    return (h.t * h.t) / (cosl * (T(0.5) * len));
   }
 
   T transmittance(V o, V d, T tmax, Rng& rng) const {
@@ -1015,14 +1152,7 @@ V3<T> Li(const World<T>& w, V3<T> o, V3<T> d, int max_bounces, Rng& rng, PathSta
       if (m[0] == float(MAT_EMITTER)) {
         V Le = World<T>::ldv(m + 6);
         T wgt = T(1);
-        if (!prev_delta) {
-          const float* q = s->rect + 12 * size_t(hit.index);
-          V eu = World<T>::ldv(q + 3), ev = World<T>::ldv(q + 6);
-          T area = length(cross(eu, ev));
-          T cosl = std::fabs(dot(hit.n, d));
-          T pl = (hit.t * hit.t) / (cosl * area);
-          wgt = power_heuristic(prev_pdf, pl);
-        }
+        if (!prev_delta) wgt = power_heuristic(prev_pdf, w.emitter_hit_pdf(hit, o, d));
         L = L + beta * Le * wgt;
         break;
       }
@@ -1062,20 +1192,8 @@ V3<T> Li(const World<T>& w, V3<T> o, V3<T> d, int max_bounces, Rng& rng, PathSta
         Le = w.env_le(wi);
         pl = T(1.0 / kFOUR_PI);
       } else {
-        int ri = w.light_rects[size_t(li - (s->has_sun ? 1 : 0) - (s->env_type != ENV_NONE ? 1 : 0))];
-        const float* q = s->rect + 12 * size_t(ri);
-        V p0 = World<T>::ldv(q), eu = World<T>::ldv(q + 3), ev = World<T>::ldv(q + 6);
-        V ln = World<T>::ldv(q + 9);
-        V y = p0 + ua * eu + ub * ev;
-        V dv = y - x;
-        T d2 = dot(dv, dv);
-        dist = std::sqrt(d2);
-        wi = dv / dist;
-        T cosl = std::fabs(dot(ln, wi));
-        T area = length(cross(eu, ev));
-        const float* m = s->mat + 12 * size_t(s->rect_mat[ri]);
-        Le = World<T>::ldv(m + 6);
-        pl = cosl > T(0) ? d2 / (cosl * area) : T(0);
+        const auto& e = w.emitters[size_t(li - (s->has_sun ? 1 : 0) - (s->env_type != ENV_NONE ? 1 : 0))];
+        w.sample_emitter(e, x, ua, ub, wi, dist, pl, Le);
       }
       if (pl > T(0)) {
         V f; T pf;

@@ -130,6 +130,18 @@ void neo_ratio_track(int W, int H, int D, const float* grid, float inv_max, floa
     tr[i] = ref_ratio_track(g, inv_max, sigma_bar, v3(ray + 6 * i), v3(ray + 6 * i + 3), tint[2 * i], tint[2 * i + 1], r);
   }
 }
+void neo_homogeneous_tr(int n, const float* ext, const float* density, const float* dist, float* out) {
+  for (int i = 0; i < n; ++i) put(out + 3 * i, ref_homogeneous_tr(v3(ext + 3 * i), density[i], dist[i]));
+}
+void neo_homogeneous_sample(int n, const float* sig_s, const float* sig_a, float density, const float* tint,
+                            uint64_t seed, int* sampled, float* t_out, float* weight) {
+  for (int i = 0; i < n; ++i) {
+    Rng r(seed, uint32_t(i), 0, 0);
+    V3f w;
+    sampled[i] = ref_homogeneous_sample(v3(sig_s), v3(sig_a), density, tint[2 * i], tint[2 * i + 1], r, t_out + i, &w);
+    put(weight + 3 * i, w);
+  }
+}
 void neo_post_processing(int n, const float* in, float* out) {
   for (int i = 0; i < n; ++i) put(out + 3 * i, ref_post_processing(v3(in + 3 * i)));
 }

@@ -58,6 +58,8 @@ def lib():
             "neo_ratio_track": (None, [C.c_int, C.c_int, C.c_int, _F, C.c_float, C.c_float, C.c_int, _F, _F,
                                        C.c_uint64, _F]),
             "neo_post_processing": (None, [C.c_int, _F, _F]),
+            "neo_homogeneous_tr": (None, [C.c_int, _F, _F, _F, _F]),
+            "neo_homogeneous_sample": (None, [C.c_int, _F, _F, C.c_float, _F, C.c_uint64, _I, _F, _F]),
             "neo_camera_dirs": (None, [C.c_void_p, C.c_int, _F, _F]),
             "neo_tile_rect": (None, [C.c_int, C.c_int, C.c_int, _I]),
             "neo_hg_p": (None, [C.c_int, _F, _F, _F]),
@@ -150,6 +152,22 @@ def vec_op(name, n, *arrays, out_shape):
     out = np.zeros(out_shape, np.float32)
     getattr(lib(), name)(n, *[fp(a) for a in arrs], fp(out))
     return out
+
+
+def homogeneous_tr(ext, density, dist):
+    ext, density, dist = f(ext).reshape(-1, 3), f(density), f(dist)
+    out = np.zeros((len(ext), 3), np.float32)
+    lib().neo_homogeneous_tr(len(ext), fp(ext), fp(density), fp(dist), fp(out))
+    return out
+
+
+def homogeneous_sample(sig_s, sig_a, density, tint, seed=0):
+    tint = f(tint).reshape(-1, 2)
+    n = len(tint)
+    sampled, t, w = np.zeros(n, np.int32), np.zeros(n, np.float32), np.zeros((n, 3), np.float32)
+    lib().neo_homogeneous_sample(n, fp(f(sig_s)), fp(f(sig_a)), float(density), fp(tint), seed, ip(sampled), fp(t),
+                                 fp(w))
+    return sampled, t, w
 
 
 class World:

@@ -19,6 +19,40 @@ def small_volume(res=32, seed=0, maj_res=4, env=True, sun=True, mult=20.0, albed
     return s
 
 
+def homogeneous_box(density=3.0, albedo=0.9, g=0.6, rotation=(20, 35, 0)) -> Scene:
+    """A rotated, non-uniformly scaled homogeneous medium over a floor, lit by
+    an area light, the sun and a sky."""
+    s = Scene()
+    s.set_homogeneous_volume(density, (albedo,) * 3, (1 - albedo,) * 3, g, position=(0, 0.1, 0),
+                             rotation=rotation, scale=(0.8, 0.5, 0.6))
+    floor = s.add_glossy((0.6, 0.6, 0.6), 0.7)
+    s.add_rectangle(floor, position=(0, -0.5, 0), rotation=(90, 0, 0), scale=(4, 4, 1))
+    light = s.add_emitter((8.0, 8.0, 8.0))
+    s.add_rectangle(light, position=(0.3, 1.2, -0.2), rotation=(-90, 0, 0), scale=(0.5, 0.5, 1))
+    s.set_sun(-np.array([1.0, 2.0, -0.7]), (1.0, 1.0, 1.0))
+    s.set_env(NE_ENV_SKY, c0=(0.2, 0.3, 0.6), c1=(0.6, 0.6, 0.6), c2=(0.1, 0.1, 0.1))
+    return s
+
+
+def emitter_mix() -> Scene:
+    """Every emitter shape: a rectangle, two spheres (one small and far, one
+    close to the walls) and an emissive triangle pair, over glossy walls."""
+    s = Scene()
+    wall = s.add_glossy((0.7, 0.7, 0.7), 0.6)
+    red = s.add_glossy((0.8, 0.2, 0.2), 0.3, 0.5)
+    s.add_rectangle(wall, position=(0, 0, 0), rotation=(90, 0, 0), scale=(2, 2, 1))
+    s.add_rectangle(wall, position=(0, 1, 1), rotation=(0, 0, 0), scale=(2, 2, 1))
+    s.add_rectangle(red, position=(-1, 1, 0), rotation=(0, 90, 0), scale=(2, 2, 1))
+    s.add_rectangle(s.add_emitter((4.0, 4.0, 4.0)), position=(0, 1.98, 0), rotation=(-90, 0, 0),
+                    scale=(0.3, 0.3, 1))
+    s.add_sphere((0.5, 0.4, 0.2), 0.2, s.add_emitter((6.0, 3.0, 1.0)))
+    s.add_sphere((-0.6, 1.5, 0.6), 0.05, s.add_emitter((60.0, 60.0, 80.0)))
+    s.add_sphere((-0.3, 0.3, -0.2), 0.3, red)
+    q = np.array([[0.7, 0.9, 0.99], [0.95, 0.9, 0.99], [0.95, 1.3, 0.99], [0.7, 1.3, 0.99]])
+    s.add_mesh(q, np.array([[0, 1, 2], [0, 2, 3]]), s.add_emitter((2.0, 5.0, 2.0)))
+    return s
+
+
 def small_cloud(res=48, maj_res=6) -> Scene:
     return configs.cloud_scene(configs.cumulus_grid(res, 2), maj_res=maj_res)
 

@@ -252,6 +252,28 @@ def test_transmittance_matches_oracle(ctx):
     assert abs(tr64.mean() - tr.mean()) < 4 * np.sqrt(2 * tr.var() / n) + 1e-4
 
 
+def test_homogeneous_medium_matches_oracle(ctx):
+    """HomogeneousMedia: closed-form transmittance (no draws) and exponential
+    free flight, trajectory by trajectory against the oracle."""
+    print_case("homogeneous medium tracking")
+    s = scenes.homogeneous_box()
+    g = ctx.upload(s)
+    w = O.World(s)
+    n = 20000
+    rays = scenes.rays_towards(n, target_box=0.4, origin_radius=1.8, seed=15)
+    rays[:, 1] += 0.1
+    tmax = np.full(n, np.inf, np.float32)
+    tr = ctx.test_transmittance(g, rays, tmax, seed=4)
+    otr = w.transmittance(rays, tmax, seed=4)
+    np.testing.assert_allclose(tr, otr, rtol=2e-5, atol=1e-7)
+    assert (tr < 1).mean() > 0.5
+    sc, t = ctx.test_delta_track(g, rays, seed=3)
+    osc, ot = w.delta_track(rays, seed=3)
+    same = (sc == osc) & (np.abs(t - ot) <= 1e-5 * np.maximum(np.abs(ot), 1e-6))
+    print(f"homogeneous delta tracking identical trajectories: {same.mean():.5f}")
+    assert same.mean() > 0.99
+
+
 # ---- full paths -----------------------------------------------------------------
 def _render_pair(ctx, s, cam, W, H, spp, bounces, seed=21, precision=NE_FP32):
     g = ctx.upload(s)
@@ -263,11 +285,15 @@ def _render_pair(ctx, s, cam, W, H, spp, bounces, seed=21, precision=NE_FP32):
     return gpu, cpu
 
 
-@pytest.mark.parametrize("which", ["volume", "surface", "cloud"])
+@pytest.mark.parametrize("which", ["volume", "surface", "cloud", "homogeneous", "emitters"])
 def test_render_matches_oracle(ctx, which):
     print_case(f"render {which}")
     if which == "volume":
         s, cam, bounces = scenes.small_volume(res=32, maj_res=4), scenes.VOLUME_CAMERA, 16
+    elif which == "emitters":
+        s, cam, bounces = scenes.emitter_mix(), scenes.BOX_CAMERA, 8
+    elif which == "homogeneous":
+        s, cam, bounces = scenes.homogeneous_box(), scenes.VOLUME_CAMERA, 16
     elif which == "cloud":
         s, cam, bounces = scenes.small_cloud(), scenes.VOLUME_CAMERA, 64
     else:
@@ -369,12 +395,16 @@ def test_handles_release_in_any_order():
     f.close()
 
 
-@pytest.mark.parametrize("which", ["volume", "surface", "cloud"])
+@pytest.mark.parametrize("which", ["volume", "surface", "cloud", "homogeneous", "emitters"])
 def test_wavefront_matches_megakernel(ctx, which):
     print_case(f"wavefront vs megakernel {which}")
     from narvalengine_amd.abi import NE_MODE_MEGAKERNEL, NE_MODE_WAVEFRONT
     if which == "volume":
         s, cam, bounces = scenes.small_volume(res=32, maj_res=4), scenes.VOLUME_CAMERA, 16
+    elif which == "emitters":
+        s, cam, bounces = scenes.emitter_mix(), scenes.BOX_CAMERA, 8
+    elif which == "homogeneous":
+        s, cam, bounces = scenes.homogeneous_box(), scenes.VOLUME_CAMERA, 16
     elif which == "cloud":
         s, cam, bounces = scenes.small_cloud(), scenes.VOLUME_CAMERA, 64
     else:

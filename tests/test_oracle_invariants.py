@@ -376,3 +376,88 @@ def test_direct_light_on_a_glossy_floor_matches_quadrature():
     Ld = (f * (cos_x * cos_y / d2)[:, None]).sum(0) * area / n * Le
     got = mean[0, 0]
     assert (np.abs(got - Ld) < 5 * se[0, 0] + 2e-3 * Ld).all(), (got, Ld, se)
+
+
+def _glossy_f(albedo, rough, metal, wo, wi):
+    """The oracle's glossy BSDF value (cosine not applied) for directions in the
+    floor frame (normal +y)."""
+    to_local = lambda v: np.stack([v[..., 0], v[..., 2], v[..., 1]], -1)
+    n = len(wi)
+    f, pdf, smp = np.zeros((n, 3), np.float32), np.zeros(n, np.float32), np.zeros((n, 3), np.float32)
+    mat = np.array([*albedo, rough, metal], np.float32)
+    L().neo_glossy_c(n, O.fp(np.tile(mat, (n, 1))), O.fp(np.tile(to_local(wo), (n, 1)).astype(np.float32)),
+                     O.fp(to_local(wi).astype(np.float32)), O.fp(np.zeros((n, 3), np.float32)), O.fp(f), O.fp(pdf),
+                     O.fp(smp))
+    return f.astype(np.float64)
+
+
+def test_direct_light_from_a_sphere_emitter_matches_quadrature():
+    """Emissive sphere over a glossy floor: cone sampling plus BSDF sampling
+    under MIS must equal the solid-angle integral of f Le cos over the cone the
+    sphere subtends (closed-form cone, midpoint quadrature)."""
+    s = Scene()
+    albedo, rough, metal = (0.6, 0.7, 0.5), 0.5, 0.1
+    floor = s.add_glossy(albedo, rough, metal)
+    s.add_rectangle(floor, position=(0, 0, 0), rotation=(90, 0, 0), scale=(6, 6, 1))
+    Le = np.array([5.0, 4.0, 3.0])
+    c, r = np.array([0.4, 0.9, 0.3]), 0.35
+    s.add_sphere(tuple(c), r, s.add_emitter(tuple(Le)))
+    cam = Camera(position=(0.0, 1.0, -1.0), look_at=(0.0, 0.0, 0.0), up=(0, 1, 0), vfov=0.005, aspect=1.0)
+    w = O.World(s, fp64=True)
+    mean, se = _pixel_stats(w, cam, 1, 1, 40000, 1)
+    x = np.zeros(3)
+    wo = _unit(np.array(cam.position) - x)
+    dc = np.linalg.norm(c - x)
+    cmax = np.sqrt(1 - (r / dc) ** 2)
+    k = 400
+    mu = 1 - (1 - cmax) * (np.arange(k) + 0.5) / k
+    ph = (np.arange(k) + 0.5) * 2 * np.pi / k
+    M, P = np.meshgrid(mu, ph, indexing="ij")
+    st = np.sqrt(1 - M * M)
+    ax = _unit(c - x)
+    u = _unit(np.cross(ax, [1.0, 0, 0]))
+    v = np.cross(ax, u)
+    wi = (st * np.cos(P))[..., None] * u + (st * np.sin(P))[..., None] * v + M[..., None] * ax
+    wi = wi.reshape(-1, 3)
+    dw = (1 - cmax) * 2 * np.pi / (k * k)
+    f = _glossy_f(albedo, rough, metal, wo, wi)
+    Ld = (f * wi[:, 1:2]).sum(0) * dw * Le
+    got = mean[0, 0]
+    assert (np.abs(got - Ld) < 5 * se[0, 0] + 2e-3 * Ld).all(), (got, Ld, se)
+
+
+def test_emissive_triangles_light_like_the_rectangle_they_tile():
+    """Two emissive triangles tiling a rectangle give the same direct light as
+    the emissive rectangle (uniform triangle sampling and its area pdf)."""
+    albedo, rough, metal = (0.7, 0.5, 0.3), 0.45, 0.2
+    Le = (6.0, 5.0, 4.0)
+    cam = Camera(position=(0.0, 1.0, -1.0), look_at=(0.0, 0.0, 0.0), up=(0, 1, 0), vfov=0.005, aspect=1.0)
+    means = []
+    for tris in (False, True):
+        s = Scene()
+        floor = s.add_glossy(albedo, rough, metal)
+        s.add_rectangle(floor, position=(0, 0, 0), rotation=(90, 0, 0), scale=(4, 4, 1))
+        light = s.add_emitter(Le)
+        if tris:
+            q = np.array([[0.0, 0.8, -0.0], [0.6, 0.8, 0.0], [0.6, 0.8, 0.4], [0.0, 0.8, 0.4]])
+            s.add_mesh(q, np.array([[0, 1, 2], [0, 2, 3]]), light)
+        else:
+            s.add_rectangle(light, position=(0.3, 0.8, 0.2), rotation=(-90, 0, 0), scale=(0.6, 0.4, 1))
+        w = O.World(s, fp64=True)
+        means.append(_pixel_stats(w, cam, 1, 1, 40000, 1))
+    (m0, s0), (m1, s1) = means
+    assert (np.abs(m0 - m1) < 5 * np.sqrt(s0 ** 2 + s1 ** 2) + 1e-3 * m0).all(), (m0, m1)
+
+
+def test_furnace_inside_an_emissive_sphere():
+    """A camera inside an emissive sphere of radiance 1 around a non-absorbing
+    homogeneous medium: radiance is 1 everywhere (area sampling from inside
+    the sphere, MIS with the hit pdf)."""
+    s = Scene()
+    s.add_sphere((0.0, 0.0, 0.0), 3.0, s.add_emitter((1.0, 1.0, 1.0)))
+    s.set_homogeneous_volume(2.0, (1.0,) * 3, (0.0,) * 3, 0.5, scale=(1.5, 1.0, 1.2))
+    cam = Camera(position=(0.0, 0.2, -2.2), look_at=(0, 0, 0), up=(0, 1, 0), vfov=40.0, aspect=1.0)
+    w = O.World(s)
+    spp = 256
+    img = w.render(cam, 8, 8, spp, 256, seed=6) / spp
+    assert abs(img.mean() - 1.0) < 0.01, img.mean()
