@@ -49,9 +49,12 @@ typedef struct ne_film ne_film;
  * Glossy = GlossyBSDF (src/core/GlossyBSDF.cpp:4-48, alpha = roughness^2);
  * emitter = DiffuseLight (src/lights/DiffuseLight.cpp:4-27). */
 enum { NE_MAT_GLOSSY = 0, NE_MAT_EMITTER = 1 };
-/* Environment: none, constant radiance env_c0, or a sky gradient
- * (zenith env_c0, horizon env_c1, below-horizon env_c2). */
-enum { NE_ENV_NONE = 0, NE_ENV_CONSTANT = 1, NE_ENV_SKY = 2 };
+/* Environment: none, constant radiance env_c0, a sky gradient (zenith
+ * env_c0, horizon env_c1, below-horizon env_c2), or an InfiniteAreaLight
+ * (src/lights/InfiniteAreaLight.h:8-130) latitude-longitude map: env_w x
+ * env_h RGB floats, row-major, x fastest, row 0 at theta = 0 of the map's
+ * z-up frame, importance-sampled. */
+enum { NE_ENV_NONE = 0, NE_ENV_CONSTANT = 1, NE_ENV_SKY = 2, NE_ENV_MAP = 3 };
 
 /* Flattened world-space scene.  Mirrors what SceneReader builds
  * (src/io/SceneReader.cpp:224-648) after InstancedModel transforms:
@@ -100,6 +103,9 @@ typedef struct ne_scene_desc {
   float env_c1[3];
   float env_c2[3];
   int32_t vol_homogeneous; /* 1: constant density vol_mult inside the cube; analytic transmittance */
+  const float* env_map;    /* NE_ENV_MAP: env_w * env_h * 3 floats */
+  int32_t env_w;
+  int32_t env_h;
 } ne_scene_desc;
 
 /* Pinhole camera: Camera(lookFrom, lookAt, up, vfov, aspect, ...)
@@ -452,6 +458,23 @@ void ne_scene_file_release(ne_scene_file* file);
  * then W*H*D whitespace-separated floats, x fastest.  grid == NULL queries the
  * resolution only; otherwise capacity (floats) must hold W*H*D. */
 ne_status ne_vol_read(const char* path, int32_t res[3], float* grid, uint64_t capacity);
+
+/* ---- output images ----------------------------------------------------------
+ * Writers for width x height RGB float images (row-major, x fastest, row 0 at
+ * the top), host-only:
+ *   NE_IMAGE_PPM16  OfflineEngine::coreLoop's output.ppm
+ *                   (src/core/OfflineEngine.cpp:80-137): P6, maxval 65535,
+ *                   pixels from the last index to the first, channel values
+ *                   clamped to [0, 1] and truncated from value * 65535
+ *   NE_IMAGE_PNG    saveImage for PNG (src/materials/Texture.h:44-60): 8-bit
+ *                   RGB, clamp to [0, 1] then value * 255 truncated
+ *   NE_IMAGE_EXR    saveImage for EXR (Texture.h:62-69): scanline OpenEXR,
+ *                   FP32 channels, no compression
+ *   NE_IMAGE_PFM    linear Portable FloatMap (rows stored bottom to top)
+ * Post-processed pixels (ne_film_tonemap) suit PPM16 and PNG; linear sums
+ * divided by spp suit EXR and PFM. */
+enum { NE_IMAGE_PPM16 = 0, NE_IMAGE_PNG = 1, NE_IMAGE_EXR = 2, NE_IMAGE_PFM = 3 };
+ne_status ne_image_write(const char* path, int32_t format, int32_t width, int32_t height, const float* rgb);
 
 #ifdef __cplusplus
 }

@@ -8,6 +8,9 @@
 //                                   renderTile(camera, index, finished) writes the
 //                                   post-processed tile into `pixels` and sets
 //                                   `finished`; coreLoop() renders all 400 tiles
+//   narval_gpu::SceneReader         SceneReader (src/io/SceneReader.h:8-40):
+//                                   loadScene(path, absolutePath), getScene(),
+//                                   getMainCamera(), getSettings()
 //
 // The facade owns C handles through RAII wrappers and turns error statuses
 // into narval_gpu::Error exceptions on the C++ side only (the C ABI itself
@@ -39,6 +42,7 @@ struct SceneSettings {
   int width = 512, height = 512;
   int spp = 1;
   int bounces = 1;
+  bool hdr = false;
 };
 // Camera(lookFrom, lookAt, up, vfov, aspect) (src/core/Camera.cpp:7-26)
 struct CameraSpec {
@@ -69,7 +73,62 @@ inline void check(const ne_ctx* ctx, ne_status s) {
 struct CtxDel { void operator()(ne_ctx* c) const { ne_ctx_destroy(c); } };
 struct SceneDel { void operator()(ne_scene* s) const { ne_scene_release(s); } };
 struct FilmDel { void operator()(ne_film* f) const { ne_film_release(f); } };
+struct FileDel { void operator()(ne_scene_file* f) const { ne_scene_file_release(f); } };
 }  // namespace detail
+
+// SceneReader over the library's scene-file reader.  A relative filePath, like
+// every asset path inside the file, is taken relative to resourcesDir (the
+// reference's RESOURCES_DIR); getScene() is the flattened descriptor that
+// Scene uploads, valid while the reader lives.
+class SceneReader {
+ public:
+  explicit SceneReader(std::string resourcesDir = "", int flags = 0)
+      : dir_(std::move(resourcesDir)), flags_(flags) {}
+  SceneReader(const std::string& filePath, bool absolutePath, std::string resourcesDir = "", int flags = 0)
+      : dir_(std::move(resourcesDir)), flags_(flags) {
+    loadScene(filePath, absolutePath);
+  }
+  void loadScene(const std::string& filePath, bool absolutePath = false) {
+    std::string path = filePath;
+    if (!absolutePath && !dir_.empty() && !filePath.empty() && filePath[0] != '/')
+      path = dir_ + (dir_.back() == '/' ? "" : "/") + filePath;
+    ne_load_options o{};
+    o.flags = flags_;
+    o.resources_dir = dir_.empty() ? nullptr : dir_.c_str();
+    ne_scene_file* f = nullptr;
+    detail::check(nullptr, ne_scene_file_read(path.c_str(), &o, &f));
+    file_.reset(f, detail::FileDel());
+    detail::check(nullptr, ne_scene_file_get(f, &desc_, &cam_, &set_));
+  }
+  const ne_scene_desc& getScene() const { return desc_; }
+  CameraSpec getMainCamera() const {
+    CameraSpec c;
+    c.position = {cam_.position[0], cam_.position[1], cam_.position[2]};
+    c.look_at = {cam_.look_at[0], cam_.look_at[1], cam_.look_at[2]};
+    c.up = {cam_.up[0], cam_.up[1], cam_.up[2]};
+    c.vfov = cam_.vfov;
+    c.aspect = cam_.aspect;
+    return c;
+  }
+  SceneSettings getSettings() const {
+    SceneSettings s;
+    s.width = set_.width;
+    s.height = set_.height;
+    s.spp = set_.spp;
+    s.bounces = set_.bounces;
+    s.hdr = set_.hdr != 0;
+    return s;
+  }
+  std::string warnings() const { return file_ ? ne_scene_file_warnings(file_.get()) : ""; }
+
+ private:
+  std::string dir_;
+  int flags_;
+  std::shared_ptr<ne_scene_file> file_;
+  ne_scene_desc desc_{};
+  ne_camera_desc cam_{};
+  ne_scene_settings set_{};
+};
 
 // One device context (one HIP stream; calls on it are serialised).
 class Context {

@@ -11,7 +11,7 @@ NE_FLAG_STAGE_TIME = 4
 STAGES = ("megakernel", "plan", "regen", "trace", "medium", "shade", "shadow", "other", "guide", "train")
 NE_MODE_AUTO, NE_MODE_MEGAKERNEL, NE_MODE_WAVEFRONT = 0, 1, 2
 NE_MAT_GLOSSY, NE_MAT_EMITTER = 0, 1
-NE_ENV_NONE, NE_ENV_CONSTANT, NE_ENV_SKY = 0, 1, 2
+NE_ENV_NONE, NE_ENV_CONSTANT, NE_ENV_SKY, NE_ENV_MAP = 0, 1, 2, 3
 NE_TEST_AABB, NE_TEST_TRIANGLE, NE_TEST_SPHERE, NE_TEST_RECTANGLE = 0, 1, 2, 3
 NE_TEST_HG_EVAL, NE_TEST_HG_SAMPLE, NE_TEST_GGX_D, NE_TEST_GGX_PDF, NE_TEST_GLOSSY, NE_TEST_HG_FWD = range(6)
 
@@ -36,6 +36,7 @@ class SceneDesc(C.Structure):
         ("has_sun", C.c_int32), ("sun_dir", C.c_float * 3), ("sun_E", C.c_float * 3),
         ("env_type", C.c_int32), ("env_c0", C.c_float * 3), ("env_c1", C.c_float * 3), ("env_c2", C.c_float * 3),
         ("vol_homogeneous", C.c_int32),
+        ("env_map", _F), ("env_w", C.c_int32), ("env_h", C.c_int32),
     ]
 
 
@@ -111,6 +112,7 @@ class LoadOptions(C.Structure):
 
 
 NE_LOAD_LENIENT = 1
+NE_IMAGE_PPM16, NE_IMAGE_PNG, NE_IMAGE_EXR, NE_IMAGE_PFM = 0, 1, 2, 3
 
 SIGNATURES = [
     ("ne_ctx_create", C.c_int32, [C.c_int32, C.POINTER(C.c_void_p)]),
@@ -169,6 +171,7 @@ SIGNATURES = [
     ("ne_scene_file_warnings", C.c_char_p, [C.c_void_p]),
     ("ne_scene_file_release", None, [C.c_void_p]),
     ("ne_vol_read", C.c_int32, [C.c_char_p, C.POINTER(C.c_int32), _F, C.c_uint64]),
+    ("ne_image_write", C.c_int32, [C.c_char_p, C.c_int32, C.c_int32, C.c_int32, _F]),
 ]
 
 _LIB = None

@@ -124,6 +124,48 @@ inline float rect_area(const float* eu, const float* ev) {
   return std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
 }
 
+// Sampling tables of an environment map: func = luminance * sin(theta of the
+// row centre); per-row CDFs over the row's W texels and the marginal CDF over
+// rows, summed in FP64 and stored in FP32 with exact 0 and 1 ends (uniform
+// where a row or the whole map is black).  inv_int = W H / sum(func).
+struct EnvTables {
+  std::vector<float> func, cdf, marg;
+  float inv_int = 0.0f;
+};
+void build_env_tables(const float* rgb, int W, int H, EnvTables& t) {
+  t.func.assign(size_t(W) * H, 0.0f);
+  t.cdf.assign(size_t(H) * (W + 1), 0.0f);
+  t.marg.assign(size_t(H) + 1, 0.0f);
+  std::vector<double> row(static_cast<size_t>(H)), pre(static_cast<size_t>(W) + 1);
+  double total = 0.0;
+  for (int j = 0; j < H; ++j) {
+    const double st = std::sin(kPI * (double(j) + 0.5) / double(H));
+    double acc = 0.0;
+    pre[0] = 0.0;
+    for (int i = 0; i < W; ++i) {
+      const float* p = rgb + 3 * (size_t(j) * W + i);
+      const float f = float((0.2126 * p[0] + 0.7152 * p[1] + 0.0722 * p[2]) * st);
+      t.func[size_t(j) * W + i] = f;
+      acc += double(f);
+      pre[size_t(i) + 1] = acc;
+    }
+    float* c = &t.cdf[size_t(j) * (W + 1)];
+    for (int i = 1; i < W; ++i) c[i] = acc > 0 ? float(pre[size_t(i)] / acc) : float(double(i) / W);
+    c[0] = 0.0f;
+    c[W] = 1.0f;
+    row[size_t(j)] = acc;
+    total += acc;
+  }
+  double acc = 0.0;
+  for (int j = 1; j < H; ++j) {
+    acc += row[size_t(j) - 1];
+    t.marg[size_t(j)] = total > 0 ? float(acc / total) : float(double(j) / H);
+  }
+  t.marg[0] = 0.0f;
+  t.marg[size_t(H)] = 1.0f;
+  t.inv_int = total > 0 ? float(double(W) * double(H) / total) : 0.0f;
+}
+
 ne_status validate_desc(ne_ctx* c, const ne_scene_desc* d) {
   if (!d) return fail(c, NE_ERR_INVALID_ARG, "scene descriptor is NULL");
   if (d->n_tri && !d->tri_v) return fail(c, NE_ERR_INVALID_ARG, "tri_v is NULL");
@@ -171,7 +213,16 @@ ne_status validate_desc(ne_ctx* c, const ne_scene_desc* d) {
       if (!(d->grid[i] >= 0.0f) || !std::isfinite(d->grid[i]))
         return fail(c, NE_ERR_INVALID_ARG, "grid densities must be finite and >= 0");
   }
-  if (d->env_type < 0 || d->env_type > 2) return fail(c, NE_ERR_INVALID_ARG, "unknown environment type");
+  if (d->env_type < 0 || d->env_type > 3) return fail(c, NE_ERR_INVALID_ARG, "unknown environment type");
+  if (d->env_type == NE_ENV_MAP) {
+    if (!d->env_map) return fail(c, NE_ERR_INVALID_ARG, "environment map is NULL");
+    if (d->env_w < 1 || d->env_h < 1 || d->env_w > 16384 || d->env_h > 16384)
+      return fail(c, NE_ERR_INVALID_ARG, "environment map size out of [1, 16384]");
+    const size_t n = size_t(d->env_w) * size_t(d->env_h) * 3;
+    for (size_t i = 0; i < n; ++i)
+      if (!(d->env_map[i] >= 0.0f) || !std::isfinite(d->env_map[i]))
+        return fail(c, NE_ERR_INVALID_ARG, "environment map values must be finite and >= 0");
+  }
   if (d->has_sun && !(len3(d->sun_dir) > 0)) return fail(c, NE_ERR_INVALID_ARG, "sun direction is zero");
   return NE_OK;
 }
@@ -488,6 +539,17 @@ ne_status ne_scene_upload(ne_ctx* c, const ne_scene_desc* d, ne_scene** out) {
   for (int a = 0; a < 3; ++a) { ds.sun_dir[a] = d->sun_dir[a]; ds.sun_E[a] = d->sun_E[a]; }
   ds.env_type = d->env_type;
   for (int a = 0; a < 3; ++a) { ds.env[a] = d->env_c0[a]; ds.env[3 + a] = d->env_c1[a]; ds.env[6 + a] = d->env_c2[a]; }
+  EnvTables et;  // outlives the asynchronous uploads below (synchronised before returning)
+  if (d->env_type == NE_ENV_MAP) {
+    build_env_tables(d->env_map, d->env_w, d->env_h, et);
+    ds.env_w = d->env_w;
+    ds.env_h = d->env_h;
+    ds.env_inv_int = et.inv_int;
+    if ((st = dev_upload(c, s, d->env_map, size_t(d->env_w) * d->env_h * 3, &ds.env_tex)) != NE_OK) return bail(st);
+    if ((st = dev_upload(c, s, et.func.data(), et.func.size(), &ds.env_func)) != NE_OK) return bail(st);
+    if ((st = dev_upload(c, s, et.cdf.data(), et.cdf.size(), &ds.env_cdf)) != NE_OK) return bail(st);
+    if ((st = dev_upload(c, s, et.marg.data(), et.marg.size(), &ds.env_marg)) != NE_OK) return bail(st);
+  }
   ds.n_lights = ds.has_sun + (ds.env_type != NE_ENV_NONE ? 1 : 0) + ds.n_emitters;
   NE_HIP(c, hipStreamSynchronize(c->stream));
   s->info.bvh_nodes = ds.n_nodes;

@@ -26,6 +26,10 @@ NE_HD float xsqrt(float x) { return sqrtf(x); }
 NE_HD double xsqrt(double x) { return sqrt(x); }
 NE_HD float xlog(float x) { return logf(x); }
 NE_HD double xlog(double x) { return log(x); }
+NE_HD float xatan2(float y, float x) { return atan2f(y, x); }
+NE_HD double xatan2(double y, double x) { return atan2(y, x); }
+NE_HD float xacos(float x) { return acosf(x); }
+NE_HD double xacos(double x) { return acos(x); }
 NE_HD float xexp(float x) { return expf(x); }
 NE_HD double xexp(double x) { return exp(x); }
 NE_HD float xsin(float x) { return sinf(x); }

@@ -67,6 +67,14 @@ struct DevScene {
   float sun_E[3];
   int32_t env_type;
   float env[9];
+  // environment map (env_type 3): RGB texels, per-texel sampling function,
+  // per-row conditional CDFs (W + 1 each), marginal CDF (H + 1)
+  const float* env_tex;
+  const float* env_func;
+  const float* env_cdf;
+  const float* env_marg;
+  int32_t env_w, env_h;
+  float env_inv_int;  // 1 / (mean of env_func)
   int32_t n_lights;
 };
 

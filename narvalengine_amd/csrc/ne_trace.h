@@ -606,8 +606,54 @@ struct Glossy {
 // ---------------------------------------------------------------------------
 // Lights
 // ---------------------------------------------------------------------------
+// Environment map (InfiniteAreaLight, src/lights/InfiniteAreaLight.h:8-130):
+// latitude-longitude RGB texels looked up as InfiniteAreaLight::Le does, in
+// the frame generateOrthonormalCS builds around (0,0,1): local
+// (x, y, z) = (d.y, -d.x, d.z), u = phi / 2 pi with phi = atan2(y, x) in
+// [0, 2 pi), v = theta / pi with theta = acos(z); row 0 is theta = 0.
+// Texels are piecewise constant.  Sampling inverts a 2-D piecewise-constant
+// distribution over luminance * sin(theta) (Distribution2D,
+// src/utils/Sampling.h:76-112, with each row's conditional over the row's W
+// texels) and uses the same frame as the lookup, so that the solid-angle pdf
+// p(u, v) / (2 pi^2 sin theta) belongs to the radiance it weights.
+template <typename R>
+NE_DEV int env_texel(const DevScene& s, Vec3<R> d, R& sin_t) {
+  R phi = xatan2(-d.x, d.y);
+  phi = phi < R(0) ? phi + R(kTWO_PI) : phi;
+  const R z = xmin(R(1), xmax(R(-1), d.z));
+  sin_t = xsqrt(xmax(R(0), R(1) - z * z));
+  const R theta = xacos(z);
+  int i = int(phi * R(1.0 / kTWO_PI) * R(s.env_w));
+  int j = int(theta * R(1.0 / kPI) * R(s.env_h));
+  i = i < s.env_w - 1 ? i : s.env_w - 1;
+  j = j < s.env_h - 1 ? j : s.env_h - 1;
+  return j * s.env_w + i;
+}
+// Largest k in [0, n) with cdf[k] <= u (cdf[0] = 0, cdf[n] = 1, u in [0, 1)).
+NE_DEV int cdf_find(const float* cdf, int n, float u) {
+  int lo = 0, hi = n;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (cdf[mid] <= u) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+template <typename R>
+NE_DEV R env_map_pdf(const DevScene& s, Vec3<R> d) {
+  R st;
+  const int k = env_texel<R>(s, d, st);
+  if (!(st > R(0))) return R(0);
+  return R(s.env_func[k]) * R(s.env_inv_int) / (R(2.0 * kPI * kPI) * st);
+}
+
 template <typename R>
 NE_DEV Vec3<R> env_le(const DevScene& s, Vec3<R> d) {
+  if (s.env_type == 3) {
+    R st;
+    const float* t = s.env_tex + 3 * size_t(env_texel<R>(s, d, st));
+    return v3(R(t[0]), R(t[1]), R(t[2]));
+  }
   if (s.env_type == 1) return v3(R(s.env[0]), R(s.env[1]), R(s.env[2]));
   if (s.env_type == 2) {
     if (d.y >= R(0))
@@ -615,6 +661,40 @@ NE_DEV Vec3<R> env_le(const DevScene& s, Vec3<R> d) {
     return v3(R(s.env[6]), R(s.env[7]), R(s.env[8]));
   }
   return v3s(R(0));
+}
+// Solid-angle pdf of env_sample for direction d.
+template <typename R>
+NE_DEV R env_pdf(const DevScene& s, Vec3<R> d) {
+  return s.env_type == 3 ? env_map_pdf<R>(s, d) : R(1.0 / kFOUR_PI);
+}
+// A direction toward the environment: uniform on the sphere for the constant
+// and sky environments, importance-sampled for a map.
+template <typename R>
+NE_DEV void env_sample(const DevScene& s, R ua, R ub, Vec3<R>& wi, R& pl, Vec3<R>& Le) {
+  if (s.env_type != 3) {
+    R z = R(1) - R(2) * ua, r = xsqrt(xmax(R(0), R(1) - z * z)), ph = R(kTWO_PI) * ub;
+    wi = v3(r * xcos(ph), r * xsin(ph), z);
+    Le = env_le<R>(s, wi);
+    pl = R(1.0 / kFOUR_PI);
+    return;
+  }
+  const int W = s.env_w, H = s.env_h;
+  const float fv = float(ub), fu = float(ua);
+  const int j = cdf_find(s.env_marg, H, fv);
+  const float m0 = s.env_marg[j], m1 = s.env_marg[j + 1];
+  const float dv = m1 > m0 ? (fv - m0) / (m1 - m0) : 0.0f;
+  const float* row = s.env_cdf + size_t(j) * size_t(W + 1);
+  const int i = cdf_find(row, W, fu);
+  const float c0 = row[i], c1 = row[i + 1];
+  const float du = c1 > c0 ? (fu - c0) / (c1 - c0) : 0.0f;
+  const R u = (R(i) + R(du)) / R(W), v = (R(j) + R(dv)) / R(H);
+  const R theta = v * R(kPI), phi = u * R(kTWO_PI);
+  const R st = xsin(theta), ct = xcos(theta);
+  wi = v3(-st * xsin(phi), st * xcos(phi), ct);
+  const size_t k = size_t(j) * size_t(W) + size_t(i);
+  pl = st > R(0) ? R(s.env_func[k]) * R(s.env_inv_int) / (R(2.0 * kPI * kPI) * st) : R(0);
+  const float* t = s.env_tex + 3 * k;
+  Le = v3(R(t[0]), R(t[1]), R(t[2]));
 }
 
 // Emissive primitives: rectangles, spheres and triangles with an emitter
@@ -800,7 +880,7 @@ NE_DEV bool path_bounce(const DevScene& s, PathState<R>& p, int max_bounces, Pat
     if (!medium) {
       if (!hs) {
         if (s.env_type != 0) {
-          R w = prev_delta ? R(1) : power_heuristic(prev_pdf, R(1.0 / kFOUR_PI));
+          R w = prev_delta ? R(1) : power_heuristic(prev_pdf, env_pdf<R>(s, d));
           L = L + beta * env_le<R>(s, d) * w;
         }
         p.L = L;
@@ -851,10 +931,7 @@ NE_DEV bool path_bounce(const DevScene& s, PathState<R>& p, int max_bounces, Pat
         pl = R(1);
         delta = true;
       } else if (s.env_type != 0 && li == env_slot) {
-        R z = R(1) - R(2) * ua, r = xsqrt(xmax(R(0), R(1) - z * z)), ph = R(kTWO_PI) * ub;
-        wi = v3(r * xcos(ph), r * xsin(ph), z);
-        Le = env_le<R>(s, wi);
-        pl = R(1.0 / kFOUR_PI);
+        env_sample<R>(s, ua, ub, wi, pl, Le);
       } else {
         const uint32_t code = s.emitters[li - env_slot - (s.env_type != 0 ? 1 : 0)];
         sample_emitter<R>(s, code, x, ua, ub, wi, dist, pl, Le);

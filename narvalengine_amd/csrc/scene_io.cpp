@@ -8,10 +8,13 @@
 //     reader for large grids;
 //   * a Wavefront OBJ reader for positions and faces (the reference imports
 //     models through Assimp, src/primitives/Model.cpp:49-91; the path uses
-//     only the triangles).
+//     only the triangles);
+//   * Radiance .hdr and .pfm readers for InfiniteAreaLight maps (the
+//     reference decodes images with stb_image).
 // Failures are thrown internally as Failure and turned into an ne_status at
 // the ABI; nothing throws across it.
 #include <algorithm>
+#include <cctype>
 #include <cerrno>
 #include <cmath>
 #include <cstdio>
@@ -323,6 +326,125 @@ void read_f32(const std::string& path, const int32_t res[3], std::vector<float>&
 }
 
 // ---------------------------------------------------------------------------
+// Environment images (linear RGB float, row 0 at the top): Portable FloatMap
+// (.pfm, "PF"/"Pf", rows stored bottom to top, negative scale = little
+// endian) and Radiance RGBE (.hdr, flat or new-style run-length scanlines,
+// value = mantissa * 2^(exponent - 136) as stb_image decodes it).
+// ---------------------------------------------------------------------------
+struct Image {
+  int w = 0, h = 0;
+  std::vector<float> rgb;
+};
+
+void read_pfm(const std::string& path, Image& img) {
+  const std::string b = read_file(path);
+  size_t p = 0;
+  auto token = [&]() {
+    while (p < b.size() && std::isspace(static_cast<unsigned char>(b[p]))) ++p;
+    size_t s = p;
+    while (p < b.size() && !std::isspace(static_cast<unsigned char>(b[p]))) ++p;
+    return b.substr(s, p - s);
+  };
+  const std::string magic = token();
+  if (magic != "PF" && magic != "Pf") raise(NE_ERR_PARSE, path + ": not a PFM file");
+  const int ch = magic == "PF" ? 3 : 1;
+  char* end = nullptr;
+  const std::string ws = token(), hs = token(), ss = token();
+  const long w = std::strtol(ws.c_str(), &end, 10);
+  const long h = std::strtol(hs.c_str(), &end, 10);
+  const double scale = std::strtod(ss.c_str(), &end);
+  if (w < 1 || h < 1 || w > 16384 || h > 16384 || scale == 0.0) raise(NE_ERR_PARSE, path + ": bad PFM header");
+  ++p;  // the single whitespace character after the scale
+  const size_t n = size_t(w) * size_t(h) * size_t(ch);
+  if (b.size() < p + 4 * n) raise(NE_ERR_PARSE, path + ": truncated PFM data");
+  const bool little = scale < 0;
+  img.w = int(w);
+  img.h = int(h);
+  img.rgb.assign(size_t(w) * size_t(h) * 3, 0.0f);
+  for (long y = 0; y < h; ++y)
+    for (long x = 0; x < w; ++x)
+      for (int c = 0; c < 3; ++c) {
+        const size_t src = p + 4 * ((size_t(y) * size_t(w) + size_t(x)) * size_t(ch) + size_t(ch == 3 ? c : 0));
+        unsigned char q[4];
+        std::memcpy(q, b.data() + src, 4);
+        if (!little) { std::swap(q[0], q[3]); std::swap(q[1], q[2]); }
+        float v;
+        std::memcpy(&v, q, 4);
+        img.rgb[((size_t(h - 1 - y)) * size_t(w) + size_t(x)) * 3 + size_t(c)] = v;  // bottom-up -> top-down
+      }
+}
+
+void read_hdr(const std::string& path, Image& img) {
+  const std::string b = read_file(path);
+  size_t p = 0;
+  auto line = [&]() {
+    size_t e = b.find('\n', p);
+    if (e == std::string::npos) raise(NE_ERR_PARSE, path + ": truncated Radiance header");
+    std::string l = b.substr(p, e - p);
+    p = e + 1;
+    return l;
+  };
+  const std::string first = line();
+  if (first.rfind("#?RADIANCE", 0) != 0 && first.rfind("#?RGBE", 0) != 0)
+    raise(NE_ERR_PARSE, path + ": not a Radiance HDR file");
+  while (true) {
+    const std::string l = line();
+    if (l.empty()) break;
+    if (l.rfind("FORMAT=", 0) == 0 && l != "FORMAT=32-bit_rle_rgbe")
+      raise(NE_ERR_UNSUPPORTED, path + ": only the 32-bit_rle_rgbe format is supported");
+  }
+  const std::string res = line();
+  int h = 0, w = 0;
+  char sy[3] = {0}, sx[3] = {0};
+  if (std::sscanf(res.c_str(), "%2s %d %2s %d", sy, &h, sx, &w) != 4 || std::string(sy) != "-Y" ||
+      std::string(sx) != "+X")
+    raise(NE_ERR_UNSUPPORTED, path + ": only the standard -Y H +X W orientation is supported");
+  if (w < 1 || h < 1 || w > 16384 || h > 16384) raise(NE_ERR_PARSE, path + ": bad image size");
+  img.w = w;
+  img.h = h;
+  img.rgb.assign(size_t(w) * size_t(h) * 3, 0.0f);
+  std::vector<unsigned char> scan(size_t(w) * 4);
+  auto byte = [&]() -> unsigned char {
+    if (p >= b.size()) raise(NE_ERR_PARSE, path + ": truncated pixel data");
+    return static_cast<unsigned char>(b[p++]);
+  };
+  for (int y = 0; y < h; ++y) {
+    const bool rle = w >= 8 && w < 32768 && p + 4 <= b.size() && static_cast<unsigned char>(b[p]) == 2 &&
+                     static_cast<unsigned char>(b[p + 1]) == 2 && ((static_cast<unsigned char>(b[p + 2]) << 8) |
+                                                                   static_cast<unsigned char>(b[p + 3])) == w;
+    if (rle) {
+      p += 4;
+      for (int c = 0; c < 4; ++c) {
+        int x = 0;
+        while (x < w) {
+          int count = byte();
+          if (count > 128) {
+            count -= 128;
+            if (x + count > w) raise(NE_ERR_PARSE, path + ": run past the end of a scanline");
+            const unsigned char v = byte();
+            for (int k = 0; k < count; ++k) scan[size_t(x++) * 4 + size_t(c)] = v;
+          } else {
+            if (count == 0 || x + count > w) raise(NE_ERR_PARSE, path + ": bad literal run");
+            for (int k = 0; k < count; ++k) scan[size_t(x++) * 4 + size_t(c)] = byte();
+          }
+        }
+      }
+    } else {
+      for (size_t k = 0; k < scan.size(); ++k) scan[k] = byte();
+    }
+    for (int x = 0; x < w; ++x) {
+      const unsigned char* q = &scan[size_t(x) * 4];
+      float* o = &img.rgb[(size_t(y) * size_t(w) + size_t(x)) * 3];
+      if (q[3] == 0) { o[0] = o[1] = o[2] = 0.0f; continue; }
+      const float f = std::ldexp(1.0f, int(q[3]) - 136);
+      o[0] = float(q[0]) * f;
+      o[1] = float(q[1]) * f;
+      o[2] = float(q[2]) * f;
+    }
+  }
+}
+
+// ---------------------------------------------------------------------------
 // Instance transforms: translate * eulerAngleXYZ(rotation in degrees) * scale
 // (getTransform, src/utils/Math.h:852-859), row-major, FP64.
 // ---------------------------------------------------------------------------
@@ -449,12 +571,13 @@ struct MaterialDef {
   bool grid_unavailable = false;  // lenient: a .vdb grid
   int32_t res[3] = {0, 0, 0};
   std::vector<float> grid;
+  Image env;  // kEnv: the radiance map
 };
 
 }  // namespace
 
 struct ne_scene_file {
-  std::vector<float> tri_v, sph, rect, mat, grid;
+  std::vector<float> tri_v, sph, rect, mat, grid, env;
   std::vector<int32_t> tri_mat, sph_mat, rect_mat;
   ne_scene_desc desc{};
   ne_camera_desc camera{};
@@ -588,8 +711,12 @@ class Loader {
         d.sun_E[a] = float(le[a]);
       }
     } else if (type == "infiniteAreaLight") {
-      text(m, "path", ctx);
+      const std::string rel = text(m, "path", ctx);
+      const std::string path = join_path(base_, rel);
       d.kind = MaterialDef::kEnv;
+      if (ends_with(rel, ".hdr")) read_hdr(path, d.env);
+      else if (ends_with(rel, ".pfm")) read_pfm(path, d.env);
+      else unsupported(ctx + ": environment images must be Radiance .hdr or .pfm ('" + rel + "')");
     } else if (type == "volume") {
       double ss[3], sa[3];
       vec3(m, "scattering", ss, ctx);
@@ -693,7 +820,11 @@ class Loader {
         f_.desc.has_sun = 1;
         for (int a = 0; a < 3; ++a) { f_.desc.sun_dir[a] = d.sun_dir[a]; f_.desc.sun_E[a] = d.sun_E[a]; }
       } else if (d.kind == MaterialDef::kEnv) {
-        unsupported(ctx + ": image-based environment lights are not supported");
+        if (f_.desc.env_type != NE_ENV_NONE) unsupported(ctx + ": one environment light per scene");
+        f_.desc.env_type = NE_ENV_MAP;
+        f_.env = d.env.rgb;
+        f_.desc.env_w = d.env.w;
+        f_.desc.env_h = d.env.h;
       } else if (d.kind == MaterialDef::kSurface && d.emitter) {
         unsupported(ctx + ": point lights are not supported");
       }
@@ -800,7 +931,6 @@ class Loader {
         d.maj_res = 1;
       }
     }
-    d.env_type = NE_ENV_NONE;
   }
 };
 
@@ -818,6 +948,7 @@ void point_desc(ne_scene_file* f) {
   d.mat = f->mat.empty() ? nullptr : f->mat.data();
   d.n_mat = uint32_t(f->mat.size() / 12);
   d.grid = f->grid.empty() ? nullptr : f->grid.data();
+  d.env_map = f->env.empty() ? nullptr : f->env.data();
 }
 
 ne_status parse_into(const char* json, size_t len, const ne_load_options* opt, const std::string& default_base,

@@ -362,10 +362,7 @@ __device__ __forceinline__ bool nee_sample(const DevScene& s, Vec3<float> x, Vec
     pl = 1.0f;
     delta = true;
   } else if (s.env_type != 0 && li == env_slot) {
-    const float z = 1.0f - 2.0f * ua, r = sqrtf(xmax(0.0f, 1.0f - z * z)), ph = float(kTWO_PI) * ub;
-    wi = v3(r * cosf(ph), r * sinf(ph), z);
-    Le = env_le<float>(s, wi);
-    pl = float(1.0 / kFOUR_PI);
+    env_sample<float>(s, ua, ub, wi, pl, Le);
   } else {
     const uint32_t code = s.emitters[li - env_slot - (s.env_type != 0 ? 1 : 0)];
     sample_emitter<float>(s, code, x, ua, ub, wi, dist, pl, Le);
@@ -490,7 +487,7 @@ __global__ __launch_bounds__(kChunk) void k_wf_shade(WfState w, WfArgs a, DevSce
     const float th = A.hit_t[i];
     if (kind == WF_KIND_MISS) {
       if (s.env_type != 0) {
-        const float wm = prev_delta ? 1.0f : power_heuristic(A.prev_pdf[i], float(1.0 / kFOUR_PI));
+        const float wm = prev_delta ? 1.0f : power_heuristic(A.prev_pdf[i], env_pdf<float>(s, d));
         const Vec3<float> E = env_le<float>(s, d);
         lx += beta.x * E.x * wm; ly += beta.y * E.y * wm; lz += beta.z * E.z * wm;
       }

@@ -11,12 +11,14 @@
 """
 from __future__ import annotations
 
+import ctypes as C
+import os
 import threading
 from dataclasses import dataclass
 
 import numpy as np
 
-from .abi import NE_FP32
+from .abi import NE_FP32, NE_IMAGE_PPM16, check, lib
 from .runtime import Context
 
 
@@ -64,14 +66,19 @@ class OfflineEngine:
         write_ppm16(path, self.pixels)
 
 
-def write_ppm16(path: str, pixels: np.ndarray):
-    """16-bit binary PPM as OfflineEngine::coreLoop writes it (OfflineEngine.cpp:119-137):
-    header 'P6\\nW H\\n65535\\n', big-endian channels, pixels in reverse index order."""
-    H, W, _ = pixels.shape
-    v = (np.clip(pixels, 0, 1) * 65535).astype(np.uint16).reshape(-1, 3)[::-1]
-    with open(path, "wb") as f:
-        f.write(f"P6\n{W} {H}\n65535\n".encode())
-        f.write(v.astype(">u2").tobytes())
+def write_image(path, pixels: np.ndarray, fmt: int = NE_IMAGE_PPM16) -> None:
+    """Writes an (H, W, 3) float image with the library's writers
+    (ne_image_write): NE_IMAGE_PPM16 is OfflineEngine::coreLoop's output.ppm
+    (OfflineEngine.cpp:80-137), NE_IMAGE_PNG / NE_IMAGE_EXR follow saveImage
+    (src/materials/Texture.h:44-69), NE_IMAGE_PFM keeps linear floats."""
+    img = np.ascontiguousarray(pixels, np.float32)
+    H, W, _ = img.shape
+    check(lib().ne_image_write(os.fsencode(str(path)), int(fmt), W, H, img.ctypes.data_as(C.POINTER(C.c_float))))
+
+
+def write_ppm16(path, pixels: np.ndarray) -> None:
+    """16-bit binary PPM as OfflineEngine::coreLoop writes it."""
+    write_image(path, pixels, NE_IMAGE_PPM16)
 
 
 class GpuPathIntegrator:

@@ -21,7 +21,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-from .abi import CameraDesc, SceneDesc, NE_MAT_GLOSSY, NE_MAT_EMITTER, NE_ENV_NONE
+from .abi import CameraDesc, SceneDesc, NE_MAT_GLOSSY, NE_MAT_EMITTER, NE_ENV_NONE, NE_ENV_MAP
 
 
 def rot_xyz(deg) -> np.ndarray:
@@ -142,6 +142,13 @@ class Scene:
     def set_env(self, kind: int, c0=(0, 0, 0), c1=(0, 0, 0), c2=(0, 0, 0)):
         self.env = dict(kind=int(kind), c0=tuple(c0), c1=tuple(c1), c2=tuple(c2))
 
+    def set_env_map(self, rgb: np.ndarray):
+        """InfiniteAreaLight (src/lights/InfiniteAreaLight.h:8-130): an (H, W, 3)
+        latitude-longitude radiance map, row 0 at theta = 0 of its z-up frame."""
+        rgb = np.ascontiguousarray(rgb, dtype=np.float32)
+        assert rgb.ndim == 3 and rgb.shape[2] == 3, "env map must be (H, W, 3)"
+        self.env = dict(kind=NE_ENV_MAP, c0=(0, 0, 0), c1=(0, 0, 0), c2=(0, 0, 0), map=rgb)
+
     # ---- flattening
     def arrays(self) -> dict:
         cat = lambda xs, w, dt: (np.ascontiguousarray(np.concatenate(xs).reshape(-1, w), dt) if xs
@@ -155,6 +162,8 @@ class Scene:
         )
         if self.volume is not None and self.volume["grid"] is not None:
             a["grid"] = self.volume["grid"]
+        if self.env is not None and "map" in self.env:
+            a["env_map"] = self.env["map"]
         return a
 
     def desc(self) -> tuple[SceneDesc, dict]:
@@ -190,6 +199,9 @@ class Scene:
             d.env_c0[:] = self.env["c0"]
             d.env_c1[:] = self.env["c1"]
             d.env_c2[:] = self.env["c2"]
+            if "map" in self.env:
+                d.env_map = _ptr(a["env_map"], C.c_float)
+                d.env_h, d.env_w = self.env["map"].shape[:2]
         else:
             d.env_type = NE_ENV_NONE
         return d, a

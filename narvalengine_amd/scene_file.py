@@ -10,7 +10,8 @@
     RESOURCES_DIR).
   * read_vol    -- the text .vol grid reader (ResourceManager::loadVolasTexture,
     src/core/ResourceManager.cpp:222-286) as a (D, H, W) float32 array.
-  * write_vol / write_f32 -- writers of both grid formats the reader takes.
+  * write_vol / write_f32 / write_pfm / write_hdr -- writers of the grid and
+    environment-image formats the reader takes.
 """
 from __future__ import annotations
 
@@ -69,7 +70,8 @@ class SceneFile:
         return self._desc, self
 
     def arrays(self) -> dict:
-        """Copies of the flattened arrays (tri_v, tri_mat, sph, sph_mat, rect, rect_mat, mat, grid)."""
+        """Copies of the flattened arrays (tri_v, tri_mat, sph, sph_mat, rect, rect_mat, mat, grid,
+        env_map)."""
         d = self._desc
 
         def arr(ptr, n, w, dt):
@@ -85,6 +87,8 @@ class SceneFile:
         if d.has_volume and not d.vol_homogeneous:
             W, H, D = d.vol_res
             out["grid"] = arr(d.grid, W * H * D, 1, np.float32).reshape(D, H, W)
+        if d.env_type == 3:
+            out["env_map"] = arr(d.env_map, d.env_w * d.env_h, 3, np.float32).reshape(d.env_h, d.env_w, 3)
         return out
 
     def close(self):
@@ -151,6 +155,67 @@ def write_vol(path, grid: np.ndarray, comment: str = "") -> None:
         for k in range(0, flat.size, 4096):
             f.write("".join(f"{float(v)!r} " for v in flat[k:k + 4096]))
         f.write("\n")
+
+
+def write_pfm(path, rgb: np.ndarray, little_endian: bool = True) -> None:
+    """Portable FloatMap of an (H, W, 3) image whose row 0 is the top (PFM
+    stores rows bottom to top)."""
+    img = np.ascontiguousarray(rgb, np.float32)
+    H, W, _ = img.shape
+    with open(path, "wb") as f:
+        f.write(f"PF\n{W} {H}\n{-1.0 if little_endian else 1.0}\n".encode())
+        f.write(img[::-1].astype("<f4" if little_endian else ">f4").tobytes())
+
+
+def _rgbe(img):
+    m = img.max(axis=2)
+    e = np.zeros(m.shape, np.int32)
+    mant, ex = np.frexp(m)
+    ok = m > 1e-32
+    e[ok] = ex[ok]
+    scale = np.where(ok, 256.0 / np.ldexp(1.0, e), 0.0)
+    q = np.zeros(img.shape[:2] + (4,), np.uint8)
+    q[..., :3] = np.clip(np.floor(img * scale[..., None]), 0, 255).astype(np.uint8)
+    q[..., 3] = np.where(ok, e + 128, 0).astype(np.uint8)
+    return q
+
+
+def write_hdr(path, rgb: np.ndarray, rle: bool = True) -> None:
+    """Radiance RGBE image of an (H, W, 3) array (row 0 at the top), with
+    new-style run-length scanlines when rle (and 8 <= W < 32768)."""
+    img = np.asarray(rgb, np.float64)
+    H, W, _ = img.shape
+    q = _rgbe(img)
+    out = bytearray(b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n" + f"-Y {H} +X {W}\n".encode())
+    for y in range(H):
+        if not (rle and 8 <= W < 32768):
+            out += q[y].tobytes()
+            continue
+        out += bytes([2, 2, W >> 8, W & 255])
+        for c in range(4):
+            ch = q[y, :, c]
+            x = 0
+            while x < W:
+                r = 1
+                while x + r < W and r < 127 and ch[x + r] == ch[x]:
+                    r += 1
+                if r >= 3:
+                    out += bytes([128 + r, int(ch[x])])
+                    x += r
+                    continue
+                s0 = x
+                while x < W and x - s0 < 128 and not (x + 2 < W and ch[x] == ch[x + 1] == ch[x + 2]):
+                    x += 1
+                out += bytes([x - s0]) + ch[s0:x].tobytes()
+    with open(path, "wb") as f:
+        f.write(bytes(out))
+
+
+def decode_rgbe(q: np.ndarray) -> np.ndarray:
+    """The radiance an RGBE quadruple stands for: mantissa * 2^(e - 136)."""
+    q = np.asarray(q)
+    f = np.where(q[..., 3] > 0, np.ldexp(1.0, q[..., 3].astype(np.int32) - 136), 0.0)
+    return (q[..., :3] * f[..., None]).astype(np.float32)
 
 
 def write_f32(path, grid: np.ndarray) -> tuple:

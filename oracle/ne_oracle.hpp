@@ -30,6 +30,7 @@
 #include <cstdint>
 #include <cstring>
 #include <limits>
+#include <memory>
 #include <thread>
 #include <vector>
 
@@ -437,6 +438,113 @@ inline bool ref_homogeneous_sample(V3f scattering, V3f absorption, float density
   return sampled;
 }
 
+// Distribution1D, src/utils/Sampling.h:5-73: a piecewise-constant function of
+// n values on [0, 1); cdf[i] = sum_{k<i} f[k] / n normalised by the integral
+// (uniform when it is 0); sampleContinuous finds the last cdf entry <= u
+// (binarySearch, src/utils/Math.h:1128-1145, clamped to [0, n - 1]).
+struct RefDistribution1D {
+  std::vector<float> func, cdf;
+  float funcInt = 0;
+  RefDistribution1D(const float* f, int n) : func(f, f + n), cdf(size_t(n) + 1) {
+    cdf[0] = 0;
+    for (int i = 1; i < n + 1; ++i) cdf[size_t(i)] = cdf[size_t(i) - 1] + func[size_t(i) - 1] / float(n);
+    funcInt = cdf[size_t(n)];
+    if (funcInt == 0)
+      for (int i = 1; i < n + 1; ++i) cdf[size_t(i)] = float(i) / float(n);
+    else
+      for (int i = 1; i < n + 1; ++i) cdf[size_t(i)] /= funcInt;
+  }
+  int count() const { return int(func.size()); }
+  float sampleContinuous(float u, float* pdf, int* off = nullptr) const {
+    int first = 0, len = int(cdf.size());  // binarySearch(size, pred)
+    while (len > 0) {
+      int half = len >> 1, middle = first + half;
+      if (cdf[size_t(middle)] <= u) { first = middle + 1; len -= half + 1; }
+      else len = half;
+    }
+    int offset = std::min(std::max(first - 1, 0), int(cdf.size()) - 2);
+    if (off) *off = offset;
+    float du = u - cdf[size_t(offset)];
+    if ((cdf[size_t(offset) + 1] - cdf[size_t(offset)]) > 0) du /= (cdf[size_t(offset) + 1] - cdf[size_t(offset)]);
+    if (pdf) *pdf = (funcInt > 0) ? func[size_t(offset)] / funcInt : 0;
+    return (float(offset) + du) / float(count());
+  }
+};
+// Distribution2D, src/utils/Sampling.h:76-112.  The reference builds each
+// row's conditional from `height` values (Sampling.h:87); cond_n selects
+// that count (the reference's height, or width for the corrected table).
+// With cond_n > width the reference reads past the row, so only
+// cond_n <= width is restated.
+struct RefDistribution2D {
+  std::vector<RefDistribution1D> cond;
+  std::unique_ptr<RefDistribution1D> marg;
+  RefDistribution2D(const float* f, int width, int height, int cond_n) {
+    cond.reserve(size_t(height));
+    for (int v = 0; v < height; ++v) cond.emplace_back(f + size_t(v) * size_t(width), cond_n);
+    std::vector<float> m;
+    for (int v = 0; v < height; ++v) m.push_back(cond[size_t(v)].funcInt);
+    marg.reset(new RefDistribution1D(m.data(), height));
+  }
+  void sampleContinuous(float u0, float u1, float* uv, float* pdf) const {
+    float p1, p0;
+    int v;
+    uv[1] = marg->sampleContinuous(u1, &p1, &v);
+    uv[0] = cond[size_t(v)].sampleContinuous(u0, &p0);
+    *pdf = p0 * p1;
+  }
+};
+
+// Sampling tables of an environment map (the corrected Distribution2D: each
+// row's conditional over the row's own W texels): func = luminance *
+// sin(theta of the row centre), FP64 sums, FP32 tables with exact 0 and 1
+// ends, uniform where a row or the map is black; inv_int = W H / sum(func).
+struct EnvTables {
+  std::vector<float> func, cdf, marg;
+  float inv_int = 0.0f;
+  void build(const float* rgb, int W, int H) {
+    func.assign(size_t(W) * H, 0.0f);
+    cdf.assign(size_t(H) * (W + 1), 0.0f);
+    marg.assign(size_t(H) + 1, 0.0f);
+    std::vector<double> row(static_cast<size_t>(H));
+    double total = 0.0;
+    for (int j = 0; j < H; ++j) {
+      const double st = std::sin(kPI * (double(j) + 0.5) / double(H));
+      std::vector<double> pre(size_t(W) + 1, 0.0);
+      double acc = 0.0;
+      for (int i = 0; i < W; ++i) {
+        const float* p = rgb + 3 * (size_t(j) * W + i);
+        const float f = float((0.2126 * p[0] + 0.7152 * p[1] + 0.0722 * p[2]) * st);
+        func[size_t(j) * W + i] = f;
+        acc += double(f);
+        pre[size_t(i) + 1] = acc;
+      }
+      float* c = &cdf[size_t(j) * (W + 1)];
+      for (int i = 1; i < W; ++i) c[i] = acc > 0 ? float(pre[size_t(i)] / acc) : float(double(i) / W);
+      c[0] = 0.0f;
+      c[W] = 1.0f;
+      row[size_t(j)] = acc;
+      total += acc;
+    }
+    double acc = 0.0;
+    for (int j = 1; j < H; ++j) {
+      acc += row[size_t(j) - 1];
+      marg[size_t(j)] = total > 0 ? float(acc / total) : float(double(j) / H);
+    }
+    marg[0] = 0.0f;
+    marg[size_t(H)] = 1.0f;
+    inv_int = total > 0 ? float(double(W) * double(H) / total) : 0.0f;
+  }
+};
+inline int cdf_find(const float* cdf, int n, float u) {
+  int lo = 0, hi = n;
+  while (hi - lo > 1) {
+    int mid = (lo + hi) >> 1;
+    if (cdf[mid] <= u) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
 // OfflineEngine::postProcessing, src/core/OfflineEngine.cpp:27-52.
 inline V3f ref_post_processing(V3f c) {
   const float exposure = 0.5f;
@@ -480,7 +588,7 @@ inline TileRect ref_tile_rect(int W, int H, int index) {
 // ---------------------------------------------------------------------------
 // Material record: [type, albedo r g b, roughness, metallic, Le r g b, pad, pad]
 enum { MAT_GLOSSY = 0, MAT_EMITTER = 1 };
-enum { ENV_NONE = 0, ENV_CONSTANT = 1, ENV_SKY = 2 };
+enum { ENV_NONE = 0, ENV_CONSTANT = 1, ENV_SKY = 2, ENV_MAP = 3 };
 
 struct SceneDesc {
   const float* tri_v; const int32_t* tri_mat; uint32_t n_tri;       // 9 floats per triangle
@@ -496,6 +604,7 @@ struct SceneDesc {
   int has_sun; float sun_dir[3]; float sun_E[3];                    // sun_dir: direction light travels
   int env_type; float env_c0[3], env_c1[3], env_c2[3];              // const: c0; sky: zenith c0, horizon c1, ground c2
   int vol_homogeneous;                                              // constant density vol_mult; grid unused
+  const float* env_map; int env_w, env_h;                           // ENV_MAP: RGB, row 0 at theta = 0
 };
 
 struct CameraDesc {
@@ -536,6 +645,7 @@ struct World {
   int mres = 1;
   std::vector<float> maj;      // majorant of sigma_t per cell (mres^3)
   float sigma_bar = 0;         // avg(sigma_t * mult)
+  EnvTables env;               // environment-map sampling tables
   int n_lights = 0;            // sun? + env? + emissive primitives
   struct Emitter { int type; uint32_t index; };  // type 0 tri, 1 sphere, 2 rect; input index
   std::vector<Emitter> emitters;
@@ -555,6 +665,7 @@ struct World {
       mres = s->vol_homogeneous ? 1 : std::max(1, s->maj_res);
       build_majorants();
     }
+    if (s->env_type == ENV_MAP) env.build(s->env_map, s->env_w, s->env_h);
     if (s->has_sun) n_lights++;
     if (s->env_type != ENV_NONE) n_lights++;
     // light-selection order: emissive rectangles, spheres, triangles, each in input order
@@ -911,7 +1022,56 @@ struct World {
   }
 
   // --- lights
+  // Environment map lookup as InfiniteAreaLight::Le (InfiniteAreaLight.h:47-55):
+  // local frame of (0,0,1) from generateOrthonormalCS, (x, y, z) = (d.y, -d.x, d.z),
+  // u = phi / 2 pi (phi in [0, 2 pi)), v = theta / pi; piecewise-constant texels.
+  int env_texel(V d, T& sin_t) const {
+    T phi = std::atan2(-d.x, d.y);
+    phi = phi < T(0) ? phi + T(kTWO_PI) : phi;
+    T z = std::min(T(1), std::max(T(-1), d.z));
+    sin_t = std::sqrt(std::max(T(0), T(1) - z * z));
+    T theta = std::acos(z);
+    int i = std::min(int(phi * T(1.0 / kTWO_PI) * T(s->env_w)), s->env_w - 1);
+    int j = std::min(int(theta * T(1.0 / kPI) * T(s->env_h)), s->env_h - 1);
+    return j * s->env_w + i;
+  }
+  T env_pdf(V d) const {
+    if (s->env_type != ENV_MAP) return T(1.0 / kFOUR_PI);
+    T st;
+    int k = env_texel(d, st);
+    if (!(st > T(0))) return T(0);
+    return T(env.func[size_t(k)]) * T(env.inv_int) / (T(2.0 * kPI * kPI) * st);
+  }
+  void env_sample(T ua, T ub, V& wi, T& pl, V& Le) const {
+    if (s->env_type != ENV_MAP) {
+      T z = T(1) - T(2) * ua, r = std::sqrt(std::max(T(0), T(1) - z * z)), ph = T(kTWO_PI) * ub;
+      wi = V(r * std::cos(ph), r * std::sin(ph), z);
+      Le = env_le(wi);
+      pl = T(1.0 / kFOUR_PI);
+      return;
+    }
+    const int W = s->env_w, H = s->env_h;
+    const float fv = float(ub), fu = float(ua);
+    int j = cdf_find(env.marg.data(), H, fv);
+    float m0 = env.marg[size_t(j)], m1 = env.marg[size_t(j) + 1];
+    float dv = m1 > m0 ? (fv - m0) / (m1 - m0) : 0.0f;
+    const float* row = env.cdf.data() + size_t(j) * size_t(W + 1);
+    int i = cdf_find(row, W, fu);
+    float c0 = row[i], c1 = row[i + 1];
+    float du = c1 > c0 ? (fu - c0) / (c1 - c0) : 0.0f;
+    T u = (T(i) + T(du)) / T(W), v = (T(j) + T(dv)) / T(H);
+    T theta = v * T(kPI), phi = u * T(kTWO_PI);
+    T st = std::sin(theta), ct = std::cos(theta);
+    wi = V(-st * std::sin(phi), st * std::cos(phi), ct);
+    size_t k = size_t(j) * size_t(W) + size_t(i);
+    pl = st > T(0) ? T(env.func[k]) * T(env.inv_int) / (T(2.0 * kPI * kPI) * st) : T(0);
+    Le = ldv(s->env_map + 3 * k);
+  }
   V env_le(V d) const {
+    if (s->env_type == ENV_MAP) {
+      T st;
+      return ldv(s->env_map + 3 * size_t(env_texel(d, st)));
+    }
     if (s->env_type == ENV_CONSTANT) return V(T(s->env_c0[0]), T(s->env_c0[1]), T(s->env_c0[2]));
     if (s->env_type == ENV_SKY) {
       V zen(T(s->env_c0[0]), T(s->env_c0[1]), T(s->env_c0[2]));
@@ -1143,7 +1303,7 @@ V3<T> Li(const World<T>& w, V3<T> o, V3<T> d, int max_bounces, Rng& rng, PathSta
     if (!medium_event) {
       if (!hs) {
         if (s->env_type != ENV_NONE) {
-          T wgt = prev_delta ? T(1) : power_heuristic(prev_pdf, T(1.0 / kFOUR_PI));
+          T wgt = prev_delta ? T(1) : power_heuristic(prev_pdf, w.env_pdf(d));
           L = L + beta * w.env_le(d) * wgt;
         }
         break;
@@ -1187,10 +1347,7 @@ V3<T> Li(const World<T>& w, V3<T> o, V3<T> d, int max_bounces, Rng& rng, PathSta
         Le = V(T(s->sun_E[0]), T(s->sun_E[1]), T(s->sun_E[2]));
         pl = T(1); delta = true;
       } else if (s->env_type != ENV_NONE && li == (s->has_sun ? 1 : 0)) {
-        T z = T(1) - T(2) * ua, r = std::sqrt(std::max(T(0), T(1) - z * z)), ph = T(kTWO_PI) * ub;
-        wi = V(r * std::cos(ph), r * std::sin(ph), z);
-        Le = w.env_le(wi);
-        pl = T(1.0 / kFOUR_PI);
+        w.env_sample(ua, ub, wi, pl, Le);
       } else {
         const auto& e = w.emitters[size_t(li - (s->has_sun ? 1 : 0) - (s->env_type != ENV_NONE ? 1 : 0))];
         w.sample_emitter(e, x, ua, ub, wi, dist, pl, Le);

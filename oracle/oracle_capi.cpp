@@ -200,6 +200,29 @@ void neo_world_majorants(void* p, float* out) {
   for (size_t i = 0; i < m.size(); ++i) out[i] = m[i];
 }
 
+// environment: sampled directions (u2: 2 uniforms per sample), solid-angle pdfs and radiance (FP32 world)
+void neo_world_env_sample(void* p, int n, const float* u2, float* dirs, float* pdf, float* le) {
+  const World<float>& W = *static_cast<OWorld*>(p)->wf;
+  for (int i = 0; i < n; ++i) {
+    V3f wi, L;
+    float pl;
+    W.env_sample(u2[2 * i], u2[2 * i + 1], wi, pl, L);
+    put(dirs + 3 * i, wi); pdf[i] = pl; put(le + 3 * i, L);
+  }
+}
+void neo_world_env_eval(void* p, int n, const float* dirs, float* pdf, float* le) {
+  const World<float>& W = *static_cast<OWorld*>(p)->wf;
+  for (int i = 0; i < n; ++i) {
+    pdf[i] = W.env_pdf(v3(dirs + 3 * i));
+    put(le + 3 * i, W.env_le(v3(dirs + 3 * i)));
+  }
+}
+// the reference's Distribution2D over f (width x height), conditionals of cond_n values
+void neo_ref_dist2d(const float* f, int width, int height, int cond_n, int n, const float* u2, float* uv, float* pdf) {
+  RefDistribution2D d(f, width, height, cond_n);
+  for (int i = 0; i < n; ++i) d.sampleContinuous(u2[2 * i], u2[2 * i + 1], uv + 2 * i, pdf + i);
+}
+
 // closest surface hit in (tmin, tmax); type -1 on miss
 void neo_world_closest(void* p, int n, const float* ray, float tmin, float tmax, float* t, int* type, int* index) {
   OWorld* w = static_cast<OWorld*>(p);

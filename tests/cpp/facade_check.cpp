@@ -5,14 +5,35 @@
 //   * pixels outside the 40 x 10 tile grid stay untouched by the tile calls
 //   * Li radiance is finite and non-negative; a ray that misses everything
 //     returns the constant environment radiance exactly
+//   * SceneReader loads the reference's surfaceAndLight test scene from the
+//     resources directory given as argv[1] (host only, before any device
+//     call), and on the device its frame renders to finite pixels
+// Exit codes: 0 pass, 1 mismatch, 3 no device / device error, 4 reader failure.
 #include <cmath>
 #include <cstdio>
 #include <vector>
 
 #include "ne_gpu.hpp"
 
-int main() {
+int main(int argc, char** argv) {
   using namespace narval_gpu;
+  SceneReader reader(argc > 1 ? argv[1] : "");
+  if (argc > 1) {
+    try {
+      reader.loadScene("scenes/testing/surfaceAndLight.json", false);
+      const ne_scene_desc& sd = reader.getScene();
+      const SceneSettings st = reader.getSettings();
+      const CameraSpec mc = reader.getMainCamera();
+      if (sd.n_rect != 2 || sd.n_mat != 2 || st.width != 600 || st.height != 300 || st.bounces != 6 ||
+          mc.aspect != 2.0f || mc.up.y != 1.0f || sd.rect[12 + 10] != 1.0f) {
+        std::fprintf(stderr, "scene reader: unexpected content\n");
+        return 4;
+      }
+    } catch (const Error& e) {
+      std::fprintf(stderr, "scene reader error %d: %s\n", int(e.status), e.what());
+      return 4;
+    }
+  }
   try {
     Context ctx(0);
     const int res = 32;
@@ -80,6 +101,21 @@ int main() {
       if (!(std::isfinite(v.x) && std::isfinite(v.y) && std::isfinite(v.z)) || v.x < 0 || v.y < 0 || v.z < 0) ++bad;
     if (L[1].x != 0.2f || L[1].y != 0.25f || L[1].z != 0.35f) ++bad;
     if (!std::isfinite(L1.x)) ++bad;
+    if (argc > 1) {  // the loaded reference scene renders on the device
+      Scene loaded(ctx, reader.getScene());
+      SceneSettings ls = reader.getSettings();
+      ls.width = 120;
+      ls.height = 60;
+      ls.spp = 4;
+      OfflineEngine eng(reader.getMainCamera(), ls, loaded, 1);
+      eng.renderFrame();
+      double sum = 0;
+      for (float v : eng.pixels) {
+        if (!std::isfinite(v) || v < 0 || v > 1) ++bad;
+        sum += v;
+      }
+      if (!(sum > 0)) ++bad;
+    }
     std::printf("{\"covered\": %d, \"mismatches\": %d, \"L0\": [%g, %g, %g]}\n", covered, bad, L[0].x, L[0].y,
                 L[0].z);
     return bad == 0 ? 0 : 1;

@@ -59,6 +59,9 @@ def lib():
                                        C.c_uint64, _F]),
             "neo_post_processing": (None, [C.c_int, _F, _F]),
             "neo_homogeneous_tr": (None, [C.c_int, _F, _F, _F, _F]),
+            "neo_world_env_sample": (None, [C.c_void_p, C.c_int, _F, _F, _F, _F]),
+            "neo_world_env_eval": (None, [C.c_void_p, C.c_int, _F, _F, _F]),
+            "neo_ref_dist2d": (None, [_F, C.c_int, C.c_int, C.c_int, C.c_int, _F, _F, _F]),
             "neo_homogeneous_sample": (None, [C.c_int, _F, _F, C.c_float, _F, C.c_uint64, _I, _F, _F]),
             "neo_camera_dirs": (None, [C.c_void_p, C.c_int, _F, _F]),
             "neo_tile_rect": (None, [C.c_int, C.c_int, C.c_int, _I]),
@@ -170,6 +173,15 @@ def homogeneous_sample(sig_s, sig_a, density, tint, seed=0):
     return sampled, t, w
 
 
+def ref_dist2d(func, cond_n, u2):
+    func = f(func)
+    H, W = func.shape
+    u2 = f(u2).reshape(-1, 2)
+    uv, pdf = np.zeros((len(u2), 2), np.float32), np.zeros(len(u2), np.float32)
+    lib().neo_ref_dist2d(fp(func), W, H, cond_n, len(u2), fp(u2), fp(uv), fp(pdf))
+    return uv, pdf
+
+
 class World:
     """Oracle scene (BVH, volume, lights) built from the same ne_scene_desc."""
 
@@ -203,6 +215,20 @@ class World:
         sc, t = np.zeros(len(rays), np.int32), np.zeros(len(rays), np.float64)
         lib().neo_world_delta_track(self.h, len(rays), fp(rays), seed, ip(sc), dp(t))
         return sc, t
+
+    def env_sample(self, u2):
+        u2 = f(u2).reshape(-1, 2)
+        n = len(u2)
+        d, p, le = np.zeros((n, 3), np.float32), np.zeros(n, np.float32), np.zeros((n, 3), np.float32)
+        lib().neo_world_env_sample(self.h, n, fp(u2), fp(d), fp(p), fp(le))
+        return d, p, le
+
+    def env_eval(self, dirs):
+        dirs = f(dirs).reshape(-1, 3)
+        n = len(dirs)
+        p, le = np.zeros(n, np.float32), np.zeros((n, 3), np.float32)
+        lib().neo_world_env_eval(self.h, n, fp(dirs), fp(p), fp(le))
+        return p, le
 
     def majorants(self, mres):
         out = np.zeros(mres ** 3, np.float32)

@@ -53,6 +53,21 @@ def emitter_mix() -> Scene:
     return s
 
 
+def env_map_scene(res=24, seed=0) -> Scene:
+    """A volume over a glossy floor under an importance-sampled environment
+    map with a bright patch (InfiniteAreaLight) and the sun."""
+    rng = np.random.default_rng(seed)
+    s = Scene()
+    s.set_volume(configs.fbm_grid(res, seed), 12.0, (0.95,) * 3, (0.05,) * 3, 0.7, maj_res=4)
+    floor = s.add_glossy((0.6, 0.55, 0.5), 0.5)
+    s.add_rectangle(floor, position=(0, -0.5, 0), rotation=(90, 0, 0), scale=(6, 6, 1))
+    m = rng.gamma(0.8, 0.3, (32, 64, 3)).astype(np.float32)
+    m[6:9, 20:26] *= 60.0
+    s.set_env_map(m)
+    s.set_sun(-np.array([1.0, 2.0, -0.7]), (0.5, 0.5, 0.5))
+    return s
+
+
 def small_cloud(res=48, maj_res=6) -> Scene:
     return configs.cloud_scene(configs.cumulus_grid(res, 2), maj_res=maj_res)
 

@@ -285,13 +285,15 @@ def _render_pair(ctx, s, cam, W, H, spp, bounces, seed=21, precision=NE_FP32):
     return gpu, cpu
 
 
-@pytest.mark.parametrize("which", ["volume", "surface", "cloud", "homogeneous", "emitters"])
+@pytest.mark.parametrize("which", ["volume", "surface", "cloud", "homogeneous", "emitters", "envmap"])
 def test_render_matches_oracle(ctx, which):
     print_case(f"render {which}")
     if which == "volume":
         s, cam, bounces = scenes.small_volume(res=32, maj_res=4), scenes.VOLUME_CAMERA, 16
     elif which == "emitters":
         s, cam, bounces = scenes.emitter_mix(), scenes.BOX_CAMERA, 8
+    elif which == "envmap":
+        s, cam, bounces = scenes.env_map_scene(), scenes.VOLUME_CAMERA, 16
     elif which == "homogeneous":
         s, cam, bounces = scenes.homogeneous_box(), scenes.VOLUME_CAMERA, 16
     elif which == "cloud":
@@ -395,7 +397,7 @@ def test_handles_release_in_any_order():
     f.close()
 
 
-@pytest.mark.parametrize("which", ["volume", "surface", "cloud", "homogeneous", "emitters"])
+@pytest.mark.parametrize("which", ["volume", "surface", "cloud", "homogeneous", "emitters", "envmap"])
 def test_wavefront_matches_megakernel(ctx, which):
     print_case(f"wavefront vs megakernel {which}")
     from narvalengine_amd.abi import NE_MODE_MEGAKERNEL, NE_MODE_WAVEFRONT
@@ -403,6 +405,8 @@ def test_wavefront_matches_megakernel(ctx, which):
         s, cam, bounces = scenes.small_volume(res=32, maj_res=4), scenes.VOLUME_CAMERA, 16
     elif which == "emitters":
         s, cam, bounces = scenes.emitter_mix(), scenes.BOX_CAMERA, 8
+    elif which == "envmap":
+        s, cam, bounces = scenes.env_map_scene(), scenes.VOLUME_CAMERA, 16
     elif which == "homogeneous":
         s, cam, bounces = scenes.homogeneous_box(), scenes.VOLUME_CAMERA, 16
     elif which == "cloud":

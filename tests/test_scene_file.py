@@ -260,3 +260,40 @@ def test_scene_file_drives_the_oracle_like_the_builder(tmp_path):
     ib = O.World(s).render(f.camera, 16, 8, 4, 4, seed=3)
     np.testing.assert_allclose(ia, ib, rtol=1e-4, atol=1e-5)
     assert ia.mean() > 0
+
+
+def test_environment_images(tmp_path):
+    """infiniteAreaLight on a point primitive: Radiance .hdr (run-length and
+    flat scanlines) and .pfm (either byte order) maps, row 0 at the top."""
+    from narvalengine_amd.scene_file import _rgbe, decode_rgbe, write_hdr, write_pfm
+    rng = np.random.default_rng(7)
+    img = rng.gamma(0.7, 2.0, (6, 40, 3)).astype(np.float32)
+    img[1, 5:30] = (3.0, 2.0, 1.0)  # long runs for the run-length coder
+    img[4] = 0.0
+    expect_hdr = decode_rgbe(_rgbe(img.astype(np.float64)))
+    # RGBE keeps 8 mantissa bits relative to the brightest channel of a pixel
+    assert np.all(np.abs(expect_hdr - img) <= img.max(axis=2, keepdims=True) / 128 + 1e-30)
+    point = {"name": "sky", "type": "point", "materialName": "env",
+             "transform": {"position": [0, 0, 0], "rotation": [0, 0, 0], "scale": [1, 1, 1]}}
+    cases = []
+    for name, writer, expect in (("rle.hdr", lambda p: write_hdr(p, img, rle=True), expect_hdr),
+                                 ("flat.hdr", lambda p: write_hdr(p, img, rle=False), expect_hdr),
+                                 ("le.pfm", lambda p: write_pfm(p, img, True), img),
+                                 ("be.pfm", lambda p: write_pfm(p, img, False), img)):
+        writer(tmp_path / name)
+        mats = [{"name": "env", "type": "infiniteAreaLight", "path": name}]
+        f = SceneFile(text=_doc(mats, [point]), resources_dir=tmp_path)
+        d, _ = f.desc()
+        assert d.env_type == abi.NE_ENV_MAP and (d.env_w, d.env_h) == (40, 6)
+        np.testing.assert_array_equal(f.arrays()["env_map"], expect)
+        cases.append(name)
+    assert len(cases) == 4
+    # a material that no point uses is loaded but lights nothing
+    d, _ = SceneFile(text=_doc([{"name": "env", "type": "infiniteAreaLight", "path": "le.pfm"}], []),
+                     resources_dir=tmp_path).desc()
+    assert d.env_type == abi.NE_ENV_NONE
+    mats = [{"name": "env", "type": "infiniteAreaLight", "path": "sky.png"}]
+    assert ".hdr or .pfm" in _code(lambda: SceneFile(text=_doc(mats, [point])))
+    (tmp_path / "bad.hdr").write_bytes(b"#?RADIANCE\nFORMAT=32-bit_rle_rgbe\n\n+Y 2 +X 2\n" + bytes(16))
+    mats = [{"name": "env", "type": "infiniteAreaLight", "path": "bad.hdr"}]
+    assert "orientation" in _code(lambda: SceneFile(text=_doc(mats, [point]), resources_dir=tmp_path))
