@@ -106,6 +106,9 @@ typedef struct ne_scene_desc {
   const float* env_map;    /* NE_ENV_MAP: env_w * env_h * 3 floats */
   int32_t env_w;
   int32_t env_h;
+  const float* point;       /* point lights: 3 floats each (position), Point (src/primitives/Point.cpp) */
+  const int32_t* point_mat; /* their emitter materials (radiance = intensity) */
+  uint32_t n_point;
 } ne_scene_desc;
 
 /* Pinhole camera: Camera(lookFrom, lookAt, up, vfov, aspect, ...)
@@ -418,7 +421,7 @@ ne_status ne_gen_icosphere(int32_t subdiv, uint64_t seed, float amp, float* vert
  * cannot be read, NE_ERR_PARSE for malformed JSON, missing members and types
  * the reference rejects with LOG(FATAL), NE_ERR_UNSUPPORTED for valid input
  * this backend cannot represent (image textures, .vdb grids, glTF models,
- * point lights, a second volume); the message is in ne_last_error(NULL). */
+ * a second volume or environment); the message is in ne_last_error(NULL). */
 typedef struct ne_scene_file ne_scene_file;
 /* SceneSettings (src/core/Settings.h:9-14) plus the renderer mode. */
 typedef struct ne_scene_settings {

@@ -37,6 +37,7 @@ class SceneDesc(C.Structure):
         ("env_type", C.c_int32), ("env_c0", C.c_float * 3), ("env_c1", C.c_float * 3), ("env_c2", C.c_float * 3),
         ("vol_homogeneous", C.c_int32),
         ("env_map", _F), ("env_w", C.c_int32), ("env_h", C.c_int32),
+        ("point", _F), ("point_mat", _I), ("n_point", C.c_uint32),
     ]
 
 

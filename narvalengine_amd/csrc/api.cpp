@@ -187,6 +187,14 @@ ne_status validate_desc(ne_ctx* c, const ne_scene_desc* d) {
     if (!(rect_area(d->rect + 12 * i + 3, d->rect + 12 * i + 6) > 0))
       return fail(c, NE_ERR_INVALID_ARG, "degenerate rectangle");
   }
+  if (d->n_point && (!d->point || !d->point_mat)) return fail(c, NE_ERR_INVALID_ARG, "point arrays are NULL");
+  if (d->n_point > kItemFirstMask) return fail(c, NE_ERR_INVALID_ARG, "too many point lights (limit 2^26)");
+  for (uint32_t i = 0; i < d->n_point; ++i) {
+    if (!chk_mat(d->point_mat[i]) || d->mat[12 * size_t(d->point_mat[i])] != float(NE_MAT_EMITTER))
+      return fail(c, NE_ERR_INVALID_ARG, "a point light needs an emitter material");
+    for (int a = 0; a < 3; ++a)
+      if (!std::isfinite(d->point[3 * size_t(i) + a])) return fail(c, NE_ERR_INVALID_ARG, "point light position");
+  }
   for (uint32_t i = 0; i < d->n_mat; ++i) {
     float t = d->mat[12 * i];
     if (t != float(NE_MAT_GLOSSY) && t != float(NE_MAT_EMITTER))
@@ -482,6 +490,12 @@ ne_status ne_scene_upload(ne_ctx* c, const ne_scene_desc* d, ne_scene** out) {
     if (emits(d->tri_mat ? d->tri_mat[q] : 0)) emitters.push_back(uint32_t(PRIM_TRI) << 30 | uint32_t(tri_slot[q]));
   for (uint32_t e : emitters)
     if ((e & 0x3FFFFFFFu) == 0x3FFFFFFFu) return bail(fail(c, NE_ERR_INVALID_ARG, "emitter missing from the BVH"));
+  std::vector<float> points(size_t(d->n_point) * 4);
+  for (uint32_t q = 0; q < d->n_point; ++q) {
+    for (int a = 0; a < 3; ++a) points[4 * size_t(q) + a] = d->point[3 * size_t(q) + a];
+    std::memcpy(&points[4 * size_t(q) + 3], &d->point_mat[q], 4);
+    emitters.push_back(uint32_t(kLightPoint) << 30 | q);
+  }
   const float4* f4 = nullptr;
   if ((st = dev_upload(c, s, reinterpret_cast<const float4*>(bvh.nodes.data()), bvh.nodes.size() / 4, &f4)) != NE_OK)
     return bail(st);
@@ -497,6 +511,8 @@ ne_status ne_scene_upload(ne_ctx* c, const ne_scene_desc* d, ne_scene** out) {
     return bail(st);
   if ((st = dev_upload(c, s, d->mat, size_t(d->n_mat) * 12, &ds.mats)) != NE_OK) return bail(st);
   if ((st = dev_upload(c, s, emitters.data(), emitters.size(), &ds.emitters)) != NE_OK) return bail(st);
+  if ((st = dev_upload(c, s, reinterpret_cast<const float4*>(points.data()), points.size() / 4, &ds.points)) != NE_OK)
+    return bail(st);
   ds.n_emitters = int32_t(emitters.size());
   // ---- volume
   ds.has_volume = d->has_volume ? 1 : 0;

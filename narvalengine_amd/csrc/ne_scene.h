@@ -24,6 +24,7 @@
 namespace ne {
 
 enum { PRIM_TRI = 0, PRIM_SPHERE = 1, PRIM_RECT = 2 };
+constexpr int kLightPoint = 3;  // light-code type of a point light (not a BVH primitive)
 
 // BVH traversal stack: kShortStack entries per lane in LDS ([depth][lane]
 // layout, no bank conflicts), deeper entries spill to a private array.  The
@@ -47,6 +48,7 @@ struct DevScene {
   const float4* rects;
   const float* mats;
   const uint32_t* emitters;  // emissive primitives: type << 30 | storage slot
+  const float4* points;      // point lights: position, material (int bits)
   uint32_t n_nodes;
   int32_t n_emitters;
   // volume

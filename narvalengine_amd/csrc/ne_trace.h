@@ -709,17 +709,32 @@ NE_DEV void env_sample(const DevScene& s, R ua, R ub, Vec3<R>& wi, R& pl, Vec3<R
 //              uniform point on the surface.  The reference returns the
 //              nearest surface point with a mis-scaled area pdf
 //              (src/primitives/Sphere.cpp:46-67); the cone is the unbiased form.
+//   point      a delta light at the point (Point::samplePointOnSurface,
+//              src/primitives/Point.cpp:14-26) of intensity = the emitter's
+//              radiance, arriving as intensity / d^2 (the reference returns
+//              the intensity with pdf 1 and no falloff).
+// Returns true for a delta light (no MIS weight).
 template <typename R>
 NE_DEV R sphere_cone_width(R dc2, R r2) {  // 1 - cos theta_max, cancellation-free
   const R sin2 = r2 / dc2;
   return sin2 / (R(1) + xsqrt(xmax(R(0), R(1) - sin2)));
 }
 template <typename R>
-NE_DEV void sample_emitter(const DevScene& s, uint32_t code, Vec3<R> x, R ua, R ub, Vec3<R>& wi, R& dist, R& pl,
+NE_DEV bool sample_emitter(const DevScene& s, uint32_t code, Vec3<R> x, R ua, R ub, Vec3<R>& wi, R& dist, R& pl,
                            Vec3<R>& Le) {
   const int type = int(code >> 30);
   const uint32_t slot = code & 0x3FFFFFFFu;
   int mat;
+  if (type == kLightPoint) {
+    const float4 pt = s.points[slot];
+    const Vec3<R> dv = v3f4<R>(pt) - x;
+    const R d2 = dot(dv, dv);
+    dist = xsqrt(d2);
+    wi = dv / dist;
+    pl = d2 > R(0) ? R(1) : R(0);
+    Le = v3f<R>(s.mats + 12 * size_t(__float_as_int(pt.w)) + 6) / d2;
+    return true;
+  }
   if (type == PRIM_RECT) {
     const float4* rc = s.rects + 4 * size_t(slot);
     const float4 f0 = rc[0], f1 = rc[1], f2 = rc[2], f3 = rc[3];
@@ -779,6 +794,7 @@ NE_DEV void sample_emitter(const DevScene& s, uint32_t code, Vec3<R> x, R ua, R 
     pl = cosl > R(0) ? d2 / (cosl * (R(0.5) * len)) : R(0);
   }
   Le = v3f<R>(s.mats + 12 * size_t(mat) + 6);
+  return false;
 }
 // Solid-angle pdf with which sample_emitter picks the point that the ray
 // (o, d) hits at distance t on emitter (type, slot) with normal n.
@@ -934,7 +950,7 @@ NE_DEV bool path_bounce(const DevScene& s, PathState<R>& p, int max_bounces, Pat
         env_sample<R>(s, ua, ub, wi, pl, Le);
       } else {
         const uint32_t code = s.emitters[li - env_slot - (s.env_type != 0 ? 1 : 0)];
-        sample_emitter<R>(s, code, x, ua, ub, wi, dist, pl, Le);
+        delta = sample_emitter<R>(s, code, x, ua, ub, wi, dist, pl, Le);
       }
       if (pl > R(0)) {
         Vec3<R> f;

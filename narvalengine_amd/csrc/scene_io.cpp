@@ -577,8 +577,8 @@ struct MaterialDef {
 }  // namespace
 
 struct ne_scene_file {
-  std::vector<float> tri_v, sph, rect, mat, grid, env;
-  std::vector<int32_t> tri_mat, sph_mat, rect_mat;
+  std::vector<float> tri_v, sph, rect, mat, grid, env, point;
+  std::vector<int32_t> tri_mat, sph_mat, rect_mat, point_mat;
   ne_scene_desc desc{};
   ne_camera_desc camera{};
   ne_scene_settings settings{};
@@ -826,7 +826,8 @@ class Loader {
         f_.desc.env_w = d.env.w;
         f_.desc.env_h = d.env.h;
       } else if (d.kind == MaterialDef::kSurface && d.emitter) {
-        unsupported(ctx + ": point lights are not supported");
+        for (int a = 0; a < 3; ++a) f_.point.push_back(float(pos[a]));
+        f_.point_mat.push_back(d.record);
       }
       // a point with a non-emitting material has no area: nothing to add
     } else if (type == "sphere") {
@@ -949,6 +950,9 @@ void point_desc(ne_scene_file* f) {
   d.n_mat = uint32_t(f->mat.size() / 12);
   d.grid = f->grid.empty() ? nullptr : f->grid.data();
   d.env_map = f->env.empty() ? nullptr : f->env.data();
+  d.point = f->point.empty() ? nullptr : f->point.data();
+  d.point_mat = f->point_mat.empty() ? nullptr : f->point_mat.data();
+  d.n_point = uint32_t(f->point_mat.size());
 }
 
 ne_status parse_into(const char* json, size_t len, const ne_load_options* opt, const std::string& default_base,

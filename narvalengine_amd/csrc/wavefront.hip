@@ -365,7 +365,7 @@ __device__ __forceinline__ bool nee_sample(const DevScene& s, Vec3<float> x, Vec
     env_sample<float>(s, ua, ub, wi, pl, Le);
   } else {
     const uint32_t code = s.emitters[li - env_slot - (s.env_type != 0 ? 1 : 0)];
-    sample_emitter<float>(s, code, x, ua, ub, wi, dist, pl, Le);
+    delta = sample_emitter<float>(s, code, x, ua, ub, wi, dist, pl, Le);
   }
   if (!(pl > 0.0f)) return false;
   float pf;

@@ -73,6 +73,8 @@ class Scene:
     rect: list = field(default_factory=list)       # 12 floats
     rect_mat: list = field(default_factory=list)
     mats: list = field(default_factory=list)       # 12 floats
+    points: list = field(default_factory=list)     # point-light positions
+    point_mat: list = field(default_factory=list)
     volume: dict | None = None
     sun: dict | None = None
     env: dict | None = None
@@ -118,6 +120,12 @@ class Scene:
         self.rect.append(np.concatenate([p0[:3], eu, ev, n]).astype(np.float32))
         self.rect_mat.append(mat)
 
+    def add_point_light(self, position, mat: int):
+        """A point primitive with an emitter material (Point, src/primitives/Point.cpp):
+        a delta light of intensity = the material's radiance."""
+        self.points.append(np.array(position, np.float32))
+        self.point_mat.append(mat)
+
     def set_volume(self, grid: np.ndarray, density_mult: float, sigma_s, sigma_a, g: float,
                    position=(0, 0, 0), rotation=(0, 0, 0), scale=(1, 1, 1), maj_res: int = 16):
         grid = np.ascontiguousarray(grid, dtype=np.float32)
@@ -159,6 +167,7 @@ class Scene:
             sph=cat(self.sph, 4, np.float32), sph_mat=np.asarray(self.sph_mat, np.int32),
             rect=cat(self.rect, 12, np.float32), rect_mat=np.asarray(self.rect_mat, np.int32),
             mat=cat(self.mats, 12, np.float32),
+            point=cat(self.points, 3, np.float32), point_mat=np.asarray(self.point_mat, np.int32),
         )
         if self.volume is not None and self.volume["grid"] is not None:
             a["grid"] = self.volume["grid"]
@@ -174,6 +183,7 @@ class Scene:
         d.sph, d.sph_mat, d.n_sph = _ptr(a["sph"], C.c_float), _ptr(a["sph_mat"], C.c_int32), len(a["sph"])
         d.rect, d.rect_mat, d.n_rect = _ptr(a["rect"], C.c_float), _ptr(a["rect_mat"], C.c_int32), len(a["rect"])
         d.mat, d.n_mat = _ptr(a["mat"], C.c_float), len(a["mat"])
+        d.point, d.point_mat, d.n_point = _ptr(a["point"], C.c_float), _ptr(a["point_mat"], C.c_int32), len(a["point"])
         if self.volume is not None:
             v = self.volume
             d.has_volume = 1

@@ -83,7 +83,8 @@ class SceneFile:
         out = dict(tri_v=arr(d.tri_v, d.n_tri, 9, np.float32), tri_mat=arr(d.tri_mat, d.n_tri, 1, np.int32),
                    sph=arr(d.sph, d.n_sph, 4, np.float32), sph_mat=arr(d.sph_mat, d.n_sph, 1, np.int32),
                    rect=arr(d.rect, d.n_rect, 12, np.float32), rect_mat=arr(d.rect_mat, d.n_rect, 1, np.int32),
-                   mat=arr(d.mat, d.n_mat, 12, np.float32))
+                   mat=arr(d.mat, d.n_mat, 12, np.float32), point=arr(d.point, d.n_point, 3, np.float32),
+                   point_mat=arr(d.point_mat, d.n_point, 1, np.int32))
         if d.has_volume and not d.vol_homogeneous:
             W, H, D = d.vol_res
             out["grid"] = arr(d.grid, W * H * D, 1, np.float32).reshape(D, H, W)

@@ -605,6 +605,7 @@ struct SceneDesc {
   int env_type; float env_c0[3], env_c1[3], env_c2[3];              // const: c0; sky: zenith c0, horizon c1, ground c2
   int vol_homogeneous;                                              // constant density vol_mult; grid unused
   const float* env_map; int env_w, env_h;                           // ENV_MAP: RGB, row 0 at theta = 0
+  const float* point; const int32_t* point_mat; uint32_t n_point;   // point lights (emitter materials)
 };
 
 struct CameraDesc {
@@ -647,7 +648,7 @@ struct World {
   float sigma_bar = 0;         // avg(sigma_t * mult)
   EnvTables env;               // environment-map sampling tables
   int n_lights = 0;            // sun? + env? + emissive primitives
-  struct Emitter { int type; uint32_t index; };  // type 0 tri, 1 sphere, 2 rect; input index
+  struct Emitter { int type; uint32_t index; };  // type 0 tri, 1 sphere, 2 rect, 3 point; input index
   std::vector<Emitter> emitters;
 
   static V ldv(const float* q) { return V(T(q[0]), T(q[1]), T(q[2])); }
@@ -676,6 +677,7 @@ struct World {
       if (emits(s->sph_mat[i])) emitters.push_back({1, i});
     for (uint32_t i = 0; i < s->n_tri; ++i)
       if (emits(s->tri_mat ? s->tri_mat[i] : 0)) emitters.push_back({0, i});
+    for (uint32_t i = 0; i < s->n_point; ++i) emitters.push_back({3, i});
     n_lights += int(emitters.size());
   }
 
@@ -1094,8 +1096,20 @@ struct World {
     T sin2 = r2 / dc2;
     return sin2 / (T(1) + std::sqrt(std::max(T(0), T(1) - sin2)));
   }
-  void sample_emitter(const Emitter& e, V x, T ua, T ub, V& wi, T& dist, T& pl, V& Le) const {
+  // Point lights: delta lights of intensity = the emitter's radiance, arriving
+  // as intensity / d^2 (the reference's Point pdf is 1 with no falloff,
+  // src/primitives/Point.cpp:14-26).  Returns true for a delta light.
+  bool sample_emitter(const Emitter& e, V x, T ua, T ub, V& wi, T& dist, T& pl, V& Le) const {
     int m;
+    if (e.type == 3) {
+      V dv = ldv(s->point + 3 * size_t(e.index)) - x;
+      T d2 = dot(dv, dv);
+      dist = std::sqrt(d2);
+      wi = dv / dist;
+      pl = d2 > T(0) ? T(1) : T(0);
+      Le = ldv(s->mat + 12 * size_t(s->point_mat[e.index]) + 6) / d2;
+      return true;
+    }
     if (e.type == 2) {
       const float* q = s->rect + 12 * size_t(e.index);
       V p0 = ldv(q), eu = ldv(q + 3), ev = ldv(q + 6), ln = ldv(q + 9);
@@ -1153,6 +1167,7 @@ struct World {
       pl = cosl > T(0) ? d2 / (cosl * (T(0.5) * len)) : T(0);
     }
     Le = ldv(s->mat + 12 * size_t(m) + 6);
+    return false;
   }
   // Solid-angle pdf of sample_emitter for the point the ray (o, d) hits.
   T emitter_hit_pdf(const SHit& h, V o, V d) const {
@@ -1350,7 +1365,7 @@ V3<T> Li(const World<T>& w, V3<T> o, V3<T> d, int max_bounces, Rng& rng, PathSta
         w.env_sample(ua, ub, wi, pl, Le);
       } else {
         const auto& e = w.emitters[size_t(li - (s->has_sun ? 1 : 0) - (s->env_type != ENV_NONE ? 1 : 0))];
-        w.sample_emitter(e, x, ua, ub, wi, dist, pl, Le);
+        delta = w.sample_emitter(e, x, ua, ub, wi, dist, pl, Le);
       }
       if (pl > T(0)) {
         V f; T pf;

@@ -36,7 +36,8 @@ def homogeneous_box(density=3.0, albedo=0.9, g=0.6, rotation=(20, 35, 0)) -> Sce
 
 def emitter_mix() -> Scene:
     """Every emitter shape: a rectangle, two spheres (one small and far, one
-    close to the walls) and an emissive triangle pair, over glossy walls."""
+    close to the walls), an emissive triangle pair and a point light, over
+    glossy walls."""
     s = Scene()
     wall = s.add_glossy((0.7, 0.7, 0.7), 0.6)
     red = s.add_glossy((0.8, 0.2, 0.2), 0.3, 0.5)
@@ -50,6 +51,7 @@ def emitter_mix() -> Scene:
     s.add_sphere((-0.3, 0.3, -0.2), 0.3, red)
     q = np.array([[0.7, 0.9, 0.99], [0.95, 0.9, 0.99], [0.95, 1.3, 0.99], [0.7, 1.3, 0.99]])
     s.add_mesh(q, np.array([[0, 1, 2], [0, 2, 3]]), s.add_emitter((2.0, 5.0, 2.0)))
+    s.add_point_light((0.2, 1.6, -0.3), s.add_emitter((0.8, 0.8, 0.6)))
     return s
 
 

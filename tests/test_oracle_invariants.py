@@ -461,3 +461,26 @@ def test_furnace_inside_an_emissive_sphere():
     spp = 256
     img = w.render(cam, 8, 8, spp, 256, seed=6) / spp
     assert abs(img.mean() - 1.0) < 0.01, img.mean()
+
+
+def test_point_light_on_a_glossy_floor_is_the_closed_form():
+    """A point light of intensity I at distance d over a glossy floor: the only
+    contribution is NEE, f(wo, wi) I cos / d^2, exact per sample."""
+    s = Scene()
+    albedo, rough, metal = (0.6, 0.5, 0.4), 0.4, 0.2
+    floor = s.add_glossy(albedo, rough, metal)
+    s.add_rectangle(floor, position=(0, 0, 0), rotation=(90, 0, 0), scale=(6, 6, 1))
+    I = np.array([3.0, 2.0, 1.0])
+    pos = np.array([0.3, 1.2, 0.4])
+    s.add_point_light(tuple(pos), s.add_emitter(tuple(I)))
+    cam = Camera(position=(0.0, 1.0, -1.0), look_at=(0.0, 0.0, 0.0), up=(0, 1, 0), vfov=0.005, aspect=1.0)
+    w = O.World(s, fp64=True)
+    img = w.render(cam, 1, 1, 64, 1, seed=2) / 64
+    x = np.zeros(3)
+    wo = _unit(np.array(cam.position) - x)
+    dv = pos - x
+    d2 = dv @ dv
+    wi = dv / np.sqrt(d2)
+    f = _glossy_f(albedo, rough, metal, wo, wi[None])[0]
+    expect = f * I * wi[1] / d2
+    np.testing.assert_allclose(img[0, 0], expect, rtol=2e-3)

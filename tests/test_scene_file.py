@@ -228,9 +228,11 @@ def test_errors_follow_the_reference_reader(tmp_path):
             "transform": {"position": [0, 0, 0], "rotation": [0, 0, 0], "scale": [1, 1, 1]}}
     assert "glTF" in _code(lambda: parse(_doc([ok_mat], [gltf])))
     pl = {"name": "p", "type": "point", "materialName": "e",
-          "transform": {"position": [0, 0, 0], "rotation": [0, 0, 0], "scale": [1, 1, 1]}}
-    assert "point lights" in _code(lambda: parse(_doc([{"name": "e", "type": "emitter", "albedo": [1, 1, 1]}],
-                                                      [pl])))
+          "transform": {"position": [0.5, 2, -1], "rotation": [0, 0, 0], "scale": [1, 1, 1]}}
+    f = parse(_doc([{"name": "e", "type": "emitter", "albedo": [3, 2, 1]}], [pl]))
+    a = f.arrays()
+    np.testing.assert_array_equal(a["point"], np.float32([[0.5, 2, -1]]))
+    np.testing.assert_array_equal(a["mat"][a["point_mat"][0], 6:9], [3, 2, 1])
 
 
 def test_scene_file_drives_the_oracle_like_the_builder(tmp_path):
