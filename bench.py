@@ -287,6 +287,7 @@ def main():
                     choices=["surface1m", "volume128", "cloud512", "batch48", "guided"])
     ap.add_argument("--mode", default="auto", choices=["auto", "megakernel", "wavefront"])
     ap.add_argument("--pool", type=int, default=0, help="wavefront path slots (0 = library default)")
+    ap.add_argument("--sort", action="store_true", help="wavefront: shade chunks in (event kind, material) order")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-count", action="store_true")
@@ -326,7 +327,7 @@ def main():
     log(f"scene on device: {info}")
     film = ctx.film(wl.width, wl.height)
     kw = dict(spp=wl.spp, max_bounces=wl.max_bounces, seed=1, samples_per_thread=args.samples_per_thread,
-              block_size=args.block, mode=mode, pool_size=args.pool)
+              block_size=args.block, mode=mode, pool_size=args.pool, sort=args.sort)
     for i in range(args.warmup):
         ctx.render(g, wl.camera, film, sample_offset=(1000 + i) * wl.spp, **kw)
         log(f"warmup {i + 1}/{args.warmup} done")
@@ -386,7 +387,7 @@ def main():
         "data": "synthetic (seeded procedural scene)",
         "config": {"workload": wl.name, "width": wl.width, "height": wl.height, "spp": wl.spp,
                    "max_bounces": wl.max_bounces, "paths_per_step": paths_per_step, "accumulation": "f64",
-                   "mode": args.mode, "pool": args.pool},
+                   "mode": args.mode, "pool": args.pool, "material_sort": args.sort},
         "rays_per_sec": round(rays / step_s, 1) if rays else None,
         "roofline": roofline,
         "cpu_baseline": cpu,

@@ -126,7 +126,8 @@ enum { NE_FP32 = 0, NE_FP64 = 1 };
 enum {
   NE_FLAG_COUNT = 1,      /* collect event counters (separate, slower pass) */
   NE_FLAG_TIME = 2,       /* time the path-tracing kernels with HIP events on the context stream */
-  NE_FLAG_STAGE_TIME = 4  /* also time every kernel launch and sum the times per stage (NE_STAGE_*) */
+  NE_FLAG_STAGE_TIME = 4, /* also time every kernel launch and sum the times per stage (NE_STAGE_*) */
+  NE_FLAG_SORT = 8        /* wavefront: shade each chunk's paths ordered by (event kind, material) */
 };
 
 /* Kernel stages reported by ne_get_stage_times. */

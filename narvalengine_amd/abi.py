@@ -8,6 +8,7 @@ NE_FP32, NE_FP64 = 0, 1
 NE_FLAG_COUNT = 1
 NE_FLAG_TIME = 2
 NE_FLAG_STAGE_TIME = 4
+NE_FLAG_SORT = 8
 STAGES = ("megakernel", "plan", "regen", "trace", "medium", "shade", "shadow", "other", "guide", "train")
 NE_MODE_AUTO, NE_MODE_MEGAKERNEL, NE_MODE_WAVEFRONT = 0, 1, 2
 NE_MAT_GLOSSY, NE_MAT_EMITTER = 0, 1

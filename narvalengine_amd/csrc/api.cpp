@@ -698,6 +698,7 @@ static ne_status render_wavefront(ne_ctx* c, const ne_scene* s, const DevCamera&
   a.W = p->width; a.H = p->height; a.x0 = p->x0; a.y0 = p->y0; a.x1 = p->x1; a.y1 = p->y1;
   a.max_bounces = p->max_bounces; a.sample_offset = p->sample_offset; a.job = p->job; a.nchunks = uint32_t(P / kChunk);
   a.seed = p->seed;
+  a.sort = (p->flags & NE_FLAG_SORT) ? 1 : 0;
   if (c->refill_blocks == 0) {
     int cus = 0, per_cu = 0;
     NE_HIP(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));

@@ -103,6 +103,7 @@ struct WfArgs {
   long long* film_fixed;
   double* film;
   uint32_t* gstate[2];  // guided mode: training-record state per path buffer (else null)
+  int32_t sort;         // shade stage sorts each chunk by (event kind, material)
 };
 
 struct WfGuideHook;  // guided mode: proposal, training records, online training

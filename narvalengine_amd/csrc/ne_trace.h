@@ -330,6 +330,15 @@ NE_DEV void hit_normal(const DevScene& s, const SurfHit<R>& h, Vec3<R> o, Vec3<R
   }
 }
 
+// Material of the surface hit coded as type << 30 | storage slot.
+NE_DEV int hit_material(const DevScene& s, uint32_t ref) {
+  const int type = int(ref >> 30);
+  const uint32_t slot = ref & 0x3FFFFFFFu;
+  if (type == PRIM_TRI) return __float_as_int(s.tris[3 * size_t(slot)].w);
+  if (type == PRIM_SPHERE) return s.sph_mat[slot];
+  return __float_as_int(s.rects[4 * size_t(slot)].w);
+}
+
 // ---------------------------------------------------------------------------
 // Volume: unit-cube interval, trilinear density, majorant DDA, tracking.
 // ---------------------------------------------------------------------------

@@ -439,13 +439,44 @@ __global__ __launch_bounds__(kChunk) void k_wf_guide(WfState w, WfArgs a, DevSce
 // training on, the first guided vertex of a path is recorded and becomes a
 // training sample when the path ends: weight r = Li * phase / pdf, Li the
 // radiance gathered after the vertex divided by the throughput after it.
-template <bool GUIDED>
+//
+// SORT: before shading, the block orders its chunk's live paths by (event
+// kind, material) with a bitonic sort of 256 keys in LDS, and thread t shades
+// the path in sorted position t.  Waves then take one branch of the kind
+// switch and one material's parameters; each path keeps its own RNG stream,
+// so the image is unchanged.
+template <bool GUIDED, bool SORT>
 __global__ __launch_bounds__(kChunk) void k_wf_shade(WfState w, WfArgs a, DevScene s, WfGuide gd) {
   __shared__ uint32_t wsum[kChunk / 64];
   const WfPaths A = w.paths[a.cur], B = w.paths[a.cur ^ 1];
   const WfShadow S = w.shadow;
-  const uint32_t c = blockIdx.x, t = threadIdx.x, i = c * kChunk + t;
-  const bool active = t < w.cnt_paths[a.cur][c];
+  const uint32_t c = blockIdx.x, t = threadIdx.x;
+  const uint32_t nlive = w.cnt_paths[a.cur][c];
+  uint32_t slot = t;
+  if constexpr (SORT) {
+    __shared__ uint32_t skey[kChunk];
+    uint32_t key = 0xFFFFFFu;  // free slots sort last
+    if (t < nlive) {
+      const uint32_t kd = A.kind[c * kChunk + t];
+      uint32_t mat = 0;
+      if (kd == WF_KIND_SURFACE) mat = uint32_t(hit_material(s, A.hit_ref[c * kChunk + t]));
+      key = (kd << 21) | (mat < 0x1FFFFFu ? mat : 0x1FFFFFu);
+    }
+    skey[t] = (key << 8) | t;
+    __syncthreads();
+    for (uint32_t k = 2; k <= kChunk; k <<= 1)
+      for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+        const uint32_t p = t ^ j;
+        if (p > t) {
+          const uint32_t x0 = skey[t], x1 = skey[p];
+          if ((x0 > x1) == ((t & k) == 0)) { skey[t] = x1; skey[p] = x0; }
+        }
+        __syncthreads();
+      }
+    slot = skey[t] & 0xFFu;
+  }
+  const uint32_t i = c * kChunk + slot;
+  const bool active = t < nlive;
   bool shadow = false, cont = false, ends = false;
   Vec3<float> x = v3s(0.0f), nd = v3s(0.0f), beta = v3s(0.0f), wi = v3s(0.0f), f = v3s(0.0f), Le = v3s(0.0f);
   float tmax = 0.0f, wt = 0.0f, pl = 0.0f, ppdf = 0.0f;
@@ -845,6 +876,7 @@ hipError_t launch_wavefront(const DevScene& s, const DevCamera& cam, WfState& w,
   const dim3 blk = dim3(unsigned(kChunk));
   if (!count) a.counters = nullptr;
   const bool hs = s.n_nodes > 0, hv = s.has_volume != 0;
+  const bool sort = a.sort != 0;
   for (uint32_t it = 0;; ++it) {
     a.cur = int(it & 1u);
     NE_MARK(kStagePlan);
@@ -896,8 +928,10 @@ hipError_t launch_wavefront(const DevScene& s, const DevCamera& cam, WfState& w,
       hipLaunchKernelGGL(k_wf_guide, grid, blk, 0, st, w, a, s, gh->g);
     }
     NE_MARK(kStageShade);
-    if (gh) hipLaunchKernelGGL(k_wf_shade<true>, grid, blk, 0, st, w, a, s, gh->g);
-    else hipLaunchKernelGGL(k_wf_shade<false>, grid, blk, 0, st, w, a, s, WfGuide{});
+    if (gh && sort) hipLaunchKernelGGL((k_wf_shade<true, true>), grid, blk, 0, st, w, a, s, gh->g);
+    else if (gh) hipLaunchKernelGGL((k_wf_shade<true, false>), grid, blk, 0, st, w, a, s, gh->g);
+    else if (sort) hipLaunchKernelGGL((k_wf_shade<false, true>), grid, blk, 0, st, w, a, s, WfGuide{});
+    else hipLaunchKernelGGL((k_wf_shade<false, false>), grid, blk, 0, st, w, a, s, WfGuide{});
     NE_MARK(kStageShadow);
 #define NE_SHADOW(CNT, S_, V_) hipLaunchKernelGGL((k_wf_shadow<CNT, S_, V_>), grid, blk, 0, st, w, a, s)
     if (hs && !hv) {

@@ -462,3 +462,25 @@ def test_wavefront_counters_and_stage_times(ctx, which):
     assert (n["medium"] == n["shade"]) == (which == "volume")
     staged = sum(v[0] for v in st.values())
     assert 0 < staged <= total * 1.02 + 0.05 and staged >= 0.5 * total
+
+
+@pytest.mark.parametrize("which", ["surface", "cloud", "emitters"])
+def test_material_sort_leaves_the_image_unchanged(ctx, which):
+    """Shading each chunk in (kind, material) order is a permutation of the
+    same paths: the fixed-point film and the event counts are identical."""
+    print_case(f"material sort {which}")
+    from narvalengine_amd.abi import NE_MODE_WAVEFRONT
+    if which == "cloud":
+        s, cam, bounces = scenes.small_cloud(), scenes.VOLUME_CAMERA, 64
+    elif which == "emitters":
+        s, cam, bounces = scenes.emitter_mix(), scenes.BOX_CAMERA, 8
+    else:
+        s, cam, bounces = scenes.small_surface(subdiv=3), scenes.BOX_CAMERA, 8
+    g = ctx.upload(s)
+    out = []
+    for srt in (False, True):
+        f = ctx.film(40, 24)
+        ctx.render(g, cam, f, spp=16, max_bounces=bounces, seed=17, mode=NE_MODE_WAVEFRONT, sort=srt, count=True)
+        out.append((f.read(), ctx.counters()))
+    np.testing.assert_array_equal(out[0][0], out[1][0])
+    assert out[0][1] == out[1][1]
