@@ -33,8 +33,8 @@ HBM_PEAK_GBS = 8000.0
 PROFILES = ROOT / "profiles"
 
 # Algorithmic bytes per counted event: a trilinear density lookup reads 8 FP32
-# corners, a majorant cell 4 B, a BVH node record 64 B (both child boxes), a
-# primitive record 48 B (triangle: 3 x float4).  Ray-state traffic of the
+# corners, a majorant cell 4 B, a BVH node record 128 B (four child boxes,
+# child references and counts), a primitive record 48 B (triangle: 3 x float4).  Ray-state traffic of the
 # wavefront stages per ray (structure-of-arrays records):
 #   trace  reads origin + direction (24 B), writes hit t / ref / kind (12 B)
 #   medium reads origin, direction, hit t, pixel, sample, RNG count (40 B),
@@ -43,7 +43,7 @@ PROFILES = ROOT / "profiles"
 #   shade  reads the 84 B path record, writes 64 B per continuation and per
 #          shadow ray
 # The film costs 24 B read + 24 B write per pixel (FP64 RGB), once per frame.
-EV = {"density_lookups": 32, "majorant_cells": 4, "bvh_nodes": 64, "prim_tests": 48}
+EV = {"density_lookups": 32, "majorant_cells": 4, "bvh_nodes": 128, "prim_tests": 48}
 RAY_BYTES = {"trace": 36, "medium": 52, "shadow": 88}
 SHADE_IN, SHADE_OUT = 84, 64
 
@@ -55,7 +55,7 @@ def stage_bytes(stage: str, c: dict, film_bytes: int) -> int:
     if stage == "megakernel":
         return sum(c[k] * b for k, b in EV.items()) + film_bytes
     if stage == "trace":
-        return ext["bvh_nodes"] * 64 + ext["prim_tests"] * 48 + c["ext_rays"] * RAY_BYTES["trace"]
+        return ext["bvh_nodes"] * EV["bvh_nodes"] + ext["prim_tests"] * 48 + c["ext_rays"] * RAY_BYTES["trace"]
     if stage == "medium":
         return ext["density_lookups"] * 32 + ext["majorant_cells"] * 4 + c["ext_rays"] * RAY_BYTES["medium"]
     if stage == "shadow":
@@ -288,6 +288,7 @@ def main():
     ap.add_argument("--mode", default="auto", choices=["auto", "megakernel", "wavefront"])
     ap.add_argument("--pool", type=int, default=0, help="wavefront path slots (0 = library default)")
     ap.add_argument("--sort", action="store_true", help="wavefront: shade chunks in (event kind, material) order")
+    ap.add_argument("--maj-res", type=int, default=0, help="majorant cells per axis (0 = the workload's)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-count", action="store_true")
@@ -319,6 +320,8 @@ def main():
     mode = {"auto": NE_MODE_AUTO, "megakernel": NE_MODE_MEGAKERNEL, "wavefront": NE_MODE_WAVEFRONT}[args.mode]
     t0 = time.perf_counter()
     wl = configs.CONFIGS[args.workload]()
+    if args.maj_res and wl.scene.volume is not None:
+        wl.scene.volume["maj_res"] = args.maj_res
     log(f"workload {wl.name}: {wl.width}x{wl.height} spp {wl.spp} bounces {wl.max_bounces} "
         f"(scene built in {time.perf_counter() - t0:.1f} s)")
     ctx = Context(0)
@@ -387,7 +390,8 @@ def main():
         "data": "synthetic (seeded procedural scene)",
         "config": {"workload": wl.name, "width": wl.width, "height": wl.height, "spp": wl.spp,
                    "max_bounces": wl.max_bounces, "paths_per_step": paths_per_step, "accumulation": "f64",
-                   "mode": args.mode, "pool": args.pool, "material_sort": args.sort},
+                   "mode": args.mode, "pool": args.pool, "material_sort": args.sort,
+                   "maj_res": wl.scene.volume["maj_res"] if wl.scene.volume is not None else None},
         "rays_per_sec": round(rays / step_s, 1) if rays else None,
         "roofline": roofline,
         "cpu_baseline": cpu,

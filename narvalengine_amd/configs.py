@@ -65,7 +65,7 @@ def icosphere(subdiv: int, seed: int, amp: float):
     return v, t
 
 
-def volume_scene(grid: np.ndarray, *, mult=20.0, albedo=0.99, g=0.85, maj_res=16) -> Scene:
+def volume_scene(grid: np.ndarray, *, mult=20.0, albedo=0.99, g=0.85, maj_res=1) -> Scene:
     s = Scene()
     s.set_volume(grid, mult, (albedo,) * 3, (1.0 - albedo,) * 3, g, maj_res=maj_res)
     # DirectionalLight from `position` towards the origin (SceneReader.cpp:160-161)
@@ -74,7 +74,7 @@ def volume_scene(grid: np.ndarray, *, mult=20.0, albedo=0.99, g=0.85, maj_res=16
     return s
 
 
-def volume128(res: int = 128, seed: int = 0, width=512, height=512, spp=64, bounces=16, maj_res=16) -> Workload:
+def volume128(res: int = 128, seed: int = 0, width=512, height=512, spp=64, bounces=16, maj_res=1) -> Workload:
     s = volume_scene(fbm_grid(res, seed), maj_res=maj_res)
     cam = Camera(position=(0.0, 0.25, -1.8), look_at=(0, 0, 0), up=(0, 1, 0), vfov=45.0, aspect=width / height)
     return Workload("volume128", s, cam, width, height, spp, bounces)
@@ -139,7 +139,7 @@ def batch48(seed: int = 48, bounces: int = 16) -> list:
     placement as volume128."""
     out = []
     for res, gseed, w, h, spp in batch48_table(seed):
-        s = volume_scene(fbm_grid(res, gseed), maj_res=16)
+        s = volume_scene(fbm_grid(res, gseed), maj_res=1)
         cam = Camera(position=(0.0, 0.25, -1.8), look_at=(0, 0, 0), up=(0, 1, 0), vfov=45.0, aspect=w / h)
         out.append(Workload(f"batch48/job{gseed}", s, cam, w, h, spp, bounces))
     return out

@@ -339,7 +339,7 @@ ne_status ne_bvh_build(const ne_scene_desc* d, float* nodes, uint32_t* refs, uin
   if (st != NE_OK) return st;
   BvhOutput bvh;
   build_bvh(prim_bounds(d), bvh);
-  if (n_nodes) *n_nodes = uint32_t(bvh.nodes.size() / 16);
+  if (n_nodes) *n_nodes = uint32_t(bvh.nodes.size() / kNodeFloats);
   if (n_refs) *n_refs = uint32_t(bvh.refs.size());
   if (depth) *depth = bvh.max_depth;
   if (nodes) std::memcpy(nodes, bvh.nodes.data(), bvh.nodes.size() * sizeof(float));
@@ -427,7 +427,7 @@ ne_status ne_scene_upload(ne_ctx* c, const ne_scene_desc* d, ne_scene** out) {
   const uint32_t nt = d->n_tri, ns = d->n_sph, nr = d->n_rect;
   BvhOutput bvh;
   build_bvh(prim_bounds(d), bvh);
-  if (bvh.max_depth + 1 > uint32_t(kShortStack + kSpillStack)) return bail(fail(c, NE_ERR_UNSUPPORTED, "BVH too deep"));
+  if (bvh.max_stack > uint32_t(kShortStack + kSpillStack)) return bail(fail(c, NE_ERR_UNSUPPORTED, "BVH too deep"));
   if (bvh.max_leaf > kItemMaxLeaf)
     return bail(fail(c, NE_ERR_UNSUPPORTED, "degenerate geometry: a BVH leaf holds more than 8 primitives"));
   std::vector<float> tris(size_t(nt) * 12), sph(size_t(ns) * 4), rects(size_t(nr) * 16);
@@ -467,15 +467,15 @@ ne_status ne_scene_upload(ne_ctx* c, const ne_scene_desc* d, ne_scene** out) {
     }
   }
   // leaf children: refs position -> storage slot of the leaf's type
-  for (size_t k = 0; k < bvh.nodes.size() / 16; ++k) {
-    float* f = &bvh.nodes[k * 16];
-    for (int side = 0; side < 2; ++side) {
+  for (size_t k = 0; k < bvh.nodes.size() / kNodeFloats; ++k) {
+    float* f = &bvh.nodes[k * kNodeFloats];
+    for (int ch = 0; ch < 4; ++ch) {
       int32_t first, ct;
-      std::memcpy(&first, &f[12 + side], 4);
-      std::memcpy(&ct, &f[14 + side], 4);
+      std::memcpy(&first, &f[24 + ch], 4);
+      std::memcpy(&ct, &f[28 + ch], 4);
       if (first < 0 || (uint32_t(ct) & kLeafCountMask) == 0) continue;
       int32_t slot = int32_t(storage[size_t(first)]);
-      std::memcpy(&f[12 + side], &slot, 4);
+      std::memcpy(&f[24 + ch], &slot, 4);
     }
   }
   // emissive primitives in the light-selection order: rectangles, spheres,
@@ -500,7 +500,7 @@ ne_status ne_scene_upload(ne_ctx* c, const ne_scene_desc* d, ne_scene** out) {
   if ((st = dev_upload(c, s, reinterpret_cast<const float4*>(bvh.nodes.data()), bvh.nodes.size() / 4, &f4)) != NE_OK)
     return bail(st);
   ds.nodes = f4;
-  ds.n_nodes = uint32_t(bvh.nodes.size() / 16);
+  ds.n_nodes = uint32_t(bvh.nodes.size() / kNodeFloats);
   if ((st = dev_upload(c, s, reinterpret_cast<const float4*>(tris.data()), tris.size() / 4, &ds.tris)) != NE_OK)
     return bail(st);
   if ((st = dev_upload(c, s, reinterpret_cast<const float4*>(sph.data()), sph.size() / 4, &ds.sph)) != NE_OK)

@@ -1,7 +1,7 @@
 // bvh_build.cpp -- host binned-SAH BVH builder producing the device layout
-// described in ne_scene.h (BVH2, both child boxes stored in the parent,
-// leaves referenced from the parent, preorder so a left inner child sits
-// right after its parent).
+// described in ne_scene.h: a binary SAH tree collapsed to 4-wide nodes that
+// store their children's boxes, leaves referenced from the parent, wide
+// nodes in preorder.
 //
 // Replaces the reference's midpoint-split builder (src/primitives/BVH.cpp:6-106);
 // traversal results (closest primitive, t) are the contract, not the tree
@@ -54,6 +54,7 @@ void build_bvh(const std::vector<PrimInput>& prims, BvhOutput& out) {
   out.nodes.clear();
   out.refs.clear();
   out.max_depth = 0;
+  out.max_stack = 0;
   out.max_leaf = 0;
   const size_t n = prims.size();
   if (n == 0) return;
@@ -180,72 +181,104 @@ void build_bvh(const std::vector<PrimInput>& prims, BvhOutput& out) {
     tasks.push_back({l, tk.start, mid, tk.depth + 1});
   }
 
-  // ---- emit: refs in leaf order, inner nodes in preorder ------------------
-  out.order.clear();
-  out.order.reserve(n);
-  std::vector<int32_t> out_index(bn.size(), -1);
-  // first pass: preorder numbering of inner nodes
-  std::vector<int32_t> st;
-  std::vector<int32_t> pre;
-  st.push_back(0);
-  while (!st.empty()) {
-    int32_t k = st.back();
-    st.pop_back();
-    const BNode& nd = bn[size_t(k)];
-    if (nd.left < 0) continue;
-    out_index[size_t(k)] = int32_t(pre.size());
-    pre.push_back(k);
-    st.push_back(nd.right);
-    st.push_back(nd.left);
-  }
-  const bool root_leaf = bn[0].left < 0;
-  size_t n_out = root_leaf ? 1 : pre.size();
-  out.nodes.assign(n_out * 16, 0.0f);
-  auto put_box = [&](float* f, int side, const Box& b) {
-    // side 0: f0 = (lmin, lmax.x) f1.xy = lmax.yz ; side 1: f1.zw = rmin.xy, f2 = (rmin.z, rmax)
-    if (side == 0) {
-      f[0] = b.mn[0]; f[1] = b.mn[1]; f[2] = b.mn[2]; f[3] = b.mx[0]; f[4] = b.mx[1]; f[5] = b.mx[2];
-    } else {
-      f[6] = b.mn[0]; f[7] = b.mn[1]; f[8] = b.mn[2]; f[9] = b.mx[0]; f[10] = b.mx[1]; f[11] = b.mx[2];
+  // ---- collapse to a 4-wide tree and emit -----------------------------------
+  // A wide node starts from its binary node's two children and opens the
+  // inner child of largest surface area until it has four children or only
+  // leaves.  Wide nodes are numbered in preorder; leaves are referenced from
+  // their parent's child slots, their primitive refs stored contiguously.
+  struct Wide {
+    int32_t bnode;
+    int32_t ch[4];
+    int n;
+  };
+  auto collapse = [&](int32_t k, Wide& w) {
+    w.bnode = k;
+    w.n = 0;
+    w.ch[w.n++] = bn[size_t(k)].left;
+    w.ch[w.n++] = bn[size_t(k)].right;
+    while (w.n < 4) {
+      int best = -1;
+      float best_area = -1.0f;
+      for (int i = 0; i < w.n; ++i) {
+        const BNode& c = bn[size_t(w.ch[i])];
+        if (c.left >= 0 && c.box.area() > best_area) { best_area = c.box.area(); best = i; }
+      }
+      if (best < 0) break;
+      const int32_t open = w.ch[best];
+      w.ch[best] = bn[size_t(open)].left;
+      w.ch[w.n++] = bn[size_t(open)].right;
     }
   };
-  auto put_int = [&](float* f, int slot, int32_t v) { std::memcpy(&f[12 + slot], &v, 4); };
-  auto emit_leaf = [&](const BNode& leaf, float* f, int side) {
+  out.order.clear();
+  out.order.reserve(n);
+  std::vector<Wide> wide;
+  std::vector<int32_t> out_index(bn.size(), -1);
+  const bool root_leaf = bn[0].left < 0;
+  if (!root_leaf) {
+    struct Item { int32_t k; uint32_t depth, pushes; };
+    std::vector<Item> st;
+    st.push_back({0, 1, 0});
+    while (!st.empty()) {
+      const Item it = st.back();
+      st.pop_back();
+      Wide w;
+      collapse(it.k, w);
+      out_index[size_t(it.k)] = int32_t(wide.size());
+      wide.push_back(w);
+      // a visit of this node pushes at most n - 1 children on the traversal stack
+      const uint32_t pushes = it.pushes + uint32_t(w.n - 1);
+      out.max_depth = std::max(out.max_depth, it.depth);
+      out.max_stack = std::max(out.max_stack, pushes);
+      for (int c = w.n - 1; c >= 0; --c)
+        if (bn[size_t(w.ch[c])].left >= 0) st.push_back({w.ch[c], it.depth + 1, pushes});
+    }
+  }
+  const size_t n_out = root_leaf ? 1 : wide.size();
+  out.nodes.assign(n_out * kNodeFloats, 0.0f);
+  auto put_box = [&](float* f, int c, const Box& b) {
+    f[c] = b.mn[0]; f[4 + c] = b.mn[1]; f[8 + c] = b.mn[2];
+    f[12 + c] = b.mx[0]; f[16 + c] = b.mx[1]; f[20 + c] = b.mx[2];
+  };
+  auto put_int = [&](float* f, int slot, int32_t v) { std::memcpy(&f[slot], &v, 4); };
+  auto emit_leaf = [&](const BNode& leaf, float* f, int c) {
     int32_t first = int32_t(out.refs.size());
     for (uint32_t i = leaf.start; i < leaf.start + leaf.count; ++i) {
       uint32_t p = idx[i];
       out.order.push_back(p);
       out.refs.push_back(p);  // patched to (type, storage) by the caller
     }
-    put_box(f, side, leaf.box);
-    put_int(f, side, first);
-    put_int(f, 2 + side, int32_t(leaf.count | (prims[idx[leaf.start]].type << kLeafTypeShift)));
+    put_box(f, c, leaf.box);
+    put_int(f, 24 + c, first);
+    put_int(f, 28 + c, int32_t(leaf.count | (prims[idx[leaf.start]].type << kLeafTypeShift)));
     out.max_leaf = std::max(out.max_leaf, leaf.count);
+  };
+  auto empty_slot = [&](float* f, int c) {
+    Box e;
+    e.reset();
+    put_box(f, c, e);
+    put_int(f, 24 + c, -1);
+    put_int(f, 28 + c, 0);
   };
   if (root_leaf) {
     float* f = &out.nodes[0];
     emit_leaf(bn[0], f, 0);
-    Box e;
-    e.reset();
-    put_box(f, 1, e);
-    put_int(f, 1, -1);
-    put_int(f, 3, 0);
+    for (int c = 1; c < 4; ++c) empty_slot(f, c);
     out.max_depth = 1;
+    out.max_stack = 0;
     return;
   }
-  for (size_t j = 0; j < pre.size(); ++j) {
-    const BNode& nd = bn[size_t(pre[j])];
-    float* f = &out.nodes[j * 16];
-    out.max_depth = std::max(out.max_depth, uint32_t(nd.depth + 1));
-    const int32_t ch[2] = {nd.left, nd.right};
-    for (int side = 0; side < 2; ++side) {
-      const BNode& c = bn[size_t(ch[side])];
-      if (c.left < 0) {
-        emit_leaf(c, f, side);
+  for (size_t j = 0; j < wide.size(); ++j) {
+    const Wide& w = wide[j];
+    float* f = &out.nodes[j * kNodeFloats];
+    for (int c = 0; c < 4; ++c) {
+      if (c >= w.n) { empty_slot(f, c); continue; }
+      const BNode& ch = bn[size_t(w.ch[c])];
+      if (ch.left < 0) {
+        emit_leaf(ch, f, c);
       } else {
-        put_box(f, side, c.box);
-        put_int(f, side, out_index[size_t(ch[side])]);
-        put_int(f, 2 + side, 0);
+        put_box(f, c, ch.box);
+        put_int(f, 24 + c, out_index[size_t(w.ch[c])]);
+        put_int(f, 28 + c, 0);
       }
     }
   }

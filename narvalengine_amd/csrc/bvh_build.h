@@ -14,12 +14,14 @@ struct PrimInput {
 // leaf child encoding in a node's count word: count | (type << kLeafTypeShift)
 constexpr uint32_t kLeafTypeShift = 28;
 constexpr uint32_t kLeafCountMask = (1u << kLeafTypeShift) - 1;
+constexpr int kNodeFloats = 32;  // 4-wide node: 8 x float4
 
 struct BvhOutput {
-  std::vector<float> nodes;     // 16 floats (4 x float4) per node, ne_scene.h layout
+  std::vector<float> nodes;     // kNodeFloats per node, ne_scene.h layout
   std::vector<uint32_t> refs;   // leaf slots -> input primitive index (leaves hold refs[first, first+count))
   std::vector<uint32_t> order;  // input primitive indices in leaf order
-  uint32_t max_depth = 0;       // inner-node levels (bounds the traversal stack)
+  uint32_t max_depth = 0;       // wide-node levels
+  uint32_t max_stack = 0;       // deepest traversal stack (children pushed along any root path)
   uint32_t max_leaf = 0;
 };
 

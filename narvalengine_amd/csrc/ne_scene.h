@@ -2,9 +2,11 @@
 // code and kernels).
 //
 // HBM layout (all arrays 16-byte aligned):
-//   nodes   BVH2, 64 B per node = 4 x float4:
-//             f0 = (lmin.xyz, lmax.x)   f1 = (lmax.yz, rmin.xy)
-//             f2 = (rmin.z, rmax.xyz)   f3 = int(left, right, lcount, rcount)
+//   nodes   4-wide BVH, 128 B per node = 8 x float4, the children's boxes
+//           as structure of arrays:
+//             f0 = min.x of children 0..3   f1 = min.y   f2 = min.z
+//             f3 = max.x                    f4 = max.y   f5 = max.z
+//             f6 = int child[4]             f7 = int count[4]
 //           a child whose count word is 0 is an inner node index; otherwise
 //           it is a leaf: count = word & 0x0FFFFFFF primitives of type
 //           word >> 28 (0 triangle, 1 sphere, 2 rectangle) at slots
@@ -27,8 +29,9 @@ enum { PRIM_TRI = 0, PRIM_SPHERE = 1, PRIM_RECT = 2 };
 constexpr int kLightPoint = 3;  // light-code type of a point light (not a BVH primitive)
 
 // BVH traversal stack: kShortStack entries per lane in LDS ([depth][lane]
-// layout, no bank conflicts), deeper entries spill to a private array.  The
-// builder bounds tree depth so the total never overflows.
+// layout, no bank conflicts), deeper entries spill to a private array.  Upload
+// rejects a tree whose deepest possible stack (children pushed along any root
+// path) exceeds the total.
 constexpr int kShortStack = 16;
 constexpr int kSpillStack = 48;
 

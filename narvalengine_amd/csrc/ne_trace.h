@@ -179,9 +179,9 @@ struct SurfHit {
 };
 
 
-// Closest (or, with ANY, first) surface hit in (tmin, tmax).  Nodes are
-// visited near child first; the far child goes on the lane's stack.  Leaves
-// are tested as soon as their box is hit.  Ties on t resolve to the lower
+// Closest (or, with ANY, first) surface hit in (tmin, tmax).  Children are
+// visited nearest first; the others go on the lane's stack.  Leaves are
+// tested as soon as their box is hit.  Ties on t resolve to the lower
 // (type, original index) key, as the oracle's linear scan does.
 // Resumable BVH traversal: the per-lane state of one ray's closest-hit (or
 // any-hit) query.  The current work item is either an inner node (its index)
@@ -219,6 +219,13 @@ NE_DEV void trav_init(TravState<R>& st, Vec3<R> o, Vec3<R> d, R tmin, R tmax) {
 // leaf child (first slot, count word = count | type << 28) -> work item
 NE_DEV uint32_t leaf_item(uint32_t first, uint32_t cw) {
   return kItemLeaf | ((cw >> 28) << 29) | (((cw & 0x0FFFFFFFu) - 1u) << kItemCountShift) | first;
+}
+
+template <typename R>
+NE_DEV void trav_push(TravState<R>& st, uint32_t item, uint32_t* lds, int lane_stride, uint32_t* spill) {
+  if (st.sp < kShortStack) lds[st.sp * lane_stride] = item;
+  else spill[st.sp - kShortStack] = item;
+  ++st.sp;
 }
 
 // Next item from the stack; true when the stack is empty (query complete).
@@ -271,29 +278,40 @@ NE_DEV bool trav_step(const DevScene& s, TravState<R>& st, uint32_t* lds, int la
     }
     return trav_pop(st, lds, lane_stride, spill);
   }
-  const float4* nd = s.nodes + 4 * size_t(it);
-  const float4 f0 = nd[0], f1 = nd[1], f2 = nd[2];
-  const int4 f3 = reinterpret_cast<const int4*>(nd)[3];
+  // 4-wide inner node: test the four child boxes, continue with the nearest
+  // hit child and push the others farthest first (entry distances sorted by a
+  // five-exchange network; misses sort last at +inf)
+  const float4* nd = s.nodes + 8 * size_t(it);
+  const float4 mnx = nd[0], mny = nd[1], mnz = nd[2], mxx = nd[3], mxy = nd[4], mxz = nd[5];
+  const int4 ch = reinterpret_cast<const int4*>(nd)[6];
+  const int4 cw = reinterpret_cast<const int4*>(nd)[7];
   cnt.node();
-  R tl, tr;
-  const bool hl = f3.x >= 0 && slab<R>(f0.x, f0.y, f0.z, f0.w, f1.x, f1.y, st.o, st.invd, st.best, tl);
-  const bool hr = f3.y >= 0 && slab<R>(f1.z, f1.w, f2.x, f2.y, f2.z, f2.w, st.o, st.invd, st.best, tr);
-  const uint32_t il = f3.z ? leaf_item(uint32_t(f3.x), uint32_t(f3.z)) : uint32_t(f3.x);
-  const uint32_t ir = f3.w ? leaf_item(uint32_t(f3.y), uint32_t(f3.w)) : uint32_t(f3.y);
-  if (hl && hr) {
-    const bool lnear = tl <= tr;
-    const uint32_t far = lnear ? ir : il;
-    if (st.sp < kShortStack) lds[st.sp * lane_stride] = far;
-    else spill[st.sp - kShortStack] = far;
-    ++st.sp;
-    st.cur = lnear ? il : ir;
-  } else if (hl) {
-    st.cur = il;
-  } else if (hr) {
-    st.cur = ir;
-  } else {
-    return trav_pop(st, lds, lane_stride, spill);
-  }
+  const R inf = xinf<R>();
+  R t0 = inf, t1 = inf, t2 = inf, t3 = inf, te;
+  if (ch.x >= 0 && slab<R>(mnx.x, mny.x, mnz.x, mxx.x, mxy.x, mxz.x, st.o, st.invd, st.best, te)) t0 = te;
+  if (ch.y >= 0 && slab<R>(mnx.y, mny.y, mnz.y, mxx.y, mxy.y, mxz.y, st.o, st.invd, st.best, te)) t1 = te;
+  if (ch.z >= 0 && slab<R>(mnx.z, mny.z, mnz.z, mxx.z, mxy.z, mxz.z, st.o, st.invd, st.best, te)) t2 = te;
+  if (ch.w >= 0 && slab<R>(mnx.w, mny.w, mnz.w, mxx.w, mxy.w, mxz.w, st.o, st.invd, st.best, te)) t3 = te;
+  uint32_t i0 = cw.x ? leaf_item(uint32_t(ch.x), uint32_t(cw.x)) : uint32_t(ch.x);
+  uint32_t i1 = cw.y ? leaf_item(uint32_t(ch.y), uint32_t(cw.y)) : uint32_t(ch.y);
+  uint32_t i2 = cw.z ? leaf_item(uint32_t(ch.z), uint32_t(cw.z)) : uint32_t(ch.z);
+  uint32_t i3 = cw.w ? leaf_item(uint32_t(ch.w), uint32_t(cw.w)) : uint32_t(ch.w);
+  auto order = [](R& ta, uint32_t& ia, R& tb, uint32_t& ib) {
+    if (tb < ta) {
+      const R t = ta; ta = tb; tb = t;
+      const uint32_t i = ia; ia = ib; ib = i;
+    }
+  };
+  order(t0, i0, t1, i1);
+  order(t2, i2, t3, i3);
+  order(t0, i0, t2, i2);
+  order(t1, i1, t3, i3);
+  order(t1, i1, t2, i2);
+  if (!(t0 < inf)) return trav_pop(st, lds, lane_stride, spill);
+  if (t3 < inf) trav_push(st, i3, lds, lane_stride, spill);
+  if (t2 < inf) trav_push(st, i2, lds, lane_stride, spill);
+  if (t1 < inf) trav_push(st, i1, lds, lane_stride, spill);
+  st.cur = i0;
   return false;
 }
 

@@ -81,7 +81,7 @@ def _bvh(scene):
     nn, nr, dep = C.c_uint32(), C.c_uint32(), C.c_uint32()
     L = abi.lib()
     assert L.ne_bvh_build(C.byref(d), None, None, C.byref(nn), C.byref(nr), C.byref(dep)) == 0
-    nodes = np.zeros((nn.value, 16), np.float32)
+    nodes = np.zeros((nn.value, 32), np.float32)
     refs = np.zeros(nr.value, np.uint32)
     assert L.ne_bvh_build(C.byref(d), nodes.ctypes.data_as(C.POINTER(C.c_float)),
                           refs.ctypes.data_as(C.POINTER(C.c_uint32)), C.byref(nn), C.byref(nr), C.byref(dep)) == 0
@@ -92,7 +92,7 @@ def _bvh(scene):
 def test_bvh_is_well_formed(which):
     s = scenes.triangle_soup() if which == "soup" else scenes.small_surface(subdiv=4)
     nodes, refs, depth, a = _bvh(s)
-    ints = nodes[:, 12:].view(np.int32)
+    ints = nodes[:, 24:].view(np.int32)  # child[4], count[4]
     n_prims = len(a["tri_v"]) + len(a["sph"]) + len(a["rect"])
     assert len(refs) == n_prims and depth <= 60
     # every primitive appears in exactly one leaf slot
@@ -111,7 +111,9 @@ def test_bvh_is_well_formed(which):
         cs = np.array([p[:3], p[:3] + p[3:6], p[:3] + p[6:9], p[:3] + p[3:6] + p[6:9]])
         return cs.min(0), cs.max(0)
 
-    boxes = {0: (nodes[:, 0:3], nodes[:, 3:6]), 1: (nodes[:, 6:9], nodes[:, 9:12])}
+    def child_box(k, c):  # structure of arrays: min.x[4], min.y[4], min.z[4], max.x[4], max.y[4], max.z[4]
+        return nodes[k, [c, 4 + c, 8 + c]], nodes[k, [12 + c, 16 + c, 20 + c]]
+
     visited = np.zeros(len(nodes), bool)
     stack = [0]
     slots = 0
@@ -119,11 +121,12 @@ def test_bvh_is_well_formed(which):
         k = stack.pop()
         assert not visited[k]
         visited[k] = True
-        for side in (0, 1):
+        assert (ints[k, :4] >= 0).sum() >= (1 if k == 0 else 2)  # a wide node has 2..4 children
+        for side in range(4):
             # leaf count word: count | (primitive type << 28); leaves hold one type
-            idx, word = ints[k, side], int(ints[k, 2 + side]) & 0xFFFFFFFF
+            idx, word = ints[k, side], int(ints[k, 4 + side]) & 0xFFFFFFFF
             cnt, leaf_type = word & 0x0FFFFFFF, word >> 28
-            lo, hi = boxes[side][0][k], boxes[side][1][k]
+            lo, hi = child_box(k, side)
             if idx < 0:
                 continue
             if cnt > 0:
@@ -135,8 +138,9 @@ def test_bvh_is_well_formed(which):
                     assert (pl >= lo - 1e-6).all() and (ph <= hi + 1e-6).all()
             else:
                 assert 0 < idx < len(nodes)
-                cl = np.minimum(nodes[idx, 0:3], nodes[idx, 6:9])
-                ch = np.maximum(nodes[idx, 3:6], nodes[idx, 9:12])
+                valid = [c for c in range(4) if ints[idx, c] >= 0]
+                cl = np.min([child_box(idx, c)[0] for c in valid], axis=0)
+                ch = np.max([child_box(idx, c)[1] for c in valid], axis=0)
                 assert (cl >= lo - 1e-6).all() and (ch <= hi + 1e-6).all()
                 stack.append(int(idx))
     assert visited.all() and slots == len(refs)
