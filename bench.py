@@ -33,8 +33,9 @@ HBM_PEAK_GBS = 8000.0
 PROFILES = ROOT / "profiles"
 
 # Algorithmic bytes per counted event: a trilinear density lookup reads 8 FP32
-# corners, a majorant cell 4 B, a BVH node record 128 B (four child boxes,
-# child references and counts), a primitive record 48 B (triangle: 3 x float4).  Ray-state traffic of the
+# corners, a majorant cell 4 B, a BVH node record 64 B (four quantized child
+# boxes, child references and leaf counts), a primitive record 48 B
+# (triangle: 3 x float4).  Ray-state traffic of the
 # wavefront stages per ray (structure-of-arrays records):
 #   trace  reads origin + direction (24 B), writes hit t / ref / kind (12 B)
 #   medium reads origin, direction, hit t, pixel, sample, RNG count (40 B),
@@ -43,7 +44,7 @@ PROFILES = ROOT / "profiles"
 #   shade  reads the 84 B path record, writes 64 B per continuation and per
 #          shadow ray
 # The film costs 24 B read + 24 B write per pixel (FP64 RGB), once per frame.
-EV = {"density_lookups": 32, "majorant_cells": 4, "bvh_nodes": 128, "prim_tests": 48}
+EV = {"density_lookups": 32, "majorant_cells": 4, "bvh_nodes": 64, "prim_tests": 48}
 RAY_BYTES = {"trace": 36, "medium": 52, "shadow": 88}
 SHADE_IN, SHADE_OUT = 84, 64
 

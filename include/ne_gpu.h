@@ -394,12 +394,14 @@ ne_status ne_test_rng(ne_ctx* ctx, uint64_t seed, uint32_t pixel, uint32_t sampl
                       float* out);
 
 /* Host-only: builds the BVH that ne_scene_upload would build for desc and
- * returns it (no device needed).  nodes gets 32 floats per 4-wide node in the
- * device layout: child boxes as min.x[4], min.y[4], min.z[4], max.x[4],
- * max.y[4], max.z[4], then int32 child[4] (inner node index, leaf first slot,
- * or -1 for an empty slot) and int32 count[4] (0 for an inner child, else
- * count | type << 28); refs one (type << 30 | original primitive index) per
- * leaf slot; depth the number of wide-node levels.
+ * returns it (no device needed).  nodes gets 16 words per 4-wide node in the
+ * device layout: float origin[3]; uint32 exponents (byte a = e_a + 127);
+ * uint32 qlo.x, qlo.y, qlo.z, qhi.x, qhi.y, qhi.z (one byte per child: child
+ * c's box on axis a is [origin_a + qlo_a[c] 2^e_a, origin_a + qhi_a[c] 2^e_a]
+ * in FP32); uint32 meta (byte c: 0 inner child, else 0x80 | type << 4 |
+ * count); uint32 0; int32 child[4] (inner node index, leaf first slot, or -1
+ * for an empty slot).  refs gets one (type << 30 | original primitive index)
+ * per leaf slot; depth the number of wide-node levels.
  * Call with NULL nodes/refs to query n_nodes / n_refs first. */
 ne_status ne_bvh_build(const ne_scene_desc* desc, float* nodes, uint32_t* refs, uint32_t* n_nodes, uint32_t* n_refs,
                        uint32_t* depth);

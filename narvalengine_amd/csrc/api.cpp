@@ -469,13 +469,14 @@ ne_status ne_scene_upload(ne_ctx* c, const ne_scene_desc* d, ne_scene** out) {
   // leaf children: refs position -> storage slot of the leaf's type
   for (size_t k = 0; k < bvh.nodes.size() / kNodeFloats; ++k) {
     float* f = &bvh.nodes[k * kNodeFloats];
+    uint32_t meta;
+    std::memcpy(&meta, &f[10], 4);
     for (int ch = 0; ch < 4; ++ch) {
-      int32_t first, ct;
-      std::memcpy(&first, &f[24 + ch], 4);
-      std::memcpy(&ct, &f[28 + ch], 4);
-      if (first < 0 || (uint32_t(ct) & kLeafCountMask) == 0) continue;
+      int32_t first;
+      std::memcpy(&first, &f[12 + ch], 4);
+      if (first < 0 || ((meta >> (8 * ch)) & kMetaLeaf) == 0) continue;
       int32_t slot = int32_t(storage[size_t(first)]);
-      std::memcpy(&f[24 + ch], &slot, 4);
+      std::memcpy(&f[12 + ch], &slot, 4);
     }
   }
   // emissive primitives in the light-selection order: rectangles, spheres,

@@ -1,6 +1,8 @@
 // bvh_build.cpp -- host binned-SAH BVH builder producing the device layout
 // described in ne_scene.h: a binary SAH tree collapsed to 4-wide nodes that
-// store their children's boxes, leaves referenced from the parent, wide
+// store their children's boxes quantized to 8 bits per plane against the
+// node's own box (conservatively: the FP32 decode the traversal performs
+// always contains the true box), leaves referenced from the parent, wide
 // nodes in preorder.
 //
 // Replaces the reference's midpoint-split builder (src/primitives/BVH.cpp:6-106);
@@ -235,52 +237,95 @@ void build_bvh(const std::vector<PrimInput>& prims, BvhOutput& out) {
   }
   const size_t n_out = root_leaf ? 1 : wide.size();
   out.nodes.assign(n_out * kNodeFloats, 0.0f);
-  auto put_box = [&](float* f, int c, const Box& b) {
-    f[c] = b.mn[0]; f[4 + c] = b.mn[1]; f[8 + c] = b.mn[2];
-    f[12 + c] = b.mx[0]; f[16 + c] = b.mx[1]; f[20 + c] = b.mx[2];
+  struct Child {
+    Box box;
+    int32_t ref = -1;   // inner node index, leaf first ref, or -1
+    uint32_t meta = 0;  // 0 inner/empty; leaf: 0x80 | type << 4 | count
   };
-  auto put_int = [&](float* f, int slot, int32_t v) { std::memcpy(&f[slot], &v, 4); };
-  auto emit_leaf = [&](const BNode& leaf, float* f, int c) {
-    int32_t first = int32_t(out.refs.size());
+  auto leaf_child = [&](const BNode& leaf) {
+    Child c;
+    c.box = leaf.box;
+    c.ref = int32_t(out.refs.size());
     for (uint32_t i = leaf.start; i < leaf.start + leaf.count; ++i) {
       uint32_t p = idx[i];
       out.order.push_back(p);
       out.refs.push_back(p);  // patched to (type, storage) by the caller
     }
-    put_box(f, c, leaf.box);
-    put_int(f, 24 + c, first);
-    put_int(f, 28 + c, int32_t(leaf.count | (prims[idx[leaf.start]].type << kLeafTypeShift)));
+    c.meta = kMetaLeaf | (prims[idx[leaf.start]].type << 4) | leaf.count;
     out.max_leaf = std::max(out.max_leaf, leaf.count);
+    return c;
   };
-  auto empty_slot = [&](float* f, int c) {
-    Box e;
-    e.reset();
-    put_box(f, c, e);
-    put_int(f, 24 + c, -1);
-    put_int(f, 28 + c, 0);
+  auto encode = [&](float* f, const Child* ch, int n) {
+    Box parent;
+    parent.reset();
+    for (int c = 0; c < n; ++c) parent.grow(ch[c].box);
+    uint32_t q[6] = {0, 0, 0, 0, 0, 0};  // lo.x lo.y lo.z hi.x hi.y hi.z, one byte per child
+    uint32_t exps = 0;
+    for (int a = 0; a < 3; ++a) {
+      const float origin = parent.mn[a];
+      f[a] = origin;
+      const double ext = double(parent.mx[a]) - double(origin);
+      int e = ext > 0 ? int(std::ceil(std::log2(ext / 254.0))) : -126;
+      e = std::min(std::max(e, -126), 127);
+      // smallest scale 2^e whose decoded planes origin + q * 2^e (FP32, as the
+      // traversal computes them) contain every child box
+      for (;; ++e) {
+        const float sc = std::ldexp(1.0f, e);
+        bool ok = true;
+        uint32_t lo[4] = {0, 0, 0, 0}, hi[4] = {0, 0, 0, 0};
+        for (int c = 0; c < n && ok; ++c) {
+          const float bl = ch[c].box.mn[a], bh = ch[c].box.mx[a];
+          double ql = std::floor((double(bl) - double(origin)) / double(sc));
+          double qh = std::ceil((double(bh) - double(origin)) / double(sc));
+          int il = int(std::min(255.0, std::max(0.0, ql))), ih = int(std::min(255.0, std::max(0.0, qh)));
+          while (il > 0 && origin + float(il) * sc > bl) --il;
+          while (ih < 255 && origin + float(ih) * sc < bh) ++ih;
+          ok = origin + float(il) * sc <= bl && origin + float(ih) * sc >= bh;
+          lo[c] = uint32_t(il);
+          hi[c] = uint32_t(ih);
+        }
+        if (ok || e == 127) {
+          for (int c = 0; c < n; ++c) {
+            q[a] |= lo[c] << (8 * c);
+            q[3 + a] |= hi[c] << (8 * c);
+          }
+          break;
+        }
+      }
+      exps |= uint32_t(e + 127) << (8 * a);
+    }
+    uint32_t meta = 0;
+    int32_t refs[4] = {-1, -1, -1, -1};
+    for (int c = 0; c < n; ++c) {
+      meta |= ch[c].meta << (8 * c);
+      refs[c] = ch[c].ref;
+    }
+    std::memcpy(&f[3], &exps, 4);
+    std::memcpy(&f[4], &q[0], 16);  // lo.x lo.y lo.z hi.x
+    std::memcpy(&f[8], &q[4], 8);   // hi.y hi.z
+    std::memcpy(&f[10], &meta, 4);
+    std::memcpy(&f[12], refs, 16);
   };
   if (root_leaf) {
-    float* f = &out.nodes[0];
-    emit_leaf(bn[0], f, 0);
-    for (int c = 1; c < 4; ++c) empty_slot(f, c);
+    const Child c = leaf_child(bn[0]);
+    encode(&out.nodes[0], &c, 1);
     out.max_depth = 1;
     out.max_stack = 0;
     return;
   }
   for (size_t j = 0; j < wide.size(); ++j) {
     const Wide& w = wide[j];
-    float* f = &out.nodes[j * kNodeFloats];
-    for (int c = 0; c < 4; ++c) {
-      if (c >= w.n) { empty_slot(f, c); continue; }
-      const BNode& ch = bn[size_t(w.ch[c])];
-      if (ch.left < 0) {
-        emit_leaf(ch, f, c);
+    Child ch[4];
+    for (int c = 0; c < w.n; ++c) {
+      const BNode& b = bn[size_t(w.ch[c])];
+      if (b.left < 0) {
+        ch[c] = leaf_child(b);
       } else {
-        put_box(f, c, ch.box);
-        put_int(f, 24 + c, out_index[size_t(w.ch[c])]);
-        put_int(f, 28 + c, 0);
+        ch[c].box = b.box;
+        ch[c].ref = out_index[size_t(w.ch[c])];
       }
     }
+    encode(&out.nodes[j * kNodeFloats], ch, w.n);
   }
 }
 

@@ -11,10 +11,8 @@ struct PrimInput {
   uint32_t type;  // leaves never mix types
 };
 
-// leaf child encoding in a node's count word: count | (type << kLeafTypeShift)
-constexpr uint32_t kLeafTypeShift = 28;
-constexpr uint32_t kLeafCountMask = (1u << kLeafTypeShift) - 1;
-constexpr int kNodeFloats = 32;  // 4-wide node: 8 x float4
+constexpr int kNodeFloats = 16;   // 4-wide quantized node: 4 x float4 (64 B)
+constexpr uint32_t kMetaLeaf = 0x80;  // child meta byte: 0x80 | type << 4 | count for a leaf, 0 otherwise
 
 struct BvhOutput {
   std::vector<float> nodes;     // kNodeFloats per node, ne_scene.h layout

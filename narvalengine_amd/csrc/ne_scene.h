@@ -2,16 +2,18 @@
 // code and kernels).
 //
 // HBM layout (all arrays 16-byte aligned):
-//   nodes   4-wide BVH, 128 B per node = 8 x float4, the children's boxes
-//           as structure of arrays:
-//             f0 = min.x of children 0..3   f1 = min.y   f2 = min.z
-//             f3 = max.x                    f4 = max.y   f5 = max.z
-//             f6 = int child[4]             f7 = int count[4]
-//           a child whose count word is 0 is an inner node index; otherwise
-//           it is a leaf: count = word & 0x0FFFFFFF primitives of type
-//           word >> 28 (0 triangle, 1 sphere, 2 rectangle) at slots
-//           [index, index + count) of that type's array.  index < 0 marks an
-//           empty slot.
+//   nodes   4-wide BVH, 64 B per node = 4 x float4, child boxes quantized
+//           to 8 bits per plane against the node's box:
+//             f0 = (origin.xyz, exponents: byte a = e_a + 127)
+//             f1 = (qlo.x, qlo.y, qlo.z, qhi.x)  f2 = (qhi.y, qhi.z, meta, 0)
+//             f3 = int child[4]
+//           each q word holds one byte per child; child c's box on axis a is
+//           [origin_a + qlo_a[c] * 2^e_a, origin_a + qhi_a[c] * 2^e_a] in FP32
+//           (the builder makes that decode contain the true box).  meta byte
+//           c is 0 for an inner child (child[c] = node index) or
+//           0x80 | type << 4 | count for a leaf of count primitives of type
+//           (0 triangle, 1 sphere, 2 rectangle) at slots [child[c],
+//           child[c] + count) of that type's array.  child < 0: empty slot.
 //   tris    48 B per triangle in leaf order: (v0, material), (v1-v0, original
 //           index), (v2-v0, 0).  Original index = position in the input.
 //   sph     (centre, radius) in leaf order + material and original-index

@@ -216,9 +216,14 @@ NE_DEV void trav_init(TravState<R>& st, Vec3<R> o, Vec3<R> d, R tmin, R tmax) {
   st.bslot = 0;
 }
 
-// leaf child (first slot, count word = count | type << 28) -> work item
-NE_DEV uint32_t leaf_item(uint32_t first, uint32_t cw) {
-  return kItemLeaf | ((cw >> 28) << 29) | (((cw & 0x0FFFFFFFu) - 1u) << kItemCountShift) | first;
+// Byte c of a packed word of quantized planes, as a float.
+NE_DEV float qbyte(uint32_t w, int c) { return float((w >> (8 * c)) & 0xFFu); }
+// Child c of a wide node -> work item: an inner node index, or for a leaf
+// (meta byte 0x80 | type << 4 | count) the packed leaf item.
+NE_DEV uint32_t child_item(uint32_t ref, uint32_t meta, int c) {
+  const uint32_t m = (meta >> (8 * c)) & 0xFFu;
+  if (!(m & 0x80u)) return ref;
+  return kItemLeaf | (((m >> 4) & 3u) << 29) | (((m & 15u) - 1u) << kItemCountShift) | ref;
 }
 
 template <typename R>
@@ -278,24 +283,35 @@ NE_DEV bool trav_step(const DevScene& s, TravState<R>& st, uint32_t* lds, int la
     }
     return trav_pop(st, lds, lane_stride, spill);
   }
-  // 4-wide inner node: test the four child boxes, continue with the nearest
-  // hit child and push the others farthest first (entry distances sorted by a
-  // five-exchange network; misses sort last at +inf)
-  const float4* nd = s.nodes + 8 * size_t(it);
-  const float4 mnx = nd[0], mny = nd[1], mnz = nd[2], mxx = nd[3], mxy = nd[4], mxz = nd[5];
-  const int4 ch = reinterpret_cast<const int4*>(nd)[6];
-  const int4 cw = reinterpret_cast<const int4*>(nd)[7];
+  // 4-wide inner node (64 B): decode the four quantized child boxes, test
+  // them, continue with the nearest hit child and push the others farthest
+  // first (entry distances sorted by a five-exchange network; misses sort
+  // last at +inf)
+  const float4* nd = s.nodes + 4 * size_t(it);
+  const float4 f0 = nd[0];
+  const uint4 f1 = reinterpret_cast<const uint4*>(nd)[1], f2 = reinterpret_cast<const uint4*>(nd)[2];
+  const int4 ch = reinterpret_cast<const int4*>(nd)[3];
   cnt.node();
+  const uint32_t ex = __float_as_uint(f0.w);
+  const float sx = __uint_as_float((ex & 0xFFu) << 23), sy = __uint_as_float(((ex >> 8) & 0xFFu) << 23),
+              sz = __uint_as_float(((ex >> 16) & 0xFFu) << 23);
   const R inf = xinf<R>();
   R t0 = inf, t1 = inf, t2 = inf, t3 = inf, te;
-  if (ch.x >= 0 && slab<R>(mnx.x, mny.x, mnz.x, mxx.x, mxy.x, mxz.x, st.o, st.invd, st.best, te)) t0 = te;
-  if (ch.y >= 0 && slab<R>(mnx.y, mny.y, mnz.y, mxx.y, mxy.y, mxz.y, st.o, st.invd, st.best, te)) t1 = te;
-  if (ch.z >= 0 && slab<R>(mnx.z, mny.z, mnz.z, mxx.z, mxy.z, mxz.z, st.o, st.invd, st.best, te)) t2 = te;
-  if (ch.w >= 0 && slab<R>(mnx.w, mny.w, mnz.w, mxx.w, mxy.w, mxz.w, st.o, st.invd, st.best, te)) t3 = te;
-  uint32_t i0 = cw.x ? leaf_item(uint32_t(ch.x), uint32_t(cw.x)) : uint32_t(ch.x);
-  uint32_t i1 = cw.y ? leaf_item(uint32_t(ch.y), uint32_t(cw.y)) : uint32_t(ch.y);
-  uint32_t i2 = cw.z ? leaf_item(uint32_t(ch.z), uint32_t(cw.z)) : uint32_t(ch.z);
-  uint32_t i3 = cw.w ? leaf_item(uint32_t(ch.w), uint32_t(cw.w)) : uint32_t(ch.w);
+#define NE_CHILD_SLAB(c, ref, tout)                                                                           \
+  if ((ref) >= 0 &&                                                                                         \
+      slab<R>(f0.x + qbyte(f1.x, c) * sx, f0.y + qbyte(f1.y, c) * sy, f0.z + qbyte(f1.z, c) * sz,           \
+              f0.x + qbyte(f1.w, c) * sx, f0.y + qbyte(f2.x, c) * sy, f0.z + qbyte(f2.y, c) * sz, st.o,    \
+              st.invd, st.best, te))                                                                        \
+    tout = te;
+  NE_CHILD_SLAB(0, ch.x, t0)
+  NE_CHILD_SLAB(1, ch.y, t1)
+  NE_CHILD_SLAB(2, ch.z, t2)
+  NE_CHILD_SLAB(3, ch.w, t3)
+#undef NE_CHILD_SLAB
+  uint32_t i0 = child_item(uint32_t(ch.x), f2.z, 0);
+  uint32_t i1 = child_item(uint32_t(ch.y), f2.z, 1);
+  uint32_t i2 = child_item(uint32_t(ch.z), f2.z, 2);
+  uint32_t i3 = child_item(uint32_t(ch.w), f2.z, 3);
   auto order = [](R& ta, uint32_t& ia, R& tb, uint32_t& ib) {
     if (tb < ta) {
       const R t = ta; ta = tb; tb = t;

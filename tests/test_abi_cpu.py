@@ -81,7 +81,7 @@ def _bvh(scene):
     nn, nr, dep = C.c_uint32(), C.c_uint32(), C.c_uint32()
     L = abi.lib()
     assert L.ne_bvh_build(C.byref(d), None, None, C.byref(nn), C.byref(nr), C.byref(dep)) == 0
-    nodes = np.zeros((nn.value, 32), np.float32)
+    nodes = np.zeros((nn.value, 16), np.float32)
     refs = np.zeros(nr.value, np.uint32)
     assert L.ne_bvh_build(C.byref(d), nodes.ctypes.data_as(C.POINTER(C.c_float)),
                           refs.ctypes.data_as(C.POINTER(C.c_uint32)), C.byref(nn), C.byref(nr), C.byref(dep)) == 0
@@ -92,7 +92,8 @@ def _bvh(scene):
 def test_bvh_is_well_formed(which):
     s = scenes.triangle_soup() if which == "soup" else scenes.small_surface(subdiv=4)
     nodes, refs, depth, a = _bvh(s)
-    ints = nodes[:, 24:].view(np.int32)  # child[4], count[4]
+    words = nodes.view(np.uint32)
+    ints = nodes[:, 12:].view(np.int32)  # child[4]
     n_prims = len(a["tri_v"]) + len(a["sph"]) + len(a["rect"])
     assert len(refs) == n_prims and depth <= 60
     # every primitive appears in exactly one leaf slot
@@ -111,38 +112,50 @@ def test_bvh_is_well_formed(which):
         cs = np.array([p[:3], p[:3] + p[3:6], p[:3] + p[6:9], p[:3] + p[3:6] + p[6:9]])
         return cs.min(0), cs.max(0)
 
-    def child_box(k, c):  # structure of arrays: min.x[4], min.y[4], min.z[4], max.x[4], max.y[4], max.z[4]
-        return nodes[k, [c, 4 + c, 8 + c]], nodes[k, [12 + c, 16 + c, 20 + c]]
+    def child_box(k, c):
+        # FP32 decode as the traversal does it: origin + byte * 2^e per plane
+        origin = nodes[k, 0:3]
+        scale = np.array([np.ldexp(np.float32(1), int((words[k, 3] >> (8 * a)) & 255) - 127) for a in range(3)],
+                         np.float32)
+        q = lambda w: np.float32((int(words[k, w]) >> (8 * c)) & 255)
+        lo = np.array([origin[a] + q(4 + a) * scale[a] for a in range(3)], np.float32)
+        hi = np.array([origin[a] + q((7, 8, 9)[a]) * scale[a] for a in range(3)], np.float32)
+        return lo, hi
 
     visited = np.zeros(len(nodes), bool)
-    stack = [0]
     slots = 0
-    while stack:
-        k = stack.pop()
+
+    def subtree(k):
+        """Visits node k; returns the union of its primitives' true boxes after
+        checking that every decoded child box contains its subtree's primitives."""
+        nonlocal slots
         assert not visited[k]
         visited[k] = True
         assert (ints[k, :4] >= 0).sum() >= (1 if k == 0 else 2)  # a wide node has 2..4 children
+        ulo, uhi = np.full(3, np.inf), np.full(3, -np.inf)
         for side in range(4):
-            # leaf count word: count | (primitive type << 28); leaves hold one type
-            idx, word = ints[k, side], int(ints[k, 4 + side]) & 0xFFFFFFFF
-            cnt, leaf_type = word & 0x0FFFFFFF, word >> 28
-            lo, hi = child_box(k, side)
+            # leaf meta byte: 0x80 | primitive type << 4 | count; leaves hold one type
+            idx, meta = ints[k, side], (int(words[k, 10]) >> (8 * side)) & 255
+            cnt, leaf_type = (meta & 15) if meta & 0x80 else 0, (meta >> 4) & 3
             if idx < 0:
                 continue
+            lo, hi = child_box(k, side)
             if cnt > 0:
                 assert 0 <= idx and idx + cnt <= len(refs)
                 slots += cnt
+                boxes = []
                 for r in refs[idx:idx + cnt]:
                     assert int(r >> 30) == leaf_type
-                    pl, ph = prim_box(r)
-                    assert (pl >= lo - 1e-6).all() and (ph <= hi + 1e-6).all()
+                    boxes.append(prim_box(r))
+                clo, chi = np.min([b[0] for b in boxes], 0), np.max([b[1] for b in boxes], 0)
             else:
                 assert 0 < idx < len(nodes)
-                valid = [c for c in range(4) if ints[idx, c] >= 0]
-                cl = np.min([child_box(idx, c)[0] for c in valid], axis=0)
-                ch = np.max([child_box(idx, c)[1] for c in valid], axis=0)
-                assert (cl >= lo - 1e-6).all() and (ch <= hi + 1e-6).all()
-                stack.append(int(idx))
+                clo, chi = subtree(int(idx))
+            assert (clo >= lo).all() and (chi <= hi).all()  # conservative quantization
+            ulo, uhi = np.minimum(ulo, clo), np.maximum(uhi, chi)
+        return ulo, uhi
+
+    subtree(0)
     assert visited.all() and slots == len(refs)
 
 
