@@ -157,12 +157,14 @@ hipError_t launch_tonemap(const double* film, int npix, float inv_spp, float* pi
 }
 
 // ---------------------------------------------------------------------------
-// majorant grid: one block per cell, max over the cell's trilinear support
-// (the cell's voxels plus a one-voxel apron per side) times sigma_bar.
+// majorant grid: `split` blocks per cell, max over the cell's trilinear
+// support (the cell's voxels plus a one-voxel apron per side) times
+// sigma_bar.  The blocks of a cell combine with an unsigned atomicMax, which
+// orders non-negative floats like their values (maj is zeroed first).
 // ---------------------------------------------------------------------------
-__global__ __launch_bounds__(256) void k_majorants(DevScene s, float* maj) {
+__global__ __launch_bounds__(256) void k_majorants(DevScene s, float* maj, int split) {
   const int M = s.mres;
-  const int cell = blockIdx.x;
+  const int cell = int(blockIdx.x) / split, part = int(blockIdx.x) % split;
   const int c[3] = {cell % M, (cell / M) % M, cell / (M * M)};
   int lo[3], hi[3];
   for (int a = 0; a < 3; ++a) {
@@ -172,11 +174,11 @@ __global__ __launch_bounds__(256) void k_majorants(DevScene s, float* maj) {
     int h = int((int64_t(c[a] + 1) * r + M - 1) / M) + 1;
     hi[a] = h < int(r) - 1 ? h : int(r) - 1;
   }
-  const int nx = hi[0] - lo[0] + 1, ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1;
-  const int total = nx * ny * nz;
+  const int64_t nx = hi[0] - lo[0] + 1, ny = hi[1] - lo[1] + 1, nz = hi[2] - lo[2] + 1;
+  const int64_t total = nx * ny * nz;
   float m = 0.0f;
-  for (int i = threadIdx.x; i < total; i += blockDim.x) {
-    int x = lo[0] + i % nx, y = lo[1] + (i / nx) % ny, z = lo[2] + i / (nx * ny);
+  for (int64_t i = int64_t(part) * blockDim.x + threadIdx.x; i < total; i += int64_t(split) * blockDim.x) {
+    const int64_t x = lo[0] + i % nx, y = lo[1] + (i / nx) % ny, z = lo[2] + i / (nx * ny);
     m = xmax(m, s.grid[size_t(z) * size_t(s.res[0]) * size_t(s.res[1]) + size_t(y) * size_t(s.res[0]) + size_t(x)]);
   }
   __shared__ float red[256];
@@ -186,11 +188,15 @@ __global__ __launch_bounds__(256) void k_majorants(DevScene s, float* maj) {
     if (int(threadIdx.x) < w) red[threadIdx.x] = xmax(red[threadIdx.x], red[threadIdx.x + w]);
     __syncthreads();
   }
-  if (threadIdx.x == 0) maj[cell] = red[0] * s.sigma_bar;
+  if (threadIdx.x == 0) atomicMax(reinterpret_cast<unsigned int*>(maj + cell), __float_as_uint(red[0] * s.sigma_bar));
 }
 
 hipError_t launch_majorants(const DevScene& s, float* maj, hipStream_t st) {
-  hipLaunchKernelGGL(k_majorants, dim3(uint32_t(s.mres * s.mres * s.mres)), dim3(256), 0, st, s, maj);
+  const uint32_t cells = uint32_t(s.mres) * uint32_t(s.mres) * uint32_t(s.mres);
+  const int split = int(cells >= 2048u ? 1u : 2048u / cells);  // at least ~2048 blocks
+  hipError_t e = hipMemsetAsync(maj, 0, cells * sizeof(float), st);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(k_majorants, dim3(cells * uint32_t(split)), dim3(256), 0, st, s, maj, split);
   return hipGetLastError();
 }
 
