@@ -668,8 +668,8 @@ NE_DEV int env_texel(const DevScene& s, Vec3<R> d, R& sin_t) {
   const R theta = xacos(z);
   int i = int(phi * R(1.0 / kTWO_PI) * R(s.env_w));
   int j = int(theta * R(1.0 / kPI) * R(s.env_h));
-  i = i < s.env_w - 1 ? i : s.env_w - 1;
-  j = j < s.env_h - 1 ? j : s.env_h - 1;
+  i = i < s.env_w - 1 ? (i > 0 ? i : 0) : s.env_w - 1;  // also keeps a NaN direction in range
+  j = j < s.env_h - 1 ? (j > 0 ? j : 0) : s.env_h - 1;
   return j * s.env_w + i;
 }
 // Largest k in [0, n) with cdf[k] <= u (cdf[0] = 0, cdf[n] = 1, u in [0, 1)).

@@ -705,7 +705,8 @@ __device__ __forceinline__ void shadow_deliver(const WfShadow& S, const WfPaths&
 }
 
 // Shadow rays of scenes without a volume: any-hit BVH occlusion with lane
-// refill over the block's pool of shadow-queue chunks.
+// refill over the block's pool of shadow-queue chunks, then delivery of the
+// unoccluded rays' contributions.
 template <bool C>
 __global__ __launch_bounds__(kChunk) void k_wf_shadow_surf(WfState w, WfArgs a, DevScene s) {
   __shared__ uint32_t lds_stack[kShortStack * kChunk];
@@ -730,8 +731,19 @@ __global__ __launch_bounds__(kChunk) void k_wf_shadow_surf(WfState w, WfArgs a, 
       trav_init(st, v3(S.ox[k], S.oy[k], S.oz[k]), v3(S.dx[k], S.dy[k], S.dz[k]), 1e-4f, S.tmax[k]);
     }
     if (trav_step<float, true, C>(s, st, stk, kChunk, spill, cnt)) {
-      if (!st.found) shadow_deliver(S, B, a, s, k, 1.0f);
+      if (st.found) S.tmax[k] = -1.0f;  // occluded: marked for the delivery pass
       have = false;
+    }
+  }
+  // Delivery pass over the block's chunks, one queue entry per thread in
+  // order: coalesced, and out of the refill loop, where a finishing lane
+  // would hold its whole wave through the delivery.
+  __syncthreads();
+  for (uint32_t j = 0; j < pool.n; ++j) {
+    const uint32_t ch = pool_chunk(j), n = pool.pre[j + 1] - pool.pre[j];
+    for (uint32_t t = threadIdx.x; t < n; t += kChunk) {
+      const uint32_t q = ch * kChunk + t;
+      if (S.tmax[q] >= 0.0f) shadow_deliver(S, B, a, s, q, 1.0f);
     }
   }
   if constexpr (C) {
