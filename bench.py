@@ -248,8 +248,13 @@ def bench_guided(args):
     guided, t_guided = timed_render(guided_spp, 3, guide=guide, train=True)
     st = guide.stats()
     log(f"guided: {guided_spp} spp in {t_guided * 1e3:.1f} ms, guide {st}")
-    # stage times and proposal queries of a guided frame (trained guide, no training)
+    # stage times and event counts of a plain and a guided frame (trained
+    # guide, no training) at 256 spp
     f = ctx.film(W, H)
+    ctx.render(g_s, wl.camera, f, spp=256, seed=4, stage_timed=True, **kw)
+    plain_stages = ctx.stage_times()
+    ctx.render(g_s, wl.camera, f, spp=256, seed=4, count=True, **kw)
+    plain_cnt = ctx.counters()
     ctx.render(g_s, wl.camera, f, spp=256, seed=4, guide=guide, stage_timed=True, **kw)
     stages = ctx.stage_times()
     ctx.render(g_s, wl.camera, f, spp=256, seed=4, guide=guide, count=True, **kw)
@@ -273,6 +278,9 @@ def bench_guided(args):
         "relmse_plain": e_plain, "relmse_guided": e_guided, "relmse_ratio_at_equal_spp": round(per_sample, 4),
         "ms_plain": round(t_plain * 1e3, 2), "ms_guided_incl_training": round(t_guided * 1e3, 2),
         "guide": st, "guided_stage_ms": {k: round(v[0], 2) for k, v in stages.items() if v[1]},
+        "plain_stage_ms": {k: round(v[0], 2) for k, v in plain_stages.items() if v[1]},
+        "ext_rays_per_frame": {"plain": plain_cnt["ext_rays"], "guided": cnt["ext_rays"]},
+        "density_lookups_per_frame": {"plain": plain_cnt["density_lookups"], "guided": cnt["density_lookups"]},
         "guide_queries_per_frame": cnt["guide_queries"],
         "roofline": roofline,
     }

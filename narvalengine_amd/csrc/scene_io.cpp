@@ -15,6 +15,7 @@
 // the ABI; nothing throws across it.
 #include <algorithm>
 #include <cctype>
+#include <charconv>
 #include <cerrno>
 #include <cmath>
 #include <cstdio>
@@ -217,9 +218,9 @@ class JsonParser {
       if (!digit(p_)) error("invalid number");
       while (digit(p_)) ++p_;
     }
-    std::string tok(s, p_);
     v.kind = Json::kNum;
-    v.num = std::strtod(tok.c_str(), nullptr);
+    const auto r = std::from_chars(s, p_, v.num);  // locale-independent, like rapidjson
+    if (r.ec == std::errc::result_out_of_range) error("number out of range");
   }
 };
 
@@ -268,12 +269,15 @@ struct Tokens {
   }
 };
 
+// Locale-independent float token (decimal point '.'), read as a double and
+// rounded to float (values beyond the float range become +-inf or 0).
 bool parse_float(const char* b, const char* q, float& out) {
-  std::string tok(b, q);
-  char* end = nullptr;
-  errno = 0;
-  out = std::strtof(tok.c_str(), &end);
-  return end == tok.c_str() + tok.size() && !tok.empty();
+  if (b < q && *b == '+') ++b;  // from_chars takes no leading '+'
+  double d = 0.0;
+  const auto r = std::from_chars(b, q, d);
+  if (b >= q || r.ptr != q || (r.ec != std::errc() && r.ec != std::errc::result_out_of_range)) return false;
+  out = float(d);
+  return true;
 }
 
 // .vol text grid: line 1 holds the resolution "W H D", line 2 is skipped (as
@@ -352,8 +356,9 @@ void read_pfm(const std::string& path, Image& img) {
   const std::string ws = token(), hs = token(), ss = token();
   const long w = std::strtol(ws.c_str(), &end, 10);
   const long h = std::strtol(hs.c_str(), &end, 10);
-  const double scale = std::strtod(ss.c_str(), &end);
-  if (w < 1 || h < 1 || w > 16384 || h > 16384 || scale == 0.0) raise(NE_ERR_PARSE, path + ": bad PFM header");
+  float scale = 0.0f;
+  if (!parse_float(ss.data(), ss.data() + ss.size(), scale)) raise(NE_ERR_PARSE, path + ": bad PFM scale");
+  if (w < 1 || h < 1 || w > 16384 || h > 16384 || scale == 0.0f) raise(NE_ERR_PARSE, path + ": bad PFM header");
   ++p;  // the single whitespace character after the scale
   const size_t n = size_t(w) * size_t(h) * size_t(ch);
   if (b.size() < p + 4 * n) raise(NE_ERR_PARSE, path + ": truncated PFM data");
