@@ -298,6 +298,8 @@ def main():
     ap.add_argument("--pool", type=int, default=0, help="wavefront path slots (0 = library default)")
     ap.add_argument("--sort", action="store_true", help="wavefront: shade chunks in (event kind, material) order")
     ap.add_argument("--maj-res", type=int, default=0, help="majorant cells per axis (0 = the workload's)")
+    ap.add_argument("--no-empty-skip", action="store_true",
+                    help="majorant DDA reads every cell (no skipping of empty runs; same image)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-count", action="store_true")
@@ -331,6 +333,8 @@ def main():
     wl = configs.CONFIGS[args.workload]()
     if args.maj_res and wl.scene.volume is not None:
         wl.scene.volume["maj_res"] = args.maj_res
+    if args.no_empty_skip and wl.scene.volume is not None:
+        wl.scene.volume["empty_skip"] = False
     log(f"workload {wl.name}: {wl.width}x{wl.height} spp {wl.spp} bounces {wl.max_bounces} "
         f"(scene built in {time.perf_counter() - t0:.1f} s)")
     ctx = Context(0)
@@ -400,7 +404,9 @@ def main():
         "config": {"workload": wl.name, "width": wl.width, "height": wl.height, "spp": wl.spp,
                    "max_bounces": wl.max_bounces, "paths_per_step": paths_per_step, "accumulation": "f64",
                    "mode": args.mode, "pool": args.pool, "material_sort": args.sort,
-                   "maj_res": wl.scene.volume["maj_res"] if wl.scene.volume is not None else None},
+                   "maj_res": wl.scene.volume["maj_res"] if wl.scene.volume is not None else None,
+                   "empty_skip": (wl.scene.volume.get("empty_skip", True) if wl.scene.volume is not None
+                                  else None)},
         "rays_per_sec": round(rays / step_s, 1) if rays else None,
         "roofline": roofline,
         "cpu_baseline": cpu,

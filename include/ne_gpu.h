@@ -109,6 +109,9 @@ typedef struct ne_scene_desc {
   const float* point;       /* point lights: 3 floats each (position), Point (src/primitives/Point.cpp) */
   const int32_t* point_mat; /* their emitter materials (radiance = intensity) */
   uint32_t n_point;
+  int32_t maj_no_skip; /* 1: the majorant DDA reads every cell it crosses; 0: empty cells carry their
+                          L1 distance to the nearest non-empty cell and runs of them are stepped
+                          over without reads (same image) */
 } ne_scene_desc;
 
 /* Pinhole camera: Camera(lookFrom, lookAt, up, vfov, aspect, ...)
@@ -377,6 +380,12 @@ ne_status ne_test_delta_track(ne_ctx* ctx, const ne_scene* scene, int32_t n, con
 /* Trilinear density (GridMedia::interpolatedDensity, GridMedia.cpp:25-38) at
  * n grid-space points of the scene's volume. */
 ne_status ne_test_density(ne_ctx* ctx, const ne_scene* scene, int32_t n, const float* gcs, float* out);
+/* The scene's majorant grid as the device holds it (maj_res^3 floats, x
+ * fastest): positive per-cell majorants, and in empty cells 0 or, with empty-
+ * space skipping, -D (D the L1 cell distance to the nearest non-empty cell).
+ * No reference counterpart: GridMedia keeps one global majorant
+ * (src/materials/GridMedia.cpp:82). */
+ne_status ne_test_majorants(ne_ctx* ctx, const ne_scene* scene, float* out, int64_t capacity);
 /* Phase / BSDF functions:
  *   NE_TEST_HG_EVAL:   in a[3n], b[3n], g[n]           -> out[n]   HG::eval (Medium.h:116-128)
  *   NE_TEST_HG_SAMPLE: in u[2n], g[n]                  -> out[3n]  HG::sample local (Medium.h:96-114)

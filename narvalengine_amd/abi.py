@@ -39,6 +39,7 @@ class SceneDesc(C.Structure):
         ("vol_homogeneous", C.c_int32),
         ("env_map", _F), ("env_w", C.c_int32), ("env_h", C.c_int32),
         ("point", _F), ("point_mat", _I), ("n_point", C.c_uint32),
+        ("maj_no_skip", C.c_int32),
     ]
 
 
@@ -142,6 +143,7 @@ SIGNATURES = [
     ("ne_test_transmittance", C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32, _F, _F, C.c_uint64, C.c_int32, _D]),
     ("ne_test_delta_track", C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32, _F, C.c_uint64, C.c_int32, _I, _D]),
     ("ne_test_density", C.c_int32, [C.c_void_p, C.c_void_p, C.c_int32, _F, _F]),
+    ("ne_test_majorants", C.c_int32, [C.c_void_p, C.c_void_p, _F, C.c_int64]),
     ("ne_test_bsdf", C.c_int32, [C.c_void_p, C.c_int32, C.c_int32, _F, _F, _F, _F, _F]),
     ("ne_test_rng", C.c_int32, [C.c_void_p, C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_int32, _F]),
     ("ne_bvh_build", C.c_int32, [C.POINTER(SceneDesc), _F, _U, _U, _U, _U]),

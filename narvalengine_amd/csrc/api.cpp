@@ -124,6 +124,28 @@ inline float rect_area(const float* eu, const float* ev) {
   return std::sqrt(c[0] * c[0] + c[1] * c[1] + c[2] * c[2]);
 }
 
+// Empty-space skipping over an M^3 majorant grid: every empty cell (majorant
+// 0) gets -D, D the L1 distance to the nearest non-empty cell (3M when there
+// is none).  The L1 distance transform is separable: per axis, a forward and
+// a backward sweep of d[k] = min(d[k], d[k -/+ 1] + 1).
+void encode_empty_distance(std::vector<float>& maj, int M) {
+  const int far = 3 * M;
+  std::vector<int> d(maj.size());
+  for (size_t i = 0; i < maj.size(); ++i) d[i] = maj[i] > 0.0f ? 0 : far;
+  const size_t stride[3] = {1, size_t(M), size_t(M) * size_t(M)};
+  for (int ax = 0; ax < 3; ++ax) {
+    const size_t st = stride[ax], su = stride[(ax + 1) % 3], sv = stride[(ax + 2) % 3];
+    for (int u = 0; u < M; ++u)
+      for (int v = 0; v < M; ++v) {
+        int* line = d.data() + size_t(u) * su + size_t(v) * sv;
+        for (int k = 1; k < M; ++k) line[k * st] = std::min(line[k * st], line[(k - 1) * st] + 1);
+        for (int k = M - 2; k >= 0; --k) line[k * st] = std::min(line[k * st], line[(k + 1) * st] + 1);
+      }
+  }
+  for (size_t i = 0; i < maj.size(); ++i)
+    if (!(maj[i] > 0.0f)) maj[i] = -float(std::min(d[i], far));
+}
+
 // Sampling tables of an environment map: func = luminance * sin(theta of the
 // row centre); per-row CDFs over the row's W texels and the marginal CDF over
 // rows, summed in FP64 and stored in FP32 with exact 0 and 1 ends (uniform
@@ -550,6 +572,11 @@ ne_status ne_scene_upload(ne_ctx* c, const ne_scene_desc* d, ne_scene** out) {
     NE_HIP(c, hipStreamSynchronize(c->stream));
     for (float v : mh) gmax = std::max(gmax, v);
     gmax = ds.sigma_bar > 0 ? gmax / ds.sigma_bar : 0.0f;
+    if (!d->maj_no_skip && std::any_of(mh.begin(), mh.end(), [](float v) { return !(v > 0.0f); })) {
+      encode_empty_distance(mh, ds.mres);
+      NE_HIP(c, hipMemcpyAsync(pm, mh.data(), ncell * sizeof(float), hipMemcpyHostToDevice, c->stream));
+      NE_HIP(c, hipStreamSynchronize(c->stream));
+    }
   }
   // ---- lights
   ds.has_sun = d->has_sun ? 1 : 0;
@@ -1010,6 +1037,18 @@ ne_status ne_test_density(ne_ctx* c, const ne_scene* s, int32_t n, const float* 
   if ((st = tmp_in(c, bg, gcs, size_t(n) * 3)) || (st = tmp_out<float>(c, bo, n))) return st;
   NE_HIP(c, launch_test_density(s->dev, n, (const float*)bg.p, (float*)bo.p, c->stream));
   if ((st = tmp_back(c, out, bo, n))) return st;
+  NE_HIP(c, hipStreamSynchronize(c->stream));
+  return NE_OK;
+}
+
+ne_status ne_test_majorants(ne_ctx* c, const ne_scene* s, float* out, int64_t capacity) {
+  if (!c || !s || !out) return fail(c, NE_ERR_INVALID_ARG, "bad arguments");
+  if (!s->dev.has_volume) return fail(c, NE_ERR_INVALID_ARG, "scene has no volume");
+  const int64_t M = s->dev.mres, n = M * M * M;
+  if (capacity < n) return fail(c, NE_ERR_INVALID_ARG, "output holds fewer than maj_res^3 floats");
+  std::lock_guard<std::mutex> lk(c->mu);
+  NE_HIP(c, hipSetDevice(c->device));
+  NE_HIP(c, hipMemcpyAsync(out, s->dev.maj, size_t(n) * sizeof(float), hipMemcpyDeviceToHost, c->stream));
   NE_HIP(c, hipStreamSynchronize(c->stream));
   return NE_OK;
 }

@@ -433,7 +433,12 @@ NE_DEV float density_ocs(const DevScene& s, Vec3<float> p) {
 }
 
 // 3-D DDA over the majorant grid along the OCS ray (oo, dd) for t in [ta, tb).
-// Calls f(c0, c1, mu) per cell; f returns false to stop.
+// Calls f(c0, c1, mu) per cell with mu > 0; f returns false to stop.
+// Empty cells may hold -D instead of 0, D >= 1 the L1 (Manhattan) distance
+// to the nearest non-empty cell: each step moves one cell along one axis, so
+// the next D - 1 cells are empty too and their majorants are not read.  Cell
+// boundaries come from the cell index, never accumulated, so the t of every
+// non-empty cell is the same with or without skipping.
 template <typename R, bool C, typename F>
 NE_DEV void majorant_dda(const DevScene& s, Vec3<R> oo, Vec3<R> dd, R ta, R tb, Count<C>& cnt, F&& f) {
   const int M = s.mres;
@@ -453,6 +458,7 @@ NE_DEV void majorant_dda(const DevScene& s, Vec3<R> oo, Vec3<R> dd, R ta, R tb, 
     else { step[a] = 0; tnext[a] = xinf<R>(); }
   }
   R t = ta;
+  int skip = 0;
   while (t < tb) {
     int ax = 0;
     if (tnext[1] < tnext[ax]) ax = 1;
@@ -460,8 +466,13 @@ NE_DEV void majorant_dda(const DevScene& s, Vec3<R> oo, Vec3<R> dd, R ta, R tb, 
     R tna = ax == 0 ? tnext[0] : (ax == 1 ? tnext[1] : tnext[2]);
     R tc = xmin(tna, tb);
     cnt.cell();
-    R mu = R(s.maj[(size_t(cell[2]) * M + cell[1]) * M + cell[0]]);
-    if (tc > t && !f(t, tc, mu)) return;
+    if (skip > 0) {
+      --skip;
+    } else {
+      const float raw = s.maj[(size_t(cell[2]) * M + cell[1]) * M + cell[0]];
+      if (raw < 0.0f) skip = int(-raw) - 1;
+      else if (raw > 0.0f && tc > t && !f(t, tc, R(raw))) return;
+    }
     t = tc;
     if (tna >= tb) break;
     int c = (ax == 0 ? cell[0] : (ax == 1 ? cell[1] : cell[2])) + (ax == 0 ? step[0] : (ax == 1 ? step[1] : step[2]));

@@ -115,50 +115,76 @@ __device__ __forceinline__ void path_pixel(const WfArgs& a, unsigned long long i
 
 // One block: exclusive scan of the chunks' free slots -> each chunk's first new
 // path id and count; advances the work counter; writes the number of paths in
-// flight this iteration to status[0].
+// flight this iteration to status[0].  The chunks go in rounds of
+// kPlanThreads * kPlanPer: each thread holds kPlanPer consecutive counts
+// (16-byte loads issued together), the block scans the per-thread sums with
+// wave shuffles and one word per wave in LDS, and rounds chain through the
+// carried path id.  Live paths = 256 * chunks - free slots.
+constexpr int kPlanPer = 16;
 __global__ __launch_bounds__(kPlanThreads) void k_wf_plan(WfState w, WfArgs a) {
-  __shared__ unsigned long long part[kPlanThreads];
+  __shared__ uint32_t wtot[kPlanThreads / 64];
   const uint32_t NC = a.nchunks;
   const uint32_t* cntA = w.cnt_paths[a.cur];
-  const uint32_t per = (NC + kPlanThreads - 1) / kPlanThreads;
-  const uint32_t c0 = threadIdx.x * per, c1 = c0 + per < NC ? c0 + per : NC;
-  unsigned long long fsum = 0, live = 0;
-  for (uint32_t c = c0; c < c1; ++c) {
-    fsum += kChunk - cntA[c];
-    live += cntA[c];
-  }
-  part[threadIdx.x] = fsum;
-  __syncthreads();
-  // inclusive Hillis-Steele scan of the per-thread free counts
-  for (int o = 1; o < kPlanThreads; o <<= 1) {
-    unsigned long long v = threadIdx.x >= uint32_t(o) ? part[threadIdx.x - o] : 0ull;
-    __syncthreads();
-    part[threadIdx.x] += v;
-    __syncthreads();
-  }
-  const unsigned long long total_free = part[kPlanThreads - 1];
   const unsigned long long base = *w.next_path;
-  unsigned long long off = base + part[threadIdx.x] - fsum;
-  for (uint32_t c = c0; c < c1; ++c) {
-    const uint32_t free_c = kChunk - cntA[c];
-    const unsigned long long left = off < a.total_paths ? a.total_paths - off : 0ull;
-    w.regen_base[c] = off;
-    w.regen_m[c] = uint32_t(left < free_c ? left : free_c);
-    off += free_c;
-  }
-  // live-path total (block reduction reuses part[] after the scan)
-  __syncthreads();
-  part[threadIdx.x] = live;
-  __syncthreads();
-  for (int o = kPlanThreads / 2; o > 0; o >>= 1) {
-    if (threadIdx.x < uint32_t(o)) part[threadIdx.x] += part[threadIdx.x + o];
+  const int lane = __lane_id(), wave = int(threadIdx.x >> 6);
+  unsigned long long carry = base;  // path id of the round's first free slot
+  for (uint32_t r0 = 0; r0 < NC; r0 += uint32_t(kPlanThreads * kPlanPer)) {
+    const uint32_t c0 = r0 + threadIdx.x * uint32_t(kPlanPer);
+    uint32_t fr[kPlanPer];
+    uint32_t fsum = 0;
+    if (c0 < NC) {
+      // c0 is a multiple of 16 and each counter array is padded to 64 words
+      const uint4* q = reinterpret_cast<const uint4*>(cntA + c0);
+      uint4 v[kPlanPer / 4];
+#pragma unroll
+      for (int k = 0; k < kPlanPer / 4; ++k) v[k] = q[k];
+#pragma unroll
+      for (int k = 0; k < kPlanPer / 4; ++k) {
+        fr[4 * k] = v[k].x; fr[4 * k + 1] = v[k].y; fr[4 * k + 2] = v[k].z; fr[4 * k + 3] = v[k].w;
+      }
+#pragma unroll
+      for (int j = 0; j < kPlanPer; ++j) {
+        fr[j] = c0 + uint32_t(j) < NC ? kChunk - fr[j] : 0u;
+        fsum += fr[j];
+      }
+    } else {
+#pragma unroll
+      for (int j = 0; j < kPlanPer; ++j) fr[j] = 0u;
+    }
+    // a round holds at most 2^22 free slots: 32-bit sums
+    uint32_t incl = fsum;
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+      const uint32_t v = __shfl_up(incl, o);
+      if (lane >= o) incl += v;
+    }
+    if (lane == 63) wtot[wave] = incl;
     __syncthreads();
+    uint32_t before = 0, rtot = 0;
+#pragma unroll
+    for (int k = 0; k < kPlanThreads / 64; ++k) {
+      before += k < wave ? wtot[k] : 0u;
+      rtot += wtot[k];
+    }
+    __syncthreads();  // wtot is rewritten by the next round
+    unsigned long long off = carry + before + incl - fsum;
+#pragma unroll
+    for (int j = 0; j < kPlanPer; ++j) {
+      if (c0 + uint32_t(j) < NC) {
+        const unsigned long long left = off < a.total_paths ? a.total_paths - off : 0ull;
+        w.regen_base[c0 + j] = off;
+        w.regen_m[c0 + j] = uint32_t(left < fr[j] ? left : fr[j]);
+        off += fr[j];
+      }
+    }
+    carry += rtot;
   }
   if (threadIdx.x == 0) {
+    const unsigned long long total_free = carry - base;
     const unsigned long long left = base < a.total_paths ? a.total_paths - base : 0ull;
     const unsigned long long m = left < total_free ? left : total_free;
     *w.next_path = base + m;
-    const unsigned long long inflight = part[0] + m;
+    const unsigned long long inflight = (unsigned long long)NC * kChunk - total_free + m;
     w.status[0] = uint32_t(inflight < 0xFFFFFFFFull ? inflight : 0xFFFFFFFFull);
   }
 }

@@ -142,6 +142,11 @@ class Context:
         self.check(self._lib.ne_test_density(self.h, gscene.h, len(gcs), _fp(gcs), _fp(out)))
         return out
 
+    def test_majorants(self, gscene, maj_res: int):
+        out = np.zeros((maj_res, maj_res, maj_res), np.float32)
+        self.check(self._lib.ne_test_majorants(self.h, gscene.h, _fp(out), out.size))
+        return out
+
     def test_bsdf(self, kind: int, n: int, ins, out_width: int):
         arrs = [None if a is None else np.ascontiguousarray(a, np.float32) for a in ins] + [None] * (4 - len(ins))
         out = np.zeros((n, out_width), np.float32)

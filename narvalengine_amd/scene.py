@@ -127,12 +127,16 @@ class Scene:
         self.point_mat.append(mat)
 
     def set_volume(self, grid: np.ndarray, density_mult: float, sigma_s, sigma_a, g: float,
-                   position=(0, 0, 0), rotation=(0, 0, 0), scale=(1, 1, 1), maj_res: int = 16):
+                   position=(0, 0, 0), rotation=(0, 0, 0), scale=(1, 1, 1), maj_res: int = 16,
+                   empty_skip: bool = True):
+        """empty_skip: the majorant DDA steps over runs of empty cells without
+        reading them (same image; off only to measure what it saves)."""
         grid = np.ascontiguousarray(grid, dtype=np.float32)
         assert grid.ndim == 3, "grid must be (D, H, W), x fastest"
         M = transform(position, rotation, scale)
         self.volume = dict(grid=grid, mult=float(density_mult), sigma_s=tuple(sigma_s), sigma_a=tuple(sigma_a),
-                           g=float(g), xf=M, inv=np.linalg.inv(M), maj_res=int(maj_res))
+                           g=float(g), xf=M, inv=np.linalg.inv(M), maj_res=int(maj_res),
+                           empty_skip=bool(empty_skip))
 
     def set_homogeneous_volume(self, density: float, sigma_s, sigma_a, g: float,
                                position=(0, 0, 0), rotation=(0, 0, 0), scale=(1, 1, 1)):
@@ -200,6 +204,7 @@ class Scene:
             d.vol_sigma_a[:] = v["sigma_a"]
             d.vol_g = v["g"]
             d.maj_res = v["maj_res"]
+            d.maj_no_skip = 0 if v.get("empty_skip", True) else 1
         if self.sun is not None:
             d.has_sun = 1
             d.sun_dir[:] = [float(x) for x in self.sun["dir"]]

@@ -606,6 +606,7 @@ struct SceneDesc {
   int vol_homogeneous;                                              // constant density vol_mult; grid unused
   const float* env_map; int env_w, env_h;                           // ENV_MAP: RGB, row 0 at theta = 0
   const float* point; const int32_t* point_mat; uint32_t n_point;   // point lights (emitter materials)
+  int maj_no_skip;                                                  // device DDA option; no effect on results
 };
 
 struct CameraDesc {

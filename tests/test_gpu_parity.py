@@ -484,3 +484,61 @@ def test_material_sort_leaves_the_image_unchanged(ctx, which):
         out.append((f.read(), ctx.counters()))
     np.testing.assert_array_equal(out[0][0], out[1][0])
     assert out[0][1] == out[1][1]
+
+
+def test_empty_space_skipping_is_exact(ctx):
+    """Empty majorant cells hold -D, D the taxicab distance to the nearest
+    non-empty cell, and the DDA steps over the next D - 1 cells without reading
+    them: non-empty majorants equal the oracle's, and images and event counts
+    are identical with the skip on and off, megakernel and wavefront."""
+    print_case("empty-space skipping")
+    from scipy.ndimage import distance_transform_cdt
+
+    from narvalengine_amd.abi import NE_MODE_MEGAKERNEL, NE_MODE_WAVEFRONT
+    M = 12
+    on = scenes.small_cloud(res=48, maj_res=M)
+    off = scenes.small_cloud(res=48, maj_res=M)
+    off.volume["empty_skip"] = False
+    g_on, g_off = ctx.upload(on), ctx.upload(off)
+    m_on, m_off = ctx.test_majorants(g_on, M), ctx.test_majorants(g_off, M)
+    om = O.World(on).majorants(M).reshape(M, M, M)
+    np.testing.assert_array_equal(m_off, om)
+    empty = om <= 0
+    assert 0.2 < empty.mean() < 0.95
+    dist = distance_transform_cdt(empty, metric="taxicab")
+    assert dist.max() >= 3  # runs long enough to skip
+    np.testing.assert_array_equal(m_on[~empty], om[~empty])
+    np.testing.assert_array_equal(m_on[empty], -dist[empty].astype(np.float32))
+    for mode in (NE_MODE_MEGAKERNEL, NE_MODE_WAVEFRONT):
+        out = []
+        for g in (g_on, g_off):
+            f = ctx.film(40, 24)
+            ctx.render(g, scenes.VOLUME_CAMERA, f, spp=16, max_bounces=64, seed=21, mode=mode, count=True)
+            out.append((f.read(), ctx.counters()))
+            f.close()
+        np.testing.assert_array_equal(out[0][0], out[1][0])
+        assert out[0][1] == out[1][1]
+        assert out[0][1]["majorant_cells"] > out[0][1]["density_lookups"] > 0
+    g_on.close()
+    g_off.close()
+
+
+def test_wavefront_pool_rounds(ctx):
+    """Pools above 16384 chunks take several rounds of the planning scan
+    (here 32771 chunks: two full rounds and a ragged third): the image equals
+    the one from a single-round pool and from a 196-chunk pool."""
+    print_case("wavefront pool rounds")
+    from narvalengine_amd.abi import NE_MODE_WAVEFRONT
+    s = scenes.small_volume(res=32, maj_res=4)
+    g = ctx.upload(s)
+    films = []
+    for pool in (1 << 22, (1 << 23) + 3 * 256, 50000):
+        f = ctx.film(64, 64)
+        ctx.render(g, scenes.VOLUME_CAMERA, f, spp=2100, max_bounces=4, seed=23, mode=NE_MODE_WAVEFRONT,
+                   pool_size=pool, count=pool == 50000)
+        films.append(f.read())
+        f.close()
+    assert ctx.counters()["paths"] == 64 * 64 * 2100
+    np.testing.assert_array_equal(films[0], films[1])
+    np.testing.assert_array_equal(films[0], films[2])
+    g.close()
