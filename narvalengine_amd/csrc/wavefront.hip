@@ -116,65 +116,71 @@ __device__ __forceinline__ void path_pixel(const WfArgs& a, unsigned long long i
 // One block: exclusive scan of the chunks' free slots -> each chunk's first new
 // path id and count; advances the work counter; writes the number of paths in
 // flight this iteration to status[0].  The chunks go in rounds of
-// kPlanThreads * kPlanPer: each thread holds kPlanPer consecutive counts
-// (16-byte loads issued together), the block scans the per-thread sums with
-// wave shuffles and one word per wave in LDS, and rounds chain through the
+// kPlanPer rows of kPlanThreads; thread t holds chunk r0 + j * kPlanThreads + t
+// of every row j, so loads and stores are coalesced.  Each row is scanned
+// with wave shuffles; one wave then scans the kPlanPer * 16 wave totals (in
+// chunk order: row by row, wave by wave) in LDS, and rounds chain through the
 // carried path id.  Live paths = 256 * chunks - free slots.
 constexpr int kPlanPer = 16;
+constexpr int kPlanWaves = kPlanThreads / 64;
 __global__ __launch_bounds__(kPlanThreads) void k_wf_plan(WfState w, WfArgs a) {
-  __shared__ uint32_t wtot[kPlanThreads / 64];
+  __shared__ uint32_t wtot[kPlanPer * kPlanWaves];   // per (row, wave) free-slot sums
+  __shared__ uint32_t wpre[kPlanPer * kPlanWaves];   // their exclusive prefix within the round
+  __shared__ uint32_t rtot;
   const uint32_t NC = a.nchunks;
   const uint32_t* cntA = w.cnt_paths[a.cur];
   const unsigned long long base = *w.next_path;
   const int lane = __lane_id(), wave = int(threadIdx.x >> 6);
   unsigned long long carry = base;  // path id of the round's first free slot
   for (uint32_t r0 = 0; r0 < NC; r0 += uint32_t(kPlanThreads * kPlanPer)) {
-    const uint32_t c0 = r0 + threadIdx.x * uint32_t(kPlanPer);
-    uint32_t fr[kPlanPer];
-    uint32_t fsum = 0;
-    if (c0 < NC) {
-      // c0 is a multiple of 16 and each counter array is padded to 64 words
-      const uint4* q = reinterpret_cast<const uint4*>(cntA + c0);
-      uint4 v[kPlanPer / 4];
-#pragma unroll
-      for (int k = 0; k < kPlanPer / 4; ++k) v[k] = q[k];
-#pragma unroll
-      for (int k = 0; k < kPlanPer / 4; ++k) {
-        fr[4 * k] = v[k].x; fr[4 * k + 1] = v[k].y; fr[4 * k + 2] = v[k].z; fr[4 * k + 3] = v[k].w;
-      }
-#pragma unroll
-      for (int j = 0; j < kPlanPer; ++j) {
-        fr[j] = c0 + uint32_t(j) < NC ? kChunk - fr[j] : 0u;
-        fsum += fr[j];
-      }
-    } else {
-#pragma unroll
-      for (int j = 0; j < kPlanPer; ++j) fr[j] = 0u;
-    }
-    // a round holds at most 2^22 free slots: 32-bit sums
-    uint32_t incl = fsum;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-      const uint32_t v = __shfl_up(incl, o);
-      if (lane >= o) incl += v;
-    }
-    if (lane == 63) wtot[wave] = incl;
-    __syncthreads();
-    uint32_t before = 0, rtot = 0;
-#pragma unroll
-    for (int k = 0; k < kPlanThreads / 64; ++k) {
-      before += k < wave ? wtot[k] : 0u;
-      rtot += wtot[k];
-    }
-    __syncthreads();  // wtot is rewritten by the next round
-    unsigned long long off = carry + before + incl - fsum;
+    uint32_t fr[kPlanPer], incl[kPlanPer];
 #pragma unroll
     for (int j = 0; j < kPlanPer; ++j) {
-      if (c0 + uint32_t(j) < NC) {
+      const uint32_t c = r0 + uint32_t(j * kPlanThreads) + threadIdx.x;
+      fr[j] = c < NC ? kChunk - cntA[c] : 0u;
+    }
+#pragma unroll
+    for (int j = 0; j < kPlanPer; ++j) incl[j] = fr[j];
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+#pragma unroll
+      for (int j = 0; j < kPlanPer; ++j) {
+        const uint32_t v = __shfl_up(incl[j], o);
+        if (lane >= o) incl[j] += v;
+      }
+    }
+    if (lane == 63) {
+#pragma unroll
+      for (int j = 0; j < kPlanPer; ++j) wtot[j * kPlanWaves + wave] = incl[j];
+    }
+    __syncthreads();
+    if (wave == 0) {  // 256 totals, 4 consecutive per lane; a round holds at most 2^22 free slots
+      constexpr int kE = kPlanPer * kPlanWaves / 64;
+      uint32_t e[kE], sum = 0;
+#pragma unroll
+      for (int q = 0; q < kE; ++q) {
+        e[q] = sum;
+        sum += wtot[lane * kE + q];
+      }
+      uint32_t sc = sum;
+#pragma unroll
+      for (int o = 1; o < 64; o <<= 1) {
+        const uint32_t v = __shfl_up(sc, o);
+        if (lane >= o) sc += v;
+      }
+#pragma unroll
+      for (int q = 0; q < kE; ++q) wpre[lane * kE + q] = sc - sum + e[q];
+      if (lane == 63) rtot = sc;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int j = 0; j < kPlanPer; ++j) {
+      const uint32_t c = r0 + uint32_t(j * kPlanThreads) + threadIdx.x;
+      if (c < NC) {
+        const unsigned long long off = carry + wpre[j * kPlanWaves + wave] + incl[j] - fr[j];
         const unsigned long long left = off < a.total_paths ? a.total_paths - off : 0ull;
-        w.regen_base[c0 + j] = off;
-        w.regen_m[c0 + j] = uint32_t(left < fr[j] ? left : fr[j]);
-        off += fr[j];
+        w.regen_base[c] = off;
+        w.regen_m[c] = uint32_t(left < fr[j] ? left : fr[j]);
       }
     }
     carry += rtot;
