@@ -419,10 +419,25 @@ NE_DEV float density_gcs(const DevScene& s, Vec3<float> gp) {
   gp = v3(xmin(gp.x, float(s.res[0])), xmin(gp.y, float(s.res[1])), xmin(gp.z, float(s.res[2])));
   int ix = int(floorf(gp.x)), iy = int(floorf(gp.y)), iz = int(floorf(gp.z));
   float dx = gp.x - float(ix), dy = gp.y - float(iy), dz = gp.z - float(iz);
-  float c000 = grid_at(s, ix, iy, iz), c100 = grid_at(s, ix + 1, iy, iz);
-  float c010 = grid_at(s, ix, iy + 1, iz), c110 = grid_at(s, ix + 1, iy + 1, iz);
-  float c001 = grid_at(s, ix, iy, iz + 1), c101 = grid_at(s, ix + 1, iy, iz + 1);
-  float c011 = grid_at(s, ix, iy + 1, iz + 1), c111 = grid_at(s, ix + 1, iy + 1, iz + 1);
+  float c000, c100, c010, c110, c001, c101, c011, c111;
+  if (ix + 1 < s.res[0] && iy + 1 < s.res[1] && iz + 1 < s.res[2]) {
+    // interior cell: every corner is in the grid; the x-neighbours are
+    // adjacent floats, read as pairs (4-byte aligned)
+    const size_t W = size_t(s.res[0]), WH = W * size_t(s.res[1]);
+    const float* p = s.grid + size_t(iz) * WH + size_t(iy) * W + size_t(ix);
+    float2 q00, q10, q01, q11;
+    __builtin_memcpy(&q00, p, 8);
+    __builtin_memcpy(&q10, p + W, 8);
+    __builtin_memcpy(&q01, p + WH, 8);
+    __builtin_memcpy(&q11, p + WH + W, 8);
+    c000 = q00.x; c100 = q00.y; c010 = q10.x; c110 = q10.y;
+    c001 = q01.x; c101 = q01.y; c011 = q11.x; c111 = q11.y;
+  } else {
+    c000 = grid_at(s, ix, iy, iz); c100 = grid_at(s, ix + 1, iy, iz);
+    c010 = grid_at(s, ix, iy + 1, iz); c110 = grid_at(s, ix + 1, iy + 1, iz);
+    c001 = grid_at(s, ix, iy, iz + 1); c101 = grid_at(s, ix + 1, iy, iz + 1);
+    c011 = grid_at(s, ix, iy + 1, iz + 1); c111 = grid_at(s, ix + 1, iy + 1, iz + 1);
+  }
   float d00 = mixf(c000, c100, dx), d10 = mixf(c010, c110, dx);
   float d01 = mixf(c001, c101, dx), d11 = mixf(c011, c111, dx);
   float d0 = mixf(d00, d10, dy), d1 = mixf(d01, d11, dy);
