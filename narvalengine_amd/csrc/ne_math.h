@@ -100,20 +100,16 @@ NE_HD Vec3<R> to_lcs(Vec3<R> v, Vec3<R> ns, Vec3<R> ss, Vec3<R> ts) {
 template <typename R> NE_HD R power_heuristic(R a, R b) { return (a * a) / (a * a + b * b); }
 
 // ---- Philox-4x32-10 (Salmon et al., SC'11) ---------------------------------
-NE_HD uint32_t mulhi32(uint32_t a, uint32_t b) {
-#if defined(__HIP_DEVICE_COMPILE__)
-  return __umulhi(a, b);
-#else
-  return uint32_t((uint64_t(a) * uint64_t(b)) >> 32);
-#endif
-}
 
 NE_HD void philox10(uint32_t c[4], uint32_t k0, uint32_t k1) {
 #pragma unroll
   for (int r = 0; r < 10; ++r) {
     if (r > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
-    uint32_t hi0 = mulhi32(0xD2511F53u, c[0]), lo0 = 0xD2511F53u * c[0];
-    uint32_t hi1 = mulhi32(0xCD9E8D57u, c[2]), lo1 = 0xCD9E8D57u * c[2];
+    // one 32x32->64 product per multiplier (a single v_mad_u64_u32 on gfx950
+    // instead of separate high and low multiplies)
+    const uint64_t p0 = uint64_t(0xD2511F53u) * c[0], p1 = uint64_t(0xCD9E8D57u) * c[2];
+    const uint32_t hi0 = uint32_t(p0 >> 32), lo0 = uint32_t(p0);
+    const uint32_t hi1 = uint32_t(p1 >> 32), lo1 = uint32_t(p1);
     uint32_t n0 = hi1 ^ c[1] ^ k0, n2 = hi0 ^ c[3] ^ k1;
     c[0] = n0; c[1] = lo1; c[2] = n2; c[3] = lo0;
   }

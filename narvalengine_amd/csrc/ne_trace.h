@@ -149,25 +149,24 @@ NE_DEV bool isect_rect(const float4* rc, Vec3<R> o, Vec3<R> d, R tmin, R tmax, R
 }
 
 // Conservative slab test against a float box (Ize 2013: far distances scaled
-// by 1 + 2 gamma(3)); NaN from 0 * inf leaves the running interval alone.
+// by 1 + 2 gamma(3)).  invd is finite (trav_init keeps |d| >= 2^-100 per
+// axis), so no (plane - o) * invd is 0 * inf and min/max see no NaN.  The far
+// scale is applied once to the smallest far distance: multiplying by a
+// positive constant is monotone, so this equals scaling each one.
+NE_DEV float fmin_(float a, float b) { return __builtin_fminf(a, b); }
+NE_DEV double fmin_(double a, double b) { return __builtin_fmin(a, b); }
+NE_DEV float fmax_(float a, float b) { return __builtin_fmaxf(a, b); }
+NE_DEV double fmax_(double a, double b) { return __builtin_fmax(a, b); }
 template <typename R>
 NE_DEV bool slab(float mnx, float mny, float mnz, float mxx, float mxy, float mxz, Vec3<R> o, Vec3<R> invd, R tmax,
                  R& tent) {
-  R t0 = R(0), t1 = tmax;
-  R tn = (R(mnx) - o.x) * invd.x, tf = (R(mxx) - o.x) * invd.x;
-  if (tn > tf) { R s = tn; tn = tf; tf = s; }
-  tf = tf * R(1.0000003576);
-  t0 = tn > t0 ? tn : t0; t1 = tf < t1 ? tf : t1;
-  tn = (R(mny) - o.y) * invd.y; tf = (R(mxy) - o.y) * invd.y;
-  if (tn > tf) { R s = tn; tn = tf; tf = s; }
-  tf = tf * R(1.0000003576);
-  t0 = tn > t0 ? tn : t0; t1 = tf < t1 ? tf : t1;
-  tn = (R(mnz) - o.z) * invd.z; tf = (R(mxz) - o.z) * invd.z;
-  if (tn > tf) { R s = tn; tn = tf; tf = s; }
-  tf = tf * R(1.0000003576);
-  t0 = tn > t0 ? tn : t0; t1 = tf < t1 ? tf : t1;
+  const R nx = (R(mnx) - o.x) * invd.x, fx = (R(mxx) - o.x) * invd.x;
+  const R ny = (R(mny) - o.y) * invd.y, fy = (R(mxy) - o.y) * invd.y;
+  const R nz = (R(mnz) - o.z) * invd.z, fz = (R(mxz) - o.z) * invd.z;
+  const R t0 = fmax_(fmax_(R(0), fmin_(nx, fx)), fmax_(fmin_(ny, fy), fmin_(nz, fz)));
+  const R far = fmin_(fmax_(nx, fx), fmin_(fmax_(ny, fy), fmax_(nz, fz))) * R(1.0000003576);
   tent = t0;
-  return t0 <= t1;
+  return t0 <= fmin_(tmax, far);
 }
 
 template <typename R>
@@ -204,7 +203,11 @@ template <typename R>
 NE_DEV void trav_init(TravState<R>& st, Vec3<R> o, Vec3<R> d, R tmin, R tmax) {
   st.o = o;
   st.d = d;
-  st.invd = v3(R(1) / d.x, R(1) / d.y, R(1) / d.z);
+  // direction components below 2^-100 in magnitude count as +-2^-100 for the
+  // box tests: invd stays finite, and the box intervals only widen
+  const R tiny = R(7.888609052210118e-31);  // 2^-100
+  auto nz = [tiny](R v) { return xfabs(v) < tiny ? (__builtin_signbit(v) ? -tiny : tiny) : v; };
+  st.invd = v3(R(1) / nz(d.x), R(1) / nz(d.y), R(1) / nz(d.z));
   st.tmin = tmin;
   st.tmax = tmax;
   st.best = tmax;
@@ -299,8 +302,9 @@ NE_DEV bool trav_step(const DevScene& s, TravState<R>& st, uint32_t* lds, int la
   R t0 = inf, t1 = inf, t2 = inf, t3 = inf, te;
 #define NE_CHILD_SLAB(c, ref, tout)                                                                           \
   if ((ref) >= 0 &&                                                                                         \
-      slab<R>(f0.x + qbyte(f1.x, c) * sx, f0.y + qbyte(f1.y, c) * sy, f0.z + qbyte(f1.z, c) * sz,           \
-              f0.x + qbyte(f1.w, c) * sx, f0.y + qbyte(f2.x, c) * sy, f0.z + qbyte(f2.y, c) * sz, st.o,    \
+      slab<R>(__builtin_fmaf(qbyte(f1.x, c), sx, f0.x), __builtin_fmaf(qbyte(f1.y, c), sy, f0.y),             \
+              __builtin_fmaf(qbyte(f1.z, c), sz, f0.z), __builtin_fmaf(qbyte(f1.w, c), sx, f0.x),             \
+              __builtin_fmaf(qbyte(f2.x, c), sy, f0.y), __builtin_fmaf(qbyte(f2.y, c), sz, f0.z), st.o,       \
               st.invd, st.best, te))                                                                        \
     tout = te;
   NE_CHILD_SLAB(0, ch.x, t0)
@@ -423,8 +427,11 @@ NE_DEV float density_gcs(const DevScene& s, Vec3<float> gp) {
   if (ix + 1 < s.res[0] && iy + 1 < s.res[1] && iz + 1 < s.res[2]) {
     // interior cell: every corner is in the grid; the x-neighbours are
     // adjacent floats, read as pairs (4-byte aligned)
+    // row index z * H + y < 2^24 (each dimension is at most 4096): a 24-bit
+    // multiply; one 32x32->64 multiply-add for the voxel offset
     const size_t W = size_t(s.res[0]), WH = W * size_t(s.res[1]);
-    const float* p = s.grid + size_t(iz) * WH + size_t(iy) * W + size_t(ix);
+    const uint32_t row = __umul24(uint32_t(iz), uint32_t(s.res[1])) + uint32_t(iy);
+    const float* p = s.grid + (uint64_t(row) * uint64_t(uint32_t(s.res[0])) + uint64_t(uint32_t(ix)));
     float2 q00, q10, q01, q11;
     __builtin_memcpy(&q00, p, 8);
     __builtin_memcpy(&q10, p + W, 8);
@@ -457,9 +464,9 @@ NE_DEV float density_ocs(const DevScene& s, Vec3<float> p) {
 template <typename R, bool C, typename F>
 NE_DEV void majorant_dda(const DevScene& s, Vec3<R> oo, Vec3<R> dd, R ta, R tb, Count<C>& cnt, F&& f) {
   const int M = s.mres;
-  const R Mr = R(M);
+  const R Mr = R(M), invM = R(1) / Mr;
   int cell[3], step[3];
-  R tnext[3];
+  R tnext[3], inv[3];
   Vec3<R> pe = oo + ta * dd;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
@@ -468,8 +475,9 @@ NE_DEV void majorant_dda(const DevScene& s, Vec3<R> oo, Vec3<R> dd, R ta, R tb, 
     c = c < 0 ? 0 : (c > M - 1 ? M - 1 : c);
     cell[a] = c;
     R da = comp(dd, a), oa = comp(oo, a);
-    if (da > R(0)) { step[a] = 1; tnext[a] = ((R(c + 1) / Mr - R(0.5)) - oa) / da; }
-    else if (da < R(0)) { step[a] = -1; tnext[a] = ((R(c) / Mr - R(0.5)) - oa) / da; }
+    inv[a] = da != R(0) ? R(1) / da : R(0);
+    if (da > R(0)) { step[a] = 1; tnext[a] = ((R(c + 1) * invM - R(0.5)) - oa) * inv[a]; }
+    else if (da < R(0)) { step[a] = -1; tnext[a] = ((R(c) * invM - R(0.5)) - oa) * inv[a]; }
     else { step[a] = 0; tnext[a] = xinf<R>(); }
   }
   R t = ta;
@@ -492,8 +500,8 @@ NE_DEV void majorant_dda(const DevScene& s, Vec3<R> oo, Vec3<R> dd, R ta, R tb, 
     if (tna >= tb) break;
     int c = (ax == 0 ? cell[0] : (ax == 1 ? cell[1] : cell[2])) + (ax == 0 ? step[0] : (ax == 1 ? step[1] : step[2]));
     if (c < 0 || c >= M) break;
-    R bnd = (ax == 0 ? step[0] : (ax == 1 ? step[1] : step[2])) > 0 ? R(c + 1) / Mr : R(c) / Mr;
-    R nt = ((bnd - R(0.5)) - comp(oo, ax)) / comp(dd, ax);
+    R bnd = (ax == 0 ? step[0] : (ax == 1 ? step[1] : step[2])) > 0 ? R(c + 1) * invM : R(c) * invM;
+    R nt = ((bnd - R(0.5)) - comp(oo, ax)) * (ax == 0 ? inv[0] : (ax == 1 ? inv[1] : inv[2]));
     if (ax == 0) { cell[0] = c; tnext[0] = nt; }
     else if (ax == 1) { cell[1] = c; tnext[1] = nt; }
     else { cell[2] = c; tnext[2] = nt; }
@@ -508,21 +516,23 @@ NE_DEV R sigma_at(const DevScene& s, Vec3<R> oo, Vec3<R> dd, R t, Count<C>& cnt)
 }
 
 // Delta tracking (GridMedia::sample, src/materials/GridMedia.cpp:71-98) with
-// per-cell majorants: step t -= log(1 - xi) / mu, accept if sigma/mu > xi'.
+// per-cell majorants: step t -= log(1 - xi) * (1 / mu), accept if
+// sigma * (1 / mu) > xi', the reference's multiply-by-inverse form.
 template <typename R, bool C>
 NE_DEV bool delta_track(const DevScene& s, Vec3<R> o, Vec3<R> d, R ta, R tb, PathRng& rng, Count<C>& cnt, R& th) {
   Vec3<R> oo = xf_point<R>(s.inv, o), dd = xf_vec<R>(s.inv, d);
   bool hit = false;
   majorant_dda<R, C>(s, oo, dd, ta, tb, cnt, [&](R c0, R c1, R mu) {
     if (!(mu > R(0)) || xisinf(mu)) return true;  // empty cell; grids are validated finite at upload
+    const R imu = R(1) / mu;
     R t = c0;
     while (true) {
       R r = R(rng.next());
-      t -= xlog(R(1) - r) / mu;
+      t -= xlog(R(1) - r) * imu;
       if (!(t < c1)) return true;
       R sig = sigma_at<R, C>(s, oo, dd, t, cnt);
       R ra = R(rng.next());
-      if (sig / mu > ra) { hit = true; th = t; return false; }
+      if (sig * imu > ra) { hit = true; th = t; return false; }
     }
   });
   return hit;
@@ -539,12 +549,13 @@ NE_DEV R ratio_track(const DevScene& s, Vec3<R> o, Vec3<R> d, R ta, R tb, PathRn
   R Tr = R(1);
   majorant_dda<R, C>(s, oo, dd, ta, tb, cnt, [&](R c0, R c1, R mu) {
     if (!(mu > R(0)) || xisinf(mu)) return true;  // empty cell; grids are validated finite at upload
+    const R imu = R(1) / mu;
     R t = c0;
     while (true) {
-      t -= xlog(R(1) - R(rng.next())) / mu;
+      t -= xlog(R(1) - R(rng.next())) * imu;
       if (!(t < c1)) return true;
       R sig = sigma_at<R, C>(s, oo, dd, t, cnt);
-      Tr *= R(1) - xmax(R(0), sig / mu);
+      Tr *= R(1) - xmax(R(0), sig * imu);
       if (Tr < R(0.1)) {
         R q = xmax(R(0.05), R(1) - Tr);
         if (R(rng.next()) < q) { Tr = R(0); return false; }

@@ -947,17 +947,18 @@ struct World {
   // returns false to stop.
   template <class F>
   void dda(V oo, V dd, T ta, T tb, F&& f) const {
-    const T M = T(mres);
+    const T M = T(mres), invM = T(1) / M;
     int cell[3], step[3];
-    T tnext[3];
+    T tnext[3], inv[3];
     V pe = oo + ta * dd;
     for (int a = 0; a < 3; ++a) {
       T g = (pe[a] + T(0.5)) * M;
       int c = int(std::floor(g));
       c = std::min(std::max(c, 0), mres - 1);
       cell[a] = c;
-      if (dd[a] > T(0)) { step[a] = 1; tnext[a] = ((T(c + 1) / M - T(0.5)) - oo[a]) / dd[a]; }
-      else if (dd[a] < T(0)) { step[a] = -1; tnext[a] = ((T(c) / M - T(0.5)) - oo[a]) / dd[a]; }
+      inv[a] = dd[a] != T(0) ? T(1) / dd[a] : T(0);  // cell boundaries: multiply by 1 / d
+      if (dd[a] > T(0)) { step[a] = 1; tnext[a] = ((T(c + 1) * invM - T(0.5)) - oo[a]) * inv[a]; }
+      else if (dd[a] < T(0)) { step[a] = -1; tnext[a] = ((T(c) * invM - T(0.5)) - oo[a]) * inv[a]; }
       else { step[a] = 0; tnext[a] = std::numeric_limits<T>::infinity(); }
     }
     T t = ta;
@@ -972,27 +973,29 @@ struct World {
       if (tnext[ax] >= tb) break;
       cell[ax] += step[ax];
       if (cell[ax] < 0 || cell[ax] >= mres) break;
-      T bnd = step[ax] > 0 ? T(cell[ax] + 1) / M : T(cell[ax]) / M;
-      tnext[ax] = ((bnd - T(0.5)) - oo[ax]) / dd[ax];
+      T bnd = step[ax] > 0 ? T(cell[ax] + 1) * invM : T(cell[ax]) * invM;
+      tnext[ax] = ((bnd - T(0.5)) - oo[ax]) * inv[ax];
     }
   }
 
   // Delta tracking over [ta, tb] with per-cell majorants (a 1^3 grid is the
   // reference's global majorant).  Same step law and acceptance test as
-  // GridMedia::sample (GridMedia.cpp:80-96).
+  // GridMedia::sample (GridMedia.cpp:80-96), which multiplies by the inverse
+  // majorant: t -= log(1 - xi) * (1 / mu), accept if sigma * (1 / mu) > xi'.
   bool delta_track(V o, V d, T ta, T tb, Rng& rng, T& t_hit) const {
     V oo = inv.point(o), dd = inv.vec(d);
     bool hit = false;
     dda(oo, dd, ta, tb, [&](T c0, T c1, T mu) {
       if (mu <= T(0)) return true;
+      const T imu = T(1) / mu;
       T t = c0;
       while (true) {
         T r = T(rng.next());
-        t -= std::log(T(1) - r) / mu;
+        t -= std::log(T(1) - r) * imu;
         if (t >= c1) return true;
         T sig = density_ocs(oo + t * dd) * T(sigma_bar);
         T ra = T(rng.next());
-        if (sig / mu > ra) { hit = true; t_hit = t; return false; }
+        if (sig * imu > ra) { hit = true; t_hit = t; return false; }
       }
     });
     return hit;
@@ -1008,12 +1011,13 @@ struct World {
     T Tr = T(1);
     dda(oo, dd, ta, tb, [&](T c0, T c1, T mu) {
       if (mu <= T(0)) return true;
+      const T imu = T(1) / mu;
       T t = c0;
       while (true) {
-        t -= std::log(T(1) - T(rng.next())) / mu;
+        t -= std::log(T(1) - T(rng.next())) * imu;
         if (t >= c1) return true;
         T sig = density_ocs(oo + t * dd) * T(sigma_bar);
-        Tr *= T(1) - std::max(T(0), sig / mu);
+        Tr *= T(1) - std::max(T(0), sig * imu);
         if (Tr < T(0.1)) {
           T q = std::max(T(0.05), T(1) - Tr);
           if (T(rng.next()) < q) { Tr = T(0); return false; }
