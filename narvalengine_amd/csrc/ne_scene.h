@@ -42,6 +42,7 @@ constexpr int kSpillStack = 48;
 constexpr uint32_t kItemLeaf = 0x80000000u;
 constexpr uint32_t kItemFirstMask = (1u << 26) - 1;  // slots per primitive type < 2^26
 constexpr uint32_t kItemCountShift = 26;
+constexpr uint32_t kItemNone = 0xFFFFFFFFu;  // no item (leaf type 3 does not exist)
 constexpr uint32_t kItemMaxLeaf = 8;  // count - 1 in 3 bits
 
 struct DevScene {

@@ -194,7 +194,7 @@ template <typename R>
 struct TravState {
   Vec3<R> o, d, invd;
   R tmin, tmax, best;
-  uint32_t cur, bkey, bslot;
+  uint32_t cur, leaf, bkey, bslot;  // leaf: parked leaf item (trav_step_wave), or kItemNone
   int sp, btype;
   bool found;
 };
@@ -212,6 +212,7 @@ NE_DEV void trav_init(TravState<R>& st, Vec3<R> o, Vec3<R> d, R tmin, R tmax) {
   st.tmax = tmax;
   st.best = tmax;
   st.cur = 0;
+  st.leaf = kItemNone;
   st.sp = 0;
   st.found = false;
   st.btype = 0;
@@ -245,47 +246,48 @@ NE_DEV bool trav_pop(TravState<R>& st, const uint32_t* lds, int lane_stride, con
   return false;
 }
 
-// One node visit or one primitive test.  Returns true when the query is
-// complete: the stack is empty, or (ANY) a hit was found.  lds: this lane's
-// short stack (entries lane_stride apart); spill: its private overflow array.
+// Test of the first primitive of leaf item `it`: updates the closest hit and
+// returns true on a hit for ANY.  next: the leaf's next primitive item, or
+// kItemNone after its last.
 template <typename R, bool ANY, bool C>
-NE_DEV bool trav_step(const DevScene& s, TravState<R>& st, uint32_t* lds, int lane_stride, uint32_t* spill,
+NE_DEV bool trav_prim(const DevScene& s, TravState<R>& st, uint32_t it, uint32_t& next, Count<C>& cnt) {
+  const uint32_t ty = (it >> 29) & 3u, k = it & kItemFirstMask;
+  R t;
+  bool ok;
+  uint32_t key = 0;
+  cnt.prim();
+  if (ty == PRIM_TRI) {
+    const float4* tp = s.tris + 3 * size_t(k);
+    ok = isect_tri<R>(tp, st.o, st.d, st.tmin, st.tmax, t);
+    if (ok) key = __float_as_uint(tp[1].w);
+  } else if (ty == PRIM_SPHERE) {
+    ok = isect_sphere<R>(s.sph[k], st.o, st.d, st.tmin, st.tmax, t);
+    if (ok) key = uint32_t(s.sph_orig[k]);
+  } else {
+    const float4* rc = s.rects + 4 * size_t(k);
+    ok = isect_rect<R>(rc, st.o, st.d, st.tmin, st.tmax, t);
+    if (ok) key = __float_as_uint(rc[2].w);
+  }
+  if (ok) {
+    if (ANY) {
+      st.found = true;
+      return true;
+    }
+    if (t < st.best ||
+        (st.found && t == st.best && (int(ty) < st.btype || (int(ty) == st.btype && key < st.bkey)))) {
+      st.best = t; st.found = true; st.btype = int(ty); st.bkey = key; st.bslot = k;
+    }
+  }
+  next = (it & (7u << kItemCountShift)) ? it + 1u - (1u << kItemCountShift) : kItemNone;
+  return false;
+}
+
+// Visit of the inner node st.cur.  Returns true when nothing is left to visit
+// (no child hit and the stack is empty).
+template <typename R, bool C>
+NE_DEV bool trav_node(const DevScene& s, TravState<R>& st, uint32_t* lds, int lane_stride, uint32_t* spill,
                       Count<C>& cnt) {
   const uint32_t it = st.cur;
-  if (it & kItemLeaf) {
-    const uint32_t ty = (it >> 29) & 3u, k = it & kItemFirstMask;
-    R t;
-    bool ok;
-    uint32_t key = 0;
-    cnt.prim();
-    if (ty == PRIM_TRI) {
-      const float4* tp = s.tris + 3 * size_t(k);
-      ok = isect_tri<R>(tp, st.o, st.d, st.tmin, st.tmax, t);
-      if (ok) key = __float_as_uint(tp[1].w);
-    } else if (ty == PRIM_SPHERE) {
-      ok = isect_sphere<R>(s.sph[k], st.o, st.d, st.tmin, st.tmax, t);
-      if (ok) key = uint32_t(s.sph_orig[k]);
-    } else {
-      const float4* rc = s.rects + 4 * size_t(k);
-      ok = isect_rect<R>(rc, st.o, st.d, st.tmin, st.tmax, t);
-      if (ok) key = __float_as_uint(rc[2].w);
-    }
-    if (ok) {
-      if (ANY) {
-        st.found = true;
-        return true;
-      }
-      if (t < st.best ||
-          (st.found && t == st.best && (int(ty) < st.btype || (int(ty) == st.btype && key < st.bkey)))) {
-        st.best = t; st.found = true; st.btype = int(ty); st.bkey = key; st.bslot = k;
-      }
-    }
-    if (it & (7u << kItemCountShift)) {  // more primitives in this leaf
-      st.cur = it + 1u - (1u << kItemCountShift);
-      return false;
-    }
-    return trav_pop(st, lds, lane_stride, spill);
-  }
   // 4-wide inner node (64 B): decode the four quantized child boxes, test
   // them, continue with the nearest hit child and push the others farthest
   // first (entry distances sorted by a five-exchange network; misses sort
@@ -333,6 +335,56 @@ NE_DEV bool trav_step(const DevScene& s, TravState<R>& st, uint32_t* lds, int la
   if (t1 < inf) trav_push(st, i1, lds, lane_stride, spill);
   st.cur = i0;
   return false;
+}
+
+// One node visit or one primitive test.  Returns true when the query is
+// complete: the stack is empty, or (ANY) a hit was found.  lds: this lane's
+// short stack (entries lane_stride apart); spill: its private overflow array.
+template <typename R, bool ANY, bool C>
+NE_DEV bool trav_step(const DevScene& s, TravState<R>& st, uint32_t* lds, int lane_stride, uint32_t* spill,
+                      Count<C>& cnt) {
+  const uint32_t it = st.cur;
+  if (it & kItemLeaf) {
+    uint32_t next;
+    if (trav_prim<R, ANY, C>(s, st, it, next, cnt)) return true;
+    if (next != kItemNone) {  // more primitives in this leaf
+      st.cur = next;
+      return false;
+    }
+    return trav_pop(st, lds, lane_stride, spill);
+  }
+  return trav_node<R, C>(s, st, lds, lane_stride, spill, cnt);
+}
+
+// Speculative traversal for the lane-refill kernels (postponed leaves, Aila
+// and Laine 2009).  Each lane parks up to one leaf in st.leaf and goes on
+// visiting inner nodes past it.  Per call the wave votes: when at most half
+// of its lanes stand at an inner node, every lane with a parked leaf tests one
+// of its primitives; otherwise the lanes at inner nodes visit one node each.
+// The node and primitive paths thus each run with most lanes busy, where a
+// per-lane step runs both on nearly every call.  Visiting order does not
+// change the closest hit (ties break on type and original index); parked
+// leaves only delay the shrinking of the search interval.  Every lane with a
+// live query calls it; returns true when this lane's query is complete.
+template <typename R, bool ANY, bool C>
+NE_DEV bool trav_step_wave(const DevScene& s, TravState<R>& st, uint32_t* lds, int lane_stride, uint32_t* spill,
+                           Count<C>& cnt) {
+  const bool at_node = !(st.cur & kItemLeaf);
+  const unsigned long long busy = __ballot(true), nodes = __ballot(at_node);
+  if (2 * __popcll(nodes) <= __popcll(busy)) {
+    if (st.leaf != kItemNone) {
+      uint32_t next;
+      if (trav_prim<R, ANY, C>(s, st, st.leaf, next, cnt)) return true;
+      st.leaf = next;
+    }
+  } else if (at_node) {
+    if (trav_node<R, C>(s, st, lds, lane_stride, spill, cnt)) st.cur = kItemNone;
+  }
+  if (st.leaf == kItemNone && st.cur != kItemNone && (st.cur & kItemLeaf)) {  // park the leaf in front
+    st.leaf = st.cur;
+    if (trav_pop(st, lds, lane_stride, spill)) st.cur = kItemNone;
+  }
+  return st.cur == kItemNone && st.leaf == kItemNone;
 }
 
 template <typename R, bool ANY, bool C>

@@ -311,7 +311,7 @@ __global__ __launch_bounds__(kChunk) void k_wf_trace(WfState w, WfArgs a, DevSce
       trav_init(st, v3(A.ox[slot], A.oy[slot], A.oz[slot]), v3(A.dx[slot], A.dy[slot], A.dz[slot]), 1e-4f,
                 xinf<float>());
     }
-    if (trav_step<float, false, C>(s, st, stk, kChunk, spill, cnt)) {
+    if (trav_step_wave<float, false, C>(s, st, stk, kChunk, spill, cnt)) {
       A.hit_t[slot] = st.found ? st.best : xinf<float>();
       A.hit_ref[slot] = st.found ? ((uint32_t(st.btype) << 30) | st.bslot) : kNoHit;
       A.kind[slot] = st.found ? uint32_t(WF_KIND_SURFACE) : uint32_t(WF_KIND_MISS);
@@ -762,7 +762,7 @@ __global__ __launch_bounds__(kChunk) void k_wf_shadow_surf(WfState w, WfArgs a, 
       cnt.shadow();
       trav_init(st, v3(S.ox[k], S.oy[k], S.oz[k]), v3(S.dx[k], S.dy[k], S.dz[k]), 1e-4f, S.tmax[k]);
     }
-    if (trav_step<float, true, C>(s, st, stk, kChunk, spill, cnt)) {
+    if (trav_step_wave<float, true, C>(s, st, stk, kChunk, spill, cnt)) {
       if (st.found) S.tmax[k] = -1.0f;  // occluded: marked for the delivery pass
       have = false;
     }

@@ -427,8 +427,10 @@ def test_wavefront_matches_megakernel(ctx, which):
 
 @pytest.mark.parametrize("which", ["volume", "surface"])
 def test_wavefront_counters_and_stage_times(ctx, which):
-    """Both pipelines trace the same paths: equal event counts; the per-launch
-    stage times add up to the whole-render time."""
+    """Both pipelines trace the same paths: equal event counts, except that the
+    wavefront's speculative traversal (parked leaves) visits a superset of the
+    megakernel's BVH nodes and primitives; the per-launch stage times add up
+    to the whole-render time."""
     print_case(f"wavefront counters/stage times {which}")
     from narvalengine_amd.abi import NE_MODE_MEGAKERNEL, NE_MODE_WAVEFRONT
     if which == "volume":
@@ -442,8 +444,10 @@ def test_wavefront_counters_and_stage_times(ctx, which):
         ctx.render(g, cam, f, spp=16, max_bounces=bounces, seed=5, mode=mode, count=True)
         counts[mode] = ctx.counters()
     mk, wf = counts[NE_MODE_MEGAKERNEL], counts[NE_MODE_WAVEFRONT]
-    for k in ("paths", "ext_rays", "shadow_rays", "density_lookups", "majorant_cells", "bvh_nodes", "prim_tests"):
+    for k in ("paths", "ext_rays", "shadow_rays", "density_lookups", "majorant_cells"):
         assert mk[k] == wf[k], (k, mk[k], wf[k])
+    for k in ("bvh_nodes", "prim_tests"):
+        assert mk[k] <= wf[k] <= 1.5 * mk[k], (k, mk[k], wf[k])
     assert wf["paths"] == 40 * 24 * 16
     assert all(mk["shadow_" + k] == 0 for k in ("density_lookups", "majorant_cells", "bvh_nodes", "prim_tests"))
     if which == "volume":
@@ -467,7 +471,9 @@ def test_wavefront_counters_and_stage_times(ctx, which):
 @pytest.mark.parametrize("which", ["surface", "cloud", "emitters"])
 def test_material_sort_leaves_the_image_unchanged(ctx, which):
     """Shading each chunk in (kind, material) order is a permutation of the
-    same paths: the fixed-point film and the event counts are identical."""
+    same paths: the fixed-point film and the event counts are identical (BVH
+    node and primitive counts aside: speculative traversal visits depend on
+    which rays share a wave)."""
     print_case(f"material sort {which}")
     from narvalengine_amd.abi import NE_MODE_WAVEFRONT
     if which == "cloud":
@@ -483,7 +489,10 @@ def test_material_sort_leaves_the_image_unchanged(ctx, which):
         ctx.render(g, cam, f, spp=16, max_bounces=bounces, seed=17, mode=NE_MODE_WAVEFRONT, sort=srt, count=True)
         out.append((f.read(), ctx.counters()))
     np.testing.assert_array_equal(out[0][0], out[1][0])
-    assert out[0][1] == out[1][1]
+    wave_dependent = ("bvh_nodes", "prim_tests", "shadow_bvh_nodes", "shadow_prim_tests")
+    c0, c1 = out[0][1], out[1][1]
+    assert {k: v for k, v in c0.items() if k not in wave_dependent} == \
+        {k: v for k, v in c1.items() if k not in wave_dependent}
 
 
 def test_empty_space_skipping_is_exact(ctx):
