@@ -50,6 +50,10 @@ hipError_t launch_render(const DevScene& s, const DevCamera& cam, const RenderAr
 // block of kChunk threads per chunk and keep each chunk's live paths packed.
 constexpr uint32_t kChunk = 256;
 constexpr uint32_t kMaxRefill = 64;  // chunks per block in the lane-refill kernels
+// Idle lanes a wave waits for before it refills.  Measured on MI355X with
+// surface1m (ms per frame): 1: 444, 8: 427, 16: 421, 24: 416, 32: 417,
+// 48: 459, 64: 771.
+constexpr int kRefillMinIdle = 24;
 // stage ids (the public NE_STAGE_* values)
 enum { kStageMegakernel = 0, kStagePlan, kStageRegen, kStageTrace, kStageMedium, kStageShade, kStageShadow,
        kStageOther, kStageGuide, kStageTrain, kStageCount, kStageEnd = -1 };
@@ -104,6 +108,7 @@ struct WfArgs {
   double* film;
   uint32_t* gstate[2];  // guided mode: training-record state per path buffer (else null)
   int32_t sort;         // shade stage sorts each chunk by (event kind, material)
+  int32_t refill_min;   // refill kernels: a wave refills once this many of its lanes are idle
 };
 
 struct WfGuideHook;  // guided mode: proposal, training records, online training

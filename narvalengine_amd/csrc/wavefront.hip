@@ -286,6 +286,14 @@ __device__ __forceinline__ uint32_t pool_fetch(RefillPool& p, bool need) {
   return pool_chunk(lo) * kChunk + (k - p.pre[lo]);
 }
 
+// Refill vote of the lanes still in a refill loop: true (on every such lane)
+// once at least `min_idle` of them, or all of them, have no ray.  Idle lanes
+// wait in between, so the refill path runs on fewer steps.
+__device__ __forceinline__ bool refill_due(bool have, int min_idle) {
+  const unsigned long long idle = __ballot(!have), looping = __ballot(true);
+  return idle != 0ull && (__popcll(idle) >= min_idle || idle == looping);
+}
+
 // Closest surface hit of every live ray: pure BVH traversal (no random
 // numbers), lane refill over the block's pool.
 template <bool C>
@@ -302,16 +310,18 @@ __global__ __launch_bounds__(kChunk) void k_wf_trace(WfState w, WfArgs a, DevSce
   uint32_t slot = kNoHit;
   bool have = false;
   while (true) {
-    const uint32_t got = pool_fetch(pool, !have);  // every lane still looping calls it
-    if (!have) {
-      if (got == kNoHit) break;
-      slot = got;
-      have = true;
-      cnt.ext();
-      trav_init(st, v3(A.ox[slot], A.oy[slot], A.oz[slot]), v3(A.dx[slot], A.dy[slot], A.dz[slot]), 1e-4f,
-                xinf<float>());
+    if (refill_due(have, a.refill_min)) {
+      const uint32_t got = pool_fetch(pool, !have);  // every lane still looping calls it
+      if (!have) {
+        if (got == kNoHit) break;
+        slot = got;
+        have = true;
+        cnt.ext();
+        trav_init(st, v3(A.ox[slot], A.oy[slot], A.oz[slot]), v3(A.dx[slot], A.dy[slot], A.dz[slot]), 1e-4f,
+                  xinf<float>());
+      }
     }
-    if (trav_step_wave<float, false, C>(s, st, stk, kChunk, spill, cnt)) {
+    if (have && trav_step_wave<float, false, C>(s, st, stk, kChunk, spill, cnt)) {
       A.hit_t[slot] = st.found ? st.best : xinf<float>();
       A.hit_ref[slot] = st.found ? ((uint32_t(st.btype) << 30) | st.bslot) : kNoHit;
       A.kind[slot] = st.found ? uint32_t(WF_KIND_SURFACE) : uint32_t(WF_KIND_MISS);
@@ -754,15 +764,17 @@ __global__ __launch_bounds__(kChunk) void k_wf_shadow_surf(WfState w, WfArgs a, 
   uint32_t k = kNoHit;
   bool have = false;
   while (true) {
-    const uint32_t got = pool_fetch(pool, !have);  // every lane still looping calls it
-    if (!have) {
-      if (got == kNoHit) break;
-      k = got;
-      have = true;
-      cnt.shadow();
-      trav_init(st, v3(S.ox[k], S.oy[k], S.oz[k]), v3(S.dx[k], S.dy[k], S.dz[k]), 1e-4f, S.tmax[k]);
+    if (refill_due(have, a.refill_min)) {
+      const uint32_t got = pool_fetch(pool, !have);  // every lane still looping calls it
+      if (!have) {
+        if (got == kNoHit) break;
+        k = got;
+        have = true;
+        cnt.shadow();
+        trav_init(st, v3(S.ox[k], S.oy[k], S.oz[k]), v3(S.dx[k], S.dy[k], S.dz[k]), 1e-4f, S.tmax[k]);
+      }
     }
-    if (trav_step_wave<float, true, C>(s, st, stk, kChunk, spill, cnt)) {
+    if (have && trav_step_wave<float, true, C>(s, st, stk, kChunk, spill, cnt)) {
       if (st.found) S.tmax[k] = -1.0f;  // occluded: marked for the delivery pass
       have = false;
     }
