@@ -283,8 +283,10 @@ NE_DEV bool trav_prim(const DevScene& s, TravState<R>& st, uint32_t it, uint32_t
 }
 
 // Visit of the inner node st.cur.  Returns true when nothing is left to visit
-// (no child hit and the stack is empty).
-template <typename R, bool C>
+// (no child hit and the stack is empty).  Closest-hit queries go nearest child
+// first; any-hit queries only move the nearest hit child to the front (one
+// pass of three exchanges): any occluder ends the query.
+template <typename R, bool ANY, bool C>
 NE_DEV bool trav_node(const DevScene& s, TravState<R>& st, uint32_t* lds, int lane_stride, uint32_t* spill,
                       Count<C>& cnt) {
   const uint32_t it = st.cur;
@@ -324,11 +326,17 @@ NE_DEV bool trav_node(const DevScene& s, TravState<R>& st, uint32_t* lds, int la
       const uint32_t i = ia; ia = ib; ib = i;
     }
   };
-  order(t0, i0, t1, i1);
-  order(t2, i2, t3, i3);
-  order(t0, i0, t2, i2);
-  order(t1, i1, t3, i3);
-  order(t1, i1, t2, i2);
+  if constexpr (ANY) {
+    order(t2, i2, t3, i3);
+    order(t1, i1, t2, i2);
+    order(t0, i0, t1, i1);
+  } else {
+    order(t0, i0, t1, i1);
+    order(t2, i2, t3, i3);
+    order(t0, i0, t2, i2);
+    order(t1, i1, t3, i3);
+    order(t1, i1, t2, i2);
+  }
   if (!(t0 < inf)) return trav_pop(st, lds, lane_stride, spill);
   if (t3 < inf) trav_push(st, i3, lds, lane_stride, spill);
   if (t2 < inf) trav_push(st, i2, lds, lane_stride, spill);
@@ -353,7 +361,7 @@ NE_DEV bool trav_step(const DevScene& s, TravState<R>& st, uint32_t* lds, int la
     }
     return trav_pop(st, lds, lane_stride, spill);
   }
-  return trav_node<R, C>(s, st, lds, lane_stride, spill, cnt);
+  return trav_node<R, ANY, C>(s, st, lds, lane_stride, spill, cnt);
 }
 
 // Speculative traversal for the lane-refill kernels (postponed leaves, Aila
@@ -378,7 +386,7 @@ NE_DEV bool trav_step_wave(const DevScene& s, TravState<R>& st, uint32_t* lds, i
       st.leaf = next;
     }
   } else if (at_node) {
-    if (trav_node<R, C>(s, st, lds, lane_stride, spill, cnt)) st.cur = kItemNone;
+    if (trav_node<R, ANY, C>(s, st, lds, lane_stride, spill, cnt)) st.cur = kItemNone;
   }
   if (st.leaf == kItemNone && st.cur != kItemNone && (st.cur & kItemLeaf)) {  // park the leaf in front
     st.leaf = st.cur;
@@ -719,6 +727,25 @@ struct Glossy {
     R pspec = D(alpha, h.z) * h.z / (R(4) * xfabs(dot(wo, h)));
     R pdiff = wi.z / R(kPI);
     return ps * pspec + (R(1) - ps) * pdiff;
+  }
+  // eval() and pdf() of one direction pair with the half vector, D, the
+  // Fresnel cosine and the lobe choice computed once (the same values)
+  NE_DEV void eval_pdf(Vec3<R> wo, Vec3<R> wi, Vec3<R>& f, R& p) const {
+    if (wo.z <= R(0) || wi.z <= R(0)) {
+      f = v3s(R(0));
+      p = R(0);
+      return;
+    }
+    Vec3<R> h = normalize(wo + wi);
+    const R dh = D(alpha, h.z), oh = dot(wo, h);
+    R Fh = F(oh);
+    R spec = dh * G1(alpha, wo.z) * G1(alpha, wi.z) * Fh / (R(4) * wo.z * wi.z);
+    R kd = (R(1) - Fh) * (R(1) - metallic);
+    f = albedo * (kd / R(kPI)) + v3s(spec);
+    R ps = p_spec(wo.z);
+    R pspec = dh * h.z / (R(4) * xfabs(oh));
+    R pdiff = wi.z / R(kPI);
+    p = ps * pspec + (R(1) - ps) * pdiff;
   }
   NE_DEV Vec3<R> sample(Vec3<R> wo, R u0, R u1, R u2) const {
     R ps = p_spec(wo.z);
@@ -1093,8 +1120,8 @@ NE_DEV bool path_bounce(const DevScene& s, PathState<R>& p, int max_bounces, Pat
           pf = p;
         } else {
           Vec3<R> wol = to_lcs(wo, ns, ss, ts), wil = to_lcs(wi, ns, ss, ts);
-          f = gm.eval(wol, wil) * xfabs(wil.z);
-          pf = gm.pdf(wol, wil);
+          gm.eval_pdf(wol, wil, f, pf);
+          f = f * xfabs(wil.z);
         }
         if (!is_black(f)) {
           R tmax = xisinf(dist) ? xinf<R>() : dist * R(1 - 1e-4);
@@ -1118,12 +1145,14 @@ NE_DEV bool path_bounce(const DevScene& s, PathState<R>& p, int max_bounces, Pat
       R u0 = R(rng.next()), u1 = R(rng.next()), u2 = R(rng.next());
       Vec3<R> wol = to_lcs(wo, ns, ss, ts);
       Vec3<R> wil = gm.sample(wol, u0, u1, u2);
-      R pf = gm.pdf(wol, wil);
+      Vec3<R> fe;
+      R pf;
+      gm.eval_pdf(wol, wil, fe, pf);
       if (!(pf > R(0))) {
         p.L = L;
         return true;
       }
-      Vec3<R> f = gm.eval(wol, wil) * xfabs(wil.z);
+      Vec3<R> f = fe * xfabs(wil.z);
       if (is_black(f)) {
         p.L = L;
         return true;

@@ -417,8 +417,8 @@ __device__ __forceinline__ bool nee_sample(const DevScene& s, Vec3<float> x, Vec
     pf = p;
   } else {
     const Vec3<float> wol = to_lcs(wo, ns, ss, ts), wil = to_lcs(wi, ns, ss, ts);
-    f = gm.eval(wol, wil) * fabsf(wil.z);
-    pf = gm.pdf(wol, wil);
+    gm.eval_pdf(wol, wil, f, pf);
+    f = f * fabsf(wil.z);
   }
   if (is_black(f)) return false;
   tmax = xisinf(dist) ? xinf<float>() : dist * float(1 - 1e-4);
@@ -645,9 +645,11 @@ __global__ __launch_bounds__(kChunk) void k_wf_shade(WfState w, WfArgs a, DevSce
       const float u0 = rng.next(), u1 = rng.next(), u2 = rng.next();
       const Vec3<float> wol = to_lcs(wo, ns, ss, ts);
       const Vec3<float> wil = gm.sample(wol, u0, u1, u2);
-      const float pf = gm.pdf(wol, wil);
+      Vec3<float> fe;
+      float pf;
+      gm.eval_pdf(wol, wil, fe, pf);
       if (pf > 0.0f) {
-        const Vec3<float> fs = gm.eval(wol, wil) * fabsf(wil.z);
+        const Vec3<float> fs = fe * fabsf(wil.z);
         if (!is_black(fs)) {
           ppdf = pf;
           nd = to_world(wil, ns, ss, ts);
