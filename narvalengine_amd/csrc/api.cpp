@@ -553,6 +553,7 @@ ne_status ne_scene_upload(ne_ctx* c, const ne_scene_desc* d, ne_scene** out) {
       st3[a] = d->vol_sigma_a[a] + d->vol_sigma_s[a];
       ds.sig_s[a] = d->vol_sigma_s[a];
       ds.sig_t[a] = d->vol_sigma_s[a] + d->vol_sigma_a[a];
+      ds.albedo[a] = ds.sig_s[a] / ds.sig_t[a];
     }
     // avg(extinction * densityMultiplier), src/materials/GridMedia.cpp:82
     ds.sigma_bar = (st3[0] * d->vol_mult + st3[1] * d->vol_mult + st3[2] * d->vol_mult) / 3.0f;

@@ -35,6 +35,9 @@ NE_HD double xexp(double x) { return exp(x); }
 NE_HD float xsin(float x) { return sinf(x); }
 NE_HD double xsin(double x) { return sin(x); }
 NE_HD float xcos(float x) { return cosf(x); }
+// sin and cos with one argument reduction (the same values as xsin, xcos)
+NE_HD void xsincos(float x, float& s, float& c) { sincosf(x, &s, &c); }
+NE_HD void xsincos(double x, double& s, double& c) { sincos(x, &s, &c); }
 NE_HD double xcos(double x) { return cos(x); }
 NE_HD float xfabs(float x) { return fabsf(x); }
 NE_HD double xfabs(double x) { return fabs(x); }

@@ -68,6 +68,7 @@ struct DevScene {
   float sigma_bar;        // avg(sigma_t * densityMultiplier)
   float sig_s[3];
   float sig_t[3];
+  float albedo[3];  // sig_s / sig_t, divided once at upload (IEEE division, as on the device)
   float g;
   // lights
   int32_t has_sun;

@@ -666,10 +666,11 @@ NE_DEV Vec3<R> hg_sample(R g, Vec3<R> d, R u0, R u1) {
   }
   ct = xmin(R(1), xmax(R(-1), ct));
   R st = xsqrt(xmax(R(0), R(1) - ct * ct));
-  R ph = R(kTWO_PI) * u1;
+  R ph = R(kTWO_PI) * u1, sp, cp;
+  xsincos(ph, sp, cp);
   Vec3<R> ss, ts;
   onb(d, ss, ts);
-  return to_world(v3(st * xcos(ph), st * xsin(ph), ct), d, ss, ts);
+  return to_world(v3(st * cp, st * sp, ct), d, ss, ts);
 }
 
 // GGXDistribution::D and ::pdf, src/core/Microfacet.cpp:5-15,58-65
@@ -753,12 +754,14 @@ struct Glossy {
       R a2 = alpha * alpha;
       R ct = xmin(R(1), xsqrt((R(1) - u1) / xmax(R(kEPS), u1 * (a2 - R(1)) + R(1))));
       R st = xsqrt(xmax(R(0), R(1) - ct * ct));
-      R ph = R(kTWO_PI) * u2;
-      Vec3<R> h = v3(st * xcos(ph), st * xsin(ph), ct);
+      R ph = R(kTWO_PI) * u2, sp, cp;
+      xsincos(ph, sp, cp);
+      Vec3<R> h = v3(st * cp, st * sp, ct);
       return R(2) * dot(wo, h) * h - wo;
     }
-    R r = xsqrt(u1), ph = R(kTWO_PI) * u2;
-    return v3(r * xcos(ph), r * xsin(ph), xsqrt(xmax(R(0), R(1) - u1)));
+    R r = xsqrt(u1), ph = R(kTWO_PI) * u2, sp, cp;
+    xsincos(ph, sp, cp);
+    return v3(r * cp, r * sp, xsqrt(xmax(R(0), R(1) - u1)));
   }
 };
 
@@ -831,8 +834,9 @@ NE_DEV R env_pdf(const DevScene& s, Vec3<R> d) {
 template <typename R>
 NE_DEV void env_sample(const DevScene& s, R ua, R ub, Vec3<R>& wi, R& pl, Vec3<R>& Le) {
   if (s.env_type != 3) {
-    R z = R(1) - R(2) * ua, r = xsqrt(xmax(R(0), R(1) - z * z)), ph = R(kTWO_PI) * ub;
-    wi = v3(r * xcos(ph), r * xsin(ph), z);
+    R z = R(1) - R(2) * ua, r = xsqrt(xmax(R(0), R(1) - z * z)), ph = R(kTWO_PI) * ub, sp, cp;
+    xsincos(ph, sp, cp);
+    wi = v3(r * cp, r * sp, z);
     Le = env_le<R>(s, wi);
     pl = R(1.0 / kFOUR_PI);
     return;
@@ -848,8 +852,10 @@ NE_DEV void env_sample(const DevScene& s, R ua, R ub, Vec3<R>& wi, R& pl, Vec3<R
   const float du = c1 > c0 ? (fu - c0) / (c1 - c0) : 0.0f;
   const R u = (R(i) + R(du)) / R(W), v = (R(j) + R(dv)) / R(H);
   const R theta = v * R(kPI), phi = u * R(kTWO_PI);
-  const R st = xsin(theta), ct = xcos(theta);
-  wi = v3(-st * xsin(phi), st * xcos(phi), ct);
+  R st, ct, sp, cp;
+  xsincos(theta, st, ct);
+  xsincos(phi, sp, cp);
+  wi = v3(-st * sp, st * cp, ct);
   const size_t k = size_t(j) * size_t(W) + size_t(i);
   pl = st > R(0) ? R(s.env_func[k]) * R(s.env_inv_int) / (R(2.0 * kPI * kPI) * st) : R(0);
   const float* t = s.env_tex + 3 * k;
@@ -923,12 +929,16 @@ NE_DEV bool sample_emitter(const DevScene& s, uint32_t code, Vec3<R> x, R ua, R 
       const R ph = R(kTWO_PI) * ub;
       Vec3<R> u, v;
       onb(w, u, v);
-      wi = u * (st * xcos(ph)) + v * (st * xsin(ph)) + w * ct;
+      R sp, cp;
+      xsincos(ph, sp, cp);
+      wi = u * (st * cp) + v * (st * sp) + w * ct;
       dist = dc * ct - xsqrt(xmax(R(0), r2 - dc2 * st * st));
       pl = R(1) / (R(kTWO_PI) * wid);
     } else {
       const R z = R(1) - R(2) * ua, rr = xsqrt(xmax(R(0), R(1) - z * z)), ph = R(kTWO_PI) * ub;
-      const Vec3<R> nl = v3(rr * xcos(ph), rr * xsin(ph), z);
+      R sp, cp;
+      xsincos(ph, sp, cp);
+      const Vec3<R> nl = v3(rr * cp, rr * sp, z);
       const Vec3<R> dv = (c + r * nl) - x;
       const R d2 = dot(dv, dv);
       dist = xsqrt(d2);

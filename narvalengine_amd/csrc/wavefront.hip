@@ -597,7 +597,7 @@ __global__ __launch_bounds__(kChunk) void k_wf_shade(WfState w, WfArgs a, DevSce
       } else {
         vertex = true;
         medium = true;
-        beta = beta * v3(s.sig_s[0] / s.sig_t[0], s.sig_s[1] / s.sig_t[1], s.sig_s[2] / s.sig_t[2]);
+        beta = beta * v3(s.albedo[0], s.albedo[1], s.albedo[2]);
       }
     }
     if (vertex) x = o + th * d;
