@@ -552,7 +552,9 @@ NE_DEV void majorant_dda(const DevScene& s, Vec3<R> oo, Vec3<R> dd, R ta, R tb, 
     if (skip > 0) {
       --skip;
     } else {
-      const float raw = s.maj[(size_t(cell[2]) * M + cell[1]) * M + cell[0]];
+      // M <= 256: 24-bit multiplies give the exact cell index
+      const float raw = s.maj[__umul24(__umul24(uint32_t(cell[2]), uint32_t(M)) + uint32_t(cell[1]), uint32_t(M)) +
+                              uint32_t(cell[0])];
       if (raw < 0.0f) skip = int(-raw) - 1;
       else if (raw > 0.0f && tc > t && !f(t, tc, R(raw))) return;
     }
