@@ -79,6 +79,23 @@ def measured_traffic(kernel: str, workload: str):
     return None
 
 
+def measured_valu(kernel: str, workload: str):
+    """VALU issue fraction and active-lane fraction of `kernel` on `workload`
+    from the committed rocprofv3 PMC summary, or None.  The hot stages are
+    bound by vector-ALU issue, which is why their HBM fractions stay low."""
+    f = PROFILES / "valu_utilization.txt"
+    if not f.exists():
+        return None
+    for line in f.read_text().splitlines():
+        parts = line.split()
+        if len(parts) >= 3 and parts[0] == workload and parts[1].startswith(kernel + "<"):
+            issue, lanes = parts[-2], parts[-1]
+            return {"resource": "valu", "issue_frac": round(float(issue.rstrip("%")) / 100, 2),
+                    "lanes_active_frac": round(float(lanes.rstrip("%")) / 100, 2),
+                    "source": "profiles/valu_utilization.txt"}
+    return None
+
+
 def log(msg):
     sys.stderr.write(f"[bench] {msg}\n")
     sys.stderr.flush()
@@ -394,7 +411,7 @@ def main():
                     "frac": round(achieved / HBM_PEAK_GBS, 5),
                     "traffic": measured_traffic(kernel, wl.name),
                     "kernel": kernel, "algorithmic_bytes_per_launch": d["algorithmic_bytes_per_launch"],
-                    "avg_launch_ms": d["avg_launch_ms"]}
+                    "avg_launch_ms": d["avg_launch_ms"], "limiter": measured_valu(kernel, wl.name)}
     cpu = None if args.no_cpu else cpu_baseline(wl, args.cpu_seconds)
     out = {
         "metric": "samples/sec", "value": round(value, 1), "unit": "samples/s", "n_gpus": 1,
