@@ -443,6 +443,14 @@ ne_status ne_scene_upload(ne_ctx* c, const ne_scene_desc* d, ne_scene** out) {
     delete s;
     return e;
   };
+  // a failing HIP call releases the partly built scene
+#define NE_HIP_BAIL(call)                                                                          \
+  do {                                                                                             \
+    hipError_t e_ = (call);                                                                        \
+    if (e_ != hipSuccess)                                                                          \
+      return bail(fail(c, e_ == hipErrorOutOfMemory ? NE_ERR_OUT_OF_MEMORY : NE_ERR_HIP,           \
+                       std::string(#call) + ": " + hipGetErrorString(e_)));                       \
+  } while (0)
   DevScene& ds = s->dev;
   // ---- surfaces: BVH over triangles, spheres, rectangles.  Each type is
   // stored in leaf order, and leaf children point at their type's storage.
@@ -563,20 +571,20 @@ ne_status ne_scene_upload(ne_ctx* c, const ne_scene_desc* d, ne_scene** out) {
     if ((st = dev_upload(c, s, homog ? &kOne : d->grid, nvox, &ds.grid)) != NE_OK) return bail(st);
     size_t ncell = size_t(ds.mres) * ds.mres * ds.mres;
     void* pm = nullptr;
-    NE_HIP(c, hipMalloc(&pm, ncell * sizeof(float)));
+    NE_HIP_BAIL(hipMalloc(&pm, ncell * sizeof(float)));
     s->allocs.push_back(pm);
     s->bytes += ncell * sizeof(float);
-    NE_HIP(c, launch_majorants(ds, static_cast<float*>(pm), c->stream));
+    NE_HIP_BAIL(launch_majorants(ds, static_cast<float*>(pm), c->stream));
     ds.maj = static_cast<const float*>(pm);
     std::vector<float> mh(ncell);
-    NE_HIP(c, hipMemcpyAsync(mh.data(), pm, ncell * sizeof(float), hipMemcpyDeviceToHost, c->stream));
-    NE_HIP(c, hipStreamSynchronize(c->stream));
+    NE_HIP_BAIL(hipMemcpyAsync(mh.data(), pm, ncell * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+    NE_HIP_BAIL(hipStreamSynchronize(c->stream));
     for (float v : mh) gmax = std::max(gmax, v);
     gmax = ds.sigma_bar > 0 ? gmax / ds.sigma_bar : 0.0f;
     if (!d->maj_no_skip && std::any_of(mh.begin(), mh.end(), [](float v) { return !(v > 0.0f); })) {
       encode_empty_distance(mh, ds.mres);
-      NE_HIP(c, hipMemcpyAsync(pm, mh.data(), ncell * sizeof(float), hipMemcpyHostToDevice, c->stream));
-      NE_HIP(c, hipStreamSynchronize(c->stream));
+      NE_HIP_BAIL(hipMemcpyAsync(pm, mh.data(), ncell * sizeof(float), hipMemcpyHostToDevice, c->stream));
+      NE_HIP_BAIL(hipStreamSynchronize(c->stream));
     }
   }
   // ---- lights
@@ -596,7 +604,7 @@ ne_status ne_scene_upload(ne_ctx* c, const ne_scene_desc* d, ne_scene** out) {
     if ((st = dev_upload(c, s, et.marg.data(), et.marg.size(), &ds.env_marg)) != NE_OK) return bail(st);
   }
   ds.n_lights = ds.has_sun + (ds.env_type != NE_ENV_NONE ? 1 : 0) + ds.n_emitters;
-  NE_HIP(c, hipStreamSynchronize(c->stream));
+  NE_HIP_BAIL(hipStreamSynchronize(c->stream));
   s->info.bvh_nodes = ds.n_nodes;
   s->info.bvh_depth = bvh.max_depth;
   s->info.bvh_leaf_prims_max = bvh.max_leaf;
@@ -607,6 +615,7 @@ ne_status ne_scene_upload(ne_ctx* c, const ne_scene_desc* d, ne_scene** out) {
   *out = s;
   return NE_OK;
 }
+#undef NE_HIP_BAIL
 
 void ne_scene_release(ne_scene* s) {
   if (!s) return;
@@ -709,11 +718,14 @@ static ne_status render_wavefront(ne_ctx* c, const ne_scene* s, const DevCamera&
   if (P > c->wf_P) {
     if (c->wf_buf) {
       NE_HIP(c, hipStreamSynchronize(c->stream));
-      NE_HIP(c, hipFree(c->wf_buf));
+      void* old = c->wf_buf;  // cleared first: a failing free never leaves it to be freed twice
       c->wf_buf = nullptr;
       c->wf_P = 0;
+      NE_HIP(c, hipFree(old));
     }
-    NE_HIP(c, hipMalloc(&c->wf_buf, wf_bytes(uint32_t(P))));
+    void* nb = nullptr;
+    NE_HIP(c, hipMalloc(&nb, wf_bytes(uint32_t(P))));
+    c->wf_buf = nb;
     c->wf_P = uint32_t(P);
   }
   if (!f->d_fixed) {
@@ -746,11 +758,14 @@ static ne_status render_wavefront(ne_ctx* c, const ne_scene* s, const DevCamera&
     if (P > c->wfg_P) {
       if (c->wfg_buf) {
         NE_HIP(c, hipStreamSynchronize(c->stream));
-        NE_HIP(c, hipFree(c->wfg_buf));
+        void* old = c->wfg_buf;
         c->wfg_buf = nullptr;
         c->wfg_P = 0;
+        NE_HIP(c, hipFree(old));
       }
-      NE_HIP(c, hipMalloc(&c->wfg_buf, wf_guide_bytes(uint32_t(P))));
+      void* nb = nullptr;
+      NE_HIP(c, hipMalloc(&nb, wf_guide_bytes(uint32_t(P))));
+      c->wfg_buf = nb;
       c->wfg_P = uint32_t(P);
     }
     wf_guide_carve(c->wfg_buf, uint32_t(P), gh->g);
@@ -818,11 +833,14 @@ static ne_status render_locked(ne_ctx* c, const ne_scene* s, const ne_camera_des
     if (need > c->partial_bytes) {
       if (c->d_partial) {
         NE_HIP(c, hipStreamSynchronize(c->stream));
-        NE_HIP(c, hipFree(c->d_partial));
+        void* old = c->d_partial;
         c->d_partial = nullptr;
         c->partial_bytes = 0;
+        NE_HIP(c, hipFree(old));
       }
-      NE_HIP(c, hipMalloc(&c->d_partial, need));
+      void* nb = nullptr;
+      NE_HIP(c, hipMalloc(&nb, need));
+      c->d_partial = static_cast<decltype(c->d_partial)>(nb);
       c->partial_bytes = need;
     }
     a.out = c->d_partial;
@@ -1161,14 +1179,18 @@ ne_status batch_prepare(ne_batch* b, const ne_batch_config& cfg, int div) {
   }
   if (items > b->item_cap) {
     NE_HIP(c, hipStreamSynchronize(c->stream));
-    if (b->d_item_time) NE_HIP(c, hipFree(b->d_item_time));
+    // handles are cleared before the calls that release them, so a failing
+    // call never leaves one to be released twice
+    void* old_times = b->d_item_time;
     b->d_item_time = nullptr;
     b->item_cap = 0;
-    if (b->exec) {
-      NE_HIP(c, hipGraphExecDestroy(b->exec));
-      b->exec = nullptr;
-    }
-    NE_HIP(c, hipMalloc(&b->d_item_time, items * sizeof(unsigned long long)));
+    hipGraphExec_t old_exec = b->exec;
+    b->exec = nullptr;
+    if (old_times) NE_HIP(c, hipFree(old_times));
+    if (old_exec) NE_HIP(c, hipGraphExecDestroy(old_exec));
+    void* p = nullptr;
+    NE_HIP(c, hipMalloc(&p, items * sizeof(unsigned long long)));
+    b->d_item_time = static_cast<unsigned long long*>(p);
     b->item_cap = items;
   }
   NE_HIP(c, hipMemcpyAsync(b->d_jobs, table.data(), table.size() * sizeof(BatchJob), hipMemcpyHostToDevice,
@@ -1178,8 +1200,9 @@ ne_status batch_prepare(ne_batch* b, const ne_batch_config& cfg, int div) {
   if (same) return NE_OK;
   if (b->exec) {
     NE_HIP(c, hipStreamSynchronize(c->stream));
-    NE_HIP(c, hipGraphExecDestroy(b->exec));
+    hipGraphExec_t old_exec = b->exec;
     b->exec = nullptr;
+    NE_HIP(c, hipGraphExecDestroy(old_exec));
   }
   BatchArgs a{};
   a.jobs = b->d_jobs;
