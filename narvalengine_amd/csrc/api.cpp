@@ -581,6 +581,21 @@ ne_status ne_scene_upload(ne_ctx* c, const ne_scene_desc* d, ne_scene** out) {
     NE_HIP_BAIL(hipStreamSynchronize(c->stream));
     for (float v : mh) gmax = std::max(gmax, v);
     gmax = ds.sigma_bar > 0 ? gmax / ds.sigma_bar : 0.0f;
+    // bounds of the cells with a positive majorant (the DDA clips rays to them)
+    {
+      const int M = ds.mres;
+      for (int a = 0; a < 3; ++a) { ds.occ_lo[a] = M; ds.occ_hi[a] = -1; }
+      for (int z = 0; z < M; ++z)
+        for (int y = 0; y < M; ++y)
+          for (int x = 0; x < M; ++x)
+            if (mh[(size_t(z) * M + y) * M + x] > 0.0f) {
+              const int cxyz[3] = {x, y, z};
+              for (int a = 0; a < 3; ++a) {
+                ds.occ_lo[a] = std::min(ds.occ_lo[a], cxyz[a]);
+                ds.occ_hi[a] = std::max(ds.occ_hi[a], cxyz[a]);
+              }
+            }
+    }
     if (!d->maj_no_skip && std::any_of(mh.begin(), mh.end(), [](float v) { return !(v > 0.0f); })) {
       encode_empty_distance(mh, ds.mres);
       NE_HIP_BAIL(hipMemcpyAsync(pm, mh.data(), ncell * sizeof(float), hipMemcpyHostToDevice, c->stream));

@@ -62,6 +62,8 @@ struct DevScene {
   int32_t homog;          // homogeneous medium: density 1 everywhere in the cube
   int32_t res[3];
   int32_t mres;
+  int32_t occ_lo[3], occ_hi[3];  // majorant cells with a positive majorant lie in [occ_lo, occ_hi]
+                                 // (occ_lo > occ_hi: none)
   float inv[12];          // WCS -> OCS, rows of a 3x4 matrix
   const float* grid;
   const float* maj;

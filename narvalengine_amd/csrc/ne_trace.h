@@ -527,6 +527,25 @@ NE_DEV void majorant_dda(const DevScene& s, Vec3<R> oo, Vec3<R> dd, R ta, R tb, 
   const R Mr = R(M), invM = R(1) / Mr;
   int cell[3], step[3];
   R tnext[3], inv[3];
+  // clip [ta, tb) to the box of the cells with a positive majorant: the box
+  // faces are cell boundaries in the DDA's own arithmetic, so the cells and
+  // boundary distances inside are those of a walk from the volume box
+#pragma unroll
+  for (int a = 0; a < 3; ++a) {
+    const R da = comp(dd, a), oa = comp(oo, a);
+    inv[a] = da != R(0) ? R(1) / da : R(0);
+    const R blo = R(s.occ_lo[a]) * invM - R(0.5), bhi = R(s.occ_hi[a] + 1) * invM - R(0.5);
+    if (s.occ_lo[a] > s.occ_hi[a]) {
+      tb = ta;  // no cell has a positive majorant
+    } else if (da != R(0)) {
+      const R t0 = (blo - oa) * inv[a], t1 = (bhi - oa) * inv[a];
+      ta = xmax(ta, xmin(t0, t1));
+      tb = xmin(tb, xmax(t0, t1));
+    } else if (!(oa >= blo && oa <= bhi)) {
+      tb = ta;  // parallel to the box and outside it
+    }
+  }
+  if (!(ta < tb)) return;
   Vec3<R> pe = oo + ta * dd;
 #pragma unroll
   for (int a = 0; a < 3; ++a) {
@@ -535,7 +554,6 @@ NE_DEV void majorant_dda(const DevScene& s, Vec3<R> oo, Vec3<R> dd, R ta, R tb, 
     c = c < 0 ? 0 : (c > M - 1 ? M - 1 : c);
     cell[a] = c;
     R da = comp(dd, a), oa = comp(oo, a);
-    inv[a] = da != R(0) ? R(1) / da : R(0);
     if (da > R(0)) { step[a] = 1; tnext[a] = ((R(c + 1) * invM - R(0.5)) - oa) * inv[a]; }
     else if (da < R(0)) { step[a] = -1; tnext[a] = ((R(c) * invM - R(0.5)) - oa) * inv[a]; }
     else { step[a] = 0; tnext[a] = xinf<R>(); }

@@ -646,6 +646,7 @@ struct World {
   Mat4<T> xf, inv;
   int mres = 1;
   std::vector<float> maj;      // majorant of sigma_t per cell (mres^3)
+  int occ_lo[3] = {0, 0, 0}, occ_hi[3] = {-1, -1, -1};  // cells with a positive majorant lie in [occ_lo, occ_hi]
   float sigma_bar = 0;         // avg(sigma_t * mult)
   EnvTables env;               // environment-map sampling tables
   int n_lights = 0;            // sun? + env? + emissive primitives
@@ -702,6 +703,17 @@ struct World {
               for (int x = lo[0]; x <= hi[0]; ++x) m = std::max(m, grid.at(x, y, z));
           maj[(size_t(cz) * mres + cy) * mres + cx] = m * sigma_bar;
         }
+    for (int a = 0; a < 3; ++a) { occ_lo[a] = mres; occ_hi[a] = -1; }
+    for (int cz = 0; cz < mres; ++cz)
+      for (int cy = 0; cy < mres; ++cy)
+        for (int cx = 0; cx < mres; ++cx)
+          if (maj[(size_t(cz) * mres + cy) * mres + cx] > 0.0f) {
+            const int c[3] = {cx, cy, cz};
+            for (int a = 0; a < 3; ++a) {
+              occ_lo[a] = std::min(occ_lo[a], c[a]);
+              occ_hi[a] = std::max(occ_hi[a], c[a]);
+            }
+          }
   }
 
   void prim_bounds(const PrimRef& p, float mn[3], float mx[3]) const {
@@ -950,13 +962,29 @@ struct World {
     const T M = T(mres), invM = T(1) / M;
     int cell[3], step[3];
     T tnext[3], inv[3];
+    // The walk is clipped to the box of the cells with a positive majorant:
+    // its faces are cell boundaries in the walk's own arithmetic, so the cells
+    // and boundary distances inside are those of a walk from the volume box.
+    for (int a = 0; a < 3; ++a) {
+      inv[a] = dd[a] != T(0) ? T(1) / dd[a] : T(0);  // cell boundaries: multiply by 1 / d
+      const T blo = T(occ_lo[a]) * invM - T(0.5), bhi = T(occ_hi[a] + 1) * invM - T(0.5);
+      if (occ_lo[a] > occ_hi[a]) {
+        tb = ta;
+      } else if (dd[a] != T(0)) {
+        const T t0 = (blo - oo[a]) * inv[a], t1 = (bhi - oo[a]) * inv[a];
+        ta = std::max(ta, std::min(t0, t1));
+        tb = std::min(tb, std::max(t0, t1));
+      } else if (!(oo[a] >= blo && oo[a] <= bhi)) {
+        tb = ta;
+      }
+    }
+    if (!(ta < tb)) return;
     V pe = oo + ta * dd;
     for (int a = 0; a < 3; ++a) {
       T g = (pe[a] + T(0.5)) * M;
       int c = int(std::floor(g));
       c = std::min(std::max(c, 0), mres - 1);
       cell[a] = c;
-      inv[a] = dd[a] != T(0) ? T(1) / dd[a] : T(0);  // cell boundaries: multiply by 1 / d
       if (dd[a] > T(0)) { step[a] = 1; tnext[a] = ((T(c + 1) * invM - T(0.5)) - oo[a]) * inv[a]; }
       else if (dd[a] < T(0)) { step[a] = -1; tnext[a] = ((T(c) * invM - T(0.5)) - oo[a]) * inv[a]; }
       else { step[a] = 0; tnext[a] = std::numeric_limits<T>::infinity(); }
