@@ -237,6 +237,60 @@ def test_ratio_tracking_expectation_is_exp_minus_optical_depth(linear):
     assert (tr >= 0).all()
 
 
+def _sparse_volume(res=32, mult=6.0, maj_res=8):
+    """A dense block in the middle of an otherwise empty grid: most majorant
+    cells are empty, and the box of the non-empty ones is smaller than the
+    volume, so the walk's clipping and empty-cell skipping are exercised."""
+    z, y, x = np.meshgrid(np.arange(res), np.arange(res), np.arange(res), indexing="ij")
+    grid = (0.4 + 0.6 * (x / (res - 1))).astype(np.float32)
+    grid[(x < 10) | (x > 21) | (y < 12) | (y > 23) | (z < 9) | (z > 20)] = 0.0
+    s = Scene()
+    s.set_volume(grid, mult, (1.0,) * 3, (0.0,) * 3, 0.0, maj_res=maj_res)
+    return s, grid
+
+
+def _optical_depth(grid, mult, o, d, steps=20000):
+    """Midpoint quadrature of mult * trilinear density along the ray through
+    the OCS unit cube [-0.5, 0.5]^3."""
+    D, H, W = grid.shape
+    with np.errstate(divide="ignore"):
+        inv = 1.0 / d
+    t0 = np.nanmax(np.minimum((-0.5 - o) * inv, (0.5 - o) * inv))
+    t1 = np.nanmin(np.maximum((-0.5 - o) * inv, (0.5 - o) * inv))
+    if not t0 < t1:
+        return 0.0
+    ts = t0 + (np.arange(steps) + 0.5) * (t1 - t0) / steps
+    gcs = ((o[None, :] + ts[:, None] * d[None, :] + 0.5) * np.array([W, H, D])).astype(np.float32)
+    gcs = np.clip(gcs, 0, None)
+    dens = np.zeros(steps, np.float32)
+    L().neo_interp_density(W, H, D, O.fp(grid), steps, O.fp(np.ascontiguousarray(gcs)), O.fp(dens))
+    return float(mult * dens.astype(np.float64).sum() * (t1 - t0) / steps)
+
+
+@pytest.mark.parametrize("ray", [((-0.9, 0.05, 0.02), (1.0, 0.0, 0.0)),
+                                 ((-0.8, -0.7, -0.6), (1.0, 0.9, 0.75)),
+                                 ((0.1, -0.9, 0.3), (-0.05, 1.0, -0.2)),
+                                 ((-0.9, 0.26, 0.0), (1.0, 0.02, 0.01))])
+def test_tracking_through_a_sparse_grid_follows_the_optical_depth(ray):
+    """Ratio tracking averages to exp(-tau) and delta tracking collides with
+    probability 1 - exp(-tau) along rays through, across and past the edge of
+    a dense block in an empty grid (tau by quadrature of the trilinear field).
+    A walk that clipped or skipped a non-empty cell would miss optical depth."""
+    s, grid = _sparse_volume()
+    o = np.array(ray[0], np.float64)
+    d = np.array(ray[1], np.float64)
+    d /= np.linalg.norm(d)
+    tau = _optical_depth(grid, 6.0, o, d)
+    w = O.World(s)
+    n = 60000
+    rays = np.tile(np.concatenate([o, d]), (n, 1)).astype(np.float32)
+    tr = w.transmittance(rays, np.full(n, np.inf, np.float32), seed=5)
+    assert abs(tr.mean() - np.exp(-tau)) < 5 * tr.std() / np.sqrt(n) + 2e-3
+    sc, _ = w.delta_track(rays, seed=6)
+    p = 1 - np.exp(-tau)
+    assert abs(sc.mean() - p) < 5 * np.sqrt(max(p * (1 - p), 1e-6) / n) + 2e-3
+
+
 # ---- post-processing, camera, tiles -------------------------------------------
 def test_post_processing_special_values():
     c = np.array([[0, 0, 0], [1e4, 1e4, 1e4], [2 * np.log(2)] * 3, [0.25, 0.5, 3]], np.float32)
