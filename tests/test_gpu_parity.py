@@ -551,3 +551,28 @@ def test_wavefront_pool_rounds(ctx):
     np.testing.assert_array_equal(films[0], films[1])
     np.testing.assert_array_equal(films[0], films[2])
     g.close()
+
+
+def test_tracking_on_a_sparse_grid_matches_oracle(ctx):
+    """A dense block in an empty grid (clipped walk, empty-cell skipping):
+    device delta-tracking trajectories and ratio-tracking transmittances
+    agree with the oracle's, ray by ray."""
+    print_case("sparse-grid tracking")
+    import test_oracle_invariants as inv
+    s, _ = inv._sparse_volume()
+    g = ctx.upload(s)
+    w = O.World(s)
+    n = 20000
+    rays = scenes.rays_towards(n, target_box=0.4, origin_radius=1.2, seed=31)
+    sc, t = ctx.test_delta_track(g, rays, seed=7)
+    osc, ot = w.delta_track(rays, seed=7)
+    same = (sc == osc) & (np.abs(t - ot) <= 1e-5 * np.maximum(np.abs(ot), 1e-6))
+    print(f"sparse delta tracking identical: {same.mean():.5f}, collisions {sc.mean():.3f}")
+    assert same.mean() > 0.99 and 0.05 < sc.mean() < 0.95
+    tmax = np.full(n, np.inf, np.float32)
+    tr = ctx.test_transmittance(g, rays, tmax, seed=8)
+    otr = w.transmittance(rays, tmax, seed=8)
+    same = np.abs(tr - otr) <= 1e-4 * np.maximum(otr, 1e-6)
+    print(f"sparse transmittance identical: {same.mean():.5f}")
+    assert same.mean() > 0.99
+    g.close()
