@@ -226,6 +226,64 @@ def rel_mse(x, ref, eps=1e-2):
     return float(np.mean((x - ref) ** 2 / (ref ** 2 + eps)))
 
 
+def bench_quality(args):
+    """Image error of each image workload: relMSE of the plain GPU render
+    (FP32 wavefront, the workload's spp) against an FP64 converged render of
+    the same scene (FP64 megakernel at ref_mult x spp; the FP64 path is
+    pinned to the oracle's FP64 integrator by the parity tests), and a CPU
+    cross-check: the oracle's own FP64 render of a 16 x 16 pixel block
+    against the same block of the converged render."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle as O
+    from narvalengine_amd import Context, configs
+    from narvalengine_amd.abi import NE_FP64, NE_MODE_MEGAKERNEL, NE_MODE_WAVEFRONT
+    ctx = Context(0)
+    threads = max(1, min(16, os.cpu_count() or 1))
+    results = {}
+    for name in ("volume128", "surface1m", "cloud512"):
+        wl = configs.CONFIGS[name]()
+        g = ctx.upload(wl.scene)
+        W, H = wl.width, wl.height
+        f = ctx.film(W, H)
+        ctx.render(g, wl.camera, f, spp=wl.spp, max_bounces=wl.max_bounces, seed=1, mode=NE_MODE_WAVEFRONT)
+        img = f.read() / wl.spp
+        f.close()
+        ref_spp = wl.spp * args.ref_mult
+        fr = ctx.film(W, H)
+        t = time.perf_counter()
+        for k in range(args.ref_mult):  # in launches of the workload's spp (progress lines in between)
+            ctx.render(g, wl.camera, fr, spp=wl.spp, sample_offset=k * wl.spp, max_bounces=wl.max_bounces, seed=2,
+                       precision=NE_FP64, mode=NE_MODE_MEGAKERNEL)
+            log(f"{name}: FP64 reference {k + 1}/{args.ref_mult}")
+        t_ref = time.perf_counter() - t
+        ref = fr.read() / ref_spp
+        fr.close()
+        e = rel_mse(img, ref)
+        # the reference's own noise adds spp / ref_spp of the image's error
+        e_corr = e / (1.0 + wl.spp / ref_spp)
+        x0, y0 = W // 2 - 8, H // 2 - 8
+        rect = (x0, y0, x0 + 16, y0 + 16)
+        t = time.perf_counter()
+        cpu = O.World(wl.scene, fp64=True).render(wl.camera, W, H, args.cpu_block_spp, wl.max_bounces, rect=rect,
+                                                   seed=3, threads=threads)
+        t_cpu = time.perf_counter() - t
+        blk = cpu[y0:y0 + 16, x0:x0 + 16] / args.cpu_block_spp
+        rblk = ref[y0:y0 + 16, x0:x0 + 16]
+        results[name] = {"spp": wl.spp, "reference_spp": ref_spp, "relmse": e, "relmse_minus_reference_noise": e_corr,
+                         "reference_seconds": round(t_ref, 2),
+                         "cpu_fp64_block": {"rect": list(rect), "spp": args.cpu_block_spp,
+                                            "mean_ratio_to_reference": float(blk.sum() / max(rblk.sum(), 1e-30)),
+                                            "relmse_to_reference": rel_mse(blk, rblk), "seconds": round(t_cpu, 2),
+                                            "threads": threads}}
+        log(f"{name}: relMSE {e:.3e} (reference {ref_spp} spp FP64, {t_ref:.1f} s), CPU block ratio "
+            f"{results[name]['cpu_fp64_block']['mean_ratio_to_reference']:.4f}")
+        g.close()
+    print(json.dumps({"metric": "relMSE vs FP64 converged render", "unit": "relMSE (mean (x-y)^2/(y^2+0.01))",
+                      "n_gpus": 1, "higher_is_better": False, "dtype": "f32 (reference f64)",
+                      "data": "synthetic (seeded procedural scenes)", "ref_mult": args.ref_mult,
+                      "workloads": results}))
+
+
 def bench_guided(args):
     """Learned proposal vs plain HG sampling at equal wall time on the 512^3
     cloud (the cloud512 scene, camera and bounce limit at 480x270): relMSE of
@@ -310,7 +368,7 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="surface1m",
-                    choices=["surface1m", "volume128", "cloud512", "batch48", "guided"])
+                    choices=["surface1m", "volume128", "cloud512", "batch48", "guided", "quality"])
     ap.add_argument("--mode", default="auto", choices=["auto", "megakernel", "wavefront"])
     ap.add_argument("--pool", type=int, default=0, help="wavefront path slots (0 = library default)")
     ap.add_argument("--sort", action="store_true", help="wavefront: shade chunks in (event kind, material) order")
@@ -323,6 +381,8 @@ def main():
     ap.add_argument("--samples-per-thread", type=int, default=0)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--plain-spp", type=int, default=1024, help="guided workload: plain run's spp (sets the time)")
+    ap.add_argument("--ref-mult", type=int, default=8, help="quality workload: reference spp = ref_mult x spp")
+    ap.add_argument("--cpu-block-spp", type=int, default=1024, help="quality workload: oracle FP64 block spp")
     ap.add_argument("--guided-vertices", type=int, default=1)
     ap.add_argument("--guide-prob", type=float, default=0.25)
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -341,6 +401,9 @@ def main():
         return
     if args.workload == "guided":
         bench_guided(args)
+        return
+    if args.workload == "quality":
+        bench_quality(args)
         return
 
     from narvalengine_amd import Context, configs
