@@ -130,7 +130,9 @@ enum {
   NE_FLAG_COUNT = 1,      /* collect event counters (separate, slower pass) */
   NE_FLAG_TIME = 2,       /* time the path-tracing kernels with HIP events on the context stream */
   NE_FLAG_STAGE_TIME = 4, /* also time every kernel launch and sum the times per stage (NE_STAGE_*) */
-  NE_FLAG_SORT = 8        /* wavefront: shade each chunk's paths ordered by (event kind, material) */
+  NE_FLAG_SORT = 8,       /* wavefront: shade each chunk's paths ordered by (event kind, material) */
+  NE_FLAG_FIXED_FILM = 16 /* wavefront: accumulate through 2^-30 fixed-point film atomics, the path taken
+                             anyway when the render's per-sample radiance slots (12 B each) exceed 16 GiB */
 };
 
 /* Kernel stages reported by ne_get_stage_times. */

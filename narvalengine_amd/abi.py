@@ -9,6 +9,7 @@ NE_FLAG_COUNT = 1
 NE_FLAG_TIME = 2
 NE_FLAG_STAGE_TIME = 4
 NE_FLAG_SORT = 8
+NE_FLAG_FIXED_FILM = 16
 STAGES = ("megakernel", "plan", "regen", "trace", "medium", "shade", "shadow", "other", "guide", "train")
 NE_MODE_AUTO, NE_MODE_MEGAKERNEL, NE_MODE_WAVEFRONT = 0, 1, 2
 NE_MAT_GLOSSY, NE_MAT_EMITTER = 0, 1

@@ -44,6 +44,8 @@ struct ne_ctx {
   uint32_t refill_blocks = 0;  // resident lane-refill blocks on the device (CUs x blocks per CU)
   void* wfg_buf = nullptr;      // guided-mode training records (grown on demand)
   uint32_t wfg_P = 0;
+  void* sbuf = nullptr;         // wavefront per-sample radiance (grown on demand)
+  size_t sbuf_bytes = 0;
 };
 
 struct ne_scene {
@@ -405,6 +407,7 @@ static void ctx_free(ne_ctx* c) {
   if (c->h_status) (void)hipHostFree(c->h_status);
   for (hipEvent_t e : c->ev_pool) (void)hipEventDestroy(e);
   if (c->wfg_buf) (void)hipFree(c->wfg_buf);
+  if (c->sbuf) (void)hipFree(c->sbuf);
   if (c->ev0) (void)hipEventDestroy(c->ev0);
   if (c->ev1) (void)hipEventDestroy(c->ev1);
   if (c->stream) (void)hipStreamDestroy(c->stream);
@@ -719,6 +722,8 @@ ne_status ne_film_tonemap(ne_film* f, int32_t spp, float* pixels) {
   return NE_OK;
 }
 
+constexpr size_t kMaxSampleBuffer = size_t(16) << 30;
+
 static ne_status render_wavefront(ne_ctx* c, const ne_scene* s, const DevCamera& dc, const ne_render_params* p,
                                   ne_film* f, bool count, bool timed, StageTimer* tm, WfGuideHook* gh = nullptr) {
   const uint64_t total = uint64_t(p->x1 - p->x0) * uint64_t(p->y1 - p->y0) * uint64_t(p->spp);
@@ -768,6 +773,28 @@ static ne_status render_wavefront(ne_ctx* c, const ne_scene* s, const DevCamera&
   a.counters = c->d_counters;
   a.film_fixed = f->d_fixed;
   a.film = f->d_sum;
+  // Per-sample radiance slots replace the film atomics when the render's
+  // samples fit in kMaxSampleBuffer (12 B each; 6.4 GB for 1920x1080 at 256
+  // spp); larger renders use the fixed-point film.
+  a.sbuf = nullptr;
+  const size_t sbytes = size_t(total) * 3 * sizeof(float);
+  if (sbytes <= kMaxSampleBuffer && !(p->flags & NE_FLAG_FIXED_FILM)) {
+    if (sbytes > c->sbuf_bytes) {
+      NE_HIP(c, hipStreamSynchronize(c->stream));
+      void* old = c->sbuf;
+      c->sbuf = nullptr;
+      c->sbuf_bytes = 0;
+      if (old) NE_HIP(c, hipFree(old));
+      void* nb = nullptr;
+      if (hipMalloc(&nb, sbytes) == hipSuccess) {
+        c->sbuf = nb;
+        c->sbuf_bytes = sbytes;
+      } else {
+        (void)hipGetLastError();  // no room: the fixed-point film takes this render
+      }
+    }
+    a.sbuf = sbytes <= c->sbuf_bytes ? static_cast<float*>(c->sbuf) : nullptr;
+  }
   a.gstate[0] = a.gstate[1] = nullptr;
   if (gh) {
     if (P > c->wfg_P) {

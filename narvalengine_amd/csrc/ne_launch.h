@@ -105,6 +105,7 @@ struct WfArgs {
   int32_t cur;
   DevCounters* counters;
   long long* film_fixed;
+  float* sbuf;  // per-sample radiance (3 floats per path of the render), or null: fixed-point film atomics
   double* film;
   uint32_t* gstate[2];  // guided mode: training-record state per path buffer (else null)
   int32_t sort;         // shade stage sorts each chunk by (event kind, material)

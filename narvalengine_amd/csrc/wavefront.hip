@@ -113,6 +113,23 @@ __device__ __forceinline__ void path_pixel(const WfArgs& a, unsigned long long i
   pix = uint32_t(a.y0 + int(py)) * uint32_t(a.W) + uint32_t(a.x0 + int(px));
 }
 
+// Per-sample radiance: with a.sbuf, a path's radiance goes to its own slot
+// (sample - sample_offset) * rect pixels + rect pixel, one plain store when
+// the path ends; k_wf_flush_samples then sums each pixel's samples in sample
+// order in FP64 and adds the fixed-point film, which still takes the final
+// vertices' light samples.  Without a.sbuf the fixed-point film takes
+// everything.  Both sums are independent of completion order.
+__device__ __forceinline__ size_t sample_slot(const WfArgs& a, uint32_t pix, uint32_t smp) {
+  const uint32_t px = pix % uint32_t(a.W), py = pix / uint32_t(a.W);
+  const size_t rw = size_t(a.x1 - a.x0), rh = size_t(a.y1 - a.y0);
+  return (size_t(smp - a.sample_offset) * rh + size_t(py - uint32_t(a.y0))) * rw + size_t(px - uint32_t(a.x0));
+}
+__device__ __forceinline__ void sample_store(const WfArgs& a, uint32_t pix, uint32_t smp, float r, float g,
+                                             float b) {
+  float* q = a.sbuf + 3 * sample_slot(a, pix, smp);
+  q[0] = r; q[1] = g; q[2] = b;
+}
+
 // One block: exclusive scan of the chunks' free slots -> each chunk's first new
 // path id and count; advances the work counter; writes the number of paths in
 // flight this iteration to status[0].  The chunks go in rounds of
@@ -683,7 +700,8 @@ __global__ __launch_bounds__(kChunk) void k_wf_shade(WfState w, WfArgs a, DevSce
       }
     }
   } else if (ends) {
-    film_add(a.film_fixed, pix, lx, ly, lz);
+    if (a.sbuf) sample_store(a, pix, smp, lx, ly, lz);
+    else film_add(a.film_fixed, pix, lx, ly, lz);
   }
   if constexpr (GUIDED) {
     // a recording path that ends now becomes a training sample
@@ -741,6 +759,8 @@ __device__ __forceinline__ void shadow_deliver(const WfShadow& S, const WfPaths&
   const float cx = S.cx[k] * q, cy = S.cy[k] * q, cz = S.cz[k] * q;
   const uint32_t tg = S.target[k];
   if (tg & kToFilm) {
+    // the final vertex's light sample: a fire-and-forget atomic on the
+    // fixed-point film (a read-modify-write of the sample slot would wait)
     film_add(a.film_fixed, tg & ~kToFilm, cx, cy, cz);
   } else {
     const uint32_t b = (k / kChunk) * kChunk + tg;
@@ -842,6 +862,30 @@ __global__ __launch_bounds__(kChunk) void k_wf_shadow(WfState w, WfArgs a, DevSc
 }
 
 // film += fixed / 2^30 and clear the fixed-point buffer
+// film pixel += sum of its samples (FP64, in sample order) + its fixed-point
+// sum, which is cleared
+__global__ void k_wf_flush_samples(const float* sbuf, long long* fixed, double* film, int W, int x0, int y0, int rw,
+                                   int rh, uint32_t spp) {
+  const size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x, npix = size_t(rw) * size_t(rh);
+  if (p >= npix) return;
+  double r = 0.0, g = 0.0, b = 0.0;
+  for (uint32_t s = 0; s < spp; ++s) {
+    const float* q = sbuf + 3 * (size_t(s) * npix + p);
+    r += double(q[0]);
+    g += double(q[1]);
+    b += double(q[2]);
+  }
+  const size_t pix = (size_t(y0) + p / size_t(rw)) * size_t(W) + size_t(x0) + p % size_t(rw);
+  long long* fx = fixed + 3 * pix;
+  const double inv = 1.0 / 1073741824.0;
+  film[3 * pix] += r + double(fx[0]) * inv;
+  film[3 * pix + 1] += g + double(fx[1]) * inv;
+  film[3 * pix + 2] += b + double(fx[2]) * inv;
+  fx[0] = 0;
+  fx[1] = 0;
+  fx[2] = 0;
+}
+
 __global__ void k_wf_flush_film(long long* fixed, double* film, size_t n) {
   size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -1007,7 +1051,15 @@ hipError_t launch_wavefront(const DevScene& s, const DevCamera& cam, WfState& w,
   }
   const size_t nf = size_t(a.W) * size_t(a.H) * 3;
   NE_MARK(kStageOther);
-  hipLaunchKernelGGL(k_wf_flush_film, dim3(uint32_t((nf + 255) / 256)), dim3(256), 0, st, a.film_fixed, a.film, nf);
+  if (a.sbuf) {
+    const int rw = a.x1 - a.x0, rh = a.y1 - a.y0;
+    const size_t npix = size_t(rw) * size_t(rh);
+    const uint32_t spp = uint32_t(a.total_paths / npix);
+    hipLaunchKernelGGL(k_wf_flush_samples, dim3(uint32_t((npix + 255) / 256)), dim3(256), 0, st, a.sbuf,
+                       a.film_fixed, a.film, a.W, a.x0, a.y0, rw, rh, spp);
+  } else {
+    hipLaunchKernelGGL(k_wf_flush_film, dim3(uint32_t((nf + 255) / 256)), dim3(256), 0, st, a.film_fixed, a.film, nf);
+  }
   NE_MARK(kStageEnd);
 #undef NE_MARK
   return hipGetLastError();

@@ -576,3 +576,32 @@ def test_tracking_on_a_sparse_grid_matches_oracle(ctx):
     print(f"sparse transmittance identical: {same.mean():.5f}")
     assert same.mean() > 0.99
     g.close()
+
+
+@pytest.mark.parametrize("which", ["volume", "surface", "emitters"])
+def test_per_sample_film_matches_fixed_point_film(ctx, which):
+    """The wavefront's two film paths: per-sample radiance slots summed per
+    pixel in FP64 (the default while the slots fit) and 2^-30 fixed-point
+    atomics (larger renders) give the same image up to the fixed-point
+    rounding, and each is reproducible."""
+    print_case(f"per-sample vs fixed-point film {which}")
+    from narvalengine_amd.abi import NE_MODE_WAVEFRONT
+    if which == "volume":
+        s, cam, bounces = scenes.small_volume(res=32, maj_res=4), scenes.VOLUME_CAMERA, 16
+    elif which == "emitters":
+        s, cam, bounces = scenes.emitter_mix(), scenes.BOX_CAMERA, 8
+    else:
+        s, cam, bounces = scenes.small_surface(subdiv=3), scenes.BOX_CAMERA, 8
+    g = ctx.upload(s)
+    out = {}
+    for fixed in (False, True, False):
+        f = ctx.film(40, 24)
+        ctx.render(g, cam, f, spp=32, max_bounces=bounces, seed=19, mode=NE_MODE_WAVEFRONT, fixed_film=fixed,
+                   pool_size=3000)
+        out.setdefault(fixed, []).append(f.read())
+        f.close()
+    np.testing.assert_array_equal(out[False][0], out[False][1])
+    # up to 32 samples x a few contributions each, rounded to 2^-30
+    np.testing.assert_allclose(out[True][0], out[False][0], rtol=1e-6, atol=256 * 2.0 ** -30)
+    assert out[False][0].sum() > 0
+    g.close()
