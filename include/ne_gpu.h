@@ -239,7 +239,10 @@ ne_status ne_render(ne_ctx* ctx, const ne_scene* scene, const ne_camera_desc* ca
  * pixels, tile `index` = (index % 40, index / 40) (OfflineEngine.cpp:18-21,
  * 54-58).  Clears the tile's film sums, renders params->spp samples of that
  * tile, synchronises, and writes post-processed pixels of the tile into
- * pixels (W*H*3 floats); other pixels of the array are left untouched. */
+ * pixels (W*H*3 floats); other pixels of the array are left untouched.
+ * Below 40 x 10 pixels the tile size is 0 on an axis: as in renderTile, the
+ * tile is empty and the call renders and writes nothing (NE_OK).  Indices
+ * outside [0, 400) and spp <= 0 are argument errors. */
 ne_status ne_render_tile(ne_ctx* ctx, const ne_scene* scene, const ne_camera_desc* cam,
                          const ne_render_params* params, int32_t index, ne_film* film, float* pixels);
 /* Duration in milliseconds of the path-tracing kernel of the last render

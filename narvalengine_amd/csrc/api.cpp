@@ -918,11 +918,16 @@ ne_status ne_render_tile(ne_ctx* c, const ne_scene* s, const ne_camera_desc* cam
   // OfflineEngine tiling: tileSize = resolution / (40, 10), tile (index % 40, index / 40)
   const int tw = p->width / 40, th = p->height / 10;
   if (index < 0 || index >= 400) return fail(c, NE_ERR_INVALID_ARG, "tile index out of range [0, 400)");
-  if (tw <= 0 || th <= 0) return fail(c, NE_ERR_INVALID_ARG, "image smaller than the 40x10 tile grid");
   std::lock_guard<std::mutex> lk(c->mu);
-  ne_render_params q = *p;
-  q.x0 = (index % 40) * tw; q.y0 = (index / 40) * th; q.x1 = q.x0 + tw; q.y1 = q.y0 + th;
   if (f->W != p->width || f->H != p->height) return fail(c, NE_ERR_INVALID_ARG, "film size differs from params");
+  // Below 40 x 10 pixels the integer tile size is 0 on an axis and renderTile's
+  // loops run zero times: the tile is empty, nothing is rendered or written.
+  ne_render_params q = *p;
+  if (tw <= 0 || th <= 0) {
+    q.x0 = 0; q.y0 = 0; q.x1 = p->width; q.y1 = p->height;  // the tile rectangle is not used
+    return check_params(c, &q);
+  }
+  q.x0 = (index % 40) * tw; q.y0 = (index / 40) * th; q.x1 = q.x0 + tw; q.y1 = q.y0 + th;
   NE_HIP(c, hipSetDevice(c->device));
   // renderTile recomputes the tile: its film sums restart from zero
   NE_HIP(c, launch_clear_rect(f->d_sum, f->W, q.x0, q.y0, q.x1, q.y1, c->stream));

@@ -366,6 +366,25 @@ def test_tile_api_matches_full_frame(ctx):
     np.testing.assert_allclose(ref.reshape(-1, 3), pp, rtol=1e-5, atol=1e-6)
 
 
+def test_tiles_of_an_image_below_the_tile_grid_are_empty(ctx):
+    """OfflineEngine.cpp:20-21,59-60: with W < 40 the integer tile width is 0,
+    renderTile's pixel loops run zero times and the pixels stay as they were."""
+    print_case("empty tiles")
+    from narvalengine_amd import NarvalError, OfflineEngine, SceneSettings
+    s = scenes.small_volume(res=8, maj_res=2)
+    eng = OfflineEngine(scenes.VOLUME_CAMERA, SceneSettings(resolution=(32, 24), spp=2, bounces=4), s, ctx=ctx)
+    eng.pixels[:] = 0.25
+    for i in (0, 17, 399):
+        eng.renderTile(scenes.VOLUME_CAMERA, i)
+    assert (eng.pixels == 0.25).all()
+    assert not eng.film.read().any()
+    with pytest.raises(NarvalError, match="tile index"):
+        eng.renderTile(scenes.VOLUME_CAMERA, 400)
+    eng.settings.spp = 0
+    with pytest.raises(NarvalError, match="spp"):
+        eng.renderTile(scenes.VOLUME_CAMERA, 0)
+
+
 def test_invalid_arguments_are_reported(ctx):
     print_case("errors")
     from narvalengine_amd import NarvalError
