@@ -10,6 +10,9 @@ cloud512 grid, uploaded exactly as the benchmark uploads them.
   minus their taxicab distance to the nearest non-empty cell; delta-tracking
   trajectories and ratio-tracking transmittances match the oracle ray by ray;
   megakernel and wavefront renders of the cloud trace the same paths.
+* batch48: the 48 uneven jobs in one persistent launch; the three smallest
+  jobs are bitwise equal to their own single-job batches, and one equals the
+  single-job megakernel render up to the film rounding.
 """
 import numpy as np
 import pytest
@@ -132,3 +135,35 @@ def test_full_cloud_same_paths_in_both_pipelines(ctx, cloud):
     assert np.isfinite(films[0]).all() and (films[0] >= 0).all()
     assert films[0].mean() > 0.01
     np.testing.assert_allclose(films[1], films[0], rtol=1e-6, atol=64 * 2.0 ** -30)
+
+
+def test_full_batch48_jobs_are_independent_of_the_batch(ctx):
+    print_case("batch48 full batch vs single jobs")
+    from narvalengine_amd import Batch
+    from narvalengine_amd.abi import NE_MODE_MEGAKERNEL
+    wls = configs.batch48()
+    jobs = []
+    for k, w in enumerate(wls):
+        jobs.append(dict(scene=ctx.upload(w.scene), camera=w.camera, width=w.width, height=w.height, spp=w.spp,
+                         max_bounces=w.max_bounces, seed=1, job=k))
+    assert len(jobs) == 48
+    b = Batch(ctx, jobs)
+    b.run()
+    sums = [b.read(k) for k in range(48)]
+    for k, j in enumerate(jobs):
+        assert np.isfinite(sums[k]).all() and sums[k].mean() > 0, k
+    small = sorted(range(48), key=lambda k: jobs[k]["width"] * jobs[k]["height"] * jobs[k]["spp"])[:3]
+    for k in small:
+        one = Batch(ctx, [jobs[k]])
+        one.run()
+        np.testing.assert_array_equal(one.read(0), sums[k])
+        one.close()
+    j = jobs[small[0]]
+    f = ctx.film(j["width"], j["height"])
+    ctx.render(j["scene"], j["camera"], f, spp=j["spp"], max_bounces=j["max_bounces"], seed=j["seed"], job=j["job"],
+               mode=NE_MODE_MEGAKERNEL)
+    np.testing.assert_allclose(sums[small[0]], f.read(), rtol=1e-6, atol=j["spp"] * 2.0 ** -30)
+    f.close()
+    b.close()
+    for j in jobs:
+        j["scene"].close()
