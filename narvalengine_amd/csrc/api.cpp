@@ -933,16 +933,15 @@ ne_status ne_render_tile(ne_ctx* c, const ne_scene* s, const ne_camera_desc* cam
   NE_HIP(c, launch_clear_rect(f->d_sum, f->W, q.x0, q.y0, q.x1, q.y1, c->stream));
   ne_status st = render_locked(c, s, cam, &q, f);
   if (st != NE_OK) return st;
-  const int npix = f->W * f->H;
+  // post-process the tile only, then one strided copy into the caller's
+  // full-frame pixel array
   DevBuf b;
-  if ((st = tmp_out<float>(c, b, size_t(npix) * 3)) != NE_OK) return st;
-  NE_HIP(c, launch_tonemap(f->d_sum, npix, 1.0f / float(p->spp), static_cast<float*>(b.p), c->stream));
-  // copy the tile rows into the caller's full-frame pixel array
-  for (int y = q.y0; y < q.y1; ++y) {
-    size_t off = (size_t(y) * f->W + q.x0) * 3;
-    NE_HIP(c, hipMemcpyAsync(pixels + off, static_cast<float*>(b.p) + off, size_t(tw) * 3 * sizeof(float),
-                             hipMemcpyDeviceToHost, c->stream));
-  }
+  if ((st = tmp_out<float>(c, b, size_t(tw) * th * 3)) != NE_OK) return st;
+  NE_HIP(c, launch_tonemap_rect(f->d_sum, f->W, q.x0, q.y0, tw, th, 1.0f / float(p->spp), static_cast<float*>(b.p),
+                                c->stream));
+  const size_t row = size_t(tw) * 3 * sizeof(float);
+  NE_HIP(c, hipMemcpy2DAsync(pixels + (size_t(q.y0) * f->W + q.x0) * 3, size_t(f->W) * 3 * sizeof(float), b.p, row,
+                             row, size_t(th), hipMemcpyDeviceToHost, c->stream));
   NE_HIP(c, hipStreamSynchronize(c->stream));
   return NE_OK;
 }

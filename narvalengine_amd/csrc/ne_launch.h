@@ -246,6 +246,9 @@ hipError_t launch_guide_adam(const GuideNet& g, float lr_table, float lr_mlp, fl
 hipError_t launch_reduce_partials(const RenderArgs& a, double* film, hipStream_t st);
 hipError_t launch_clear_rect(double* film, int W, int x0, int y0, int x1, int y1, hipStream_t st);
 hipError_t launch_tonemap(const double* film, int npix, float inv_spp, float* pixels, hipStream_t st);
+// Post-processes the film pixels of [x0, x0 + rw) x [y0, y0 + rh) into a packed rw x rh x 3 array.
+hipError_t launch_tonemap_rect(const double* film, int W, int x0, int y0, int rw, int rh, float inv_spp, float* out,
+                               hipStream_t st);
 
 hipError_t launch_test_primitive(int kind, int n, const float* rays, const float* prims, int32_t* hit, float* t0,
                                  float* t1, hipStream_t st);

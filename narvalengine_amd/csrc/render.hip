@@ -92,13 +92,23 @@ __global__ void k_reduce_partials(RenderArgs a, double* film) {
 
 // OfflineEngine::postProcessing (src/core/OfflineEngine.cpp:27-52):
 // clamp(pow(1 - exp(-0.5 c), 1/2.2), 0, 1) per channel.
+__device__ __forceinline__ float post_process(double sum, float inv_spp) {
+  float c = float(sum * double(inv_spp));
+  float m = 1.0f - expf(-c * 0.5f);
+  m = powf(m, 1.0f / 2.2f);
+  return xmin(xmax(m, 0.0f), 1.0f);
+}
 __global__ void k_tonemap(const double* film, int n, float inv_spp, float* pixels) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n * 3) return;
-  float c = float(film[i] * double(inv_spp));
-  float m = 1.0f - expf(-c * 0.5f);
-  m = powf(m, 1.0f / 2.2f);
-  pixels[i] = xmin(xmax(m, 0.0f), 1.0f);
+  pixels[i] = post_process(film[i], inv_spp);
+}
+// The same for one pixel rectangle, packed: out[(y * rw + x) * 3 + ch].
+__global__ void k_tonemap_rect(const double* film, int W, int x0, int y0, int rw, int rh, float inv_spp, float* out) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= rw * rh * 3) return;
+  const int ch = i % 3, p = i / 3, x = p % rw, y = p / rw;
+  out[i] = post_process(film[(size_t(y0 + y) * W + size_t(x0 + x)) * 3 + ch], inv_spp);
 }
 
 // Zeroes the film sums of the pixel rectangle [x0,x1) x [y0,y1).
@@ -153,6 +163,14 @@ hipError_t launch_reduce_partials(const RenderArgs& a, double* film, hipStream_t
 
 hipError_t launch_tonemap(const double* film, int npix, float inv_spp, float* pixels, hipStream_t st) {
   hipLaunchKernelGGL(k_tonemap, dim3(uint32_t((npix * 3 + 255) / 256)), dim3(256), 0, st, film, npix, inv_spp, pixels);
+  return hipGetLastError();
+}
+hipError_t launch_tonemap_rect(const double* film, int W, int x0, int y0, int rw, int rh, float inv_spp, float* out,
+                               hipStream_t st) {
+  const int n = rw * rh * 3;
+  if (n <= 0) return hipSuccess;
+  hipLaunchKernelGGL(k_tonemap_rect, dim3(uint32_t((n + 255) / 256)), dim3(256), 0, st, film, W, x0, y0, rw, rh,
+                     inv_spp, out);
   return hipGetLastError();
 }
 
