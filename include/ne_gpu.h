@@ -305,7 +305,7 @@ ne_status ne_batch_read(ne_batch* batch, int32_t job, double* sum_rgb);
  * direction feed an MLP 35 -> 64 x 4 (ReLU) -> 40 on BF16 MFMA tiles (FP32
  * accumulation) that outputs an 8-lobe von Mises-Fisher mixture.  It is trained
  * online with Adam on the tracer's own radiance samples and combined with HG
- * sampling by one-sample MIS (0.5 / 0.5), so the estimator stays unbiased.
+ * sampling by one-sample MIS ((1 - guide_prob) HG + guide_prob guide), so the estimator stays unbiased.
  * New component: the reference has no learned sampling. */
 typedef struct ne_guide_config {
   int32_t log2_table; /* hash entries per level = 2^log2_table (default 19, range 10..22) */

@@ -200,14 +200,25 @@ ne_status validate_desc(ne_ctx* c, const ne_scene_desc* d) {
   if (d->n_tri > kItemFirstMask || d->n_sph > kItemFirstMask || d->n_rect > kItemFirstMask)
     return fail(c, NE_ERR_INVALID_ARG, "too many primitives of one type (limit 2^26)");
   auto chk_mat = [&](int32_t m) { return m >= 0 && uint32_t(m) < d->n_mat; };
-  for (uint32_t i = 0; i < d->n_tri; ++i)
+  // Every coordinate must be finite: the BVH builder cannot bound a primitive
+  // with an infinite or NaN extent, and a box that misses it would drop hits.
+  auto finite = [](const float* v, size_t n) {
+    for (size_t k = 0; k < n; ++k)
+      if (!std::isfinite(v[k])) return false;
+    return true;
+  };
+  for (uint32_t i = 0; i < d->n_tri; ++i) {
     if (!chk_mat(d->tri_mat ? d->tri_mat[i] : 0)) return fail(c, NE_ERR_INVALID_ARG, "triangle material out of range");
+    if (!finite(d->tri_v + 9 * size_t(i), 9)) return fail(c, NE_ERR_INVALID_ARG, "triangle vertices must be finite");
+  }
   for (uint32_t i = 0; i < d->n_sph; ++i) {
     if (!chk_mat(d->sph_mat[i])) return fail(c, NE_ERR_INVALID_ARG, "sphere material out of range");
+    if (!finite(d->sph + 4 * size_t(i), 4)) return fail(c, NE_ERR_INVALID_ARG, "sphere centre and radius must be finite");
     if (!(d->sph[4 * i + 3] > 0)) return fail(c, NE_ERR_INVALID_ARG, "sphere radius must be > 0");
   }
   for (uint32_t i = 0; i < d->n_rect; ++i) {
     if (!chk_mat(d->rect_mat[i])) return fail(c, NE_ERR_INVALID_ARG, "rectangle material out of range");
+    if (!finite(d->rect + 12 * size_t(i), 12)) return fail(c, NE_ERR_INVALID_ARG, "rectangle corners must be finite");
     if (!(rect_area(d->rect + 12 * i + 3, d->rect + 12 * i + 6) > 0))
       return fail(c, NE_ERR_INVALID_ARG, "degenerate rectangle");
   }
@@ -363,6 +374,7 @@ ne_status ne_bvh_build(const ne_scene_desc* d, float* nodes, uint32_t* refs, uin
   if (st != NE_OK) return st;
   BvhOutput bvh;
   build_bvh(prim_bounds(d), bvh);
+  if (bvh.quant_failed) return fail(nullptr, NE_ERR_INVALID_ARG, "BVH quantization failed: non-finite bounds");
   if (n_nodes) *n_nodes = uint32_t(bvh.nodes.size() / kNodeFloats);
   if (n_refs) *n_refs = uint32_t(bvh.refs.size());
   if (depth) *depth = bvh.max_depth;
@@ -460,6 +472,7 @@ ne_status ne_scene_upload(ne_ctx* c, const ne_scene_desc* d, ne_scene** out) {
   const uint32_t nt = d->n_tri, ns = d->n_sph, nr = d->n_rect;
   BvhOutput bvh;
   build_bvh(prim_bounds(d), bvh);
+  if (bvh.quant_failed) return bail(fail(c, NE_ERR_INVALID_ARG, "BVH quantization failed: non-finite bounds"));
   if (bvh.max_stack > uint32_t(kShortStack + kSpillStack)) return bail(fail(c, NE_ERR_UNSUPPORTED, "BVH too deep"));
   if (bvh.max_leaf > kItemMaxLeaf)
     return bail(fail(c, NE_ERR_UNSUPPORTED, "degenerate geometry: a BVH leaf holds more than 8 primitives"));
@@ -823,13 +836,21 @@ static ne_status render_wavefront(ne_ctx* c, const ne_scene* s, const DevCamera&
   return NE_OK;
 }
 
-static ne_status render_locked(ne_ctx* c, const ne_scene* s, const ne_camera_desc* cam, const ne_render_params* p,
-                               ne_film* f) {
+// Every argument check of a render call, made before anything touches the film.
+static ne_status check_render(ne_ctx* c, const ne_scene* s, const ne_camera_desc* cam, const ne_render_params* p,
+                              const ne_film* f) {
   ne_status st = check_params(c, p);
   if (st != NE_OK) return st;
   if (!s || !cam || !f) return fail(c, NE_ERR_INVALID_ARG, "scene/camera/film is NULL");
   if (s->ctx != c || f->ctx != c) return fail(c, NE_ERR_INVALID_ARG, "scene/film belong to another context");
   if (f->W != p->width || f->H != p->height) return fail(c, NE_ERR_INVALID_ARG, "film size differs from params");
+  return NE_OK;
+}
+
+static ne_status render_locked(ne_ctx* c, const ne_scene* s, const ne_camera_desc* cam, const ne_render_params* p,
+                               ne_film* f) {
+  ne_status st = check_render(c, s, cam, p, f);
+  if (st != NE_OK) return st;
   NE_HIP(c, hipSetDevice(c->device));
   const bool count = (p->flags & NE_FLAG_COUNT) != 0;
   const bool timed = (p->flags & NE_FLAG_TIME) != 0;
@@ -919,6 +940,7 @@ ne_status ne_render_tile(ne_ctx* c, const ne_scene* s, const ne_camera_desc* cam
   const int tw = p->width / 40, th = p->height / 10;
   if (index < 0 || index >= 400) return fail(c, NE_ERR_INVALID_ARG, "tile index out of range [0, 400)");
   std::lock_guard<std::mutex> lk(c->mu);
+  if (f->ctx != c) return fail(c, NE_ERR_INVALID_ARG, "scene/film belong to another context");
   if (f->W != p->width || f->H != p->height) return fail(c, NE_ERR_INVALID_ARG, "film size differs from params");
   // Below 40 x 10 pixels the integer tile size is 0 on an axis and renderTile's
   // loops run zero times: the tile is empty, nothing is rendered or written.
@@ -928,10 +950,13 @@ ne_status ne_render_tile(ne_ctx* c, const ne_scene* s, const ne_camera_desc* cam
     return check_params(c, &q);
   }
   q.x0 = (index % 40) * tw; q.y0 = (index / 40) * th; q.x1 = q.x0 + tw; q.y1 = q.y0 + th;
+  // a rejected call leaves the film as it was
+  ne_status st = check_render(c, s, cam, &q, f);
+  if (st != NE_OK) return st;
   NE_HIP(c, hipSetDevice(c->device));
   // renderTile recomputes the tile: its film sums restart from zero
   NE_HIP(c, launch_clear_rect(f->d_sum, f->W, q.x0, q.y0, q.x1, q.y1, c->stream));
-  ne_status st = render_locked(c, s, cam, &q, f);
+  st = render_locked(c, s, cam, &q, f);
   if (st != NE_OK) return st;
   // post-process the tile only, then one strided copy into the caller's
   // full-frame pixel array
@@ -1000,6 +1025,11 @@ ne_status ne_get_counters(ne_ctx* c, ne_counters* out) {
 ne_status ne_li(ne_ctx* c, const ne_scene* s, int32_t n, const float* rays, int32_t max_bounces, uint64_t seed,
                 uint32_t sample_index, float* radiance) {
   if (!c || !s || n < 0 || (n > 0 && (!rays || !radiance))) return fail(c, NE_ERR_INVALID_ARG, "bad ne_li arguments");
+  // the shadow-ray sub-streams of a path are keyed by vertex below 2^11: a
+  // longer path would run them into the main stream's counter range
+  if (max_bounces < 0 || max_bounces > kMaxVertices)
+    return fail(c, NE_ERR_INVALID_ARG, "max_bounces out of range [0, 2047]");
+  if (s->ctx != c) return fail(c, NE_ERR_INVALID_ARG, "scene belongs to another context");
   if (n == 0) return NE_OK;
   std::lock_guard<std::mutex> lk(c->mu);
   NE_HIP(c, hipSetDevice(c->device));

@@ -265,7 +265,8 @@ void build_bvh(const std::vector<PrimInput>& prims, BvhOutput& out) {
       const float origin = parent.mn[a];
       f[a] = origin;
       const double ext = double(parent.mx[a]) - double(origin);
-      int e = ext > 0 ? int(std::ceil(std::log2(ext / 254.0))) : -126;
+      int e = ext > 0 && std::isfinite(ext) ? int(std::ceil(std::log2(ext / 254.0))) : -126;
+      if (!std::isfinite(ext)) e = 127;
       e = std::min(std::max(e, -126), 127);
       // smallest scale 2^e whose decoded planes origin + q * 2^e (FP32, as the
       // traversal computes them) contain every child box
@@ -285,6 +286,9 @@ void build_bvh(const std::vector<PrimInput>& prims, BvhOutput& out) {
           hi[c] = uint32_t(ih);
         }
         if (ok || e == 127) {
+          // no scale contains the children: only possible with non-finite
+          // bounds, which upload rejects; never emit a box that is too small
+          if (!ok) out.quant_failed = true;
           for (int c = 0; c < n; ++c) {
             q[a] |= lo[c] << (8 * c);
             q[3 + a] |= hi[c] << (8 * c);

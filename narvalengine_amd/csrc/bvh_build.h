@@ -21,6 +21,7 @@ struct BvhOutput {
   uint32_t max_depth = 0;       // wide-node levels
   uint32_t max_stack = 0;       // deepest traversal stack (children pushed along any root path)
   uint32_t max_leaf = 0;
+  bool quant_failed = false;    // some child box has no conservative 8-bit encoding (non-finite bounds)
 };
 
 void build_bvh(const std::vector<PrimInput>& prims, BvhOutput& out);

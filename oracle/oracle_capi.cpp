@@ -288,6 +288,21 @@ void neo_world_delta_track(void* p, int n, const float* ray, uint64_t seed, int*
     }
   }
 }
+// Integrator::Li for a batch of rays (src/integrators/Integrator.h:23): ray i
+// draws from the stream (seed; i, sample, 0), as the device's ne_li keys it.
+void neo_li(void* p, int n, const float* ray, int max_bounces, uint64_t seed, uint32_t sample, double* out) {
+  OWorld* w = static_cast<OWorld*>(p);
+  for (int i = 0; i < n; ++i) {
+    Rng r(seed, uint32_t(i), sample, 0);
+    if (w->prec) {
+      V3<double> L = Li<double>(*w->wd, V3<double>(v3(ray + 6 * i)), V3<double>(v3(ray + 6 * i + 3)), max_bounces, r);
+      out[3 * i] = L.x; out[3 * i + 1] = L.y; out[3 * i + 2] = L.z;
+    } else {
+      V3f L = Li<float>(*w->wf, v3(ray + 6 * i), v3(ray + 6 * i + 3), max_bounces, r);
+      out[3 * i] = L.x; out[3 * i + 1] = L.y; out[3 * i + 2] = L.z;
+    }
+  }
+}
 // Full render of pixels [x0,x1)x[y0,y1): sum += per-pixel radiance sums.
 // stats (optional): [extension rays, shadow rays]
 void neo_render(void* p, const CameraDesc* cam, int W, int H, int x0, int y0, int x1, int y1, int spp,

@@ -76,6 +76,7 @@ def lib():
             "neo_world_closest_brute": (None, [C.c_void_p, C.c_int, _F, C.c_float, C.c_float, _F, _I, _I]),
             "neo_world_transmittance": (None, [C.c_void_p, C.c_int, _F, _F, C.c_uint64, _D]),
             "neo_world_delta_track": (None, [C.c_void_p, C.c_int, _F, C.c_uint64, _I, _D]),
+            "neo_li": (None, [C.c_void_p, C.c_int, _F, C.c_int, C.c_uint64, C.c_uint32, _D]),
             "neo_render": (None, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                   C.c_int, C.c_int, C.c_uint64, C.c_uint32, C.c_uint32, C.c_int, _D,
                                   C.POINTER(C.c_uint64)]),
@@ -215,6 +216,13 @@ class World:
         sc, t = np.zeros(len(rays), np.int32), np.zeros(len(rays), np.float64)
         lib().neo_world_delta_track(self.h, len(rays), fp(rays), seed, ip(sc), dp(t))
         return sc, t
+
+    def li(self, rays, max_bounces, seed=0, sample=0):
+        """Radiance along each ray (n, 6) -> (n, 3) float64; ray i uses stream (seed; i, sample, 0)."""
+        rays = f(rays).reshape(-1, 6)
+        out = np.zeros((len(rays), 3), np.float64)
+        lib().neo_li(self.h, len(rays), fp(rays), max_bounces, seed, sample, dp(out))
+        return out
 
     def env_sample(self, u2):
         u2 = f(u2).reshape(-1, 2)

@@ -175,3 +175,28 @@ def test_invalid_scene_is_rejected_without_device():
     d2, keep2 = s2.desc()
     assert abi.lib().ne_bvh_build(C.byref(d2), None, None, None, None, None) == 1
     assert b"finite" in abi.lib().ne_last_error(None)
+
+
+@pytest.mark.parametrize("bad", ["vertex", "radius", "centre", "corner"])
+def test_non_finite_geometry_is_rejected_before_the_bvh_build(bad):
+    """A primitive with an infinite or NaN coordinate has no conservative
+    box, so the scene is rejected instead of being built with boxes that miss it."""
+    from narvalengine_amd.scene import Scene
+    s = Scene()
+    m = s.add_glossy((0.5, 0.5, 0.5), 0.5)
+    s.add_sphere((0, 0, 0), 0.5, m)
+    s.add_mesh(np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]]), np.array([[0, 1, 2]]), m)
+    s.add_rectangle(m, position=(0, 0, 1), rotation=(0, 0, 0), scale=(1, 1, 1))
+    if bad == "vertex":
+        s.tri_v[-1][0, 4] = np.inf
+    elif bad == "radius":
+        s.sph[-1][3] = np.inf
+    elif bad == "centre":
+        s.sph[-1][1] = np.nan
+    d, keep = s.desc()
+    if bad == "corner":
+        keep_rect = np.array(np.ctypeslib.as_array(d.rect, shape=(12,)))
+        keep_rect[5] = -np.inf
+        d.rect = keep_rect.ctypes.data_as(C.POINTER(C.c_float))
+    assert abi.lib().ne_bvh_build(C.byref(d), None, None, None, None, None) == 1
+    assert b"finite" in abi.lib().ne_last_error(None)

@@ -305,7 +305,7 @@ def test_render_matches_oracle(ctx, which):
     gpu, cpu = _render_pair(ctx, s, cam, W, H, spp, bounces)
     same = np.all(np.abs(gpu - cpu) <= 1e-4 * np.maximum(np.abs(cpu), 1e-3), axis=2)
     print(f"{which}: identical pixels {same.mean():.4f}; mean gpu {gpu.mean() / spp:.5f} cpu {cpu.mean() / spp:.5f}")
-    assert same.mean() > 0.9
+    assert same.mean() >= 0.99
     # image means agree within Monte Carlo error
     assert abs(gpu.mean() - cpu.mean()) <= 0.02 * abs(cpu.mean()) + 1e-6
 
@@ -316,7 +316,7 @@ def test_render_fp64_mode_matches_oracle_fp64(ctx):
     gpu, cpu = _render_pair(ctx, s, scenes.VOLUME_CAMERA, 24, 24, 8, 16, precision=NE_FP64)
     same = np.all(np.abs(gpu - cpu) <= 1e-9 * np.maximum(np.abs(cpu), 1e-3), axis=2)
     print(f"fp64 identical pixels {same.mean():.4f}")
-    assert same.mean() > 0.9
+    assert same.mean() >= 0.99
 
 
 def test_render_is_deterministic_across_launch_shapes(ctx):
@@ -387,19 +387,100 @@ def test_tiles_of_an_image_below_the_tile_grid_are_empty(ctx):
 
 def test_invalid_arguments_are_reported(ctx):
     print_case("errors")
-    from narvalengine_amd import NarvalError
+    from narvalengine_amd import Context, NarvalError
     from narvalengine_amd.scene import Scene
     s = Scene()
     m = s.add_glossy((0.5, 0.5, 0.5), 0.5)
     s.add_sphere((0, 0, 0), -1.0, m)
     with pytest.raises(NarvalError, match="radius"):
         ctx.upload(s)
+    # non-finite geometry cannot be bounded by the BVH: rejected at upload
+    for bad in ("sphere", "triangle"):
+        s = Scene()
+        m = s.add_glossy((0.5, 0.5, 0.5), 0.5)
+        if bad == "sphere":
+            s.add_sphere((0, 0, 0), float("inf"), m)
+        else:
+            s.add_mesh(np.array([[0, 0, 0], [1, 0, 0], [0, np.inf, 0]]), np.array([[0, 1, 2]]), m)
+        with pytest.raises(NarvalError, match="finite"):
+            ctx.upload(s)
     g = ctx.upload(scenes.small_volume(res=8, maj_res=2))
     film = ctx.film(8, 8)
     with pytest.raises(NarvalError, match="spp"):
         ctx.render(g, scenes.VOLUME_CAMERA, film, spp=0, max_bounces=4)
     with pytest.raises(NarvalError, match="rectangle"):
         ctx.render(g, scenes.VOLUME_CAMERA, film, spp=1, max_bounces=4, rect=(0, 0, 9, 8))
+    rays = scenes.rays_towards(4)
+    with pytest.raises(NarvalError, match="max_bounces"):
+        ctx.li(g, rays, 2048)
+    with pytest.raises(NarvalError, match="max_bounces"):
+        ctx.li(g, rays, -1)
+    other = Context(0)
+    try:
+        with pytest.raises(NarvalError, match="another context"):
+            other.li(g, rays, 4)
+    finally:
+        other.close()
+
+
+def test_rejected_tile_call_leaves_the_film_unchanged(ctx):
+    print_case("rejected tile call")
+    from narvalengine_amd import NarvalError, OfflineEngine, SceneSettings
+    s = scenes.small_volume(res=8, maj_res=2)
+    eng = OfflineEngine(scenes.VOLUME_CAMERA, SceneSettings(resolution=(80, 20), spp=2, bounces=4), s, ctx=ctx)
+    for i in range(400):
+        eng.renderTile(scenes.VOLUME_CAMERA, i)
+    before = eng.film.read()
+    assert before.any()
+    eng.settings.spp = 0
+    for i in (0, 41, 399):
+        with pytest.raises(NarvalError, match="spp"):
+            eng.renderTile(scenes.VOLUME_CAMERA, i)
+    np.testing.assert_array_equal(eng.film.read(), before)
+
+
+def _li_case(which):
+    if which == "volume":
+        return scenes.small_volume(res=32, maj_res=4), scenes.VOLUME_CAMERA, 16
+    if which == "emitters":
+        return scenes.emitter_mix(), scenes.BOX_CAMERA, 8
+    if which == "envmap":
+        return scenes.env_map_scene(), scenes.VOLUME_CAMERA, 16
+    if which == "homogeneous":
+        return scenes.homogeneous_box(), scenes.VOLUME_CAMERA, 16
+    if which == "cloud":
+        return scenes.small_cloud(), scenes.VOLUME_CAMERA, 64
+    return scenes.small_surface(subdiv=3), scenes.BOX_CAMERA, 8
+
+
+@pytest.mark.parametrize("which", ["volume", "surface", "cloud", "homogeneous", "emitters", "envmap"])
+def test_li_matches_oracle_li(ctx, which):
+    """Integrator::Li (src/integrators/Integrator.h:23) through ne_li: ray i
+    on stream (seed; i, sample, 0), per ray against the oracle's Li on the
+    same stream.  Rays leave the camera towards random points of the scene."""
+    print_case(f"ne_li {which}")
+    s, cam, bounces = _li_case(which)
+    g = ctx.upload(s)
+    w = O.World(s)
+    rng = np.random.default_rng(17)
+    n = 3000
+    o = np.tile(np.asarray(cam.position, np.float64), (n, 1))
+    tgt = np.asarray(cam.look_at, np.float64) + rng.uniform(-0.9, 0.9, (n, 3))
+    d = tgt - o
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.concatenate([o, d], axis=1).astype(np.float32)
+    lit = 0
+    for sample in (0, 5):
+        gpu = ctx.li(g, rays, bounces, seed=29, sample=sample).astype(np.float64)
+        cpu = w.li(rays, bounces, seed=29, sample=sample)
+        same = np.all(np.abs(gpu - cpu) <= 1e-4 * np.maximum(np.abs(cpu), 1e-3), axis=1)
+        print(f"ne_li {which} sample {sample}: identical rays {same.mean():.4f}; "
+              f"mean gpu {gpu.mean():.5f} cpu {cpu.mean():.5f}")
+        assert np.isfinite(gpu).all() and (gpu >= 0).all()
+        assert same.mean() >= 0.99
+        assert abs(gpu.mean() - cpu.mean()) <= 0.01 * abs(cpu.mean()) + 1e-6
+        lit += int((cpu > 0).any(axis=1).sum())
+    assert lit > n // 4  # the rays see light, so the comparison is not of zeros
 
 
 def test_handles_release_in_any_order():
