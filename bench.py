@@ -2,15 +2,18 @@
 
 One step = one full-frame render of the workload at its full spp (every
 pixel, every sample, up to max_bounces scatter vertices), inputs already
-resident in HBM.  The default workload is `surface1m` (1024x1024, 256 spp,
-8 bounces, 1.31M-triangle mesh + spheres + rectangles, SAH BVH); pass
---workload volume128 / cloud512 for the volume scenes, or batch48 for the
-48-job batch in one persistent graph launch (one step = one launch over every
-job at full spp).
+resident in HBM.  The default workload is `cloud512` (512^3 cumulus grid with
+16^3 majorant cells, 1920x1080, 256 spp, up to 128 bounces, ratio-tracked
+shadow rays); pass --workload surface1m / volume128 for the other image
+scenes, or batch48 for the 48-job batch in one persistent graph launch (one
+step = one launch over every job at full spp).
 
 Prints one JSON line: samples/s (value), rays/s, the dominant kernel's
 HBM roofline fraction and the CPU baseline (the oracle timed on this host's
-cores on a bounded crop of the same workload).  Inside the timed steps every
+cores, at 16 threads and at every core the process may use, on a bounded
+sample of the same workload).  For cloud512 the line also carries the
+equal-time relMSE of guided against plain rendering on the full frame,
+against an FP64 converged render that CPU probe pixels accept.  Inside the timed steps every
 kernel launch is bracketed by HIP events on the library's stream, which gives
 each stage's average launch duration; the stage's algorithmic bytes come from
 a separate counting run of the same frame (same launches, same paths).
@@ -101,30 +104,58 @@ def log(msg):
     sys.stderr.flush()
 
 
+def host_cpu() -> dict:
+    """The host's CPU model (/proc/cpuinfo, as lscpu reports it), the cores this
+    process may run on (nproc) and the machine's logical CPU count."""
+    model = "unknown"
+    try:
+        for line in Path("/proc/cpuinfo").read_text().splitlines():
+            if line.lower().startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    try:
+        nproc = len(os.sched_getaffinity(0))
+    except AttributeError:
+        nproc = os.cpu_count() or 1
+    return {"model": model, "nproc": nproc, "logical_cpus": os.cpu_count()}
+
+
 def cpu_baseline(wl, budget_s: float):
     """Oracle (CPU restatement, FP32, same counter RNG and estimator) over the
     whole frame at the first k samples per pixel, k sized for about budget_s
     seconds: the same distribution of paths as the GPU frame (every pixel), so
-    samples/s compare like for like.  A 1-sample pilot frame sets k."""
+    samples/s compare like for like.  A 1-sample pilot frame sets k.  Two legs:
+    16 threads (OfflineEngine's pool, src/core/OfflineEngine.cpp:18) and every
+    core the process may use (nproc); `value` is the nproc leg."""
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle as O
-    threads = max(1, min(16, os.cpu_count() or 1))
+    host = host_cpu()
     t0 = time.perf_counter()
     w = O.World(wl.scene)
     build_s = time.perf_counter() - t0
 
-    def run(k, offset):
+    def run(k, offset, threads):
         t = time.perf_counter()
         w.render(wl.camera, wl.width, wl.height, k, wl.max_bounces, seed=1, sample_offset=offset, threads=threads)
         return time.perf_counter() - t, wl.width * wl.height * k
 
-    dt, n = run(1, 0)
-    k = int(max(1, min(wl.spp - 1, budget_s * (n / max(dt, 1e-9)) / (wl.width * wl.height))))
-    dt, n = run(k, 1)
-    return {"value": n / dt, "unit": "samples/s", "cores": threads, "kind": "port",
-            "sample": f"whole {wl.width}x{wl.height} frame at samples 1..{k} of {wl.spp} per pixel, "
-                      f"{wl.max_bounces} bounces ({n} paths, {dt:.1f} s; 1-sample pilot frame and oracle scene "
-                      f"build {build_s:.1f} s not timed)"}
+    legs = []
+    for threads in sorted({16, host["nproc"]}):
+        dt, n = run(1, 0, threads)
+        k = int(max(1, min(wl.spp - 1, budget_s * (n / max(dt, 1e-9)) / (wl.width * wl.height))))
+        dt, n = run(k, 1, threads)
+        legs.append({"threads": threads, "value": n / dt, "paths": n, "seconds": round(dt, 2), "k": k})
+        log(f"CPU baseline, {threads} threads: {n / dt / 1e6:.2f} M samples/s ({n} paths in {dt:.1f} s)")
+    top = max(legs, key=lambda leg: leg["threads"])
+    return {"value": top["value"], "unit": "samples/s", "cores": top["threads"], "kind": "port",
+            "sample": f"whole {wl.width}x{wl.height} frame at samples 1..{top['k']} of {wl.spp} per pixel, "
+                      f"{wl.max_bounces} bounces ({top['paths']} paths, {top['seconds']:.1f} s; 1-sample pilot frame "
+                      f"and oracle scene build {build_s:.1f} s not timed)",
+            "cpu_model": host["model"], "nproc": host["nproc"], "logical_cpus": host["logical_cpus"],
+            "legs": [{"threads": leg["threads"], "value": round(leg["value"], 1), "paths": leg["paths"],
+                      "seconds": leg["seconds"]} for leg in legs]}
 
 
 def copy_calibration(nbytes: int, reps: int = 5):
@@ -224,6 +255,143 @@ GUIDE_FLOP_PER_QUERY = 2 * (35 * 64 + 3 * 64 * 64 + 64 * 40)  # 34,176
 def rel_mse(x, ref, eps=1e-2):
     """mean over pixels and channels of (x - y)^2 / (y^2 + eps), linear radiance"""
     return float(np.mean((x - ref) ** 2 / (ref ** 2 + eps)))
+
+
+def probe_pixels(W: int, H: int, side: int = 16) -> np.ndarray:
+    """The fixed probe set of the converged-reference check: a side x side grid
+    over the cloud's part of the frame (x in [0.32, 0.68] W, y in [0.36, 0.66] H)."""
+    xs = np.round(np.linspace(0.32 * W, 0.68 * W, side)).astype(np.int32)
+    ys = np.round(np.linspace(0.36 * H, 0.66 * H, side)).astype(np.int32)
+    return np.array([(x, y) for y in ys for x in xs], np.int32)
+
+
+def converged_reference(ctx, g, wl, passes: int, seed: int):
+    """FP64 megakernel render at passes x spp, in passes of the workload's spp.
+    Returns the mean image, the variance of that mean per pixel and channel
+    (from the spread of the passes) and the seconds taken."""
+    from narvalengine_amd.abi import NE_FP64, NE_MODE_MEGAKERNEL
+    W, H = wl.width, wl.height
+    f = ctx.film(W, H)
+    s1 = np.zeros((H, W, 3))
+    s2 = np.zeros((H, W, 3))
+    prev = np.zeros((H, W, 3))
+    t = time.perf_counter()
+    for k in range(passes):
+        ctx.render(g, wl.camera, f, spp=wl.spp, sample_offset=k * wl.spp, max_bounces=wl.max_bounces, seed=seed,
+                   precision=NE_FP64, mode=NE_MODE_MEGAKERNEL)
+        cur = f.read()
+        img = (cur - prev) / wl.spp
+        prev = cur
+        s1 += img
+        s2 += img * img
+        log(f"FP64 reference pass {k + 1}/{passes}")
+    secs = time.perf_counter() - t
+    f.close()
+    mean = s1 / passes
+    var_mean = np.maximum(s2 / passes - mean * mean, 0.0) / max(passes - 1, 1)
+    return mean, var_mean, secs
+
+
+def probe_check(wl, ref, ref_var, spp: int, threads: int, seed: int) -> dict:
+    """Accepts the GPU FP64 reference only where the CPU oracle agrees: the
+    oracle renders the probe pixels in FP64 at spp samples each (independent
+    streams), and every probe and channel must lie within 5 standard errors of
+    the reference (both estimates' variances from their own samples)."""
+    sys.path.insert(0, str(ROOT / "tests"))
+    import oracle as O
+    W, H = wl.width, wl.height
+    xy = probe_pixels(W, H)
+    w = O.World(wl.scene, fp64=True)
+    t = time.perf_counter()
+    s1, s2 = w.render_pixels(wl.camera, W, H, xy, spp, wl.max_bounces, seed=seed, threads=threads)
+    secs = time.perf_counter() - t
+    m = s1 / spp
+    var_m = np.maximum(s2 / spp - m * m, 0.0) / (spp - 1)
+    r = ref[xy[:, 1], xy[:, 0]]
+    rv = ref_var[xy[:, 1], xy[:, 0]]
+    z = (r - m) / np.sqrt(np.maximum(var_m + rv, 1e-30))
+    return {"probes": int(len(xy)), "spp": spp, "threads": threads, "seconds": round(secs, 2),
+            "max_abs_z": round(float(np.abs(z).max()), 3), "mean_z2": round(float(np.mean(z * z)), 3),
+            "mean_ratio_reference_to_cpu": round(float(r.sum() / max(m.sum(), 1e-30)), 5),
+            "rel_stderr_cpu": round(float(np.sqrt(var_m.mean()) / max(m.mean(), 1e-30)), 6),
+            "accepted": bool(np.abs(z).max() < 5.0)}
+
+
+def relmse_parts(img, ref, ref_var, eps=1e-2):
+    """relMSE of img against the reference, and the same with the reference's
+    own noise (its per-pixel variance) taken out."""
+    raw = rel_mse(img, ref, eps)
+    noise = float(np.mean(ref_var / (ref ** 2 + eps)))
+    return raw, raw - noise
+
+
+def equal_time_relmse(ctx, g, wl, args, threads: int) -> dict:
+    """Equal-time relMSE of guided against plain rendering on the full frame
+    of the workload, against an FP64 converged render that the CPU probes
+    accept.  Plain: the FP32 wavefront at the workload's spp.  Guided: a fresh
+    guide, trained online while it renders, then frozen for the rest of the
+    same wall time (training included); both phases' samples count."""
+    from narvalengine_amd import Guide
+    W, H, spp = wl.width, wl.height, wl.spp
+    ref, ref_var, ref_s = converged_reference(ctx, g, wl, args.ref_passes, seed=1000)
+    log(f"FP64 reference: {args.ref_passes * spp} spp in {ref_s:.1f} s")
+    probes = probe_check(wl, ref, ref_var, args.probe_spp, threads, seed=2000)
+    log(f"CPU probes: {probes}")
+    kw = dict(max_bounces=wl.max_bounces)
+
+    def render(f, n, seed, offset=0, **extra):
+        ctx.synchronize()
+        t = time.perf_counter()
+        ctx.render(g, wl.camera, f, spp=n, seed=seed, sample_offset=offset, **kw, **extra)
+        return time.perf_counter() - t
+
+    f = ctx.film(W, H)
+    render(f, spp, 7)  # warm
+    f.clear()
+    t_plain = render(f, spp, 11)
+    plain = f.read() / spp
+    e_plain, e_plain_c = relmse_parts(plain, ref, ref_var)
+    log(f"plain: {spp} spp in {t_plain * 1e3:.1f} ms, relMSE {e_plain:.4e} ({e_plain_c:.4e} without reference noise)")
+    out = {"frame": [W, H], "budget_ms": round(t_plain * 1e3, 2),
+           "reference": {"kind": "FP64 megakernel", "spp": args.ref_passes * spp, "seconds": round(ref_s, 1),
+                         "relmse_noise": float(np.mean(ref_var / (ref ** 2 + 1e-2))), "cpu_probes": probes},
+           "plain": {"spp": spp, "ms": round(t_plain * 1e3, 2), "relmse": e_plain, "relmse_minus_ref_noise": e_plain_c},
+           "guided": []}
+    for prob in args.guide_probs:
+        gkw = dict(guided_vertices=args.guided_vertices, guide_prob=prob)
+        # the frozen guide's time per sample, from a pilot guide (not timed into the run)
+        pilot = Guide(ctx, seed=21, **gkw)
+        render(f, 16, 3, guide=pilot, train=False)
+        t16 = render(f, 16, 4, guide=pilot, train=False)
+        pilot.close()
+        guide = Guide(ctx, seed=22, **gkw)
+        f.clear()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ctx.render(g, wl.camera, f, spp=args.train_spp, seed=13, guide=guide, train=True, **kw)
+        ctx.synchronize()
+        t_train = time.perf_counter() - t0
+        rest = max(0, int((t_plain - t_train) / (t16 / 16)))
+        if rest:
+            ctx.render(g, wl.camera, f, spp=rest, seed=13, sample_offset=args.train_spp, guide=guide, train=False, **kw)
+        ctx.synchronize()
+        t_g = time.perf_counter() - t0
+        n = args.train_spp + rest
+        img = f.read() / n
+        e, e_c = relmse_parts(img, ref, ref_var)
+        st = guide.stats()
+        guide.close()
+        rec = {"guide_prob": prob, "guided_vertices": args.guided_vertices, "train_spp": args.train_spp,
+               "frozen_spp": rest, "spp": n, "ms": round(t_g * 1e3, 2), "train_ms": round(t_train * 1e3, 2),
+               "train_steps": st["steps"], "relmse": e, "relmse_minus_ref_noise": e_c,
+               "ratio_to_plain": round(e_c / e_plain_c, 4),
+               # relMSE x time: the ratio at exactly equal time for an unbiased estimator
+               "ratio_to_plain_time_normalised": round(e_c * t_g / (e_plain_c * t_plain), 4),
+               "ratio_at_equal_spp": round(e_c * n / (e_plain_c * spp), 4)}
+        out["guided"].append(rec)
+        log(f"guided p={prob}: {rec}")
+    f.close()
+    return out
 
 
 def bench_quality(args):
@@ -367,7 +535,7 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="surface1m",
+    ap.add_argument("--workload", default="cloud512",
                     choices=["surface1m", "volume128", "cloud512", "batch48", "guided", "quality"])
     ap.add_argument("--mode", default="auto", choices=["auto", "megakernel", "wavefront"])
     ap.add_argument("--pool", type=int, default=0, help="wavefront path slots (0 = library default)")
@@ -385,9 +553,18 @@ def main():
     ap.add_argument("--cpu-block-spp", type=int, default=1024, help="quality workload: oracle FP64 block spp")
     ap.add_argument("--guided-vertices", type=int, default=1)
     ap.add_argument("--guide-prob", type=float, default=0.25)
+    ap.add_argument("--no-quality", action="store_true",
+                    help="cloud512: skip the equal-time relMSE block (FP64 reference, CPU probes, guided runs)")
+    ap.add_argument("--ref-passes", type=int, default=16,
+                    help="equal-time block: FP64 reference spp = ref_passes x the workload's spp")
+    ap.add_argument("--probe-spp", type=int, default=65536, help="equal-time block: CPU FP64 spp per probe pixel")
+    ap.add_argument("--train-spp", type=int, default=1, help="equal-time block: spp rendered while the guide trains")
+    ap.add_argument("--guide-probs", default="0.5,0.25",
+                    help="equal-time block: one guided run per proposal probability")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
                     help="only run plain copies of BYTES (counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
+    args.guide_probs = [float(x) for x in args.guide_probs.split(",") if x]
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the path shards by "
               "pixel tiles / sample ranges with a final FP64 reduce, which is not part of this build", file=sys.stderr)
@@ -476,6 +653,9 @@ def main():
                     "kernel": kernel, "algorithmic_bytes_per_launch": d["algorithmic_bytes_per_launch"],
                     "avg_launch_ms": d["avg_launch_ms"], "limiter": measured_valu(kernel, wl.name)}
     cpu = None if args.no_cpu else cpu_baseline(wl, args.cpu_seconds)
+    quality = None
+    if wl.name == "cloud512" and not args.no_quality:
+        quality = equal_time_relmse(ctx, g, wl, args, threads=host_cpu()["nproc"])
     out = {
         "metric": "samples/sec", "value": round(value, 1), "unit": "samples/s", "n_gpus": 1,
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(step_s * 1e3, 3),
@@ -491,6 +671,8 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
+        "gpu_vs_cpu_legs": {str(leg["threads"]): round(value / leg["value"], 1) for leg in cpu["legs"]} if cpu else None,
+        "equal_time_relmse": quality,
         "stages": stages,
         "counters_per_step": counters,
     }

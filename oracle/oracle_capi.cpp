@@ -319,4 +319,60 @@ void neo_render(void* p, const CameraDesc* cam, int W, int H, int x0, int y0, in
   if (stats) { stats[0] = ps.ext_rays; stats[1] = ps.shadow_rays; }
 }
 
+// Probe pixels at high spp: for each of the n pixels (xy[2k], xy[2k+1]) the
+// spp camera samples of render() (same streams, same jitter), summed in FP64
+// with the sum of squares of each sample's radiance (for a per-pixel variance).
+// Work is spread over threads in chunks of 1024 samples.
+void neo_render_pixels(void* p, const CameraDesc* cd, int W, int H, int n, const int* xy, int spp, int max_bounces,
+                       uint64_t seed, uint32_t sample_offset, uint32_t job, int nthreads, double* sum, double* sumsq) {
+  OWorld* w = static_cast<OWorld*>(p);
+  const int chunk = 1024, nch = (spp + chunk - 1) / chunk;
+  const size_t items = size_t(n) * size_t(nch);
+  std::vector<double> part(items * 6, 0.0);
+  V3f pos(cd->position[0], cd->position[1], cd->position[2]);
+  V3f at(cd->look_at[0], cd->look_at[1], cd->look_at[2]);
+  V3f up(cd->up[0], cd->up[1], cd->up[2]);
+  Camera cam = ref_camera(pos, at, up, cd->vfov, cd->aspect, 1.0f);
+  std::atomic<size_t> next{0};
+  auto worker = [&]() {
+    while (true) {
+      size_t it = next.fetch_add(1);
+      if (it >= items) break;
+      const int k = int(it / size_t(nch)), c = int(it % size_t(nch));
+      const int x = xy[2 * k], y = xy[2 * k + 1];
+      const uint32_t pix = uint32_t(y) * uint32_t(W) + uint32_t(x);
+      double* acc = &part[it * 6];
+      for (int si = c * chunk; si < std::min(spp, (c + 1) * chunk); ++si) {
+        Rng rng(seed, pix, sample_offset + uint32_t(si), job);
+        float ju = rng.next(), jv = rng.next();
+        V3f dir = ref_camera_dir(cam, (float(x) + ju) / float(W), (float(y) + jv) / float(H));
+        double L[3];
+        if (w->prec) {
+          V3<double> r = Li<double>(*w->wd, V3<double>(pos), V3<double>(dir), max_bounces, rng);
+          L[0] = r.x; L[1] = r.y; L[2] = r.z;
+        } else {
+          V3f r = Li<float>(*w->wf, pos, dir, max_bounces, rng);
+          L[0] = r.x; L[1] = r.y; L[2] = r.z;
+        }
+        for (int a = 0; a < 3; ++a) { acc[a] += L[a]; acc[3 + a] += L[a] * L[a]; }
+      }
+    }
+  };
+  const int nt = std::max(1, nthreads);
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nt; ++t) pool.emplace_back(worker);
+  worker();
+  for (auto& t : pool) t.join();
+  for (int k = 0; k < n; ++k)
+    for (int a = 0; a < 3; ++a) {
+      double s1 = 0.0, s2 = 0.0;
+      for (int c = 0; c < nch; ++c) {  // fixed order: the result does not depend on thread timing
+        s1 += part[(size_t(k) * nch + c) * 6 + a];
+        s2 += part[(size_t(k) * nch + c) * 6 + 3 + a];
+      }
+      sum[3 * k + a] = s1;
+      sumsq[3 * k + a] = s2;
+    }
+}
+
 }  // extern "C"

@@ -77,6 +77,8 @@ def lib():
             "neo_world_transmittance": (None, [C.c_void_p, C.c_int, _F, _F, C.c_uint64, _D]),
             "neo_world_delta_track": (None, [C.c_void_p, C.c_int, _F, C.c_uint64, _I, _D]),
             "neo_li": (None, [C.c_void_p, C.c_int, _F, C.c_int, C.c_uint64, C.c_uint32, _D]),
+            "neo_render_pixels": (None, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, _I, C.c_int, C.c_int,
+                                         C.c_uint64, C.c_uint32, C.c_uint32, C.c_int, _D, _D]),
             "neo_render": (None, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                   C.c_int, C.c_int, C.c_uint64, C.c_uint32, C.c_uint32, C.c_int, _D,
                                   C.POINTER(C.c_uint64)]),
@@ -223,6 +225,17 @@ class World:
         out = np.zeros((len(rays), 3), np.float64)
         lib().neo_li(self.h, len(rays), fp(rays), max_bounces, seed, sample, dp(out))
         return out
+
+    def render_pixels(self, camera, W, H, xy, spp, max_bounces, seed=0, sample_offset=0, job=0, threads=8):
+        """Per-pixel FP64 (sum, sum of squares) of spp camera samples at pixels xy (n, 2),
+        with the streams and jitter of render()."""
+        xy = np.ascontiguousarray(xy, np.int32).reshape(-1, 2)
+        n = len(xy)
+        s1, s2 = np.zeros((n, 3), np.float64), np.zeros((n, 3), np.float64)
+        cd = camera.desc()
+        lib().neo_render_pixels(self.h, C.byref(cd), W, H, n, ip(xy), spp, max_bounces, seed, sample_offset, job,
+                                threads, dp(s1), dp(s2))
+        return s1, s2
 
     def env_sample(self, u2):
         u2 = f(u2).reshape(-1, 2)

@@ -538,3 +538,20 @@ def test_point_light_on_a_glossy_floor_is_the_closed_form():
     f = _glossy_f(albedo, rough, metal, wo, wi[None])[0]
     expect = f * I * wi[1] / d2
     np.testing.assert_allclose(img[0, 0], expect, rtol=2e-3)
+
+
+def test_probe_pixels_equal_the_full_render_at_those_pixels():
+    """render_pixels draws each probe pixel's samples from the same streams as
+    render (the probe set of the converged-reference check)."""
+    s = scenes.small_volume(res=16, maj_res=2)
+    for fp64 in (False, True):
+        w = O.World(s, fp64=fp64)
+        full = w.render(scenes.VOLUME_CAMERA, 20, 16, 8, 16, seed=9, threads=4)
+        xy = np.array([[0, 0], [7, 5], [19, 15], [10, 8]])
+        s1, s2 = w.render_pixels(scenes.VOLUME_CAMERA, 20, 16, xy, 8, 16, seed=9, threads=3)
+        np.testing.assert_array_equal(s1, full[xy[:, 1], xy[:, 0]])
+        assert (s2 >= s1 * s1 / 8 - 1e-12).all()
+        # more samples than one chunk: still the sum of the same streams
+        a1, _ = w.render_pixels(scenes.VOLUME_CAMERA, 20, 16, xy[:1], 2500, 16, seed=9, threads=4)
+        b = w.render(scenes.VOLUME_CAMERA, 20, 16, 2500, 16, rect=(0, 0, 1, 1), seed=9, threads=1)
+        np.testing.assert_allclose(a1[0], b[0, 0], rtol=1e-12)
