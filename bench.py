@@ -220,16 +220,46 @@ def bench_batch(args):
     if not args.no_cpu:
         sys.path.insert(0, str(ROOT / "tests"))
         import oracle as O
-        threads = max(1, min(16, os.cpu_count() or 1))
-        n, dt = 0, 0.0
-        for w in wls:  # every job, whole frame, first sample of each pixel
-            world = O.World(w.scene)
-            t = time.perf_counter()
-            world.render(w.camera, w.width, w.height, 1, w.max_bounces, seed=1, threads=threads)
-            dt += time.perf_counter() - t
-            n += w.width * w.height
-        cpu = {"value": n / dt, "unit": "samples/s", "cores": threads, "kind": "port",
-               "sample": f"every job, whole frame at 1 sample per pixel ({n} paths, {dt:.1f} s)"}
+        host = host_cpu()
+        legs = []
+        for threads in sorted({16, host["nproc"]}):
+            n, dt = 0, 0.0
+            for w in wls:  # every job, whole frame, first sample of each pixel
+                world = O.World(w.scene)
+                t = time.perf_counter()
+                world.render(w.camera, w.width, w.height, 1, w.max_bounces, seed=1, threads=threads)
+                dt += time.perf_counter() - t
+                n += w.width * w.height
+            legs.append({"threads": threads, "value": round(n / dt, 1), "paths": n, "seconds": round(dt, 2)})
+        top = legs[-1]
+        cpu = {"value": top["value"], "unit": "samples/s", "cores": top["threads"], "kind": "port",
+               "sample": f"every job, whole frame at 1 sample per pixel ({top['paths']} paths, {top['seconds']:.1f} s)",
+               "cpu_model": host["model"], "nproc": host["nproc"], "legs": legs}
+    # the plain method beside it: the same jobs one after another through
+    # ne_render (the default FP32 pipeline), each into its own film
+    seq = None
+    if not args.no_sequential:
+        films = [ctx.film(j["width"], j["height"]) for j in jobs]
+        for j, f in zip(jobs, films):  # untimed pass: pools and films allocated
+            ctx.render(j["scene"], j["camera"], f, spp=j["spp"], max_bounces=j["max_bounces"], seed=1, job=j["job"],
+                       sync=False)
+        ctx.synchronize()
+        for f in films:
+            f.clear()
+        done_ms = []
+        t0 = time.perf_counter()
+        for j, f in zip(jobs, films):
+            ctx.render(j["scene"], j["camera"], f, spp=j["spp"], max_bounces=j["max_bounces"], seed=1, job=j["job"])
+            done_ms.append((time.perf_counter() - t0) * 1e3)
+        t_seq = time.perf_counter() - t0
+        seq = {"method": "ne_render per job, one after another (FP32 wavefront)", "makespan_ms": round(t_seq * 1e3, 1),
+               "value": round(paths / t_seq, 1), "unit": "samples/s",
+               "batched_speedup": round(t_seq / (wall / args.steps), 3),
+               "job_completion_ms": {"min": round(min(done_ms), 2), "median": round(float(np.median(done_ms)), 2),
+                                     "max": round(max(done_ms), 2)}}
+        log(f"sequential ne_render: {t_seq * 1e3:.1f} ms for all jobs ({paths / t_seq / 1e9:.3f} G samples/s)")
+        for f in films:
+            f.close()
     jm = np.asarray(job_ms, np.float64)
     out = {
         "metric": "samples/sec", "value": round(value, 1), "unit": "samples/s", "n_gpus": 1,
@@ -243,6 +273,7 @@ def bench_batch(args):
         "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
         "job_completion_ms": {"min": round(float(jm.min()), 2), "median": round(float(np.median(jm)), 2),
                               "max": round(float(jm.max()), 2), "all": [round(float(x), 2) for x in jm]},
+        "sequential": seq,
         "counters_per_step": counters,
     }
     print(json.dumps(out))
@@ -358,7 +389,8 @@ def equal_time_relmse(ctx, g, wl, args, threads: int) -> dict:
            "plain": {"spp": spp, "ms": round(t_plain * 1e3, 2), "relmse": e_plain, "relmse_minus_ref_noise": e_plain_c},
            "guided": []}
     for prob in args.guide_probs:
-        gkw = dict(guided_vertices=args.guided_vertices, guide_prob=prob)
+        gkw = dict(guided_vertices=args.guided_vertices, guide_prob=prob, cache_res=args.guide_cache,
+                   cache_dirs=args.guide_cache_dirs)
         # the frozen guide's time per sample, from a pilot guide (not timed into the run)
         pilot = Guide(ctx, seed=21, **gkw)
         render(f, 16, 3, guide=pilot, train=False)
@@ -381,7 +413,8 @@ def equal_time_relmse(ctx, g, wl, args, threads: int) -> dict:
         e, e_c = relmse_parts(img, ref, ref_var)
         st = guide.stats()
         guide.close()
-        rec = {"guide_prob": prob, "guided_vertices": args.guided_vertices, "train_spp": args.train_spp,
+        rec = {"guide_prob": prob, "guided_vertices": args.guided_vertices, "cache_res": args.guide_cache,
+               "cache_dirs": args.guide_cache_dirs, "train_spp": args.train_spp,
                "frozen_spp": rest, "spp": n, "ms": round(t_g * 1e3, 2), "train_ms": round(t_train * 1e3, 2),
                "train_steps": st["steps"], "relmse": e, "relmse_minus_ref_noise": e_c,
                "ratio_to_plain": round(e_c / e_plain_c, 4),
@@ -390,8 +423,221 @@ def equal_time_relmse(ctx, g, wl, args, threads: int) -> dict:
                "ratio_at_equal_spp": round(e_c * n / (e_plain_c * spp), 4)}
         out["guided"].append(rec)
         log(f"guided p={prob}: {rec}")
+    if args.batched:
+        out["batched"] = batched_legs(ctx, g, wl, args, ref, ref_var, t_plain, e_plain_c)
     f.close()
     return out
+
+
+def batched_legs(ctx, g, wl, args, ref, ref_var, t_plain, e_plain_c):
+    """The frame through the persistent batch kernel (one job, one graph
+    launch): plain, and guided+batched at the plain wavefront's wall time
+    (a wavefront render trains the guide online for args.train_spp, then one
+    batch launch renders the rest with the frozen, tabulated proposal)."""
+    from narvalengine_amd import Batch, Guide
+    W, H, spp = wl.width, wl.height, wl.spp
+    job = dict(scene=g, camera=wl.camera, width=W, height=H, spp=spp, max_bounces=wl.max_bounces, seed=17, job=0)
+    b = Batch(ctx, [job])
+    cfg, _ = b.autotune(spp_divisor=16)
+    b.run(cfg)
+    ms = b.run(cfg)
+    e, e_c = relmse_parts(b.read(0) / spp, ref, ref_var)
+    b.close()
+    legs = {"plain_batched": {"spp": spp, "ms": round(ms, 2), "item_shape": list(cfg), "relmse": e,
+                              "relmse_minus_ref_noise": e_c, "samples_per_s": round(W * H * spp / (ms * 1e-3), 1)}}
+    log(f"plain batched: {legs['plain_batched']}")
+    out = []
+    for prob in args.guide_probs:
+        gkw = dict(guided_vertices=args.guided_vertices, guide_prob=prob, cache_res=args.guide_cache,
+                   cache_dirs=args.guide_cache_dirs)
+        pilot = Guide(ctx, seed=21, **gkw)
+        pj = dict(job, spp=16, guide=pilot)
+        pb = Batch(ctx, [pj])
+        pb.run(cfg)
+        t16 = pb.run(cfg) * 1e-3
+        pb.close()
+        pilot.close()
+        guide = Guide(ctx, seed=22, **gkw)
+        f = ctx.film(W, H)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ctx.render(g, wl.camera, f, spp=args.train_spp, seed=13, guide=guide, train=True, max_bounces=wl.max_bounces)
+        ctx.synchronize()
+        t_train = time.perf_counter() - t0
+        rest = max(1, int((t_plain - t_train) / (t16 / 16)))
+        gb = Batch(ctx, [dict(job, spp=rest, seed=14, guide=guide)])
+        # the graph is captured and the tabulation refreshed on the first launch
+        # of a configuration: done here, before the clock restarts
+        t_cap = time.perf_counter()
+        gb.run(cfg)
+        t_cap = time.perf_counter() - t_cap
+        ctx.synchronize()
+        t1 = time.perf_counter()
+        gb.run(cfg)
+        t_batch = time.perf_counter() - t1
+        n = args.train_spp + rest
+        img = (f.read() + gb.read(0)) / n
+        e, e_c = relmse_parts(img, ref, ref_var)
+        t_g = t_train + t_batch
+        st = guide.stats()
+        rec = {"guide_prob": prob, "guided_vertices": args.guided_vertices, "cache_res": args.guide_cache,
+               "train_spp": args.train_spp, "batched_spp": rest, "spp": n, "ms": round(t_g * 1e3, 2),
+               "train_ms": round(t_train * 1e3, 2), "train_steps": st["steps"], "relmse": e,
+               "relmse_minus_ref_noise": e_c, "ratio_to_plain": round(e_c / e_plain_c, 4),
+               "ratio_to_plain_time_normalised": round(e_c * t_g / (e_plain_c * t_plain), 4),
+               "ratio_at_equal_spp": round(e_c * n / (e_plain_c * spp), 4)}
+        out.append(rec)
+        log(f"guided+batched p={prob}: {rec}")
+        gb.close()
+        f.close()
+        guide.close()
+    legs["guided_batched"] = out
+    return legs
+
+
+def pinhole_rays(cam, W, H, xy, n, rng):
+    """n jittered camera rays per pixel (x, y) of a pinhole camera looking
+    from cam.position at cam.look_at with vertical field of view cam.vfov."""
+    pos = np.asarray(cam.position, np.float64)
+    fwd = np.asarray(cam.look_at, np.float64) - pos
+    fwd /= np.linalg.norm(fwd)
+    right = np.cross(fwd, np.asarray(cam.up, np.float64))
+    right /= np.linalg.norm(right)
+    up = np.cross(right, fwd)
+    th = np.tan(np.radians(cam.vfov) / 2)
+    px = np.repeat(xy[:, 0], n) + rng.random(len(xy) * n)
+    py = np.repeat(xy[:, 1], n) + rng.random(len(xy) * n)
+    u = (2 * px / W - 1) * th * (W / H)
+    v = (1 - 2 * py / H) * th
+    d = fwd[None] + u[:, None] * right[None] + v[:, None] * up[None]
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    return np.concatenate([np.tile(pos, (len(d), 1)), d], axis=1).astype(np.float32)
+
+
+def bench_decompose(args):
+    """Where the plain estimator's variance comes from on the cloud.  Camera
+    rays through the probe pixels (cloud512's 16 x 16 grid over the cloud),
+    args.decompose_spp jittered rays each, traced by Integrator::Li (ne_li)
+    with identical streams in five scene variants, split every sample into
+        escape  the camera ray leaves without a collision (sky seen directly)
+        v1      the first scattering vertex: its light samples and the MIS
+                environment hit of its continuation
+        rest    everything after the phase continuation of the first vertex
+    and v1 / rest further into the sun's and the sky's share.  Each share's
+    per-sample variance is normalised as relMSE is, Var / (mean^2 + 0.01), and
+    averaged over the pixels; a directional proposal at the scattering
+    vertices only changes the `rest` share."""
+    from narvalengine_amd import Context, configs
+    wl = configs.cloud512()
+    W, H = wl.width, wl.height
+    xy = probe_pixels(W, H)
+    n = args.decompose_spp
+    rays = pinhole_rays(wl.camera, W, H, xy, n, np.random.default_rng(5))
+    ctx = Context(0)
+
+    def li(scene, bounces):
+        g = ctx.upload(scene)
+        out = ctx.li(g, rays, bounces, seed=77).astype(np.float64).reshape(len(xy), n, 3)
+        g.close()
+        return out
+
+    full = li(wl.scene, wl.max_bounces)
+    b0 = li(wl.scene, 0)
+    b1 = li(wl.scene, 1)
+    sun_only = configs.cloud512().scene
+    sun_only.set_env(sun_only.env["kind"], c0=(0, 0, 0), c1=(0, 0, 0), c2=(0, 0, 0))
+    sky_only = configs.cloud512().scene
+    sky_only.set_sun(direction=sky_only.sun["dir"], radiance=(0.0, 0.0, 0.0))
+    sun_full, sun_b1 = li(sun_only, wl.max_bounces), li(sun_only, 1)
+    sky_full, sky_b1 = li(sky_only, wl.max_bounces), li(sky_only, 1)
+    parts = {"escape": b0, "v1": b1 - b0, "rest": full - b1,
+             "v1_sun": sun_b1, "v1_sky": sky_b1 - b0, "rest_sun": sun_full - sun_b1, "rest_sky": sky_full - sky_b1}
+    mean = full.mean(axis=1)
+    norm = mean ** 2 + 1e-2
+
+    def relvar(x):
+        return float(np.mean(x.var(axis=1) / norm))
+
+    def relcov(x, y):
+        return float(np.mean(((x - x.mean(axis=1, keepdims=True)) * (y - y.mean(axis=1, keepdims=True))).mean(axis=1)
+                             / norm))
+
+    total = relvar(full)
+    out = {"metric": "per-sample variance shares of the plain estimator (cloud512 probe pixels)",
+           "pixels": int(len(xy)), "samples_per_pixel": n, "relvar_total": total,
+           "relvar": {k: relvar(v) for k, v in parts.items()},
+           "share": {k: round(relvar(v) / total, 4) for k, v in parts.items()},
+           "relcov": {"escape,v1": relcov(parts["escape"], parts["v1"]),
+                      "escape,rest": relcov(parts["escape"], parts["rest"]),
+                      "v1,rest": relcov(parts["v1"], parts["rest"])},
+           "check_sum_of_parts": round(float(np.abs(b0 + parts["v1"] + parts["rest"] - full).max()), 9),
+           "check_sun_plus_sky": round(float(np.abs(sun_full + sky_full - full).max()), 5),
+           "mean_radiance": [round(float(x), 5) for x in mean.mean(axis=0)],
+           "fraction_escaping": round(float((b0.sum(axis=2) > 0).mean()), 4)}
+    print(json.dumps(out))
+
+
+def bench_guide_sweep(args):
+    """Per-sample value of the learned proposal on the cloud at a reduced frame
+    (the cloud512 scene, camera and 128 bounces at args.sweep_width x
+    args.sweep_height).  For each (guided vertices, proposal probability): a
+    fresh guide trains online over args.sweep_train_spp, then frozen renders
+    of 256 spp (4 seeds) are compared with plain renders of 256 spp against an
+    FP64 render at args.sweep_ref_spp.  Reports relMSE at equal spp, the time
+    per frame, and their product (inverse efficiency) relative to plain."""
+    from narvalengine_amd import Context, Guide, configs
+    from narvalengine_amd.abi import NE_FP64, NE_MODE_MEGAKERNEL
+    W, H = args.sweep_width, args.sweep_height
+    wl = configs.cloud512(width=W, height=H)
+    ctx = Context(0)
+    g = ctx.upload(wl.scene)
+    kw = dict(max_bounces=wl.max_bounces)
+    f = ctx.film(W, H)
+    t = time.perf_counter()
+    ctx.render(g, wl.camera, f, spp=args.sweep_ref_spp, seed=1000, precision=NE_FP64, mode=NE_MODE_MEGAKERNEL, **kw)
+    ref = f.read() / args.sweep_ref_spp
+    log(f"reference {args.sweep_ref_spp} spp FP64 in {time.perf_counter() - t:.1f} s")
+
+    def frames(n_seeds, **extra):
+        errs, ms = [], []
+        for k in range(n_seeds):
+            f.clear()
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            ctx.render(g, wl.camera, f, spp=256, seed=50 + k, **kw, **extra)
+            ms.append((time.perf_counter() - t0) * 1e3)
+            errs.append(rel_mse(f.read() / 256, ref))
+        return float(np.mean(errs)), float(np.median(ms))
+
+    frames(1)
+    e_plain, ms_plain = frames(4)
+    log(f"plain: relMSE {e_plain:.4e}, {ms_plain:.2f} ms per frame")
+    rows = []
+    for spec in args.sweep_configs.split(","):
+        parts = spec.split(":")
+        v, p = parts[0], parts[1]
+        nc = int(parts[2]) if len(parts) > 2 else 0
+        guide = Guide(ctx, seed=31, guided_vertices=int(v), guide_prob=float(p), cache_res=nc,
+                      cache_dirs=args.guide_cache_dirs)
+        f.clear()
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        ctx.render(g, wl.camera, f, spp=args.sweep_train_spp, seed=7, guide=guide, train=True, **kw)
+        ctx.synchronize()
+        t_train = time.perf_counter() - t0
+        frames(1, guide=guide)
+        e, ms = frames(4, guide=guide)
+        st = guide.stats()
+        guide.close()
+        row = {"guided_vertices": int(v), "guide_prob": float(p), "cache_res": nc, "train_steps": st["steps"],
+               "train_s": round(t_train, 2), "relmse": e, "ms_per_frame": round(ms, 3),
+               "relmse_ratio_equal_spp": round(e / e_plain, 4), "time_ratio": round(ms / ms_plain, 4),
+               "inverse_efficiency_ratio": round(e * ms / (e_plain * ms_plain), 4)}
+        rows.append(row)
+        log(f"guided {spec}: {row}")
+    print(json.dumps({"metric": "learned proposal vs phase sampling (cloud, reduced frame)", "frame": [W, H],
+                      "reference_spp": args.sweep_ref_spp, "train_spp": args.sweep_train_spp,
+                      "plain": {"relmse": e_plain, "ms_per_frame": round(ms_plain, 3)}, "guided": rows}))
 
 
 def bench_quality(args):
@@ -536,16 +782,19 @@ def main():
     ap.add_argument("--steps", type=int, default=3)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="cloud512",
-                    choices=["surface1m", "volume128", "cloud512", "batch48", "guided", "quality"])
+                    choices=["surface1m", "volume128", "cloud512", "batch48", "guided", "quality", "decompose", "guide-sweep"])
     ap.add_argument("--mode", default="auto", choices=["auto", "megakernel", "wavefront"])
     ap.add_argument("--pool", type=int, default=0, help="wavefront path slots (0 = library default)")
     ap.add_argument("--sort", action="store_true", help="wavefront: shade chunks in (event kind, material) order")
+    ap.add_argument("--spatial", action="store_true",
+                    help="wavefront: tracking stages take each chunk's rays in Morton order of their origins")
     ap.add_argument("--maj-res", type=int, default=0, help="majorant cells per axis (0 = the workload's)")
     ap.add_argument("--no-empty-skip", action="store_true",
                     help="majorant DDA reads every cell (no skipping of empty runs; same image)")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-count", action="store_true")
+    ap.add_argument("--no-sequential", action="store_true", help="batch48: skip the one-job-after-another leg")
     ap.add_argument("--samples-per-thread", type=int, default=0)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--plain-spp", type=int, default=1024, help="guided workload: plain run's spp (sets the time)")
@@ -553,12 +802,25 @@ def main():
     ap.add_argument("--cpu-block-spp", type=int, default=1024, help="quality workload: oracle FP64 block spp")
     ap.add_argument("--guided-vertices", type=int, default=1)
     ap.add_argument("--guide-prob", type=float, default=0.25)
+    ap.add_argument("--decompose-spp", type=int, default=1024, help="decompose workload: rays per probe pixel")
+    ap.add_argument("--sweep-width", type=int, default=240)
+    ap.add_argument("--sweep-height", type=int, default=135)
+    ap.add_argument("--sweep-ref-spp", type=int, default=16384)
+    ap.add_argument("--sweep-train-spp", type=int, default=4096)
+    ap.add_argument("--sweep-configs", default="1:0.25,1:0.5,4:0.25,4:0.5,128:0.1,128:0.25,128:0.5,"
+                                                 "1:0.5:16,4:0.5:16,128:0.25:16,128:0.5:16",
+                    help="guide-sweep workload: guided_vertices:guide_prob[:cache_res] triples "
+                         "(cache_res 0 or absent: per-vertex network)")
+    ap.add_argument("--guide-cache", type=int, default=16, help="equal-time block: tabulation cells per axis")
+    ap.add_argument("--guide-cache-dirs", type=int, default=4, help="tabulation direction bins per axis")
     ap.add_argument("--no-quality", action="store_true",
                     help="cloud512: skip the equal-time relMSE block (FP64 reference, CPU probes, guided runs)")
     ap.add_argument("--ref-passes", type=int, default=16,
                     help="equal-time block: FP64 reference spp = ref_passes x the workload's spp")
     ap.add_argument("--probe-spp", type=int, default=65536, help="equal-time block: CPU FP64 spp per probe pixel")
     ap.add_argument("--train-spp", type=int, default=1, help="equal-time block: spp rendered while the guide trains")
+    ap.add_argument("--no-batched", dest="batched", action="store_false",
+                    help="equal-time block: skip the batch-kernel legs (plain batched, guided+batched)")
     ap.add_argument("--guide-probs", default="0.5,0.25",
                     help="equal-time block: one guided run per proposal probability")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="BYTES",
@@ -582,6 +844,12 @@ def main():
     if args.workload == "quality":
         bench_quality(args)
         return
+    if args.workload == "decompose":
+        bench_decompose(args)
+        return
+    if args.workload == "guide-sweep":
+        bench_guide_sweep(args)
+        return
 
     from narvalengine_amd import Context, configs
     from narvalengine_amd.abi import NE_MODE_AUTO, NE_MODE_MEGAKERNEL, NE_MODE_WAVEFRONT
@@ -600,7 +868,7 @@ def main():
     log(f"scene on device: {info}")
     film = ctx.film(wl.width, wl.height)
     kw = dict(spp=wl.spp, max_bounces=wl.max_bounces, seed=1, samples_per_thread=args.samples_per_thread,
-              block_size=args.block, mode=mode, pool_size=args.pool, sort=args.sort)
+              block_size=args.block, mode=mode, pool_size=args.pool, sort=args.sort, spatial=args.spatial)
     for i in range(args.warmup):
         ctx.render(g, wl.camera, film, sample_offset=(1000 + i) * wl.spp, **kw)
         log(f"warmup {i + 1}/{args.warmup} done")
@@ -663,7 +931,7 @@ def main():
         "data": "synthetic (seeded procedural scene)",
         "config": {"workload": wl.name, "width": wl.width, "height": wl.height, "spp": wl.spp,
                    "max_bounces": wl.max_bounces, "paths_per_step": paths_per_step, "accumulation": "f64",
-                   "mode": args.mode, "pool": args.pool, "material_sort": args.sort,
+                   "mode": args.mode, "pool": args.pool, "material_sort": args.sort, "spatial_order": args.spatial,
                    "maj_res": wl.scene.volume["maj_res"] if wl.scene.volume is not None else None,
                    "empty_skip": (wl.scene.volume.get("empty_skip", True) if wl.scene.volume is not None
                                   else None)},

@@ -131,8 +131,10 @@ enum {
   NE_FLAG_TIME = 2,       /* time the path-tracing kernels with HIP events on the context stream */
   NE_FLAG_STAGE_TIME = 4, /* also time every kernel launch and sum the times per stage (NE_STAGE_*) */
   NE_FLAG_SORT = 8,       /* wavefront: shade each chunk's paths ordered by (event kind, material) */
-  NE_FLAG_FIXED_FILM = 16 /* wavefront: accumulate through 2^-30 fixed-point film atomics, the path taken
+  NE_FLAG_FIXED_FILM = 16, /* wavefront: accumulate through 2^-30 fixed-point film atomics, the path taken
                              anyway when the render's per-sample radiance slots (12 B each) exceed 16 GiB */
+  NE_FLAG_SPATIAL = 32     /* wavefront: the delta- and ratio-tracking stages take each chunk's rays in
+                              Morton order of their origins (same paths, same image) */
 };
 
 /* Kernel stages reported by ne_get_stage_times. */
@@ -278,6 +280,10 @@ typedef struct ne_job {
   uint64_t seed;
   uint32_t job; /* RNG stream word: paths are keyed (seed; pixel, sample, job) */
   uint32_t reserved;
+  /* NULL: plain phase sampling.  Else a guide with a tabulated proposal
+   * (cache_res > 0) from the same context: the job's medium vertices sample it
+   * as ne_render_guided does (guided+batched; no training inside the batch). */
+  const struct ne_guide* guide;
 } ne_job;
 /* Work-item shape: tile_w x tile_h pixels x samples_per_item samples. */
 typedef struct ne_batch_config {
@@ -320,6 +326,12 @@ typedef struct ne_guide_config {
   float guide_prob;        /* probability of sampling the guide at a guided vertex (default 0.25,
                               range [0, 0.9]); the rest samples HG (one-sample MIS) */
   uint64_t seed;      /* parameter initialisation */
+  int32_t cache_res;  /* > 0: the proposal is tabulated on cache_res^3 cells of the volume's unit cube x
+                         cache_dirs^2 octahedral bins of the incoming direction (re-tabulated after every
+                         training step) and vertices read their entry instead of evaluating the network;
+                         needed by the guided megakernel and guided batch jobs.  0 (default): per-vertex
+                         network evaluation in the wavefront.  Range [0, 128]. */
+  int32_t cache_dirs; /* direction bins per axis (default 4, range [1, 16]) */
 } ne_guide_config;
 typedef struct ne_guide_stats {
   uint64_t steps;     /* Adam steps taken */
@@ -342,13 +354,25 @@ ne_status ne_guide_eval(ne_guide* guide, int32_t n, const float* queries, int32_
  * gradient of -sum r log q / sum r and, if update != 0, an Adam step.  loss:
  * the weighted loss before the step. */
 ne_status ne_guide_train_step(ne_guide* guide, int32_t n, const float* samples, int32_t update, float* loss);
+/* The tabulated proposal (cache_res > 0), brought up to date with the
+ * parameters first: entries = cache_res^3 * cache_dirs^2 records of 40 floats
+ * (per lobe: weight, unit mean xyz, kappa), cell (x fastest) outer, direction
+ * bin inner.  Cell (i, j, k) is evaluated at the unit-cube point
+ * ((i + .5) / cache_res, ...), bin b = by * cache_dirs + bx at the octahedral
+ * point ((bx + .5) / cache_dirs, (by + .5) / cache_dirs) of [0, 1]^2, where a
+ * unit direction d maps to (p + 1) / 2 with p = d.xy / |d|_1, folded as
+ * (1 - |p.yx|) sign(p) for d.z < 0.  mix NULL: only the entry count. */
+ne_status ne_guide_get_cache(ne_guide* guide, float* mix, uint64_t* entries);
 /* ne_render with the guide: the first guided_vertices medium scattering
  * vertices of every path sample the one-sample MIS mixture
- * (1 - guide_prob) HG + guide_prob guide, later ones HG (FP32 wavefront
- * pipeline).  With
- * train != 0 the first guided vertex of every path is recorded; when the path
- * ends it becomes a training sample (weight Li * phase / pdf), and an Adam step
- * runs whenever half a batch is queued. */
+ * (1 - guide_prob) HG + guide_prob guide, later ones HG (FP32).  The
+ * wavefront pipeline (mode AUTO or WAVEFRONT) evaluates the network per
+ * vertex, or reads the tabulated proposal when cache_res > 0; the megakernel
+ * (mode MEGAKERNEL) needs the tabulation and does not train.  With train != 0
+ * (wavefront) one guided vertex of every path, chosen uniformly, is recorded;
+ * when the path ends it becomes a training sample (weight m Li phase / pdf, m
+ * the path's guided-vertex count), and an Adam step runs whenever half a batch
+ * is queued (the tabulation is refreshed after each step). */
 ne_status ne_render_guided(ne_ctx* ctx, const ne_scene* scene, const ne_camera_desc* cam,
                            const ne_render_params* params, ne_film* film, ne_guide* guide, int32_t train);
 /* Sizes: hash-table floats, MLP floats (weights [in][out] per layer, padded to

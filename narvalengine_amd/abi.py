@@ -10,6 +10,7 @@ NE_FLAG_TIME = 2
 NE_FLAG_STAGE_TIME = 4
 NE_FLAG_SORT = 8
 NE_FLAG_FIXED_FILM = 16
+NE_FLAG_SPATIAL = 32
 STAGES = ("megakernel", "plan", "regen", "trace", "medium", "shade", "shadow", "other", "guide", "train")
 NE_MODE_AUTO, NE_MODE_MEGAKERNEL, NE_MODE_WAVEFRONT = 0, 1, 2
 NE_MAT_GLOSSY, NE_MAT_EMITTER = 0, 1
@@ -86,7 +87,7 @@ class Job(C.Structure):
     """ne_job: one render job of a batch."""
     _fields_ = [("scene", C.c_void_p), ("camera", CameraDesc), ("width", C.c_int32), ("height", C.c_int32),
                 ("spp", C.c_int32), ("max_bounces", C.c_int32), ("seed", C.c_uint64), ("job", C.c_uint32),
-                ("reserved", C.c_uint32)]
+                ("reserved", C.c_uint32), ("guide", C.c_void_p)]
 
 
 class BatchConfig(C.Structure):
@@ -99,7 +100,8 @@ class GuideConfig(C.Structure):
     """ne_guide_config."""
     _fields_ = [("log2_table", C.c_int32), ("base_res", C.c_int32), ("max_res", C.c_int32), ("batch", C.c_int32),
                 ("lr_table", C.c_float), ("lr_mlp", C.c_float), ("beta1", C.c_float), ("beta2", C.c_float),
-                ("eps", C.c_float), ("guided_vertices", C.c_int32), ("guide_prob", C.c_float), ("seed", C.c_uint64)]
+                ("eps", C.c_float), ("guided_vertices", C.c_int32), ("guide_prob", C.c_float), ("seed", C.c_uint64),
+                ("cache_res", C.c_int32), ("cache_dirs", C.c_int32)]
 
 
 class GuideStats(C.Structure):
@@ -161,6 +163,7 @@ SIGNATURES = [
     ("ne_render_guided", C.c_int32, [C.c_void_p, C.c_void_p, C.POINTER(CameraDesc), C.POINTER(RenderParams),
                                      C.c_void_p, C.c_void_p, C.c_int32]),
     ("ne_guide_release", None, [C.c_void_p]),
+    ("ne_guide_get_cache", C.c_int32, [C.c_void_p, _F, C.POINTER(C.c_uint64)]),
     ("ne_guide_reset", C.c_int32, [C.c_void_p]),
     ("ne_guide_get_stats", C.c_int32, [C.c_void_p, C.POINTER(GuideStats)]),
     ("ne_guide_eval", C.c_int32, [C.c_void_p, C.c_int32, _F, C.c_int32, _F]),

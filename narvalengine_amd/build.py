@@ -24,7 +24,7 @@ FACADE_SRC = ROOT / "tests" / "cpp" / "facade_check.cpp"
 FACADE_BIN = ROOT / "tests" / "cpp" / "facade_check"
 SOURCES = ["render.hip", "wavefront.hip", "batch.hip", "guide.hip", "api.cpp", "bvh_build.cpp", "generators.cpp",
            "scene_io.cpp", "image_io.cpp"]
-HEADERS = ["ne_math.h", "ne_scene.h", "ne_trace.h", "ne_launch.h", "bvh_build.h"]
+HEADERS = ["ne_math.h", "ne_scene.h", "ne_trace.h", "ne_launch.h", "ne_vmf.h", "ne_guide.h", "bvh_build.h"]
 
 
 def hipcc() -> str:

@@ -63,6 +63,26 @@ struct ne_film {
   long long* d_fixed = nullptr;  // wavefront fixed-point accumulator, flushed into d_sum per render
 };
 
+struct ne_guide {
+  ne_ctx* ctx = nullptr;
+  ne_guide_config cfg{};
+  GuideNet net{};
+  GuideBatch batch{};
+  GuideTrain train{};
+  std::vector<void*> allocs;
+  size_t table_n = 0;
+  uint64_t steps = 0, samples = 0;
+  float last_loss = 0.0f;
+  GuideCache cache{};        // tabulated proposal (cache.mix == nullptr: none)
+  float* cache_mix = nullptr;
+  uint64_t version = 1;      // bumped by every parameter change
+  uint64_t cache_version = 0;  // parameters the tabulation was made from
+};
+
+// the guide's proposal tabulation and its use in a render (defined with the guide API)
+static ne_status guide_refresh_cache(ne_ctx* c, ne_guide* g);
+static GuideUse guide_use(const ne_guide* g);
+
 namespace {
 
 thread_local std::string g_err;
@@ -389,6 +409,8 @@ ne_status ne_bvh_build(const ne_scene_desc* d, float* nodes, uint32_t* refs, uin
   }
   return NE_OK;
 }
+
+static void ctx_free(ne_ctx* c);
 
 ne_status ne_ctx_create(int32_t device, ne_ctx** out) {
   if (!out) return fail(nullptr, NE_ERR_INVALID_ARG, "out is NULL");
@@ -772,7 +794,7 @@ static ne_status render_wavefront(ne_ctx* c, const ne_scene* s, const DevCamera&
   a.W = p->width; a.H = p->height; a.x0 = p->x0; a.y0 = p->y0; a.x1 = p->x1; a.y1 = p->y1;
   a.max_bounces = p->max_bounces; a.sample_offset = p->sample_offset; a.job = p->job; a.nchunks = uint32_t(P / kChunk);
   a.seed = p->seed;
-  a.sort = (p->flags & NE_FLAG_SORT) ? 1 : 0;
+  a.sort = ((p->flags & NE_FLAG_SORT) ? 1 : 0) | ((p->flags & NE_FLAG_SPATIAL) ? 2 : 0);
   a.refill_min = kRefillMinIdle;
   if (c->refill_blocks == 0) {
     int cus = 0, per_cu = 0;
@@ -847,8 +869,31 @@ static ne_status check_render(ne_ctx* c, const ne_scene* s, const ne_camera_desc
   return NE_OK;
 }
 
+// Megakernel launch arguments: one thread per (pixel, chunk of S samples); S
+// from the params or sized for about 2^22 threads.
+static RenderArgs megakernel_args(ne_ctx* c, const ne_render_params* p) {
+  RenderArgs a{};
+  a.W = p->width; a.H = p->height; a.x0 = p->x0; a.y0 = p->y0; a.x1 = p->x1; a.y1 = p->y1;
+  a.spp = p->spp; a.max_bounces = p->max_bounces; a.sample_offset = p->sample_offset; a.job = p->job;
+  a.seed = p->seed;
+  const int64_t npix = int64_t(p->x1 - p->x0) * (p->y1 - p->y0);
+  int S = p->samples_per_thread;
+  if (S <= 0) {
+    const int64_t target_threads = int64_t(1) << 22;
+    int64_t want = (npix * p->spp + target_threads - 1) / target_threads;
+    S = int(std::min<int64_t>(std::max<int64_t>(want, 1), p->spp));
+  }
+  S = std::min(S, p->spp);
+  a.S = S;
+  a.nchunks = (p->spp + S - 1) / S;
+  a.ntx = (p->x1 - p->x0 + 7) / 8;
+  a.nty = (p->y1 - p->y0 + 7) / 8;
+  a.counters = c->d_counters;
+  return a;
+}
+
 static ne_status render_locked(ne_ctx* c, const ne_scene* s, const ne_camera_desc* cam, const ne_render_params* p,
-                               ne_film* f) {
+                               ne_film* f, const GuideUse* gu = nullptr) {
   ne_status st = check_render(c, s, cam, p, f);
   if (st != NE_OK) return st;
   NE_HIP(c, hipSetDevice(c->device));
@@ -872,23 +917,9 @@ static ne_status render_locked(ne_ctx* c, const ne_scene* s, const ne_camera_des
     c->stage_timed = st == NE_OK && tm;
     return st;
   }
-  RenderArgs a{};
-  a.W = p->width; a.H = p->height; a.x0 = p->x0; a.y0 = p->y0; a.x1 = p->x1; a.y1 = p->y1;
-  a.spp = p->spp; a.max_bounces = p->max_bounces; a.sample_offset = p->sample_offset; a.job = p->job;
-  a.seed = p->seed;
+  RenderArgs a = megakernel_args(c, p);
+  if (gu) a.guide = *gu;
   const int64_t npix = int64_t(p->x1 - p->x0) * (p->y1 - p->y0);
-  int S = p->samples_per_thread;
-  if (S <= 0) {
-    const int64_t target_threads = int64_t(1) << 22;
-    int64_t want = (npix * p->spp + target_threads - 1) / target_threads;
-    S = int(std::min<int64_t>(std::max<int64_t>(want, 1), p->spp));
-  }
-  S = std::min(S, p->spp);
-  a.S = S;
-  a.nchunks = (p->spp + S - 1) / S;
-  a.ntx = (p->x1 - p->x0 + 7) / 8;
-  a.nty = (p->y1 - p->y0 + 7) / 8;
-  a.counters = c->d_counters;
   if (a.nchunks == 1) {
     a.out = f->d_sum;
   } else {
@@ -1210,6 +1241,8 @@ struct ne_batch {
   std::vector<float> job_ms;
   uint32_t grid_blocks = 0;
   double ticks_per_ms = 0;
+  bool surfaces = true;  // some job has surfaces or emitters (the general batch kernel)
+  bool guided = false;   // some job samples a guide's tabulated proposal
 };
 
 namespace {
@@ -1251,6 +1284,7 @@ ne_status batch_prepare(ne_batch* b, const ne_batch_config& cfg, int div) {
     T.chunks = uint32_t((T.spp + cfg.samples_per_item - 1) / cfg.samples_per_item);
     T.first_item = items;
     T.film = b->d_films + b->film_off[j];
+    if (J.guide) T.guide = guide_use(J.guide);
     items += (unsigned long long)T.tiles_x * T.tiles_y * T.chunks;
   }
   if (items > b->item_cap) {
@@ -1294,7 +1328,7 @@ ne_status batch_prepare(ne_batch* b, const ne_batch_config& cfg, int div) {
   NE_HIP(c, hipStreamSynchronize(c->stream));
   hipGraph_t graph = nullptr;
   NE_HIP(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-  hipError_t e = launch_batch(a, b->film_words, b->grid_blocks, b->ticks_per_ms, c->stream);
+  hipError_t e = launch_batch(a, b->surfaces, b->guided, b->film_words, b->grid_blocks, b->ticks_per_ms, c->stream);
   hipError_t e2 = hipStreamEndCapture(c->stream, &graph);
   if (e != hipSuccess || e2 != hipSuccess) {
     if (graph) (void)hipGraphDestroy(graph);
@@ -1315,6 +1349,9 @@ ne_status batch_launch(ne_batch* b, const ne_batch_config& cfg, int div, float* 
   ne_ctx* c = b->ctx;
   ne_status st = batch_prepare(b, cfg, div);
   if (st != NE_OK) return st;
+  // guided jobs read their guide's current tabulation
+  for (const ne_job& J : b->jobs)
+    if (J.guide && (st = guide_refresh_cache(c, const_cast<ne_guide*>(J.guide))) != NE_OK) return st;
   NE_HIP(c, hipEventRecord(b->ev0, c->stream));
   NE_HIP(c, hipGraphLaunch(b->exec, c->stream));
   NE_HIP(c, hipEventRecord(b->ev1, c->stream));
@@ -1340,6 +1377,8 @@ ne_status ne_batch_create(ne_ctx* c, const ne_job* jobs, int32_t n, ne_batch** o
       return fail(c, NE_ERR_INVALID_ARG, "job image size out of range");
     if (J.spp <= 0) return fail(c, NE_ERR_INVALID_ARG, "job spp must be positive");
     if (J.max_bounces < 0 || J.max_bounces > kMaxVertices) return fail(c, NE_ERR_INVALID_ARG, "job max_bounces out of range");
+    if (J.guide && (J.guide->ctx != c || !J.guide->cache_mix))
+      return fail(c, NE_ERR_INVALID_ARG, "a job's guide must come from the same context and tabulate its proposal");
   }
   std::lock_guard<std::mutex> lk(c->mu);
   NE_HIP(c, hipSetDevice(c->device));
@@ -1351,9 +1390,15 @@ ne_status ne_batch_create(ne_ctx* c, const ne_job* jobs, int32_t n, ne_batch** o
     b->film_words += size_t(jobs[j].width) * size_t(jobs[j].height) * 3;
   }
   auto bail = [&](ne_status st) { batch_free(b); delete b; return st; };
+  b->surfaces = false;
+  b->guided = false;
+  for (int32_t j = 0; j < n; ++j) {
+    if (scene_has_surfaces(jobs[j].scene->dev)) b->surfaces = true;
+    if (jobs[j].guide) b->guided = true;
+  }
   int cus = 0, per_cu = 0, khz = 0;
   if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device) != hipSuccess ||
-      batch_blocks_per_cu(&per_cu) != hipSuccess ||
+      batch_blocks_per_cu(b->surfaces, b->guided, &per_cu) != hipSuccess ||
       hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, c->device) != hipSuccess || khz <= 0)
     return bail(fail(c, NE_ERR_HIP, "device attribute query failed"));
   b->grid_blocks = uint32_t(std::max(1, cus * std::max(1, per_cu)));
@@ -1443,17 +1488,6 @@ ne_status ne_batch_read(ne_batch* b, int32_t job, double* sum) {
 }
 
 // ---- learned guide -------------------------------------------------------------
-struct ne_guide {
-  ne_ctx* ctx = nullptr;
-  ne_guide_config cfg{};
-  GuideNet net{};
-  GuideBatch batch{};
-  GuideTrain train{};
-  std::vector<void*> allocs;
-  size_t table_n = 0;
-  uint64_t steps = 0, samples = 0;
-  float last_loss = 0.0f;
-};
 
 namespace {
 
@@ -1476,8 +1510,11 @@ bool guide_alloc(ne_guide* g, T*& p, size_t count) {
 ne_status ne_guide_create(ne_ctx* c, const ne_guide_config* cfg_in, ne_guide** out) {
   if (!c || !out) return fail(c, NE_ERR_INVALID_ARG, "ctx/out is NULL");
   *out = nullptr;
-  ne_guide_config cfg{19, 16, 512, 65536, 1e-2f, 1e-3f, 0.9f, 0.99f, 1e-8f, 1, 0.25f, 0};
+  ne_guide_config cfg{19, 16, 512, 65536, 1e-2f, 1e-3f, 0.9f, 0.99f, 1e-8f, 1, 0.25f, 0, 0, 4};
   if (cfg_in) cfg = *cfg_in;
+  if (cfg.cache_res < 0 || cfg.cache_res > 128) return fail(c, NE_ERR_INVALID_ARG, "cache_res must be in [0, 128]");
+  if (cfg.cache_res > 0 && (cfg.cache_dirs < 1 || cfg.cache_dirs > 16))
+    return fail(c, NE_ERR_INVALID_ARG, "cache_dirs must be in [1, 16]");
   if (cfg.log2_table < 10 || cfg.log2_table > 22) return fail(c, NE_ERR_INVALID_ARG, "log2_table must be in [10, 22]");
   if (cfg.base_res < 2 || cfg.max_res < cfg.base_res || cfg.max_res > 4096)
     return fail(c, NE_ERR_INVALID_ARG, "level resolutions out of range");
@@ -1504,7 +1541,8 @@ ne_status ne_guide_create(ne_ctx* c, const ne_guide_config* cfg_in, ne_guide** o
   GuideTrain& t = g->train;
   t.waves = 512;
   auto bail = [&](ne_status st) { guide_free(g); delete g; return st; };
-  bool ok = guide_alloc(g, n.table, g->table_n) && guide_alloc(g, n.table_grad, g->table_n) &&
+  bool ok = guide_alloc(g, n.table, g->table_n) && guide_alloc(g, n.table_bf, g->table_n / kGuideFeat) &&
+            guide_alloc(g, n.table_grad, g->table_n) &&
             guide_alloc(g, n.table_m, g->table_n) && guide_alloc(g, n.table_v, g->table_n) &&
             guide_alloc(g, n.mlp, kGuideParams) && guide_alloc(g, n.mlp_grad, kGuideParams) &&
             guide_alloc(g, n.mlp_m, kGuideParams) && guide_alloc(g, n.mlp_v, kGuideParams) &&
@@ -1514,6 +1552,13 @@ ne_status ne_guide_create(ne_ctx* c, const ne_guide_config* cfg_in, ne_guide** o
             guide_alloc(g, t.dy, cap * kGuideWidth) && guide_alloc(g, t.dy2, cap * kGuideWidth) &&
             guide_alloc(g, t.denc, cap * kGuideEnc) && guide_alloc(g, t.partial, size_t(t.waves) * kGuideParams) &&
             guide_alloc(g, t.loss, 2);
+  if (ok && cfg.cache_res > 0) {
+    const size_t entries = size_t(cfg.cache_res) * cfg.cache_res * cfg.cache_res * size_t(cfg.cache_dirs) * cfg.cache_dirs;
+    ok = guide_alloc(g, g->cache_mix, entries * 40);
+    g->cache.mix = g->cache_mix;
+    g->cache.nc = cfg.cache_res;
+    g->cache.nd = cfg.cache_dirs;
+  }
   if (!ok) return bail(fail(c, NE_ERR_OUT_OF_MEMORY, "guide allocation failed"));
   g->batch.cap = uint32_t(cap);
   if (hipMemsetAsync(g->batch.count, 0, sizeof(uint32_t), c->stream) != hipSuccess ||
@@ -1547,6 +1592,7 @@ ne_status ne_guide_reset(ne_guide* g) {
   g->steps = 0;
   g->samples = 0;
   g->last_loss = 0.0f;
+  g->version += 1;
   return NE_OK;
 }
 
@@ -1604,6 +1650,7 @@ ne_status ne_guide_train_step(ne_guide* g, int32_t n, const float* smp, int32_t 
   g->samples += uint64_t(n);
   if (update) {
     g->steps += 1;
+    g->version += 1;
     NE_HIP(c, launch_guide_adam(g->net, g->cfg.lr_table, g->cfg.lr_mlp, g->cfg.beta1, g->cfg.beta2, g->cfg.eps,
                                 int(g->steps), c->stream));
     NE_HIP(c, hipStreamSynchronize(c->stream));
@@ -1639,6 +1686,7 @@ ne_status ne_guide_set_params(ne_guide* g, const float* table, const float* mlp)
   if (mlp) NE_HIP(c, hipMemcpy(g->net.mlp, mlp, kGuideParams * sizeof(float), hipMemcpyHostToDevice));
   NE_HIP(c, launch_guide_pack(g->net, c->stream));
   NE_HIP(c, hipStreamSynchronize(c->stream));
+  g->version += 1;
   return NE_OK;
 }
 
@@ -1652,19 +1700,56 @@ ne_status ne_guide_get_grads(ne_guide* g, float* table_grad, float* mlp_grad) {
   return NE_OK;
 }
 
+// Re-tabulates the guide's proposal if its parameters changed since the last
+// tabulation (context lock held).
+static ne_status guide_refresh_cache(ne_ctx* c, ne_guide* g) {
+  if (!g->cache_mix || g->cache_version == g->version) return NE_OK;
+  NE_HIP(c, launch_guide_bake(g->net, g->cache, g->cache_mix, c->stream));
+  g->cache_version = g->version;
+  return NE_OK;
+}
+
+static GuideUse guide_use(const ne_guide* g) {
+  GuideUse u{};
+  u.cache = g->cache;
+  u.max_depth = g->cfg.guided_vertices;
+  u.prob = g->cfg.guide_prob;
+  return u;
+}
+
+ne_status ne_guide_get_cache(ne_guide* g, float* mix, uint64_t* entries) {
+  if (!g) return fail(nullptr, NE_ERR_INVALID_ARG, "guide is NULL");
+  ne_ctx* c = g->ctx;
+  if (!g->cache_mix) return fail(c, NE_ERR_INVALID_ARG, "the guide tabulates no proposal (cache_res = 0)");
+  const uint64_t n = uint64_t(g->cache.nc) * g->cache.nc * g->cache.nc * uint64_t(g->cache.nd) * g->cache.nd;
+  if (entries) *entries = n;
+  if (!mix) return NE_OK;
+  std::lock_guard<std::mutex> lk(c->mu);
+  NE_HIP(c, hipSetDevice(c->device));
+  ne_status st = guide_refresh_cache(c, g);
+  if (st != NE_OK) return st;
+  NE_HIP(c, hipMemcpyAsync(mix, g->cache_mix, size_t(n) * 40 * sizeof(float), hipMemcpyDeviceToHost, c->stream));
+  NE_HIP(c, hipStreamSynchronize(c->stream));
+  return NE_OK;
+}
+
 ne_status ne_render_guided(ne_ctx* c, const ne_scene* s, const ne_camera_desc* cam, const ne_render_params* p,
                            ne_film* f, ne_guide* g, int32_t train) {
   if (!c) return fail(nullptr, NE_ERR_INVALID_ARG, "ctx is NULL");
   if (!g || g->ctx != c) return fail(c, NE_ERR_INVALID_ARG, "guide is NULL or from another context");
-  if (!p || p->precision != NE_FP32 || p->mode == NE_MODE_MEGAKERNEL)
-    return fail(c, NE_ERR_INVALID_ARG, "guided rendering runs the FP32 wavefront pipeline");
+  if (!p || p->precision != NE_FP32) return fail(c, NE_ERR_INVALID_ARG, "guided rendering is FP32");
+  if (p->mode == NE_MODE_MEGAKERNEL && (!g->cache_mix || train))
+    return fail(c, NE_ERR_INVALID_ARG,
+                "the guided megakernel needs a tabulated proposal (cache_res > 0) and does not train");
   std::lock_guard<std::mutex> lk(c->mu);
-  ne_status st = check_params(c, p);
+  ne_status st = check_render(c, s, cam, p, f);
   if (st != NE_OK) return st;
-  if (!s || !cam || !f) return fail(c, NE_ERR_INVALID_ARG, "scene/camera/film is NULL");
-  if (s->ctx != c || f->ctx != c) return fail(c, NE_ERR_INVALID_ARG, "scene/film belong to another context");
-  if (f->W != p->width || f->H != p->height) return fail(c, NE_ERR_INVALID_ARG, "film size differs from params");
   NE_HIP(c, hipSetDevice(c->device));
+  if ((st = guide_refresh_cache(c, g)) != NE_OK) return st;
+  if (p->mode == NE_MODE_MEGAKERNEL) {
+    const GuideUse u = guide_use(g);
+    return render_locked(c, s, cam, p, f, &u);
+  }
   const bool count = (p->flags & NE_FLAG_COUNT) != 0;
   const bool timed = (p->flags & NE_FLAG_TIME) != 0;
   if (count) {
@@ -1684,6 +1769,8 @@ ne_status ne_render_guided(ne_ctx* c, const ne_scene* s, const ne_camera_desc* c
   hook.g.train = train ? 1 : 0;
   hook.g.max_depth = g->cfg.guided_vertices;
   hook.g.prob = g->cfg.guide_prob;
+  hook.g.cache = g->cache;
+  hook.cache_mix = g->cache_mix;
   hook.train = g->train;
   hook.lr_table = g->cfg.lr_table; hook.lr_mlp = g->cfg.lr_mlp;
   hook.beta1 = g->cfg.beta1; hook.beta2 = g->cfg.beta2; hook.eps = g->cfg.eps;
@@ -1697,6 +1784,11 @@ ne_status ne_render_guided(ne_ctx* c, const ne_scene* s, const ne_camera_desc* c
   c->stage_timed = st == NE_OK && tm;
   g->steps = hook.steps;
   g->samples += hook.samples;
+  if (g->steps > steps0) {
+    // every step of the render re-tabulated the proposal right after its update
+    g->version += g->steps - steps0;
+    if (g->cache_mix && st == NE_OK) g->cache_version = g->version;
+  }
   if (st == NE_OK && g->steps > steps0) {  // loss of the last step taken during the render
     float lw[2] = {0, 0};
     NE_HIP(c, hipMemcpyAsync(lw, g->train.loss, sizeof(lw), hipMemcpyDeviceToHost, c->stream));

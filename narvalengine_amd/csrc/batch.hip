@@ -43,6 +43,10 @@ __device__ __forceinline__ int batch_job_of(const unsigned long long* first, int
 // At least 3 waves per SIMD: the register budget drops from 185 to 168
 // VGPRs with 8 dwords spilled; measured on MI355X, batch48 ran 1636 ms per
 // launch at 2 waves and 1368 ms at 3 (4 waves spill 48 dwords: 1550 ms).
+// SURF = false: every job is a volume lit by the sun and the environment
+// (no surfaces, no emitters), and the bounce carries no surface code.
+// GUIDED: jobs with a guide sample its tabulated proposal at medium vertices.
+template <bool SURF, bool GUIDED>
 __global__ __launch_bounds__(kBatchBlock, 3) void k_batch(BatchArgs b) {
   __shared__ uint32_t lds_stack[kShortStack * kBatchBlock];
   __shared__ unsigned long long first[kMaxBatchJobs];
@@ -114,7 +118,7 @@ __global__ __launch_bounds__(kBatchBlock, 3) void k_batch(BatchArgs b) {
     }
     if (!have) break;  // queue empty and no path left for this lane
     const BatchJob& J = b.jobs[lj];
-    if (path_bounce<float, false>(J.scene, ps, J.max_bounces, rng, lds, kBatchBlock, cnt)) {
+    if (path_bounce<float, false, SURF, GUIDED>(J.scene, ps, J.max_bounces, rng, lds, kBatchBlock, cnt, &J.guide)) {
       film_add(J.film, lpix, ps.L.x, ps.L.y, ps.L.z);
       atomicMax(&b.item_time[lq], (unsigned long long)wall_clock64());
       have = false;
@@ -151,14 +155,25 @@ __global__ void k_fixed_to_double(const long long* in, double* out, size_t n) {
   if (i < n) out[i] = double(in[i]) * (1.0 / 1073741824.0);
 }
 
-hipError_t batch_blocks_per_cu(int* blocks) {
-  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, reinterpret_cast<const void*>(&k_batch), kBatchBlock, 0);
+static const void* batch_kernel(bool surfaces, bool guided) {
+  if (surfaces) return guided ? reinterpret_cast<const void*>(&k_batch<true, true>)
+                              : reinterpret_cast<const void*>(&k_batch<true, false>);
+  return guided ? reinterpret_cast<const void*>(&k_batch<false, true>)
+                : reinterpret_cast<const void*>(&k_batch<false, false>);
 }
 
-hipError_t launch_batch(const BatchArgs& b, size_t film_words, uint32_t grid_blocks, double ticks_per_ms,
-                        hipStream_t st) {
+hipError_t batch_blocks_per_cu(bool surfaces, bool guided, int* blocks) {
+  return hipOccupancyMaxActiveBlocksPerMultiprocessor(blocks, batch_kernel(surfaces, guided), kBatchBlock, 0);
+}
+
+hipError_t launch_batch(const BatchArgs& b, bool surfaces, bool guided, size_t film_words, uint32_t grid_blocks,
+                        double ticks_per_ms, hipStream_t st) {
   hipLaunchKernelGGL(k_batch_init, dim3(2048), dim3(256), 0, st, b, film_words);
-  hipLaunchKernelGGL(k_batch, dim3(grid_blocks), dim3(kBatchBlock), 0, st, b);
+  const dim3 grid(grid_blocks), blk(kBatchBlock);
+  if (surfaces && guided) hipLaunchKernelGGL((k_batch<true, true>), grid, blk, 0, st, b);
+  else if (surfaces) hipLaunchKernelGGL((k_batch<true, false>), grid, blk, 0, st, b);
+  else if (guided) hipLaunchKernelGGL((k_batch<false, true>), grid, blk, 0, st, b);
+  else hipLaunchKernelGGL((k_batch<false, false>), grid, blk, 0, st, b);
   hipLaunchKernelGGL(k_batch_job_times, dim3(uint32_t(b.njobs)), dim3(256), 0, st, b, ticks_per_ms);
   return hipGetLastError();
 }

@@ -19,7 +19,7 @@
 // runs layer by layer over the whole batch; weight gradients are reduced from
 // per-wave partial sums in a fixed order (deterministic), hash-feature
 // gradients are scattered with atomics.  Adam updates FP32 masters; BF16
-// operand copies are re-packed after each step.
+// operand copies (MLP weights and hash features) are re-packed after each step.
 #include "ne_guide.h"
 
 namespace ne {
@@ -51,6 +51,36 @@ __global__ __launch_bounds__(kGuideBlock) void k_guide_eval(GuideNet g, int n, c
           guide_head(&out[wv][lane * kLdO], o);
         }
       }
+    }
+  }
+}
+
+// Tabulated proposal: the mixture at every (cell centre, direction-bin centre)
+// of the cache, written to cache.mix (40 floats per entry).
+__global__ __launch_bounds__(kGuideBlock) void k_guide_bake(GuideNet g, GuideCache c, float* mix, int n) {
+  __shared__ uint16_t xa[kWavesPerBlock][16 * kLdA];
+  __shared__ uint16_t xb[kWavesPerBlock][16 * kLdA];
+  __shared__ float out[kWavesPerBlock][16 * kLdO];
+  const int wv = threadIdx.x >> 6, lane = __lane_id();
+  const int nb = c.nd * c.nd;
+  const int tiles = (n + 15) / 16;
+  for (int t = blockIdx.x * kWavesPerBlock + wv; t < tiles; t += gridDim.x * kWavesPerBlock) {
+    wave_sync();
+    guide_encode_tile(g, xa[wv], [&](int i, Vec3<float>& p, Vec3<float>& d, bool& valid) {
+      const int q = t * 16 + i;
+      valid = q < n;
+      const int e = valid ? q : 0;
+      const int cell = e / nb, bin = e % nb;
+      const int cx = cell % c.nc, cy = (cell / c.nc) % c.nc, cz = cell / (c.nc * c.nc);
+      const float inv = 1.0f / float(c.nc);
+      p = v3((float(cx) + 0.5f) * inv, (float(cy) + 0.5f) * inv, (float(cz) + 0.5f) * inv);
+      const float invd = 1.0f / float(c.nd);
+      oct_decode((float(bin % c.nd) + 0.5f) * invd, (float(bin / c.nd) + 0.5f) * invd, d.x, d.y, d.z);
+    });
+    guide_mlp_tile(g, xa[wv], xb[wv], out[wv]);
+    if (lane < 16) {
+      const int q = t * 16 + lane;
+      if (q < n) guide_head(&out[wv][lane * kLdO], mix + 40 * size_t(q));
     }
   }
 }
@@ -260,33 +290,80 @@ __global__ __launch_bounds__(kGuideBlock) void k_guide_train_bwd(GuideNet g, Gui
   }
 }
 
-// mlp_grad[p] = sum over waves of the partials (fixed order: deterministic)
-__global__ void k_guide_reduce(GuideNet g, GuideTrain t) {
-  const int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= kGuideParams) return;
+// mlp_grad[p] = sum over waves of the partials, in a fixed order
+// (deterministic): a block takes 64 parameters, its 4 thread groups sum the
+// waves w = j mod 4 each, and the 4 group sums are added in order.
+__global__ __launch_bounds__(256) void k_guide_reduce(GuideNet g, GuideTrain t) {
+  __shared__ float part[4][64];
+  const int j = int(threadIdx.x >> 6), col = int(threadIdx.x & 63);
+  const int p = int(blockIdx.x) * 64 + col;
   float s = 0.0f;
-  for (int w = 0; w < t.waves; ++w) s += t.partial[size_t(w) * kGuideParams + p];
-  g.mlp_grad[p] = s;
+  if (p < kGuideParams)
+    for (int w = j; w < t.waves; w += 4) s += t.partial[size_t(w) * kGuideParams + p];
+  part[j][col] = s;
+  __syncthreads();
+  if (j == 0 && p < kGuideParams) g.mlp_grad[p] = ((part[0][col] + part[1][col]) + part[2][col]) + part[3][col];
 }
 
-// hash-feature gradients: scatter denc * corner weight (scaled) into table_grad
-__global__ void k_guide_table_grad(GuideNet g, GuideBatch b, GuideTrain t) {
+// hash-feature gradients: scatter denc * corner weight (scaled) into
+// table_grad.  One thread per sample, levels in turn.  On the coarse levels
+// (resolution <= 64) the samples of a wave mostly share corners, so the wave
+// first merges equal entries: the lowest pending lane's entry is taken, the
+// lanes holding it are summed across the wave and one lane adds the sum (up
+// to 4 entries per corner); whatever is left, and every finer level, adds
+// directly.
+__global__ __launch_bounds__(256) void k_guide_table_grad(GuideNet g, GuideBatch b, GuideTrain t) {
   const float scale = t.loss[1] > 0.0f ? 1.0f / t.loss[1] : 0.0f;
   const uint32_t n = min(*b.count, b.cap);
-  const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;  // (sample, level)
-  if (i >= n * kGuideLevels) return;
-  const uint32_t q = i / kGuideLevels;
-  const int l = int(i % kGuideLevels);
-  const Vec3<float> p = v3(b.s[q], b.s[b.cap + q], b.s[2 * b.cap + q]);
-  const float d0 = t.denc[size_t(q) * kGuideEnc + 2 * l] * scale;
-  const float d1 = t.denc[size_t(q) * kGuideEnc + 2 * l + 1] * scale;
-  for (int k = 0; k < 8; ++k) {
-    uint32_t off;
-    float w;
-    guide_corner(g, l, p, k, off, w);
-    if (d0 != 0.0f) atomicAdd(&g.table_grad[off], w * d0);
-    if (d1 != 0.0f) atomicAdd(&g.table_grad[off + 1], w * d1);
+  const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+  const bool valid = q < n;  // no early exit: every lane takes part in the wave merges
+  const int lane = __lane_id();
+  const uint32_t qq = valid ? q : 0u;
+  const Vec3<float> p = v3(b.s[qq], b.s[b.cap + qq], b.s[2 * b.cap + qq]);
+  for (int l = 0; l < kGuideLevels; ++l) {
+    const float d0 = valid ? t.denc[size_t(qq) * kGuideEnc + 2 * l] * scale : 0.0f;
+    const float d1 = valid ? t.denc[size_t(qq) * kGuideEnc + 2 * l + 1] * scale : 0.0f;
+    const bool coarse = g.res[l] <= 64;  // the same for every lane
+    for (int k = 0; k < 8; ++k) {
+      uint32_t ent;
+      float w;
+      guide_corner(g, l, p, k, ent, w);
+      const float v0 = w * d0, v1 = w * d1;
+      bool pend = valid && (v0 != 0.0f || v1 != 0.0f);
+      if (coarse) {
+        for (int it = 0; it < 4; ++it) {
+          const uint64_t m = __ballot(pend);
+          if (m == 0) break;
+          const int leader = __ffsll((long long)m) - 1;
+          const uint32_t key = uint32_t(__shfl(int(ent), leader));
+          const bool mine = pend && ent == key;
+          float s0 = mine ? v0 : 0.0f, s1 = mine ? v1 : 0.0f;
+          for (int o = 32; o > 0; o >>= 1) {
+            s0 += __shfl_xor(s0, o);
+            s1 += __shfl_xor(s1, o);
+          }
+          if (lane == leader) {
+            atomicAdd(&g.table_grad[2 * size_t(key)], s0);
+            atomicAdd(&g.table_grad[2 * size_t(key) + 1], s1);
+          }
+          pend = pend && !mine;
+        }
+      }
+      if (pend) {
+        atomicAdd(&g.table_grad[2 * size_t(ent)], v0);
+        atomicAdd(&g.table_grad[2 * size_t(ent) + 1], v1);
+      }
+    }
   }
+}
+
+// BF16 copy of the hash features that the encoder reads (after every change
+// of the FP32 masters)
+__global__ void k_guide_pack_table(GuideNet g, size_t entries) {
+  const size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= entries) return;
+  const float2 v = reinterpret_cast<const float2*>(g.table)[i];
+  g.table_bf[i] = uint32_t(f2bf(v.x)) | (uint32_t(f2bf(v.y)) << 16);
 }
 
 __global__ void k_guide_adam(float* prm, float* grad, float* m, float* v, size_t n, float lr, float b1, float b2,
@@ -354,6 +431,8 @@ hipError_t launch_guide_init(const GuideNet& g, uint64_t seed, hipStream_t st) {
 
 hipError_t launch_guide_pack(const GuideNet& g, hipStream_t st) {
   hipLaunchKernelGGL(k_guide_pack, dim3((kGuideWeights + 255) / 256), dim3(256), 0, st, g);
+  const size_t entries = table_words(g) / kGuideFeat;
+  hipLaunchKernelGGL(k_guide_pack_table, dim3(uint32_t((entries + 255) / 256)), dim3(256), 0, st, g, entries);
   return hipGetLastError();
 }
 
@@ -368,6 +447,14 @@ hipError_t launch_guide_raw(const GuideNet& g, int n, const float* in6, float* o
   const int tiles = (n + 15) / 16;
   const int blocks = std::max(1, std::min(4096, (tiles + kWavesPerBlock - 1) / kWavesPerBlock));
   hipLaunchKernelGGL(k_guide_eval, dim3(blocks), dim3(kGuideBlock), 0, st, g, n, in6, out40, 1);
+  return hipGetLastError();
+}
+
+hipError_t launch_guide_bake(const GuideNet& g, const GuideCache& c, float* mix, hipStream_t st) {
+  const int n = c.nc * c.nc * c.nc * c.nd * c.nd;
+  const int tiles = (n + 15) / 16;
+  const int blocks = std::max(1, std::min(8192, (tiles + kWavesPerBlock - 1) / kWavesPerBlock));
+  hipLaunchKernelGGL(k_guide_bake, dim3(blocks), dim3(kGuideBlock), 0, st, g, c, mix, n);
   return hipGetLastError();
 }
 
@@ -387,8 +474,8 @@ hipError_t launch_guide_grad(const GuideNet& g, const GuideBatch& b, const Guide
   hipLaunchKernelGGL((k_guide_train_bwd<2>), dim3(blocks), blk, 0, st, g, b, t, t.dy, t.dy2);
   hipLaunchKernelGGL((k_guide_train_bwd<1>), dim3(blocks), blk, 0, st, g, b, t, t.dy2, t.dy);
   hipLaunchKernelGGL((k_guide_train_bwd<0>), dim3(blocks), blk, 0, st, g, b, t, t.dy, t.dy2);
-  hipLaunchKernelGGL(k_guide_reduce, dim3((kGuideParams + 255) / 256), dim3(256), 0, st, g, t);
-  hipLaunchKernelGGL(k_guide_table_grad, dim3((b.cap * kGuideLevels + 255) / 256), dim3(256), 0, st, g, b, t);
+  hipLaunchKernelGGL(k_guide_reduce, dim3((kGuideParams + 63) / 64), dim3(256), 0, st, g, t);
+  hipLaunchKernelGGL(k_guide_table_grad, dim3((b.cap + 255) / 256), dim3(256), 0, st, g, b, t);
   return hipGetLastError();
 }
 

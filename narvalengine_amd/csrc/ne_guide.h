@@ -5,6 +5,7 @@
 #pragma once
 #include "ne_launch.h"
 #include "ne_math.h"
+#include "ne_vmf.h"
 
 namespace ne {
 
@@ -46,8 +47,9 @@ __device__ __forceinline__ uint32_t guide_cell_index(const GuideNet& g, int l, u
 }
 
 // Trilinear corner k (bit 0 x, bit 1 y, bit 2 z) of level l at unit position p:
-// table word offset of feature 0 and the weight.
-__device__ __forceinline__ void guide_corner(const GuideNet& g, int l, Vec3<float> p, int k, uint32_t& off,
+// table entry index (feature f of the FP32 master at word 2 * entry + f; both
+// BF16 features in table_bf[entry]) and the weight.
+__device__ __forceinline__ void guide_corner(const GuideNet& g, int l, Vec3<float> p, int k, uint32_t& entry,
                                              float& w) {
   const float r = float(g.res[l]);
   const float sx = xmin(xmax(p.x, 0.0f), 1.0f) * r, sy = xmin(xmax(p.y, 0.0f), 1.0f) * r,
@@ -58,7 +60,7 @@ __device__ __forceinline__ void guide_corner(const GuideNet& g, int l, Vec3<floa
   const uint32_t x = uint32_t(fx) + (k & 1), y = uint32_t(fy) + ((k >> 1) & 1), z = uint32_t(fz) + ((k >> 2) & 1);
   w = ((k & 1) ? tx : 1.0f - tx) * (((k >> 1) & 1) ? ty : 1.0f - ty) * (((k >> 2) & 1) ? tz : 1.0f - tz);
   const uint32_t T = 1u << g.log2T;
-  off = (uint32_t(l) * T + guide_cell_index(g, l, x, y, z)) * kGuideFeat;
+  entry = uint32_t(l) * T + guide_cell_index(g, l, x, y, z);
 }
 
 // ---- MLP tile (one wave, 16 rows) -------------------------------------------------
@@ -113,11 +115,12 @@ __device__ __forceinline__ void guide_encode_tile(const GuideNet& g, uint16_t* x
     if (valid) {
 #pragma unroll
       for (int k = 0; k < 8; ++k) {
-        uint32_t off;
+        uint32_t ent;
         float w;
-        guide_corner(g, l, p, k, off, w);
-        e0 += w * g.table[off];
-        e1 += w * g.table[off + 1];
+        guide_corner(g, l, p, k, ent, w);
+        const uint32_t h = g.table_bf[ent];  // 4 B per corner: features 0 and 1 as BF16
+        e0 += w * bf2f(uint16_t(h & 0xFFFFu));
+        e1 += w * bf2f(uint16_t(h >> 16));
       }
     }
     xs[row * kLdA + 2 * l] = f2bf(e0);
@@ -166,53 +169,6 @@ __device__ __forceinline__ void guide_head(const float* y, float* mix) {
     mix[5 * k + 3] = n > 1e-12f ? o[3] * inv : 1.0f;
     mix[5 * k + 4] = guide_kappa(o[4]);
   }
-}
-
-// ---- vMF mixture ------------------------------------------------------------------
-// vMF density kappa / (2 pi (1 - exp(-2 kappa))) exp(kappa (mu.w - 1)), stable for
-// large kappa; the normaliser equals kappa / (4 pi sinh kappa) exp(kappa).
-__device__ __forceinline__ float vmf_pdf(float kappa, float cosv) {
-  return kappa / (float(kTWO_PI) * -expm1f(-2.0f * kappa)) * expf(kappa * (cosv - 1.0f));
-}
-// d log vMF / d kappa = c - L(kappa), L the Langevin function coth k - 1/k
-__device__ __forceinline__ float vmf_dlog_dkappa(float kappa, float cosv) {
-  const float L = kappa < 1e-2f ? kappa / 3.0f - kappa * kappa * kappa / 45.0f : 1.0f / tanhf(kappa) - 1.0f / kappa;
-  return cosv - L;
-}
-
-// vMF sample around mu (inverse CDF of cos theta in the stable form).
-// cos theta = 1 + log(u + (1 - u) exp(-2 kappa)) / kappa, written with
-// log1p / expm1 so it stays exact as kappa -> 0 (uniform sphere).
-__device__ __forceinline__ Vec3<float> vmf_sample(Vec3<float> mu, float kappa, float u0, float u1) {
-  const float c = 1.0f + log1pf((1.0f - u0) * expm1f(-2.0f * kappa)) / kappa;
-  const float cc = xmin(1.0f, xmax(-1.0f, c));
-  const float sn = sqrtf(xmax(0.0f, 1.0f - cc * cc));
-  const float ph = float(kTWO_PI) * u1;
-  Vec3<float> ss, ts;
-  onb(mu, ss, ts);
-  return to_world(v3(sn * cosf(ph), sn * sinf(ph), cc), mu, ss, ts);
-}
-
-// Mixture density at w; mix: 8 lobes of (weight, unit mean xyz, kappa).
-__device__ __forceinline__ float vmf_mixture_pdf(const float* mix, Vec3<float> w) {
-  float q = 0.0f;
-  for (int k = 0; k < kGuideLobes; ++k) {
-    const float* m = mix + 5 * k;
-    q += m[0] * vmf_pdf(m[4], m[1] * w.x + m[2] * w.y + m[3] * w.z);
-  }
-  return q;
-}
-
-// Lobe chosen by u (inverse CDF over the weights), then a vMF sample.
-__device__ __forceinline__ Vec3<float> vmf_mixture_sample(const float* mix, float u, float u0, float u1) {
-  int k = 0;
-  float acc = mix[0];
-  while (k < kGuideLobes - 1 && u >= acc) {
-    ++k;
-    acc += mix[5 * k];
-  }
-  const float* m = mix + 5 * k;
-  return vmf_sample(v3(m[1], m[2], m[3]), m[4], u0, u1);
 }
 
 }  // namespace ne

@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "ne_scene.h"
+#include "ne_vmf.h"
 
 namespace ne {
 
@@ -40,7 +41,15 @@ struct RenderArgs {
   int32_t ntx, nty; // 8x8 pixel tiles covering the rect
   double* out;      // film (nchunks == 1) or partial sums [nchunks][rect pixels][3]
   DevCounters* counters;
+  GuideUse guide;   // guide.cache.mix != nullptr: guided medium vertices (FP32 only)
 };
+
+// Whether a scene needs the surface code of the path tracers: a BVH or any
+// emitter (a volume lit only by the sun and the environment does not).
+inline bool scene_has_surfaces(const DevScene& s) {
+  const int emitters = s.n_lights - (s.has_sun ? 1 : 0) - (s.env_type != 0 ? 1 : 0);
+  return s.n_nodes > 0 || emitters > 0;
+}
 
 hipError_t launch_render(const DevScene& s, const DevCamera& cam, const RenderArgs& a, int precision, bool count,
                          int block, int occupancy, hipStream_t st);
@@ -108,7 +117,8 @@ struct WfArgs {
   float* sbuf;  // per-sample radiance (3 floats per path of the render), or null: fixed-point film atomics
   double* film;
   uint32_t* gstate[2];  // guided mode: training-record state per path buffer (else null)
-  int32_t sort;         // shade stage sorts each chunk by (event kind, material)
+  int32_t sort;         // bit 0: shade sorts each chunk by (event kind, material);
+                        // bit 1: the tracking stages take each chunk's rays in Morton order
   int32_t refill_min;   // refill kernels: a wave refills once this many of its lanes are idle
 };
 
@@ -128,6 +138,7 @@ struct BatchJob {
   uint32_t tiles_x, tiles_y, chunks;  // work items of the job: tiles_x * tiles_y * chunks
   unsigned long long first_item;      // global index of its first item
   long long* film;                    // W * H * 3 fixed-point sums
+  GuideUse guide;                     // guide.cache.mix != nullptr: guided medium vertices
 };
 struct BatchArgs {
   const BatchJob* jobs;
@@ -140,17 +151,20 @@ struct BatchArgs {
   float* job_ms;                      // per-job completion time
   long long* films;                   // all films, contiguous
 };
-hipError_t batch_blocks_per_cu(int* blocks);
+// surfaces = false: every job's scene is a volume lit only by the sun and the
+// environment (no BVH, no emitters); the kernel then carries no surface code.
+// guided: some job samples a guide's tabulated proposal.
+hipError_t batch_blocks_per_cu(bool surfaces, bool guided, int* blocks);
 // init (clear films, queue, timestamps) + persistent kernel + per-job times
-hipError_t launch_batch(const BatchArgs& b, size_t film_words, uint32_t grid_blocks, double ticks_per_ms,
-                        hipStream_t st);
+hipError_t launch_batch(const BatchArgs& b, bool surfaces, bool guided, size_t film_words, uint32_t grid_blocks,
+                        double ticks_per_ms, hipStream_t st);
 hipError_t launch_fixed_to_double(const long long* in, double* out, size_t n, hipStream_t st);
 
 // ---- learned directional guide ------------------------------------------------
 // Hash-grid encoding (kGuideLevels x 2 features) + MLP 35 -> 64 x 4 (ReLU) -> 40
 // (8 vMF lobes x {logit, mean xyz, log kappa}), widths padded to multiples of
 // 16 for the 16x16x16 BF16 MFMA tiles: layer l maps kGuideIn[l] -> kGuideOut[l].
-constexpr int kGuideLevels = 16, kGuideFeat = 2, kGuideLobes = 8, kGuideLayers = 5;
+constexpr int kGuideLevels = 16, kGuideFeat = 2, kGuideLobes = kVmfLobes, kGuideLayers = 5;
 constexpr int kGuideEnc = kGuideLevels * kGuideFeat;  // 32
 constexpr int kGuideInputs = kGuideEnc + 3;           // 35: encoding + direction
 constexpr int kGuideOutputs = kGuideLobes * 5;        // 40
@@ -167,7 +181,8 @@ __host__ __device__ constexpr int guide_b_off(int l) {
 constexpr int kGuideParams = guide_b_off(kGuideLayers);                                   // + 304 biases
 
 struct GuideNet {
-  float* table;        // kGuideLevels * T * 2 hash features (FP32, read by the encoder)
+  float* table;        // kGuideLevels * T * 2 hash features (FP32 master, updated by Adam)
+  uint32_t* table_bf;  // kGuideLevels * T entries: both features as BF16 in one word (read by the encoder)
   float* table_grad;
   float* table_m;
   float* table_v;
@@ -219,11 +234,13 @@ struct WfGuide {
   int32_t max_depth;  // scattering vertices with depth < max_depth use the guide
   float prob;         // probability of sampling the proposal at a guided vertex
   float* mix;         // 40 mixture floats per path slot (this iteration's guided vertices)
+  GuideCache cache;   // cache.mix != nullptr: vertices read the tabulated proposal (no per-vertex network)
 };
 
 struct WfGuideHook {
   WfGuide g;
   GuideTrain train;
+  float* cache_mix;     // writable cache table (re-tabulated after every training step), or nullptr
   float lr_table, lr_mlp, beta1, beta2, eps;
   uint32_t min_batch;   // a training step runs once this many samples are queued
   uint64_t steps;       // Adam steps (updated by the launcher)
@@ -234,6 +251,8 @@ void wf_guide_carve(void* base, uint32_t P, WfGuide& g);
 
 hipError_t launch_guide_init(const GuideNet& g, uint64_t seed, hipStream_t st);
 hipError_t launch_guide_pack(const GuideNet& g, hipStream_t st);  // FP32 master -> BF16 operands
+// the proposal tabulated over the cache's cells and direction bins (into mix)
+hipError_t launch_guide_bake(const GuideNet& g, const GuideCache& c, float* mix, hipStream_t st);
 // outputs: n x 40 mixture parameters (weights, unit means, kappa)
 hipError_t launch_guide_eval(const GuideNet& g, int n, const float* in6, float* out40, hipStream_t st);
 // raw network outputs (n x 40) before the head, for parity tests

@@ -7,6 +7,7 @@
 #pragma once
 #include "ne_math.h"
 #include "ne_scene.h"
+#include "ne_vmf.h"
 
 namespace ne {
 
@@ -1009,12 +1010,15 @@ NE_DEV R emitter_hit_pdf(const DevScene& s, int type, uint32_t slot, Vec3<R> o, 
 
 // Shadow transmittance: surface occlusion, then ratio tracking through the
 // volume over [0, tmax].
-template <typename R, bool C>
-NE_DEV R transmittance(const DevScene& s, Vec3<R> o, Vec3<R> d, R tmax, PathRng& rng, uint32_t* lds, int stride,
-                       Count<C>& cnt) {
+// SURF = false: a scene without surfaces (no BVH): only the medium attenuates.
+template <typename R, bool C, bool SURF>
+NE_DEV R transmittance_t(const DevScene& s, Vec3<R> o, Vec3<R> d, R tmax, PathRng& rng, uint32_t* lds, int stride,
+                         Count<C>& cnt) {
   cnt.shadow();
-  SurfHit<R> h;
-  if (traverse<R, true, C>(s, o, d, R(1e-4), tmax, lds, stride, h, cnt)) return R(0);
+  if constexpr (SURF) {
+    SurfHit<R> h;
+    if (traverse<R, true, C>(s, o, d, R(1e-4), tmax, lds, stride, h, cnt)) return R(0);
+  }
   if (!s.has_volume) return R(1);
   Vec3<R> oo = xf_point<R>(s.inv, o), dd = xf_vec<R>(s.inv, d);
   R ta, tb;
@@ -1023,6 +1027,12 @@ NE_DEV R transmittance(const DevScene& s, Vec3<R> o, Vec3<R> d, R tmax, PathRng&
   tb = xmin(tb, tmax);
   if (ta >= tb) return R(1);
   return ratio_track<R, C>(s, o, d, ta, tb, rng, cnt);
+}
+
+template <typename R, bool C>
+NE_DEV R transmittance(const DevScene& s, Vec3<R> o, Vec3<R> d, R tmax, PathRng& rng, uint32_t* lds, int stride,
+                       Count<C>& cnt) {
+  return transmittance_t<R, C, true>(s, o, d, tmax, rng, lds, stride, cnt);
 }
 
 // ---------------------------------------------------------------------------
@@ -1053,10 +1063,18 @@ NE_DEV void path_init(PathState<R>& p, Vec3<R> o, Vec3<R> d) {
 // medium collision), emission / escape, and at a scatter vertex next-event
 // estimation plus the sampled continuation.  Returns true when the path has
 // ended (p.L is final).  trace_path loops over it; the persistent batch
-// kernel interleaves bounces of different paths in one wave.
-template <typename R, bool C>
+// kernel interleaves bounces of different paths in one wave.  SURF = false
+// compiles the surface code out, for scenes with neither surfaces nor
+// emitters (a volume lit by the sun and the environment): the same events,
+// arithmetic and draws, with fewer registers.
+// GUIDED (FP32): medium vertices at depth < gu.max_depth sample the one-sample
+// MIS mixture (1 - prob) HG + prob proposal, the proposal read from the
+// guide's tabulation (gu.cache; nullptr: plain HG), with the wavefront's draw
+// order (selector, two for the direction, one for the lobe), and the light
+// sample's MIS weight uses the mixture pdf.
+template <typename R, bool C, bool SURF = true, bool GUIDED = false>
 NE_DEV bool path_bounce(const DevScene& s, PathState<R>& p, int max_bounces, PathRng& rng, uint32_t* lds,
-                        int stride, Count<C>& cnt) {
+                        int stride, Count<C>& cnt, const GuideUse* gu = nullptr) {
   Vec3<R> o = p.o, d = p.d, L = p.L, beta = p.beta;
   const R prev_pdf = p.prev_pdf;
   const bool prev_delta = p.prev_delta;
@@ -1067,7 +1085,8 @@ NE_DEV bool path_bounce(const DevScene& s, PathState<R>& p, int max_bounces, Pat
   {
     cnt.ext();
     SurfHit<R> hit;
-    bool hs = traverse<R, false, C>(s, o, d, tmin, xinf<R>(), lds, stride, hit, cnt);
+    bool hs = false;
+    if constexpr (SURF) hs = traverse<R, false, C>(s, o, d, tmin, xinf<R>(), lds, stride, hit, cnt);
     R tS = hs ? hit.t : xinf<R>();
     bool medium = false;
     Vec3<R> x, n;
@@ -1091,18 +1110,21 @@ NE_DEV bool path_bounce(const DevScene& s, PathState<R>& p, int max_bounces, Pat
         p.L = L;
         return true;
       }
-      hit_normal<R>(s, hit, o, d, n, mat);
-      const float* m = s.mats + 12 * size_t(mat);
-      if (m[0] == 1.0f) {  // emitter
-        Vec3<R> Le = v3f<R>(m + 6);
-        R w = R(1);
-        if (!prev_delta) w = power_heuristic(prev_pdf, emitter_hit_pdf<R>(s, hit.type, hit.slot, o, d, hit.t, n));
-        L = L + beta * Le * w;
-        p.L = L;
-        return true;
+      if constexpr (SURF) {
+        hit_normal<R>(s, hit, o, d, n, mat);
+        const float* m = s.mats + 12 * size_t(mat);
+        if (m[0] == 1.0f) {  // emitter
+          Vec3<R> Le = v3f<R>(m + 6);
+          R w = R(1);
+          if (!prev_delta) w = power_heuristic(prev_pdf, emitter_hit_pdf<R>(s, hit.type, hit.slot, o, d, hit.t, n));
+          L = L + beta * Le * w;
+          p.L = L;
+          return true;
+        }
+        x = o + hit.t * d;
       }
-      x = o + hit.t * d;
     }
+    const bool med = !SURF || medium;  // without surfaces every vertex is a medium collision
     if (depth >= max_bounces) {
       p.L = L;
       return true;
@@ -1111,7 +1133,7 @@ NE_DEV bool path_bounce(const DevScene& s, PathState<R>& p, int max_bounces, Pat
     Vec3<R> wo = -d;
     Glossy<R> gm;
     Vec3<R> ns, ss, ts;
-    if (medium) {
+    if (med) {
       beta = beta * v3(R(s.sig_s[0]) / R(s.sig_t[0]), R(s.sig_s[1]) / R(s.sig_t[1]), R(s.sig_s[2]) / R(s.sig_t[2]));
     } else {
       const float* m = s.mats + 12 * size_t(mat);
@@ -1121,6 +1143,12 @@ NE_DEV bool path_bounce(const DevScene& s, PathState<R>& p, int max_bounces, Pat
       gm.metallic = R(m[5]);
       ns = dot(n, wo) < R(0) ? -n : n;
       onb(ns, ss, ts);
+    }
+    // the guided proposal of this vertex (tabulated entry), if any
+    const float* mix = nullptr;
+    if constexpr (GUIDED) {
+      if (med && gu->cache.mix && depth < gu->max_depth)
+        mix = guide_cache_at(gu->cache, xf_point<float>(s.inv, x) + v3s(0.5f), d);
     }
     R ul = R(rng.next()), ua = R(rng.next()), ub = R(rng.next());
     if (s.n_lights > 0) {
@@ -1137,14 +1165,14 @@ NE_DEV bool path_bounce(const DevScene& s, PathState<R>& p, int max_bounces, Pat
         delta = true;
       } else if (s.env_type != 0 && li == env_slot) {
         env_sample<R>(s, ua, ub, wi, pl, Le);
-      } else {
+      } else if constexpr (SURF) {
         const uint32_t code = s.emitters[li - env_slot - (s.env_type != 0 ? 1 : 0)];
         delta = sample_emitter<R>(s, code, x, ua, ub, wi, dist, pl, Le);
       }
       if (pl > R(0)) {
         Vec3<R> f;
         R pf;
-        if (medium) {
+        if (med) {
           R p = hg_p(g, dot(d, wi));
           f = v3s(p);
           pf = p;
@@ -1156,16 +1184,34 @@ NE_DEV bool path_bounce(const DevScene& s, PathState<R>& p, int max_bounces, Pat
         if (!is_black(f)) {
           R tmax = xisinf(dist) ? xinf<R>() : dist * R(1 - 1e-4);
           PathRng srng = rng.shadow_stream(depth);
-          R tr = transmittance<R, C>(s, x, wi, tmax, srng, lds, stride, cnt);
+          R tr = transmittance_t<R, C, SURF>(s, x, wi, tmax, srng, lds, stride, cnt);
           if (tr > R(0)) {
-            R w = delta ? R(1) : power_heuristic(pl, pf);
+            R pm = pf;
+            if constexpr (GUIDED) {
+              if (mix) pm = (1.0f - gu->prob) * pf + gu->prob * vmf_mixture_pdf(mix, wi);
+            }
+            R w = delta ? R(1) : power_heuristic(pl, pm);
             L = L + beta * f * Le * (tr * w * nl / pl);
           }
         }
       }
     }
     // ---- continue the path
-    if (medium) {
+    if (GUIDED && mix) {
+      if constexpr (GUIDED) {
+        const float us = rng.next(), u0 = rng.next(), u1 = rng.next(), u2 = rng.next();
+        const Vec3<float> wi = us < gu->prob ? vmf_mixture_sample(mix, u2, u0, u1) : hg_sample(g, d, u0, u1);
+        const float ph = hg_p(g, dot(d, wi));
+        const float pm = (1.0f - gu->prob) * ph + gu->prob * vmf_mixture_pdf(mix, wi);
+        if (!(pm > 0.0f)) {
+          p.L = L;
+          return true;
+        }
+        p.prev_pdf = pm;
+        p.d = wi;
+        p.beta = beta * (ph / pm);
+      }
+    } else if (med) {
       R u0 = R(rng.next()), u1 = R(rng.next());
       Vec3<R> wi = hg_sample(g, d, u0, u1);
       p.prev_pdf = hg_p(g, dot(d, wi));
@@ -1201,12 +1247,12 @@ NE_DEV bool path_bounce(const DevScene& s, PathState<R>& p, int max_bounces, Pat
 
 // One whole path (the oracle's Li event and draw order; camera jitter is drawn
 // by the caller).
-template <typename R, bool C>
+template <typename R, bool C, bool SURF = true, bool GUIDED = false>
 NE_DEV Vec3<R> trace_path(const DevScene& s, Vec3<R> o, Vec3<R> d, int max_bounces, PathRng& rng, uint32_t* lds,
-                          int stride, Count<C>& cnt) {
+                          int stride, Count<C>& cnt, const GuideUse* gu = nullptr) {
   PathState<R> p;
   path_init(p, o, d);
-  while (!path_bounce<R, C>(s, p, max_bounces, rng, lds, stride, cnt)) {
+  while (!path_bounce<R, C, SURF, GUIDED>(s, p, max_bounces, rng, lds, stride, cnt, gu)) {
   }
   return p.L;
 }

@@ -8,6 +8,8 @@
 // registers and written without atomics, so the result is bit-reproducible
 // for any launch shape: chunk partials are summed in chunk order by
 // k_reduce_partials.
+#include <type_traits>
+
 #include "ne_launch.h"
 #include "ne_trace.h"
 
@@ -25,7 +27,9 @@ __device__ __forceinline__ uint32_t xcd_logical_block(uint32_t nreal) {
   return lb;
 }
 
-template <typename R, bool C, int BLOCK, int WAVES = 0>
+// SURF = false: a volume lit only by the sun and the environment (no surface
+// code); GUIDED: medium vertices sample the guide's tabulated proposal.
+template <typename R, bool C, int BLOCK, int WAVES = 0, bool SURF = true, bool GUIDED = false>
 __global__ __launch_bounds__(BLOCK, WAVES) void k_render(DevScene s, DevCamera cam, RenderArgs a,
                                                           uint32_t nreal_blocks) {
   __shared__ uint32_t lds_stack[kShortStack * BLOCK];
@@ -53,7 +57,8 @@ __global__ __launch_bounds__(BLOCK, WAVES) void k_render(DevScene s, DevCamera c
     float ju = rng.next(), jv = rng.next();
     float u = (float(px) + ju) / float(a.W), v = (float(py) + jv) / float(a.H);
     Vec3<float> dir = -normalize(ll + u * hz + v * vt - pos);
-    Vec3<R> L = trace_path<R, C>(s, vcast<R>(pos), vcast<R>(dir), a.max_bounces, rng, lds, BLOCK, cnt);
+    Vec3<R> L = trace_path<R, C, SURF, GUIDED>(s, vcast<R>(pos), vcast<R>(dir), a.max_bounces, rng, lds, BLOCK, cnt,
+                                               &a.guide);
     acc0 += double(L.x); acc1 += double(L.y); acc2 += double(L.z);
   }
   if (a.nchunks == 1) {
@@ -135,7 +140,17 @@ template <typename R, bool C, int BLOCK, int WAVES = 0>
 static hipError_t launch_render_t(const DevScene& s, const DevCamera& cam, const RenderArgs& a, hipStream_t st) {
   uint32_t waves = uint32_t(a.ntx) * uint32_t(a.nty) * uint32_t(a.nchunks);
   uint32_t nreal = (waves + (BLOCK / 64) - 1) / (BLOCK / 64);
-  hipLaunchKernelGGL((k_render<R, C, BLOCK, WAVES>), dim3(pad8(nreal)), dim3(BLOCK), 0, st, s, cam, a, nreal);
+  const dim3 grid(pad8(nreal)), blk(BLOCK);
+  const bool surf = scene_has_surfaces(s);
+  if constexpr (std::is_same<R, float>::value) {
+    if (a.guide.cache.mix) {
+      if (surf) hipLaunchKernelGGL((k_render<R, C, BLOCK, WAVES, true, true>), grid, blk, 0, st, s, cam, a, nreal);
+      else hipLaunchKernelGGL((k_render<R, C, BLOCK, WAVES, false, true>), grid, blk, 0, st, s, cam, a, nreal);
+      return hipGetLastError();
+    }
+  }
+  if (surf) hipLaunchKernelGGL((k_render<R, C, BLOCK, WAVES, true, false>), grid, blk, 0, st, s, cam, a, nreal);
+  else hipLaunchKernelGGL((k_render<R, C, BLOCK, WAVES, false, false>), grid, blk, 0, st, s, cam, a, nreal);
   return hipGetLastError();
 }
 

@@ -59,6 +59,41 @@ __device__ __forceinline__ uint32_t block_compact(bool pred, uint32_t* wsum, uin
   return before + uint32_t(__popcll(mask & ((1ull << lane) - 1ull)));
 }
 
+// Spatial order of a chunk's rays: thread t of the block gets the chunk slot
+// of the t-th ray in Morton order of its origin (8 bits per axis of the
+// volume's unit cube, clamped), live rays (t < n) first.  The 23-bit key and
+// the slot share one word, sorted bitonically in LDS.  Every thread of the
+// block must call it.
+__device__ __forceinline__ uint32_t spread3(uint32_t v) {  // 8 bits -> every third bit
+  v = (v | (v << 8)) & 0x0F00Fu;
+  v = (v | (v << 4)) & 0x0C30C3u;
+  v = (v | (v << 2)) & 0x249249u;
+  return v;
+}
+__device__ __forceinline__ uint32_t chunk_spatial_slot(const DevScene& s, bool live, Vec3<float> o, uint32_t* skey) {
+  const uint32_t t = threadIdx.x;
+  uint32_t key = 0xFFFFFFu;  // free slots sort last
+  if (live) {
+    const Vec3<float> u = xf_point<float>(s.inv, o) + v3s(0.5f);
+    const uint32_t x = uint32_t(xmin(xmax(u.x, 0.0f), 0.999f) * 256.0f);
+    const uint32_t y = uint32_t(xmin(xmax(u.y, 0.0f), 0.999f) * 256.0f);
+    const uint32_t z = uint32_t(xmin(xmax(u.z, 0.0f), 0.999f) * 256.0f);
+    key = (spread3(x) | (spread3(y) << 1) | (spread3(z) << 2)) >> 1;  // 23 bits, below the free marker
+  }
+  skey[t] = (key << 8) | t;
+  __syncthreads();
+  for (uint32_t k = 2; k <= kChunk; k <<= 1)
+    for (uint32_t j = k >> 1; j > 0; j >>= 1) {
+      const uint32_t p = t ^ j;
+      if (p > t) {
+        const uint32_t x0 = skey[t], x1 = skey[p];
+        if ((x0 > x1) == ((t & k) == 0)) { skey[t] = x1; skey[p] = x0; }
+      }
+      __syncthreads();
+    }
+  return skey[t] & 0xFFu;
+}
+
 // Sums a per-thread counter over the block and adds it once to *dst.
 template <bool C>
 __device__ __forceinline__ void block_count(unsigned long long* dst, uint32_t v, uint32_t* red) {
@@ -354,13 +389,24 @@ __global__ __launch_bounds__(kChunk) void k_wf_trace(WfState w, WfArgs a, DevSce
 
 // Delta tracking through the volume up to the surface hit (or infinity).
 // HAS_SURF: the trace stage ran and set kind / hit_t.
-template <bool C, bool HAS_SURF>
+// SPATIAL: the chunk's rays are taken in Morton order of their origins, so a
+// wave tracks through neighbouring voxels (each path keeps its own stream:
+// same trajectories).
+template <bool C, bool HAS_SURF, bool SPATIAL>
 __global__ __launch_bounds__(kChunk) void k_wf_medium(WfState w, WfArgs a, DevScene s) {
   __shared__ uint32_t red[kChunk / 64];
+  __shared__ uint32_t skey[SPATIAL ? kChunk : 1];
   const WfPaths A = w.paths[a.cur];
-  const uint32_t c = blockIdx.x, t = threadIdx.x, i = c * kChunk + t;
+  const uint32_t c = blockIdx.x, t = threadIdx.x;
+  const uint32_t n = w.cnt_paths[a.cur][c];
+  uint32_t slot = t;
+  if constexpr (SPATIAL) {
+    const uint32_t q = c * kChunk + t;
+    slot = chunk_spatial_slot(s, t < n, t < n ? v3(A.ox[q], A.oy[q], A.oz[q]) : v3s(0.0f), skey);
+  }
+  const uint32_t i = c * kChunk + slot;
   Count<C> cnt;
-  if (t < w.cnt_paths[a.cur][c]) {
+  if (t < n) {
     const Vec3<float> o = v3(A.ox[i], A.oy[i], A.oz[i]), d = v3(A.dx[i], A.dy[i], A.dz[i]);
     const float tS = HAS_SURF ? A.hit_t[i] : xinf<float>();
     if (!HAS_SURF) cnt.ext();
@@ -444,6 +490,15 @@ __device__ __forceinline__ bool nee_sample(const DevScene& s, Vec3<float> x, Vec
   return true;
 }
 
+// Uniform in [0, 1) from (pixel, sample, vertex), independent of the path's
+// own stream (the training reservoir draws no path random numbers, so a
+// render samples the same paths with training on or off).
+__device__ __forceinline__ float reservoir_u01(uint32_t pix, uint32_t smp, uint32_t depth) {
+  uint32_t c[4] = {pix, smp, depth, 0x72657376u};
+  philox10(c, 0x5bd1e995u, 0x1b873593u);
+  return float(c[0] >> 8) * (1.0f / 16777216.0f);
+}
+
 // Guided mode: the learned proposal of every medium vertex that will be
 // guided (depth < max_depth, below the bounce limit), 16 queries per MFMA tile,
 // into gd.mix (40 floats per path slot).
@@ -495,9 +550,10 @@ __global__ __launch_bounds__(kChunk) void k_wf_guide(WfState w, WfArgs a, DevSce
 // Shading.  GUIDED: medium vertices at depth < max_depth sample a one-sample
 // MIS mixture of HG and the learned vMF proposal (probabilities 1 - prob,
 // prob) that k_wf_guide evaluated for them.  With
-// training on, the first guided vertex of a path is recorded and becomes a
-// training sample when the path ends: weight r = Li * phase / pdf, Li the
-// radiance gathered after the vertex divided by the throughput after it.
+// training on, one guided vertex of each path (chosen uniformly by a
+// reservoir) is recorded and becomes a training sample when the path ends:
+// weight r = m Li * phase / pdf, Li the radiance gathered after the vertex
+// divided by the throughput after it, m the path's number of guided vertices.
 //
 // SORT: before shading, the block orders its chunk's live paths by (event
 // kind, material) with a bitonic sort of 256 keys in LDS, and thread t shades
@@ -568,9 +624,9 @@ __global__ __launch_bounds__(kChunk) void k_wf_shade(WfState w, WfArgs a, DevSce
         float* const src[kGuideRecF] = {R.px, R.py, R.pz, R.dx, R.dy, R.dz, R.wx, R.wy,
                                         R.wz, R.ratio, R.bx, R.by, R.bz, R.lx, R.ly, R.lz};
         for (int k = 0; k < kGuideRecF; ++k) rec[k] = src[k][i];
-        if (rst == 1) {  // radiance baseline: everything gathered before this segment
+        if ((rst & 3u) == 1u) {  // radiance baseline: everything gathered before this segment
           rec[13] = lx; rec[14] = ly; rec[15] = lz;
-          rst = 2;
+          rst = (rst & ~3u) | 2u;
         }
       }
     }
@@ -630,7 +686,11 @@ __global__ __launch_bounds__(kChunk) void k_wf_shade(WfState w, WfArgs a, DevSce
     shadow = medium ? nee_sample<true>(s, x, d, gm, wo, ns, ss, ts, ul, ua, ub, wi, tmax, f, Le, wt, pl, delta)
                     : nee_sample<false>(s, x, d, gm, wo, ns, ss, ts, ul, ua, ub, wi, tmax, f, Le, wt, pl, delta);
     const bool guided = GUIDED && medium && int(depth) < gd.max_depth;
-    const float* mix = guided ? gd.mix + 40 * size_t(i) : nullptr;  // written by k_wf_guide
+    // the proposal: the tabulated entry of the vertex's cell and direction, or
+    // the network's output for this path that k_wf_guide wrote
+    const float* mix = nullptr;
+    if (guided)
+      mix = gd.cache.mix ? guide_cache_at(gd.cache, xf_point<float>(s.inv, x) + v3s(0.5f), d) : gd.mix + 40 * size_t(i);
     if (guided && shadow && !delta)  // the phase-sampling strategy is the mixture
       wt = power_heuristic(pl, (1.0f - gd.prob) * hg_p(s.g, dot(d, wi)) + gd.prob * vmf_mixture_pdf(mix, wi));
     if (shadow) f = beta * f * Le;  // NEE weight with the throughput before this vertex's BSDF weight
@@ -648,14 +708,22 @@ __global__ __launch_bounds__(kChunk) void k_wf_shade(WfState w, WfArgs a, DevSce
         ppdf = pm;
         beta = beta * (ph / pm);
         cont = true;
-        if (gd.train && rst == 0) {  // record the first guided vertex of the path
-          const Vec3<float> p = xf_point<float>(s.inv, x);
-          rec[0] = p.x + 0.5f; rec[1] = p.y + 0.5f; rec[2] = p.z + 0.5f;
-          rec[3] = d.x; rec[4] = d.y; rec[5] = d.z;
-          rec[6] = nd.x; rec[7] = nd.y; rec[8] = nd.z;
-          rec[9] = ph / pm;
-          rec[10] = beta.x; rec[11] = beta.y; rec[12] = beta.z;
-          rst = 1;
+        if (gd.train) {
+          // One record per path, a reservoir over its guided vertices: the
+          // m-th guided vertex replaces the record with probability 1/m, so
+          // every guided vertex is the recorded one with probability 1/m_end
+          // (the sample's weight is scaled by m_end when the path ends).
+          const uint32_t m = (rst >> 2) + 1u;
+          const bool take = m == 1u || reservoir_u01(pix, smp, depth) * float(m) < 1.0f;
+          if (take) {
+            const Vec3<float> p = xf_point<float>(s.inv, x);
+            rec[0] = p.x + 0.5f; rec[1] = p.y + 0.5f; rec[2] = p.z + 0.5f;
+            rec[3] = d.x; rec[4] = d.y; rec[5] = d.z;
+            rec[6] = nd.x; rec[7] = nd.y; rec[8] = nd.z;
+            rec[9] = ph / pm;
+            rec[10] = beta.x; rec[11] = beta.y; rec[12] = beta.z;
+          }
+          rst = (m << 2) | (take ? 1u : (rst & 3u));
         }
       }
     } else {
@@ -706,7 +774,7 @@ __global__ __launch_bounds__(kChunk) void k_wf_shade(WfState w, WfArgs a, DevSce
   if constexpr (GUIDED) {
     // a recording path that ends now becomes a training sample
     float r = 0.0f;
-    if (ends && !cont && rst == 2) {
+    if (ends && !cont && (rst & 3u) == 2u) {
       const float dl[3] = {lx - rec[13], ly - rec[14], lz - rec[15]};
       float li = 0.0f;
       int nc = 0;
@@ -715,7 +783,7 @@ __global__ __launch_bounds__(kChunk) void k_wf_shade(WfState w, WfArgs a, DevSce
           li += dl[q] / rec[10 + q];
           ++nc;
         }
-      r = nc ? li / float(nc) * rec[9] : 0.0f;
+      r = nc ? li / float(nc) * rec[9] * float(rst >> 2) : 0.0f;
       if (!(r > 0.0f) || xisinf(r)) r = 0.0f;
     }
     uint32_t nsmp;
@@ -821,15 +889,25 @@ __global__ __launch_bounds__(kChunk) void k_wf_shadow_surf(WfState w, WfArgs a, 
 
 // Shadow rays: any-hit occlusion (HAS_SURF) then ratio tracking (HAS_VOL),
 // the megakernel's transmittance() split by scene content.
-template <bool C, bool HAS_SURF, bool HAS_VOL>
+// SPATIAL: the chunk's shadow rays are taken in Morton order of their origins
+// (each keeps its own stream: same transmittances).
+template <bool C, bool HAS_SURF, bool HAS_VOL, bool SPATIAL>
 __global__ __launch_bounds__(kChunk) void k_wf_shadow(WfState w, WfArgs a, DevScene s) {
   __shared__ uint32_t lds_stack[HAS_SURF ? kShortStack * kChunk : 1];
   __shared__ uint32_t red[kChunk / 64];
+  __shared__ uint32_t skey[SPATIAL ? kChunk : 1];
   const WfShadow S = w.shadow;
   const WfPaths B = w.paths[a.cur ^ 1];
-  const uint32_t c = blockIdx.x, t = threadIdx.x, k = c * kChunk + t;
+  const uint32_t c = blockIdx.x, t = threadIdx.x;
+  const uint32_t n = w.cnt_shadow[c];
+  uint32_t slot = t;
+  if constexpr (SPATIAL) {
+    const uint32_t q = c * kChunk + t;
+    slot = chunk_spatial_slot(s, t < n, t < n ? v3(S.ox[q], S.oy[q], S.oz[q]) : v3s(0.0f), skey);
+  }
+  const uint32_t k = c * kChunk + slot;
   Count<C> cnt;
-  if (t < w.cnt_shadow[c]) {
+  if (t < n) {
     PathRng rng;
     rng.init(a.seed, S.pix[k], S.smp[k], a.job);
     PathRng srng = rng.shadow_stream(int(S.vert[k]));
@@ -978,14 +1056,17 @@ hipError_t launch_wavefront(const DevScene& s, const DevCamera& cam, WfState& w,
   const dim3 blk = dim3(unsigned(kChunk));
   if (!count) a.counters = nullptr;
   const bool hs = s.n_nodes > 0, hv = s.has_volume != 0;
-  const bool sort = a.sort != 0;
+  const bool sort = (a.sort & 1) != 0;
+  const bool spatial = (a.sort & 2) != 0;  // tracking stages in Morton order of the ray origins
   for (uint32_t it = 0;; ++it) {
     a.cur = int(it & 1u);
     NE_MARK(kStagePlan);
     hipLaunchKernelGGL(k_wf_plan, dim3(1), dim3(unsigned(kPlanThreads)), 0, st, w, a);
-    // every 8 iterations: stop once nothing is in flight (status written by
-    // plan); guided training: a step on the queued samples when enough are in
-    if ((it & 7u) == 7u) {
+    // every 8 iterations (every iteration while the guide trains): stop once
+    // nothing is in flight (status written by plan); guided training: a step
+    // on the queued samples when enough are in
+    const bool training = gh && gh->g.train;
+    if ((it & 7u) == 7u || (training && it > 0)) {
       if ((e = hipMemcpyAsync(h_status, w.status, sizeof(uint32_t), hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
       if (gh && gh->g.train &&
           (e = hipMemcpyAsync(h_status + 1, gh->g.batch.count, sizeof(uint32_t), hipMemcpyDeviceToHost, st)) !=
@@ -1001,6 +1082,8 @@ hipError_t launch_wavefront(const DevScene& s, const DevCamera& cam, WfState& w,
         if ((e = launch_guide_adam(gh->g.net, gh->lr_table, gh->lr_mlp, gh->beta1, gh->beta2, gh->eps,
                                    int(gh->steps), st)) != hipSuccess)
           return e;
+        if (gh->cache_mix && (e = launch_guide_bake(gh->g.net, gh->g.cache, gh->cache_mix, st)) != hipSuccess)
+          return e;
         hipLaunchKernelGGL(k_guide_reset_count, dim3(1), dim3(1), 0, st, gh->g.batch.count);
       }
     }
@@ -1013,19 +1096,21 @@ hipError_t launch_wavefront(const DevScene& s, const DevCamera& cam, WfState& w,
     }
     if (hv) {
       NE_MARK(kStageMedium);
-      if (count) {
-        if (hs) hipLaunchKernelGGL((k_wf_medium<true, true>), grid, blk, 0, st, w, a, s);
-        else hipLaunchKernelGGL((k_wf_medium<true, false>), grid, blk, 0, st, w, a, s);
+#define NE_MEDIUM(CNT, S_, SP) hipLaunchKernelGGL((k_wf_medium<CNT, S_, SP>), grid, blk, 0, st, w, a, s)
+      if (spatial) {
+        if (count) { if (hs) NE_MEDIUM(true, true, true); else NE_MEDIUM(true, false, true); }
+        else { if (hs) NE_MEDIUM(false, true, true); else NE_MEDIUM(false, false, true); }
       } else {
-        if (hs) hipLaunchKernelGGL((k_wf_medium<false, true>), grid, blk, 0, st, w, a, s);
-        else hipLaunchKernelGGL((k_wf_medium<false, false>), grid, blk, 0, st, w, a, s);
+        if (count) { if (hs) NE_MEDIUM(true, true, false); else NE_MEDIUM(true, false, false); }
+        else { if (hs) NE_MEDIUM(false, true, false); else NE_MEDIUM(false, false, false); }
       }
+#undef NE_MEDIUM
     }
     if (!hs && !hv) {
       NE_MARK(kStageOther);
       hipLaunchKernelGGL(k_wf_all_miss, grid, blk, 0, st, w, a);
     }
-    if (gh) {
+    if (gh && !gh->g.cache.mix) {  // per-vertex network evaluation (no tabulated proposal)
       NE_MARK(kStageGuide);
       hipLaunchKernelGGL(k_wf_guide, grid, blk, 0, st, w, a, s, gh->g);
     }
@@ -1035,7 +1120,11 @@ hipError_t launch_wavefront(const DevScene& s, const DevCamera& cam, WfState& w,
     else if (sort) hipLaunchKernelGGL((k_wf_shade<false, true>), grid, blk, 0, st, w, a, s, WfGuide{});
     else hipLaunchKernelGGL((k_wf_shade<false, false>), grid, blk, 0, st, w, a, s, WfGuide{});
     NE_MARK(kStageShadow);
-#define NE_SHADOW(CNT, S_, V_) hipLaunchKernelGGL((k_wf_shadow<CNT, S_, V_>), grid, blk, 0, st, w, a, s)
+#define NE_SHADOW(CNT, S_, V_)                                                             \
+  do {                                                                                     \
+    if (spatial) hipLaunchKernelGGL((k_wf_shadow<CNT, S_, V_, true>), grid, blk, 0, st, w, a, s); \
+    else hipLaunchKernelGGL((k_wf_shadow<CNT, S_, V_, false>), grid, blk, 0, st, w, a, s);      \
+  } while (0)
     if (hs && !hv) {
       if (count) hipLaunchKernelGGL(k_wf_shadow_surf<true>, pgrid, blk, 0, st, w, a, s);
       else hipLaunchKernelGGL(k_wf_shadow_surf<false>, pgrid, blk, 0, st, w, a, s);

@@ -6,7 +6,7 @@ import ctypes as C
 import numpy as np
 
 from .abi import (BatchConfig, Counters, GuideConfig, GuideStats, Job, NarvalError, RenderParams, SceneInfo, check, lib, NE_FP32,
-                  NE_FLAG_COUNT, NE_FLAG_TIME, NE_FLAG_STAGE_TIME, NE_FLAG_SORT, NE_FLAG_FIXED_FILM, STAGES)
+                  NE_FLAG_COUNT, NE_FLAG_TIME, NE_FLAG_STAGE_TIME, NE_FLAG_SORT, NE_FLAG_FIXED_FILM, NE_FLAG_SPATIAL, STAGES)
 
 
 def _fp(a: np.ndarray):
@@ -58,7 +58,7 @@ class Context:
                samples_per_thread: int = 0, block_size: int = 0, sync: bool = True, timed: bool = False,
                stage_timed: bool = False, guide=None, train: bool = False,
                occupancy: int = 0, mode: int = 0, pool_size: int = 0, sort: bool = False,
-               fixed_film: bool = False):
+               fixed_film: bool = False, spatial: bool = False):
         p = RenderParams()
         p.width, p.height = film.width, film.height
         x0, y0, x1, y1 = rect if rect is not None else (0, 0, film.width, film.height)
@@ -67,7 +67,7 @@ class Context:
         p.sample_offset, p.job, p.precision = sample_offset, job, precision
         p.flags = (NE_FLAG_COUNT if count else 0) | (NE_FLAG_TIME if timed else 0) | \
             (NE_FLAG_STAGE_TIME if stage_timed else 0) | (NE_FLAG_SORT if sort else 0) | \
-            (NE_FLAG_FIXED_FILM if fixed_film else 0)
+            (NE_FLAG_FIXED_FILM if fixed_film else 0) | (NE_FLAG_SPATIAL if spatial else 0)
         p.samples_per_thread, p.block_size, p.occupancy = samples_per_thread, block_size, occupancy
         p.mode, p.pool_size = mode, pool_size
         cam = camera.desc()
@@ -241,7 +241,9 @@ class Batch:
             a.camera = j["camera"].desc()
             a.width, a.height, a.spp, a.max_bounces = j["width"], j["height"], j["spp"], j["max_bounces"]
             a.seed, a.job = j.get("seed", 0), j.get("job", 0)
-        self._keep = [j["scene"] for j in jobs]  # scenes outlive the batch
+            g = j.get("guide")
+            a.guide = (g.h.value if isinstance(g.h, C.c_void_p) else g.h) if g is not None else None
+        self._keep = [j["scene"] for j in jobs] + [j["guide"] for j in jobs if j.get("guide") is not None]
         h = C.c_void_p()
         ctx.check(ctx._lib.ne_batch_create(ctx.h, arr, len(jobs), C.byref(h)))
         self.h = h
@@ -289,10 +291,10 @@ class Guide:
     def __init__(self, ctx: Context, log2_table: int = 19, base_res: int = 16, max_res: int = 512,
                  batch: int = 65536, lr_table: float = 1e-2, lr_mlp: float = 1e-3, beta1: float = 0.9,
                  beta2: float = 0.99, eps: float = 1e-8, seed: int = 0, guided_vertices: int = 1,
-                 guide_prob: float = 0.25):
+                 guide_prob: float = 0.25, cache_res: int = 0, cache_dirs: int = 4):
         self.ctx = ctx
         cfg = GuideConfig(log2_table, base_res, max_res, batch, lr_table, lr_mlp, beta1, beta2, eps, guided_vertices,
-                          guide_prob, seed)
+                          guide_prob, seed, cache_res, cache_dirs)
         h = C.c_void_p()
         ctx.check(ctx._lib.ne_guide_create(ctx.h, C.byref(cfg), C.byref(h)))
         self.h = h
@@ -331,6 +333,14 @@ class Guide:
         m = np.zeros(self.mlp_floats, np.float32)
         self.ctx.check(self.ctx._lib.ne_guide_get_grads(self.h, _fp(t), _fp(m)))
         return t, m
+
+    def cache(self) -> np.ndarray:
+        """The tabulated proposal: (cache_res^3 * cache_dirs^2, 40) mixture records."""
+        n = C.c_uint64()
+        self.ctx.check(self.ctx._lib.ne_guide_get_cache(self.h, None, C.byref(n)))
+        out = np.zeros((n.value, 40), np.float32)
+        self.ctx.check(self.ctx._lib.ne_guide_get_cache(self.h, _fp(out), C.byref(n)))
+        return out
 
     def stats(self) -> dict:
         s = GuideStats()

@@ -12,7 +12,8 @@ file to follow; its definition is:
     (x ^ y*2654435761 ^ z*805459861) mod T (32-bit arithmetic).
   MLP: x0 = [encode(p) (32), direction (3), 0 x 13]; h_l = x_l W_l + b_l,
     x_{l+1} = ReLU(h_l) for l < 4; outputs = h_4[:40].  Device arithmetic:
-    weights and layer inputs rounded to BF16, FP32 accumulation.
+    hash features, weights and layer inputs rounded to BF16, FP32
+    accumulation; gradients reach the FP32 masters straight through.
   head: 8 lobes of (logit, mean xyz, log kappa): pi = softmax(logits),
     mu = mean / |mean|, kappa = exp(clamp(log kappa, -10, 8)).
   vMF: p(w) = kappa / (2 pi (1 - exp(-2 kappa))) exp(kappa (mu.w - 1)).
@@ -86,7 +87,7 @@ def forward(queries, table, mlp, res_list, log2T, quantize=True):
     q = np.asarray(queries, np.float64)
     W, b = unpack(mlp)
     x = np.zeros((len(q), 48))
-    x[:, :32] = encode(q[:, :3], table, res_list, log2T)
+    x[:, :32] = encode(q[:, :3], bf16(table) if quantize else table, res_list, log2T)
     x[:, 32:35] = q[:, 3:6]
     if quantize:
         x = bf16(x)

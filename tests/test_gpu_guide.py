@@ -235,3 +235,129 @@ def test_trained_guide_gains_over_phase_sampling(ctx):
     assert ratio < 0.99, (plain, guided)
     g.close()
     gs.close()
+
+
+def _oct_centres(nd):
+    """Bin-centre directions of the octahedral map that include/ne_gpu.h
+    documents for ne_guide_get_cache, in FP32 with one rounding per operation
+    (the device's order), so the network sees bit-identical inputs."""
+    f = np.float32
+    b = np.arange(nd * nd)
+    inv = f(1) / f(nd)
+    u = ((b % nd).astype(f) + f(0.5)) * inv
+    v = ((b // nd).astype(f) + f(0.5)) * inv
+    px, py = f(2) * u - f(1), f(2) * v - f(1)
+    pz = (f(1) - np.abs(px)) - np.abs(py)
+    fold = pz < 0
+    ox = (f(1) - np.abs(py)) * np.where(px >= 0, f(1), f(-1))
+    oy = (f(1) - np.abs(px)) * np.where(py >= 0, f(1), f(-1))
+    px, py = np.where(fold, ox, px).astype(f), np.where(fold, oy, py).astype(f)
+    n = np.sqrt((px * px + py * py) + pz * pz).astype(f)
+    return np.stack([px / n, py / n, pz / n], 1).astype(f)
+
+
+def test_tabulated_proposal_is_the_network_at_entry_centres(ctx):
+    """Every tabulated entry equals the network's mixture at its cell centre and
+    direction-bin centre (the same encoder and MFMA tiles evaluate both)."""
+    print_case("guide tabulation vs network")
+    nc, nd = 5, 3
+    g = Guide(ctx, log2_table=LOG2T, batch=4096, seed=1, cache_res=nc, cache_dirs=nd)
+    table, mlp = _random_params(g, 8)
+    g.set_params(table, mlp)
+    tab = g.cache()
+    assert tab.shape == (nc ** 3 * nd * nd, 40)
+    e = np.arange(len(tab))
+    cell, b = e // (nd * nd), e % (nd * nd)
+    f = np.float32
+    inv = f(1) / f(nc)
+    p = np.stack([((cell % nc).astype(f) + f(0.5)) * inv, ((cell // nc % nc).astype(f) + f(0.5)) * inv,
+                  ((cell // (nc * nc)).astype(f) + f(0.5)) * inv], 1)
+    d = _oct_centres(nd)[b]
+    net = g.eval(np.concatenate([p, d], 1).astype(np.float32))
+    np.testing.assert_array_equal(tab, net)
+    # a parameter change re-tabulates on the next read
+    g.set_params(table * 0.5, mlp)
+    assert np.abs(g.cache() - tab).max() > 1e-4
+    g.close()
+
+
+def _trained_cached_guide(ctx, gs, cam, **kw):
+    g = Guide(ctx, log2_table=16, batch=16384, seed=4, cache_res=8, cache_dirs=4, **kw)
+    f = ctx.film(32, 32)
+    ctx.render(gs, cam, f, spp=128, max_bounces=64, seed=99, guide=g, train=True)
+    assert g.stats()["steps"] >= 1
+    f.close()
+    return g
+
+
+def test_guided_megakernel_traces_the_guided_wavefront_paths(ctx):
+    """With the tabulated proposal the megakernel and the wavefront make the same
+    draws and the same arithmetic at guided vertices: same image."""
+    print_case("guided megakernel vs guided wavefront")
+    from narvalengine_amd.abi import NE_MODE_MEGAKERNEL, NE_MODE_WAVEFRONT
+    s, cam = _cloud()
+    gs = ctx.upload(s)
+    g = _trained_cached_guide(ctx, gs, cam, guided_vertices=4, guide_prob=0.5)
+    out = []
+    for mode in (NE_MODE_WAVEFRONT, NE_MODE_MEGAKERNEL):
+        f = ctx.film(40, 24)
+        ctx.render(gs, cam, f, spp=16, max_bounces=64, seed=13, guide=g, mode=mode)
+        out.append(f.read())
+        f.close()
+    assert out[0].any()
+    np.testing.assert_allclose(out[1], out[0], rtol=1e-6, atol=1e-9)
+    g.close()
+
+
+def test_guided_batch_job_equals_its_single_job_render(ctx):
+    """Guided + batched: a guided job in a batch with plain jobs is bitwise the
+    guided job batched alone, and equals the guided megakernel render of it
+    up to the film's fixed-point quantisation."""
+    print_case("guided batch job vs single-job render")
+    from narvalengine_amd import Batch
+    from narvalengine_amd.abi import NE_MODE_MEGAKERNEL
+    s, cam = _cloud()
+    gs = ctx.upload(s)
+    g = _trained_cached_guide(ctx, gs, cam, guided_vertices=4, guide_prob=0.5)
+    import scenes
+    other = ctx.upload(scenes.small_volume(res=16, maj_res=2))
+    jobs = [dict(scene=other, camera=scenes.VOLUME_CAMERA, width=24, height=16, spp=8, max_bounces=16, seed=3, job=1),
+            dict(scene=gs, camera=cam, width=40, height=24, spp=12, max_bounces=64, seed=5, job=2, guide=g),
+            dict(scene=other, camera=scenes.VOLUME_CAMERA, width=16, height=16, spp=4, max_bounces=16, seed=4, job=3)]
+    b = Batch(ctx, jobs)
+    b.run((8, 8, 4))
+    got = b.read(1)
+    alone = Batch(ctx, [jobs[1]])
+    alone.run((16, 4, 2))
+    np.testing.assert_array_equal(alone.read(0), got)
+    f = ctx.film(40, 24)
+    ctx.render(gs, cam, f, spp=12, max_bounces=64, seed=5, job=2, guide=g, mode=NE_MODE_MEGAKERNEL)
+    np.testing.assert_allclose(got, f.read(), rtol=1e-6, atol=12 * 2.0 ** -30 * 4)
+    # the plain jobs are untouched by the guided one
+    plain = Batch(ctx, [jobs[0]])
+    plain.run((8, 8, 4))
+    np.testing.assert_array_equal(plain.read(0), b.read(0))
+    for x in (b, alone, plain):
+        x.close()
+    g.close()
+
+
+def test_tabulated_guided_render_is_unbiased(ctx):
+    """The tabulated proposal leaves the expectation unchanged, as the
+    per-vertex network does (the first four vertices of every path guided)."""
+    print_case("tabulated guided render unbiased")
+    s, cam = _cloud()
+    gs = ctx.upload(s)
+    g = _trained_cached_guide(ctx, gs, cam, guided_vertices=4, guide_prob=0.5)
+    W = H = 32
+    spp = 256
+    fp, fg = ctx.film(W, H), ctx.film(W, H)
+    ctx.render(gs, cam, fp, spp=spp, max_bounces=64, seed=1)
+    ctx.render(gs, cam, fg, spp=spp, max_bounces=64, seed=2, guide=g)
+    a = fp.read().mean(2) / spp
+    b = fg.read().mean(2) / spp
+    dlt = (b - a).reshape(-1)
+    z = dlt.mean() / (dlt.std() / np.sqrt(dlt.size))
+    assert abs(z) < 5.0, (z, a.mean(), b.mean())
+    assert abs(b.mean() - a.mean()) < 0.03 * a.mean()
+    g.close()
