@@ -247,6 +247,19 @@ ne_status ne_render(ne_ctx* ctx, const ne_scene* scene, const ne_camera_desc* ca
  * outside [0, 400) and spp <= 0 are argument errors. */
 ne_status ne_render_tile(ne_ctx* ctx, const ne_scene* scene, const ne_camera_desc* cam,
                          const ne_render_params* params, int32_t index, ne_film* film, float* pixels);
+/* ne_render without any device allocation.  No counterpart in the reference:
+ * it is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc).  It runs the
+ * megakernel pipeline only (params->mode AUTO or MEGAKERNEL, FP32 or FP64;
+ * no NE_FLAG_COUNT / NE_FLAG_STAGE_TIME) and gives the same film sums as
+ * ne_render with NE_MODE_MEGAKERNEL and the same params.  When the samples
+ * of a pixel are split into chunks (spp > samples per thread), the chunk
+ * partial sums, (x1-x0)*(y1-y0)*chunks*3 doubles, live in a static device
+ * array of 32 MiB per GPU, held until the process ends; larger requests are
+ * argument errors.  Every call on a GPU shares that array: calls that overlap
+ * in time on one GPU (on different contexts) race. */
+ne_status ne_render_noalloc(ne_ctx* ctx, const ne_scene* scene, const ne_camera_desc* cam,
+                            const ne_render_params* params, ne_film* film);
 /* Duration in milliseconds of the path-tracing kernel of the last render
  * issued with NE_FLAG_TIME (waits for it to finish). */
 ne_status ne_get_kernel_time(ne_ctx* ctx, float* ms);

@@ -135,6 +135,8 @@ SIGNATURES = [
     ("ne_film_read", C.c_int32, [C.c_void_p, _D]),
     ("ne_film_tonemap", C.c_int32, [C.c_void_p, C.c_int32, _F]),
     ("ne_render", C.c_int32, [C.c_void_p, C.c_void_p, C.POINTER(CameraDesc), C.POINTER(RenderParams), C.c_void_p]),
+    ("ne_render_noalloc", C.c_int32, [C.c_void_p, C.c_void_p, C.POINTER(CameraDesc), C.POINTER(RenderParams),
+                                      C.c_void_p]),
     ("ne_render_tile", C.c_int32, [C.c_void_p, C.c_void_p, C.POINTER(CameraDesc), C.POINTER(RenderParams),
                                    C.c_int32, C.c_void_p, _F]),
     ("ne_get_kernel_time", C.c_int32, [C.c_void_p, _F]),

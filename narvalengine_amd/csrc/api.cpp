@@ -46,6 +46,7 @@ struct ne_ctx {
   uint32_t wfg_P = 0;
   void* sbuf = nullptr;         // wavefront per-sample radiance (grown on demand)
   size_t sbuf_bytes = 0;
+  double* scratch = nullptr;    // this device's copy of the static scratch array (ne_render_noalloc)
 };
 
 struct ne_scene {
@@ -83,9 +84,36 @@ struct ne_guide {
 static ne_status guide_refresh_cache(ne_ctx* c, ne_guide* g);
 static GuideUse guide_use(const ne_guide* g);
 
+// ---- static device scratch ------------------------------------------------------
+// The allocation-free render (ne_render_noalloc) keeps its megakernel's
+// per-chunk partial sums in this fixed array of the code object: each device
+// holds its own copy until the process ends.  In host code `scratch` names a
+// host-side stand-in, never a device address: kernels, copies and fills take
+// the per-device address that hipGetSymbolAddress returns (resolved once per
+// device, with that device current, when a context is created on it).
+constexpr size_t kStaticScratchBytes = size_t(32) << 20;
+static __device__ __align__(256) unsigned char scratch[kStaticScratchBytes];
+
 namespace {
 
 thread_local std::string g_err;
+
+std::mutex g_scratch_mu;
+std::vector<double*> g_scratch_dev;  // per device index; nullptr until resolved
+
+// The calling thread's current device must be `device`.
+hipError_t resolve_scratch(int device, double** out) {
+  std::lock_guard<std::mutex> lk(g_scratch_mu);
+  if (size_t(device) >= g_scratch_dev.size()) g_scratch_dev.resize(size_t(device) + 1, nullptr);
+  if (!g_scratch_dev[size_t(device)]) {
+    void* p = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return e;
+    g_scratch_dev[size_t(device)] = static_cast<double*>(p);
+  }
+  *out = g_scratch_dev[size_t(device)];
+  return hipSuccess;
+}
 
 ne_status fail(ne_ctx* c, ne_status st, const std::string& msg) {
   if (c) c->err = msg; else g_err = msg;
@@ -427,6 +455,10 @@ ne_status ne_ctx_create(int32_t device, ne_ctx** out) {
       hipEventCreate(&c->ev1) != hipSuccess) {
     delete c;
     return fail(nullptr, NE_ERR_HIP, "context setup failed");
+  }
+  if (resolve_scratch(device, &c->scratch) != hipSuccess) {
+    ctx_free(c);
+    return fail(nullptr, NE_ERR_HIP, "static scratch address lookup failed");
   }
   *out = c;
   return NE_OK;
@@ -962,6 +994,42 @@ ne_status ne_render(ne_ctx* c, const ne_scene* s, const ne_camera_desc* cam, con
   if (!c) return fail(nullptr, NE_ERR_INVALID_ARG, "ctx is NULL");
   std::lock_guard<std::mutex> lk(c->mu);
   return render_locked(c, s, cam, p, f);
+}
+
+ne_status ne_render_noalloc(ne_ctx* c, const ne_scene* s, const ne_camera_desc* cam, const ne_render_params* p,
+                           ne_film* f) {
+  if (!c) return fail(nullptr, NE_ERR_INVALID_ARG, "ctx is NULL");
+  std::lock_guard<std::mutex> lk(c->mu);
+  ne_status st = check_render(c, s, cam, p, f);
+  if (st != NE_OK) return st;
+  if (p->mode == NE_MODE_WAVEFRONT) return fail(c, NE_ERR_INVALID_ARG, "ne_render_noalloc runs the megakernel only");
+  if (p->flags & (NE_FLAG_COUNT | NE_FLAG_STAGE_TIME))
+    return fail(c, NE_ERR_INVALID_ARG, "ne_render_noalloc does not count events or time stages");
+  RenderArgs a = megakernel_args(c, p);
+  const size_t npix = size_t(p->x1 - p->x0) * size_t(p->y1 - p->y0);
+  if (a.nchunks == 1) {
+    a.out = f->d_sum;  // each thread adds its pixel's sums into the film: no scratch
+  } else {
+    const size_t need = npix * size_t(a.nchunks) * 3 * sizeof(double);
+    if (need > kStaticScratchBytes)
+      return fail(c, NE_ERR_INVALID_ARG,
+                  "partial sums need " + std::to_string(need) + " bytes, the static scratch holds " +
+                      std::to_string(kStaticScratchBytes) + " (render fewer pixels or raise samples_per_thread)");
+    // every (chunk, pixel) slot the reduction reads is written by the render
+    // first, so the previous call's contents need no clearing
+    a.out = c->scratch;
+  }
+  NE_HIP(c, hipSetDevice(c->device));
+  const bool timed = (p->flags & NE_FLAG_TIME) != 0;
+  if (timed) NE_HIP(c, hipEventRecord(c->ev0, c->stream));
+  NE_HIP(c, launch_render(s->dev, make_camera(cam), a, p->precision, false, p->block_size ? p->block_size : 128,
+                          p->occupancy, c->stream));
+  if (timed) {
+    NE_HIP(c, hipEventRecord(c->ev1, c->stream));
+    c->timed = true;
+  }
+  if (a.nchunks > 1) NE_HIP(c, launch_reduce_partials(a, f->d_sum, c->stream));
+  return NE_OK;
 }
 
 ne_status ne_render_tile(ne_ctx* c, const ne_scene* s, const ne_camera_desc* cam, const ne_render_params* p,

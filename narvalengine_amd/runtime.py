@@ -58,7 +58,7 @@ class Context:
                samples_per_thread: int = 0, block_size: int = 0, sync: bool = True, timed: bool = False,
                stage_timed: bool = False, guide=None, train: bool = False,
                occupancy: int = 0, mode: int = 0, pool_size: int = 0, sort: bool = False,
-               fixed_film: bool = False, spatial: bool = False):
+               fixed_film: bool = False, noalloc: bool = False, spatial: bool = False):
         p = RenderParams()
         p.width, p.height = film.width, film.height
         x0, y0, x1, y1 = rect if rect is not None else (0, 0, film.width, film.height)
@@ -71,7 +71,9 @@ class Context:
         p.samples_per_thread, p.block_size, p.occupancy = samples_per_thread, block_size, occupancy
         p.mode, p.pool_size = mode, pool_size
         cam = camera.desc()
-        if guide is not None:
+        if noalloc:  # megakernel with its partial sums in the static device scratch
+            self.check(self._lib.ne_render_noalloc(self.h, gscene.h, C.byref(cam), C.byref(p), film.h))
+        elif guide is not None:
             self.check(self._lib.ne_render_guided(self.h, gscene.h, C.byref(cam), C.byref(p), film.h, guide.h,
                                                   1 if train else 0))
         else:
