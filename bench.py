@@ -296,26 +296,29 @@ def probe_pixels(W: int, H: int, side: int = 16) -> np.ndarray:
     return np.array([(x, y) for y in ys for x in xs], np.int32)
 
 
-def converged_reference(ctx, g, wl, passes: int, seed: int):
-    """FP64 megakernel render at passes x spp, in passes of the workload's spp.
-    Returns the mean image, the variance of that mean per pixel and channel
-    (from the spread of the passes) and the seconds taken."""
+def converged_reference(ctx, g, wl, spp: int, passes: int, seed: int):
+    """FP64 megakernel render at spp samples per pixel, in `passes` launches
+    of spp / passes.  Returns the mean image, the variance of that mean per
+    pixel and channel (from the spread of the passes, passes - 1 degrees of
+    freedom) and the seconds taken."""
     from narvalengine_amd.abi import NE_FP64, NE_MODE_MEGAKERNEL
     W, H = wl.width, wl.height
+    per = spp // passes
     f = ctx.film(W, H)
     s1 = np.zeros((H, W, 3))
     s2 = np.zeros((H, W, 3))
     prev = np.zeros((H, W, 3))
     t = time.perf_counter()
     for k in range(passes):
-        ctx.render(g, wl.camera, f, spp=wl.spp, sample_offset=k * wl.spp, max_bounces=wl.max_bounces, seed=seed,
+        ctx.render(g, wl.camera, f, spp=per, sample_offset=k * per, max_bounces=wl.max_bounces, seed=seed,
                    precision=NE_FP64, mode=NE_MODE_MEGAKERNEL)
         cur = f.read()
-        img = (cur - prev) / wl.spp
+        img = (cur - prev) / per
         prev = cur
         s1 += img
         s2 += img * img
-        log(f"FP64 reference pass {k + 1}/{passes}")
+        if (k + 1) % 8 == 0 or k + 1 == passes:
+            log(f"FP64 reference pass {k + 1}/{passes}")
     secs = time.perf_counter() - t
     f.close()
     mean = s1 / passes
@@ -323,11 +326,14 @@ def converged_reference(ctx, g, wl, passes: int, seed: int):
     return mean, var_mean, secs
 
 
-def probe_check(wl, ref, ref_var, spp: int, threads: int, seed: int) -> dict:
+def probe_check(wl, ref, ref_var, ref_dof: int, spp: int, threads: int, seed: int) -> dict:
     """Accepts the GPU FP64 reference only where the CPU oracle agrees: the
     oracle renders the probe pixels in FP64 at spp samples each (independent
-    streams), and every probe and channel must lie within 5 standard errors of
-    the reference (both estimates' variances from their own samples)."""
+    streams), and every probe and channel must agree with the reference within
+    the Bonferroni bound of a 1% family-wise test: |z| below the Student-t
+    quantile 1 - 0.01 / (2 n) with the reference's degrees of freedom (both
+    estimates' variances from their own samples)."""
+    from scipy.stats import t as student_t
     sys.path.insert(0, str(ROOT / "tests"))
     import oracle as O
     W, H = wl.width, wl.height
@@ -341,11 +347,13 @@ def probe_check(wl, ref, ref_var, spp: int, threads: int, seed: int) -> dict:
     r = ref[xy[:, 1], xy[:, 0]]
     rv = ref_var[xy[:, 1], xy[:, 0]]
     z = (r - m) / np.sqrt(np.maximum(var_m + rv, 1e-30))
+    bound = float(student_t.ppf(1.0 - 0.01 / (2 * z.size), ref_dof))
     return {"probes": int(len(xy)), "spp": spp, "threads": threads, "seconds": round(secs, 2),
-            "max_abs_z": round(float(np.abs(z).max()), 3), "mean_z2": round(float(np.mean(z * z)), 3),
+            "max_abs_z": round(float(np.abs(z).max()), 3), "z_bound": round(bound, 3),
+            "mean_z2": round(float(np.mean(z * z)), 3),
             "mean_ratio_reference_to_cpu": round(float(r.sum() / max(m.sum(), 1e-30)), 5),
             "rel_stderr_cpu": round(float(np.sqrt(var_m.mean()) / max(m.mean(), 1e-30)), 6),
-            "accepted": bool(np.abs(z).max() < 5.0)}
+            "accepted": bool(np.abs(z).max() < bound)}
 
 
 def relmse_parts(img, ref, ref_var, eps=1e-2):
@@ -359,140 +367,125 @@ def relmse_parts(img, ref, ref_var, eps=1e-2):
 def equal_time_relmse(ctx, g, wl, args, threads: int) -> dict:
     """Equal-time relMSE of guided against plain rendering on the full frame
     of the workload, against an FP64 converged render that the CPU probes
-    accept.  Plain: the FP32 wavefront at the workload's spp.  Guided: a fresh
-    guide, trained online while it renders, then frozen for the rest of the
-    same wall time (training included); both phases' samples count."""
-    from narvalengine_amd import Guide
+    accept.  For each budget of K frames: plain is the FP32 wavefront at K x
+    the workload's spp, and sets the wall time.  Guided: a fresh guide trains
+    online while it renders args.train_spp, then the frozen, tabulated guide
+    renders the rest of the same wall time (training included; both phases'
+    samples count), through the wavefront (guided) and through the persistent
+    batch kernel (guided+batched).  The batch kernel's plain frame is timed
+    beside them."""
     W, H, spp = wl.width, wl.height, wl.spp
-    ref, ref_var, ref_s = converged_reference(ctx, g, wl, args.ref_passes, seed=1000)
-    log(f"FP64 reference: {args.ref_passes * spp} spp in {ref_s:.1f} s")
-    probes = probe_check(wl, ref, ref_var, args.probe_spp, threads, seed=2000)
+    ref_spp = args.ref_mult * spp
+    ref, ref_var, ref_s = converged_reference(ctx, g, wl, ref_spp, args.ref_passes, seed=1000)
+    log(f"FP64 reference: {ref_spp} spp in {ref_s:.1f} s")
+    probes = probe_check(wl, ref, ref_var, args.ref_passes - 1, args.probe_spp, threads, seed=2000)
     log(f"CPU probes: {probes}")
-    kw = dict(max_bounces=wl.max_bounces)
-
-    def render(f, n, seed, offset=0, **extra):
-        ctx.synchronize()
-        t = time.perf_counter()
-        ctx.render(g, wl.camera, f, spp=n, seed=seed, sample_offset=offset, **kw, **extra)
-        return time.perf_counter() - t
-
     f = ctx.film(W, H)
-    render(f, spp, 7)  # warm
-    f.clear()
-    t_plain = render(f, spp, 11)
-    plain = f.read() / spp
-    e_plain, e_plain_c = relmse_parts(plain, ref, ref_var)
-    log(f"plain: {spp} spp in {t_plain * 1e3:.1f} ms, relMSE {e_plain:.4e} ({e_plain_c:.4e} without reference noise)")
-    out = {"frame": [W, H], "budget_ms": round(t_plain * 1e3, 2),
-           "reference": {"kind": "FP64 megakernel", "spp": args.ref_passes * spp, "seconds": round(ref_s, 1),
-                         "relmse_noise": float(np.mean(ref_var / (ref ** 2 + 1e-2))), "cpu_probes": probes},
-           "plain": {"spp": spp, "ms": round(t_plain * 1e3, 2), "relmse": e_plain, "relmse_minus_ref_noise": e_plain_c},
-           "guided": []}
-    for prob in args.guide_probs:
-        gkw = dict(guided_vertices=args.guided_vertices, guide_prob=prob, cache_res=args.guide_cache,
-                   cache_dirs=args.guide_cache_dirs)
-        # the frozen guide's time per sample, from a pilot guide (not timed into the run)
-        pilot = Guide(ctx, seed=21, **gkw)
-        render(f, 16, 3, guide=pilot, train=False)
-        t16 = render(f, 16, 4, guide=pilot, train=False)
-        pilot.close()
-        guide = Guide(ctx, seed=22, **gkw)
-        f.clear()
-        ctx.synchronize()
-        t0 = time.perf_counter()
-        ctx.render(g, wl.camera, f, spp=args.train_spp, seed=13, guide=guide, train=True, **kw)
-        ctx.synchronize()
-        t_train = time.perf_counter() - t0
-        rest = max(0, int((t_plain - t_train) / (t16 / 16)))
-        if rest:
-            ctx.render(g, wl.camera, f, spp=rest, seed=13, sample_offset=args.train_spp, guide=guide, train=False, **kw)
-        ctx.synchronize()
-        t_g = time.perf_counter() - t0
-        n = args.train_spp + rest
-        img = f.read() / n
-        e, e_c = relmse_parts(img, ref, ref_var)
-        st = guide.stats()
-        guide.close()
-        rec = {"guide_prob": prob, "guided_vertices": args.guided_vertices, "cache_res": args.guide_cache,
-               "cache_dirs": args.guide_cache_dirs, "train_spp": args.train_spp,
-               "frozen_spp": rest, "spp": n, "ms": round(t_g * 1e3, 2), "train_ms": round(t_train * 1e3, 2),
-               "train_steps": st["steps"], "relmse": e, "relmse_minus_ref_noise": e_c,
-               "ratio_to_plain": round(e_c / e_plain_c, 4),
-               # relMSE x time: the ratio at exactly equal time for an unbiased estimator
-               "ratio_to_plain_time_normalised": round(e_c * t_g / (e_plain_c * t_plain), 4),
-               "ratio_at_equal_spp": round(e_c * n / (e_plain_c * spp), 4)}
-        out["guided"].append(rec)
-        log(f"guided p={prob}: {rec}")
-    if args.batched:
-        out["batched"] = batched_legs(ctx, g, wl, args, ref, ref_var, t_plain, e_plain_c)
+    ctx.render(g, wl.camera, f, spp=spp, seed=7, max_bounces=wl.max_bounces)  # warm
+    out = {"frame": [W, H], "target": "guided+batched relMSE <= 0.5 x plain at equal time",
+           "reference": {"kind": "FP64 megakernel", "spp": ref_spp, "passes": args.ref_passes,
+                         "seconds": round(ref_s, 1), "relmse_noise": float(np.mean(ref_var / (ref ** 2 + 1e-2))),
+                         "cpu_probes": probes},
+           "budgets": []}
+    for K in args.budgets:
+        out["budgets"].append(equal_time_budget(ctx, g, wl, args, ref, ref_var, f, K))
     f.close()
+    gb = [r["ratio_to_plain_time_normalised"] for b in out["budgets"] for r in b.get("guided_batched", [])]
+    out["best_guided_batched_ratio"] = min(gb) if gb else None
     return out
 
 
-def batched_legs(ctx, g, wl, args, ref, ref_var, t_plain, e_plain_c):
-    """The frame through the persistent batch kernel (one job, one graph
-    launch): plain, and guided+batched at the plain wavefront's wall time
-    (a wavefront render trains the guide online for args.train_spp, then one
-    batch launch renders the rest with the frozen, tabulated proposal)."""
+def equal_time_budget(ctx, g, wl, args, ref, ref_var, f, K: int) -> dict:
     from narvalengine_amd import Batch, Guide
     W, H, spp = wl.width, wl.height, wl.spp
-    job = dict(scene=g, camera=wl.camera, width=W, height=H, spp=spp, max_bounces=wl.max_bounces, seed=17, job=0)
-    b = Batch(ctx, [job])
-    cfg, _ = b.autotune(spp_divisor=16)
-    b.run(cfg)
-    ms = b.run(cfg)
-    e, e_c = relmse_parts(b.read(0) / spp, ref, ref_var)
-    b.close()
-    legs = {"plain_batched": {"spp": spp, "ms": round(ms, 2), "item_shape": list(cfg), "relmse": e,
-                              "relmse_minus_ref_noise": e_c, "samples_per_s": round(W * H * spp / (ms * 1e-3), 1)}}
-    log(f"plain batched: {legs['plain_batched']}")
-    out = []
+    kw = dict(max_bounces=wl.max_bounces)
+    n_plain = K * spp
+    f.clear()
+    ctx.synchronize()
+    t0 = time.perf_counter()
+    ctx.render(g, wl.camera, f, spp=n_plain, seed=11, **kw)
+    ctx.synchronize()
+    t_plain = time.perf_counter() - t0
+    e_plain, e_plain_c = relmse_parts(f.read() / n_plain, ref, ref_var)
+    log(f"budget {K} frames: plain {n_plain} spp in {t_plain * 1e3:.1f} ms, relMSE {e_plain:.4e}")
+    res = {"frames": K, "budget_ms": round(t_plain * 1e3, 2),
+           "plain": {"spp": n_plain, "ms": round(t_plain * 1e3, 2), "relmse": e_plain,
+                     "relmse_minus_ref_noise": e_plain_c},
+           "guided": [], "guided_batched": []}
+
+    def score(name, prob, n, t_all, t_train, steps, img, extra):
+        e, e_c = relmse_parts(img, ref, ref_var)
+        rec = {"guide_prob": prob, "guided_vertices": args.guided_vertices, "cache_res": args.guide_cache,
+               "cache_dirs": args.guide_cache_dirs, "train_spp": args.train_spp, "spp": n,
+               "ms": round(t_all * 1e3, 2), "train_ms": round(t_train * 1e3, 2), "train_steps": steps,
+               "relmse": e, "relmse_minus_ref_noise": e_c, "ratio_to_plain": round(e_c / e_plain_c, 4),
+               # relMSE x time: the ratio at exactly equal time for an unbiased estimator
+               "ratio_to_plain_time_normalised": round(e_c * t_all / (e_plain_c * t_plain), 4),
+               "ratio_at_equal_spp": round(e_c * n / (e_plain_c * n_plain), 4), **extra}
+        log(f"budget {K}, {name} p={prob}: {rec}")
+        return rec
+
+    # the batch kernel's plain frame
+    job = dict(scene=g, camera=wl.camera, width=W, height=H, spp=n_plain, max_bounces=wl.max_bounces, seed=17, job=0)
+    if args.batched:
+        b = Batch(ctx, [job])
+        cfg, _ = b.autotune(spp_divisor=16 * K)
+        b.run(cfg)
+        ms = b.run(cfg)
+        e, e_c = relmse_parts(b.read(0) / n_plain, ref, ref_var)
+        b.close()
+        res["plain_batched"] = {"spp": n_plain, "ms": round(ms, 2), "item_shape": list(cfg), "relmse": e,
+                                "relmse_minus_ref_noise": e_c,
+                                "samples_per_s": round(W * H * n_plain / (ms * 1e-3), 1)}
+        log(f"budget {K}: plain batched {res['plain_batched']}")
     for prob in args.guide_probs:
         gkw = dict(guided_vertices=args.guided_vertices, guide_prob=prob, cache_res=args.guide_cache,
                    cache_dirs=args.guide_cache_dirs)
+        # time per sample of the frozen guide in each form, from a pilot guide
         pilot = Guide(ctx, seed=21, **gkw)
-        pj = dict(job, spp=16, guide=pilot)
-        pb = Batch(ctx, [pj])
-        pb.run(cfg)
-        t16 = pb.run(cfg) * 1e-3
-        pb.close()
-        pilot.close()
-        guide = Guide(ctx, seed=22, **gkw)
-        f = ctx.film(W, H)
+        ctx.render(g, wl.camera, f, spp=16, seed=3, guide=pilot, **kw)
         ctx.synchronize()
         t0 = time.perf_counter()
-        ctx.render(g, wl.camera, f, spp=args.train_spp, seed=13, guide=guide, train=True, max_bounces=wl.max_bounces)
+        ctx.render(g, wl.camera, f, spp=64, seed=4, guide=pilot, **kw)
         ctx.synchronize()
-        t_train = time.perf_counter() - t0
-        rest = max(1, int((t_plain - t_train) / (t16 / 16)))
-        gb = Batch(ctx, [dict(job, spp=rest, seed=14, guide=guide)])
-        # the graph is captured and the tabulation refreshed on the first launch
-        # of a configuration: done here, before the clock restarts
-        t_cap = time.perf_counter()
-        gb.run(cfg)
-        t_cap = time.perf_counter() - t_cap
-        ctx.synchronize()
-        t1 = time.perf_counter()
-        gb.run(cfg)
-        t_batch = time.perf_counter() - t1
-        n = args.train_spp + rest
-        img = (f.read() + gb.read(0)) / n
-        e, e_c = relmse_parts(img, ref, ref_var)
-        t_g = t_train + t_batch
-        st = guide.stats()
-        rec = {"guide_prob": prob, "guided_vertices": args.guided_vertices, "cache_res": args.guide_cache,
-               "train_spp": args.train_spp, "batched_spp": rest, "spp": n, "ms": round(t_g * 1e3, 2),
-               "train_ms": round(t_train * 1e3, 2), "train_steps": st["steps"], "relmse": e,
-               "relmse_minus_ref_noise": e_c, "ratio_to_plain": round(e_c / e_plain_c, 4),
-               "ratio_to_plain_time_normalised": round(e_c * t_g / (e_plain_c * t_plain), 4),
-               "ratio_at_equal_spp": round(e_c * n / (e_plain_c * spp), 4)}
-        out.append(rec)
-        log(f"guided+batched p={prob}: {rec}")
-        gb.close()
-        f.close()
-        guide.close()
-    legs["guided_batched"] = out
-    return legs
+        per_wf = (time.perf_counter() - t0) / 64
+        per_b = None
+        if args.batched:
+            pb = Batch(ctx, [dict(job, spp=64, guide=pilot)])
+            pb.run(cfg)
+            per_b = pb.run(cfg) * 1e-3 / 64
+            pb.close()
+        pilot.close()
+        for form in ("guided", "guided_batched") if args.batched else ("guided",):
+            guide = Guide(ctx, seed=22, **gkw)
+            f.clear()
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            ctx.render(g, wl.camera, f, spp=args.train_spp, seed=13, guide=guide, train=True, **kw)
+            ctx.synchronize()
+            t_train = time.perf_counter() - t0
+            steps = guide.stats()["steps"]
+            if form == "guided":
+                rest = max(1, int((t_plain - t_train) / per_wf))
+                ctx.render(g, wl.camera, f, spp=rest, seed=13, sample_offset=args.train_spp, guide=guide, **kw)
+                ctx.synchronize()
+                t_all = time.perf_counter() - t0
+                img = f.read() / (args.train_spp + rest)
+                extra = {"frozen_spp": rest}
+            else:
+                rest = max(1, int((t_plain - t_train) / per_b))
+                gb = Batch(ctx, [dict(job, spp=rest, seed=14, guide=guide)])
+                gb.run(cfg)  # graph capture and tabulation on the first launch of the shape, untimed
+                ctx.synchronize()
+                t1 = time.perf_counter()
+                gb.run(cfg)
+                t_all = t_train + (time.perf_counter() - t1)
+                img = (f.read() + gb.read(0)) / (args.train_spp + rest)
+                gb.close()
+                extra = {"batched_spp": rest, "item_shape": list(cfg)}
+            res[form].append(score(form, prob, args.train_spp + rest, t_all, t_train, steps, img, extra))
+            guide.close()
+    return res
 
 
 def pinhole_rays(cam, W, H, xy, n, rng):
@@ -798,7 +791,8 @@ def main():
     ap.add_argument("--samples-per-thread", type=int, default=0)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--plain-spp", type=int, default=1024, help="guided workload: plain run's spp (sets the time)")
-    ap.add_argument("--ref-mult", type=int, default=8, help="quality workload: reference spp = ref_mult x spp")
+    ap.add_argument("--ref-mult", type=int, default=16,
+                    help="reference spp = ref_mult x the workload's spp (quality workload and equal-time block)")
     ap.add_argument("--cpu-block-spp", type=int, default=1024, help="quality workload: oracle FP64 block spp")
     ap.add_argument("--guided-vertices", type=int, default=1)
     ap.add_argument("--guide-prob", type=float, default=0.25)
@@ -815,10 +809,12 @@ def main():
     ap.add_argument("--guide-cache-dirs", type=int, default=4, help="tabulation direction bins per axis")
     ap.add_argument("--no-quality", action="store_true",
                     help="cloud512: skip the equal-time relMSE block (FP64 reference, CPU probes, guided runs)")
-    ap.add_argument("--ref-passes", type=int, default=16,
-                    help="equal-time block: FP64 reference spp = ref_passes x the workload's spp")
+    ap.add_argument("--ref-passes", type=int, default=64,
+                    help="equal-time block: the FP64 reference in this many launches (its variance estimate)")
     ap.add_argument("--probe-spp", type=int, default=65536, help="equal-time block: CPU FP64 spp per probe pixel")
     ap.add_argument("--train-spp", type=int, default=1, help="equal-time block: spp rendered while the guide trains")
+    ap.add_argument("--budgets", default="1,4",
+                    help="equal-time block: time budgets in frames of the workload's spp (plain wavefront time)")
     ap.add_argument("--no-batched", dest="batched", action="store_false",
                     help="equal-time block: skip the batch-kernel legs (plain batched, guided+batched)")
     ap.add_argument("--guide-probs", default="0.5,0.25",
@@ -827,6 +823,7 @@ def main():
                     help="only run plain copies of BYTES (counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
     args.guide_probs = [float(x) for x in args.guide_probs.split(",") if x]
+    args.budgets = [int(x) for x in args.budgets.split(",") if x]
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {args.gpus}); the path shards by "
               "pixel tiles / sample ranges with a final FP64 reduce, which is not part of this build", file=sys.stderr)

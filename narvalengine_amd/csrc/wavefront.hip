@@ -560,8 +560,12 @@ __global__ __launch_bounds__(kChunk) void k_wf_guide(WfState w, WfArgs a, DevSce
 // the path in sorted position t.  Waves then take one branch of the kind
 // switch and one material's parameters; each path keeps its own RNG stream,
 // so the image is unchanged.
-template <bool GUIDED, bool SORT>
-__global__ __launch_bounds__(kChunk) void k_wf_shade(WfState w, WfArgs a, DevScene s, WfGuide gd) {
+//
+// TRAIN (guided renders that train): the training record travels with the
+// path; frozen guided renders carry none, and are held to the plain stage's
+// 7 waves per SIMD (71 VGPRs, no spills).
+template <bool GUIDED, bool SORT, bool TRAIN>
+__global__ __launch_bounds__(kChunk, (GUIDED && !TRAIN) ? 7 : 1) void k_wf_shade(WfState w, WfArgs a, DevScene s, WfGuide gd) {
   __shared__ uint32_t wsum[kChunk / 64];
   const WfPaths A = w.paths[a.cur], B = w.paths[a.cur ^ 1];
   const WfShadow S = w.shadow;
@@ -617,7 +621,7 @@ __global__ __launch_bounds__(kChunk) void k_wf_shade(WfState w, WfArgs a, DevSce
     smp = A.smp[i];
     lx = A.Lx[i]; ly = A.Ly[i]; lz = A.Lz[i];
     beta = v3(A.bx[i], A.by[i], A.bz[i]);
-    if constexpr (GUIDED) {
+    if constexpr (TRAIN) {
       const WfGuideRec& R = gd.rec[a.cur];
       rst = R.state[i];
       if (rst != 0) {
@@ -708,7 +712,7 @@ __global__ __launch_bounds__(kChunk) void k_wf_shade(WfState w, WfArgs a, DevSce
         ppdf = pm;
         beta = beta * (ph / pm);
         cont = true;
-        if (gd.train) {
+        if constexpr (TRAIN) {
           // One record per path, a reservoir over its guided vertices: the
           // m-th guided vertex replaces the record with probability 1/m, so
           // every guided vertex is the recorded one with probability 1/m_end
@@ -758,7 +762,7 @@ __global__ __launch_bounds__(kChunk) void k_wf_shade(WfState w, WfArgs a, DevSce
     B.pix[b] = pix; B.smp[b] = smp; B.rngn[b] = rngn;
     B.depth[b] = depth + 1u;  // prev_delta cleared
     B.prev_pdf[b] = ppdf;
-    if constexpr (GUIDED) {
+    if constexpr (TRAIN) {
       const WfGuideRec& R = gd.rec[a.cur ^ 1];
       R.state[b] = rst;
       if (rst != 0) {
@@ -771,7 +775,7 @@ __global__ __launch_bounds__(kChunk) void k_wf_shade(WfState w, WfArgs a, DevSce
     if (a.sbuf) sample_store(a, pix, smp, lx, ly, lz);
     else film_add(a.film_fixed, pix, lx, ly, lz);
   }
-  if constexpr (GUIDED) {
+  if constexpr (TRAIN) {
     // a recording path that ends now becomes a training sample
     float r = 0.0f;
     if (ends && !cont && (rst & 3u) == 2u) {
@@ -1115,10 +1119,12 @@ hipError_t launch_wavefront(const DevScene& s, const DevCamera& cam, WfState& w,
       hipLaunchKernelGGL(k_wf_guide, grid, blk, 0, st, w, a, s, gh->g);
     }
     NE_MARK(kStageShade);
-    if (gh && sort) hipLaunchKernelGGL((k_wf_shade<true, true>), grid, blk, 0, st, w, a, s, gh->g);
-    else if (gh) hipLaunchKernelGGL((k_wf_shade<true, false>), grid, blk, 0, st, w, a, s, gh->g);
-    else if (sort) hipLaunchKernelGGL((k_wf_shade<false, true>), grid, blk, 0, st, w, a, s, WfGuide{});
-    else hipLaunchKernelGGL((k_wf_shade<false, false>), grid, blk, 0, st, w, a, s, WfGuide{});
+    if (gh && training && sort) hipLaunchKernelGGL((k_wf_shade<true, true, true>), grid, blk, 0, st, w, a, s, gh->g);
+    else if (gh && training) hipLaunchKernelGGL((k_wf_shade<true, false, true>), grid, blk, 0, st, w, a, s, gh->g);
+    else if (gh && sort) hipLaunchKernelGGL((k_wf_shade<true, true, false>), grid, blk, 0, st, w, a, s, gh->g);
+    else if (gh) hipLaunchKernelGGL((k_wf_shade<true, false, false>), grid, blk, 0, st, w, a, s, gh->g);
+    else if (sort) hipLaunchKernelGGL((k_wf_shade<false, true, false>), grid, blk, 0, st, w, a, s, WfGuide{});
+    else hipLaunchKernelGGL((k_wf_shade<false, false, false>), grid, blk, 0, st, w, a, s, WfGuide{});
     NE_MARK(kStageShadow);
 #define NE_SHADOW(CNT, S_, V_)                                                             \
   do {                                                                                     \
