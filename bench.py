@@ -348,9 +348,14 @@ def probe_check(wl, ref, ref_var, ref_dof: int, spp: int, threads: int, seed: in
     rv = ref_var[xy[:, 1], xy[:, 0]]
     z = (r - m) / np.sqrt(np.maximum(var_m + rv, 1e-30))
     bound = float(student_t.ppf(1.0 - 0.01 / (2 * z.size), ref_dof))
+    worst = np.argsort(-np.abs(z).reshape(-1))[:4]
     return {"probes": int(len(xy)), "spp": spp, "threads": threads, "seconds": round(secs, 2),
             "max_abs_z": round(float(np.abs(z).max()), 3), "z_bound": round(bound, 3),
-            "mean_z2": round(float(np.mean(z * z)), 3),
+            "mean_z2": round(float(np.mean(z * z)), 3), "count_abs_z_over_3": int((np.abs(z) > 3).sum()),
+            "worst": [{"pixel": [int(xy[k // 3, 0]), int(xy[k // 3, 1])], "channel": int(k % 3),
+                       "z": round(float(z.reshape(-1)[k]), 3), "reference": float(r.reshape(-1)[k]),
+                       "cpu": float(m.reshape(-1)[k]), "reference_stderr": float(np.sqrt(rv.reshape(-1)[k])),
+                       "cpu_stderr": float(np.sqrt(var_m.reshape(-1)[k]))} for k in worst],
             "mean_ratio_reference_to_cpu": round(float(r.sum() / max(m.sum(), 1e-30)), 5),
             "rel_stderr_cpu": round(float(np.sqrt(var_m.mean()) / max(m.mean(), 1e-30)), 6),
             "accepted": bool(np.abs(z).max() < bound)}
@@ -416,7 +421,7 @@ def equal_time_budget(ctx, g, wl, args, ref, ref_var, f, K: int) -> dict:
     def score(name, prob, n, t_all, t_train, steps, img, extra):
         e, e_c = relmse_parts(img, ref, ref_var)
         rec = {"guide_prob": prob, "guided_vertices": args.guided_vertices, "cache_res": args.guide_cache,
-               "cache_dirs": args.guide_cache_dirs, "train_spp": args.train_spp, "spp": n,
+               "cache_dirs": args.guide_cache_dirs, "train_spp": args.train_spp, "train_pool": args.train_pool, "spp": n,
                "ms": round(t_all * 1e3, 2), "train_ms": round(t_train * 1e3, 2), "train_steps": steps,
                "relmse": e, "relmse_minus_ref_noise": e_c, "ratio_to_plain": round(e_c / e_plain_c, 4),
                # relMSE x time: the ratio at exactly equal time for an unbiased estimator
@@ -461,7 +466,10 @@ def equal_time_budget(ctx, g, wl, args, ref, ref_var, f, K: int) -> dict:
             f.clear()
             ctx.synchronize()
             t0 = time.perf_counter()
-            ctx.render(g, wl.camera, f, spp=args.train_spp, seed=13, guide=guide, train=True, **kw)
+            # a small path pool while training: each wavefront iteration ends
+            # enough paths for one training step, and iterations are short
+            ctx.render(g, wl.camera, f, spp=args.train_spp, seed=13, guide=guide, train=True,
+                       pool_size=args.train_pool, **kw)
             ctx.synchronize()
             t_train = time.perf_counter() - t0
             steps = guide.stats()["steps"]
@@ -812,8 +820,10 @@ def main():
     ap.add_argument("--ref-passes", type=int, default=64,
                     help="equal-time block: the FP64 reference in this many launches (its variance estimate)")
     ap.add_argument("--probe-spp", type=int, default=65536, help="equal-time block: CPU FP64 spp per probe pixel")
-    ap.add_argument("--train-spp", type=int, default=1, help="equal-time block: spp rendered while the guide trains")
-    ap.add_argument("--budgets", default="1,4",
+    ap.add_argument("--train-spp", type=int, default=8, help="equal-time block: spp rendered while the guide trains")
+    ap.add_argument("--train-pool", type=int, default=1 << 17,
+                    help="equal-time block: wavefront path slots of the training render")
+    ap.add_argument("--budgets", default="1,16",
                     help="equal-time block: time budgets in frames of the workload's spp (plain wavefront time)")
     ap.add_argument("--no-batched", dest="batched", action="store_false",
                     help="equal-time block: skip the batch-kernel legs (plain batched, guided+batched)")
