@@ -434,7 +434,7 @@ def equal_time_budget(ctx, g, wl, args, ref, ref_var, f, K: int) -> dict:
     job = dict(scene=g, camera=wl.camera, width=W, height=H, spp=n_plain, max_bounces=wl.max_bounces, seed=17, job=0)
     if args.batched:
         b = Batch(ctx, [job])
-        cfg, _ = b.autotune(spp_divisor=16 * K)
+        cfg, _ = b.autotune(spp_divisor=16)
         b.run(cfg)
         ms = b.run(cfg)
         e, e_c = relmse_parts(b.read(0) / n_plain, ref, ref_var)
@@ -799,7 +799,7 @@ def main():
     ap.add_argument("--samples-per-thread", type=int, default=0)
     ap.add_argument("--block", type=int, default=0)
     ap.add_argument("--plain-spp", type=int, default=1024, help="guided workload: plain run's spp (sets the time)")
-    ap.add_argument("--ref-mult", type=int, default=16,
+    ap.add_argument("--ref-mult", type=int, default=64,
                     help="reference spp = ref_mult x the workload's spp (quality workload and equal-time block)")
     ap.add_argument("--cpu-block-spp", type=int, default=1024, help="quality workload: oracle FP64 block spp")
     ap.add_argument("--guided-vertices", type=int, default=1)
