@@ -24,32 +24,29 @@
 
 namespace ne {
 
+// n queries (position, direction) -> mixture (or raw outputs), 64 per wave tile.
 __global__ __launch_bounds__(kGuideBlock) void k_guide_eval(GuideNet g, int n, const float* in6, float* out40,
                                                             int raw) {
-  __shared__ uint16_t xa[kWavesPerBlock][16 * kLdA];
-  __shared__ uint16_t xb[kWavesPerBlock][16 * kLdA];
-  __shared__ float out[kWavesPerBlock][16 * kLdO];
+  __shared__ __align__(16) unsigned char buf[kWavesPerBlock][kWave64Bytes];
   const int wv = threadIdx.x >> 6, lane = __lane_id();
-  const int tiles = (n + 15) / 16;
+  const int tiles = (n + 63) / 64;
   for (int t = blockIdx.x * kWavesPerBlock + wv; t < tiles; t += gridDim.x * kWavesPerBlock) {
     wave_sync();
-    guide_encode_tile(g, xa[wv], [&](int i, Vec3<float>& p, Vec3<float>& d, bool& valid) {
-      const int q = t * 16 + i;
+    guide_encode_tile64(g, reinterpret_cast<uint16_t*>(buf[wv]), [&](int i, Vec3<float>& p, Vec3<float>& d, bool& valid) {
+      const int q = t * 64 + i;
       valid = q < n;
       const float* s = in6 + 6 * size_t(valid ? q : 0);
       p = v3(s[0], s[1], s[2]);
       d = v3(s[3], s[4], s[5]);
     });
-    guide_mlp_tile(g, xa[wv], xb[wv], out[wv]);
-    if (lane < 16) {
-      const int q = t * 16 + lane;
-      if (q < n) {
-        float* o = out40 + 40 * size_t(q);
-        if (raw) {
-          for (int c = 0; c < kGuideOutputs; ++c) o[c] = out[wv][lane * kLdO + c];
-        } else {
-          guide_head(&out[wv][lane * kLdO], o);
-        }
+    const float* out = guide_mlp_tile64(g, buf[wv]);
+    const int q = t * 64 + lane;
+    if (q < n) {
+      float* o = out40 + 40 * size_t(q);
+      if (raw) {
+        for (int c = 0; c < kGuideOutputs; ++c) o[c] = out[lane * kLdOut64 + c];
+      } else {
+        guide_head(&out[lane * kLdOut64], o);
       }
     }
   }
@@ -58,16 +55,14 @@ __global__ __launch_bounds__(kGuideBlock) void k_guide_eval(GuideNet g, int n, c
 // Tabulated proposal: the mixture at every (cell centre, direction-bin centre)
 // of the cache, written to cache.mix (40 floats per entry).
 __global__ __launch_bounds__(kGuideBlock) void k_guide_bake(GuideNet g, GuideCache c, float* mix, int n) {
-  __shared__ uint16_t xa[kWavesPerBlock][16 * kLdA];
-  __shared__ uint16_t xb[kWavesPerBlock][16 * kLdA];
-  __shared__ float out[kWavesPerBlock][16 * kLdO];
+  __shared__ __align__(16) unsigned char buf[kWavesPerBlock][kWave64Bytes];
   const int wv = threadIdx.x >> 6, lane = __lane_id();
   const int nb = c.nd * c.nd;
-  const int tiles = (n + 15) / 16;
+  const int tiles = (n + 63) / 64;
   for (int t = blockIdx.x * kWavesPerBlock + wv; t < tiles; t += gridDim.x * kWavesPerBlock) {
     wave_sync();
-    guide_encode_tile(g, xa[wv], [&](int i, Vec3<float>& p, Vec3<float>& d, bool& valid) {
-      const int q = t * 16 + i;
+    guide_encode_tile64(g, reinterpret_cast<uint16_t*>(buf[wv]), [&](int i, Vec3<float>& p, Vec3<float>& d, bool& valid) {
+      const int q = t * 64 + i;
       valid = q < n;
       const int e = valid ? q : 0;
       const int cell = e / nb, bin = e % nb;
@@ -77,11 +72,9 @@ __global__ __launch_bounds__(kGuideBlock) void k_guide_bake(GuideNet g, GuideCac
       const float invd = 1.0f / float(c.nd);
       oct_decode((float(bin % c.nd) + 0.5f) * invd, (float(bin / c.nd) + 0.5f) * invd, d.x, d.y, d.z);
     });
-    guide_mlp_tile(g, xa[wv], xb[wv], out[wv]);
-    if (lane < 16) {
-      const int q = t * 16 + lane;
-      if (q < n) guide_head(&out[wv][lane * kLdO], mix + 40 * size_t(q));
-    }
+    const float* out = guide_mlp_tile64(g, buf[wv]);
+    const int q = t * 64 + lane;
+    if (q < n) guide_head(&out[lane * kLdOut64], mix + 40 * size_t(q));
   }
 }
 
@@ -437,14 +430,14 @@ hipError_t launch_guide_pack(const GuideNet& g, hipStream_t st) {
 }
 
 hipError_t launch_guide_eval(const GuideNet& g, int n, const float* in6, float* out40, hipStream_t st) {
-  const int tiles = (n + 15) / 16;
+  const int tiles = (n + 63) / 64;
   const int blocks = std::max(1, std::min(4096, (tiles + kWavesPerBlock - 1) / kWavesPerBlock));
   hipLaunchKernelGGL(k_guide_eval, dim3(blocks), dim3(kGuideBlock), 0, st, g, n, in6, out40, 0);
   return hipGetLastError();
 }
 
 hipError_t launch_guide_raw(const GuideNet& g, int n, const float* in6, float* out40, hipStream_t st) {
-  const int tiles = (n + 15) / 16;
+  const int tiles = (n + 63) / 64;
   const int blocks = std::max(1, std::min(4096, (tiles + kWavesPerBlock - 1) / kWavesPerBlock));
   hipLaunchKernelGGL(k_guide_eval, dim3(blocks), dim3(kGuideBlock), 0, st, g, n, in6, out40, 1);
   return hipGetLastError();
@@ -452,7 +445,7 @@ hipError_t launch_guide_raw(const GuideNet& g, int n, const float* in6, float* o
 
 hipError_t launch_guide_bake(const GuideNet& g, const GuideCache& c, float* mix, hipStream_t st) {
   const int n = c.nc * c.nc * c.nc * c.nd * c.nd;
-  const int tiles = (n + 15) / 16;
+  const int tiles = (n + 63) / 64;
   const int blocks = std::max(1, std::min(8192, (tiles + kWavesPerBlock - 1) / kWavesPerBlock));
   hipLaunchKernelGGL(k_guide_bake, dim3(blocks), dim3(kGuideBlock), 0, st, g, c, mix, n);
   return hipGetLastError();

@@ -150,6 +150,129 @@ __device__ __forceinline__ void guide_mlp_tile(const GuideNet& g, uint16_t* xa, 
   wave_sync();
 }
 
+// ---- MLP on 64-row tiles (evaluation and tabulation) ------------------------------
+// One wave evaluates 64 queries: four 16-row tiles share every weight fragment,
+// so a layer reads its weights once per 64 queries.  K = 32 BF16 MFMA
+// (v_mfma_f32_16x16x32_bf16), lane l: A row l % 16 and B column l % 16 with
+// k = 8 (l / 16) .. +7 of the 32-wide k step; D column l % 16, rows
+// 4 (l / 16) .. +3.  Layer 0's 48 inputs end with one K = 16 step (k = 4 (l / 16)
+// .. +3, the layout of the 16-row path).
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+constexpr int kLd64 = 72;        // BF16 row stride of a 64-row activation tile (144 B: 16-byte aligned rows)
+constexpr int kLdOut64 = 41;     // FP32 row stride of the 64-row output tile (40 used)
+constexpr int kTile64Bytes = 64 * kLd64 * 2;                 // one activation tile
+constexpr int kOut64Bytes = 64 * kLdOut64 * 4;               // the output tile (shares the second buffer)
+constexpr int kWave64Bytes = kTile64Bytes + (kOut64Bytes > kTile64Bytes ? kOut64Bytes : kTile64Bytes);
+
+__device__ __forceinline__ bf16x8 ld_bf16x8(const uint16_t* p) {
+  const uint4 v = *reinterpret_cast<const uint4*>(p);
+  return __builtin_bit_cast(bf16x8, v);
+}
+
+template <int L>
+__device__ __forceinline__ void guide_layer64(const GuideNet& g, const uint16_t* xs, uint16_t* ys, float* out) {
+  constexpr int KIN = guide_in(L), NOUT = guide_out(L), NT = NOUT / 16;
+  const int lane = __lane_id();
+  const int r = lane & 15, k8 = 8 * (lane >> 4), k4 = 4 * (lane >> 4);
+  f32x4 acc[4][NT];
+#pragma unroll
+  for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) acc[rt][nt] = f32x4{0.0f, 0.0f, 0.0f, 0.0f};
+  const uint16_t* wf = g.wf + guide_w_off(L);
+#pragma unroll
+  for (int ks = 0; ks < KIN / 32; ++ks) {
+    bf16x8 b[NT];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) b[nt] = ld_bf16x8(wf + (16 * nt + r) * KIN + 32 * ks + k8);
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt) {
+      const bf16x8 a = ld_bf16x8(xs + (16 * rt + r) * kLd64 + 32 * ks + k8);
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt) acc[rt][nt] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b[nt], acc[rt][nt], 0, 0, 0);
+    }
+  }
+  if constexpr (KIN % 32 == 16) {  // the last 16 inputs
+    constexpr int k0 = KIN - 16;
+    bf16x4 b[NT];
+#pragma unroll
+    for (int nt = 0; nt < NT; ++nt) b[nt] = *reinterpret_cast<const bf16x4*>(wf + (16 * nt + r) * KIN + k0 + k4);
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt) {
+      const bf16x4 a = *reinterpret_cast<const bf16x4*>(xs + (16 * rt + r) * kLd64 + k0 + k4);
+#pragma unroll
+      for (int nt = 0; nt < NT; ++nt) acc[rt][nt] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b[nt], acc[rt][nt], 0, 0, 0);
+    }
+  }
+  const float* bias = g.mlp + guide_b_off(L);
+#pragma unroll
+  for (int nt = 0; nt < NT; ++nt) {
+    const int col = 16 * nt + r;
+    const float bc = bias[col];
+#pragma unroll
+    for (int rt = 0; rt < 4; ++rt)
+#pragma unroll
+      for (int j = 0; j < 4; ++j) {
+        const int row = 16 * rt + k4 + j;
+        const float h = acc[rt][nt][j] + bc;
+        if (L < kGuideLayers - 1) ys[row * kLd64 + col] = f2bf(h > 0.0f ? h : 0.0f);
+        else if (col < kGuideOutputs) out[row * kLdOut64 + col] = h;
+      }
+  }
+}
+
+// Encoding + direction of 64 queries (lane = row) into xs (BF16 [64][kLd64],
+// columns 35..47 zero); the accessor f(row, pos, dir, valid) as for 16 rows.
+template <typename F>
+__device__ __forceinline__ void guide_encode_tile64(const GuideNet& g, uint16_t* xs, F&& f) {
+  const int row = __lane_id();
+  Vec3<float> p, d;
+  bool valid;
+  f(row, p, d, valid);
+  uint16_t* x = xs + row * kLd64;
+#pragma unroll 4
+  for (int l = 0; l < kGuideLevels; ++l) {
+    float e0 = 0.0f, e1 = 0.0f;
+    if (valid) {
+#pragma unroll
+      for (int k = 0; k < 8; ++k) {
+        uint32_t ent;
+        float w;
+        guide_corner(g, l, p, k, ent, w);
+        const uint32_t h = g.table_bf[ent];
+        e0 += w * bf2f(uint16_t(h & 0xFFFFu));
+        e1 += w * bf2f(uint16_t(h >> 16));
+      }
+    }
+    x[2 * l] = f2bf(e0);
+    x[2 * l + 1] = f2bf(e1);
+  }
+  x[32] = f2bf(valid ? d.x : 0.0f);
+  x[33] = f2bf(valid ? d.y : 0.0f);
+  x[34] = f2bf(valid ? d.z : 0.0f);
+  for (int c = 35; c < 48; ++c) x[c] = 0;
+}
+
+// Full forward of a 64-row tile held in buf (kWave64Bytes): raw outputs in
+// the returned [64][kLdOut64] FP32 tile.
+__device__ __forceinline__ const float* guide_mlp_tile64(const GuideNet& g, unsigned char* buf) {
+  uint16_t* xa = reinterpret_cast<uint16_t*>(buf);
+  uint16_t* xb = reinterpret_cast<uint16_t*>(buf + kTile64Bytes);
+  float* out = reinterpret_cast<float*>(buf + kTile64Bytes);
+  wave_sync();
+  guide_layer64<0>(g, xa, xb, out);
+  wave_sync();
+  guide_layer64<1>(g, xb, xa, out);
+  wave_sync();
+  guide_layer64<2>(g, xa, xb, out);
+  wave_sync();
+  guide_layer64<3>(g, xb, xa, out);
+  wave_sync();
+  guide_layer64<4>(g, xa, xb, out);  // the FP32 output overwrites xb, no longer read
+  wave_sync();
+  return out;
+}
+
 // kappa from its raw output: exp of the raw value clamped to [-10, 8]
 __device__ __forceinline__ float guide_kappa(float raw) { return expf(xmin(xmax(raw, -10.0f), 8.0f)); }
 
