@@ -637,8 +637,27 @@ ne_status ne_scene_upload(ne_ctx* c, const ne_scene_desc* d, ne_scene** out) {
     ds.sigma_bar = (st3[0] * d->vol_mult + st3[1] * d->vol_mult + st3[2] * d->vol_mult) / 3.0f;
     ds.g = d->vol_g;
     ds.mres = homog ? 1 : d->maj_res;
-    size_t nvox = size_t(ds.res[0]) * ds.res[1] * ds.res[2];
-    if ((st = dev_upload(c, s, homog ? &kOne : d->grid, nvox, &ds.grid)) != NE_OK) return bail(st);
+    {
+      // row-major upload into a scratch buffer, bricked copy kept (zero padded,
+      // plus one brick of slack after the last)
+      const size_t nvox = size_t(ds.res[0]) * ds.res[1] * ds.res[2];
+      const uint32_t px = grid_pad_x(ds.res[0]), py = grid_pad_y(ds.res[1]), pz = grid_pad_z(ds.res[2]);
+      const size_t nb = (size_t(px) * py * pz + kBrickFloats) * sizeof(float);
+      DevBuf lin;
+      NE_HIP_BAIL(hipMalloc(&lin.p, nvox * sizeof(float)));
+      NE_HIP_BAIL(hipMemcpyAsync(lin.p, homog ? &kOne : d->grid, nvox * sizeof(float), hipMemcpyHostToDevice,
+                                 c->stream));
+      void* pg = nullptr;
+      NE_HIP_BAIL(hipMalloc(&pg, nb));
+      s->allocs.push_back(pg);
+      s->bytes += nb;
+      NE_HIP_BAIL(hipMemsetAsync(pg, 0, nb, c->stream));
+      NE_HIP_BAIL(launch_brick_grid(static_cast<const float*>(lin.p), static_cast<float*>(pg), ds.res, c->stream));
+      NE_HIP_BAIL(hipStreamSynchronize(c->stream));  // before the scratch buffer is freed
+      ds.grid = static_cast<const float*>(pg);
+      ds.grid_step_y = (px / 4u) * kBrickFloats;
+      ds.grid_step_z = (px / 4u) * (py / 4u) * kBrickFloats;
+    }
     size_t ncell = size_t(ds.mres) * ds.mres * ds.mres;
     void* pm = nullptr;
     NE_HIP_BAIL(hipMalloc(&pm, ncell * sizeof(float)));

@@ -284,6 +284,8 @@ hipError_t launch_test_rng(uint64_t seed, uint32_t pixel, uint32_t sample, uint3
                            hipStream_t st);
 hipError_t launch_li(const DevScene& s, int n, const float* rays, int max_bounces, uint64_t seed, uint32_t sample,
                      float* out, hipStream_t st);
+// row-major density -> bricked layout (grid_offset), zero padding
+hipError_t launch_brick_grid(const float* lin, float* out, const int res[3], hipStream_t st);
 hipError_t launch_majorants(const DevScene& s, float* maj, hipStream_t st);
 
 }  // namespace ne

@@ -20,9 +20,11 @@
 //           arrays; rect 4 x float4 in leaf order:
 //           (p0, material), (eu, area), (ev, original index), (n, 0).
 //   mats    12 floats per material (see ne_gpu.h).
-//   grid    R32F x-fastest density; maj: mres^3 floats, sigma_t majorant per
-//           cell (grid value max over the cell's trilinear support x sigma_bar).
+//   grid    R32F density in 4 x 4 x 2 voxel bricks (grid_offset below);
+//           maj: mres^3 floats, sigma_t majorant per cell (grid value max over
+//           the cell's trilinear support x sigma_bar).
 #pragma once
+#include <hip/hip_runtime.h>
 #include <stdint.h>
 
 namespace ne {
@@ -45,6 +47,20 @@ constexpr uint32_t kItemCountShift = 26;
 constexpr uint32_t kItemNone = 0xFFFFFFFFu;  // no item (leaf type 3 does not exist)
 constexpr uint32_t kItemMaxLeaf = 8;  // count - 1 in 3 bits
 
+// Density grid layout: bricks of 4 x 4 x 2 voxels (32 floats: one 128-byte
+// cache line), stored x-fastest, then y, then z.  A trilinear lookup's 2 x 2 x 2
+// corners then span 2.3 lines on average instead of 4 in a row-major grid.
+// The grid is padded with zeros to grid_pad_{x,y,z}(res) >= res + 2 voxels per
+// axis: corners at an index >= res read 0 without a bounds check.
+constexpr uint32_t kBrickFloats = 32;
+__host__ __device__ inline uint32_t grid_pad_x(int32_t r) { return (uint32_t(r) + 2u + 3u) & ~3u; }
+__host__ __device__ inline uint32_t grid_pad_y(int32_t r) { return (uint32_t(r) + 2u + 3u) & ~3u; }
+__host__ __device__ inline uint32_t grid_pad_z(int32_t r) { return (uint32_t(r) + 2u + 1u) & ~1u; }
+__host__ __device__ inline uint64_t grid_offset(uint32_t x, uint32_t y, uint32_t z, uint32_t step_y, uint32_t step_z) {
+  return uint64_t(z >> 1) * step_z + uint64_t(y >> 2) * step_y + uint64_t(x >> 2) * kBrickFloats +
+         (((z & 1u) << 4) | ((y & 3u) << 2) | (x & 3u));
+}
+
 struct DevScene {
   const float4* nodes;
   const float4* tris;
@@ -65,7 +81,9 @@ struct DevScene {
   int32_t occ_lo[3], occ_hi[3];  // majorant cells with a positive majorant lie in [occ_lo, occ_hi]
                                  // (occ_lo > occ_hi: none)
   float inv[12];          // WCS -> OCS, rows of a 3x4 matrix
-  const float* grid;
+  const float* grid;      // bricked (grid_offset), zero-padded to at least res + 2 voxels per axis
+  uint32_t grid_step_y;   // floats from one brick row (4 voxels in y) to the next
+  uint32_t grid_step_z;   // floats from one brick slab (2 voxels in z) to the next
   const float* maj;
   float sigma_bar;        // avg(sigma_t * densityMultiplier)
   float sig_s[3];

@@ -470,12 +470,11 @@ NE_DEV bool box_interval(const DevScene& s, Vec3<R> oo, Vec3<R> dd, R& ta, R& tb
 }
 
 NE_DEV float grid_at(const DevScene& s, int x, int y, int z) {
-  if (x >= s.res[0] || y >= s.res[1] || z >= s.res[2]) return 0.0f;
-  return s.grid[size_t(z) * size_t(s.res[0]) * size_t(s.res[1]) + size_t(y) * size_t(s.res[0]) + size_t(x)];
+  return s.grid[grid_offset(uint32_t(x), uint32_t(y), uint32_t(z), s.grid_step_y, s.grid_step_z)];
 }
 // GridMedia::sampleAt -> interpolatedDensity (src/materials/GridMedia.cpp:25-43):
 // OCS [-.5,.5]^3 -> grid space (p + .5) * res, clamp, cell-corner trilinear,
-// corners at index >= res read as 0.
+// corners at index >= res read as 0 (the grid's zero padding).
 // A homogeneous medium (HomogeneousMedia, src/materials/HomogeneousMedia.cpp)
 // reads density 1 everywhere; the multiplier carries its density.
 NE_DEV float density_gcs(const DevScene& s, Vec3<float> gp) {
@@ -484,28 +483,29 @@ NE_DEV float density_gcs(const DevScene& s, Vec3<float> gp) {
   gp = v3(xmin(gp.x, float(s.res[0])), xmin(gp.y, float(s.res[1])), xmin(gp.z, float(s.res[2])));
   int ix = int(floorf(gp.x)), iy = int(floorf(gp.y)), iz = int(floorf(gp.z));
   float dx = gp.x - float(ix), dy = gp.y - float(iy), dz = gp.z - float(iz);
-  float c000, c100, c010, c110, c001, c101, c011, c111;
-  if (ix + 1 < s.res[0] && iy + 1 < s.res[1] && iz + 1 < s.res[2]) {
-    // interior cell: every corner is in the grid; the x-neighbours are
-    // adjacent floats, read as pairs (4-byte aligned)
-    // row index z * H + y < 2^24 (each dimension is at most 4096): a 24-bit
-    // multiply; one 32x32->64 multiply-add for the voxel offset
-    const size_t W = size_t(s.res[0]), WH = W * size_t(s.res[1]);
-    const uint32_t row = __umul24(uint32_t(iz), uint32_t(s.res[1])) + uint32_t(iy);
-    const float* p = s.grid + (uint64_t(row) * uint64_t(uint32_t(s.res[0])) + uint64_t(uint32_t(ix)));
-    float2 q00, q10, q01, q11;
-    __builtin_memcpy(&q00, p, 8);
-    __builtin_memcpy(&q10, p + W, 8);
-    __builtin_memcpy(&q01, p + WH, 8);
-    __builtin_memcpy(&q11, p + WH + W, 8);
-    c000 = q00.x; c100 = q00.y; c010 = q10.x; c110 = q10.y;
-    c001 = q01.x; c101 = q01.y; c011 = q11.x; c111 = q11.y;
-  } else {
-    c000 = grid_at(s, ix, iy, iz); c100 = grid_at(s, ix + 1, iy, iz);
-    c010 = grid_at(s, ix, iy + 1, iz); c110 = grid_at(s, ix + 1, iy + 1, iz);
-    c001 = grid_at(s, ix, iy, iz + 1); c101 = grid_at(s, ix + 1, iy, iz + 1);
-    c011 = grid_at(s, ix, iy + 1, iz + 1); c111 = grid_at(s, ix + 1, iy + 1, iz + 1);
+  // 0 <= i <= res on every axis and the padding holds index res + 1: every
+  // corner is in the allocation.  Within a brick the x-neighbour is the next
+  // float, so each (y, z) corner row is one 8-byte read (4-byte aligned); a
+  // corner row that crosses into the next brick in x (ix % 4 == 3) takes its
+  // second float from there (the first read's second float then lies in the
+  // same brick or the next one, which the allocation's slack covers).
+  const float* p = s.grid + grid_offset(uint32_t(ix), uint32_t(iy), uint32_t(iz), s.grid_step_y, s.grid_step_z);
+  const uint32_t oy = (iy & 3) != 3 ? 4u : s.grid_step_y - 12u;
+  const uint32_t oz = (iz & 1) == 0 ? 16u : s.grid_step_z - 16u;
+  float2 q00, q10, q01, q11;
+  __builtin_memcpy(&q00, p, 8);
+  __builtin_memcpy(&q10, p + oy, 8);
+  __builtin_memcpy(&q01, p + oz, 8);
+  __builtin_memcpy(&q11, p + oy + oz, 8);
+  if ((ix & 3) == 3) {
+    constexpr uint32_t ox = kBrickFloats - 3u;  // x + 1 in the next brick
+    q00.y = p[ox];
+    q10.y = p[oy + ox];
+    q01.y = p[oz + ox];
+    q11.y = p[oy + oz + ox];
   }
+  const float c000 = q00.x, c100 = q00.y, c010 = q10.x, c110 = q10.y;
+  const float c001 = q01.x, c101 = q01.y, c011 = q11.x, c111 = q11.y;
   float d00 = mixf(c000, c100, dx), d10 = mixf(c010, c110, dx);
   float d01 = mixf(c001, c101, dx), d11 = mixf(c011, c111, dx);
   float d0 = mixf(d00, d10, dy), d1 = mixf(d01, d11, dy);

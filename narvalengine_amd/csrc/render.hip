@@ -212,7 +212,7 @@ __global__ __launch_bounds__(256) void k_majorants(DevScene s, float* maj, int s
   float m = 0.0f;
   for (int64_t i = int64_t(part) * blockDim.x + threadIdx.x; i < total; i += int64_t(split) * blockDim.x) {
     const int64_t x = lo[0] + i % nx, y = lo[1] + (i / nx) % ny, z = lo[2] + i / (nx * ny);
-    m = xmax(m, s.grid[size_t(z) * size_t(s.res[0]) * size_t(s.res[1]) + size_t(y) * size_t(s.res[0]) + size_t(x)]);
+    m = xmax(m, grid_at(s, int(x), int(y), int(z)));
   }
   __shared__ float red[256];
   red[threadIdx.x] = m;
@@ -222,6 +222,31 @@ __global__ __launch_bounds__(256) void k_majorants(DevScene s, float* maj, int s
     __syncthreads();
   }
   if (threadIdx.x == 0) atomicMax(reinterpret_cast<unsigned int*>(maj + cell), __float_as_uint(red[0] * s.sigma_bar));
+}
+
+// Row-major (x-fastest) density -> bricked layout, zeros in the padding: one
+// thread per padded voxel, in storage order.
+__global__ void k_brick_grid(const float* lin, float* out, int rx, int ry, int rz, uint32_t px, uint32_t py,
+                             uint64_t total) {
+  const uint64_t i = uint64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const uint32_t inner = uint32_t(i & (kBrickFloats - 1u));
+  const uint64_t b = i / kBrickFloats;
+  const uint32_t nbx = px / 4u, nby = py / 4u;
+  const uint32_t x = uint32_t(b % nbx) * 4u + (inner & 3u);
+  const uint32_t y = uint32_t((b / nbx) % nby) * 4u + ((inner >> 2) & 3u);
+  const uint32_t z = uint32_t(b / (uint64_t(nbx) * nby)) * 2u + (inner >> 4);
+  out[i] = (x < uint32_t(rx) && y < uint32_t(ry) && z < uint32_t(rz))
+               ? lin[(uint64_t(z) * uint32_t(ry) + y) * uint32_t(rx) + x]
+               : 0.0f;
+}
+
+hipError_t launch_brick_grid(const float* lin, float* out, const int res[3], hipStream_t st) {
+  const uint32_t px = grid_pad_x(res[0]), py = grid_pad_y(res[1]), pz = grid_pad_z(res[2]);
+  const uint64_t total = uint64_t(px) * py * pz;
+  hipLaunchKernelGGL(k_brick_grid, dim3(uint32_t((total + 255) / 256)), dim3(256), 0, st, lin, out, res[0], res[1],
+                     res[2], px, py, total);
+  return hipGetLastError();
 }
 
 hipError_t launch_majorants(const DevScene& s, float* maj, hipStream_t st) {
