@@ -3,7 +3,8 @@
 //
 // Work is cut into items: one item = a tile of tile_w x tile_h pixels of one
 // job x samples_per_item consecutive samples.  Items of all jobs form one
-// global queue (job by job, tiles outer, sample chunks inner); a wave takes the
+// global queue (job by job; within a job, passes of up to kBatchPassSamples
+// samples per pixel, tiles, sample chunks of the pass: batch_item_place); a wave takes the
 // next item with one global atomic and hands its paths to free lanes.  A lane
 // runs one bounce per loop iteration (path_bounce) and takes a new path as soon
 // as its path ends, so neither uneven path lengths nor uneven jobs leave lanes
@@ -27,6 +28,29 @@ __global__ void k_batch_init(BatchArgs b, size_t film_words) {
   if (i == 0) {
     *b.next_item = 0;
     *b.start_time = (unsigned long long)wall_clock64();
+  }
+}
+
+// Item order within a job.  Tiles outer with every sample chunk of a tile in
+// a row keeps the paths in flight on a few neighbouring tiles, which share
+// voxels in the caches; at high spp those few tiles also take every film
+// atomic in flight.  So the samples are cut into passes of kBatchPassSamples
+// per pixel: pass outer, then tile, then the pass's chunks.
+constexpr uint32_t kBatchPassSamples = 256;
+__device__ __forceinline__ void batch_item_place(const BatchJob& J, uint32_t spi, unsigned long long local,
+                                                 uint32_t& chunk, uint32_t& tile) {
+  const uint32_t ntiles = J.tiles_x * J.tiles_y;
+  const uint32_t cg = max(1u, kBatchPassSamples / spi);  // chunks per full pass
+  const uint32_t full = J.chunks / cg, rem = J.chunks % cg;
+  const unsigned long long per_pass = (unsigned long long)ntiles * cg;
+  if (local < per_pass * full) {
+    const uint32_t g = uint32_t(local / per_pass), r = uint32_t(local % per_pass);
+    tile = r / cg;
+    chunk = g * cg + r % cg;
+  } else {  // the last, shorter pass
+    const uint32_t r = uint32_t(local - per_pass * full);
+    tile = r / rem;
+    chunk = full * cg + r % rem;
   }
 }
 
@@ -85,7 +109,8 @@ __global__ __launch_bounds__(kBatchBlock, 3) void k_batch(BatchArgs b) {
         ij = batch_job_of(first, b.njobs, q);
         const BatchJob& J = b.jobs[ij];
         const unsigned long long local = q - first[ij];
-        const uint32_t chunk = uint32_t(local % J.chunks), tile = uint32_t(local / J.chunks);
+        uint32_t chunk, tile;
+        batch_item_place(J, b.spi, local, chunk, tile);
         ix0 = (tile % J.tiles_x) * b.tile_w;
         iy0 = (tile / J.tiles_x) * b.tile_h;
         iw = min(b.tile_w, uint32_t(J.W) - ix0);
