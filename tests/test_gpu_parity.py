@@ -9,6 +9,7 @@ import scenes
 from conftest import print_case
 from narvalengine_amd import abi
 from narvalengine_amd.abi import NE_FP32, NE_FP64
+from narvalengine_amd.scene import Scene
 
 pytestmark = pytest.mark.gpu
 RTOL = 1e-5
@@ -212,6 +213,32 @@ def test_density_bit_exact(ctx):
     grid = s.volume["grid"]
     exp = np.zeros(len(pts), np.float32)
     O.lib().neo_interp_density(32, 32, 32, O.fp(grid), len(pts), O.fp(O.f(pts)), O.fp(exp))
+    np.testing.assert_array_equal(got, exp)
+
+
+@pytest.mark.parametrize("shape", [(1, 1, 1), (3, 7, 5), (2, 4, 4), (9, 17, 33), (6, 5, 130)])
+def test_density_bit_exact_on_uneven_grids(ctx, shape):
+    """The device keeps the grid in 4 x 4 x 2-voxel bricks with zero padding;
+    sizes that are not multiples of the brick (and 1-voxel axes) read the
+    same trilinear densities as the oracle's row-major grid, on the upper
+    faces (corners at index res read 0) and at every brick boundary."""
+    print_case(f"density uneven grid {shape}")
+    D, H, W = shape
+    rng = np.random.default_rng(sum(shape))
+    grid = rng.uniform(0.0, 1.0, shape).astype(np.float32)
+    s = Scene()
+    s.set_volume(grid, 5.0, (0.9,) * 3, (0.1,) * 3, 0.3, maj_res=1)
+    g = ctx.upload(s)
+    res = np.array([W, H, D], np.float32)
+    pts = rng.uniform(0.0, 1.0, (20000, 3)).astype(np.float32) * res
+    # every integer corner, brick edges and the upper faces exactly
+    ii = np.stack(np.meshgrid(np.arange(W + 1), np.arange(H + 1), np.arange(D + 1), indexing="ij"), -1)
+    corners = ii.reshape(-1, 3).astype(np.float32)[:5000]
+    near = corners[: len(corners) // 2] + np.float32(0.999)
+    pts = np.concatenate([pts, corners, np.minimum(near, res)]).astype(np.float32)
+    got = ctx.test_density(g, pts)
+    exp = np.zeros(len(pts), np.float32)
+    O.lib().neo_interp_density(W, H, D, O.fp(grid), len(pts), O.fp(O.f(pts)), O.fp(exp))
     np.testing.assert_array_equal(got, exp)
 
 
