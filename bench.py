@@ -82,6 +82,13 @@ def measured_traffic(kernel: str, workload: str):
     return None
 
 
+def batch_config(cfg) -> dict:
+    """A batch configuration (tile_w, tile_h, samples_per_item, executor) for the JSON line."""
+    if len(cfg) > 3 and cfg[3] == 1:
+        return {"executor": "wavefront", "lanes": 4}
+    return {"executor": "persistent", "item_shape": list(cfg[:3])}
+
+
 def measured_valu(kernel: str, workload: str):
     """VALU issue fraction and active-lane fraction of `kernel` on `workload`
     from the committed rocprofv3 PMC summary, or None.  The hot stages are
@@ -172,9 +179,10 @@ def copy_calibration(nbytes: int, reps: int = 5):
 
 
 def bench_batch(args):
-    """48 uneven volume jobs in one persistent graph launch.  The item shape is
-    autotuned first (untimed, on a 1/16-spp calibration run); a step is one
-    launch that renders every job at full spp."""
+    """48 uneven volume jobs in one ne_batch_run.  The executor (persistent
+    kernel with an item shape, or wavefront lanes) is autotuned first (untimed,
+    on a 1/16-spp calibration run); a step is one run that renders every job
+    at full spp.  The executor not chosen is timed beside it."""
     from narvalengine_amd import Batch, Context, configs
     from narvalengine_amd.abi import NE_MODE_MEGAKERNEL
     t0 = time.perf_counter()
@@ -185,7 +193,7 @@ def bench_batch(args):
                  max_bounces=w.max_bounces, seed=1, job=k) for k, w in enumerate(wls)]
     b = Batch(ctx, jobs)
     cfg, tune_ms = b.autotune(spp_divisor=16)
-    log(f"autotuned item shape {cfg} ({tune_ms:.1f} ms calibration run)")
+    log(f"autotuned batch configuration {batch_config(cfg)} ({tune_ms:.1f} ms calibration run)")
     for i in range(args.warmup):
         b.run()
     paths = sum(j["width"] * j["height"] * j["spp"] for j in jobs)
@@ -199,6 +207,17 @@ def bench_batch(args):
     wall = time.perf_counter() - t_start
     value = args.steps * paths / wall
     launch_s = sum(ms) / len(ms) / 1e3
+    persistent = batch_config(cfg)["executor"] == "persistent"
+    # the other executor, timed once after a warm-up run
+    executors = {batch_config(cfg)["executor"]: {"ms": round(launch_s * 1e3, 2), "batch_config": batch_config(cfg)}}
+    others = [(8, 8, 4, 1)] if persistent else [(8, 8, 1, 0), (8, 8, 4, 0)]
+    for oc in others:
+        b.run(oc)
+        t_o = b.run(oc)
+        name = batch_config(oc)["executor"]
+        if name not in executors or t_o < executors[name]["ms"]:
+            executors[name] = {"ms": round(t_o, 2), "batch_config": batch_config(oc)}
+    log(f"executors: {executors}")
     roofline, counters = None, None
     if not args.no_count:
         tot = {}
@@ -212,10 +231,13 @@ def bench_batch(args):
         counters = tot
         algo = sum(tot[k] * b_ for k, b_ in EV.items()) + 2 * 8 * 3 * sum(j["width"] * j["height"] for j in jobs)
         achieved = algo / launch_s / 1e9
+        # persistent: the one k_batch launch; wavefront lanes: every stage
+        # kernel of the run on all lanes together (the run's wall time)
         roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                    "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": measured_traffic("k_batch", "batch48"),
-                    "kernel": "k_batch", "algorithmic_bytes_per_launch": int(algo),
-                    "avg_launch_ms": round(launch_s * 1e3, 3)}
+                    "frac": round(achieved / HBM_PEAK_GBS, 5),
+                    "traffic": measured_traffic("k_batch", "batch48") if persistent else None,
+                    "kernel": "k_batch" if persistent else "wavefront stages on 4 streams (whole run)",
+                    "algorithmic_bytes_per_launch": int(algo), "avg_launch_ms": round(launch_s * 1e3, 3)}
     cpu = None
     if not args.no_cpu:
         sys.path.insert(0, str(ROOT / "tests"))
@@ -266,13 +288,14 @@ def bench_batch(args):
         "steps": args.steps, "warmup": args.warmup, "ms_per_step": round(wall / args.steps * 1e3, 3),
         "higher_is_better": True, "scaling": "weak", "vs_baseline": None, "dtype": "f32",
         "data": "synthetic (seeded procedural grids)",
-        "config": {"workload": "batch48", "jobs": len(jobs), "paths_per_step": paths, "item_shape": list(cfg),
+        "config": {"workload": "batch48", "jobs": len(jobs), "paths_per_step": paths, "batch_config": batch_config(cfg),
                    "accumulation": "fixed-point i64"},
         "roofline": roofline,
         "cpu_baseline": cpu,
         "gpu_vs_cpu": round(value / cpu["value"], 1) if cpu else None,
         "job_completion_ms": {"min": round(float(jm.min()), 2), "median": round(float(np.median(jm)), 2),
                               "max": round(float(jm.max()), 2), "all": [round(float(x), 2) for x in jm]},
+        "executors": executors,
         "sequential": seq,
         "counters_per_step": counters,
     }
@@ -439,7 +462,7 @@ def equal_time_budget(ctx, g, wl, args, ref, ref_var, f, K: int) -> dict:
         ms = b.run(cfg)
         e, e_c = relmse_parts(b.read(0) / n_plain, ref, ref_var)
         b.close()
-        res["plain_batched"] = {"spp": n_plain, "ms": round(ms, 2), "item_shape": list(cfg), "relmse": e,
+        res["plain_batched"] = {"spp": n_plain, "ms": round(ms, 2), "batch_config": batch_config(cfg), "relmse": e,
                                 "relmse_minus_ref_noise": e_c,
                                 "samples_per_s": round(W * H * n_plain / (ms * 1e-3), 1)}
         log(f"budget {K}: plain batched {res['plain_batched']}")
@@ -483,14 +506,14 @@ def equal_time_budget(ctx, g, wl, args, ref, ref_var, f, K: int) -> dict:
             else:
                 rest = max(1, int((t_plain - t_train) / per_b))
                 gb = Batch(ctx, [dict(job, spp=rest, seed=14, guide=guide)])
-                gb.run(cfg)  # graph capture and tabulation on the first launch of the shape, untimed
+                gb.run(cfg)  # graph capture / pools and tabulation on the first run, untimed
                 ctx.synchronize()
                 t1 = time.perf_counter()
                 gb.run(cfg)
                 t_all = t_train + (time.perf_counter() - t1)
                 img = (f.read() + gb.read(0)) / (args.train_spp + rest)
                 gb.close()
-                extra = {"batched_spp": rest, "item_shape": list(cfg)}
+                extra = {"batched_spp": rest, "batch_config": batch_config(cfg)}
             res[form].append(score(form, prob, args.train_spp + rest, t_all, t_train, steps, img, extra))
             guide.close()
     return res

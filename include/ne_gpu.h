@@ -278,12 +278,18 @@ ne_status ne_li(ne_ctx* ctx, const ne_scene* scene, int32_t n, const float* rays
 
 /* ---- batched rendering ----------------------------------------------------
  * Many render jobs of uneven size (own scene, camera, image size, spp) in one
- * persistent kernel launch, captured in a hipGraph.  The reference renders one
- * scene per OfflineEngine (src/core/OfflineEngine.cpp:8-25, 78-145); this
- * entry point has no counterpart there.  Work items (a tile of one job x a
- * range of samples) go through one global queue; per-job sums are 64-bit
- * fixed point (2^-30 units), so a job's result is bitwise independent of the
- * batch it runs in and of the item shape.  Scenes must outlive the batch. */
+ * call.  The reference renders one scene per OfflineEngine
+ * (src/core/OfflineEngine.cpp:8-25, 78-145); this entry point has no
+ * counterpart there.  Two executors:
+ *   NE_BATCH_PERSISTENT: one persistent kernel launch, captured in a hipGraph;
+ *     work items (a tile of one job x a range of samples) go through one
+ *     global queue and each lane steps its path one bounce at a time.
+ *   NE_BATCH_WAVEFRONT: jobs, longest first, on NE_BATCH_LANES streams, each
+ *     with its own path pool, through the wavefront pipeline (the FP32
+ *     ne_render path); one job's tail overlaps the others' work.
+ * Per-job sums are 64-bit fixed point (2^-30 units), so a job's result is
+ * bitwise independent of the batch it runs in, of the item shape and of the
+ * order of the lanes (for a given executor).  Scenes must outlive the batch. */
 typedef struct ne_job {
   const ne_scene* scene;
   ne_camera_desc camera;
@@ -298,21 +304,29 @@ typedef struct ne_job {
    * as ne_render_guided does (guided+batched; no training inside the batch). */
   const struct ne_guide* guide;
 } ne_job;
-/* Work-item shape: tile_w x tile_h pixels x samples_per_item samples. */
+/* Executor and, for NE_BATCH_PERSISTENT, the work-item shape: tile_w x tile_h
+ * pixels x samples_per_item samples (ignored by NE_BATCH_WAVEFRONT). */
+enum { NE_BATCH_PERSISTENT = 0, NE_BATCH_WAVEFRONT = 1 };
+#define NE_BATCH_LANES 4 /* streams of the wavefront executor */
 typedef struct ne_batch_config {
-  int32_t tile_w, tile_h, samples_per_item, reserved;
+  int32_t tile_w, tile_h, samples_per_item;
+  int32_t executor; /* NE_BATCH_PERSISTENT or NE_BATCH_WAVEFRONT */
 } ne_batch_config;
 typedef struct ne_batch ne_batch;
 ne_status ne_batch_create(ne_ctx* ctx, const ne_job* jobs, int32_t n, ne_batch** out);
 void ne_batch_release(ne_batch* batch);
-/* Times each candidate item shape on a calibration run (every job at
- * ceil(spp / spp_divisor) samples), keeps the fastest for later runs and
- * returns it with its time. */
+/* Times each candidate (8 item shapes of the persistent kernel and the
+ * wavefront executor) on a calibration run (every job at ceil(spp /
+ * spp_divisor) samples); with spp_divisor > 1 the fastest persistent shape and
+ * the wavefront executor are then compared on one full-spp run each.  Keeps
+ * the winner for later runs and returns it with its calibration time. */
 ne_status ne_batch_autotune(ne_batch* batch, int32_t spp_divisor, ne_batch_config* best, float* best_ms);
-/* Renders every job at full spp in one graph launch (cfg NULL: the tuned
- * shape, else 8 x 8 x 4) and waits; ms = launch duration (HIP events). */
+/* Renders every job at full spp (cfg NULL: the tuned configuration, else the
+ * persistent kernel with 8 x 8 x 4 items) and waits; ms = the graph launch's
+ * duration (HIP events) or, for the wavefront executor, the wall time from the
+ * first job's start to the last one's end. */
 ne_status ne_batch_run(ne_batch* batch, const ne_batch_config* cfg, float* ms);
-/* Completion time of each job in the last run, ms from the launch start. */
+/* Completion time of each job in the last run, ms from the run's start. */
 ne_status ne_batch_job_times(ne_batch* batch, float* ms);
 /* Linear radiance sums of job `job` (width * height * 3 doubles). */
 ne_status ne_batch_read(ne_batch* batch, int32_t job, double* sum_rgb);

@@ -13,6 +13,8 @@ NE_FLAG_FIXED_FILM = 16
 NE_FLAG_SPATIAL = 32
 STAGES = ("megakernel", "plan", "regen", "trace", "medium", "shade", "shadow", "other", "guide", "train")
 NE_MODE_AUTO, NE_MODE_MEGAKERNEL, NE_MODE_WAVEFRONT = 0, 1, 2
+NE_BATCH_PERSISTENT, NE_BATCH_WAVEFRONT = 0, 1  # batch executors
+NE_BATCH_LANES = 4
 NE_MAT_GLOSSY, NE_MAT_EMITTER = 0, 1
 NE_ENV_NONE, NE_ENV_CONSTANT, NE_ENV_SKY, NE_ENV_MAP = 0, 1, 2, 3
 NE_TEST_AABB, NE_TEST_TRIANGLE, NE_TEST_SPHERE, NE_TEST_RECTANGLE = 0, 1, 2, 3
@@ -91,9 +93,9 @@ class Job(C.Structure):
 
 
 class BatchConfig(C.Structure):
-    """ne_batch_config: work-item shape."""
+    """ne_batch_config: executor and, for the persistent kernel, the work-item shape."""
     _fields_ = [("tile_w", C.c_int32), ("tile_h", C.c_int32), ("samples_per_item", C.c_int32),
-                ("reserved", C.c_int32)]
+                ("executor", C.c_int32)]
 
 
 class GuideConfig(C.Structure):

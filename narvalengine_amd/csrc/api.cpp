@@ -8,6 +8,8 @@
 #include <cstdio>
 #include <cstring>
 #include <atomic>
+#include <chrono>
+#include <thread>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -1309,6 +1311,16 @@ ne_status ne_test_rng(ne_ctx* c, uint64_t seed, uint32_t pixel, uint32_t sample,
 }  // extern "C"
 
 // ---- batched rendering -------------------------------------------------------
+// A stream of the wavefront executor, with its own path pool and status word.
+struct BatchLane {
+  hipStream_t stream = nullptr;
+  void* pool = nullptr;
+  uint32_t pool_P = 0;
+  void* sbuf = nullptr;  // per-sample radiance slots (12 B per sample of the lane's largest job)
+  size_t sbuf_bytes = 0;
+  uint32_t* status = nullptr;  // pinned host words read by the wavefront loop
+};
+
 struct ne_batch {
   ne_ctx* ctx = nullptr;
   std::vector<ne_job> jobs;
@@ -1330,6 +1342,7 @@ struct ne_batch {
   double ticks_per_ms = 0;
   bool surfaces = true;  // some job has surfaces or emitters (the general batch kernel)
   bool guided = false;   // some job samples a guide's tabulated proposal
+  std::vector<BatchLane> lanes;  // wavefront executor (created on its first run)
 };
 
 namespace {
@@ -1343,11 +1356,19 @@ void batch_free(ne_batch* b) {
   if (b->d_item_time) (void)hipFree(b->d_item_time);
   if (b->ev0) (void)hipEventDestroy(b->ev0);
   if (b->ev1) (void)hipEventDestroy(b->ev1);
+  for (BatchLane& L : b->lanes) {
+    if (L.stream) (void)hipStreamDestroy(L.stream);
+    if (L.pool) (void)hipFree(L.pool);
+    if (L.sbuf) (void)hipFree(L.sbuf);
+    if (L.status) (void)hipHostFree(L.status);
+  }
+  b->lanes.clear();
 }
 
 bool cfg_valid(const ne_batch_config& c) {
-  return c.tile_w >= 1 && c.tile_h >= 1 && c.tile_w * c.tile_h <= 4096 && c.samples_per_item >= 1 &&
-         c.samples_per_item <= 65536;
+  if (c.executor == NE_BATCH_WAVEFRONT) return true;  // no item shape
+  return c.executor == NE_BATCH_PERSISTENT && c.tile_w >= 1 && c.tile_h >= 1 && c.tile_w * c.tile_h <= 4096 &&
+         c.samples_per_item >= 1 && c.samples_per_item <= 65536;
 }
 
 // Builds the job table for an item shape and sample divisor and (re)captures
@@ -1451,6 +1472,169 @@ ne_status batch_launch(ne_batch* b, const ne_batch_config& cfg, int div, float* 
   return NE_OK;
 }
 
+// Path-pool slots and sample count of job j at ceil(spp / div) samples, as
+// ne_render sizes them.
+static uint64_t lane_job_pool(const ne_job& J, int div, uint64_t* samples) {
+  const DevScene& ds = J.scene->dev;
+  const uint64_t total = uint64_t(J.width) * uint64_t(J.height) * uint64_t((J.spp + div - 1) / div);
+  const uint64_t def_pool = (ds.n_nodes > 0 && !ds.has_volume) ? (uint64_t(1) << 23) : (uint64_t(1) << 22);
+  const uint64_t P = std::min<uint64_t>(def_pool, total);
+  if (samples) *samples = total;
+  return (P + kChunk - 1) / kChunk * kChunk;
+}
+
+// One job through the wavefront pipeline on a lane (called from the lane's
+// own host thread): the FP32 ne_render path at ceil(spp / div) samples into
+// the job's fixed-point batch film, a guided job with its frozen tabulated
+// proposal.  A job whose samples fit in kMaxSampleBuffer uses per-sample
+// slots, summed into the film at the end (a property of the job alone, so
+// its result does not depend on the batch); larger ones take fixed-point
+// atomics.  The lane's pool and slots hold the job (sized before the run).
+hipError_t lane_render(ne_batch* b, BatchLane& L, size_t j, int div, uint32_t refill_blocks) {
+  const ne_job& J = b->jobs[j];
+  const DevScene& ds = J.scene->dev;
+  uint64_t total = 0;
+  const uint64_t P = lane_job_pool(J, div, &total);
+  if (P > L.pool_P) return hipErrorInvalidValue;
+  hipError_t e;
+  WfState w{};
+  wf_carve(L.pool, uint32_t(P), w);
+  WfArgs a{};
+  a.W = J.width; a.H = J.height; a.x0 = 0; a.y0 = 0; a.x1 = J.width; a.y1 = J.height;
+  a.max_bounces = J.max_bounces; a.sample_offset = 0; a.job = J.job; a.nchunks = uint32_t(P / kChunk);
+  a.seed = J.seed;
+  a.sort = 0;
+  a.refill_min = kRefillMinIdle;
+  a.refill = std::min<uint32_t>(kMaxRefill, std::max<uint32_t>(1, (a.nchunks + refill_blocks - 1) / refill_blocks));
+  a.total_paths = total;
+  a.counters = nullptr;
+  a.film_fixed = b->d_films + b->film_off[j];
+  a.film = nullptr;  // the fixed-point sums stay in the batch film
+  const size_t sbytes = size_t(total) * 3 * sizeof(float);
+  a.sbuf = nullptr;
+  if (sbytes <= kMaxSampleBuffer) {
+    if (sbytes > L.sbuf_bytes) return hipErrorInvalidValue;
+    a.sbuf = static_cast<float*>(L.sbuf);
+  }
+  a.gstate[0] = a.gstate[1] = nullptr;
+  WfGuideHook hook{};
+  WfGuideHook* gh = nullptr;
+  if (J.guide) {  // frozen: no training records, no steps
+    hook.g.net = J.guide->net;
+    hook.g.batch = J.guide->batch;
+    hook.g.train = 0;
+    hook.g.max_depth = J.guide->cfg.guided_vertices;
+    hook.g.prob = J.guide->cfg.guide_prob;
+    hook.g.cache = J.guide->cache;
+    gh = &hook;
+  }
+  if ((e = launch_wavefront(ds, make_camera(&J.camera), w, a, false, L.stream, L.status, nullptr, gh)) != hipSuccess)
+    return e;
+  return hipStreamSynchronize(L.stream);
+}
+
+// The wavefront executor: NE_BATCH_LANES host threads, one per lane, take the
+// jobs longest first (pixels x spp x max bounces) from a shared counter.
+ne_status batch_run_lanes(ne_batch* b, int div, float* ms) {
+  ne_ctx* c = b->ctx;
+  const size_t n = b->jobs.size();
+  const int nl = int(std::min<size_t>(NE_BATCH_LANES, n));
+  for (int i = int(b->lanes.size()); i < nl; ++i) {
+    BatchLane L;
+    NE_HIP(c, hipStreamCreate(&L.stream));
+    b->lanes.push_back(L);  // owned by the batch from here on
+    void* st = nullptr;
+    NE_HIP(c, hipHostMalloc(&st, 64));
+    b->lanes.back().status = static_cast<uint32_t*>(st);
+  }
+  if (c->refill_blocks == 0) {
+    int cus = 0, per_cu = 0;
+    NE_HIP(c, hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c->device));
+    NE_HIP(c, wf_refill_blocks_per_cu(&per_cu));
+    c->refill_blocks = uint32_t(std::max(1, cus * std::max(1, per_cu)));
+  }
+  // every lane can take any job: pools and sample slots for the largest, made
+  // before the timed run (slots for the jobs that use them, lane_render)
+  uint64_t Pmax = 0;
+  size_t sbytes = 0;
+  for (const ne_job& J : b->jobs) {
+    uint64_t t = 0;
+    Pmax = std::max(Pmax, lane_job_pool(J, div, &t));
+    const size_t jb = size_t(t) * 3 * sizeof(float);
+    if (jb <= kMaxSampleBuffer) sbytes = std::max(sbytes, jb);
+  }
+  for (int i = 0; i < nl; ++i) {
+    BatchLane& L = b->lanes[i];
+    if (Pmax > L.pool_P) {
+      void* old = L.pool;
+      L.pool = nullptr;
+      L.pool_P = 0;
+      if (old) NE_HIP(c, hipFree(old));
+      void* nb = nullptr;
+      NE_HIP(c, hipMalloc(&nb, wf_bytes(uint32_t(Pmax))));
+      L.pool = nb;
+      L.pool_P = uint32_t(Pmax);
+    }
+    if (sbytes > L.sbuf_bytes) {
+      void* old = L.sbuf;
+      L.sbuf = nullptr;
+      L.sbuf_bytes = 0;
+      if (old) NE_HIP(c, hipFree(old));
+      void* nb = nullptr;
+      NE_HIP(c, hipMalloc(&nb, sbytes));
+      L.sbuf = nb;
+      L.sbuf_bytes = sbytes;
+    }
+  }
+  ne_status st;
+  for (const ne_job& J : b->jobs)
+    if (J.guide && (st = guide_refresh_cache(c, const_cast<ne_guide*>(J.guide))) != NE_OK) return st;
+  NE_HIP(c, hipMemsetAsync(b->d_films, 0, b->film_words * sizeof(long long), c->stream));
+  NE_HIP(c, hipStreamSynchronize(c->stream));
+  std::vector<size_t> order(n);
+  for (size_t j = 0; j < n; ++j) order[j] = j;
+  auto work = [&](size_t j) {
+    const ne_job& J = b->jobs[j];
+    return double(J.width) * J.height * J.spp * double(J.max_bounces + 1);
+  };
+  std::stable_sort(order.begin(), order.end(), [&](size_t x, size_t y) { return work(x) > work(y); });
+  std::atomic<size_t> next{0};
+  std::atomic<int> failed{int(hipSuccess)};
+  std::vector<float> done(n, 0.0f);
+  const uint32_t rb = c->refill_blocks;
+  const int device = c->device;
+  const auto t0 = std::chrono::steady_clock::now();
+  auto worker = [&](int i) {
+    hipError_t e = hipSetDevice(device);
+    while (e == hipSuccess && failed.load() == int(hipSuccess)) {
+      const size_t k = next.fetch_add(1);
+      if (k >= n) break;
+      e = lane_render(b, b->lanes[i], order[k], div, rb);
+      done[order[k]] = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    }
+    if (e != hipSuccess) {
+      int expect = int(hipSuccess);
+      failed.compare_exchange_strong(expect, int(e));
+    }
+  };
+  std::vector<std::thread> threads;
+  for (int i = 0; i < nl; ++i) threads.emplace_back(worker, i);
+  for (std::thread& t : threads) t.join();
+  const float wall = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (failed.load() != int(hipSuccess)) {
+    const hipError_t e = hipError_t(failed.load());
+    return fail(c, e == hipErrorOutOfMemory ? NE_ERR_OUT_OF_MEMORY : NE_ERR_HIP,
+                std::string("batch wavefront lane: ") + hipGetErrorString(e));
+  }
+  if (ms) *ms = wall;
+  b->job_ms = done;
+  return NE_OK;
+}
+
+ne_status batch_execute(ne_batch* b, const ne_batch_config& cfg, int div, float* ms) {
+  return cfg.executor == NE_BATCH_WAVEFRONT ? batch_run_lanes(b, div, ms) : batch_launch(b, cfg, div, ms);
+}
+
 }  // namespace
 
 ne_status ne_batch_create(ne_ctx* c, const ne_job* jobs, int32_t n, ne_batch** out) {
@@ -1521,22 +1705,44 @@ ne_status ne_batch_autotune(ne_batch* b, int32_t div, ne_batch_config* best, flo
   if (div < 1) return fail(c, NE_ERR_INVALID_ARG, "spp_divisor must be >= 1");
   std::lock_guard<std::mutex> lk(c->mu);
   NE_HIP(c, hipSetDevice(c->device));
-  // candidate item shapes: square and wide tiles, few to many samples per item
-  static const ne_batch_config cands[] = {{8, 8, 1, 0},  {8, 8, 4, 0},  {8, 8, 16, 0},  {16, 4, 4, 0},
-                                          {16, 4, 16, 0}, {16, 16, 1, 0}, {16, 16, 4, 0}, {32, 8, 4, 0}};
-  float bms = 0;
-  ne_batch_config bc = cands[0];
+  // candidates: the persistent kernel with square and wide tiles, few to many
+  // samples per item, and the wavefront executor
+  static const ne_batch_config cands[] = {
+      {8, 8, 1, NE_BATCH_PERSISTENT},   {8, 8, 4, NE_BATCH_PERSISTENT},   {8, 8, 16, NE_BATCH_PERSISTENT},
+      {16, 4, 4, NE_BATCH_PERSISTENT},  {16, 4, 16, NE_BATCH_PERSISTENT}, {16, 16, 1, NE_BATCH_PERSISTENT},
+      {16, 16, 4, NE_BATCH_PERSISTENT}, {32, 8, 4, NE_BATCH_PERSISTENT},  {8, 8, 4, NE_BATCH_WAVEFRONT}};
+  float pms = 0, wms = 0;  // calibration times: fastest persistent shape, wavefront executor
+  ne_batch_config pc = cands[0], wc = cands[0];
   bool first = true;
   for (const ne_batch_config& cfg : cands) {
     float ms = 0;
-    ne_status st = batch_launch(b, cfg, div, nullptr);  // graph build + warm-up
-    if (st == NE_OK) st = batch_launch(b, cfg, div, &ms);
+    ne_status st = batch_execute(b, cfg, div, nullptr);  // graph build / pools + warm-up
+    if (st == NE_OK) st = batch_execute(b, cfg, div, &ms);
     if (st != NE_OK) return st;
-    if (first || ms < bms) { bms = ms; bc = cfg; first = false; }
+    if (cfg.executor == NE_BATCH_WAVEFRONT) {
+      wms = ms;
+      wc = cfg;
+    } else if (first || ms < pms) {
+      pms = ms;
+      pc = cfg;
+      first = false;
+    }
   }
-  b->tuned = bc;
-  if (best) *best = bc;
-  if (best_ms) *best_ms = bms;
+  // The wavefront executor's per-job costs do not shrink with the spp, so a
+  // calibration run favours the persistent kernel: the two are compared again
+  // on one full-spp run each.
+  bool wave = wms < pms;
+  if (div > 1) {
+    float pf = 0, wf = 0;
+    ne_status st = batch_execute(b, pc, 1, &pf);
+    if (st == NE_OK) st = batch_execute(b, wc, 1, nullptr);  // pools sized for full spp before timing
+    if (st == NE_OK) st = batch_execute(b, wc, 1, &wf);
+    if (st != NE_OK) return st;
+    wave = wf < pf;
+  }
+  b->tuned = wave ? wc : pc;
+  if (best) *best = b->tuned;
+  if (best_ms) *best_ms = wave ? wms : pms;
   return NE_OK;
 }
 
@@ -1547,7 +1753,7 @@ ne_status ne_batch_run(ne_batch* b, const ne_batch_config* cfg, float* ms) {
   if (!cfg_valid(use)) return fail(c, NE_ERR_INVALID_ARG, "batch item shape out of range");
   std::lock_guard<std::mutex> lk(c->mu);
   NE_HIP(c, hipSetDevice(c->device));
-  return batch_launch(b, use, 1, ms);
+  return batch_execute(b, use, 1, ms);
 }
 
 ne_status ne_batch_job_times(ne_batch* b, float* ms) {

@@ -124,7 +124,9 @@ struct WfArgs {
 
 struct WfGuideHook;  // guided mode: proposal, training records, online training
 // Renders a.total_paths paths through a pool of a.nchunks * kChunk slots;
-// returns after the film flush is enqueued.  h_status: pinned host words.
+// returns after the film flush is enqueued.  a.film == nullptr: the sums stay
+// in a.film_fixed (2^-30 fixed point; per-sample slots, if any, are summed into
+// it).  h_status: pinned host words.
 hipError_t launch_wavefront(const DevScene& s, const DevCamera& cam, WfState& w, WfArgs a, bool count,
                             hipStream_t st, uint32_t* h_status, StageTimer* tm, WfGuideHook* gh);
 // ---- persistent batched renderer -------------------------------------------

@@ -968,6 +968,26 @@ __global__ void k_wf_flush_samples(const float* sbuf, long long* fixed, double* 
   fx[2] = 0;
 }
 
+// Batch films keep 2^-30 fixed-point sums: each pixel's samples are summed
+// as above and the sum, rounded once, joins the pixel's fixed-point sum.
+__global__ void k_wf_flush_samples_fixed(const float* sbuf, long long* fixed, int W, int x0, int y0, int rw, int rh,
+                                         uint32_t spp) {
+  const size_t p = size_t(blockIdx.x) * blockDim.x + threadIdx.x, npix = size_t(rw) * size_t(rh);
+  if (p >= npix) return;
+  double r = 0.0, g = 0.0, b = 0.0;
+  for (uint32_t s = 0; s < spp; ++s) {
+    const float* q = sbuf + 3 * (size_t(s) * npix + p);
+    r += double(q[0]);
+    g += double(q[1]);
+    b += double(q[2]);
+  }
+  const size_t pix = (size_t(y0) + p / size_t(rw)) * size_t(W) + size_t(x0) + p % size_t(rw);
+  long long* fx = fixed + 3 * pix;
+  fx[0] += __double2ll_rn(r * 1073741824.0);
+  fx[1] += __double2ll_rn(g * 1073741824.0);
+  fx[2] += __double2ll_rn(b * 1073741824.0);
+}
+
 __global__ void k_wf_flush_film(long long* fixed, double* film, size_t n) {
   size_t i = size_t(blockIdx.x) * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -1150,9 +1170,13 @@ hipError_t launch_wavefront(const DevScene& s, const DevCamera& cam, WfState& w,
     const int rw = a.x1 - a.x0, rh = a.y1 - a.y0;
     const size_t npix = size_t(rw) * size_t(rh);
     const uint32_t spp = uint32_t(a.total_paths / npix);
-    hipLaunchKernelGGL(k_wf_flush_samples, dim3(uint32_t((npix + 255) / 256)), dim3(256), 0, st, a.sbuf,
-                       a.film_fixed, a.film, a.W, a.x0, a.y0, rw, rh, spp);
-  } else {
+    if (a.film)
+      hipLaunchKernelGGL(k_wf_flush_samples, dim3(uint32_t((npix + 255) / 256)), dim3(256), 0, st, a.sbuf,
+                         a.film_fixed, a.film, a.W, a.x0, a.y0, rw, rh, spp);
+    else  // the caller keeps the fixed-point sums (batch films)
+      hipLaunchKernelGGL(k_wf_flush_samples_fixed, dim3(uint32_t((npix + 255) / 256)), dim3(256), 0, st, a.sbuf,
+                         a.film_fixed, a.W, a.x0, a.y0, rw, rh, spp);
+  } else if (a.film) {  // a.film == nullptr: the caller keeps the fixed-point sums (batch films)
     hipLaunchKernelGGL(k_wf_flush_film, dim3(uint32_t((nf + 255) / 256)), dim3(256), 0, st, a.film_fixed, a.film, nf);
   }
   NE_MARK(kStageEnd);

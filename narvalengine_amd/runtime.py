@@ -229,9 +229,11 @@ __all__ = ["Context", "GpuScene", "Film", "NarvalError"]
 
 
 class Batch:
-    """Many render jobs in one persistent launch (ne_batch).  jobs: list of
-    dicts with scene (GpuScene), camera, width, height, spp, max_bounces and
-    optional seed / job."""
+    """Many render jobs in one call (ne_batch).  jobs: list of dicts with
+    scene (GpuScene), camera, width, height, spp, max_bounces and optional
+    seed / job / guide.  A configuration is (tile_w, tile_h, samples_per_item)
+    for the persistent kernel, or (tile_w, tile_h, samples_per_item, executor)
+    with executor NE_BATCH_PERSISTENT or NE_BATCH_WAVEFRONT."""
 
     def __init__(self, ctx: Context, jobs: list):
         self.ctx = ctx
@@ -253,13 +255,13 @@ class Batch:
     def autotune(self, spp_divisor: int = 8):
         cfg, ms = BatchConfig(), C.c_float()
         self.ctx.check(self.ctx._lib.ne_batch_autotune(self.h, spp_divisor, C.byref(cfg), C.byref(ms)))
-        return (cfg.tile_w, cfg.tile_h, cfg.samples_per_item), float(ms.value)
+        return (cfg.tile_w, cfg.tile_h, cfg.samples_per_item, cfg.executor), float(ms.value)
 
     def run(self, cfg=None) -> float:
         ms = C.c_float()
         p = None
         if cfg is not None:
-            c = BatchConfig(cfg[0], cfg[1], cfg[2], 0)
+            c = BatchConfig(cfg[0], cfg[1], cfg[2], cfg[3] if len(cfg) > 3 else 0)
             p = C.byref(c)
         self.ctx.check(self.ctx._lib.ne_batch_run(self.h, p, C.byref(ms)))
         return float(ms.value)

@@ -1,6 +1,7 @@
-"""Batched persistent renderer: per-job sums bitwise independent of the batch
-and of the work-item shape, equal to the single-job render up to the film's
-fixed-point quantisation, and statistically equal to the oracle."""
+"""Batched rendering, both executors (persistent kernel, wavefront lanes):
+per-job sums bitwise independent of the batch and of the work-item shape or
+lane order, equal to the single-job render up to the film's fixed-point
+quantisation, and statistically equal to the oracle."""
 import numpy as np
 import pytest
 
@@ -8,7 +9,7 @@ import oracle as O
 import scenes
 from conftest import print_case
 from narvalengine_amd import Batch, configs
-from narvalengine_amd.abi import NE_FP32, NE_MODE_MEGAKERNEL
+from narvalengine_amd.abi import NE_BATCH_WAVEFRONT, NE_FP32, NE_MODE_MEGAKERNEL, NE_MODE_WAVEFRONT
 from narvalengine_amd.scene import Camera
 
 pytestmark = pytest.mark.gpu
@@ -59,6 +60,44 @@ def test_batch_matches_the_single_job_render(ctx):
                    job=j["job"], mode=NE_MODE_MEGAKERNEL)
         ref = f.read()
         np.testing.assert_allclose(b.read(k), ref, rtol=1e-6, atol=j["spp"] * 2.0 ** -30)
+    b.close()
+
+
+def test_wavefront_executor_is_bitwise_independent_of_batching(ctx):
+    """The wavefront executor runs the jobs on several streams, longest first:
+    which lane takes a job, and what runs beside it, leaves its sums bitwise
+    unchanged (fixed-point film)."""
+    print_case("batch wavefront executor vs single-job batches")
+    jobs = _jobs(ctx)
+    wf = (8, 8, 4, NE_BATCH_WAVEFRONT)
+    b = Batch(ctx, jobs)
+    assert b.run(wf) > 0
+    full = [b.read(k) for k in range(len(jobs))]
+    b.run(wf)
+    for k in range(len(jobs)):
+        np.testing.assert_array_equal(b.read(k), full[k])
+    for k, j in enumerate(jobs):
+        single = Batch(ctx, [j])
+        single.run(wf)
+        np.testing.assert_array_equal(single.read(0), full[k])
+        single.close()
+    t = b.job_times()
+    assert (t > 0).all()
+    b.close()
+
+
+def test_wavefront_executor_matches_the_wavefront_render(ctx):
+    """Same pipeline and paths as ne_render's wavefront; only the accumulation
+    differs (2^-30 fixed point for every path against FP64 per-sample sums)."""
+    print_case("batch wavefront executor vs wavefront render")
+    jobs = _jobs(ctx)
+    b = Batch(ctx, jobs)
+    b.run((8, 8, 4, NE_BATCH_WAVEFRONT))
+    for k, j in enumerate(jobs):
+        f = ctx.film(j["width"], j["height"])
+        ctx.render(j["scene"], j["camera"], f, spp=j["spp"], max_bounces=j["max_bounces"], seed=j["seed"],
+                   job=j["job"], mode=NE_MODE_WAVEFRONT)
+        np.testing.assert_allclose(b.read(k), f.read(), rtol=1e-6, atol=4 * j["spp"] * 2.0 ** -30)
     b.close()
 
 

@@ -342,6 +342,36 @@ def test_guided_batch_job_equals_its_single_job_render(ctx):
     g.close()
 
 
+def test_guided_job_in_the_wavefront_executor(ctx):
+    """Guided + batched through the wavefront executor: the guided job equals
+    its frozen guided wavefront render up to the fixed-point film, and the
+    plain jobs beside it are bitwise those of a batch without it."""
+    print_case("guided batch job, wavefront executor")
+    from narvalengine_amd import Batch
+    from narvalengine_amd.abi import NE_BATCH_WAVEFRONT, NE_MODE_WAVEFRONT
+    import scenes
+    s, cam = _cloud()
+    gs = ctx.upload(s)
+    g = _trained_cached_guide(ctx, gs, cam, guided_vertices=4, guide_prob=0.5)
+    other = ctx.upload(scenes.small_volume(res=16, maj_res=2))
+    jobs = [dict(scene=other, camera=scenes.VOLUME_CAMERA, width=24, height=16, spp=8, max_bounces=16, seed=3, job=1),
+            dict(scene=gs, camera=cam, width=40, height=24, spp=12, max_bounces=64, seed=5, job=2, guide=g),
+            dict(scene=other, camera=scenes.VOLUME_CAMERA, width=16, height=16, spp=4, max_bounces=16, seed=4, job=3)]
+    wf = (8, 8, 4, NE_BATCH_WAVEFRONT)
+    b = Batch(ctx, jobs)
+    b.run(wf)
+    f = ctx.film(40, 24)
+    ctx.render(gs, cam, f, spp=12, max_bounces=64, seed=5, job=2, guide=g, mode=NE_MODE_WAVEFRONT)
+    np.testing.assert_allclose(b.read(1), f.read(), rtol=1e-6, atol=12 * 2.0 ** -30 * 4)
+    plain = Batch(ctx, [jobs[0], jobs[2]])
+    plain.run(wf)
+    np.testing.assert_array_equal(plain.read(0), b.read(0))
+    np.testing.assert_array_equal(plain.read(1), b.read(2))
+    for x in (b, plain):
+        x.close()
+    g.close()
+
+
 def test_tabulated_guided_render_is_unbiased(ctx):
     """The tabulated proposal leaves the expectation unchanged, as the
     per-vertex network does (the first four vertices of every path guided)."""
