@@ -101,6 +101,35 @@ def test_wavefront_executor_matches_the_wavefront_render(ctx):
     b.close()
 
 
+def test_both_executors_on_mixed_scenes(ctx):
+    """Surface, emitter, homogeneous-medium and grid-volume jobs in one batch:
+    each executor matches its single-job render (the persistent kernel the
+    megakernel, the wavefront executor the wavefront) up to the fixed-point
+    film, and the two executors agree within the same bound."""
+    print_case("batch executors on mixed scenes")
+    specs = [(scenes.small_surface(subdiv=2), scenes.BOX_CAMERA, 24, 24, 6, 6),
+             (scenes.emitter_mix(), scenes.BOX_CAMERA, 20, 16, 5, 6),
+             (scenes.homogeneous_box(), scenes.BOX_CAMERA, 16, 20, 4, 8),
+             (scenes.small_volume(res=16, maj_res=2), scenes.VOLUME_CAMERA, 18, 14, 7, 12)]
+    jobs = [dict(scene=ctx.upload(s), camera=cam, width=w, height=h, spp=spp, max_bounces=mb, seed=40 + k, job=k)
+            for k, (s, cam, w, h, spp, mb) in enumerate(specs)]
+    per = Batch(ctx, jobs)
+    per.run((8, 8, 2))
+    wf = Batch(ctx, jobs)
+    wf.run((8, 8, 2, NE_BATCH_WAVEFRONT))
+    for k, j in enumerate(jobs):
+        tol = dict(rtol=1e-6, atol=4 * j["spp"] * 2.0 ** -30)
+        for mode, b in ((NE_MODE_MEGAKERNEL, per), (NE_MODE_WAVEFRONT, wf)):
+            f = ctx.film(j["width"], j["height"])
+            ctx.render(j["scene"], j["camera"], f, spp=j["spp"], max_bounces=j["max_bounces"], seed=j["seed"],
+                       job=j["job"], mode=mode)
+            np.testing.assert_allclose(b.read(k), f.read(), **tol)
+        np.testing.assert_allclose(per.read(k), wf.read(k), rtol=2e-6, atol=8 * j["spp"] * 2.0 ** -30)
+        assert per.read(k).sum() > 0
+    per.close()
+    wf.close()
+
+
 def test_batch_matches_the_oracle(ctx):
     print_case("batch vs oracle")
     jobs = _jobs(ctx)[:2]
