@@ -1541,11 +1541,15 @@ ne_status batch_run_lanes(ne_batch* b, int div, float* ms) {
   const int nl = int(std::min<size_t>(NE_BATCH_LANES, n));
   for (int i = int(b->lanes.size()); i < nl; ++i) {
     BatchLane L;
-    NE_HIP(c, hipStreamCreate(&L.stream));
-    b->lanes.push_back(L);  // owned by the batch from here on
     void* st = nullptr;
-    NE_HIP(c, hipHostMalloc(&st, 64));
-    b->lanes.back().status = static_cast<uint32_t*>(st);
+    hipError_t e = hipStreamCreate(&L.stream);
+    if (e == hipSuccess) e = hipHostMalloc(&st, 64);
+    if (e != hipSuccess) {  // a lane is kept only whole
+      if (L.stream) (void)hipStreamDestroy(L.stream);
+      return fail(c, NE_ERR_HIP, std::string("batch lane set-up: ") + hipGetErrorString(e));
+    }
+    L.status = static_cast<uint32_t*>(st);
+    b->lanes.push_back(L);
   }
   if (c->refill_blocks == 0) {
     int cus = 0, per_cu = 0;
@@ -1618,7 +1622,13 @@ ne_status batch_run_lanes(ne_batch* b, int div, float* ms) {
     }
   };
   std::vector<std::thread> threads;
-  for (int i = 0; i < nl; ++i) threads.emplace_back(worker, i);
+  for (int i = 0; i < nl; ++i) {
+    try {
+      threads.emplace_back(worker, i);
+    } catch (...) {
+      worker(i);  // no thread to spare: this one serves the lane (until the queue is empty)
+    }
+  }
   for (std::thread& t : threads) t.join();
   const float wall = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
   if (failed.load() != int(hipSuccess)) {
