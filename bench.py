@@ -52,8 +52,15 @@ RAY_BYTES = {"trace": 36, "medium": 52, "shadow": 88}
 SHADE_IN, SHADE_OUT = 84, 64
 
 
-def stage_bytes(stage: str, c: dict, film_bytes: int) -> int:
-    """Algorithmic bytes a stage moves over one frame, from the event counters."""
+# a new path's record: origin, direction, throughput, radiance (48 B), pixel,
+# sample, RNG count, depth (16 B), previous pdf (4 B)
+REGEN_BYTES = 68
+
+
+def stage_bytes(stage: str, c: dict, film_bytes: int, regen_in_medium: bool = False) -> int:
+    """Algorithmic bytes a stage moves over one frame, from the event counters.
+    regen_in_medium: volume-only scenes make their new paths inside the medium
+    stage (no separate regen launch), whose bytes then count there."""
     ext = {k: c[k] - c["shadow_" + k] for k in EV}
     sh = {k: c["shadow_" + k] for k in EV}
     if stage == "megakernel":
@@ -61,7 +68,8 @@ def stage_bytes(stage: str, c: dict, film_bytes: int) -> int:
     if stage == "trace":
         return ext["bvh_nodes"] * EV["bvh_nodes"] + ext["prim_tests"] * 48 + c["ext_rays"] * RAY_BYTES["trace"]
     if stage == "medium":
-        return ext["density_lookups"] * 32 + ext["majorant_cells"] * 4 + c["ext_rays"] * RAY_BYTES["medium"]
+        return (ext["density_lookups"] * 32 + ext["majorant_cells"] * 4 + c["ext_rays"] * RAY_BYTES["medium"] +
+                (c["paths"] * REGEN_BYTES if regen_in_medium else 0))
     if stage == "shadow":
         return sum(sh[k] * b for k, b in EV.items()) + c["shadow_rays"] * RAY_BYTES["shadow"]
     if stage == "shade":
@@ -935,7 +943,7 @@ def main():
                 continue
             per_launch_ms = stage_ms[name] / stage_n[name]
             launches_per_frame = stage_n[name] / args.steps
-            b = stage_bytes(name, counters, film_bytes) / launches_per_frame
+            b = stage_bytes(name, counters, film_bytes, regen_in_medium=not stage_n.get("regen")) / launches_per_frame
             stages[name] = {"ms_per_frame": round(stage_ms[name] / args.steps, 3),
                             "launches_per_frame": launches_per_frame,
                             "avg_launch_ms": round(per_launch_ms, 4),

@@ -247,11 +247,11 @@ __global__ __launch_bounds__(kPlanThreads) void k_wf_plan(WfState w, WfArgs a) {
   }
 }
 
-// New paths fill the free tail of each chunk.
-__global__ __launch_bounds__(kChunk) void k_wf_regen(WfState w, WfArgs a, DevCamera cam) {
+// New paths fill the free tail of chunk c: thread t < m writes the camera ray
+// of path regen_base[c] + t into slot n0 + t.
+__device__ __forceinline__ void regen_chunk(const WfState& w, const WfArgs& a, const DevCamera& cam, uint32_t c,
+                                            uint32_t t, uint32_t n0, uint32_t m) {
   const WfPaths A = w.paths[a.cur];
-  const uint32_t c = blockIdx.x, t = threadIdx.x;
-  const uint32_t n0 = w.cnt_paths[a.cur][c], m = w.regen_m[c];
   if (t < m) {
     const uint32_t slot = c * kChunk + n0 + t;
     uint32_t pix, smp;
@@ -275,6 +275,12 @@ __global__ __launch_bounds__(kChunk) void k_wf_regen(WfState w, WfArgs a, DevCam
     A.prev_pdf[slot] = 0.0f;
     if (a.gstate[a.cur]) a.gstate[a.cur][slot] = 0u;
   }
+}
+
+__global__ __launch_bounds__(kChunk) void k_wf_regen(WfState w, WfArgs a, DevCamera cam) {
+  const uint32_t c = blockIdx.x, t = threadIdx.x;
+  const uint32_t n0 = w.cnt_paths[a.cur][c], m = w.regen_m[c];
+  regen_chunk(w, a, cam, c, t, n0, m);
   __syncthreads();  // every thread has read n0
   if (t == 0) w.cnt_paths[a.cur][c] = n0 + m;
 }
@@ -392,13 +398,24 @@ __global__ __launch_bounds__(kChunk) void k_wf_trace(WfState w, WfArgs a, DevSce
 // SPATIAL: the chunk's rays are taken in Morton order of their origins, so a
 // wave tracks through neighbouring voxels (each path keeps its own stream:
 // same trajectories).
-template <bool C, bool HAS_SURF, bool SPATIAL>
-__global__ __launch_bounds__(kChunk) void k_wf_medium(WfState w, WfArgs a, DevScene s) {
+// REGEN (volume-only scenes): the block first makes its chunk's new paths, as
+// k_wf_regen would, which saves that stage's launch every iteration.
+template <bool C, bool HAS_SURF, bool SPATIAL, bool REGEN>
+__global__ __launch_bounds__(kChunk) void k_wf_medium(WfState w, WfArgs a, DevScene s, DevCamera cam) {
   __shared__ uint32_t red[kChunk / 64];
   __shared__ uint32_t skey[SPATIAL ? kChunk : 1];
   const WfPaths A = w.paths[a.cur];
   const uint32_t c = blockIdx.x, t = threadIdx.x;
-  const uint32_t n = w.cnt_paths[a.cur][c];
+  uint32_t n;
+  if constexpr (REGEN) {
+    const uint32_t n0 = w.cnt_paths[a.cur][c], m = w.regen_m[c];
+    regen_chunk(w, a, cam, c, t, n0, m);
+    __syncthreads();  // the new paths' records are visible to the whole block; every thread has read n0
+    if (t == 0) w.cnt_paths[a.cur][c] = n0 + m;
+    n = n0 + m;
+  } else {
+    n = w.cnt_paths[a.cur][c];
+  }
   uint32_t slot = t;
   if constexpr (SPATIAL) {
     const uint32_t q = c * kChunk + t;
@@ -1111,8 +1128,11 @@ hipError_t launch_wavefront(const DevScene& s, const DevCamera& cam, WfState& w,
         hipLaunchKernelGGL(k_guide_reset_count, dim3(1), dim3(1), 0, st, gh->g.batch.count);
       }
     }
-    NE_MARK(kStageRegen);
-    hipLaunchKernelGGL(k_wf_regen, grid, blk, 0, st, w, a, cam);
+    const bool fused = hv && !hs && !spatial;  // regen inside the medium stage
+    if (!fused) {
+      NE_MARK(kStageRegen);
+      hipLaunchKernelGGL(k_wf_regen, grid, blk, 0, st, w, a, cam);
+    }
     if (hs) {
       NE_MARK(kStageTrace);
       if (count) hipLaunchKernelGGL(k_wf_trace<true>, pgrid, blk, 0, st, w, a, s);
@@ -1120,13 +1140,14 @@ hipError_t launch_wavefront(const DevScene& s, const DevCamera& cam, WfState& w,
     }
     if (hv) {
       NE_MARK(kStageMedium);
-#define NE_MEDIUM(CNT, S_, SP) hipLaunchKernelGGL((k_wf_medium<CNT, S_, SP>), grid, blk, 0, st, w, a, s)
+#define NE_MEDIUM(CNT, S_, SP, RG) \
+  hipLaunchKernelGGL((k_wf_medium<CNT, S_, SP, RG>), grid, blk, 0, st, w, a, s, cam)
       if (spatial) {
-        if (count) { if (hs) NE_MEDIUM(true, true, true); else NE_MEDIUM(true, false, true); }
-        else { if (hs) NE_MEDIUM(false, true, true); else NE_MEDIUM(false, false, true); }
+        if (count) { if (hs) NE_MEDIUM(true, true, true, false); else NE_MEDIUM(true, false, true, false); }
+        else { if (hs) NE_MEDIUM(false, true, true, false); else NE_MEDIUM(false, false, true, false); }
       } else {
-        if (count) { if (hs) NE_MEDIUM(true, true, false); else NE_MEDIUM(true, false, false); }
-        else { if (hs) NE_MEDIUM(false, true, false); else NE_MEDIUM(false, false, false); }
+        if (count) { if (hs) NE_MEDIUM(true, true, false, false); else NE_MEDIUM(true, false, false, true); }
+        else { if (hs) NE_MEDIUM(false, true, false, false); else NE_MEDIUM(false, false, false, true); }
       }
 #undef NE_MEDIUM
     }
