@@ -170,7 +170,8 @@ typedef struct ne_render_params {
   int32_t occupancy;          /* megakernel FP32 variant built for >= this many waves per SIMD: 0, 4, 6 */
   int32_t mode;               /* NE_MODE_*: 0 = auto (wavefront for FP32, megakernel for FP64) */
   int32_t pool_size;          /* wavefront path slots (rounded up to 256); 0 = library default
-                                 (2^23 for surface-only scenes, else 2^22) */
+                                 (2^23 for surface-only scenes and for renders of 2^28 paths
+                                 or more, else 2^22) */
 } ne_render_params;
 
 enum { NE_MODE_AUTO = 0, NE_MODE_MEGAKERNEL = 1, NE_MODE_WAVEFRONT = 2 };

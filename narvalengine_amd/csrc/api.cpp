@@ -812,15 +812,23 @@ ne_status ne_film_tonemap(ne_film* f, int32_t spp, float* pixels) {
 
 constexpr size_t kMaxSampleBuffer = size_t(16) << 30;
 
+// Default wavefront pool (slots) for a render of `total` paths, measured on
+// MI355X: 2^23 for surface-only scenes, where fewer, larger BVH launches
+// amortise each launch's tail; with a volume 2^22, where a larger pool adds
+// cache pressure, except for long renders (2^28 paths and more, e.g. the
+// 1920x1080 cloud at 256 spp), whose iterations' fixed costs (launches, the
+// plan scan) halve with 2^23 slots: 236.4 -> 231.5 ms per cloud frame, while
+// a 512^2 x 64-spp volume frame took 12.0 ms at 2^22 and 13.4 ms at 2^23.
+static uint64_t default_pool(const DevScene& ds, uint64_t total) {
+  if (ds.n_nodes > 0 && !ds.has_volume) return uint64_t(1) << 23;
+  return total >= (uint64_t(1) << 28) ? (uint64_t(1) << 23) : (uint64_t(1) << 22);
+}
+
 static ne_status render_wavefront(ne_ctx* c, const ne_scene* s, const DevCamera& dc, const ne_render_params* p,
                                   ne_film* f, bool count, bool timed, StageTimer* tm, WfGuideHook* gh = nullptr) {
   const uint64_t total = uint64_t(p->x1 - p->x0) * uint64_t(p->y1 - p->y0) * uint64_t(p->spp);
-  // pool slots: whole chunks, no more than the job needs.  Default (measured
-  // on MI355X): 2^23 for surface-only scenes, where fewer, larger BVH launches
-  // amortise each launch's tail; 2^22 with a volume, where a larger pool only
-  // adds cache pressure.
-  const uint64_t def_pool = (s->dev.n_nodes > 0 && !s->dev.has_volume) ? (uint64_t(1) << 23) : (uint64_t(1) << 22);
-  uint64_t P = p->pool_size > 0 ? uint64_t(p->pool_size) : def_pool;
+  // pool slots: whole chunks, no more than the job needs
+  uint64_t P = p->pool_size > 0 ? uint64_t(p->pool_size) : default_pool(s->dev, total);
   P = std::min<uint64_t>(P, total);
   P = (P + kChunk - 1) / kChunk * kChunk;
   if (P > c->wf_P) {
@@ -1477,8 +1485,7 @@ ne_status batch_launch(ne_batch* b, const ne_batch_config& cfg, int div, float* 
 static uint64_t lane_job_pool(const ne_job& J, int div, uint64_t* samples) {
   const DevScene& ds = J.scene->dev;
   const uint64_t total = uint64_t(J.width) * uint64_t(J.height) * uint64_t((J.spp + div - 1) / div);
-  const uint64_t def_pool = (ds.n_nodes > 0 && !ds.has_volume) ? (uint64_t(1) << 23) : (uint64_t(1) << 22);
-  const uint64_t P = std::min<uint64_t>(def_pool, total);
+  const uint64_t P = std::min<uint64_t>(default_pool(ds, total), total);
   if (samples) *samples = total;
   return (P + kChunk - 1) / kChunk * kChunk;
 }
