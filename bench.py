@@ -434,7 +434,10 @@ def equal_time_relmse(ctx, g, wl, args, threads: int) -> dict:
 def equal_time_budget(ctx, g, wl, args, ref, ref_var, f, K: int) -> dict:
     from narvalengine_amd import Batch, Guide
     W, H, spp = wl.width, wl.height, wl.spp
-    kw = dict(max_bounces=wl.max_bounces)
+    # every wavefront render of the comparison on the pool the plain frame
+    # gets by default (2^23 slots at this size), so that the pilot's time per
+    # sample holds for the longer renders and no form runs on a smaller pool
+    kw = dict(max_bounces=wl.max_bounces, pool_size=1 << 23)
     n_plain = K * spp
     f.clear()
     ctx.synchronize()
@@ -481,15 +484,16 @@ def equal_time_budget(ctx, g, wl, args, ref, ref_var, f, K: int) -> dict:
         pilot = Guide(ctx, seed=21, **gkw)
         ctx.render(g, wl.camera, f, spp=16, seed=3, guide=pilot, **kw)
         ctx.synchronize()
+        # one full frame each (its pool is that of the longer renders)
         t0 = time.perf_counter()
-        ctx.render(g, wl.camera, f, spp=64, seed=4, guide=pilot, **kw)
+        ctx.render(g, wl.camera, f, spp=spp, seed=4, guide=pilot, **kw)
         ctx.synchronize()
-        per_wf = (time.perf_counter() - t0) / 64
+        per_wf = (time.perf_counter() - t0) / spp
         per_b = None
         if args.batched:
-            pb = Batch(ctx, [dict(job, spp=64, guide=pilot)])
+            pb = Batch(ctx, [dict(job, spp=spp, guide=pilot)])
             pb.run(cfg)
-            per_b = pb.run(cfg) * 1e-3 / 64
+            per_b = pb.run(cfg) * 1e-3 / spp
             pb.close()
         pilot.close()
         for form in ("guided", "guided_batched") if args.batched else ("guided",):
@@ -500,7 +504,7 @@ def equal_time_budget(ctx, g, wl, args, ref, ref_var, f, K: int) -> dict:
             # a small path pool while training: each wavefront iteration ends
             # enough paths for one training step, and iterations are short
             ctx.render(g, wl.camera, f, spp=args.train_spp, seed=13, guide=guide, train=True,
-                       pool_size=args.train_pool, **kw)
+                       **dict(kw, pool_size=args.train_pool))
             ctx.synchronize()
             t_train = time.perf_counter() - t0
             steps = guide.stats()["steps"]
