@@ -435,7 +435,8 @@ ne_status ne_test_transmittance(ne_ctx* ctx, const ne_scene* scene, int32_t n, c
 ne_status ne_test_delta_track(ne_ctx* ctx, const ne_scene* scene, int32_t n, const float* rays, uint64_t seed,
                               int32_t precision, int32_t* scattered, double* t);
 /* Trilinear density (GridMedia::interpolatedDensity, GridMedia.cpp:25-38) at
- * n grid-space points of the scene's volume. */
+ * n grid-space points of the scene's volume (even points through the lookup
+ * for grids under 4 GiB, odd points through the general one). */
 ne_status ne_test_density(ne_ctx* ctx, const ne_scene* scene, int32_t n, const float* gcs, float* out);
 /* The scene's majorant grid as the device holds it (maj_res^3 floats, x
  * fastest): positive per-cell majorants, and in empty cells 0 or, with empty-

@@ -477,10 +477,36 @@ NE_DEV float grid_at(const DevScene& s, int x, int y, int z) {
 // corners at index >= res read as 0 (the grid's zero padding).
 // A homogeneous medium (HomogeneousMedia, src/materials/HomogeneousMedia.cpp)
 // reads density 1 everywhere; the multiplier carries its density.
+// SMALL (grid_small scenes): corner addresses are 32-bit byte offsets from
+// the grid base (uniform), so each load takes a scalar base and a 32-bit lane
+// offset and no lane does 64-bit address arithmetic; same corners, same value.
+template <bool SMALL = false>
 NE_DEV float density_gcs(const DevScene& s, Vec3<float> gp) {
   if (s.homog) return 1.0f;
   gp = v3(xmax(0.0f, gp.x), xmax(0.0f, gp.y), xmax(0.0f, gp.z));
-  gp = v3(xmin(gp.x, float(s.res[0])), xmin(gp.y, float(s.res[1])), xmin(gp.z, float(s.res[2])));
+  // std::min of finite values: v_min_f32 (a compare-and-select otherwise)
+  gp = v3(__builtin_fminf(gp.x, float(s.res[0])), __builtin_fminf(gp.y, float(s.res[1])),
+          __builtin_fminf(gp.z, float(s.res[2])));
+  if constexpr (SMALL) {
+    // floor once per axis; int(floorf(x)) converts back to floorf(x) exactly
+    // (|x| <= 4096), so the fractions are those below
+    const float fx = floorf(gp.x), fy = floorf(gp.y), fz = floorf(gp.z);
+    const uint32_t ix = uint32_t(fx), iy = uint32_t(fy), iz = uint32_t(fz);
+    const float dx = gp.x - fx, dy = gp.y - fy, dz = gp.z - fz;
+    const uint32_t b = ((iz >> 1) * s.grid_step_z + (iy >> 2) * s.grid_step_y + (ix >> 2) * kBrickFloats +
+                        (((iz & 1u) << 4) | ((iy & 3u) << 2) | (ix & 3u))) * 4u;
+    const uint32_t bx = (ix & 3u) != 3u ? 4u : (kBrickFloats - 3u) * 4u;  // x + 1: same brick or the next
+    const uint32_t by = (iy & 3u) != 3u ? 16u : (s.grid_step_y - 12u) * 4u;
+    const uint32_t bz = (iz & 1u) == 0u ? 64u : (s.grid_step_z - 16u) * 4u;
+    const char* g = reinterpret_cast<const char*>(s.grid);
+    auto at = [&](uint32_t o) { return *reinterpret_cast<const float*>(g + size_t(o)); };
+    const float c000 = at(b), c100 = at(b + bx), c010 = at(b + by), c110 = at(b + by + bx);
+    const float c001 = at(b + bz), c101 = at(b + bz + bx), c011 = at(b + by + bz), c111 = at(b + by + bz + bx);
+    const float d00 = mixf(c000, c100, dx), d10 = mixf(c010, c110, dx);
+    const float d01 = mixf(c001, c101, dx), d11 = mixf(c011, c111, dx);
+    const float d0 = mixf(d00, d10, dy), d1 = mixf(d01, d11, dy);
+    return mixf(d0, d1, dz);
+  }
   int ix = int(floorf(gp.x)), iy = int(floorf(gp.y)), iz = int(floorf(gp.z));
   float dx = gp.x - float(ix), dy = gp.y - float(iy), dz = gp.z - float(iz);
   // 0 <= i <= res on every axis and the padding holds index res + 1: every
@@ -511,8 +537,9 @@ NE_DEV float density_gcs(const DevScene& s, Vec3<float> gp) {
   float d0 = mixf(d00, d10, dy), d1 = mixf(d01, d11, dy);
   return mixf(d0, d1, dz);
 }
+template <bool SMALL = false>
 NE_DEV float density_ocs(const DevScene& s, Vec3<float> p) {
-  return density_gcs(s, (p + v3s(0.5f)) * v3(float(s.res[0]), float(s.res[1]), float(s.res[2])));
+  return density_gcs<SMALL>(s, (p + v3s(0.5f)) * v3(float(s.res[0]), float(s.res[1]), float(s.res[2])));
 }
 
 // 3-D DDA over the majorant grid along the OCS ray (oo, dd) for t in [ta, tb).
@@ -589,17 +616,17 @@ NE_DEV void majorant_dda(const DevScene& s, Vec3<R> oo, Vec3<R> dd, R ta, R tb, 
   }
 }
 
-template <typename R, bool C>
+template <typename R, bool C, bool SMALL = false>
 NE_DEV R sigma_at(const DevScene& s, Vec3<R> oo, Vec3<R> dd, R t, Count<C>& cnt) {
   Vec3<R> p = oo + t * dd;
   cnt.lookup();
-  return R(density_ocs(s, v3(float(p.x), float(p.y), float(p.z)))) * R(s.sigma_bar);
+  return R(density_ocs<SMALL>(s, v3(float(p.x), float(p.y), float(p.z)))) * R(s.sigma_bar);
 }
 
 // Delta tracking (GridMedia::sample, src/materials/GridMedia.cpp:71-98) with
 // per-cell majorants: step t -= log(1 - xi) * (1 / mu), accept if
 // sigma * (1 / mu) > xi', the reference's multiply-by-inverse form.
-template <typename R, bool C>
+template <typename R, bool C, bool SMALL = false>
 NE_DEV bool delta_track(const DevScene& s, Vec3<R> o, Vec3<R> d, R ta, R tb, PathRng& rng, Count<C>& cnt, R& th) {
   Vec3<R> oo = xf_point<R>(s.inv, o), dd = xf_vec<R>(s.inv, d);
   bool hit = false;
@@ -611,7 +638,7 @@ NE_DEV bool delta_track(const DevScene& s, Vec3<R> o, Vec3<R> d, R ta, R tb, Pat
       R r = R(rng.next());
       t -= xlog(R(1) - r) * imu;
       if (!(t < c1)) return true;
-      R sig = sigma_at<R, C>(s, oo, dd, t, cnt);
+      R sig = sigma_at<R, C, SMALL>(s, oo, dd, t, cnt);
       R ra = R(rng.next());
       if (sig * imu > ra) { hit = true; th = t; return false; }
     }
@@ -620,7 +647,7 @@ NE_DEV bool delta_track(const DevScene& s, Vec3<R> o, Vec3<R> d, R ta, R tb, Pat
 }
 
 // Ratio tracking with the reference's Russian roulette (GridMedia.cpp:54-66).
-template <typename R, bool C>
+template <typename R, bool C, bool SMALL = false>
 // A homogeneous medium takes the expectation in closed form,
 // HomogeneousMedia::Tr (HomogeneousMedia.cpp:15-18) with the grey extinction
 // of the tracking step law, and draws nothing.
@@ -635,7 +662,7 @@ NE_DEV R ratio_track(const DevScene& s, Vec3<R> o, Vec3<R> d, R ta, R tb, PathRn
     while (true) {
       t -= xlog(R(1) - R(rng.next())) * imu;
       if (!(t < c1)) return true;
-      R sig = sigma_at<R, C>(s, oo, dd, t, cnt);
+      R sig = sigma_at<R, C, SMALL>(s, oo, dd, t, cnt);
       Tr *= R(1) - xmax(R(0), sig * imu);
       if (Tr < R(0.1)) {
         R q = xmax(R(0.05), R(1) - Tr);

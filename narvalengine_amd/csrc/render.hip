@@ -333,10 +333,13 @@ __global__ void k_test_delta(DevScene s, int n, const float* rays, uint64_t seed
   }
 }
 
+// Even points take the 32-bit-offset lookup when the grid allows it, odd
+// points the general one, so one call checks both against the caller's values.
 __global__ void k_test_density(DevScene s, int n, const float* gcs, float* out) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
-  out[i] = density_gcs(s, v3f<float>(gcs + 3 * i));
+  const Vec3<float> p = v3f<float>(gcs + 3 * i);
+  out[i] = (s.grid_small && (i & 1) == 0) ? density_gcs<true>(s, p) : density_gcs<false>(s, p);
 }
 
 __global__ void k_test_bsdf(int kind, int n, const float* in0, const float* in1, const float* in2, const float* in3,

@@ -400,7 +400,8 @@ __global__ __launch_bounds__(kChunk) void k_wf_trace(WfState w, WfArgs a, DevSce
 // same trajectories).
 // REGEN (volume-only scenes): the block first makes its chunk's new paths, as
 // k_wf_regen would, which saves that stage's launch every iteration.
-template <bool C, bool HAS_SURF, bool SPATIAL, bool REGEN>
+// SMALL: the scene's grid is under 4 GiB (density lookups by 32-bit offsets).
+template <bool C, bool HAS_SURF, bool SPATIAL, bool REGEN, bool SMALL>
 __global__ __launch_bounds__(kChunk) void k_wf_medium(WfState w, WfArgs a, DevScene s, DevCamera cam) {
   __shared__ uint32_t red[kChunk / 64];
   __shared__ uint32_t skey[SPATIAL ? kChunk : 1];
@@ -438,7 +439,7 @@ __global__ __launch_bounds__(kChunk) void k_wf_medium(WfState w, WfArgs a, DevSc
         PathRng rng;
         rng.init(a.seed, A.pix[i], A.smp[i], a.job);
         rng.seek(A.rngn[i]);
-        medium = delta_track<float, C>(s, o, d, ta, tb, rng, cnt, th);
+        medium = delta_track<float, C, SMALL>(s, o, d, ta, tb, rng, cnt, th);
         A.rngn[i] = rng.consumed();
       }
     }
@@ -912,7 +913,8 @@ __global__ __launch_bounds__(kChunk) void k_wf_shadow_surf(WfState w, WfArgs a, 
 // the megakernel's transmittance() split by scene content.
 // SPATIAL: the chunk's shadow rays are taken in Morton order of their origins
 // (each keeps its own stream: same transmittances).
-template <bool C, bool HAS_SURF, bool HAS_VOL, bool SPATIAL>
+// SMALL: the scene's grid is under 4 GiB (density lookups by 32-bit offsets).
+template <bool C, bool HAS_SURF, bool HAS_VOL, bool SPATIAL, bool SMALL>
 __global__ __launch_bounds__(kChunk) void k_wf_shadow(WfState w, WfArgs a, DevScene s) {
   __shared__ uint32_t lds_stack[HAS_SURF ? kShortStack * kChunk : 1];
   __shared__ uint32_t red[kChunk / 64];
@@ -946,7 +948,7 @@ __global__ __launch_bounds__(kChunk) void k_wf_shadow(WfState w, WfArgs a, DevSc
       if (box_interval<float>(s, oo, dd, ta, tb)) {
         ta = xmax(ta, 0.0f);
         tb = xmin(tb, tmax);
-        if (ta < tb) tr = ratio_track<float, C>(s, x, wi, ta, tb, srng, cnt);
+        if (ta < tb) tr = ratio_track<float, C, SMALL>(s, x, wi, ta, tb, srng, cnt);
       }
     }
     if (tr > 0.0f) shadow_deliver(S, B, a, s, k, tr);
@@ -1099,6 +1101,7 @@ hipError_t launch_wavefront(const DevScene& s, const DevCamera& cam, WfState& w,
   const bool hs = s.n_nodes > 0, hv = s.has_volume != 0;
   const bool sort = (a.sort & 1) != 0;
   const bool spatial = (a.sort & 2) != 0;  // tracking stages in Morton order of the ray origins
+  const bool small = s.grid_small != 0;    // density lookups by 32-bit offsets
   for (uint32_t it = 0;; ++it) {
     a.cur = int(it & 1u);
     NE_MARK(kStagePlan);
@@ -1140,8 +1143,11 @@ hipError_t launch_wavefront(const DevScene& s, const DevCamera& cam, WfState& w,
     }
     if (hv) {
       NE_MARK(kStageMedium);
-#define NE_MEDIUM(CNT, S_, SP, RG) \
-  hipLaunchKernelGGL((k_wf_medium<CNT, S_, SP, RG>), grid, blk, 0, st, w, a, s, cam)
+#define NE_MEDIUM(CNT, S_, SP, RG)                                                                       \
+  do {                                                                                                   \
+    if (small) hipLaunchKernelGGL((k_wf_medium<CNT, S_, SP, RG, true>), grid, blk, 0, st, w, a, s, cam); \
+    else hipLaunchKernelGGL((k_wf_medium<CNT, S_, SP, RG, false>), grid, blk, 0, st, w, a, s, cam);     \
+  } while (0)
       if (spatial) {
         if (count) { if (hs) NE_MEDIUM(true, true, true, false); else NE_MEDIUM(true, false, true, false); }
         else { if (hs) NE_MEDIUM(false, true, true, false); else NE_MEDIUM(false, false, true, false); }
@@ -1167,10 +1173,15 @@ hipError_t launch_wavefront(const DevScene& s, const DevCamera& cam, WfState& w,
     else if (sort) hipLaunchKernelGGL((k_wf_shade<false, true, false>), grid, blk, 0, st, w, a, s, WfGuide{});
     else hipLaunchKernelGGL((k_wf_shade<false, false, false>), grid, blk, 0, st, w, a, s, WfGuide{});
     NE_MARK(kStageShadow);
-#define NE_SHADOW(CNT, S_, V_)                                                             \
-  do {                                                                                     \
-    if (spatial) hipLaunchKernelGGL((k_wf_shadow<CNT, S_, V_, true>), grid, blk, 0, st, w, a, s); \
-    else hipLaunchKernelGGL((k_wf_shadow<CNT, S_, V_, false>), grid, blk, 0, st, w, a, s);      \
+#define NE_SHADOW_SP(CNT, S_, V_, SM)                                                             \
+  do {                                                                                            \
+    if (spatial) hipLaunchKernelGGL((k_wf_shadow<CNT, S_, V_, true, SM>), grid, blk, 0, st, w, a, s); \
+    else hipLaunchKernelGGL((k_wf_shadow<CNT, S_, V_, false, SM>), grid, blk, 0, st, w, a, s);      \
+  } while (0)
+#define NE_SHADOW(CNT, S_, V_)                       \
+  do {                                               \
+    if (V_ && small) NE_SHADOW_SP(CNT, S_, V_, true); \
+    else NE_SHADOW_SP(CNT, S_, V_, false);           \
   } while (0)
     if (hs && !hv) {
       if (count) hipLaunchKernelGGL(k_wf_shadow_surf<true>, pgrid, blk, 0, st, w, a, s);
@@ -1183,6 +1194,7 @@ hipError_t launch_wavefront(const DevScene& s, const DevCamera& cam, WfState& w,
       else if (hv) NE_SHADOW(false, false, true); else NE_SHADOW(false, false, false);
     }
 #undef NE_SHADOW
+#undef NE_SHADOW_SP
     if ((e = hipGetLastError()) != hipSuccess) return e;
   }
   const size_t nf = size_t(a.W) * size_t(a.H) * 3;
