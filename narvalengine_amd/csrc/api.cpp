@@ -659,7 +659,7 @@ ne_status ne_scene_upload(ne_ctx* c, const ne_scene_desc* d, ne_scene** out) {
       ds.grid = static_cast<const float*>(pg);
       ds.grid_step_y = (px / 4u) * kBrickFloats;
       ds.grid_step_z = (px / 4u) * (py / 4u) * kBrickFloats;
-      ds.grid_small = nb < (size_t(1) << 32) ? 1 : 0;
+      ds.grid_small = nb < (size_t(1) << 32) && ds.grid_step_z < (1u << 24) ? 1 : 0;
     }
     size_t ncell = size_t(ds.mres) * ds.mres * ds.mres;
     void* pm = nullptr;

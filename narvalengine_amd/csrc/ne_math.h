@@ -173,6 +173,32 @@ struct PathRng {
     return r;
   }
   NE_HD float next() { return float(next_u32() >> 8) * (1.0f / 16777216.0f); }
+  // The same draw, but it also leaves at least one word buffered (a block
+  // is generated when one word or none is left, the leftover word drawn
+  // first), so that next_buffered() right after it needs no refill.  A loop
+  // that draws one or two words per pass then has one refill site instead
+  // of two, each of which costs the whole wave a Philox block whenever any
+  // of its lanes needs one.  Same words in the same order as next().
+  NE_HD float next_keep1() {
+    uint32_t r;
+    if (idx >= 3) {
+      uint32_t c[4] = {pixel, sample, block, job};
+      philox10(c, k0, k1);
+      block += 1;
+      if (idx == 3) { r = b0; b0 = c[0]; b1 = c[1]; b2 = c[2]; b3 = c[3]; idx = 0; }
+      else { r = c[0]; b0 = c[1]; b1 = c[2]; b2 = c[3]; idx = 1; }
+    } else {
+      r = b0;
+      shift();
+    }
+    return float(r >> 8) * (1.0f / 16777216.0f);
+  }
+  // The next buffered word; only directly after next_keep1().
+  NE_HD float next_buffered() {
+    const uint32_t r = b0;
+    shift();
+    return float(r >> 8) * (1.0f / 16777216.0f);
+  }
 };
 
 }  // namespace ne

@@ -84,7 +84,7 @@ struct DevScene {
   const float* grid;      // bricked (grid_offset), zero-padded to at least res + 2 voxels per axis
   uint32_t grid_step_y;   // floats from one brick row (4 voxels in y) to the next
   uint32_t grid_step_z;   // floats from one brick slab (2 voxels in z) to the next
-  int32_t grid_small;     // the bricked grid spans under 2^32 bytes: 32-bit byte offsets address it
+  int32_t grid_small;     // the bricked grid spans under 2^32 bytes and grid_step_z < 2^24: 32-bit byte offsets address it
   const float* maj;
   float sigma_bar;        // avg(sigma_t * densityMultiplier)
   float sig_s[3];

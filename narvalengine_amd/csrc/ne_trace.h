@@ -493,7 +493,9 @@ NE_DEV float density_gcs(const DevScene& s, Vec3<float> gp) {
     const float fx = floorf(gp.x), fy = floorf(gp.y), fz = floorf(gp.z);
     const uint32_t ix = uint32_t(fx), iy = uint32_t(fy), iz = uint32_t(fz);
     const float dx = gp.x - fx, dy = gp.y - fy, dz = gp.z - fz;
-    const uint32_t b = ((iz >> 1) * s.grid_step_z + (iy >> 2) * s.grid_step_y + (ix >> 2) * kBrickFloats +
+    // both steps are under 2^24 and every offset under 2^32: 24-bit multiplies are exact
+    const uint32_t b = (__umul24(iz >> 1, uint32_t(s.grid_step_z)) + __umul24(iy >> 2, uint32_t(s.grid_step_y)) +
+                        (ix >> 2) * kBrickFloats +
                         (((iz & 1u) << 4) | ((iy & 3u) << 2) | (ix & 3u))) * 4u;
     const uint32_t bx = (ix & 3u) != 3u ? 4u : (kBrickFloats - 3u) * 4u;  // x + 1: same brick or the next
     const uint32_t by = (iy & 3u) != 3u ? 16u : (s.grid_step_y - 12u) * 4u;
@@ -635,11 +637,11 @@ NE_DEV bool delta_track(const DevScene& s, Vec3<R> o, Vec3<R> d, R ta, R tb, Pat
     const R imu = R(1) / mu;
     R t = c0;
     while (true) {
-      R r = R(rng.next());
+      R r = R(rng.next_keep1());
       t -= xlog(R(1) - r) * imu;
       if (!(t < c1)) return true;
       R sig = sigma_at<R, C, SMALL>(s, oo, dd, t, cnt);
-      R ra = R(rng.next());
+      R ra = R(rng.next_buffered());
       if (sig * imu > ra) { hit = true; th = t; return false; }
     }
   });
@@ -660,13 +662,13 @@ NE_DEV R ratio_track(const DevScene& s, Vec3<R> o, Vec3<R> d, R ta, R tb, PathRn
     const R imu = R(1) / mu;
     R t = c0;
     while (true) {
-      t -= xlog(R(1) - R(rng.next())) * imu;
+      t -= xlog(R(1) - R(rng.next_keep1())) * imu;
       if (!(t < c1)) return true;
       R sig = sigma_at<R, C, SMALL>(s, oo, dd, t, cnt);
       Tr *= R(1) - xmax(R(0), sig * imu);
       if (Tr < R(0.1)) {
         R q = xmax(R(0.05), R(1) - Tr);
-        if (R(rng.next()) < q) { Tr = R(0); return false; }
+        if (R(rng.next_buffered()) < q) { Tr = R(0); return false; }
         Tr /= R(1) - q;
       }
     }
