@@ -693,6 +693,22 @@ ne_status ne_scene_upload(ne_ctx* c, const ne_scene_desc* d, ne_scene** out) {
       NE_HIP_BAIL(hipMemcpyAsync(pm, mh.data(), ncell * sizeof(float), hipMemcpyHostToDevice, c->stream));
       NE_HIP_BAIL(hipStreamSynchronize(c->stream));
     }
+    // the tracking loops' cell reads: the value and its reciprocal in one 8-byte load
+    // (host IEEE division: the quotient the device's division returns)
+    {
+      std::vector<float2> mp(ncell);
+      for (size_t i = 0; i < ncell; ++i) {
+        const float v = mh[i];
+        mp[i] = make_float2(v, v > 0.0f && std::isfinite(v) ? 1.0f / v : 0.0f);
+      }
+      void* pp = nullptr;
+      NE_HIP_BAIL(hipMalloc(&pp, ncell * sizeof(float2)));
+      s->allocs.push_back(pp);
+      s->bytes += ncell * sizeof(float2);
+      NE_HIP_BAIL(hipMemcpyAsync(pp, mp.data(), ncell * sizeof(float2), hipMemcpyHostToDevice, c->stream));
+      NE_HIP_BAIL(hipStreamSynchronize(c->stream));
+      ds.maj_pair = static_cast<const float2*>(pp);
+    }
   }
   // ---- lights
   ds.has_sun = d->has_sun ? 1 : 0;

@@ -86,6 +86,7 @@ struct DevScene {
   uint32_t grid_step_z;   // floats from one brick slab (2 voxels in z) to the next
   int32_t grid_small;     // the bricked grid spans under 2^32 bytes and grid_step_z < 2^24: 32-bit byte offsets address it
   const float* maj;
+  const float2* maj_pair;  // per cell {maj value, 1 / maj (correctly rounded; 0 unless maj is positive and finite)}
   float sigma_bar;        // avg(sigma_t * densityMultiplier)
   float sig_s[3];
   float sig_t[3];

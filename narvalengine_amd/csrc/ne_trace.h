@@ -483,10 +483,10 @@ NE_DEV float grid_at(const DevScene& s, int x, int y, int z) {
 template <bool SMALL = false>
 NE_DEV float density_gcs(const DevScene& s, Vec3<float> gp) {
   if (s.homog) return 1.0f;
-  gp = v3(xmax(0.0f, gp.x), xmax(0.0f, gp.y), xmax(0.0f, gp.z));
-  // std::min of finite values: v_min_f32 (a compare-and-select otherwise)
-  gp = v3(__builtin_fminf(gp.x, float(s.res[0])), __builtin_fminf(gp.y, float(s.res[1])),
-          __builtin_fminf(gp.z, float(s.res[2])));
+  // clamp to [0, res] (std::max with 0, then std::min with res): for finite
+  // values the median of (x, 0, res) is that clamp, one v_med3_f32
+  gp = v3(__builtin_amdgcn_fmed3f(gp.x, 0.0f, float(s.res[0])), __builtin_amdgcn_fmed3f(gp.y, 0.0f, float(s.res[1])),
+          __builtin_amdgcn_fmed3f(gp.z, 0.0f, float(s.res[2])));
   if constexpr (SMALL) {
     // floor once per axis; int(floorf(x)) converts back to floorf(x) exactly
     // (|x| <= 4096), so the fractions are those below
@@ -544,8 +544,29 @@ NE_DEV float density_ocs(const DevScene& s, Vec3<float> p) {
   return density_gcs<SMALL>(s, (p + v3s(0.5f)) * v3(float(s.res[0]), float(s.res[1]), float(s.res[2])));
 }
 
+// 1 / mu for the tracking step: in FP32 the cell's stored reciprocal (the
+// correctly rounded quotient, the same bits as the division), in FP64 the
+// division itself.
+NE_DEV float maj_inverse(float, float inv) { return inv; }
+NE_DEV double maj_inverse(double mu, float) { return 1.0 / mu; }
+
+// log(1 - u) of a draw u in [0, 1 - 2^-24].  The argument is a normal float
+// in [2^-24, 1], for which the device logf is log2 by v_log_f32 times ln 2
+// carried in two parts (0x1.62e42ep-1 + 0x1.efa39ep-25, the product's error
+// kept by an fma); these are those operations without its subnormal scaling
+// and its infinity and NaN selects, so the result has the same bits.
+NE_DEV float log1m_draw(float u) {
+  const float y = __builtin_amdgcn_logf(1.0f - u);
+  const float hi = y * 0x1.62e42ep-1f;
+  float e = __builtin_fmaf(y, 0x1.62e42ep-1f, -hi);
+  e = __builtin_fmaf(y, 0x1.efa39ep-25f, e);
+  return hi + e;
+}
+NE_DEV double log1m_draw(double u) { return xlog(1.0 - u); }
+
 // 3-D DDA over the majorant grid along the OCS ray (oo, dd) for t in [ta, tb).
-// Calls f(c0, c1, mu) per cell with mu > 0; f returns false to stop.
+// Calls f(c0, c1, mu, inv) per cell with mu > 0 (inv its stored FP32
+// reciprocal); f returns false to stop.
 // Empty cells may hold -D instead of 0, D >= 1 the L1 (Manhattan) distance
 // to the nearest non-empty cell: each step moves one cell along one axis, so
 // the next D - 1 cells are empty too and their majorants are not read.  Cell
@@ -601,10 +622,11 @@ NE_DEV void majorant_dda(const DevScene& s, Vec3<R> oo, Vec3<R> dd, R ta, R tb, 
       --skip;
     } else {
       // M <= 256: 24-bit multiplies give the exact cell index
-      const float raw = s.maj[__umul24(__umul24(uint32_t(cell[2]), uint32_t(M)) + uint32_t(cell[1]), uint32_t(M)) +
-                              uint32_t(cell[0])];
+      const float2 mp = s.maj_pair[__umul24(__umul24(uint32_t(cell[2]), uint32_t(M)) + uint32_t(cell[1]),
+                                            uint32_t(M)) + uint32_t(cell[0])];
+      const float raw = mp.x;
       if (raw < 0.0f) skip = int(-raw) - 1;
-      else if (raw > 0.0f && tc > t && !f(t, tc, R(raw))) return;
+      else if (raw > 0.0f && tc > t && !f(t, tc, R(raw), mp.y)) return;
     }
     t = tc;
     if (tna >= tb) break;
@@ -632,13 +654,13 @@ template <typename R, bool C, bool SMALL = false>
 NE_DEV bool delta_track(const DevScene& s, Vec3<R> o, Vec3<R> d, R ta, R tb, PathRng& rng, Count<C>& cnt, R& th) {
   Vec3<R> oo = xf_point<R>(s.inv, o), dd = xf_vec<R>(s.inv, d);
   bool hit = false;
-  majorant_dda<R, C>(s, oo, dd, ta, tb, cnt, [&](R c0, R c1, R mu) {
+  majorant_dda<R, C>(s, oo, dd, ta, tb, cnt, [&](R c0, R c1, R mu, float inv) {
     if (!(mu > R(0)) || xisinf(mu)) return true;  // empty cell; grids are validated finite at upload
-    const R imu = R(1) / mu;
+    const R imu = maj_inverse(mu, inv);
     R t = c0;
     while (true) {
       R r = R(rng.next_keep1());
-      t -= xlog(R(1) - r) * imu;
+      t -= log1m_draw(r) * imu;
       if (!(t < c1)) return true;
       R sig = sigma_at<R, C, SMALL>(s, oo, dd, t, cnt);
       R ra = R(rng.next_buffered());
@@ -657,12 +679,12 @@ NE_DEV R ratio_track(const DevScene& s, Vec3<R> o, Vec3<R> d, R ta, R tb, PathRn
   if (s.homog) return xexp(-R(s.sigma_bar) * (tb - ta));
   Vec3<R> oo = xf_point<R>(s.inv, o), dd = xf_vec<R>(s.inv, d);
   R Tr = R(1);
-  majorant_dda<R, C>(s, oo, dd, ta, tb, cnt, [&](R c0, R c1, R mu) {
+  majorant_dda<R, C>(s, oo, dd, ta, tb, cnt, [&](R c0, R c1, R mu, float inv) {
     if (!(mu > R(0)) || xisinf(mu)) return true;  // empty cell; grids are validated finite at upload
-    const R imu = R(1) / mu;
+    const R imu = maj_inverse(mu, inv);
     R t = c0;
     while (true) {
-      t -= xlog(R(1) - R(rng.next_keep1())) * imu;
+      t -= log1m_draw(R(rng.next_keep1())) * imu;
       if (!(t < c1)) return true;
       R sig = sigma_at<R, C, SMALL>(s, oo, dd, t, cnt);
       Tr *= R(1) - xmax(R(0), sig * imu);
