@@ -114,7 +114,11 @@ __device__ __forceinline__ void block_count(unsigned long long* dst, uint32_t v,
 __global__ void k_wf_init(WfState w, uint32_t nchunks) {
   const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c < nchunks) w.cnt_paths[0][c] = 0;
-  if (c == 0) *w.next_path = 0;
+  if (c == 0) {
+    *w.next_path = 0;
+    w.status[1] = 0u;
+    w.status[2] = 0u;
+  }
 }
 
 // path id -> (pixel, sample): one sample index at a time over the rect; full
@@ -589,6 +593,14 @@ __global__ __launch_bounds__(kChunk, (GUIDED && !TRAIN) ? 7 : 1) void k_wf_shade
   const WfShadow S = w.shadow;
   const uint32_t c = blockIdx.x, t = threadIdx.x;
   const uint32_t nlive = w.cnt_paths[a.cur][c];
+  if (nlive == 0) {  // an empty chunk (most of them once the pool drains): nothing continues
+    if (t == 0) {
+      w.cnt_paths[a.cur ^ 1][c] = 0u;
+      w.cnt_shadow[c] = 0u;
+      if (c == 0) w.status[1 + a.cur] = 0u;
+    }
+    return;
+  }
   uint32_t slot = t;
   if constexpr (SORT) {
     __shared__ uint32_t skey[kChunk];
@@ -833,6 +845,10 @@ __global__ __launch_bounds__(kChunk, (GUIDED && !TRAIN) ? 7 : 1) void k_wf_shade
   if (t == 0) {
     w.cnt_paths[a.cur ^ 1][c] = ncont;
     w.cnt_shadow[c] = nshadow;
+    // live flags: status[1 + b] != 0 iff buffer b holds a path; this launch
+    // sets the next buffer's and clears its own (read, if at all, before it)
+    if (ncont) w.status[2 - a.cur] = 1u;
+    if (c == 0) w.status[1 + a.cur] = 0u;
   }
   if (a.counters) {
     uint32_t nend;
@@ -1102,22 +1118,39 @@ hipError_t launch_wavefront(const DevScene& s, const DevCamera& cam, WfState& w,
   const bool sort = (a.sort & 1) != 0;
   const bool spatial = (a.sort & 2) != 0;  // tracking stages in Morton order of the ray origins
   const bool small = s.grid_small != 0;    // density lookups by 32-bit offsets
+  // Drain: once every path of the render has been issued, the plan would
+  // only count the paths in flight, so it is no longer launched (nor regen);
+  // the loop ends on the live flag the shading stage keeps instead.
+  bool drain = false;
   for (uint32_t it = 0;; ++it) {
     a.cur = int(it & 1u);
-    NE_MARK(kStagePlan);
-    hipLaunchKernelGGL(k_wf_plan, dim3(1), dim3(unsigned(kPlanThreads)), 0, st, w, a);
+    const bool planned = !drain;  // this iteration's new paths are planned
+    if (planned) {
+      NE_MARK(kStagePlan);
+      hipLaunchKernelGGL(k_wf_plan, dim3(1), dim3(unsigned(kPlanThreads)), 0, st, w, a);
+    }
     // every 8 iterations (every iteration while the guide trains): stop once
-    // nothing is in flight (status written by plan); guided training: a step
-    // on the queued samples when enough are in
+    // nothing is in flight (status written by plan, or the live flag);
+    // guided training: a step on the queued samples when enough are in
     const bool training = gh && gh->g.train;
     if ((it & 7u) == 7u || (training && it > 0)) {
-      if ((e = hipMemcpyAsync(h_status, w.status, sizeof(uint32_t), hipMemcpyDeviceToHost, st)) != hipSuccess) return e;
+      if ((e = hipMemcpyAsync(h_status, w.status + (planned ? 0 : 1 + a.cur), sizeof(uint32_t),
+                              hipMemcpyDeviceToHost, st)) != hipSuccess)
+        return e;
+      if (planned && (e = hipMemcpyAsync(h_status + 2, w.next_path, sizeof(unsigned long long),
+                                         hipMemcpyDeviceToHost, st)) != hipSuccess)
+        return e;
       if (gh && gh->g.train &&
           (e = hipMemcpyAsync(h_status + 1, gh->g.batch.count, sizeof(uint32_t), hipMemcpyDeviceToHost, st)) !=
               hipSuccess)
         return e;
       if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
       if (h_status[0] == 0) break;
+      if (planned) {
+        unsigned long long issued;
+        std::memcpy(&issued, h_status + 2, sizeof(issued));
+        drain = issued >= a.total_paths;  // from the next iteration on
+      }
       if (gh && gh->g.train && h_status[1] >= gh->min_batch) {
         NE_MARK(kStageTrain);
         if ((e = launch_guide_grad(gh->g.net, gh->g.batch, gh->train, st)) != hipSuccess) return e;
@@ -1131,8 +1164,8 @@ hipError_t launch_wavefront(const DevScene& s, const DevCamera& cam, WfState& w,
         hipLaunchKernelGGL(k_guide_reset_count, dim3(1), dim3(1), 0, st, gh->g.batch.count);
       }
     }
-    const bool fused = hv && !hs && !spatial;  // regen inside the medium stage
-    if (!fused) {
+    const bool fused = hv && !hs && !spatial && planned;  // regen inside the medium stage
+    if (!fused && planned) {
       NE_MARK(kStageRegen);
       hipLaunchKernelGGL(k_wf_regen, grid, blk, 0, st, w, a, cam);
     }
@@ -1152,8 +1185,15 @@ hipError_t launch_wavefront(const DevScene& s, const DevCamera& cam, WfState& w,
         if (count) { if (hs) NE_MEDIUM(true, true, true, false); else NE_MEDIUM(true, false, true, false); }
         else { if (hs) NE_MEDIUM(false, true, true, false); else NE_MEDIUM(false, false, true, false); }
       } else {
-        if (count) { if (hs) NE_MEDIUM(true, true, false, false); else NE_MEDIUM(true, false, false, true); }
-        else { if (hs) NE_MEDIUM(false, true, false, false); else NE_MEDIUM(false, false, false, true); }
+        if (count) {
+          if (hs) NE_MEDIUM(true, true, false, false);
+          else if (fused) NE_MEDIUM(true, false, false, true);
+          else NE_MEDIUM(true, false, false, false);
+        } else {
+          if (hs) NE_MEDIUM(false, true, false, false);
+          else if (fused) NE_MEDIUM(false, false, false, true);
+          else NE_MEDIUM(false, false, false, false);
+        }
       }
 #undef NE_MEDIUM
     }
