@@ -70,24 +70,40 @@ enum { WF_KIND_MEDIUM = 0, WF_KIND_SURFACE = 1, WF_KIND_MISS = 2 };
 constexpr uint32_t kNoHit = 0xFFFFFFFFu;
 constexpr uint32_t kPrevDelta = 1u << 16;  // depth word: set while the last vertex was the camera
 
+// A per-slot array of 4-byte elements.  Pools hold at most 2^28 slots, so
+// element i lies 4 * i < 2^32 bytes from the base: indexing by that 32-bit
+// byte offset lets a device access take the scalar-base form (a uniform
+// base, one 32-bit lane offset shared by every array) instead of a 64-bit
+// address per lane and array.
+template <typename T>
+struct SlotArray {
+  static_assert(sizeof(T) == 4, "4-byte elements");
+  T* p;
+  __host__ __device__ T& operator[](uint32_t i) const {
+    return *reinterpret_cast<T*>(reinterpret_cast<char*>(p) + size_t(i * 4u));
+  }
+};
+using SlotF = SlotArray<float>;
+using SlotU = SlotArray<uint32_t>;
+
 // One path buffer, structure of arrays over P slots.
 struct WfPaths {
-  float *ox, *oy, *oz, *dx, *dy, *dz;  // current ray
-  float *bx, *by, *bz;                 // path throughput
-  float *Lx, *Ly, *Lz;                 // radiance gathered so far
-  float* prev_pdf;                     // pdf of the sampled direction (emission MIS)
-  float* hit_t;                        // surface hit / medium collision distance
-  float* pad;
-  uint32_t *pix, *smp, *rngn, *depth;  // pixel, sample, draws consumed, depth | kPrevDelta
-  uint32_t* hit_ref;                   // (prim type << 30) | storage slot
-  uint32_t* kind;                      // WF_KIND_*
+  SlotF ox, oy, oz, dx, dy, dz;  // current ray
+  SlotF bx, by, bz;              // path throughput
+  SlotF Lx, Ly, Lz;              // radiance gathered so far
+  SlotF prev_pdf;                // pdf of the sampled direction (emission MIS)
+  SlotF hit_t;                   // surface hit / medium collision distance
+  SlotF pad;
+  SlotU pix, smp, rngn, depth;   // pixel, sample, draws consumed, depth | kPrevDelta
+  SlotU hit_ref;                 // (prim type << 30) | storage slot
+  SlotU kind;                    // WF_KIND_*
 };
 // Shadow-ray queue (per chunk, packed): origin, direction, extent, (beta f) Le,
 // MIS weight, light pdf, target (slot within the chunk of the next buffer, or
 // film pixel | 2^31), RNG key.
 struct WfShadow {
-  float *ox, *oy, *oz, *dx, *dy, *dz, *tmax, *cx, *cy, *cz, *wt, *pl;
-  uint32_t *target, *pix, *smp, *vert;
+  SlotF ox, oy, oz, dx, dy, dz, tmax, cx, cy, cz, wt, pl;
+  SlotU target, pix, smp, vert;
 };
 struct WfState {
   WfPaths paths[2];

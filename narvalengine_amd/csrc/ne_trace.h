@@ -488,11 +488,12 @@ NE_DEV float density_gcs(const DevScene& s, Vec3<float> gp) {
   gp = v3(__builtin_amdgcn_fmed3f(gp.x, 0.0f, float(s.res[0])), __builtin_amdgcn_fmed3f(gp.y, 0.0f, float(s.res[1])),
           __builtin_amdgcn_fmed3f(gp.z, 0.0f, float(s.res[2])));
   if constexpr (SMALL) {
-    // floor once per axis; int(floorf(x)) converts back to floorf(x) exactly
-    // (|x| <= 4096), so the fractions are those below
-    const float fx = floorf(gp.x), fy = floorf(gp.y), fz = floorf(gp.z);
-    const uint32_t ix = uint32_t(fx), iy = uint32_t(fy), iz = uint32_t(fz);
-    const float dx = gp.x - fx, dy = gp.y - fy, dz = gp.z - fz;
+    // x >= 0 here: the truncating conversion is floor(x), and v_fract_f32
+    // returns x - floor(x), which is exact (the low bits of x): the same
+    // index and fraction as int(floorf(x)) and x - float(ix) below
+    const uint32_t ix = uint32_t(gp.x), iy = uint32_t(gp.y), iz = uint32_t(gp.z);
+    const float dx = __builtin_amdgcn_fractf(gp.x), dy = __builtin_amdgcn_fractf(gp.y),
+                dz = __builtin_amdgcn_fractf(gp.z);
     // both steps are under 2^24 and every offset under 2^32: 24-bit multiplies are exact
     const uint32_t b = (__umul24(iz >> 1, uint32_t(s.grid_step_z)) + __umul24(iy >> 2, uint32_t(s.grid_step_y)) +
                         (ix >> 2) * kBrickFloats +

@@ -121,21 +121,38 @@ __global__ void k_wf_init(WfState w, uint32_t nchunks) {
   }
 }
 
+// id / n and id % n for a path id and 0 < n < 2^32 with a quotient below
+// 2^32.  The correctly rounded double quotient is within 2^-20 of the true
+// one, so its floor is the true quotient or one off, and one correction step
+// each way makes it exact (a few instructions instead of a 64-bit integer
+// division).
+__device__ __forceinline__ void divmod_id(unsigned long long id, uint32_t n, uint32_t& q, uint32_t& r) {
+  const double e = floor(double(id) / double(n));
+  long long qq = (long long)e;
+  long long rr = (long long)id - qq * (long long)n;
+  if (rr < 0) { qq -= 1; rr += n; }
+  if (rr >= (long long)n) { qq += 1; rr -= n; }
+  q = uint32_t(qq);
+  r = uint32_t(rr);
+}
+
 // path id -> (pixel, sample): one sample index at a time over the rect; full
 // 8x8 tiles first (row-major tile order), then the right strip, then the
-// bottom strip (row-major): a bijection onto the rect.
-__device__ __forceinline__ void path_pixel(const WfArgs& a, unsigned long long id, uint32_t& pix, uint32_t& smp) {
+// bottom strip (row-major): a bijection onto the rect.  (px, py): the pixel's
+// coordinates within the rect.  Rects hold at most 2^30 pixels and renders
+// fewer than 2^32 samples per pixel.
+__device__ __forceinline__ void path_pixel(const WfArgs& a, unsigned long long id, uint32_t& pix, uint32_t& smp,
+                                           uint32_t& px, uint32_t& py) {
   const uint32_t rw = uint32_t(a.x1 - a.x0), rh = uint32_t(a.y1 - a.y0);
-  const unsigned long long npix = (unsigned long long)rw * rh;
-  smp = uint32_t(id / npix);
-  const uint32_t p = uint32_t(id % npix);
+  uint32_t p;
+  divmod_id(id, rw * rh, smp, p);
   const uint32_t full_rows = rh / 8, full_cols = rw / 8;
   const uint32_t tiles_full = full_rows * full_cols;
-  uint32_t px, py;
   if (p < tiles_full * 64) {
     const uint32_t t = p / 64, l = p % 64;
-    px = (t % full_cols) * 8 + (l & 7);
-    py = (t / full_cols) * 8 + (l >> 3);
+    const uint32_t ty = t / full_cols;
+    px = (t - ty * full_cols) * 8 + (l & 7);
+    py = ty * 8 + (l >> 3);
   } else {
     uint32_t r = p - tiles_full * 64;
     const uint32_t strip_w = rw - full_cols * 8;
@@ -258,13 +275,13 @@ __device__ __forceinline__ void regen_chunk(const WfState& w, const WfArgs& a, c
   const WfPaths A = w.paths[a.cur];
   if (t < m) {
     const uint32_t slot = c * kChunk + n0 + t;
-    uint32_t pix, smp;
-    path_pixel(a, w.regen_base[c] + t, pix, smp);
+    uint32_t pix, smp, rx, ry;
+    path_pixel(a, w.regen_base[c] + t, pix, smp, rx, ry);
     smp += a.sample_offset;
     PathRng rng;
     rng.init(a.seed, pix, smp, a.job);
     const float ju = rng.next(), jv = rng.next();
-    const uint32_t px = pix % uint32_t(a.W), py = pix / uint32_t(a.W);
+    const uint32_t px = uint32_t(a.x0) + rx, py = uint32_t(a.y0) + ry;  // pix % W, pix / W
     const float u = (float(px) + ju) / float(a.W), v = (float(py) + jv) / float(a.H);
     const Vec3<float> pos = v3f<float>(cam.pos), ll = v3f<float>(cam.lower_left);
     const Vec3<float> hz = v3f<float>(cam.horizontal), vt = v3f<float>(cam.vertical);
@@ -1053,17 +1070,17 @@ void wf_carve(void* base, uint32_t P, WfState& w) {
   const size_t NC = P / kChunk;
   for (int b = 0; b < 2; ++b) {
     WfPaths& q = w.paths[b];
-    float** fl[kPathF] = {&q.ox, &q.oy, &q.oz, &q.dx, &q.dy, &q.dz, &q.bx, &q.by,
-                          &q.bz, &q.Lx, &q.Ly, &q.Lz, &q.prev_pdf, &q.hit_t, &q.pad};
-    for (int k = 0; k < kPathF; ++k, p += stride) *fl[k] = reinterpret_cast<float*>(p);
-    uint32_t** us[kPathU] = {&q.pix, &q.smp, &q.rngn, &q.depth, &q.hit_ref, &q.kind};
-    for (int k = 0; k < kPathU; ++k, p += stride) *us[k] = reinterpret_cast<uint32_t*>(p);
+    SlotF* fl[kPathF] = {&q.ox, &q.oy, &q.oz, &q.dx, &q.dy, &q.dz, &q.bx, &q.by,
+                         &q.bz, &q.Lx, &q.Ly, &q.Lz, &q.prev_pdf, &q.hit_t, &q.pad};
+    for (int k = 0; k < kPathF; ++k, p += stride) fl[k]->p = reinterpret_cast<float*>(p);
+    SlotU* us[kPathU] = {&q.pix, &q.smp, &q.rngn, &q.depth, &q.hit_ref, &q.kind};
+    for (int k = 0; k < kPathU; ++k, p += stride) us[k]->p = reinterpret_cast<uint32_t*>(p);
   }
   WfShadow& S = w.shadow;
-  float** sf[kShadowF] = {&S.ox, &S.oy, &S.oz, &S.dx, &S.dy, &S.dz, &S.tmax, &S.cx, &S.cy, &S.cz, &S.wt, &S.pl};
-  for (int k = 0; k < kShadowF; ++k, p += stride) *sf[k] = reinterpret_cast<float*>(p);
-  uint32_t** su[kShadowU] = {&S.target, &S.pix, &S.smp, &S.vert};
-  for (int k = 0; k < kShadowU; ++k, p += stride) *su[k] = reinterpret_cast<uint32_t*>(p);
+  SlotF* sf[kShadowF] = {&S.ox, &S.oy, &S.oz, &S.dx, &S.dy, &S.dz, &S.tmax, &S.cx, &S.cy, &S.cz, &S.wt, &S.pl};
+  for (int k = 0; k < kShadowF; ++k, p += stride) sf[k]->p = reinterpret_cast<float*>(p);
+  SlotU* su[kShadowU] = {&S.target, &S.pix, &S.smp, &S.vert};
+  for (int k = 0; k < kShadowU; ++k, p += stride) su[k]->p = reinterpret_cast<uint32_t*>(p);
   uint32_t** cs[4] = {&w.cnt_paths[0], &w.cnt_paths[1], &w.cnt_shadow, &w.regen_m};
   for (int k = 0; k < 4; ++k, p += align256(NC * 4)) *cs[k] = reinterpret_cast<uint32_t*>(p);
   w.regen_base = reinterpret_cast<unsigned long long*>(p);

@@ -200,3 +200,32 @@ def test_non_finite_geometry_is_rejected_before_the_bvh_build(bad):
         d.rect = keep_rect.ctypes.data_as(C.POINTER(C.c_float))
     assert abi.lib().ne_bvh_build(C.byref(d), None, None, None, None, None) == 1
     assert b"finite" in abi.lib().ne_last_error(None)
+
+
+def test_path_id_divmod_method_is_exact():
+    """The wavefront's path id -> (sample, pixel) split takes floor of the
+    correctly rounded double quotient and one correction step each way (the
+    quotient is within 2^-20 of the true one for quotients below 2^32).  The
+    same IEEE operations in numpy against Python's exact divmod, over the
+    stated range: divisors up to 2^30 (rect pixel counts) and quotients below
+    2^32, at and around exact multiples."""
+    rng = np.random.default_rng(5)
+
+    def method(i, n):
+        q = int(np.floor(np.float64(i) / np.float64(n)))
+        r = i - q * n
+        if r < 0:
+            q, r = q - 1, r + n
+        if r >= n:
+            q, r = q + 1, r - n
+        return q, r
+
+    divisors = [1, 2, 3, 7, 64, 255, 1920 * 1080, 32768 * 32767, 2 ** 30 - 1, 2 ** 30]
+    divisors += [int(x) for x in rng.integers(1, 2 ** 30, 20)]
+    for n in divisors:
+        ids = [int(x) for x in rng.integers(0, n * 2 ** 32, 200, dtype=np.uint64)]
+        for k in (1, 2, 3, 2 ** 20 + 1, 2 ** 31, 2 ** 32 - 1):
+            ids += [k * n - 1, k * n, k * n + 1]
+        for i in ids:
+            if 0 <= i < n * 2 ** 32:
+                assert method(i, n) == divmod(i, n), (i, n)

@@ -552,6 +552,29 @@ def test_wavefront_matches_megakernel(ctx, which):
     np.testing.assert_array_equal(films[1], films[2])  # the pool size does not change the result
 
 
+def test_wavefront_rect_with_strips_matches_megakernel(ctx):
+    """A rect at an offset whose sides are not multiples of 8: the wavefront's
+    path-to-pixel map (8x8 tiles, then the right strip, then the bottom strip)
+    covers each pixel of the rect once per sample, as the megakernel's
+    per-pixel loop does, and nothing outside it."""
+    print_case("wavefront rect with strips")
+    from narvalengine_amd.abi import NE_MODE_MEGAKERNEL, NE_MODE_WAVEFRONT
+    g = ctx.upload(scenes.small_volume(res=32, maj_res=4))
+    rect = (3, 5, 42, 36)  # 39 x 31 pixels: 4 x 3 full tiles, a 7-pixel strip right, 7 rows below
+    films = []
+    for mode, pool in [(NE_MODE_MEGAKERNEL, 0), (NE_MODE_WAVEFRONT, 0), (NE_MODE_WAVEFRONT, 512)]:
+        f = ctx.film(45, 37)
+        ctx.render(g, scenes.VOLUME_CAMERA, f, spp=8, max_bounces=16, seed=17, rect=rect, mode=mode,
+                   pool_size=pool)
+        films.append(f.read())
+    np.testing.assert_allclose(films[1], films[0], rtol=1e-6, atol=8 * 2.0 ** -30)
+    np.testing.assert_array_equal(films[1], films[2])
+    inside = np.zeros((37, 45), bool)
+    inside[5:36, 3:42] = True
+    assert not films[1][~inside].any()
+    assert (films[1][inside].sum(axis=-1) > 0).mean() > 0.5
+
+
 @pytest.mark.parametrize("which", ["volume", "surface"])
 def test_wavefront_counters_and_stage_times(ctx, which):
     """Both pipelines trace the same paths: equal event counts, except that the
