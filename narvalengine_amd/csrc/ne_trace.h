@@ -613,31 +613,43 @@ NE_DEV void majorant_dda(const DevScene& s, Vec3<R> oo, Vec3<R> dd, R ta, R tb, 
   R t = ta;
   int skip = 0;
   while (t < tb) {
-    int ax = 0;
-    if (tnext[1] < tnext[ax]) ax = 1;
-    if (tnext[2] < tnext[ax]) ax = 2;
-    R tna = ax == 0 ? tnext[0] : (ax == 1 ? tnext[1] : tnext[2]);
+    // exit axis: 1 if tnext[1] < tnext[0], then 2 if tnext[2] is below that
+    // one (ties keep the lower axis); kept as the two comparisons, which the
+    // selects below use as they are
+    const bool s1 = tnext[1] < tnext[0];
+    const R t01 = s1 ? tnext[1] : tnext[0];
+    const bool s2 = tnext[2] < t01;
+    const R tna = s2 ? tnext[2] : t01;
     R tc = xmin(tna, tb);
     cnt.cell();
     if (skip > 0) {
       --skip;
     } else {
-      // M <= 256: 24-bit multiplies give the exact cell index
-      const float2 mp = s.maj_pair[__umul24(__umul24(uint32_t(cell[2]), uint32_t(M)) + uint32_t(cell[1]),
-                                            uint32_t(M)) + uint32_t(cell[0])];
+      // M <= 256: 24-bit multiplies give the exact cell index, and the
+      // table (at most 2^24 pairs) is read by a 32-bit byte offset
+      const uint32_t ci = uint32_t(cell[0]) + __umul24(uint32_t(cell[1]), uint32_t(M)) +
+                          __umul24(uint32_t(cell[2]), uint32_t(M) * uint32_t(M));
+      const float2 mp =
+          *reinterpret_cast<const float2*>(reinterpret_cast<const char*>(s.maj_pair) + size_t(ci * 8u));
       const float raw = mp.x;
       if (raw < 0.0f) skip = int(-raw) - 1;
       else if (raw > 0.0f && tc > t && !f(t, tc, R(raw), mp.y)) return;
     }
     t = tc;
     if (tna >= tb) break;
-    int c = (ax == 0 ? cell[0] : (ax == 1 ? cell[1] : cell[2])) + (ax == 0 ? step[0] : (ax == 1 ? step[1] : step[2]));
+    const int sa = s2 ? step[2] : (s1 ? step[1] : step[0]);
+    const int c = (s2 ? cell[2] : (s1 ? cell[1] : cell[0])) + sa;
     if (c < 0 || c >= M) break;
-    R bnd = (ax == 0 ? step[0] : (ax == 1 ? step[1] : step[2])) > 0 ? R(c + 1) * invM : R(c) * invM;
-    R nt = ((bnd - R(0.5)) - comp(oo, ax)) * (ax == 0 ? inv[0] : (ax == 1 ? inv[1] : inv[2]));
-    if (ax == 0) { cell[0] = c; tnext[0] = nt; }
-    else if (ax == 1) { cell[1] = c; tnext[1] = nt; }
-    else { cell[2] = c; tnext[2] = nt; }
+    const R bnd = sa > 0 ? R(c + 1) * invM : R(c) * invM;
+    const R oa = s2 ? oo.z : (s1 ? oo.y : oo.x);
+    const R nt = ((bnd - R(0.5)) - oa) * (s2 ? inv[2] : (s1 ? inv[1] : inv[0]));
+    const bool a1 = s1 && !s2, a0 = !s1 && !s2;
+    cell[0] = a0 ? c : cell[0];
+    tnext[0] = a0 ? nt : tnext[0];
+    cell[1] = a1 ? c : cell[1];
+    tnext[1] = a1 ? nt : tnext[1];
+    cell[2] = s2 ? c : cell[2];
+    tnext[2] = s2 ? nt : tnext[2];
   }
 }
 
