@@ -112,8 +112,10 @@ struct WfState {
   uint32_t* cnt_shadow;            // shadow rays per chunk
   uint32_t* regen_m;               // new paths per chunk this iteration
   unsigned long long* regen_base;  // first new path id per chunk
-  uint32_t* status;                // [0]: paths in flight this iteration
+  uint32_t* status;                // [0]: paths in flight this iteration; [1], [2]: live flags of the
+                                   // two path buffers; [3]: nonzero if a plan look-back gave up
   unsigned long long* next_path;   // work counter
+  unsigned long long* plan_lb;     // plan look-back words, one per plan block
 };
 size_t wf_bytes(uint32_t P);
 // resident 256-thread blocks per CU of the lane-refill trace kernel
@@ -136,6 +138,7 @@ struct WfArgs {
   int32_t sort;         // bit 0: shade sorts each chunk by (event kind, material);
                         // bit 1: the tracking stages take each chunk's rays in Morton order
   int32_t refill_min;   // refill kernels: a wave refills once this many of its lanes are idle
+  uint32_t plan_epoch;  // tags this plan launch's look-back words (1, 2, ... within a render)
 };
 
 struct WfGuideHook;  // guided mode: proposal, training records, online training

@@ -114,10 +114,12 @@ __device__ __forceinline__ void block_count(unsigned long long* dst, uint32_t v,
 __global__ void k_wf_init(WfState w, uint32_t nchunks) {
   const uint32_t c = blockIdx.x * blockDim.x + threadIdx.x;
   if (c < nchunks) w.cnt_paths[0][c] = 0;
+  if (c < (nchunks + kPlanThreads - 1) / kPlanThreads) w.plan_lb[c] = 0ull;  // epoch 0: no word yet
   if (c == 0) {
     *w.next_path = 0;
     w.status[1] = 0u;
     w.status[2] = 0u;
+    w.status[3] = 0u;
   }
 }
 
@@ -186,80 +188,101 @@ __device__ __forceinline__ void sample_store(const WfArgs& a, uint32_t pix, uint
   q[0] = r; q[1] = g; q[2] = b;
 }
 
-// One block: exclusive scan of the chunks' free slots -> each chunk's first new
-// path id and count; advances the work counter; writes the number of paths in
-// flight this iteration to status[0].  The chunks go in rounds of
-// kPlanPer rows of kPlanThreads; thread t holds chunk r0 + j * kPlanThreads + t
-// of every row j, so loads and stores are coalesced.  Each row is scanned
-// with wave shuffles; one wave then scans the kPlanPer * 16 wave totals (in
-// chunk order: row by row, wave by wave) in LDS, and rounds chain through the
-// carried path id.  Live paths = 256 * chunks - free slots.
-constexpr int kPlanPer = 16;
+// The plan: an exclusive scan of the chunks' free slots gives each chunk
+// its first new path id and count; it advances the work counter and writes
+// the number of paths in flight this iteration to status[0].  One block per
+// kPlanThreads chunks (a thread per chunk, loads and stores coalesced): the
+// block scans its own chunks (wave shuffles, then the wave totals in LDS)
+// and takes the free slots of all earlier blocks by decoupled look-back.
+// Each block publishes one 64-bit word, (epoch << 32) | (kind << 31) |
+// count: first its own total (kind 0), then the total of every chunk up to
+// its last (kind 1).  Wave 0 reads 64 predecessors' words per round, nearest
+// first, and adds counts up to the nearest inclusive total.  Words of
+// another launch carry another epoch and are waited for; every block's
+// predecessors run before or beside it, so the wait ends, and a bounded
+// spin turns a word that never comes into status[3] instead of a hang.
+constexpr uint32_t kPlanSpinLimit = 1u << 22;
+__device__ __forceinline__ unsigned long long plan_word(const unsigned long long* lb, int j, uint32_t epoch,
+                                                        uint32_t* fail) {
+  if (j < 0) return (1ull << 31);  // before the first block: an inclusive total of 0
+  unsigned long long v = __hip_atomic_load(lb + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  for (uint32_t k = 0; uint32_t(v >> 32) != epoch; ++k) {
+    if (k == kPlanSpinLimit) {
+      __hip_atomic_store(fail, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      return (1ull << 31);
+    }
+    __builtin_amdgcn_s_sleep(1);
+    v = __hip_atomic_load(lb + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  }
+  return v;
+}
 constexpr int kPlanWaves = kPlanThreads / 64;
 __global__ __launch_bounds__(kPlanThreads) void k_wf_plan(WfState w, WfArgs a) {
-  __shared__ uint32_t wtot[kPlanPer * kPlanWaves];   // per (row, wave) free-slot sums
-  __shared__ uint32_t wpre[kPlanPer * kPlanWaves];   // their exclusive prefix within the round
-  __shared__ uint32_t rtot;
-  const uint32_t NC = a.nchunks;
-  const uint32_t* cntA = w.cnt_paths[a.cur];
-  const unsigned long long base = *w.next_path;
+  __shared__ uint32_t wtot[kPlanWaves];  // wave totals, then their exclusive prefix
+  __shared__ uint32_t btot;
+  __shared__ unsigned long long bexcl;  // free slots of the chunks before this block
+  __shared__ unsigned long long sbase;  // the work counter, read once before this block publishes
+  const uint32_t NC = a.nchunks, b = blockIdx.x;
+  const uint32_t c = b * uint32_t(kPlanThreads) + threadIdx.x;
   const int lane = __lane_id(), wave = int(threadIdx.x >> 6);
-  unsigned long long carry = base;  // path id of the round's first free slot
-  for (uint32_t r0 = 0; r0 < NC; r0 += uint32_t(kPlanThreads * kPlanPer)) {
-    uint32_t fr[kPlanPer], incl[kPlanPer];
+  // every block reads the counter before its first word, and the last block
+  // advances it only after it has a word from every block before it
+  if (threadIdx.x == 0) sbase = *w.next_path;
+  const uint32_t fr = c < NC ? kChunk - w.cnt_paths[a.cur][c] : 0u;
+  uint32_t incl = fr;
 #pragma unroll
-    for (int j = 0; j < kPlanPer; ++j) {
-      const uint32_t c = r0 + uint32_t(j * kPlanThreads) + threadIdx.x;
-      fr[j] = c < NC ? kChunk - cntA[c] : 0u;
-    }
-#pragma unroll
-    for (int j = 0; j < kPlanPer; ++j) incl[j] = fr[j];
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-#pragma unroll
-      for (int j = 0; j < kPlanPer; ++j) {
-        const uint32_t v = __shfl_up(incl[j], o);
-        if (lane >= o) incl[j] += v;
-      }
-    }
-    if (lane == 63) {
-#pragma unroll
-      for (int j = 0; j < kPlanPer; ++j) wtot[j * kPlanWaves + wave] = incl[j];
-    }
-    __syncthreads();
-    if (wave == 0) {  // 256 totals, 4 consecutive per lane; a round holds at most 2^22 free slots
-      constexpr int kE = kPlanPer * kPlanWaves / 64;
-      uint32_t e[kE], sum = 0;
-#pragma unroll
-      for (int q = 0; q < kE; ++q) {
-        e[q] = sum;
-        sum += wtot[lane * kE + q];
-      }
-      uint32_t sc = sum;
-#pragma unroll
-      for (int o = 1; o < 64; o <<= 1) {
-        const uint32_t v = __shfl_up(sc, o);
-        if (lane >= o) sc += v;
-      }
-#pragma unroll
-      for (int q = 0; q < kE; ++q) wpre[lane * kE + q] = sc - sum + e[q];
-      if (lane == 63) rtot = sc;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int j = 0; j < kPlanPer; ++j) {
-      const uint32_t c = r0 + uint32_t(j * kPlanThreads) + threadIdx.x;
-      if (c < NC) {
-        const unsigned long long off = carry + wpre[j * kPlanWaves + wave] + incl[j] - fr[j];
-        const unsigned long long left = off < a.total_paths ? a.total_paths - off : 0ull;
-        w.regen_base[c] = off;
-        w.regen_m[c] = uint32_t(left < fr[j] ? left : fr[j]);
-      }
-    }
-    carry += rtot;
+  for (int o = 1; o < 64; o <<= 1) {
+    const uint32_t v = __shfl_up(incl, o);
+    if (lane >= o) incl += v;
   }
-  if (threadIdx.x == 0) {
-    const unsigned long long total_free = carry - base;
+  if (lane == 63) wtot[wave] = incl;
+  __syncthreads();
+  if (wave == 0) {
+    const uint32_t x = lane < kPlanWaves ? wtot[lane] : 0u;
+    uint32_t sc = x;
+#pragma unroll
+    for (int o = 1; o < kPlanWaves; o <<= 1) {
+      const uint32_t v = __shfl_up(sc, o);
+      if (lane >= o) sc += v;
+    }
+    if (lane < kPlanWaves) wtot[lane] = sc - x;
+    if (lane == kPlanWaves - 1) btot = sc;
+  }
+  __syncthreads();
+  if (wave == 0) {
+    const uint32_t agg = btot, ep = a.plan_epoch;
+    unsigned long long* lb = w.plan_lb;
+    if (lane == 0)
+      __hip_atomic_store(lb + b, ((unsigned long long)ep << 32) | (b == 0 ? (1ull << 31) : 0ull) | agg,
+                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    unsigned long long excl = 0;
+    for (int j = int(b) - 1; j >= 0; j -= 64) {
+      const unsigned long long v = plan_word(lb, j - lane, ep, w.status + 3);
+      const unsigned long long pm = __ballot(((v >> 31) & 1ull) != 0ull);
+      const int f = pm ? __ffsll((long long)pm) - 1 : 63;  // nearest inclusive total
+      unsigned long long cnt = lane <= f ? (v & 0x7FFFFFFFull) : 0ull;
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) cnt += __shfl_xor(cnt, o);
+      excl += cnt;
+      if (pm) break;
+    }
+    if (lane == 0) {
+      if (b > 0)
+        __hip_atomic_store(lb + b, ((unsigned long long)ep << 32) | (1ull << 31) | (excl + agg),
+                           __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      bexcl = excl;
+    }
+  }
+  __syncthreads();
+  const unsigned long long base = sbase;
+  if (c < NC) {
+    const unsigned long long off = base + bexcl + wtot[wave] + incl - fr;
+    const unsigned long long left = off < a.total_paths ? a.total_paths - off : 0ull;
+    w.regen_base[c] = off;
+    w.regen_m[c] = uint32_t(left < fr ? left : fr);
+  }
+  if (b == gridDim.x - 1 && threadIdx.x == 0) {  // the last block holds the total
+    const unsigned long long total_free = bexcl + btot;
     const unsigned long long left = base < a.total_paths ? a.total_paths - base : 0ull;
     const unsigned long long m = left < total_free ? left : total_free;
     *w.next_path = base + m;
@@ -1059,9 +1082,9 @@ static constexpr int kPathF = 15, kPathU = 6, kShadowF = 12, kShadowU = 4;
 static inline size_t align256(size_t x) { return (x + 255) & ~size_t(255); }
 
 size_t wf_bytes(uint32_t P) {
-  const size_t NC = P / kChunk;
+  const size_t NC = P / kChunk, NB = (NC + kPlanThreads - 1) / kPlanThreads;
   return size_t(2 * (kPathF + kPathU) + kShadowF + kShadowU) * align256(size_t(P) * 4) + 4 * align256(NC * 4) +
-         align256(NC * 8) + 2 * 256;
+         align256(NC * 8) + 2 * 256 + align256(NB * 8);
 }
 
 void wf_carve(void* base, uint32_t P, WfState& w) {
@@ -1088,6 +1111,8 @@ void wf_carve(void* base, uint32_t P, WfState& w) {
   w.status = reinterpret_cast<uint32_t*>(p);
   p += 256;
   w.next_path = reinterpret_cast<unsigned long long*>(p);
+  p += 256;
+  w.plan_lb = reinterpret_cast<unsigned long long*>(p);
 }
 
 // guided-mode arrays: 2 buffers x (kGuideRecF floats + 1 state word) per slot,
@@ -1144,7 +1169,9 @@ hipError_t launch_wavefront(const DevScene& s, const DevCamera& cam, WfState& w,
     const bool planned = !drain;  // this iteration's new paths are planned
     if (planned) {
       NE_MARK(kStagePlan);
-      hipLaunchKernelGGL(k_wf_plan, dim3(1), dim3(unsigned(kPlanThreads)), 0, st, w, a);
+      a.plan_epoch += 1u;
+      hipLaunchKernelGGL(k_wf_plan, dim3((NC + kPlanThreads - 1) / kPlanThreads), dim3(unsigned(kPlanThreads)), 0,
+                         st, w, a);
     }
     // every 8 iterations (every iteration while the guide trains): stop once
     // nothing is in flight (status written by plan, or the live flag);
@@ -1157,11 +1184,15 @@ hipError_t launch_wavefront(const DevScene& s, const DevCamera& cam, WfState& w,
       if (planned && (e = hipMemcpyAsync(h_status + 2, w.next_path, sizeof(unsigned long long),
                                          hipMemcpyDeviceToHost, st)) != hipSuccess)
         return e;
+      if ((e = hipMemcpyAsync(h_status + 4, w.status + 3, sizeof(uint32_t), hipMemcpyDeviceToHost, st)) !=
+          hipSuccess)
+        return e;
       if (gh && gh->g.train &&
           (e = hipMemcpyAsync(h_status + 1, gh->g.batch.count, sizeof(uint32_t), hipMemcpyDeviceToHost, st)) !=
               hipSuccess)
         return e;
       if ((e = hipStreamSynchronize(st)) != hipSuccess) return e;
+      if (h_status[4] != 0) return hipErrorLaunchTimeOut;  // a plan block's look-back gave up
       if (h_status[0] == 0) break;
       if (planned) {
         unsigned long long issued;

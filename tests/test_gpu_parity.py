@@ -575,6 +575,23 @@ def test_wavefront_rect_with_strips_matches_megakernel(ctx):
     assert (films[1][inside].sum(axis=-1) > 0).mean() > 0.5
 
 
+def test_plan_over_many_blocks_matches_one_block(ctx):
+    """The plan scans the chunks' free slots in blocks of 1024 chunks chained
+    by look-back: a 2^21-slot pool (8192 chunks of 256 slots, 8 plan blocks)
+    renders the same film as a 1024-slot pool (4 chunks, one plan block)."""
+    print_case("plan over many blocks")
+    from narvalengine_amd.abi import NE_MODE_WAVEFRONT
+    g = ctx.upload(scenes.small_volume(res=32, maj_res=4))
+    films = []
+    for pool in (1 << 21, 1024):
+        f = ctx.film(256, 256)
+        ctx.render(g, scenes.VOLUME_CAMERA, f, spp=32, max_bounces=16, seed=23, mode=NE_MODE_WAVEFRONT,
+                   pool_size=pool)
+        films.append(f.read())
+    assert films[0].any()
+    np.testing.assert_array_equal(films[0], films[1])
+
+
 @pytest.mark.parametrize("which", ["volume", "surface"])
 def test_wavefront_counters_and_stage_times(ctx, which):
     """Both pipelines trace the same paths: equal event counts, except that the
