@@ -480,22 +480,6 @@ def equal_time_budget(ctx, g, wl, args, ref, ref_var, f, K: int) -> dict:
     for prob in args.guide_probs:
         gkw = dict(guided_vertices=args.guided_vertices, guide_prob=prob, cache_res=args.guide_cache,
                    cache_dirs=args.guide_cache_dirs)
-        # time per sample of the frozen guide in each form, from a pilot guide
-        pilot = Guide(ctx, seed=21, **gkw)
-        ctx.render(g, wl.camera, f, spp=16, seed=3, guide=pilot, **kw)
-        ctx.synchronize()
-        # one full frame each (its pool is that of the longer renders)
-        t0 = time.perf_counter()
-        ctx.render(g, wl.camera, f, spp=spp, seed=4, guide=pilot, **kw)
-        ctx.synchronize()
-        per_wf = (time.perf_counter() - t0) / spp
-        per_b = None
-        if args.batched:
-            pb = Batch(ctx, [dict(job, spp=spp, guide=pilot)])
-            pb.run(cfg)
-            per_b = pb.run(cfg) * 1e-3 / spp
-            pb.close()
-        pilot.close()
         for form in ("guided", "guided_batched") if args.batched else ("guided",):
             guide = Guide(ctx, seed=22, **gkw)
             f.clear()
@@ -508,25 +492,51 @@ def equal_time_budget(ctx, g, wl, args, ref, ref_var, f, K: int) -> dict:
             ctx.synchronize()
             t_train = time.perf_counter() - t0
             steps = guide.stats()["steps"]
-            if form == "guided":
-                rest = max(1, int((t_plain - t_train) / per_wf))
-                ctx.render(g, wl.camera, f, spp=rest, seed=13, sample_offset=args.train_spp, guide=guide, **kw)
-                ctx.synchronize()
-                t_all = time.perf_counter() - t0
-                img = f.read() / (args.train_spp + rest)
-                extra = {"frozen_spp": rest}
-            else:
-                rest = max(1, int((t_plain - t_train) / per_b))
-                gb = Batch(ctx, [dict(job, spp=rest, seed=14, guide=guide)])
-                gb.run(cfg)  # graph capture / pools and tabulation on the first run, untimed
-                ctx.synchronize()
-                t1 = time.perf_counter()
-                gb.run(cfg)
-                t_all = t_train + (time.perf_counter() - t1)
-                img = (f.read() + gb.read(0)) / (args.train_spp + rest)
-                gb.close()
-                extra = {"batched_spp": rest, "batch_config": batch_config(cfg)}
-            res[form].append(score(form, prob, args.train_spp + rest, t_all, t_train, steps, img, extra))
+            # frozen-guide samples (kept in the image) fill the rest of the
+            # budget at the trained guide's own rate: a quarter-frame pilot
+            # sets half of what is left, that render's rate sets the rest
+            # (each render call has a fixed tail, which a short pilot
+            # overstates per sample)
+            done, acc = [0], [f.read()]
+
+            def frozen(n, k):
+                """n more frozen-guide samples (the k-th render); the timed seconds."""
+                if form == "guided":
+                    t1 = time.perf_counter()
+                    ctx.render(g, wl.camera, f, spp=n, seed=13, sample_offset=args.train_spp + done[0], guide=guide,
+                               **kw)
+                    ctx.synchronize()
+                    dt = time.perf_counter() - t1
+                else:
+                    # a first run untimed (graph capture, pools, tabulation), the second timed
+                    bt = Batch(ctx, [dict(job, spp=n, seed=14 + k, guide=guide)])
+                    bt.run(cfg)
+                    ctx.synchronize()
+                    t1 = time.perf_counter()
+                    bt.run(cfg)
+                    ctx.synchronize()
+                    dt = time.perf_counter() - t1
+                    acc[0] = acc[0] + bt.read(0)
+                    bt.close()
+                done[0] += n
+                return dt
+
+            n_pilot = max(1, spp // 4)
+            dt = frozen(n_pilot, 0)
+            spent, per = t_train + dt, dt / n_pilot
+            n1 = int(0.5 * (t_plain - spent) / per)
+            if n1 > 0:
+                dt = frozen(n1, 1)
+                spent, per = spent + dt, dt / n1
+            n2 = int((t_plain - spent) / per)
+            if n2 > 0:
+                spent += frozen(n2, 2)
+            img = (f.read() if form == "guided" else acc[0]) / (args.train_spp + done[0])
+            extra = {("frozen_spp" if form == "guided" else "batched_spp"): done[0],
+                     "renders": [n_pilot, max(n1, 0), max(n2, 0)]}
+            if form != "guided":
+                extra["batch_config"] = batch_config(cfg)
+            res[form].append(score(form, prob, args.train_spp + done[0], spent, t_train, steps, img, extra))
             guide.close()
     return res
 
