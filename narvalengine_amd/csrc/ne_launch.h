@@ -100,7 +100,10 @@ struct WfPaths {
 };
 // Shadow-ray queue (per chunk, packed): origin, direction, extent, (beta f) Le,
 // MIS weight, light pdf, target (slot within the chunk of the next buffer, or
-// film pixel | 2^31), RNG key.
+// film pixel | 2^31), RNG key.  In scenes with a volume, origin, pixel,
+// sample and vertex are written only for rays of paths that end (a
+// continuing path's next record holds them); surface-only scenes write them
+// for every ray (WfArgs::shadow_full).
 struct WfShadow {
   SlotF ox, oy, oz, dx, dy, dz, tmax, cx, cy, cz, wt, pl;
   SlotU target, pix, smp, vert;
@@ -139,6 +142,7 @@ struct WfArgs {
                         // bit 1: the tracking stages take each chunk's rays in Morton order
   int32_t refill_min;   // refill kernels: a wave refills once this many of its lanes are idle
   uint32_t plan_epoch;  // tags this plan launch's look-back words (1, 2, ... within a render)
+  int32_t shadow_full;  // every shadow record carries its origin and RNG key (surface-only scenes)
 };
 
 struct WfGuideHook;  // guided mode: proposal, training records, online training

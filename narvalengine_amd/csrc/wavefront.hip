@@ -874,13 +874,15 @@ __global__ __launch_bounds__(kChunk, (GUIDED && !TRAIN) ? 7 : 1) void k_wf_shade
   }
   if (shadow) {
     const uint32_t q = c * kChunk + k;
-    S.ox[q] = x.x; S.oy[q] = x.y; S.oz[q] = x.z;
     S.dx[q] = wi.x; S.dy[q] = wi.y; S.dz[q] = wi.z;
     S.tmax[q] = tmax;
     S.cx[q] = f.x; S.cy[q] = f.y; S.cz[q] = f.z;
     S.wt[q] = wt; S.pl[q] = pl;
     S.target[q] = cont ? j : (kToFilm | pix);
-    S.pix[q] = pix; S.smp[q] = smp; S.vert[q] = depth;
+    if (!cont || a.shadow_full) {  // else a continuing path's record in B holds these (shadow_source)
+      S.ox[q] = x.x; S.oy[q] = x.y; S.oz[q] = x.z;
+      S.pix[q] = pix; S.smp[q] = smp; S.vert[q] = depth;
+    }
   }
   if (t == 0) {
     w.cnt_paths[a.cur ^ 1][c] = ncont;
@@ -894,6 +896,27 @@ __global__ __launch_bounds__(kChunk, (GUIDED && !TRAIN) ? 7 : 1) void k_wf_shade
     uint32_t nend;
     (void)block_compact(ends && !cont, wsum, nend);
     if (t == 0 && nend) atomicAdd(&a.counters->paths, (unsigned long long)nend);
+  }
+}
+
+// Origin, pixel, sample and vertex of shadow ray k (target word tg): a ray
+// of a continuing path takes them from that path's record in the next
+// buffer (its origin is the vertex, its depth the vertex's plus one), a ray
+// of a path that ends from its own record.
+__device__ __forceinline__ Vec3<float> shadow_origin(const WfShadow& S, const WfPaths& B, uint32_t k, uint32_t tg) {
+  if (tg & kToFilm) return v3(S.ox[k], S.oy[k], S.oz[k]);
+  const uint32_t b = (k / kChunk) * kChunk + tg;
+  return v3(B.ox[b], B.oy[b], B.oz[b]);
+}
+__device__ __forceinline__ void shadow_source(const WfShadow& S, const WfPaths& B, uint32_t k, uint32_t tg,
+                                              Vec3<float>& x, uint32_t& pix, uint32_t& smp, uint32_t& vert) {
+  if (tg & kToFilm) {
+    x = v3(S.ox[k], S.oy[k], S.oz[k]);
+    pix = S.pix[k]; smp = S.smp[k]; vert = S.vert[k];
+  } else {
+    const uint32_t b = (k / kChunk) * kChunk + tg;
+    x = v3(B.ox[b], B.oy[b], B.oz[b]);
+    pix = B.pix[b]; smp = B.smp[b]; vert = B.depth[b] - 1u;
   }
 }
 
@@ -982,15 +1005,18 @@ __global__ __launch_bounds__(kChunk) void k_wf_shadow(WfState w, WfArgs a, DevSc
   uint32_t slot = t;
   if constexpr (SPATIAL) {
     const uint32_t q = c * kChunk + t;
-    slot = chunk_spatial_slot(s, t < n, t < n ? v3(S.ox[q], S.oy[q], S.oz[q]) : v3s(0.0f), skey);
+    slot = chunk_spatial_slot(s, t < n, t < n ? shadow_origin(S, B, q, S.target[q]) : v3s(0.0f), skey);
   }
   const uint32_t k = c * kChunk + slot;
   Count<C> cnt;
   if (t < n) {
+    Vec3<float> x;
+    uint32_t spix, ssmp, svert;
+    shadow_source(S, B, k, S.target[k], x, spix, ssmp, svert);
     PathRng rng;
-    rng.init(a.seed, S.pix[k], S.smp[k], a.job);
-    PathRng srng = rng.shadow_stream(int(S.vert[k]));
-    const Vec3<float> x = v3(S.ox[k], S.oy[k], S.oz[k]), wi = v3(S.dx[k], S.dy[k], S.dz[k]);
+    rng.init(a.seed, spix, ssmp, a.job);
+    PathRng srng = rng.shadow_stream(int(svert));
+    const Vec3<float> wi = v3(S.dx[k], S.dy[k], S.dz[k]);
     const float tmax = S.tmax[k];
     cnt.shadow();
     float tr = 1.0f;
@@ -1160,6 +1186,9 @@ hipError_t launch_wavefront(const DevScene& s, const DevCamera& cam, WfState& w,
   const bool sort = (a.sort & 1) != 0;
   const bool spatial = (a.sort & 2) != 0;  // tracking stages in Morton order of the ray origins
   const bool small = s.grid_small != 0;    // density lookups by 32-bit offsets
+  // the volume shadow stage reads a continuing path's origin and RNG key
+  // from its next record; the surface-only shadow stage reads whole records
+  a.shadow_full = hv ? 0 : 1;
   // Drain: once every path of the render has been issued, the plan would
   // only count the paths in flight, so it is no longer launched (nor regen);
   // the loop ends on the live flag the shading stage keeps instead.
