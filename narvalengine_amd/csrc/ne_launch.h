@@ -100,10 +100,11 @@ struct WfPaths {
 };
 // Shadow-ray queue (per chunk, packed): origin, direction, extent, (beta f) Le,
 // MIS weight, light pdf, target (slot within the chunk of the next buffer, or
-// film pixel | 2^31), RNG key.  In scenes with a volume, origin, pixel,
-// sample and vertex are written only for rays of paths that end (a
-// continuing path's next record holds them); surface-only scenes write them
-// for every ray (WfArgs::shadow_full).
+// film pixel | 2^31), RNG key.  Surface-only scenes pack each chunk's rays
+// with whole records (WfArgs::shadow_full).  Scenes with a volume put a
+// continuing path's ray at the path's own index in the chunk (light pdf 0
+// where it has none), with no target, origin or RNG key (the path's next
+// record holds them), and the rays of paths that end after those, whole.
 struct WfShadow {
   SlotF ox, oy, oz, dx, dy, dz, tmax, cx, cy, cz, wt, pl;
   SlotU target, pix, smp, vert;

@@ -820,9 +820,18 @@ __global__ __launch_bounds__(kChunk, (GUIDED && !TRAIN) ? 7 : 1) void k_wf_shade
     ends = !cont;
     rngn = rng.consumed();
   }
-  uint32_t ncont, nshadow;
+  uint32_t ncont, nshadow, k;
   const uint32_t j = block_compact(cont, wsum, ncont);
-  const uint32_t k = block_compact(shadow, wsum, nshadow);
+  if (a.shadow_full) {
+    k = block_compact(shadow, wsum, nshadow);
+  } else {
+    // a continuing path's shadow ray sits at the path's own index j (a hole,
+    // light pdf 0, where it has none); the rays of paths that end follow
+    uint32_t nend;
+    const uint32_t e = block_compact(shadow && !cont, wsum, nend);
+    k = cont ? j : ncont + e;
+    nshadow = ncont + nend;
+  }
   if (cont) {
     const uint32_t b = c * kChunk + j;
     B.ox[b] = x.x; B.oy[b] = x.y; B.oz[b] = x.z;
@@ -877,12 +886,14 @@ __global__ __launch_bounds__(kChunk, (GUIDED && !TRAIN) ? 7 : 1) void k_wf_shade
     S.dx[q] = wi.x; S.dy[q] = wi.y; S.dz[q] = wi.z;
     S.tmax[q] = tmax;
     S.cx[q] = f.x; S.cy[q] = f.y; S.cz[q] = f.z;
-    S.wt[q] = wt; S.pl[q] = pl;
-    S.target[q] = cont ? j : (kToFilm | pix);
-    if (!cont || a.shadow_full) {  // else a continuing path's record in B holds these (shadow_source)
+    S.wt[q] = wt; S.pl[q] = pl;  // pl > 0: nee_sample has no ray otherwise
+    if (!cont || a.shadow_full) {  // else the ray's index and the path's next record say these
+      S.target[q] = cont ? j : (kToFilm | pix);
       S.ox[q] = x.x; S.oy[q] = x.y; S.oz[q] = x.z;
       S.pix[q] = pix; S.smp[q] = smp; S.vert[q] = depth;
     }
+  } else if (cont && !a.shadow_full) {
+    S.pl[c * kChunk + j] = 0.0f;  // a hole
   }
   if (t == 0) {
     w.cnt_paths[a.cur ^ 1][c] = ncont;
@@ -899,33 +910,34 @@ __global__ __launch_bounds__(kChunk, (GUIDED && !TRAIN) ? 7 : 1) void k_wf_shade
   }
 }
 
-// Origin, pixel, sample and vertex of shadow ray k (target word tg): a ray
-// of a continuing path takes them from that path's record in the next
-// buffer (its origin is the vertex, its depth the vertex's plus one), a ray
-// of a path that ends from its own record.
-__device__ __forceinline__ Vec3<float> shadow_origin(const WfShadow& S, const WfPaths& B, uint32_t k, uint32_t tg) {
-  if (tg & kToFilm) return v3(S.ox[k], S.oy[k], S.oz[k]);
-  const uint32_t b = (k / kChunk) * kChunk + tg;
-  return v3(B.ox[b], B.oy[b], B.oz[b]);
-}
-__device__ __forceinline__ void shadow_source(const WfShadow& S, const WfPaths& B, uint32_t k, uint32_t tg,
+// Shadow-queue layouts.  Full (WfArgs::shadow_full): the chunk's rays packed,
+// each record whole.  Otherwise (scenes with a volume): slot k < ncont of the
+// chunk is the ray of the continuing path at the same index of the next
+// buffer (a hole, light pdf 0, if that path has none), whose record there
+// holds the ray's origin (the vertex), pixel, sample and vertex (its depth
+// minus one); the rays of paths that end follow, each record whole.
+// own: slot k is a continuing path's.
+__device__ __forceinline__ void shadow_source(const WfShadow& S, const WfPaths& B, uint32_t k, bool own,
                                               Vec3<float>& x, uint32_t& pix, uint32_t& smp, uint32_t& vert) {
-  if (tg & kToFilm) {
+  if (own) {
+    x = v3(B.ox[k], B.oy[k], B.oz[k]);
+    pix = B.pix[k]; smp = B.smp[k]; vert = B.depth[k] - 1u;
+  } else {
     x = v3(S.ox[k], S.oy[k], S.oz[k]);
     pix = S.pix[k]; smp = S.smp[k]; vert = S.vert[k];
-  } else {
-    const uint32_t b = (k / kChunk) * kChunk + tg;
-    x = v3(B.ox[b], B.oy[b], B.oz[b]);
-    pix = B.pix[b]; smp = B.smp[b]; vert = B.depth[b] - 1u;
   }
 }
 
 // NEE contribution of an unoccluded shadow ray k (transmittance tr): to the
 // continuing path in the next buffer (same chunk) or straight to the film.
 __device__ __forceinline__ void shadow_deliver(const WfShadow& S, const WfPaths& B, const WfArgs& a,
-                                               const DevScene& s, uint32_t k, float tr) {
+                                               const DevScene& s, uint32_t k, float tr, bool own = false) {
   const float q = tr * S.wt[k] * float(s.n_lights) / S.pl[k];
   const float cx = S.cx[k] * q, cy = S.cy[k] * q, cz = S.cz[k] * q;
+  if (own) {
+    B.Lx[k] += cx; B.Ly[k] += cy; B.Lz[k] += cz;
+    return;
+  }
   const uint32_t tg = S.target[k];
   if (tg & kToFilm) {
     // the final vertex's light sample: a fire-and-forget atomic on the
@@ -1002,17 +1014,22 @@ __global__ __launch_bounds__(kChunk) void k_wf_shadow(WfState w, WfArgs a, DevSc
   const WfPaths B = w.paths[a.cur ^ 1];
   const uint32_t c = blockIdx.x, t = threadIdx.x;
   const uint32_t n = w.cnt_shadow[c];
+  const uint32_t nc = a.shadow_full ? 0u : w.cnt_paths[a.cur ^ 1][c];  // slots of continuing paths' rays
   uint32_t slot = t;
   if constexpr (SPATIAL) {
     const uint32_t q = c * kChunk + t;
-    slot = chunk_spatial_slot(s, t < n, t < n ? shadow_origin(S, B, q, S.target[q]) : v3s(0.0f), skey);
+    slot = chunk_spatial_slot(s, t < n,
+                              t < n ? (t < nc ? v3(B.ox[q], B.oy[q], B.oz[q]) : v3(S.ox[q], S.oy[q], S.oz[q]))
+                                    : v3s(0.0f),
+                              skey);
   }
   const uint32_t k = c * kChunk + slot;
+  const bool own = slot < nc;
   Count<C> cnt;
-  if (t < n) {
+  if (t < n && (!own || S.pl[k] > 0.0f)) {  // holes skipped
     Vec3<float> x;
     uint32_t spix, ssmp, svert;
-    shadow_source(S, B, k, S.target[k], x, spix, ssmp, svert);
+    shadow_source(S, B, k, own, x, spix, ssmp, svert);
     PathRng rng;
     rng.init(a.seed, spix, ssmp, a.job);
     PathRng srng = rng.shadow_stream(int(svert));
@@ -1033,7 +1050,7 @@ __global__ __launch_bounds__(kChunk) void k_wf_shadow(WfState w, WfArgs a, DevSc
         if (ta < tb) tr = ratio_track<float, C, SMALL>(s, x, wi, ta, tb, srng, cnt);
       }
     }
-    if (tr > 0.0f) shadow_deliver(S, B, a, s, k, tr);
+    if (tr > 0.0f) shadow_deliver(S, B, a, s, k, tr, own);
   }
   if constexpr (C) {
     block_count<C>(&a.counters->shadow_rays, cnt.n_shadow, red);
