@@ -917,15 +917,15 @@ __global__ __launch_bounds__(kChunk, (GUIDED && !TRAIN) ? 7 : 1) void k_wf_shade
 // holds the ray's origin (the vertex), pixel, sample and vertex (its depth
 // minus one); the rays of paths that end follow, each record whole.
 // own: slot k is a continuing path's.
+// (Branch-free: each array chosen per lane, so the six loads go out together.)
 __device__ __forceinline__ void shadow_source(const WfShadow& S, const WfPaths& B, uint32_t k, bool own,
                                               Vec3<float>& x, uint32_t& pix, uint32_t& smp, uint32_t& vert) {
-  if (own) {
-    x = v3(B.ox[k], B.oy[k], B.oz[k]);
-    pix = B.pix[k]; smp = B.smp[k]; vert = B.depth[k] - 1u;
-  } else {
-    x = v3(S.ox[k], S.oy[k], S.oz[k]);
-    pix = S.pix[k]; smp = S.smp[k]; vert = S.vert[k];
-  }
+  const SlotF ax = own ? B.ox : S.ox, ay = own ? B.oy : S.oy, az = own ? B.oz : S.oz;
+  const SlotU ap = own ? B.pix : S.pix, as = own ? B.smp : S.smp, av = own ? B.depth : S.vert;
+  x = v3(ax[k], ay[k], az[k]);
+  pix = ap[k];
+  smp = as[k];
+  vert = av[k] - (own ? 1u : 0u);
 }
 
 // NEE contribution of an unoccluded shadow ray k (transmittance tr): to the
@@ -1026,15 +1026,25 @@ __global__ __launch_bounds__(kChunk) void k_wf_shadow(WfState w, WfArgs a, DevSc
   const uint32_t k = c * kChunk + slot;
   const bool own = slot < nc;
   Count<C> cnt;
-  if (t < n && (!own || S.pl[k] > 0.0f)) {  // holes skipped
-    Vec3<float> x;
-    uint32_t spix, ssmp, svert;
+  Vec3<float> x = v3s(0.0f), wi = v3s(0.0f);
+  uint32_t spix = 0, ssmp = 0, svert = 0;
+  float tmax = 0.0f, pl = 0.0f;
+  // every load of the ray goes out at once, before the hole test (a hole's
+  // record and path record exist, so the loads stay in bounds); the empty
+  // asm takes the values here, so the loads are not sunk past the test into
+  // one round trip each
+  if (t < n) {
     shadow_source(S, B, k, own, x, spix, ssmp, svert);
+    wi = v3(S.dx[k], S.dy[k], S.dz[k]);
+    tmax = S.tmax[k];
+    pl = S.pl[k];
+    asm volatile("" ::"v"(x.x), "v"(x.y), "v"(x.z), "v"(spix), "v"(ssmp), "v"(svert), "v"(wi.x), "v"(wi.y),
+                 "v"(wi.z), "v"(tmax), "v"(pl));
+  }
+  if (t < n && (!own || pl > 0.0f)) {  // holes skipped
     PathRng rng;
     rng.init(a.seed, spix, ssmp, a.job);
     PathRng srng = rng.shadow_stream(int(svert));
-    const Vec3<float> wi = v3(S.dx[k], S.dy[k], S.dz[k]);
-    const float tmax = S.tmax[k];
     cnt.shadow();
     float tr = 1.0f;
     if (HAS_SURF) {
