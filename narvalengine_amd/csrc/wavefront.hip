@@ -471,6 +471,12 @@ __global__ __launch_bounds__(kChunk) void k_wf_medium(WfState w, WfArgs a, DevSc
   if (t < n) {
     const Vec3<float> o = v3(A.ox[i], A.oy[i], A.oz[i]), d = v3(A.dx[i], A.dy[i], A.dz[i]);
     const float tS = HAS_SURF ? A.hit_t[i] : xinf<float>();
+    // the stream's key goes out with the ray, not after the box test (one
+    // round trip); the empty asm takes the values here so that the loads
+    // are not sunk past the test
+    const uint32_t ppix = A.pix[i], psmp = A.smp[i], prngn = A.rngn[i];
+    asm volatile("" ::"v"(o.x), "v"(o.y), "v"(o.z), "v"(d.x), "v"(d.y), "v"(d.z), "v"(tS), "v"(ppix), "v"(psmp),
+                 "v"(prngn));
     if (!HAS_SURF) cnt.ext();
     bool medium = false;
     float th = 0.0f;
@@ -481,8 +487,8 @@ __global__ __launch_bounds__(kChunk) void k_wf_medium(WfState w, WfArgs a, DevSc
       tb = xmin(tb, tS);
       if (ta < tb) {
         PathRng rng;
-        rng.init(a.seed, A.pix[i], A.smp[i], a.job);
-        rng.seek(A.rngn[i]);
+        rng.init(a.seed, ppix, psmp, a.job);
+        rng.seek(prngn);
         medium = delta_track<float, C, SMALL>(s, o, d, ta, tb, rng, cnt, th);
         A.rngn[i] = rng.consumed();
       }
@@ -669,7 +675,7 @@ __global__ __launch_bounds__(kChunk, (GUIDED && !TRAIN) ? 7 : 1) void k_wf_shade
   bool shadow = false, cont = false, ends = false;
   Vec3<float> x = v3s(0.0f), nd = v3s(0.0f), beta = v3s(0.0f), wi = v3s(0.0f), f = v3s(0.0f), Le = v3s(0.0f);
   float tmax = 0.0f, wt = 0.0f, pl = 0.0f, ppdf = 0.0f;
-  uint32_t pix = 0, smp = 0, rngn = 0, depth = 0;
+  uint32_t pix = 0, smp = 0, rngn = 0, depth = 0, rngn0 = 0;
   float lx = 0.0f, ly = 0.0f, lz = 0.0f;
   // phase 1: what the segment hit
   uint32_t kind = WF_KIND_MISS;
@@ -705,6 +711,11 @@ __global__ __launch_bounds__(kChunk, (GUIDED && !TRAIN) ? 7 : 1) void k_wf_shade
       }
     }
     const float th = A.hit_t[i];
+    // the draw count goes out with the record (the empty asm, after the
+    // record's last load, keeps it from being sunk into the vertex branch,
+    // one round trip later)
+    rngn0 = A.rngn[i];
+    asm volatile("" ::"v"(rngn0));
     if (kind == WF_KIND_MISS) {
       if (s.env_type != 0) {
         const float wm = prev_delta ? 1.0f : power_heuristic(A.prev_pdf[i], env_pdf<float>(s, d));
@@ -754,7 +765,7 @@ __global__ __launch_bounds__(kChunk, (GUIDED && !TRAIN) ? 7 : 1) void k_wf_shade
     const Vec3<float> wo = -d;
     PathRng rng;
     rng.init(a.seed, pix, smp, a.job);
-    rng.seek(A.rngn[i]);
+    rng.seek(rngn0);
     const float ul = rng.next(), ua = rng.next(), ub = rng.next();
     bool delta = false;
     shadow = medium ? nee_sample<true>(s, x, d, gm, wo, ns, ss, ts, ul, ua, ub, wi, tmax, f, Le, wt, pl, delta)
