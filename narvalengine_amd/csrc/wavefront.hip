@@ -293,32 +293,48 @@ __global__ __launch_bounds__(kPlanThreads) void k_wf_plan(WfState w, WfArgs a) {
 
 // New paths fill the free tail of chunk c: thread t < m writes the camera ray
 // of path regen_base[c] + t into slot n0 + t.
+// Path id -> its camera ray, written as slot's record; also returned (ray
+// origin and direction, pixel, sample, draws consumed).
+__device__ __forceinline__ void regen_path(const WfArgs& a, const DevCamera& cam, const WfPaths& A, uint32_t slot,
+                                           unsigned long long id, Vec3<float>& o, Vec3<float>& d, uint32_t& pix,
+                                           uint32_t& smp, uint32_t& rngn) {
+  uint32_t rx, ry;
+  path_pixel(a, id, pix, smp, rx, ry);
+  smp += a.sample_offset;
+  PathRng rng;
+  rng.init(a.seed, pix, smp, a.job);
+  const float ju = rng.next(), jv = rng.next();
+  const uint32_t px = uint32_t(a.x0) + rx, py = uint32_t(a.y0) + ry;  // pix % W, pix / W
+  const float u = (float(px) + ju) / float(a.W), v = (float(py) + jv) / float(a.H);
+  const Vec3<float> pos = v3f<float>(cam.pos), ll = v3f<float>(cam.lower_left);
+  const Vec3<float> hz = v3f<float>(cam.horizontal), vt = v3f<float>(cam.vertical);
+  const Vec3<float> dir = -normalize(ll + u * hz + v * vt - pos);
+  rngn = rng.consumed();
+  A.ox[slot] = pos.x; A.oy[slot] = pos.y; A.oz[slot] = pos.z;
+  A.dx[slot] = dir.x; A.dy[slot] = dir.y; A.dz[slot] = dir.z;
+  A.bx[slot] = 1.0f; A.by[slot] = 1.0f; A.bz[slot] = 1.0f;
+  A.Lx[slot] = 0.0f; A.Ly[slot] = 0.0f; A.Lz[slot] = 0.0f;
+  A.pix[slot] = pix; A.smp[slot] = smp;
+  A.rngn[slot] = rngn;
+  A.depth[slot] = kPrevDelta;  // depth 0, camera ray (delta)
+  A.prev_pdf[slot] = 0.0f;
+  if (a.gstate[a.cur]) a.gstate[a.cur][slot] = 0u;
+  o = pos;
+  d = dir;
+}
 __device__ __forceinline__ void regen_chunk(const WfState& w, const WfArgs& a, const DevCamera& cam, uint32_t c,
                                             uint32_t t, uint32_t n0, uint32_t m) {
-  const WfPaths A = w.paths[a.cur];
   if (t < m) {
-    const uint32_t slot = c * kChunk + n0 + t;
-    uint32_t pix, smp, rx, ry;
-    path_pixel(a, w.regen_base[c] + t, pix, smp, rx, ry);
-    smp += a.sample_offset;
-    PathRng rng;
-    rng.init(a.seed, pix, smp, a.job);
-    const float ju = rng.next(), jv = rng.next();
-    const uint32_t px = uint32_t(a.x0) + rx, py = uint32_t(a.y0) + ry;  // pix % W, pix / W
-    const float u = (float(px) + ju) / float(a.W), v = (float(py) + jv) / float(a.H);
-    const Vec3<float> pos = v3f<float>(cam.pos), ll = v3f<float>(cam.lower_left);
-    const Vec3<float> hz = v3f<float>(cam.horizontal), vt = v3f<float>(cam.vertical);
-    const Vec3<float> dir = -normalize(ll + u * hz + v * vt - pos);
-    A.ox[slot] = pos.x; A.oy[slot] = pos.y; A.oz[slot] = pos.z;
-    A.dx[slot] = dir.x; A.dy[slot] = dir.y; A.dz[slot] = dir.z;
-    A.bx[slot] = 1.0f; A.by[slot] = 1.0f; A.bz[slot] = 1.0f;
-    A.Lx[slot] = 0.0f; A.Ly[slot] = 0.0f; A.Lz[slot] = 0.0f;
-    A.pix[slot] = pix; A.smp[slot] = smp;
-    A.rngn[slot] = rng.consumed();
-    A.depth[slot] = kPrevDelta;  // depth 0, camera ray (delta)
-    A.prev_pdf[slot] = 0.0f;
-    if (a.gstate[a.cur]) a.gstate[a.cur][slot] = 0u;
+    Vec3<float> o, d;
+    uint32_t pix, smp, rngn;
+    regen_path(a, cam, w.paths[a.cur], c * kChunk + n0 + t, w.regen_base[c] + t, o, d, pix, smp, rngn);
   }
+}
+
+// Paths of chunk c in the current buffer: the live count, plus the new paths
+// when the medium stage made them this iteration (it leaves the count).
+__device__ __forceinline__ uint32_t chunk_paths(const WfState& w, const WfArgs& a, uint32_t c) {
+  return w.cnt_paths[a.cur][c] + (a.regen_fused ? w.regen_m[c] : 0u);
 }
 
 __global__ __launch_bounds__(kChunk) void k_wf_regen(WfState w, WfArgs a, DevCamera cam) {
@@ -452,12 +468,19 @@ __global__ __launch_bounds__(kChunk) void k_wf_medium(WfState w, WfArgs a, DevSc
   const WfPaths A = w.paths[a.cur];
   const uint32_t c = blockIdx.x, t = threadIdx.x;
   uint32_t n;
+  // a new path (REGEN) is made by the thread that tracks it, so its ray stays
+  // in registers: no block barrier, no reload; the live count is left as it
+  // was (a.regen_fused: later stages add regen_m)
+  bool fresh = false;
+  Vec3<float> fo = v3s(0.0f), fd = v3s(0.0f);
+  uint32_t fpix = 0, fsmp = 0, frngn = 0;
   if constexpr (REGEN) {
-    const uint32_t n0 = w.cnt_paths[a.cur][c], m = w.regen_m[c];
-    regen_chunk(w, a, cam, c, t, n0, m);
-    __syncthreads();  // the new paths' records are visible to the whole block; every thread has read n0
-    if (t == 0) w.cnt_paths[a.cur][c] = n0 + m;
-    n = n0 + m;
+    const uint32_t n0 = w.cnt_paths[a.cur][c];
+    n = n0 + w.regen_m[c];
+    if (t >= n0 && t < n) {
+      fresh = true;
+      regen_path(a, cam, A, c * kChunk + t, w.regen_base[c] + (t - n0), fo, fd, fpix, fsmp, frngn);
+    }
   } else {
     n = w.cnt_paths[a.cur][c];
   }
@@ -469,14 +492,22 @@ __global__ __launch_bounds__(kChunk) void k_wf_medium(WfState w, WfArgs a, DevSc
   const uint32_t i = c * kChunk + slot;
   Count<C> cnt;
   if (t < n) {
-    const Vec3<float> o = v3(A.ox[i], A.oy[i], A.oz[i]), d = v3(A.dx[i], A.dy[i], A.dz[i]);
-    const float tS = HAS_SURF ? A.hit_t[i] : xinf<float>();
-    // the stream's key goes out with the ray, not after the box test (one
-    // round trip); the empty asm takes the values here so that the loads
-    // are not sunk past the test
-    const uint32_t ppix = A.pix[i], psmp = A.smp[i], prngn = A.rngn[i];
-    asm volatile("" ::"v"(o.x), "v"(o.y), "v"(o.z), "v"(d.x), "v"(d.y), "v"(d.z), "v"(tS), "v"(ppix), "v"(psmp),
-                 "v"(prngn));
+    Vec3<float> o = fo, d = fd;
+    float tS = xinf<float>();
+    uint32_t ppix = fpix, psmp = fsmp, prngn = frngn;
+    if (!fresh) {
+      o = v3(A.ox[i], A.oy[i], A.oz[i]);
+      d = v3(A.dx[i], A.dy[i], A.dz[i]);
+      if (HAS_SURF) tS = A.hit_t[i];
+      // the stream's key goes out with the ray, not after the box test (one
+      // round trip); the empty asm takes the values here so that the loads
+      // are not sunk past the test
+      ppix = A.pix[i];
+      psmp = A.smp[i];
+      prngn = A.rngn[i];
+      asm volatile("" ::"v"(o.x), "v"(o.y), "v"(o.z), "v"(d.x), "v"(d.y), "v"(d.z), "v"(tS), "v"(ppix),
+                   "v"(psmp), "v"(prngn));
+    }
     if (!HAS_SURF) cnt.ext();
     bool medium = false;
     float th = 0.0f;
@@ -580,7 +611,7 @@ __global__ __launch_bounds__(kChunk) void k_wf_guide(WfState w, WfArgs a, DevSce
   const WfPaths A = w.paths[a.cur];
   const uint32_t c = blockIdx.x, t = threadIdx.x, i = c * kChunk + t;
   bool q = false;
-  if (t < w.cnt_paths[a.cur][c] && A.kind[i] == WF_KIND_MEDIUM) {
+  if (t < chunk_paths(w, a, c) && A.kind[i] == WF_KIND_MEDIUM) {
     const int depth = int(A.depth[i] & 0xFFFFu);
     q = depth < gd.max_depth && depth < a.max_bounces;
   }
@@ -638,7 +669,7 @@ __global__ __launch_bounds__(kChunk, (GUIDED && !TRAIN) ? 7 : 1) void k_wf_shade
   const WfPaths A = w.paths[a.cur], B = w.paths[a.cur ^ 1];
   const WfShadow S = w.shadow;
   const uint32_t c = blockIdx.x, t = threadIdx.x;
-  const uint32_t nlive = w.cnt_paths[a.cur][c];
+  const uint32_t nlive = chunk_paths(w, a, c);
   if (nlive == 0) {  // an empty chunk (most of them once the pool drains): nothing continues
     if (t == 0) {
       w.cnt_paths[a.cur ^ 1][c] = 0u;
@@ -1280,6 +1311,7 @@ hipError_t launch_wavefront(const DevScene& s, const DevCamera& cam, WfState& w,
       }
     }
     const bool fused = hv && !hs && !spatial && planned;  // regen inside the medium stage
+    a.regen_fused = fused ? 1 : 0;
     if (!fused && planned) {
       NE_MARK(kStageRegen);
       hipLaunchKernelGGL(k_wf_regen, grid, blk, 0, st, w, a, cam);
