@@ -57,10 +57,13 @@ SHADE_IN, SHADE_OUT = 84, 64
 REGEN_BYTES = 68
 
 
-def stage_bytes(stage: str, c: dict, film_bytes: int, regen_in_medium: bool = False) -> int:
+def stage_bytes(stage: str, c: dict, film_bytes: int, regen_in_medium: bool = False,
+                shadow_in_shade: bool = False) -> int:
     """Algorithmic bytes a stage moves over one frame, from the event counters.
     regen_in_medium: volume-only scenes make their new paths inside the medium
-    stage (no separate regen launch), whose bytes then count there."""
+    stage (no separate regen launch), whose bytes then count there.
+    shadow_in_shade: volume-only plain renders track the shadow rays in the
+    shading stage, which then moves their lookups and cells and no queue."""
     ext = {k: c[k] - c["shadow_" + k] for k in EV}
     sh = {k: c["shadow_" + k] for k in EV}
     if stage == "megakernel":
@@ -74,6 +77,8 @@ def stage_bytes(stage: str, c: dict, film_bytes: int, regen_in_medium: bool = Fa
         return sum(sh[k] * b for k, b in EV.items()) + c["shadow_rays"] * RAY_BYTES["shadow"]
     if stage == "shade":
         cont = max(c["ext_rays"] - c["paths"], 0)
+        if shadow_in_shade:
+            return c["ext_rays"] * SHADE_IN + cont * SHADE_OUT + sum(sh[k] * b for k, b in EV.items())
         return c["ext_rays"] * SHADE_IN + (cont + c["shadow_rays"]) * SHADE_OUT
     return 0
 
@@ -957,7 +962,9 @@ def main():
                 continue
             per_launch_ms = stage_ms[name] / stage_n[name]
             launches_per_frame = stage_n[name] / args.steps
-            b = stage_bytes(name, counters, film_bytes, regen_in_medium=not stage_n.get("regen")) / launches_per_frame
+            b = stage_bytes(name, counters, film_bytes, regen_in_medium=not stage_n.get("regen"),
+                            shadow_in_shade=not stage_n.get("shadow") and counters["shadow_rays"] > 0)
+            b /= launches_per_frame
             stages[name] = {"ms_per_frame": round(stage_ms[name] / args.steps, 3),
                             "launches_per_frame": launches_per_frame,
                             "avg_launch_ms": round(per_launch_ms, 4),

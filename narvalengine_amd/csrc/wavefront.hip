@@ -663,7 +663,11 @@ __global__ __launch_bounds__(kChunk) void k_wf_guide(WfState w, WfArgs a, DevSce
 // TRAIN (guided renders that train): the training record travels with the
 // path; frozen guided renders carry none, and are held to the plain stage's
 // 7 waves per SIMD (71 VGPRs, no spills).
-template <bool GUIDED, bool SORT, bool TRAIN>
+// FUSE (volume-only scenes, plain renders): the stage tracks each vertex's
+// shadow ray itself (ratio tracking, as k_wf_shadow does, 2: by 32-bit grid
+// offsets, 1: 64-bit) and adds the light sample to the path's radiance or
+// the film; no shadow queue, no shadow stage.
+template <bool GUIDED, bool SORT, bool TRAIN, int FUSE = 0>
 __global__ __launch_bounds__(kChunk, (GUIDED && !TRAIN) ? 7 : 1) void k_wf_shade(WfState w, WfArgs a, DevScene s, WfGuide gd) {
   __shared__ uint32_t wsum[kChunk / 64];
   const WfPaths A = w.paths[a.cur], B = w.paths[a.cur ^ 1];
@@ -862,9 +866,11 @@ __global__ __launch_bounds__(kChunk, (GUIDED && !TRAIN) ? 7 : 1) void k_wf_shade
     ends = !cont;
     rngn = rng.consumed();
   }
-  uint32_t ncont, nshadow, k;
+  uint32_t ncont, nshadow = 0, k = 0;
   const uint32_t j = block_compact(cont, wsum, ncont);
-  if (a.shadow_full) {
+  if constexpr (FUSE > 0) {
+    // no queue
+  } else if (a.shadow_full) {
     k = block_compact(shadow, wsum, nshadow);
   } else {
     // a continuing path's shadow ray sits at the path's own index j (a hole,
@@ -879,7 +885,7 @@ __global__ __launch_bounds__(kChunk, (GUIDED && !TRAIN) ? 7 : 1) void k_wf_shade
     B.ox[b] = x.x; B.oy[b] = x.y; B.oz[b] = x.z;
     B.dx[b] = nd.x; B.dy[b] = nd.y; B.dz[b] = nd.z;
     B.bx[b] = beta.x; B.by[b] = beta.y; B.bz[b] = beta.z;
-    B.Lx[b] = lx; B.Ly[b] = ly; B.Lz[b] = lz;
+    if constexpr (FUSE == 0) { B.Lx[b] = lx; B.Ly[b] = ly; B.Lz[b] = lz; }  // else after the light sample
     B.pix[b] = pix; B.smp[b] = smp; B.rngn[b] = rngn;
     B.depth[b] = depth + 1u;  // prev_delta cleared
     B.prev_pdf[b] = ppdf;
@@ -923,7 +929,39 @@ __global__ __launch_bounds__(kChunk, (GUIDED && !TRAIN) ? 7 : 1) void k_wf_shade
       gd.batch.s[size_t(9) * cap + q] = r;
     }
   }
-  if (shadow) {
+  if constexpr (FUSE > 0) {
+    // the shadow ray, tracked here: the draws and arithmetic of k_wf_shadow
+    // and shadow_deliver, so a continuing path's radiance is the same one
+    // addition (lx + c) and a final vertex's sample the same film atomic
+    bool lit = false;
+    float cx = 0.0f, cy = 0.0f, cz = 0.0f;
+    if (shadow) {
+      PathRng prng;
+      prng.init(a.seed, pix, smp, a.job);
+      PathRng srng = prng.shadow_stream(int(depth));
+      Count<false> nc;
+      float tr = 1.0f;
+      const Vec3<float> oo = xf_point<float>(s.inv, x), dd = xf_vec<float>(s.inv, wi);
+      float ta, tb;
+      if (box_interval<float>(s, oo, dd, ta, tb)) {
+        ta = xmax(ta, 0.0f);
+        tb = xmin(tb, tmax);
+        if (ta < tb) tr = ratio_track<float, false, (FUSE == 2)>(s, x, wi, ta, tb, srng, nc);
+      }
+      if (tr > 0.0f) {
+        const float q = tr * wt * float(s.n_lights) / pl;
+        cx = f.x * q; cy = f.y * q; cz = f.z * q;
+        lit = true;
+        if (!cont) film_add(a.film_fixed, pix, cx, cy, cz);
+      }
+    }
+    if (cont) {
+      const uint32_t b = c * kChunk + j;
+      B.Lx[b] = lit ? lx + cx : lx;
+      B.Ly[b] = lit ? ly + cy : ly;
+      B.Lz[b] = lit ? lz + cz : lz;
+    }
+  } else if (shadow) {
     const uint32_t q = c * kChunk + k;
     S.dx[q] = wi.x; S.dy[q] = wi.y; S.dz[q] = wi.z;
     S.tmax[q] = tmax;
@@ -1353,13 +1391,19 @@ hipError_t launch_wavefront(const DevScene& s, const DevCamera& cam, WfState& w,
       hipLaunchKernelGGL(k_wf_guide, grid, blk, 0, st, w, a, s, gh->g);
     }
     NE_MARK(kStageShade);
-    if (gh && training && sort) hipLaunchKernelGGL((k_wf_shade<true, true, true>), grid, blk, 0, st, w, a, s, gh->g);
+    // volume-only plain renders: shade tracks the shadow rays itself
+    const bool fuse = hv && !hs && !gh && !count && !spatial;
+    if (fuse && sort && small) hipLaunchKernelGGL((k_wf_shade<false, true, false, 2>), grid, blk, 0, st, w, a, s, WfGuide{});
+    else if (fuse && sort) hipLaunchKernelGGL((k_wf_shade<false, true, false, 1>), grid, blk, 0, st, w, a, s, WfGuide{});
+    else if (fuse && small) hipLaunchKernelGGL((k_wf_shade<false, false, false, 2>), grid, blk, 0, st, w, a, s, WfGuide{});
+    else if (fuse) hipLaunchKernelGGL((k_wf_shade<false, false, false, 1>), grid, blk, 0, st, w, a, s, WfGuide{});
+    else if (gh && training && sort) hipLaunchKernelGGL((k_wf_shade<true, true, true>), grid, blk, 0, st, w, a, s, gh->g);
     else if (gh && training) hipLaunchKernelGGL((k_wf_shade<true, false, true>), grid, blk, 0, st, w, a, s, gh->g);
     else if (gh && sort) hipLaunchKernelGGL((k_wf_shade<true, true, false>), grid, blk, 0, st, w, a, s, gh->g);
     else if (gh) hipLaunchKernelGGL((k_wf_shade<true, false, false>), grid, blk, 0, st, w, a, s, gh->g);
     else if (sort) hipLaunchKernelGGL((k_wf_shade<false, true, false>), grid, blk, 0, st, w, a, s, WfGuide{});
     else hipLaunchKernelGGL((k_wf_shade<false, false, false>), grid, blk, 0, st, w, a, s, WfGuide{});
-    NE_MARK(kStageShadow);
+    if (!fuse) NE_MARK(kStageShadow);
 #define NE_SHADOW_SP(CNT, S_, V_, SM)                                                             \
   do {                                                                                            \
     if (spatial) hipLaunchKernelGGL((k_wf_shadow<CNT, S_, V_, true, SM>), grid, blk, 0, st, w, a, s); \
@@ -1370,7 +1414,9 @@ hipError_t launch_wavefront(const DevScene& s, const DevCamera& cam, WfState& w,
     if (V_ && small) NE_SHADOW_SP(CNT, S_, V_, true); \
     else NE_SHADOW_SP(CNT, S_, V_, false);           \
   } while (0)
-    if (hs && !hv) {
+    if (fuse) {
+      // done in the shading stage
+    } else if (hs && !hv) {
       if (count) hipLaunchKernelGGL(k_wf_shadow_surf<true>, pgrid, blk, 0, st, w, a, s);
       else hipLaunchKernelGGL(k_wf_shadow_surf<false>, pgrid, blk, 0, st, w, a, s);
     } else if (count) {
