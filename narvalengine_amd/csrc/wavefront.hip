@@ -1391,12 +1391,20 @@ hipError_t launch_wavefront(const DevScene& s, const DevCamera& cam, WfState& w,
       hipLaunchKernelGGL(k_wf_guide, grid, blk, 0, st, w, a, s, gh->g);
     }
     NE_MARK(kStageShade);
-    // volume-only plain renders: shade tracks the shadow rays itself
-    const bool fuse = hv && !hs && !gh && !count && !spatial;
-    if (fuse && sort && small) hipLaunchKernelGGL((k_wf_shade<false, true, false, 2>), grid, blk, 0, st, w, a, s, WfGuide{});
-    else if (fuse && sort) hipLaunchKernelGGL((k_wf_shade<false, true, false, 1>), grid, blk, 0, st, w, a, s, WfGuide{});
-    else if (fuse && small) hipLaunchKernelGGL((k_wf_shade<false, false, false, 2>), grid, blk, 0, st, w, a, s, WfGuide{});
-    else if (fuse) hipLaunchKernelGGL((k_wf_shade<false, false, false, 1>), grid, blk, 0, st, w, a, s, WfGuide{});
+    // volume-only renders (plain, or with a frozen guide and a grid under
+    // 4 GiB: the other guided variant would spill): shade tracks the shadow
+    // rays itself
+    const bool fuse = hv && !hs && !count && !spatial && !training && (!gh || small);
+#define NE_SHADE_FUSED(G, SO, GD)                                                                           \
+  do {                                                                                                      \
+    if (small) hipLaunchKernelGGL((k_wf_shade<G, SO, false, 2>), grid, blk, 0, st, w, a, s, GD);           \
+    else hipLaunchKernelGGL((k_wf_shade<G, SO, false, 1>), grid, blk, 0, st, w, a, s, GD);                 \
+  } while (0)
+    if (fuse && gh && sort) NE_SHADE_FUSED(true, true, gh->g);
+    else if (fuse && gh) NE_SHADE_FUSED(true, false, gh->g);
+    else if (fuse && sort) NE_SHADE_FUSED(false, true, WfGuide{});
+    else if (fuse) NE_SHADE_FUSED(false, false, WfGuide{});
+#undef NE_SHADE_FUSED
     else if (gh && training && sort) hipLaunchKernelGGL((k_wf_shade<true, true, true>), grid, blk, 0, st, w, a, s, gh->g);
     else if (gh && training) hipLaunchKernelGGL((k_wf_shade<true, false, true>), grid, blk, 0, st, w, a, s, gh->g);
     else if (gh && sort) hipLaunchKernelGGL((k_wf_shade<true, true, false>), grid, blk, 0, st, w, a, s, gh->g);
