@@ -140,12 +140,13 @@ enum {
 /* Kernel stages reported by ne_get_stage_times. */
 enum {
   NE_STAGE_MEGAKERNEL = 0, /* one thread per (pixel, sample chunk), whole paths */
-  NE_STAGE_PLAN = 1,       /* wavefront: hand out new path ids per chunk */
-  NE_STAGE_REGEN = 2,      /* wavefront: camera rays into free slots */
+  NE_STAGE_PLAN = 1,       /* wavefront: hand out new path ids per chunk (until every path is issued) */
+  NE_STAGE_REGEN = 2,      /* wavefront: camera rays into free slots (volume-only scenes: inside MEDIUM) */
   NE_STAGE_TRACE = 3,      /* wavefront: BVH closest hit */
   NE_STAGE_MEDIUM = 4,     /* wavefront: delta tracking */
-  NE_STAGE_SHADE = 5,      /* wavefront: emission, NEE sample, scattering, compaction */
-  NE_STAGE_SHADOW = 6,     /* wavefront: occlusion + ratio tracking */
+  NE_STAGE_SHADE = 5,      /* wavefront: emission, NEE sample, scattering, compaction (volume-only
+                              scenes without training or counting: also the shadow rays' ratio tracking) */
+  NE_STAGE_SHADOW = 6,     /* wavefront: occlusion + ratio tracking (when not in SHADE) */
   NE_STAGE_OTHER = 7,      /* initialisation, film reduction / flush */
   NE_STAGE_GUIDE = 8,      /* guided wavefront: learned proposal evaluation (MFMA) */
   NE_STAGE_TRAIN = 9,      /* guided wavefront: online training steps */
