@@ -575,6 +575,26 @@ def test_wavefront_rect_with_strips_matches_megakernel(ctx):
     assert (films[1][inside].sum(axis=-1) > 0).mean() > 0.5
 
 
+def test_fused_shadow_tracking_with_a_point_light(ctx):
+    """A volume-only scene lit by a point light and the sun: the shading stage
+    tracks the shadow rays itself, toward a light at a finite distance (the
+    ray's extent) as well as the sun. Same paths as the megakernel, and the
+    pool size changes nothing."""
+    print_case("fused shadow tracking, point light")
+    from narvalengine_amd.abi import NE_MODE_MEGAKERNEL, NE_MODE_WAVEFRONT
+    s = scenes.small_volume(res=32, maj_res=4, env=False)
+    s.add_point_light((0.3, 0.35, -0.2), s.add_emitter((3.0, 2.5, 2.0)))
+    g = ctx.upload(s)
+    films = []
+    for mode, pool in [(NE_MODE_MEGAKERNEL, 0), (NE_MODE_WAVEFRONT, 0), (NE_MODE_WAVEFRONT, 1000)]:
+        f = ctx.film(40, 24)
+        ctx.render(g, scenes.VOLUME_CAMERA, f, spp=16, max_bounces=16, seed=29, mode=mode, pool_size=pool)
+        films.append(f.read())
+    assert films[1].any()
+    np.testing.assert_allclose(films[1], films[0], rtol=1e-6, atol=16 * 2.0 ** -30)
+    np.testing.assert_array_equal(films[1], films[2])
+
+
 def test_plan_over_many_blocks_matches_one_block(ctx):
     """The plan scans the chunks' free slots in blocks of 1024 chunks chained
     by look-back: a 2^21-slot pool (8192 chunks of 256 slots, 8 plan blocks)
