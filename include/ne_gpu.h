@@ -133,8 +133,10 @@ enum {
   NE_FLAG_SORT = 8,       /* wavefront: shade each chunk's paths ordered by (event kind, material) */
   NE_FLAG_FIXED_FILM = 16, /* wavefront: accumulate through 2^-30 fixed-point film atomics, the path taken
                              anyway when the render's per-sample radiance slots (12 B each) exceed 16 GiB */
-  NE_FLAG_SPATIAL = 32     /* wavefront: the delta- and ratio-tracking stages take each chunk's rays in
+  NE_FLAG_SPATIAL = 32,    /* wavefront: the delta- and ratio-tracking stages take each chunk's rays in
                               Morton order of their origins (same paths, same image) */
+  NE_FLAG_WIDE_OFFSETS = 64 /* wavefront: read the density grid by 64-bit offsets even under 4 GiB, the
+                               kernels a grid of 4 GiB or more takes (same addresses, same image) */
 };
 
 /* Kernel stages reported by ne_get_stage_times. */

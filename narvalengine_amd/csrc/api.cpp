@@ -873,6 +873,7 @@ static ne_status render_wavefront(ne_ctx* c, const ne_scene* s, const DevCamera&
   a.max_bounces = p->max_bounces; a.sample_offset = p->sample_offset; a.job = p->job; a.nchunks = uint32_t(P / kChunk);
   a.seed = p->seed;
   a.sort = ((p->flags & NE_FLAG_SORT) ? 1 : 0) | ((p->flags & NE_FLAG_SPATIAL) ? 2 : 0);
+  a.wide_offsets = (p->flags & NE_FLAG_WIDE_OFFSETS) ? 1 : 0;
   a.refill_min = kRefillMinIdle;
   if (c->refill_blocks == 0) {
     int cus = 0, per_cu = 0;

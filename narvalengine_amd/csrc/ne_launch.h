@@ -144,6 +144,7 @@ struct WfArgs {
   int32_t refill_min;   // refill kernels: a wave refills once this many of its lanes are idle
   uint32_t plan_epoch;  // tags this plan launch's look-back words (1, 2, ... within a render)
   int32_t shadow_full;  // every shadow record carries its origin and RNG key (surface-only scenes)
+  int32_t wide_offsets;  // density lookups by 64-bit offsets whatever the grid's size
   int32_t regen_fused;  // this iteration's medium stage makes the new paths and leaves the live
                         // counts as they were: later stages add regen_m
 };

@@ -1292,7 +1292,7 @@ hipError_t launch_wavefront(const DevScene& s, const DevCamera& cam, WfState& w,
   const bool hs = s.n_nodes > 0, hv = s.has_volume != 0;
   const bool sort = (a.sort & 1) != 0;
   const bool spatial = (a.sort & 2) != 0;  // tracking stages in Morton order of the ray origins
-  const bool small = s.grid_small != 0;    // density lookups by 32-bit offsets
+  const bool small = s.grid_small != 0 && !a.wide_offsets;  // density lookups by 32-bit offsets
   // the volume shadow stage reads a continuing path's origin and RNG key
   // from its next record; the surface-only shadow stage reads whole records
   a.shadow_full = hv ? 0 : 1;

@@ -595,6 +595,33 @@ def test_fused_shadow_tracking_with_a_point_light(ctx):
     np.testing.assert_array_equal(films[1], films[2])
 
 
+@pytest.mark.parametrize("which", ["cloud", "point_light"])
+def test_wide_grid_offsets_change_nothing(ctx, which):
+    """NE_FLAG_WIDE_OFFSETS runs the kernels a grid of 4 GiB or more takes
+    (64-bit density offsets: medium, fused shading, shadow stage) on a small
+    grid. They read the same voxels, so the films and event counts equal the
+    32-bit-offset kernels', plain and counted, in a full pool and a draining
+    one."""
+    print_case(f"wide grid offsets {which}")
+    from narvalengine_amd.abi import NE_MODE_WAVEFRONT
+    if which == "cloud":
+        s = scenes.small_cloud()
+    else:
+        s = scenes.small_volume(res=32, maj_res=4, env=False)
+        s.add_point_light((0.3, 0.35, -0.2), s.add_emitter((3.0, 2.5, 2.0)))
+    g = ctx.upload(s)
+    for pool, count in [(0, False), (1000, False), (0, True)]:
+        out = []
+        for wide in (False, True):
+            f = ctx.film(40, 24)
+            ctx.render(g, scenes.VOLUME_CAMERA, f, spp=8, max_bounces=32, seed=41, mode=NE_MODE_WAVEFRONT,
+                       pool_size=pool, count=count, wide=wide)
+            out.append((f.read(), ctx.counters() if count else None))
+        assert out[0][0].any()
+        np.testing.assert_array_equal(out[0][0], out[1][0])
+        assert out[0][1] == out[1][1]
+
+
 def test_plan_over_many_blocks_matches_one_block(ctx):
     """The plan scans the chunks' free slots in blocks of 1024 chunks chained
     by look-back: a 2^21-slot pool (8192 chunks of 256 slots, 8 plan blocks)
