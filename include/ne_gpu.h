@@ -135,8 +135,11 @@ enum {
                              anyway when the render's per-sample radiance slots (12 B each) exceed 16 GiB */
   NE_FLAG_SPATIAL = 32,    /* wavefront: the delta- and ratio-tracking stages take each chunk's rays in
                               Morton order of their origins (same paths, same image) */
-  NE_FLAG_WIDE_OFFSETS = 64 /* wavefront: read the density grid by 64-bit offsets even under 4 GiB, the
-                               kernels a grid of 4 GiB or more takes (same addresses, same image) */
+  NE_FLAG_WIDE_OFFSETS = 64, /* wavefront: read the density grid by 64-bit offsets even under 4 GiB, the
+                                kernels a grid of 4 GiB or more takes (same addresses, same image) */
+  NE_FLAG_TEST_PLAN_STALL = 128 /* test hook for the failure path: the wavefront plan's look-back waits for
+                                   words that never come, its bounded wait gives up after 1024 spins, and
+                                   the render fails with NE_ERR_HIP (a launch time-out) instead of hanging */
 };
 
 /* Kernel stages reported by ne_get_stage_times. */

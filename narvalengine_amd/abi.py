@@ -12,6 +12,7 @@ NE_FLAG_SORT = 8
 NE_FLAG_FIXED_FILM = 16
 NE_FLAG_SPATIAL = 32
 NE_FLAG_WIDE_OFFSETS = 64
+NE_FLAG_TEST_PLAN_STALL = 128
 STAGES = ("megakernel", "plan", "regen", "trace", "medium", "shade", "shadow", "other", "guide", "train")
 NE_MODE_AUTO, NE_MODE_MEGAKERNEL, NE_MODE_WAVEFRONT = 0, 1, 2
 NE_BATCH_PERSISTENT, NE_BATCH_WAVEFRONT = 0, 1  # batch executors

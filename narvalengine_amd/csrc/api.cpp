@@ -874,6 +874,7 @@ static ne_status render_wavefront(ne_ctx* c, const ne_scene* s, const DevCamera&
   a.seed = p->seed;
   a.sort = ((p->flags & NE_FLAG_SORT) ? 1 : 0) | ((p->flags & NE_FLAG_SPATIAL) ? 2 : 0);
   a.wide_offsets = (p->flags & NE_FLAG_WIDE_OFFSETS) ? 1 : 0;
+  a.plan_stall = (p->flags & NE_FLAG_TEST_PLAN_STALL) ? 1u : 0u;
   a.refill_min = kRefillMinIdle;
   if (c->refill_blocks == 0) {
     int cus = 0, per_cu = 0;

@@ -145,6 +145,7 @@ struct WfArgs {
   uint32_t plan_epoch;  // tags this plan launch's look-back words (1, 2, ... within a render)
   int32_t shadow_full;  // every shadow record carries its origin and RNG key (surface-only scenes)
   int32_t wide_offsets;  // density lookups by 64-bit offsets whatever the grid's size
+  uint32_t plan_stall;   // test hook: the plan's look-back waits for an epoch that never comes
   int32_t regen_fused;  // this iteration's medium stage makes the new paths and leaves the live
                         // counts as they were: later stages add regen_m
 };

@@ -201,13 +201,15 @@ __device__ __forceinline__ void sample_store(const WfArgs& a, uint32_t pix, uint
 // another launch carry another epoch and are waited for; every block's
 // predecessors run before or beside it, so the wait ends, and a bounded
 // spin turns a word that never comes into status[3] instead of a hang.
-constexpr uint32_t kPlanSpinLimit = 1u << 22;
+// a.plan_stall (a test of that path) makes the readers wait for the next
+// epoch, whose words never come, with a short spin.
+constexpr uint32_t kPlanSpinLimit = 1u << 22, kPlanStallSpins = 1u << 10;
 __device__ __forceinline__ unsigned long long plan_word(const unsigned long long* lb, int j, uint32_t epoch,
-                                                        uint32_t* fail) {
+                                                        uint32_t* fail, uint32_t limit) {
   if (j < 0) return (1ull << 31);  // before the first block: an inclusive total of 0
   unsigned long long v = __hip_atomic_load(lb + j, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   for (uint32_t k = 0; uint32_t(v >> 32) != epoch; ++k) {
-    if (k == kPlanSpinLimit) {
+    if (k == limit) {
       __hip_atomic_store(fail, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
       return (1ull << 31);
     }
@@ -251,13 +253,14 @@ __global__ __launch_bounds__(kPlanThreads) void k_wf_plan(WfState w, WfArgs a) {
   __syncthreads();
   if (wave == 0) {
     const uint32_t agg = btot, ep = a.plan_epoch;
+    const uint32_t want = ep + a.plan_stall, limit = a.plan_stall ? kPlanStallSpins : kPlanSpinLimit;
     unsigned long long* lb = w.plan_lb;
     if (lane == 0)
       __hip_atomic_store(lb + b, ((unsigned long long)ep << 32) | (b == 0 ? (1ull << 31) : 0ull) | agg,
                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     unsigned long long excl = 0;
     for (int j = int(b) - 1; j >= 0; j -= 64) {
-      const unsigned long long v = plan_word(lb, j - lane, ep, w.status + 3);
+      const unsigned long long v = plan_word(lb, j - lane, want, w.status + 3, limit);
       const unsigned long long pm = __ballot(((v >> 31) & 1ull) != 0ull);
       const int f = pm ? __ffsll((long long)pm) - 1 : 63;  // nearest inclusive total
       unsigned long long cnt = lane <= f ? (v & 0x7FFFFFFFull) : 0ull;

@@ -7,7 +7,7 @@ import numpy as np
 
 from .abi import (BatchConfig, Counters, GuideConfig, GuideStats, Job, NarvalError, RenderParams, SceneInfo, check, lib, NE_FP32,
                   NE_FLAG_COUNT, NE_FLAG_TIME, NE_FLAG_STAGE_TIME, NE_FLAG_SORT, NE_FLAG_FIXED_FILM, NE_FLAG_SPATIAL,
-                  NE_FLAG_WIDE_OFFSETS, STAGES)
+                  NE_FLAG_WIDE_OFFSETS, NE_FLAG_TEST_PLAN_STALL, STAGES)
 
 
 def _fp(a: np.ndarray):
@@ -59,7 +59,8 @@ class Context:
                samples_per_thread: int = 0, block_size: int = 0, sync: bool = True, timed: bool = False,
                stage_timed: bool = False, guide=None, train: bool = False,
                occupancy: int = 0, mode: int = 0, pool_size: int = 0, sort: bool = False,
-               fixed_film: bool = False, noalloc: bool = False, spatial: bool = False, wide: bool = False):
+               fixed_film: bool = False, noalloc: bool = False, spatial: bool = False, wide: bool = False,
+               plan_stall: bool = False):
         p = RenderParams()
         p.width, p.height = film.width, film.height
         x0, y0, x1, y1 = rect if rect is not None else (0, 0, film.width, film.height)
@@ -69,7 +70,7 @@ class Context:
         p.flags = (NE_FLAG_COUNT if count else 0) | (NE_FLAG_TIME if timed else 0) | \
             (NE_FLAG_STAGE_TIME if stage_timed else 0) | (NE_FLAG_SORT if sort else 0) | \
             (NE_FLAG_FIXED_FILM if fixed_film else 0) | (NE_FLAG_SPATIAL if spatial else 0) | \
-            (NE_FLAG_WIDE_OFFSETS if wide else 0)
+            (NE_FLAG_WIDE_OFFSETS if wide else 0) | (NE_FLAG_TEST_PLAN_STALL if plan_stall else 0)
         p.samples_per_thread, p.block_size, p.occupancy = samples_per_thread, block_size, occupancy
         p.mode, p.pool_size = mode, pool_size
         cam = camera.desc()

@@ -639,6 +639,27 @@ def test_plan_over_many_blocks_matches_one_block(ctx):
     np.testing.assert_array_equal(films[0], films[1])
 
 
+def test_plan_look_back_that_gives_up_fails_the_render(ctx):
+    """The failure path of the plan's bounded look-back wait, forced from the
+    host (NE_FLAG_TEST_PLAN_STALL: the plan blocks wait for words of a later
+    launch, which never come, for 1024 spins): every kernel still ends, the
+    render fails with NE_ERR_HIP instead of hanging, and the context then
+    renders the same film as before."""
+    print_case("plan look-back gives up")
+    from narvalengine_amd import NarvalError
+    from narvalengine_amd.abi import NE_MODE_WAVEFRONT
+    g = ctx.upload(scenes.small_volume(res=32, maj_res=4))
+    kw = dict(spp=32, max_bounces=16, seed=23, mode=NE_MODE_WAVEFRONT, pool_size=1 << 21)
+    f0 = ctx.film(256, 256)
+    ctx.render(g, scenes.VOLUME_CAMERA, f0, **kw)
+    with pytest.raises(NarvalError, match="launch_wavefront"):
+        ctx.render(g, scenes.VOLUME_CAMERA, ctx.film(256, 256), plan_stall=True, **kw)
+    f1 = ctx.film(256, 256)
+    ctx.render(g, scenes.VOLUME_CAMERA, f1, **kw)
+    assert f0.read().any()
+    np.testing.assert_array_equal(f0.read(), f1.read())
+
+
 @pytest.mark.parametrize("which", ["volume", "surface"])
 def test_wavefront_counters_and_stage_times(ctx, which):
     """Both pipelines trace the same paths: equal event counts, except that the
