@@ -139,7 +139,9 @@ enum {
                                 kernels a grid of 4 GiB or more takes (same addresses, same image) */
   NE_FLAG_TEST_PLAN_STALL = 128 /* test hook for the failure path: the wavefront plan's look-back waits for
                                    words that never come, its bounded wait gives up after 1024 spins, and
-                                   the render fails with NE_ERR_HIP (a launch time-out) instead of hanging */
+                                   the render fails with NE_ERR_HIP (a launch time-out) instead of hanging.
+                                   Only pools of more than 2^18 slots (1024 chunks) have a look-back:
+                                   smaller pools and the megakernel render as usual */
 };
 
 /* Kernel stages reported by ne_get_stage_times. */
